@@ -1,0 +1,211 @@
+/* agx.h - C ABI of the MI355X-native PS_AGONY2D collide-and-solve path.
+ *
+ * The reference has no FFI: its boundary is the C++ class PS_AGONY2D::Simulation
+ * (PhysicsSimulation-Agony/PS_AGONY2D/Simulation.h:36-255) and its ObjectAPI facade
+ * (SimulationAPIs/ObjectAPI.h:14-127). This header is the thin C layer a host binding
+ * calls instead; the C++ mirror of Simulation/ObjectAPI lives in
+ * physicssimulation-agony_amd/host/GpuSimulation.h and calls only these functions.
+ *
+ * Conventions: every function returns 0 (AGX_OK) or a negative AGX_ERR_* code and never
+ * throws. Callers own host buffers; a world owns its device buffers and one HIP stream.
+ * One world must be externally synchronised; distinct worlds may be used from distinct
+ * threads. agx_world_last_error() describes the last failure on a world,
+ * agx_last_error() the last failure of a call without a world.
+ */
+#ifndef AGX_H
+#define AGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AGX_OK 0
+#define AGX_ERR_INVALID_ARGUMENT (-1)
+#define AGX_ERR_CAPACITY (-2)
+#define AGX_ERR_HIP (-3)
+#define AGX_ERR_STATE (-4)
+#define AGX_ERR_NO_DEVICE (-5)
+
+/* Shape types: Types/Types.h:38-45 (ColliderShapeType; capsules are not on this path). */
+#define AGX_SHAPE_CIRCLE 0
+#define AGX_SHAPE_BOX 1
+#define AGX_SHAPE_POLYGON 2
+
+typedef struct agx_builder agx_builder;
+typedef struct agx_world agx_world;
+
+/* Simulation::SimulationSettings (Simulation.h:50-76), FixedClock::Settings
+ * (SimulationImpl/FixedClock.h:12-17), Integrator::Settings (Integration/Integrator.h:14-19)
+ * and ContinuousCollisionDetector::Settings (CCD/ContinuousCollisionDetector.h:29-33).
+ * Sleeping, joints and collision events are not part of this path. */
+typedef struct agx_settings {
+    float gravity_x, gravity_y;            /* default (0, -9.81) */
+    uint32_t velocity_iterations;          /* default 6 */
+    uint32_t position_iterations;          /* default 3 */
+    uint32_t ccd_enabled;                  /* default 1 */
+    float ccd_fast_factor;                 /* default 1.0 */
+    float ccd_core_fraction;               /* default 0.5 */
+    float update_interval;                 /* default 1/300 s */
+    float max_delta_per_update;            /* default 1/20 s */
+    float time_scale;                      /* default 1 */
+    uint32_t track_errors;                 /* computeConstraintErrors each step (Simulation.h:75) */
+    uint32_t track_residual;               /* relative impulse change after every velocity sweep */
+    float residual_target;                 /* > 0: stop velocity sweeps once residual <= target */
+    uint32_t collide_only;                 /* 1: integrate + collide, skip the solver (broad/narrow timing) */
+} agx_settings;
+
+/* One manifold record (64 bytes): NarrowPhaseCollisionDetector.h:26-61 with the collider
+ * handles replaced by dense collider indices of the canonical A/B; impulses are the
+ * persistent accumulators (jn, jt) of the two contact points. */
+typedef struct agx_manifold {
+    uint32_t a, b;
+    float normal_x, normal_y;
+    float depth;
+    uint32_t count;
+    float p0_x, p0_y, p1_x, p1_y;
+    uint32_t id0, id1;
+    float jn0, jt0, jn1, jt1;
+} agx_manifold;
+
+/* Per-step counters of the last step. */
+typedef struct agx_stats {
+    uint64_t pairs;                /* broad-phase pairs */
+    uint64_t manifolds;            /* narrow-phase manifolds */
+    uint64_t contacts;             /* contact points */
+    uint32_t colours;              /* solver colours */
+    uint32_t islands;              /* islands of dynamic bodies */
+    uint32_t ccd_hits;             /* bodies moved back by CCD */
+    uint32_t velocity_sweeps;      /* velocity sweeps actually run (adaptive mode may stop early) */
+    uint32_t overflow;             /* nonzero: a capacity was exceeded, results are truncated */
+    uint32_t reserved;
+    float velocity_error;          /* computeConstraintErrors().x */
+    float position_error;          /* computeConstraintErrors().y */
+    float residual[64];            /* relative impulse change after velocity sweep k */
+} agx_stats;
+
+/* World sizes. */
+typedef struct agx_counts {
+    uint32_t scenes, bodies, static_bodies, colliders, static_colliders, vertices, materials, reserved;
+} agx_counts;
+
+/* Body fields readable/writable through agx_world_{read,write}_body_f32
+ * (SoA/BodySoA.h:14-36). */
+enum {
+    AGX_BODY_COM_X = 0, AGX_BODY_COM_Y, AGX_BODY_ROTATION, AGX_BODY_VEL_X, AGX_BODY_VEL_Y,
+    AGX_BODY_ANG_VEL, AGX_BODY_COS, AGX_BODY_SIN, AGX_BODY_INV_MASS, AGX_BODY_INV_INERTIA,
+    AGX_BODY_MASS, AGX_BODY_INERTIA, AGX_BODY_LOCAL_COM_X, AGX_BODY_LOCAL_COM_Y,
+    AGX_BODY_FIELD_COUNT
+};
+/* Collider fields readable through agx_world_read_collider_f32 (SoA/ColliderSoA.h:10-32). */
+enum {
+    AGX_COL_WORLD_X = 0, AGX_COL_WORLD_Y, AGX_COL_WORLD_ROT, AGX_COL_COS, AGX_COL_SIN,
+    AGX_COL_AABB_MIN_X, AGX_COL_AABB_MIN_Y, AGX_COL_AABB_MAX_X, AGX_COL_AABB_MAX_Y,
+    AGX_COL_PREV_X, AGX_COL_PREV_Y, AGX_COL_COM_OFF_X, AGX_COL_COM_OFF_Y, AGX_COL_LOCAL_ROT,
+    AGX_COL_P0, AGX_COL_P1, AGX_COL_MIN_EXTENT, AGX_COL_MAX_EXTENT,
+    AGX_COL_FIELD_COUNT
+};
+/* Collider integer fields readable through agx_world_read_collider_u32. */
+enum { AGX_COL_BODY = 0, AGX_COL_TYPE, AGX_COL_MATERIAL, AGX_COL_SLOT, AGX_COL_POLY_START, AGX_COL_POLY_COUNT, AGX_COL_SCENE };
+
+/* Step phases, in step order (Simulation.cpp:255-351), for agx_world_run_phase. */
+enum {
+    AGX_PHASE_INTEGRATE = 0,   /* CCD snapshot, velocity-Verlet, wrap, cos/sin */
+    AGX_PHASE_POSE,            /* collider world poses + AABBs */
+    AGX_PHASE_CCD,             /* swept core circles vs static colliders */
+    AGX_PHASE_BROAD,           /* Morton, radix sort, LBVH build/refit, pair traversal */
+    AGX_PHASE_NARROW,          /* SAT manifolds, compaction, warm-start lookup */
+    AGX_PHASE_SOLVE,           /* colouring, constraint prep, warm start, velocity and position sweeps, scatter */
+    AGX_PHASE_ISLANDS,         /* union-find islands */
+    AGX_PHASE_FINISH,          /* contact cache for the next step, step counters */
+    AGX_PHASE_COUNT
+};
+
+void agx_settings_default(agx_settings* s);
+const char* agx_last_error(void);
+int agx_device_count(int* out);
+
+/* ---- scene building: ObjectAPI::createCircle/createBox/createPolygon (ObjectAPI.h:34-37),
+ *      Simulation::createMaterial (Simulation.h:155). Creation semantics follow
+ *      SimulationImpl/ObjectManager.cpp:71-389 (mass from the caller, inertia and centre of
+ *      mass from the shape, group-ordered dense indices). ---- */
+int agx_builder_create(uint32_t num_scenes, agx_builder** out);
+int agx_builder_destroy(agx_builder* b);
+int agx_builder_add_material(agx_builder* b, float elasticity, float static_friction,
+                             float dynamic_friction, uint32_t* out_index);
+/* Adds n bodies with one collider each to scene 'scene', in call order. shape_type[i] picks
+ * the shape; p0/p1 are radius (circle) or width/height (box); polygons take vert_count[i]
+ * local (x, y) vertices starting at vertex vert_start[i] of 'verts'. out_dense_index (may be
+ * NULL) receives each body's dense index at its creation; a rejected body (invalid polygon)
+ * gets UINT32_MAX and the call returns the number of rejected bodies. */
+int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uint8_t* shape_type,
+                           const float* pos_x, const float* pos_y, const float* rotation,
+                           const float* vel_x, const float* vel_y, const float* ang_vel,
+                           const float* mass, const uint32_t* material, const float* p0,
+                           const float* p1, const uint32_t* vert_start, const uint32_t* vert_count,
+                           const float* verts, uint32_t* out_dense_index);
+/* Host-side views of a built scene before upload (ObjectAPI viewers, ObjectAPI.h:96-101):
+ * dense-order body and collider fields of one scene, and its polygon vertex/normal pool. */
+int agx_builder_scene_counts(agx_builder* b, uint32_t scene, agx_counts* out);
+int agx_builder_read_body_f32(agx_builder* b, uint32_t scene, uint32_t field, float* out, uint32_t n);
+int agx_builder_read_collider(agx_builder* b, uint32_t scene, uint32_t field, uint32_t as_u32, void* out, uint32_t n);
+int agx_builder_read_vertices(agx_builder* b, uint32_t scene, float* verts, float* normals, uint32_t n);
+/* Uploads the built scenes to device 0 (Simulation::refreshSpatialData, Simulation.cpp:561-579,
+ * runs once here) and returns the world. The builder may be destroyed afterwards. */
+int agx_world_create(agx_builder* b, const agx_settings* settings, agx_world** out);
+int agx_world_destroy(agx_world* w);
+const char* agx_world_last_error(agx_world* w);
+
+int agx_world_set_settings(agx_world* w, const agx_settings* s);   /* getSimulationSettings() = */
+int agx_world_get_settings(agx_world* w, agx_settings* s);
+int agx_world_counts(agx_world* w, agx_counts* out);
+
+/* Simulation::update (Simulation.cpp:32-74): fixed-clock sub-steps, then the post-update pose
+ * pass. Returns the number of physics steps taken through *out_steps (may be NULL). */
+int agx_world_update(agx_world* w, float dt, uint32_t* out_steps);
+/* n_steps physics steps of 'dt' then the post-update pose pass (update() with a clock that
+ * yields exactly n_steps). Asynchronous on the world's stream; call agx_world_sync to wait. */
+int agx_world_step(agx_world* w, float dt, uint32_t n_steps);
+/* One phase of a physics step (tests and tools drive a step phase by phase). */
+int agx_world_run_phase(agx_world* w, uint32_t phase, float dt);
+int agx_world_post_update(agx_world* w);
+int agx_world_sync(agx_world* w);
+/* Captures agx_world_step(w, dt, 1) as a HIP graph and replays it n_steps times. */
+int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps);
+
+/* Read-outs: ObjectAPI::getBodies/getColliders viewers (ObjectAPI.h:96-101),
+ * Simulation::getBodyCollisionData (Simulation.h:186), getBroadPhaseCollisionDetector
+ * (Simulation.h:183), islands() (Simulation.h:203), getDebugData (Simulation.h:189). */
+int agx_world_read_body_f32(agx_world* w, uint32_t field, float* out, uint32_t n);
+int agx_world_write_body_f32(agx_world* w, uint32_t field, const float* in, uint32_t n);
+int agx_world_read_collider_f32(agx_world* w, uint32_t field, float* out, uint32_t n);
+int agx_world_read_collider_u32(agx_world* w, uint32_t field, uint32_t* out, uint32_t n);
+int agx_world_read_pairs(agx_world* w, uint32_t* a, uint32_t* b, uint64_t cap, uint64_t* count);
+/* which = 0: manifolds of the current step (pre-solve after NARROW, solved after SOLVE);
+ * which = 1: the previous step's manifolds (the warm-start source). */
+int agx_world_read_manifolds(agx_world* w, uint32_t which, agx_manifold* out, uint64_t cap, uint64_t* count);
+int agx_world_write_manifolds(agx_world* w, uint32_t which, const agx_manifold* in, uint64_t count);
+/* Solve order of the last SOLVE phase: order[c] = manifold solved at position c; constraints
+ * [colour_start[k], colour_start[k+1]) share colour k and no dynamic body. */
+int agx_world_read_solve_order(agx_world* w, uint32_t* order, uint64_t cap, uint32_t* colour_start,
+                               uint32_t colour_cap, uint32_t* colours);
+int agx_world_read_islands(agx_world* w, uint32_t* island_id, uint32_t n, uint32_t* island_count);
+int agx_world_read_stats(agx_world* w, agx_stats* out);
+
+/* Standalone broad phase over injected AABBs (BroadPhaseCollisionDetector::findCollisions,
+ * Broad/BroadPhaseCollisionDetector.cpp:109-244): aabb holds n x (min_x, min_y, max_x, max_y),
+ * owner[i] the owning body, is_dynamic[i] 1 for dynamic colliders, scene[i] (may be NULL) the
+ * scene of each collider. Writes pairs (dynamic x dynamic of different owners, dynamic x static)
+ * of strictly overlapping AABBs in one scene to pairs_a/pairs_b; *count receives the total, and
+ * AGX_ERR_CAPACITY is returned when it exceeds cap. */
+int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t* is_dynamic,
+                         const uint32_t* scene, uint32_t n, uint32_t* pairs_a, uint32_t* pairs_b,
+                         uint64_t cap, uint64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AGX_H */

@@ -1,0 +1,201 @@
+// C API of the oracle for ctypes (test infrastructure only; see oracle.h).
+#include "oracle.h"
+
+#include <cstring>
+#include <string>
+
+using namespace orc;
+
+namespace {
+
+std::vector<float>* f32Field(World& w, const char* name) {
+    struct E { const char* n; std::vector<float> World::*p; };
+    static const E table[] = {
+        { "comX", &World::comX }, { "comY", &World::comY }, { "velX", &World::velX }, { "velY", &World::velY },
+        { "rot", &World::rot }, { "angVel", &World::angVel }, { "mass", &World::mass }, { "invMass", &World::invMass },
+        { "inertia", &World::inertia }, { "invInertia", &World::invInertia }, { "cos", &World::cosv }, { "sin", &World::sinv },
+        { "localComX", &World::localComX }, { "localComY", &World::localComY }, { "boundRadius", &World::boundRadius },
+        { "localOffX", &World::localOffX }, { "localOffY", &World::localOffY }, { "comOffX", &World::comOffX },
+        { "comOffY", &World::comOffY }, { "localRot", &World::localRot }, { "wX", &World::wX }, { "wY", &World::wY },
+        { "wRot", &World::wRot }, { "wCos", &World::wCos }, { "wSin", &World::wSin }, { "minX", &World::minX },
+        { "minY", &World::minY }, { "maxX", &World::maxX }, { "maxY", &World::maxY }, { "p0", &World::p0 }, { "p1", &World::p1 },
+        { "minExtent", &World::minExtent }, { "maxExtent", &World::maxExtent }, { "prevComX", &World::prevComX },
+        { "prevComY", &World::prevComY }, { "prevWX", &World::prevWX }, { "prevWY", &World::prevWY },
+    };
+    for (const E& e : table) if (std::strcmp(e.n, name) == 0) return &(w.*(e.p));
+    return nullptr;
+}
+
+std::vector<uint32_t>* u32Field(World& w, const char* name) {
+    struct E { const char* n; std::vector<uint32_t> World::*p; };
+    static const E table[] = {
+        { "bodySlot", &World::bodySlot }, { "islandId", &World::islandId }, { "colBody", &World::colBody },
+        { "colMaterial", &World::colMaterial }, { "colSlot", &World::colSlot }, { "polyStart", &World::polyStart },
+        { "polyCount", &World::polyCount },
+    };
+    for (const E& e : table) if (std::strcmp(e.n, name) == 0) return &(w.*(e.p));
+    return nullptr;
+}
+
+World& W(void* h) { return *static_cast<World*>(h); }
+
+}  // namespace
+
+extern "C" {
+
+void* orc_new(void) { World* w = new World(); w->materials.push_back(Material{}); return w; }
+void orc_free(void* h) { delete static_cast<World*>(h); }
+
+void orc_set_settings(void* h, float gx, float gy, uint32_t vIt, uint32_t pIt, int ccd, float fast, float core) {
+    Settings& s = W(h).settings;
+    s.gravityX = gx; s.gravityY = gy; s.velocityIterations = vIt; s.positionIterations = pIt;
+    s.ccd = ccd != 0; s.ccdFastFactor = fast; s.ccdCoreFraction = core;
+}
+
+uint32_t orc_add_material(void* h, float e, float us, float ud) { return addMaterial(W(h), Material{ e, us, ud }); }
+
+// Vectorised creation in call order; out_index receives each body's dense index at its
+// creation (later static creations may move earlier bodies; read colBody/bodySlot after).
+int orc_add_bodies(void* h, uint32_t n, const uint8_t* type, const float* px, const float* py, const float* rot,
+                   const float* vx, const float* vy, const float* av, const float* mass, const uint32_t* material,
+                   const float* p0, const float* p1, const uint32_t* vstart, const uint32_t* vcount,
+                   const float* vxy, uint32_t* out_index) {
+    World& w = W(h);
+    int bad = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t r;
+        if (type[i] == SHAPE_CIRCLE) r = createCircle(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i]);
+        else if (type[i] == SHAPE_BOX) r = createBox(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i], p1[i]);
+        else r = createPolygon(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i],
+                               reinterpret_cast<const V2*>(vxy) + vstart[i], vcount[i]);
+        if (out_index) out_index[i] = r;
+        if (r == UINT32_MAX) bad++;
+    }
+    return bad;
+}
+
+// Simulation::refreshSpatialData (Simulation.cpp:561-579): cos/sin of every body, poses and AABBs of every collider.
+void orc_refresh(void* h) {
+    World& w = W(h);
+    for (uint32_t i = 0; i < w.bodyCount(); i++) fastCosSin(w.rot[i], w.cosv[i], w.sinv[i]);
+    colliderPoses(w, true);
+    colliderAABBs(w, true);
+    w.spatialDirty = false;
+    w.postUpdateDone = true;
+}
+
+void orc_counts(void* h, uint32_t* out) {
+    World& w = W(h);
+    out[0] = w.bodyCount(); out[1] = w.staticBodies; out[2] = w.colliderCount(); out[3] = w.staticColliders;
+    out[4] = (uint32_t)w.verts.size(); out[5] = (uint32_t)w.materials.size();
+    out[6] = (uint32_t)w.manifolds.size(); out[7] = (uint32_t)w.pairA.size(); out[8] = w.islandCount;
+    out[9] = (uint32_t)w.prevManifolds.size();
+}
+
+int orc_get_f32(void* h, const char* name, float* out, uint32_t n) {
+    std::vector<float>* v = f32Field(W(h), name);
+    if (!v || v->size() != n) return -1;
+    std::memcpy(out, v->data(), n * 4u);
+    return 0;
+}
+int orc_set_f32(void* h, const char* name, const float* in, uint32_t n) {
+    std::vector<float>* v = f32Field(W(h), name);
+    if (!v) return -1;
+    v->assign(in, in + n);
+    return 0;
+}
+int orc_get_u32(void* h, const char* name, uint32_t* out, uint32_t n) {
+    std::vector<uint32_t>* v = u32Field(W(h), name);
+    if (!v || v->size() != n) return -1;
+    std::memcpy(out, v->data(), n * 4u);
+    return 0;
+}
+int orc_get_types(void* h, uint8_t* out, uint32_t n) {
+    World& w = W(h);
+    if (w.colType.size() != n) return -1;
+    std::memcpy(out, w.colType.data(), n);
+    return 0;
+}
+int orc_get_verts(void* h, float* vxy, float* nxy, uint32_t n) {
+    World& w = W(h);
+    if (w.verts.size() != n) return -1;
+    std::memcpy(vxy, w.verts.data(), n * 8u);
+    std::memcpy(nxy, w.normals.data(), n * 8u);
+    return 0;
+}
+
+void orc_fast_cossin(const float* x, float* c, float* s, uint32_t n) { for (uint32_t i = 0; i < n; i++) fastCosSin(x[i], c[i], s[i]); }
+void orc_wrap(const float* x, float* y, uint32_t n) { for (uint32_t i = 0; i < n; i++) y[i] = wrapRotation(x[i]); }
+
+void orc_snapshot(void* h) { snapshotPoses(W(h)); }
+void orc_integrate(void* h, float dt) { integrate(W(h), dt); }
+void orc_wrap_cossin(void* h) { wrapAndCosSin(W(h)); }
+void orc_poses(void* h, int all) { colliderPoses(W(h), all != 0); }
+void orc_aabbs(void* h, int all) { colliderAABBs(W(h), all != 0); }
+int orc_ccd(void* h, float* hitTime) {
+    std::vector<float> t;
+    const bool any = ccdResolve(W(h), &t);
+    if (hitTime) std::memcpy(hitTime, t.data(), t.size() * 4u);
+    return any ? 1 : 0;
+}
+
+int64_t orc_broad_bvh(void* h, uint32_t* a, uint32_t* b, uint64_t cap) {
+    World& w = W(h);
+    broadPhaseBvh(w);
+    const uint64_t n = w.pairA.size();
+    if (n > cap) return -(int64_t)n;
+    std::memcpy(a, w.pairA.data(), n * 4u);
+    std::memcpy(b, w.pairB.data(), n * 4u);
+    return (int64_t)n;
+}
+int64_t orc_broad_brute(void* h, uint32_t* a, uint32_t* b, uint64_t cap) {
+    std::vector<uint32_t> pa, pb;
+    broadPhaseBrute(W(h), pa, pb);
+    if (pa.size() > cap) return -(int64_t)pa.size();
+    std::memcpy(a, pa.data(), pa.size() * 4u);
+    std::memcpy(b, pb.data(), pb.size() * 4u);
+    return (int64_t)pa.size();
+}
+
+int64_t orc_narrow(void* h, const uint32_t* pa, const uint32_t* pb, uint64_t n, Manifold* out, uint64_t cap) {
+    std::vector<Manifold> m;
+    narrowPhase(W(h), pa, pb, n, m);
+    if (m.size() > cap) return -(int64_t)m.size();
+    std::memcpy(out, m.data(), m.size() * sizeof(Manifold));
+    return (int64_t)m.size();
+}
+
+void orc_set_prev_manifolds(void* h, const Manifold* m, uint64_t n) { W(h).prevManifolds.assign(m, m + n); }
+void orc_warm_start(void* h, Manifold* m, uint64_t n) {
+    std::vector<Manifold> v(m, m + n);
+    warmStartFromCache(W(h), v);
+    std::memcpy(m, v.data(), n * sizeof(Manifold));
+}
+
+void orc_solve(void* h, Manifold* m, uint64_t n, const uint32_t* order, uint32_t vIt, uint32_t pIt, float* residual) {
+    std::vector<Manifold> v(m, m + n);
+    solve(W(h), v, order, vIt, pIt, residual);
+    std::memcpy(m, v.data(), n * sizeof(Manifold));
+}
+
+uint32_t orc_islands(void* h, const Manifold* m, uint64_t n) {
+    World& w = W(h);
+    islands(w, std::vector<Manifold>(m, m + n));
+    return w.islandCount;
+}
+
+void orc_step(void* h, float dt) { step(W(h), dt); }
+void orc_post_update(void* h) { postUpdate(W(h)); }
+void orc_update(void* h, float dt, uint32_t steps) {
+    for (uint32_t i = 0; i < steps; i++) step(W(h), dt);
+    postUpdate(W(h));
+}
+int64_t orc_get_manifolds(void* h, Manifold* out, uint64_t cap) {
+    World& w = W(h);
+    if (w.manifolds.size() > cap) return -(int64_t)w.manifolds.size();
+    std::memcpy(out, w.manifolds.data(), w.manifolds.size() * sizeof(Manifold));
+    return (int64_t)w.manifolds.size();
+}
+void orc_errors(void* h, float* out) { out[0] = W(h).velocityError; out[1] = W(h).positionError; }
+
+}  // extern "C"

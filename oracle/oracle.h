@@ -1,0 +1,132 @@
+// Oracle: CPU restatement of PS_AGONY2D's per-step collide-and-solve path.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product
+// path: it may be loaded by tests/, by __graft_entry__.smoke() and by the
+// cpu_baseline leg of bench.py, and by nothing else. The product never links,
+// loads or calls it.
+//
+// Every function follows the reference file:line named in its comment. The
+// reference source (a C++20 engine) is not buildable in this image (it
+// includes <tracy/Tracy.hpp> unconditionally and needs the C++17 library
+// constant std::hardware_destructive_interference_size, absent from
+// libstdc++ 11), so parity is pinned by closed forms, brute force and
+// invariants in tests/test_oracle_*.py instead of by a compiled reference.
+//
+// Build with -ffp-contract=off: arithmetic is written in the reference's
+// operation order, and the HIP path uses the same order, so phases with the
+// same inputs agree bit for bit.
+#pragma once
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+
+namespace orc {
+
+struct V2 { float x, y; };
+inline V2 operator+(V2 a, V2 b) { return { a.x + b.x, a.y + b.y }; }
+inline V2 operator-(V2 a, V2 b) { return { a.x - b.x, a.y - b.y }; }
+inline V2 operator-(V2 a) { return { -a.x, -a.y }; }
+inline V2 operator*(V2 a, float s) { return { a.x * s, a.y * s }; }
+inline V2 operator*(float s, V2 a) { return { s * a.x, s * a.y }; }
+inline float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+
+enum ShapeType : uint8_t { SHAPE_CIRCLE = 0, SHAPE_BOX = 1, SHAPE_POLYGON = 2 };
+
+struct Material { float elasticity = 1.0f, staticFriction = 0.0f, dynamicFriction = 0.0f; };
+
+// One manifold; field order matches the device record (BodyCollisionData,
+// Narrow/NarrowPhaseCollisionDetector.h:26-61, with handles replaced by the
+// dense collider indices of the canonical A/B).
+struct Manifold {
+    uint32_t a, b;
+    float nx, ny;
+    float depth;
+    uint32_t count;
+    float p0x, p0y, p1x, p1y;
+    uint32_t id0, id1;
+    float jn0, jt0, jn1, jt1;
+};
+static_assert(sizeof(Manifold) == 64, "manifold record is 64 bytes");
+
+struct Settings {
+    float gravityX = 0.0f, gravityY = -9.81f;
+    uint32_t velocityIterations = 6;   // Simulation.h:56
+    uint32_t positionIterations = 3;   // Simulation.h:57
+    bool ccd = true;                   // Simulation.h:66
+    float ccdFastFactor = 1.0f;        // CCD/ContinuousCollisionDetector.h:31
+    float ccdCoreFraction = 0.5f;      // CCD/ContinuousCollisionDetector.h:32
+};
+
+struct World {
+    Settings settings;
+    std::vector<Material> materials;   // index 0 = default material
+
+    // Bodies, dense order: [Static | Dynamic] (GroupRanges.h:13; sleeping is off).
+    uint32_t staticBodies = 0;
+    std::vector<float> comX, comY, velX, velY, rot, angVel;
+    std::vector<float> mass, invMass, inertia, invInertia, cosv, sinv;
+    std::vector<float> localComX, localComY, boundRadius;
+    std::vector<uint32_t> bodySlot, islandId;
+    std::vector<std::vector<uint32_t>> bodyColliders;
+
+    // Colliders, dense order follows the owner's group.
+    uint32_t staticColliders = 0;
+    std::vector<uint32_t> colBody, colMaterial, colSlot;
+    std::vector<uint8_t> colType;
+    std::vector<float> localOffX, localOffY, comOffX, comOffY, localRot;
+    std::vector<float> wX, wY, wRot, wCos, wSin;
+    std::vector<float> minX, minY, maxX, maxY;
+    std::vector<float> p0, p1;                 // circle: radius; box: half width, half height
+    std::vector<uint32_t> polyStart, polyCount;
+    std::vector<float> minExtent, maxExtent;
+    std::vector<V2> verts, normals;            // polygon pool (local frame, CCW)
+
+    // Step state.
+    std::vector<float> prevComX, prevComY, prevWX, prevWY;
+    bool spatialDirty = true;
+    bool postUpdateDone = true;
+    std::vector<Manifold> prevManifolds;       // previous step's manifolds (warm start cache)
+    std::vector<Manifold> manifolds;           // this step's manifolds
+    std::vector<uint32_t> pairA, pairB;        // this step's broad-phase pairs
+    std::vector<uint32_t> islandStart, islandMembers;
+    uint32_t islandCount = 0;
+    // BodyCollisionSolver::computeConstraintErrors (BodyCollisionSolver.cpp:132-183), after the solve.
+    float velocityError = 0.0f, positionError = 0.0f;
+
+    uint32_t bodyCount() const { return (uint32_t)comX.size(); }
+    uint32_t colliderCount() const { return (uint32_t)colBody.size(); }
+};
+
+// ---- builder (SimulationImpl/ObjectManager.cpp:71-389, PhysicsGeometry.cpp) ----
+uint32_t addMaterial(World& w, const Material& m);
+// Returns the dense body index, or UINT32_MAX on invalid input (reference: nullopt).
+uint32_t createCircle(World& w, float px, float py, float rot, float vx, float vy, float av,
+                      float mass, uint32_t material, float radius);
+uint32_t createBox(World& w, float px, float py, float rot, float vx, float vy, float av,
+                   float mass, uint32_t material, float width, float height);
+uint32_t createPolygon(World& w, float px, float py, float rot, float vx, float vy, float av,
+                       float mass, uint32_t material, const V2* localVerts, uint32_t count);
+
+// ---- phases ----
+void fastCosSin(float x, float& c, float& s);                 // FastCosSin.cpp:26-109
+float wrapRotation(float r);                                  // LeftOutFunctions.cpp:13-102
+void snapshotPoses(World& w);                                 // CCD/ContinuousCollisionDetector.cpp:276-299
+void integrate(World& w, float dt);                           // Integration/Integrator.cpp:99-209
+void wrapAndCosSin(World& w);                                 // Simulation.cpp:291-292
+void colliderPoses(World& w, bool all);                       // Simulation.cpp:673-736
+void colliderAABBs(World& w, bool all);                       // ColliderAABBBuilder.cpp:8-136
+bool ccdResolve(World& w, std::vector<float>* hitTimeOut);    // CCD/ContinuousCollisionDetector.cpp:442-531
+void broadPhaseBvh(World& w);                                 // Broad/*: BVH build + pair traversal
+void broadPhaseBrute(const World& w, std::vector<uint32_t>& a, std::vector<uint32_t>& b);
+void narrowPhase(World& w, const uint32_t* pa, const uint32_t* pb, size_t n,
+                 std::vector<Manifold>& out);                 // Narrow/NarrowPhaseCollisionDetector.cpp:638-767
+void warmStartFromCache(const World& w, std::vector<Manifold>& cur);  // :272-297 + :692-767
+// Sequential impulses in the given order (order[c] = manifold index of constraint c).
+// residualOut (optional, size velocityIterations) receives r_k after velocity sweep k.
+void solve(World& w, std::vector<Manifold>& m, const uint32_t* order,
+           uint32_t vIt, uint32_t pIt, float* residualOut);   // Solving/BodyCollisionSolver.cpp:38-980
+void islands(World& w, const std::vector<Manifold>& m);       // IslandManager.cpp:7-116
+void step(World& w, float dt);                                // Simulation.cpp:255-351
+void postUpdate(World& w);                                    // Simulation.cpp:640-655
+
+}  // namespace orc

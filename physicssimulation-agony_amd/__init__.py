@@ -1,0 +1,363 @@
+"""MI355X collide-and-solve path of PS_AGONY2D: Python binding over the C ABI in include/agx.h.
+
+The compute path is libagx.so (hand-written gfx950 HIP kernels); this module only moves
+arrays across the C ABI. It fails loudly when libagx.so is missing: there is no CPU fallback.
+
+Names mirror the reference's public API where one exists:
+``World.update`` is ``Simulation::update`` (PS_AGONY2D/Simulation.cpp:32-74), ``World.step``
+runs whole physics steps (``Simulation::physicsStep``, Simulation.cpp:255-351), and
+``Builder.create_bodies`` covers ``ObjectAPI::createCircle/createBox/createPolygon``
+(SimulationAPIs/ObjectAPI.h:34-37) for many bodies at once.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libagx.so"
+
+SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON = 0, 1, 2
+
+(PHASE_INTEGRATE, PHASE_POSE, PHASE_CCD, PHASE_BROAD, PHASE_NARROW, PHASE_SOLVE,
+ PHASE_ISLANDS, PHASE_FINISH) = range(8)
+
+BODY_FIELDS = ["com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel", "cos", "sin",
+               "inv_mass", "inv_inertia", "mass", "inertia", "local_com_x", "local_com_y"]
+COLLIDER_F32 = ["world_x", "world_y", "world_rot", "cos", "sin", "aabb_min_x", "aabb_min_y",
+                "aabb_max_x", "aabb_max_y", "prev_x", "prev_y", "com_off_x", "com_off_y",
+                "local_rot", "p0", "p1", "min_extent", "max_extent"]
+COLLIDER_U32 = ["body", "type", "material", "slot", "poly_start", "poly_count", "scene"]
+
+MANIFOLD_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("nx", "<f4"), ("ny", "<f4"), ("depth", "<f4"),
+                           ("count", "<u4"), ("p0x", "<f4"), ("p0y", "<f4"), ("p1x", "<f4"), ("p1y", "<f4"),
+                           ("id0", "<u4"), ("id1", "<u4"), ("jn0", "<f4"), ("jt0", "<f4"), ("jn1", "<f4"),
+                           ("jt1", "<f4")])
+assert MANIFOLD_DTYPE.itemsize == 64
+
+
+class AgxError(RuntimeError):
+    """A failed call across the C ABI (carries the library's error code and message)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"agx error {code}: {message}")
+        self.code = code
+
+
+class Settings(C.Structure):
+    """agx_settings: Simulation::SimulationSettings (Simulation.h:50-76) and friends."""
+    _fields_ = [("gravity_x", C.c_float), ("gravity_y", C.c_float),
+                ("velocity_iterations", C.c_uint32), ("position_iterations", C.c_uint32),
+                ("ccd_enabled", C.c_uint32), ("ccd_fast_factor", C.c_float), ("ccd_core_fraction", C.c_float),
+                ("update_interval", C.c_float), ("max_delta_per_update", C.c_float), ("time_scale", C.c_float),
+                ("track_errors", C.c_uint32), ("track_residual", C.c_uint32), ("residual_target", C.c_float),
+                ("collide_only", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("manifolds", C.c_uint64), ("contacts", C.c_uint64),
+                ("colours", C.c_uint32), ("islands", C.c_uint32), ("ccd_hits", C.c_uint32),
+                ("velocity_sweeps", C.c_uint32), ("overflow", C.c_uint32), ("reserved", C.c_uint32),
+                ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * 64)]
+
+
+class Counts(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("scenes", "bodies", "static_bodies", "colliders", "static_colliders",
+                                          "vertices", "materials", "reserved")]
+
+
+_LIB = None
+
+# Every exported function of include/agx.h with its ctypes signature.
+_P = C.c_void_p
+_SIGS = {
+    "agx_settings_default": (None, [C.POINTER(Settings)]),
+    "agx_last_error": (C.c_char_p, []),
+    "agx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "agx_builder_create": (C.c_int, [C.c_uint32, C.POINTER(_P)]),
+    "agx_builder_destroy": (C.c_int, [_P]),
+    "agx_builder_add_material": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_uint32)]),
+    "agx_builder_add_bodies": (C.c_int, [_P, C.c_uint32, C.c_uint32] + [_P] * 15),
+    "agx_builder_scene_counts": (C.c_int, [_P, C.c_uint32, C.POINTER(Counts)]),
+    "agx_builder_read_body_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
+    "agx_builder_read_collider": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
+    "agx_builder_read_vertices": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32]),
+    "agx_world_create": (C.c_int, [_P, C.POINTER(Settings), C.POINTER(_P)]),
+    "agx_world_destroy": (C.c_int, [_P]),
+    "agx_world_last_error": (C.c_char_p, [_P]),
+    "agx_world_set_settings": (C.c_int, [_P, C.POINTER(Settings)]),
+    "agx_world_get_settings": (C.c_int, [_P, C.POINTER(Settings)]),
+    "agx_world_counts": (C.c_int, [_P, C.POINTER(Counts)]),
+    "agx_world_update": (C.c_int, [_P, C.c_float, C.POINTER(C.c_uint32)]),
+    "agx_world_step": (C.c_int, [_P, C.c_float, C.c_uint32]),
+    "agx_world_run_phase": (C.c_int, [_P, C.c_uint32, C.c_float]),
+    "agx_world_post_update": (C.c_int, [_P]),
+    "agx_world_sync": (C.c_int, [_P]),
+    "agx_world_step_graph": (C.c_int, [_P, C.c_float, C.c_uint32]),
+    "agx_world_read_body_f32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
+    "agx_world_write_body_f32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
+    "agx_world_read_collider_f32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
+    "agx_world_read_collider_u32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
+    "agx_world_read_pairs": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_read_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_write_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64]),
+    "agx_world_read_solve_order": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "agx_world_read_islands": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "agx_world_read_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "agx_broadphase_pairs": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+}
+EXPORTED = tuple(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """Loads libagx.so (built in this directory by __graft_entry__.build / make)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C {PKG_DIR}` or __graft_entry__.build()")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_settings(**overrides) -> Settings:
+    s = Settings()
+    lib().agx_settings_default(C.byref(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+def _check(rc: int, world=None):
+    if rc != 0:
+        msg = (lib().agx_world_last_error(world) if world else lib().agx_last_error()).decode()
+        raise AgxError(rc, msg)
+
+
+class Builder:
+    """Scene builder (ObjectAPI creation + Simulation::createMaterial), S independent scenes."""
+
+    def __init__(self, num_scenes: int = 1):
+        self._h = C.c_void_p()
+        _check(lib().agx_builder_create(num_scenes, C.byref(self._h)))
+        self.num_scenes = num_scenes
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.agx_builder_destroy(self._h)
+            self._h = None
+
+    def create_material(self, elasticity: float, static_friction: float, dynamic_friction: float) -> int:
+        out = C.c_uint32()
+        _check(lib().agx_builder_add_material(self._h, elasticity, static_friction, dynamic_friction, C.byref(out)))
+        return out.value
+
+    def create_bodies(self, scene: int, bodies: dict) -> np.ndarray:
+        """Creates bodies from the arrays of a scene description (see scenes.py); returns dense indices."""
+        b = normalise_bodies(bodies)
+        n = len(b["type"])
+        out = np.empty(n, np.uint32)
+        rc = lib().agx_builder_add_bodies(
+            self._h, scene, n, _ptr(b["type"]), _ptr(b["px"]), _ptr(b["py"]), _ptr(b["rot"]), _ptr(b["vx"]),
+            _ptr(b["vy"]), _ptr(b["av"]), _ptr(b["mass"]), _ptr(b["material"]), _ptr(b["p0"]), _ptr(b["p1"]),
+            _ptr(b["vstart"]), _ptr(b["vcount"]), _ptr(b["verts"]), _ptr(out))
+        if rc < 0:
+            _check(rc)
+        self.rejected = rc
+        return out
+
+    def counts(self, scene: int = 0) -> Counts:
+        c = Counts()
+        _check(lib().agx_builder_scene_counts(self._h, scene, C.byref(c)))
+        return c
+
+    def body(self, scene: int, field: str) -> np.ndarray:
+        n = self.counts(scene).bodies
+        out = np.empty(n, np.float32)
+        _check(lib().agx_builder_read_body_f32(self._h, scene, BODY_FIELDS.index(field), _ptr(out), n))
+        return out
+
+    def collider(self, scene: int, field: str) -> np.ndarray:
+        n = self.counts(scene).colliders
+        as_u32 = field in COLLIDER_U32
+        out = np.empty(n, np.uint32 if as_u32 else np.float32)
+        idx = COLLIDER_U32.index(field) if as_u32 else COLLIDER_F32.index(field)
+        _check(lib().agx_builder_read_collider(self._h, scene, idx, int(as_u32), _ptr(out), n))
+        return out
+
+    def vertices(self, scene: int = 0):
+        n = self.counts(scene).vertices
+        v = np.empty((n, 2), np.float32)
+        nn = np.empty((n, 2), np.float32)
+        _check(lib().agx_builder_read_vertices(self._h, scene, _ptr(v), _ptr(nn), n))
+        return v, nn
+
+
+def normalise_bodies(bodies: dict) -> dict:
+    """Contiguous typed arrays for the body-creation call."""
+    n = len(bodies["type"])
+    out = {"type": np.ascontiguousarray(bodies["type"], np.uint8)}
+    for k in ("px", "py", "rot", "vx", "vy", "av", "mass", "p0", "p1"):
+        out[k] = np.ascontiguousarray(bodies.get(k, np.zeros(n)), np.float32)
+    out["material"] = np.ascontiguousarray(bodies.get("material", np.zeros(n)), np.uint32)
+    out["vstart"] = np.ascontiguousarray(bodies.get("vstart", np.zeros(n)), np.uint32)
+    out["vcount"] = np.ascontiguousarray(bodies.get("vcount", np.zeros(n)), np.uint32)
+    verts = bodies.get("verts")
+    out["verts"] = np.ascontiguousarray(verts if verts is not None and len(verts) else np.zeros((1, 2)), np.float32)
+    return out
+
+
+class World:
+    """A device world of one or more scenes (Simulation, Simulation.h:36-255)."""
+
+    def __init__(self, builder: Builder, settings: Settings | None = None):
+        self._h = C.c_void_p()
+        s = settings if settings is not None else default_settings()
+        _check(lib().agx_world_create(builder._h, C.byref(s), C.byref(self._h)))
+        c = Counts()
+        _check(lib().agx_world_counts(self._h, C.byref(c)), self._h)
+        self.counts = c
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().agx_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _LIB is not None:
+            self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- settings / stepping
+    def settings(self) -> Settings:
+        s = Settings()
+        _check(lib().agx_world_get_settings(self._h, C.byref(s)), self._h)
+        return s
+
+    def set_settings(self, s: Settings):
+        _check(lib().agx_world_set_settings(self._h, C.byref(s)), self._h)
+
+    def update(self, dt: float) -> int:
+        n = C.c_uint32()
+        _check(lib().agx_world_update(self._h, dt, C.byref(n)), self._h)
+        return n.value
+
+    def step(self, dt: float, n_steps: int = 1):
+        _check(lib().agx_world_step(self._h, dt, n_steps), self._h)
+
+    def run_phase(self, phase: int, dt: float):
+        _check(lib().agx_world_run_phase(self._h, phase, dt), self._h)
+
+    def post_update(self):
+        _check(lib().agx_world_post_update(self._h), self._h)
+
+    def sync(self):
+        _check(lib().agx_world_sync(self._h), self._h)
+
+    # -- read-outs
+    def body(self, field: str) -> np.ndarray:
+        out = np.empty(self.counts.bodies, np.float32)
+        _check(lib().agx_world_read_body_f32(self._h, BODY_FIELDS.index(field), _ptr(out), len(out)), self._h)
+        return out
+
+    def set_body(self, field: str, values):
+        v = np.ascontiguousarray(values, np.float32)
+        _check(lib().agx_world_write_body_f32(self._h, BODY_FIELDS.index(field), _ptr(v), len(v)), self._h)
+
+    def collider(self, field: str) -> np.ndarray:
+        n = self.counts.colliders
+        if field in COLLIDER_U32:
+            out = np.empty(n, np.uint32)
+            _check(lib().agx_world_read_collider_u32(self._h, COLLIDER_U32.index(field), _ptr(out), n), self._h)
+        else:
+            out = np.empty(n, np.float32)
+            _check(lib().agx_world_read_collider_f32(self._h, COLLIDER_F32.index(field), _ptr(out), n), self._h)
+        return out
+
+    def pairs(self):
+        cnt = C.c_uint64()
+        rc = lib().agx_world_read_pairs(self._h, None, None, 0, C.byref(cnt))
+        n = cnt.value
+        a = np.empty(max(n, 1), np.uint32)
+        b = np.empty(max(n, 1), np.uint32)
+        _check(lib().agx_world_read_pairs(self._h, _ptr(a), _ptr(b), n, C.byref(cnt)), self._h)
+        return a[:n], b[:n]
+
+    def manifolds(self, which: int = 0) -> np.ndarray:
+        cnt = C.c_uint64()
+        lib().agx_world_read_manifolds(self._h, which, None, 0, C.byref(cnt))
+        out = np.empty(max(cnt.value, 1), MANIFOLD_DTYPE)
+        _check(lib().agx_world_read_manifolds(self._h, which, _ptr(out), cnt.value, C.byref(cnt)), self._h)
+        return out[:cnt.value]
+
+    def set_warm_start_source(self, manifolds: np.ndarray):
+        m = np.ascontiguousarray(manifolds, MANIFOLD_DTYPE)
+        _check(lib().agx_world_write_manifolds(self._h, 1, _ptr(m), len(m)), self._h)
+
+    def solve_order(self):
+        k = C.c_uint32()
+        starts = np.empty(2048, np.uint32)
+        lib().agx_world_read_solve_order(self._h, None, 0, _ptr(starts), len(starts), C.byref(k))
+        total = int(starts[k.value]) if k.value else 0
+        order = np.empty(max(total, 1), np.uint32)
+        _check(lib().agx_world_read_solve_order(self._h, _ptr(order), total, _ptr(starts), len(starts), C.byref(k)),
+               self._h)
+        return order[:total], starts[:k.value + 1].copy()
+
+    def islands(self):
+        ids = np.empty(self.counts.bodies, np.uint32)
+        cnt = C.c_uint32()
+        _check(lib().agx_world_read_islands(self._h, _ptr(ids), len(ids), C.byref(cnt)), self._h)
+        return ids, cnt.value
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().agx_world_read_stats(self._h, C.byref(s)), self._h)
+        return s
+
+
+def broadphase_pairs(aabb: np.ndarray, owner: np.ndarray, is_dynamic: np.ndarray, scene=None, cap=None):
+    """BroadPhaseCollisionDetector::findCollisions on injected AABBs (n x 4: minX, minY, maxX, maxY)."""
+    aabb = np.ascontiguousarray(aabb, np.float32)
+    owner = np.ascontiguousarray(owner, np.uint32)
+    dyn = np.ascontiguousarray(is_dynamic, np.uint8)
+    sc = None if scene is None else np.ascontiguousarray(scene, np.uint32)
+    n = len(owner)
+    cap = cap if cap is not None else max(1 << 16, 64 * n)
+    a = np.empty(cap, np.uint32)
+    b = np.empty(cap, np.uint32)
+    cnt = C.c_uint64()
+    _check(lib().agx_broadphase_pairs(_ptr(aabb), _ptr(owner), _ptr(dyn), _ptr(sc), n, _ptr(a), _ptr(b), cap,
+                                      C.byref(cnt)))
+    return a[:cnt.value], b[:cnt.value]
+
+
+def device_count() -> int:
+    n = C.c_int()
+    lib().agx_device_count(C.byref(n))
+    return n.value
+
+
+def build_scene(builder: Builder, scene: dict, scene_index: int = 0) -> np.ndarray:
+    """Creates a scene description's materials (first scene only) and bodies."""
+    if scene_index == 0 or not getattr(builder, "_materials_done", False):
+        for m in scene.get("materials", []):
+            builder.create_material(*m)
+        builder._materials_done = True
+    return builder.create_bodies(scene_index, scene["bodies"])

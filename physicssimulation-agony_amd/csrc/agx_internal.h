@@ -1,0 +1,235 @@
+// Internal declarations of the MI355X collide-and-solve library (device layout, kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+#include "../../include/agx.h"
+
+namespace agx {
+
+constexpr uint32_t WAVE = 64;
+constexpr uint32_t MAX_COLOURS = 1280;     // 128 first-fit colours + one per overflow round
+constexpr uint32_t MAX_COLOUR_ROUNDS = 1024;
+constexpr uint32_t MAX_SWEEPS = 64;
+constexpr uint32_t TRAVERSAL_STACK = 64;
+constexpr uint32_t SORT_TILE = 4096;        // keys per radix-sort block (256 threads x 16)
+constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one scene per chunk)
+
+// Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).
+struct Counters {
+    // persistent across steps (not cleared at step start)
+    uint32_t prevManifolds;    // manifolds of the previous step (warm-start source)
+    uint32_t hashCapacity;     // power of two >= 2 * prevManifolds (>= 16)
+    // per step
+    uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
+    uint32_t manifolds;
+    uint32_t contacts;
+    uint32_t colours;
+    uint32_t islands;
+    uint32_t ccdHits;
+    uint32_t sweeps;
+    uint32_t overflow;         // bit 0 pairs, bit 1 traversal stack, bit 2 colouring, bit 3 hash
+    uint32_t scratch[8];       // scan totals
+    uint32_t colourRounds;
+    uint32_t remaining[MAX_COLOUR_ROUNDS + 1];
+    uint32_t colourCount[MAX_COLOURS + 1];
+    uint32_t colourStart[MAX_COLOURS + 2];
+    uint32_t colourFill[MAX_COLOURS + 1];
+    double residualNum[MAX_SWEEPS];
+    double residualDen[MAX_SWEEPS];
+    float velocityError, positionError;
+    uint32_t barrier[4];       // grid-barrier arrival counters (one per cooperative kernel)
+};
+
+// One BVH over a fixed collider list (dynamic or static), Karras layout:
+// internal nodes [0, n-1), leaves [n-1, 2n-1); node 0 is the root (a leaf when n == 1).
+struct Tree {
+    uint32_t n = 0;
+    uint32_t sortBlocks = 0;
+    uint32_t* keys = nullptr;      // sorted (scene << mortonBits | morton)
+    uint32_t* vals = nullptr;      // sorted collider indices
+    uint32_t* keysAlt = nullptr;
+    uint32_t* valsAlt = nullptr;
+    uint32_t* hist = nullptr;      // 256 * sortBlocks
+    float4* nodeAabb = nullptr;    // 2n-1
+    uint2* child = nullptr;        // n-1: unified child indices
+    uint2* range = nullptr;        // n-1: first, last sorted leaf
+    uint32_t* parent = nullptr;    // 2n-1
+    uint32_t* flags = nullptr;     // n-1 refit arrival counters
+    uint32_t* leafBody = nullptr;  // n
+    uint32_t* sceneStart = nullptr;  // S+1: sorted leaf range of each scene
+    uint32_t* list = nullptr;      // n: colliders of this tree (ascending index)
+    uint32_t* chunkScene = nullptr;  // bounds chunks
+    uint2* chunkRange = nullptr;
+    uint32_t chunks = 0;
+    float4* sceneBounds = nullptr;   // S: (minX, minY, maxX, maxY)
+};
+
+// Constraint planes, one float4 per constraint per plane, in colour order.
+struct Constraints {
+    uint4* idx;      // bodyA, bodyB, count, manifold
+    float4* nrm;     // nx, ny, staticFriction, dynamicFriction
+    float4* inv;     // invMassA, invMassB, invInertiaA, invInertiaB
+    float4* r0;      // rAn0, rBn0, rAt0, rBt0
+    float4* r1;      // rAn1, rBn1, rAt1, rBt1
+    float4* k0;      // normalMass0, tangentMass0, bias0, normalMass1
+    float4* k1;      // tangentMass1, bias1, penetrationBase, unused
+    float4* imp;     // jn0, jt0, jn1, jt1
+};
+
+struct DeviceWorld {
+    uint32_t S = 0, N = 0, C = 0, nDyn = 0, nStat = 0, nVerts = 0, nMat = 0;
+    uint32_t sceneBits = 0, mortonBits = 30;
+    // bodies
+    float4* pos = nullptr;       // comX, comY, rotation, 0
+    float4* vel = nullptr;       // vx, vy, angular velocity, 0
+    float2* rcs = nullptr;       // cos, sin
+    float2* inv = nullptr;       // invMass, invInertia (static iff invMass == 0)
+    float4* prev = nullptr;      // CCD snapshot: comX, comY, cos, sin before integration
+    uint32_t* bodyScene = nullptr;
+    uint32_t* dynBodies = nullptr;   // dynamic body list (ascending)
+    uint32_t* bodyToDyn = nullptr;   // body -> position in dynBodies (UINT32_MAX for static)
+    uint32_t nDynBodies = 0;
+    uint32_t* islandId = nullptr;
+    uint32_t* ufParent = nullptr;
+    uint32_t* ufRoot = nullptr;      // scan scratch (N)
+    float* hitTime = nullptr;        // CCD, per body, 1 between steps
+    // colliders
+    uint32_t* colBody = nullptr;
+    uint32_t* colScene = nullptr;
+    uint32_t* colSlot = nullptr;
+    float2* comOff = nullptr;
+    float* localRot = nullptr;
+    uint4* shape = nullptr;          // type, polyStart, polyCount, material
+    float2* dims = nullptr;          // p0, p1
+    float2* extent = nullptr;        // minExtent, maxExtent
+    float4* pose = nullptr;          // wX, wY, wCos, wSin
+    float* wRot = nullptr;
+    float2* prevW = nullptr;
+    float4* aabb = nullptr;
+    float2* verts = nullptr;
+    float2* normals = nullptr;
+    float4* materials = nullptr;     // elasticity, staticFriction, dynamicFriction, 0
+    // broad phase
+    Tree dyn, stat;
+    uint64_t pairCap = 0;
+    uint2* pairs = nullptr;
+    uint32_t* waveCount = nullptr;   // per query wave (self + cross passes)
+    uint32_t* waveOffset = nullptr;
+    uint32_t queryWaves = 0;
+    // narrow phase
+    uint64_t manCap = 0;
+    agx_manifold* slots = nullptr;   // per pair slot
+    uint32_t* flags = nullptr;       // per pair: 1 if a manifold was written
+    uint32_t* flagScan = nullptr;
+    agx_manifold* man[2] = { nullptr, nullptr };   // current / previous (ping-pong)
+    int cur = 0;
+    unsigned long long* hashKeys = nullptr;
+    uint32_t* hashVals = nullptr;
+    uint64_t hashCapMax = 0;
+    // solver
+    uint8_t* colour = nullptr;       // low 8 bits of the colour (first-fit)
+    uint16_t* colourId = nullptr;    // full colour id per manifold
+    unsigned long long* claim = nullptr;   // per body
+    unsigned long long* used = nullptr;    // per body, 2 words (128 colours)
+    uint32_t* order = nullptr;       // constraint slot -> manifold
+    Constraints cs{};
+    // scan scratch
+    uint32_t* scanBlock = nullptr;
+    uint32_t scanBlockCap = 0;
+    Counters* ctr = nullptr;
+    // launch shape
+    int numCU = 0;
+    int coopBlocks = 0;              // co-resident blocks for cooperative kernels
+};
+
+// ---------------------------------------------------------------- device math
+
+namespace dm {
+constexpr double FULL_PI = 3.14159265358979323846;
+constexpr float HALF_PI = (float)(FULL_PI * 0.5);
+constexpr float INV_HALF_PI = (float)(1.0 / (FULL_PI * 0.5));
+constexpr float TWO_PI = (float)(FULL_PI * 2.0);
+constexpr float INV_TWO_PI = (float)(1.0 / (FULL_PI * 2.0));
+// The reference's odd polynomial for sin on [0, pi/2] (FastCosSin.cpp:10-14).
+constexpr float C0 = (float)9.9999999999953215e-01;
+constexpr float C1 = (float)-1.6666666534403832e-01;
+constexpr float C2 = (float)8.3332302391869520e-03;
+constexpr float C3 = (float)-1.9818200679665358e-04;
+constexpr float C4 = (float)2.6182346015248291e-06;
+
+// FastCosSin.cpp:44-75: quadrant by truncation, mirrored angle, polynomial, clamp, sqrt.
+__host__ __device__ inline void fastCosSin(float x, float& c, float& s) {
+    const int q = (int)(x * INV_HALF_PI);
+    const float u = x - (float)q * HALF_PI;
+    const float mirror = (float)(q & 1);
+    const float a = mirror * (HALF_PI - 2.0f * u) + u;
+    const float a2 = a * a;
+    float sm = C4;
+    sm = sm * a2 + C3;
+    sm = sm * a2 + C2;
+    sm = sm * a2 + C1;
+    sm = sm * a2 + C0;
+    sm = sm * a;
+    sm = fminf(sm, 1.0f);
+    const float cm = sqrtf(1.0f - sm * sm);
+    s = (q & 2) ? -sm : sm;
+    c = ((q + 1) & 2) ? -cm : cm;
+}
+
+// LeftOutFunctions.cpp:30-43: wrap to [0, 2pi) with a fused multiply-subtract.
+__host__ __device__ inline float wrapRotation(float r) {
+    const float wrapCount = truncf(r * INV_TWO_PI);
+    r = fmaf(-wrapCount, TWO_PI, r);
+    if (r < 0.0f) r += TWO_PI;
+    return r;
+}
+}  // namespace dm
+
+// ---------------------------------------------------------------- grid barrier
+// Cooperative kernels only (all blocks co-resident). Every wave waits for its own stores,
+// the block synchronises, thread 0 publishes with an agent-scope release (L2 write-back)
+// and waits with an agent-scope acquire (L2 invalidate) until all blocks have arrived.
+__device__ inline void gridBarrier(uint32_t* counter, uint32_t nblocks, uint32_t& generation) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        generation++;
+        const uint32_t target = generation * nblocks;
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+            __builtin_amdgcn_s_sleep(1);
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- host-side launchers
+// Each returns hipSuccess or the first launch error. All run on 'st'.
+hipError_t launchFillU32(hipStream_t st, uint32_t* p, uint32_t v, size_t n);
+hipError_t launchFillBytes(hipStream_t st, void* p, size_t bytes);   // zero fill
+// Exclusive scan of in[0..n) into out (in place allowed); total into *totalPtr (may be null).
+// n = *countPtr when countPtr is not null (device-side count, bounded by countHost), else countHost.
+hipError_t launchScan(hipStream_t st, const uint32_t* in, uint32_t* out, const uint32_t* countPtr,
+                      uint32_t* totalPtr, uint64_t countHost, uint32_t* blockScratch, uint32_t blockCap);
+
+hipError_t launchRefreshCosSin(hipStream_t st, DeviceWorld& w);
+hipError_t launchIntegrate(hipStream_t st, DeviceWorld& w, float dt, float gx, float gy, int snapshot);
+hipError_t launchPose(hipStream_t st, DeviceWorld& w, int all, int prevFromSnapshot, int writePrev);
+hipError_t launchCcd(hipStream_t st, DeviceWorld& w, float fastFactor, float coreFraction);
+
+hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t);
+hipError_t launchPairs(hipStream_t st, DeviceWorld& w);
+
+hipError_t launchNarrow(hipStream_t st, DeviceWorld& w);
+hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w);
+
+hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
+hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
+                       float residualTarget, int trackErrors);
+hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
+// Co-resident blocks per CU for the cooperative solver kernels at 256 threads.
+int cooperativeBlocksPerCU();
+
+}  // namespace agx

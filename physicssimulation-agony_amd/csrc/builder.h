@@ -1,0 +1,55 @@
+// Host-side scene building for the device world: ObjectManager creation semantics
+// (SimulationImpl/ObjectManager.cpp:71-389, PhysicsGeometry.cpp:14-298).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace agx {
+
+struct Vec { float x, y; };
+
+// One scene in dense order: [Static | Dynamic] bodies, colliders following their owner's group.
+struct SceneData {
+    std::vector<float> comX, comY, velX, velY, rot, angVel, mass, invMass, inertia, invInertia, cosv, sinv;
+    std::vector<float> localComX, localComY, boundRadius;
+    std::vector<uint32_t> bodySlot;
+    std::vector<std::vector<uint32_t>> bodyColliders;
+    uint32_t staticBodies = 0;
+    std::vector<uint32_t> colBody, colMaterial, colSlot, polyStart, polyCount;
+    std::vector<uint8_t> colType;
+    std::vector<float> localOffX, localOffY, comOffX, comOffY, localRot, wCos, wSin, p0, p1, minExtent, maxExtent;
+    uint32_t staticColliders = 0;
+    std::vector<Vec> verts, normals;
+
+    uint32_t bodyCount() const { return (uint32_t)comX.size(); }
+    uint32_t colliderCount() const { return (uint32_t)colBody.size(); }
+
+    // Returns the new body's dense index or UINT32_MAX (invalid polygon outline).
+    uint32_t createCircle(float px, float py, float rot, float vx, float vy, float av, float mass,
+                          uint32_t material, float radius);
+    uint32_t createBox(float px, float py, float rot, float vx, float vy, float av, float mass,
+                       uint32_t material, float width, float height);
+    uint32_t createPolygon(float px, float py, float rot, float vx, float vy, float av, float mass,
+                           uint32_t material, const Vec* verts, uint32_t count);
+    uint32_t materialCount = 1;   // set by the builder owner so out-of-range materials map to 0
+
+private:
+    uint32_t appendBody(Vec pos, Vec vel, float rot, float av, float mass, float invMass, float inertia,
+                        float invInertia, Vec localCom);
+    uint32_t appendCollider(uint32_t body, Vec off, float localRot, uint32_t material, uint8_t type, float p0,
+                            float p1, uint32_t ps, uint32_t pc);
+    void placeCollider(uint32_t c, float minExt, float maxExt);
+    void swapBodies(uint32_t a, uint32_t b);
+    void swapColliders(uint32_t a, uint32_t b);
+};
+
+struct Material { float elasticity, staticFriction, dynamicFriction; };
+
+struct Builder {
+    std::vector<SceneData> scenes;
+    std::vector<Material> materials{ Material{ 1.0f, 0.0f, 0.0f } };   // Types/Material.h:8-10 default
+};
+
+}  // namespace agx

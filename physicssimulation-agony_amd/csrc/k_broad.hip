@@ -1,0 +1,395 @@
+// Broad phase: per-scene bounds, Morton keys, LDS radix sort, Karras LBVH, refit, pair traversal.
+//
+// The pair set equals the reference's (Broad/BroadPhaseCollisionDetector.cpp:109-244): node
+// AABBs are exact unions, so pruning with the strict overlap test never drops a strictly
+// overlapping leaf pair and the set does not depend on tree topology.
+#include "agx_internal.h"
+
+#include <cfloat>
+
+namespace agx {
+
+namespace {
+
+// Order-preserving float <-> int mapping for atomic min/max on floats.
+__device__ inline int floatOrder(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ inline float orderFloat(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__global__ void kBoundsInit(int4* b, uint32_t S) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) b[s] = make_int4(floatOrder(FLT_MAX), floatOrder(FLT_MAX), floatOrder(-FLT_MAX), floatOrder(-FLT_MAX));
+}
+
+// Bounds of each scene's AABBs (BvhTree.cpp:148-188), one block per chunk of one scene.
+__global__ __launch_bounds__(256) void kBounds(const uint32_t* __restrict__ list, const float4* __restrict__ aabb,
+                                               const uint32_t* __restrict__ chunkScene, const uint2* __restrict__ chunkRange,
+                                               int4* __restrict__ bounds) {
+    const uint2 r = chunkRange[blockIdx.x];
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    for (uint32_t k = r.x + threadIdx.x; k < r.y; k += 256) {
+        const float4 a = aabb[list[k]];
+        mnx = fminf(mnx, a.x); mny = fminf(mny, a.y); mxx = fmaxf(mxx, a.z); mxy = fmaxf(mxy, a.w);
+    }
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+        mnx = fminf(mnx, __shfl_down(mnx, off, 64)); mny = fminf(mny, __shfl_down(mny, off, 64));
+        mxx = fmaxf(mxx, __shfl_down(mxx, off, 64)); mxy = fmaxf(mxy, __shfl_down(mxy, off, 64));
+    }
+    __shared__ float4 ws[4];
+    if ((threadIdx.x & 63u) == 0) ws[threadIdx.x >> 6] = make_float4(mnx, mny, mxx, mxy);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float4 t = ws[0];
+        for (int k = 1; k < 4; k++) {
+            t.x = fminf(t.x, ws[k].x); t.y = fminf(t.y, ws[k].y); t.z = fmaxf(t.z, ws[k].z); t.w = fmaxf(t.w, ws[k].w);
+        }
+        int4* b = bounds + chunkScene[blockIdx.x];
+        atomicMin(&b->x, floatOrder(t.x));
+        atomicMin(&b->y, floatOrder(t.y));
+        atomicMax(&b->z, floatOrder(t.z));
+        atomicMax(&b->w, floatOrder(t.w));
+    }
+}
+
+__device__ inline uint32_t part1By1(uint32_t x) {   // spreads the low 16 bits to even positions
+    x &= 0x0000ffffu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// Morton key of the AABB centre quantised to the scene bounds (BvhTree.cpp:191-280, with
+// mortonBits/2 bits per axis), scene id above the Morton bits.
+__global__ __launch_bounds__(256) void kMorton(const uint32_t* __restrict__ list, uint32_t n, const float4* __restrict__ aabb,
+                                               const uint32_t* __restrict__ colScene, const int4* __restrict__ bounds,
+                                               uint32_t mortonBits, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t c = list[k];
+    const uint32_t s = colScene[c];
+    const int4 bi = bounds[s];
+    const float bMinX = orderFloat(bi.x), bMinY = orderFloat(bi.y);
+    const float bMaxX = fmaxf(orderFloat(bi.z), bMinX + 1e-5f), bMaxY = fmaxf(orderFloat(bi.w), bMinY + 1e-5f);
+    const uint32_t axisBits = mortonBits >> 1;
+    const float qMax = (float)((1u << axisBits) - 1u);
+    const float scaleX = qMax / (bMaxX - bMinX), scaleY = qMax / (bMaxY - bMinY);
+    const float4 a = aabb[c];
+    const float tx = fminf(fmaxf((a.x + a.z) * (scaleX * 0.5f) + (-bMinX * scaleX), 0.0f), qMax);
+    const float ty = fminf(fmaxf((a.y + a.w) * (scaleY * 0.5f) + (-bMinY * scaleY), 0.0f), qMax);
+    const uint32_t code = (part1By1((uint32_t)ty) << 1) | part1By1((uint32_t)tx);
+    keys[k] = mortonBits >= 32 ? code : ((s << mortonBits) | code);
+    vals[k] = c;
+}
+
+// ---------------------------------------------------------------- radix sort (8-bit digits)
+// Pass 1: per-block digit histogram into hist[digit * blocks + block].
+__global__ __launch_bounds__(256) void kRadixHist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                  uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    for (uint32_t k = threadIdx.x; k < SORT_TILE; k += 256) {
+        const uint32_t i = base + k;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Pass 3: stable scatter. Each wave ranks 16 rounds of 64 consecutive keys with ballot
+// matching (8 ballots per round) and per-wave digit counters in LDS; waves are combined in
+// order, then added to the scanned global digit offsets.
+__global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                     uint32_t n, uint32_t shift, const uint32_t* __restrict__ histScan,
+                                                     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
+    constexpr uint32_t ROUNDS = SORT_TILE / 256;   // 16 keys per lane
+    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t base[4][256];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    for (uint32_t w = 0; w < 4; w++) cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rk[ROUNDS], dg[ROUNDS], ky[ROUNDS], vl[ROUNDS];
+    const uint32_t waveBase = blockIdx.x * SORT_TILE + wid * (SORT_TILE / 4);
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+        const uint32_t i = waveBase + r * 64 + lane;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys[i] : 0u;
+        const uint32_t d = (key >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+        for (uint32_t bit = 0; bit < 8; bit++) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t m = __ballot(set);
+            peers &= set ? m : ~m;
+        }
+        uint32_t before = 0;
+        if (valid) before = cnt[wid][d];
+        const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)peers) - 1);
+        if (valid && lane == leader) cnt[wid][d] = before + (uint32_t)__popcll(peers);
+        rk[r] = before + (uint32_t)__popcll(peers & ltMask);
+        dg[r] = d;
+        ky[r] = key;
+        vl[r] = valid ? vals[i] : 0u;
+    }
+    __syncthreads();
+    {
+        const uint32_t d = threadIdx.x;
+        uint32_t run = histScan[d * gridDim.x + blockIdx.x];
+        for (uint32_t w = 0; w < 4; w++) { base[w][d] = run; run += cnt[w][d]; }
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+        const uint32_t i = waveBase + r * 64 + lane;
+        if (i < n) {
+            const uint32_t dst = base[wid][dg[r]] + rk[r];
+            keysOut[dst] = ky[r];
+            valsOut[dst] = vl[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- Karras LBVH
+__device__ inline int prefixDelta(const uint32_t* __restrict__ keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = keys[i], b = keys[j];
+    return a == b ? 32 + __clz((int)((uint32_t)i ^ (uint32_t)j)) : __clz((int)(a ^ b));
+}
+
+// Karras 2012: one thread per internal node; ties broken by leaf index.
+__global__ __launch_bounds__(256) void kKarras(const uint32_t* __restrict__ keys, uint32_t n, uint2* __restrict__ child,
+                                               uint2* __restrict__ range, uint32_t* __restrict__ parent,
+                                               uint32_t* __restrict__ flags) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int nn = (int)n;
+    if (i >= nn - 1) return;
+    const int dR = prefixDelta(keys, nn, i, i + 1), dL = prefixDelta(keys, nn, i, i - 1);
+    const int d = dR > dL ? 1 : -1;
+    const int dMin = prefixDelta(keys, nn, i, i - d);
+    int lMax = 2;
+    while (prefixDelta(keys, nn, i, i + lMax * d) > dMin) lMax <<= 1;
+    int l = 0;
+    for (int t = lMax >> 1; t >= 1; t >>= 1)
+        if (prefixDelta(keys, nn, i, i + (l + t) * d) > dMin) l += t;
+    const int j = i + l * d;
+    const int dNode = prefixDelta(keys, nn, i, j);
+    int s = 0;
+    int t = l;
+    while (t > 1) {
+        t = (t + 1) >> 1;
+        if (prefixDelta(keys, nn, i, i + (s + t) * d) > dNode) s += t;
+    }
+    const int gamma = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const uint32_t leafBase = n - 1;
+    const uint32_t left = (lo == gamma) ? leafBase + (uint32_t)gamma : (uint32_t)gamma;
+    const uint32_t right = (hi == gamma + 1) ? leafBase + (uint32_t)(gamma + 1) : (uint32_t)(gamma + 1);
+    child[i] = make_uint2(left, right);
+    range[i] = make_uint2((uint32_t)lo, (uint32_t)hi);
+    parent[left] = (uint32_t)i;
+    parent[right] = (uint32_t)i;
+    flags[i] = 0;
+    if (i == 0) parent[0] = 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(256) void kLeaves(const uint32_t* __restrict__ vals, uint32_t n, const float4* __restrict__ aabb,
+                                               const uint32_t* __restrict__ colBody, float4* __restrict__ nodeAabb,
+                                               uint32_t* __restrict__ leafBody, uint32_t* __restrict__ parent) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t c = vals[j];
+    nodeAabb[n - 1 + j] = aabb[c];
+    leafBody[j] = colBody[c];
+    if (n == 1) parent[0] = 0xFFFFFFFFu;
+}
+
+__device__ inline void storeCoherent(float4* p, float4 v) {
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+    __hip_atomic_store(q, ((uint64_t)__float_as_uint(v.y) << 32) | __float_as_uint(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, ((uint64_t)__float_as_uint(v.w) << 32) | __float_as_uint(v.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float4 loadCoherent(const float4* p) {
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+    const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)),
+                       __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+}
+
+// Bottom-up refit (BvhTree.cpp:426-530 computes the same unions top-down in reverse order):
+// the second thread to reach a node merges its children. Node boxes written here use
+// agent-coherent stores, completed before the arrival counter is bumped.
+__global__ __launch_bounds__(256) void kRefit(uint32_t n, const uint2* __restrict__ child, const uint32_t* __restrict__ parent,
+                                              uint32_t* __restrict__ flags, float4* __restrict__ nodeAabb) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || n < 2) return;
+    uint32_t node = n - 1 + j;
+    while (true) {
+        const uint32_t p = parent[node];
+        if (p == 0xFFFFFFFFu) break;
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) break;
+        const uint2 ch = child[p];
+        const float4 a = loadCoherent(nodeAabb + ch.x);
+        const float4 b = loadCoherent(nodeAabb + ch.y);
+        storeCoherent(nodeAabb + p, make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w)));
+        node = p;
+    }
+}
+
+// ---------------------------------------------------------------- pair traversal
+// One query per lane, one wave-uniform stack of internal nodes in LDS (packet traversal):
+// queries are consecutive Morton-sorted dynamic leaves, so a wave's paths overlap. A child is
+// visited when any lane's strict overlap test passes (ballot); leaf children emit pairs for
+// the passing lanes. SELF restricts a query to leaves after it (each pair once) and skips
+// pairs of one body. COUNT: per-wave totals. WRITE: pairs at the scanned wave offsets, in the
+// same deterministic order.
+struct Query { float4 box; uint32_t leaf, col, body, sf, sl; bool valid; };
+
+template <bool SELF>
+__device__ inline bool childActive(const Query& q, const Tree& t, uint32_t node, float4& box) {
+    const uint32_t leafBase = t.n - 1;
+    uint32_t first, last;
+    if (node >= leafBase) { first = last = node - leafBase; }
+    else { const uint2 r = t.range[node]; first = r.x; last = r.y; }
+    box = t.nodeAabb[node];
+    if (!q.valid) return false;
+    if (first >= q.sl || last < q.sf) return false;
+    if (SELF && last <= q.leaf) return false;
+    return q.box.x < box.z && q.box.z > box.x && q.box.y < box.w && q.box.w > box.y;
+}
+
+template <bool SELF, bool WRITE>
+__global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_t* __restrict__ colScene,
+                                                 uint32_t* __restrict__ waveCount, const uint32_t* __restrict__ waveOffset,
+                                                 uint32_t waveBaseIndex, uint2* __restrict__ pairs, uint64_t cap,
+                                                 Counters* ctr) {
+    __shared__ uint32_t stackMem[4][TRAVERSAL_STACK];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t* stack = stackMem[wid];
+    const uint32_t waveGlobal = blockIdx.x * 4 + wid;
+    if (waveGlobal * 64 >= qt.n) return;   // whole wave past the last query (wave-uniform exit)
+    const uint32_t i = waveGlobal * 64 + lane;
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    Query q;
+    q.valid = i < qt.n;
+    q.leaf = i;
+    q.col = q.valid ? qt.vals[i] : 0u;
+    q.box = q.valid ? qt.nodeAabb[qt.n - 1 + i] : make_float4(0, 0, 0, 0);
+    q.body = q.valid ? qt.leafBody[i] : 0u;
+    const uint32_t scene = q.valid ? colScene[q.col] : 0u;
+    q.sf = q.valid ? tt.sceneStart[scene] : 0u;
+    q.sl = q.valid ? tt.sceneStart[scene + 1] : 0u;
+    uint32_t written = 0;
+    const uint64_t outBase = WRITE ? (uint64_t)waveOffset[waveBaseIndex + waveGlobal] : 0ull;
+    const uint32_t leafBase = tt.n - 1;
+    auto emitLeaf = [&](uint32_t node, bool act) {
+        const uint32_t j = node - leafBase;
+        const uint32_t other = tt.vals[j];
+        if (SELF) act = act && tt.leafBody[j] != q.body;
+        const uint64_t m = __ballot(act);
+        if (WRITE && act) {
+            const uint64_t dst = outBase + written + (uint64_t)__popcll(m & ltMask);
+            if (dst < cap) pairs[dst] = make_uint2(q.col, other);
+        }
+        written += (uint32_t)__popcll(m);
+    };
+    if (tt.n > 0 && __ballot(q.valid) != 0) {
+        uint32_t sp = 0;
+        float4 box;
+        const bool rootAct = childActive<SELF>(q, tt, 0, box);
+        if (tt.n == 1) {
+            emitLeaf(0, rootAct);
+        } else if (__ballot(rootAct) != 0) {
+            stack[sp++] = 0;
+        }
+        while (sp > 0) {
+            const uint32_t node = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
+            sp--;
+            const uint2 ch = tt.child[node];
+            float4 bl, br;
+            const bool al = childActive<SELF>(q, tt, ch.x, bl);
+            const bool ar = childActive<SELF>(q, tt, ch.y, br);
+            const bool leafL = ch.x >= leafBase, leafR = ch.y >= leafBase;
+            if (leafL) emitLeaf(ch.x, al);
+            if (leafR) emitLeaf(ch.y, ar);
+            const bool pushR = !leafR && __ballot(ar) != 0;
+            const bool pushL = !leafL && __ballot(al) != 0;
+            if (sp + (pushR ? 1 : 0) + (pushL ? 1 : 0) > TRAVERSAL_STACK) {
+                if (lane == 0) atomicOr(&ctr->overflow, 2u);
+                break;
+            }
+            if (pushR) { if (lane == 0) stack[sp] = ch.y; sp++; }
+            if (pushL) { if (lane == 0) stack[sp] = ch.x; sp++; }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (!WRITE && lane == 0) waveCount[waveBaseIndex + waveGlobal] = written;
+}
+
+__global__ void kPairTotal(Counters* ctr, uint64_t cap, const uint32_t* total) {
+    ctr->pairs = *total;
+    if ((uint64_t)*total > cap) ctr->overflow |= 1u;
+}
+
+inline uint32_t blocksFor(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
+
+}  // namespace
+
+hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t) {
+    if (t.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kBoundsInit, dim3(blocksFor(w.S, 256)), dim3(256), 0, st, reinterpret_cast<int4*>(t.sceneBounds), w.S);
+    hipLaunchKernelGGL(kBounds, dim3(t.chunks), dim3(256), 0, st, t.list, w.aabb, t.chunkScene, t.chunkRange,
+                       reinterpret_cast<int4*>(t.sceneBounds));
+    hipLaunchKernelGGL(kMorton, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.list, t.n, w.aabb, w.colScene,
+                       reinterpret_cast<const int4*>(t.sceneBounds), w.mortonBits, t.keys, t.vals);
+    const uint32_t keyBits = w.mortonBits + w.sceneBits;
+    const uint32_t passes = (keyBits + 7) / 8;
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t shift = p * 8;
+        hipLaunchKernelGGL(kRadixHist, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.n, shift, t.hist);
+        const uint64_t histN = 256ull * t.sortBlocks;
+        hipError_t e = launchScan(st, t.hist, t.hist, nullptr, nullptr, histN, w.scanBlock, w.scanBlockCap);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kRadixScatter, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, shift, t.hist,
+                           t.keysAlt, t.valsAlt);
+        std::swap(t.keys, t.keysAlt);
+        std::swap(t.vals, t.valsAlt);
+    }
+    if (t.n > 1)
+        hipLaunchKernelGGL(kKarras, dim3(blocksFor(t.n - 1, 256)), dim3(256), 0, st, t.keys, t.n, t.child, t.range,
+                           t.parent, t.flags);
+    hipLaunchKernelGGL(kLeaves, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.vals, t.n, w.aabb, w.colBody, t.nodeAabb,
+                       t.leafBody, t.parent);
+    if (t.n > 1)
+        hipLaunchKernelGGL(kRefit, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.n, t.child, t.parent, t.flags, t.nodeAabb);
+    return hipGetLastError();
+}
+
+hipError_t launchPairs(hipStream_t st, DeviceWorld& w) {
+    if (w.dyn.n == 0) {
+        return launchFillU32(st, &w.ctr->pairs, 0u, 1);
+    }
+    const uint32_t waves = w.queryWaves;          // ceil(nDyn / 64)
+    const uint32_t blocks = blocksFor(waves, 4);
+    hipLaunchKernelGGL((kTraverse<true, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
+                       w.waveOffset, 0u, w.pairs, w.pairCap, w.ctr);
+    hipLaunchKernelGGL((kTraverse<false, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
+                       w.waveOffset, waves, w.pairs, w.pairCap, w.ctr);
+    uint32_t* total = &w.ctr->scratch[0];
+    hipError_t e = launchScan(st, w.waveCount, w.waveOffset, nullptr, total, 2ull * waves, w.scanBlock, w.scanBlockCap);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((kTraverse<true, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
+                       w.waveOffset, 0u, w.pairs, w.pairCap, w.ctr);
+    hipLaunchKernelGGL((kTraverse<false, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
+                       w.waveOffset, waves, w.pairs, w.pairCap, w.ctr);
+    hipLaunchKernelGGL(kPairTotal, dim3(1), dim3(1), 0, st, w.ctr, w.pairCap, total);
+    return hipGetLastError();
+}
+
+}  // namespace agx

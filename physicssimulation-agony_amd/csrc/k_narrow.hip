@@ -1,0 +1,590 @@
+// Narrow phase: one lane per broad-phase pair, canonical A/B, SAT + clipping manifolds
+// (Narrow/CollisionMethods/*.cpp), deterministic compaction and the warm-start cache
+// (Narrow/ContactCache.h, NarrowPhaseCollisionDetector.cpp:272-297, 692-767).
+#include "agx_internal.h"
+
+#include <cfloat>
+
+namespace agx {
+
+namespace {
+
+struct V { float x, y; };
+__device__ inline V operator+(V a, V b) { return { a.x + b.x, a.y + b.y }; }
+__device__ inline V operator-(V a, V b) { return { a.x - b.x, a.y - b.y }; }
+__device__ inline V operator-(V a) { return { -a.x, -a.y }; }
+__device__ inline V operator*(V a, float s) { return { a.x * s, a.y * s }; }
+__device__ inline V operator*(float s, V a) { return { s * a.x, s * a.y }; }
+__device__ inline float dot(V a, V b) { return a.x * b.x + a.y * b.y; }
+__device__ inline V ld(const float2* p, uint32_t i) { const float2 v = p[i]; return { v.x, v.y }; }
+
+constexpr float ZERO_DIV = 1e-4f;          // Narrow/Constants.h:7-9
+constexpr float ZERO_DIV_SQ = 1e-8f;
+constexpr float SAT_EPS = 1e-4f;
+constexpr uint32_t REF_TAG = 0x80000000u;  // Narrow/Constants.h:23
+
+struct Shapes {
+    const float4* pose;
+    const uint4* shape;
+    const float2* dims;
+    const float2* verts;
+    const float2* normals;
+};
+
+__device__ inline uint32_t nextIdx(uint32_t i, uint32_t n) { return i + 1 == n ? 0u : i + 1; }
+
+__device__ inline V flipSign(V v, float sign) {      // HelperMethods.h:23-31
+    const uint32_t sb = __float_as_uint(sign) & 0x80000000u;
+    return { __uint_as_float(__float_as_uint(v.x) ^ sb), __uint_as_float(__float_as_uint(v.y) ^ sb) };
+}
+
+__device__ inline bool clipSegment(V& p1, V& p2, V pp, V pn) {   // HelperMethods.h:33-52
+    const float d1 = dot(p1 - pp, pn);
+    const float d2 = dot(p2 - pp, pn);
+    if (d1 >= 0 && d2 >= 0) return false;
+    if (d1 < 0 && d2 < 0) return true;
+    const float t = d1 / (d1 - d2);
+    const V dir = p2 - p1;
+    const V inter = p1 + dir * t;
+    if (d1 >= 0) p2 = inter; else p1 = inter;
+    return false;
+}
+
+__device__ inline void emit(agx_manifold& m, uint32_t a, uint32_t b, V n, float depth, uint32_t count, V p0, V p1,
+                            uint32_t id0, uint32_t id1) {
+    m.a = a; m.b = b; m.normal_x = n.x; m.normal_y = n.y; m.depth = depth; m.count = count;
+    m.p0_x = p0.x; m.p0_y = p0.y;
+    m.p1_x = count > 1 ? p1.x : 0.0f; m.p1_y = count > 1 ? p1.y : 0.0f;
+    m.id0 = id0; m.id1 = count > 1 ? id1 : 0u;
+    m.jn0 = 0.0f; m.jt0 = 0.0f; m.jn1 = 0.0f; m.jt1 = 0.0f;
+}
+
+// CircleCircle.cpp:130-176.
+__device__ bool circleCircle(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V pA{ pa.x, pa.y }, pB{ pb.x, pb.y };
+    const float rA = s.dims[a].x, rB = s.dims[b].x;
+    const V delta = pB - pA;
+    const float rsum = rA + rB;
+    const float sq = dot(delta, delta);
+    if (sq >= rsum * rsum) return false;
+    const float dist = sqrtf(sq);
+    const float depth = rsum - dist;
+    V n;
+    if (dist < ZERO_DIV) n = { 1.0f, 0.0f };
+    else { const float inv = 1.0f / dist; n = delta * inv; }
+    emit(out, a, b, n, depth, 1, pA + n * rA, { 0, 0 }, 0, 0);
+    return true;
+}
+
+// CircleBox.cpp:9-120.
+__device__ bool circleBox(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V pA{ pa.x, pa.y }, pB{ pb.x, pb.y };
+    const float rA = s.dims[a].x, hw = s.dims[b].x, hh = s.dims[b].y;
+    const float cosB = pb.z, sinB = pb.w;
+    const V dg = pA - pB;
+    const V cl{ dg.x * cosB + dg.y * sinB, dg.x * -sinB + dg.y * cosB };
+    const V closest{ fminf(fmaxf(cl.x, -hw), hw), fminf(fmaxf(cl.y, -hh), hh) };
+    const V dl = closest - cl;
+    const float sq = dot(dl, dl);
+    if (sq >= rA * rA) return false;
+    if (sq >= ZERO_DIV_SQ) {
+        const float dist = sqrtf(sq);
+        const float inv = 1.0f / dist;
+        const V wd{ cosB * dl.x - sinB * dl.y, sinB * dl.x + cosB * dl.y };
+        const V n = wd * inv;
+        uint32_t id = (uint32_t)(cl.x > hw);
+        id |= (uint32_t)(cl.x < -hw) << 1;
+        id |= (uint32_t)(cl.y > hh) << 2;
+        id |= (uint32_t)(cl.y < -hh) << 3;
+        emit(out, a, b, n, rA - dist, 1, pA + n * rA, { 0, 0 }, id, 0);
+        return true;
+    }
+    const float dx = hw - fabsf(cl.x);
+    const float dy = hh - fabsf(cl.y);
+    const bool useY = dy <= dx;
+    const float minPen = useY ? dy : dx;
+    const float sign = copysignf(1.0f, useY ? cl.y : cl.x);
+    V nl{ 0.0f, 0.0f };
+    if (useY) nl.y = sign; else nl.x = sign;
+    const uint32_t id = 1u << ((uint32_t)(sign <= 0) + ((uint32_t)useY << 1));
+    const V n = -V{ cosB * nl.x - sinB * nl.y, sinB * nl.x + cosB * nl.y };
+    emit(out, a, b, n, rA + minPen, 1, pA + n * rA, { 0, 0 }, id, 0);
+    return true;
+}
+
+// CirclePolygon.cpp:10-103 with HelperMethods.h:73-106.
+__device__ bool circlePolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V pA{ pa.x, pa.y }, pB{ pb.x, pb.y };
+    const float cosB = pb.z, sinB = pb.w;
+    const float rA = s.dims[a].x;
+    const uint4 sh = s.shape[b];
+    const float2* v = s.verts + sh.y;
+    const float2* nn = s.normals + sh.y;
+    const uint32_t count = sh.z;
+    const V dg = pA - pB;
+    const V cl{ dg.x * cosB + dg.y * sinB, dg.x * -sinB + dg.y * cosB };
+    float maxSep = -FLT_MAX;
+    uint32_t edge = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const float sep = dot(ld(nn, i), cl - ld(v, i));
+        if (sep > rA) return false;
+        if (sep > maxSep) { maxSep = sep; edge = i; }
+    }
+    V nl;
+    float sep;
+    if (maxSep >= ZERO_DIV) {
+        const V va = ld(v, edge), vb = ld(v, nextIdx(edge, count));
+        const V ab = vb - va;
+        const float denom = dot(ab, ab);
+        V cp;
+        if (denom < ZERO_DIV_SQ) cp = va;
+        else { const float t = dot(cl - va, ab) / denom; cp = va + ab * fminf(fmaxf(t, 0.0f), 1.0f); }
+        const V d = cl - cp;
+        const float dsq = dot(d, d);
+        if (dsq > rA * rA) return false;
+        if (dsq < ZERO_DIV_SQ) { sep = 0.0f; nl = ld(nn, edge); }
+        else { sep = sqrtf(dsq); nl = d * (1.0f / sep); }
+    } else {
+        nl = ld(nn, edge);
+        sep = maxSep;
+    }
+    const V n = -V{ cosB * nl.x - sinB * nl.y, sinB * nl.x + cosB * nl.y };
+    emit(out, a, b, n, rA - sep, 1, pA + n * rA, { 0, 0 }, edge, 0);
+    return true;
+}
+
+// BoxBox.cpp:13-227.
+__device__ bool boxBox(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V pA{ pa.x, pa.y }, pB{ pb.x, pb.y };
+    const float cosA = pa.z, sinA = pa.w, cosB = pb.z, sinB = pb.w;
+    const float hwA = s.dims[a].x, hhA = s.dims[a].y, hwB = s.dims[b].x, hhB = s.dims[b].y;
+    const V rightA{ cosA, sinA }, upA{ -sinA, cosA }, rightB{ cosB, sinB }, upB{ -sinB, cosB };
+    const float relCos = cosA * cosB + sinA * sinB;
+    const float relSin = cosA * sinB - sinA * cosB;
+    const float arc = fabsf(relCos), ars = fabsf(relSin);
+    const V cd = pB - pA;
+    // Axes [A right, A up, B right, B up]: projected radii sum and centre distance.
+    const float ov0r = hwB * arc + hhB * ars + hwA, ov1r = hwB * ars + hhB * arc + hhA;
+    const float ov2r = hwA * arc + hhA * ars + hwB, ov3r = hwA * ars + hhA * arc + hhB;
+    const float ax[4] = { cosA, -sinA, cosB, -sinB };
+    const float ay[4] = { sinA, cosA, sinB, cosB };
+    float cdo[4], ov[4];
+    const float rr[4] = { ov0r, ov1r, ov2r, ov3r };
+#pragma unroll
+    for (int k = 0; k < 4; k++) { cdo[k] = cd.x * ax[k] + cd.y * ay[k]; ov[k] = rr[k] - fabsf(cdo[k]); }
+    if (ov[0] < 0.0f || ov[1] < 0.0f || ov[2] < 0.0f || ov[3] < 0.0f) return false;
+    const float depth = fminf(fminf(ov[0], ov[1]), fminf(ov[2], ov[3]));
+    const uint32_t n0 = ov[0] != depth;
+    const uint32_t n01 = n0 & (uint32_t)(ov[1] != depth);
+    const uint32_t n012 = n01 & (uint32_t)(ov[2] != depth);
+    const uint32_t best = n0 + n01 + n012;
+    float bax = ax[0], bay = ay[0], bcd = cdo[0];
+#pragma unroll
+    for (uint32_t k = 1; k < 4; k++) if (best == k) { bax = ax[k]; bay = ay[k]; bcd = cdo[k]; }
+    const V normal = flipSign({ bax, bay }, bcd);
+    const bool refIsA = best < 2;
+    const V pRef = refIsA ? pA : pB, pInc = refIsA ? pB : pA;
+    const V rRef = refIsA ? rightA : rightB, uRef = refIsA ? upA : upB;
+    const V rInc = refIsA ? rightB : rightA, uInc = refIsA ? upB : upA;
+    const float hwRef = refIsA ? hwA : hwB, hhRef = refIsA ? hhA : hhB;
+    const float hwInc = refIsA ? hwB : hwA, hhInc = refIsA ? hhB : hhA;
+    const V refNormal = refIsA ? normal : -normal;
+    const float sign = copysignf(1.0f, refIsA ? bcd : -bcd);
+    V refStart, refEnd, refCenter, side;
+    {
+        const bool useX = best == 0 || best == 2;
+        if (useX) { refCenter = pRef + rRef * (sign * hwRef); side = uRef; }
+        else { refCenter = pRef + uRef * (sign * hhRef); side = rRef; }
+        const V off = side * (useX ? hhRef : hwRef);
+        refStart = refCenter + off;
+        refEnd = refCenter - off;
+    }
+    V incStart, incEnd;
+    {
+        const bool axisIsUp = (best & 1u) != 0;
+        const float sideTerm = (best == 0 || best == 3) ? -relSin : relSin;
+        const float dotX = sign * (axisIsUp ? sideTerm : relCos);
+        const float dotY = sign * (axisIsUp ? relCos : sideTerm);
+        const bool useX = fabsf(dotX) > fabsf(dotY);
+        const float incSign = -copysignf(1.0f, useX ? dotX : dotY);
+        V fc, off;
+        if (useX) { fc = pInc + rInc * (incSign * hwInc); off = uInc * hhInc; }
+        else { fc = pInc + uInc * (incSign * hhInc); off = rInc * hwInc; }
+        incStart = fc + off;
+        incEnd = fc - off;
+    }
+    V c0 = incStart, c1 = incEnd;
+    if (clipSegment(c0, c1, refStart, -side)) return false;
+    if (clipSegment(c0, c1, refEnd, side)) return false;
+    const float refPlane = dot(refCenter, refNormal);
+    const bool k0 = dot(c0, refNormal) <= refPlane + SAT_EPS;
+    const bool k1 = dot(c1, refNormal) <= refPlane + SAT_EPS;
+    const uint32_t idBase = best << 1;
+    if (!k0 && !k1) return false;
+    if (k0 && k1) emit(out, a, b, normal, depth, 2, c0, c1, idBase, idBase | 1u);
+    else if (k0) emit(out, a, b, normal, depth, 1, c0, { 0, 0 }, idBase, 0);
+    else emit(out, a, b, normal, depth, 1, c1, { 0, 0 }, idBase | 1u, 0);
+    return true;
+}
+
+// PolygonPolygon.cpp:18-235. B's vertices and normals in A's frame are recomputed on use
+// (the same expression each time, so values are identical to a stored copy).
+__device__ bool polygonPolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V pA{ pa.x, pa.y }, pB{ pb.x, pb.y };
+    const float cosA = pa.z, sinA = pa.w, cosB = pb.z, sinB = pb.w;
+    const uint4 sa = s.shape[a], sb = s.shape[b];
+    const uint32_t countA = sa.z, countB = sb.z;
+    const float2* vA = s.verts + sa.y;
+    const float2* nA = s.normals + sa.y;
+    const float2* vB = s.verts + sb.y;
+    const float2* nB = s.normals + sb.y;
+    const V rightA{ cosA, sinA }, upA{ -sinA, cosA };
+    const V cd = pB - pA;
+    const V dl{ dot(cd, rightA), dot(cd, upA) };
+    const float relCos = cosA * cosB + sinA * sinB;
+    const float relSin = cosA * sinB - sinA * cosB;
+    auto vBA = [&](uint32_t i) -> V {
+        const V v = ld(vB, i);
+        return { dl.x + relCos * v.x - relSin * v.y, dl.y + relSin * v.x + relCos * v.y };
+    };
+    auto nBA = [&](uint32_t i) -> V {
+        const V nb = ld(nB, i);
+        return { relCos * nb.x - relSin * nb.y, relSin * nb.x + relCos * nb.y };
+    };
+    V nl{ 0.0f, 0.0f };
+    float depth = FLT_MAX;
+    bool refIsA = true;
+    uint32_t bestIdx = 0;
+    float bestFacing = FLT_MAX;
+    for (uint32_t i = 0; i < countA; i++) {
+        const V axis = ld(nA, i);
+        float minB = FLT_MAX;
+        for (uint32_t j = 0; j < countB; j++) minB = fminf(minB, dot(vBA(j), axis));
+        const float overlap = dot(ld(vA, i), axis) - minB;
+        if (overlap < 0.0f) return false;
+        const float facing = -dot(axis, dl);
+        if (overlap > depth) continue;
+        if (overlap < depth || facing < bestFacing) { depth = overlap; nl = axis; refIsA = true; bestIdx = i; bestFacing = facing; }
+    }
+    for (uint32_t i = 0; i < countB; i++) {
+        const V axis = nBA(i);
+        float minA = FLT_MAX;
+        for (uint32_t j = 0; j < countA; j++) minA = fminf(minA, dot(ld(vA, j), axis));
+        const float overlap = dot(vBA(i), axis) - minA;
+        if (overlap < 0.0f) return false;
+        const float facing = dot(axis, dl);
+        if (overlap > depth) continue;
+        if (overlap < depth || facing < bestFacing) { depth = overlap; nl = -axis; refIsA = false; bestIdx = i; bestFacing = facing; }
+    }
+    const uint32_t refCount = refIsA ? countA : countB;
+    const uint32_t incCount = refIsA ? countB : countA;
+    const V refN = refIsA ? nl : -nl;
+    const uint32_t i0 = bestIdx, i1 = nextIdx(i0, refCount);
+    const V refStart = refIsA ? ld(vA, i0) : vBA(i0);
+    const V refEnd = refIsA ? ld(vA, i1) : vBA(i1);
+    const V refCenter = (refStart + refEnd) * 0.5f;
+    const V edge = refEnd - refStart;
+    const float elen2 = dot(edge, edge);
+    const V side = edge * (1.0f / sqrtf(elen2));
+    uint32_t inc = 0;
+    float minDot = FLT_MAX;
+    for (uint32_t i = 0; i < incCount; i++) {
+        const V nLocal = refIsA ? nBA(i) : ld(nA, i);
+        const float d = dot(refN, nLocal);
+        if (d < minDot) { minDot = d; inc = i; }
+    }
+    const uint32_t inc2 = nextIdx(inc, incCount);
+    V c0 = refIsA ? vBA(inc) : ld(vA, inc);
+    V c1 = refIsA ? vBA(inc2) : ld(vA, inc2);
+    if (clipSegment(c0, c1, refStart, side)) return false;
+    if (clipSegment(c0, c1, refEnd, -side)) return false;
+    const float refPlane = dot(refCenter, refN);
+    const uint32_t tag = refIsA ? REF_TAG : 0u;
+    const bool k0 = dot(c0, refN) <= refPlane + SAT_EPS;
+    const bool k1 = dot(c1, refN) <= refPlane + SAT_EPS;
+    if (!k0 && !k1) return false;
+    const V nWorld = rightA * nl.x + upA * nl.y;
+    const V w0 = pA + rightA * c0.x + upA * c0.y;
+    const V w1 = pA + rightA * c1.x + upA * c1.y;
+    if (k0 && k1) emit(out, a, b, nWorld, depth, 2, w0, w1, tag | inc, tag | inc2);
+    else if (k0) emit(out, a, b, nWorld, depth, 1, w0, { 0, 0 }, tag | inc, 0);
+    else emit(out, a, b, nWorld, depth, 1, w1, { 0, 0 }, tag | inc2, 0);
+    return true;
+}
+
+// BoxPolygon.cpp:14-319 (polygon B's frame).
+__device__ bool boxPolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V pB{ pb.x, pb.y };
+    const float cosA = pa.z, sinA = pa.w, cosB = pb.z, sinB = pb.w;
+    const float hwA = s.dims[a].x, hhA = s.dims[a].y;
+    const uint4 sb = s.shape[b];
+    const float2* vB = s.verts + sb.y;
+    const float2* nB = s.normals + sb.y;
+    const uint32_t count = sb.z;
+    const V rightB{ cosB, sinB }, upB{ -sinB, cosB };
+    const float relCos = cosA * cosB + sinA * sinB;
+    const float relSin = cosA * sinB - sinA * cosB;
+    const V rA{ relCos, -relSin }, uA{ relSin, relCos };
+    const V dBA = V{ pa.x, pa.y } - pB;
+    const V bc{ dot(dBA, rightB), dot(dBA, upB) };
+    V nl;
+    float depth;
+    uint32_t bestType;
+    uint32_t bestIdx = 0;
+    float bestFacing;
+    float pcx2, pcy2;
+    {
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        for (uint32_t i = 0; i < count; i++) { const float p = dot(ld(vB, i), rA); mn = fminf(mn, p); mx = fmaxf(mx, p); }
+        const float bp = dot(bc, rA);
+        mn -= bp; mx -= bp;
+        const float overlap = hwA + fminf(-mn, mx);
+        if (overlap < 0.0f) return false;
+        pcx2 = mn + mx;
+        depth = overlap;
+        nl = flipSign(rA, pcx2);
+        bestType = 0;
+        bestFacing = -fabsf(pcx2);
+    }
+    {
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        for (uint32_t i = 0; i < count; i++) { const float p = dot(ld(vB, i), uA); mn = fminf(mn, p); mx = fmaxf(mx, p); }
+        const float bp = dot(bc, uA);
+        mn -= bp; mx -= bp;
+        const float overlap = hhA + fminf(-mn, mx);
+        if (overlap < 0.0f) return false;
+        pcy2 = mn + mx;
+        const float facing = -fabsf(pcy2);
+        if (overlap < depth || (overlap == depth && facing < bestFacing)) {
+            depth = overlap; nl = flipSign(uA, pcy2); bestType = 1; bestFacing = facing;
+        }
+    }
+    const V pcRel = pcx2 * rA + pcy2 * uA;
+    for (uint32_t i = 0; i < count; i++) {
+        const V axis = ld(nB, i);
+        const float boxRadius = hwA * fabsf(dot(axis, rA)) + hhA * fabsf(dot(axis, uA));
+        const float bcp = dot(bc, axis);
+        const float overlap = dot(axis, ld(vB, i)) - (bcp - boxRadius);
+        if (overlap < 0.0f) return false;
+        const float facing = dot(axis, pcRel);
+        if (overlap < depth || (overlap == depth && facing < bestFacing)) {
+            depth = overlap; nl = -axis; bestType = 2; bestIdx = i; bestFacing = facing;
+        }
+    }
+    const bool refIsBox = bestType != 2;
+    V refN, refCenter, side, refStart, refEnd, incStart, incEnd;
+    uint32_t inc1, inc2;
+    if (refIsBox) {
+        refN = nl;
+        const bool useX = bestType == 0;
+        const float sign = useX ? dot(refN, rA) : dot(refN, uA);
+        if (useX) { refCenter = bc + rA * (sign * hwA); side = uA; }
+        else { refCenter = bc + uA * (sign * hhA); side = rA; }
+        const V off = side * (useX ? hhA : hwA);
+        refStart = refCenter + off;
+        refEnd = refCenter - off;
+        inc1 = 0;
+        float minDot = FLT_MAX;
+        for (uint32_t i = 0; i < count; i++) {
+            const float d = dot(refN, ld(nB, i));
+            if (d < minDot) { minDot = d; inc1 = i; }
+        }
+        inc2 = nextIdx(inc1, count);
+        incStart = ld(vB, inc1);
+        incEnd = ld(vB, inc2);
+    } else {
+        refN = -nl;
+        const uint32_t i0 = bestIdx, i1 = nextIdx(i0, count);
+        refStart = ld(vB, i0);
+        refEnd = ld(vB, i1);
+        refCenter = (refStart + refEnd) * 0.5f;
+        const V edge = refEnd - refStart;
+        const float elen2 = dot(edge, edge);
+        side = edge * (-1.0f / sqrtf(elen2));
+        const float dotX = dot(refN, rA);
+        const float dotY = dot(refN, uA);
+        const bool useX = fabsf(dotX) > fabsf(dotY);
+        const float sign = -copysignf(1.0f, useX ? dotX : dotY);
+        // Narrow/Constants.h:34-37 BOX_INCIDENT_EDGE_IDS[useX][sign > 0].
+        if (useX) { inc1 = sign > 0 ? 0u : 3u; inc2 = sign > 0 ? 1u : 2u; }
+        else { inc1 = sign > 0 ? 0u : 1u; inc2 = sign > 0 ? 3u : 2u; }
+        V fc, off;
+        if (useX) { fc = bc + rA * (sign * hwA); off = uA * hhA; }
+        else { fc = bc + uA * (sign * hhA); off = rA * hwA; }
+        incStart = fc + off;
+        incEnd = fc - off;
+    }
+    V c0 = incStart, c1 = incEnd;
+    if (clipSegment(c0, c1, refStart, -side)) return false;
+    if (clipSegment(c0, c1, refEnd, side)) return false;
+    const uint32_t tag = refIsBox ? REF_TAG : 0u;
+    const float refPlane = dot(refCenter, refN);
+    const bool k0 = dot(c0, refN) <= refPlane + SAT_EPS;
+    const bool k1 = dot(c1, refN) <= refPlane + SAT_EPS;
+    if (!k0 && !k1) return false;
+    const V nWorld = rightB * nl.x + upB * nl.y;
+    const V w0 = pB + rightB * c0.x + upB * c0.y;
+    const V w1 = pB + rightB * c1.x + upB * c1.y;
+    if (k0 && k1) emit(out, a, b, nWorld, depth, 2, w0, w1, tag | inc1, tag | inc2);
+    else if (k0) emit(out, a, b, nWorld, depth, 1, w0, { 0, 0 }, tag | inc1, 0);
+    else emit(out, a, b, nWorld, depth, 1, w1, { 0, 0 }, tag | inc2, 0);
+    return true;
+}
+
+// NarrowPhaseCollisionDetector.cpp:653-667 canonical order, then SymmetricMatrix dispatch.
+__global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, Counters* __restrict__ ctr, uint64_t cap,
+                                               Shapes s, agx_manifold* __restrict__ slots, uint32_t* __restrict__ flags) {
+    const uint32_t n = (uint32_t)min((uint64_t)ctr->pairs, cap);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t a = pairs[p].x, b = pairs[p].y;
+        uint32_t ta = s.shape[a].x, tb = s.shape[b].x;
+        if (ta > tb) { const uint32_t t = a; a = b; b = t; const uint32_t u = ta; ta = tb; tb = u; }
+        else if (ta == tb && b > a) { const uint32_t t = a; a = b; b = t; }
+        agx_manifold m;
+        bool hit;
+        const uint32_t slot = tb * (tb + 1) / 2 + ta;
+        switch (slot) {
+        case 0: hit = circleCircle(s, a, b, m); break;
+        case 1: hit = circleBox(s, a, b, m); break;
+        case 2: hit = boxBox(s, a, b, m); break;
+        case 3: hit = circlePolygon(s, a, b, m); break;
+        case 4: hit = boxPolygon(s, a, b, m); break;
+        default: hit = polygonPolygon(s, a, b, m); break;
+        }
+        if (hit) slots[p] = m;
+        flags[p] = hit ? 1u : 0u;
+    }
+}
+
+__global__ void kClampPairs(Counters* ctr, uint64_t cap) {
+    ctr->scratch[1] = (uint32_t)min((uint64_t)ctr->pairs, cap);
+}
+
+__device__ inline unsigned long long pairKey(uint32_t sa, uint32_t sb) {   // NarrowPhaseCollisionDetector.h:215-218
+    const unsigned long long lo = min(sa, sb), hi = max(sa, sb);
+    return (lo << 32) | hi;
+}
+__device__ inline uint32_t hashSlot(unsigned long long key, uint32_t capLog2) {   // ContactCache.h:27-30
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - capLog2));
+}
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+
+// Compaction in pair order and warm start (finalizeResults, NarrowPhaseCollisionDetector.cpp:692-767):
+// impulses are copied from the previous manifold of the same collider pair when A/B order and
+// the contact id match.
+__global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__ slots, const uint32_t* __restrict__ flags,
+                                                const uint32_t* __restrict__ scan, Counters* __restrict__ ctr,
+                                                agx_manifold* __restrict__ cur, const agx_manifold* __restrict__ prev,
+                                                const unsigned long long* __restrict__ hashKeys,
+                                                const uint32_t* __restrict__ hashVals, const uint32_t* __restrict__ colSlot) {
+    const uint32_t n = ctr->scratch[1];
+    const uint32_t prevCount = ctr->prevManifolds;
+    const uint32_t cap = ctr->hashCapacity;
+    const uint32_t capLog2 = cap ? (uint32_t)__builtin_ctz(cap) : 0u;
+    uint32_t contacts = 0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        if (!flags[p]) continue;
+        agx_manifold m = slots[p];
+        contacts += m.count;
+        if (prevCount > 0 && cap > 0) {
+            const unsigned long long key = pairKey(colSlot[m.a], colSlot[m.b]);
+            uint32_t h = hashSlot(key, capLog2);
+            uint32_t found = 0xFFFFFFFFu;
+            for (uint32_t probe = 0; probe < cap; probe++) {
+                const unsigned long long k = hashKeys[h];
+                if (k == key) { found = hashVals[h]; break; }
+                if (k == EMPTY_KEY) break;
+                h = (h + 1) & (cap - 1);
+            }
+            if (found != 0xFFFFFFFFu) {
+                const agx_manifold& q = prev[found];
+                if (q.a == m.a && q.b == m.b) {
+                    if (q.count > 0 && m.id0 == q.id0) { m.jn0 = q.jn0; m.jt0 = q.jt0; }
+                    else if (q.count > 1 && m.id0 == q.id1) { m.jn0 = q.jn1; m.jt0 = q.jt1; }
+                    if (m.count > 1) {
+                        if (q.count > 0 && m.id1 == q.id0) { m.jn1 = q.jn0; m.jt1 = q.jt0; }
+                        else if (q.count > 1 && m.id1 == q.id1) { m.jn1 = q.jn1; m.jt1 = q.jt1; }
+                    }
+                }
+            }
+        }
+        cur[scan[p]] = m;
+    }
+    for (uint32_t off = 32; off > 0; off >>= 1) contacts += __shfl_down(contacts, off, 64);
+    if ((threadIdx.x & 63u) == 0 && contacts) atomicAdd(&ctr->contacts, contacts);
+}
+
+__global__ void kSetManifolds(Counters* ctr) { ctr->manifolds = ctr->scratch[2]; }
+
+// rebuildContactCache (NarrowPhaseCollisionDetector.cpp:272-297): table sized to the manifold
+// count, only manifolds with a nonzero accumulator are keyed.
+__global__ void kHashSize(Counters* ctr, uint32_t capMax) {
+    const uint32_t m = ctr->manifolds;
+    uint32_t cap = 16;
+    while (cap < 2u * m && cap < capMax) cap <<= 1;
+    ctr->hashCapacity = cap;
+    ctr->prevManifolds = m;
+}
+
+__global__ __launch_bounds__(256) void kHashClear(unsigned long long* keys, const Counters* ctr) {
+    const uint32_t cap = ctr->hashCapacity;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) keys[i] = EMPTY_KEY;
+}
+
+__global__ __launch_bounds__(256) void kHashInsert(const agx_manifold* __restrict__ cur, Counters* ctr,
+                                                   unsigned long long* keys, uint32_t* vals, const uint32_t* __restrict__ colSlot) {
+    const uint32_t n = ctr->prevManifolds;
+    const uint32_t cap = ctr->hashCapacity;
+    const uint32_t capLog2 = (uint32_t)__builtin_ctz(cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const agx_manifold m = cur[i];
+        bool any = m.jn0 != 0.0f || m.jt0 != 0.0f;
+        if (m.count > 1) any = any || m.jn1 != 0.0f || m.jt1 != 0.0f;
+        if (!any) continue;
+        const unsigned long long key = pairKey(colSlot[m.a], colSlot[m.b]);
+        uint32_t h = hashSlot(key, capLog2);
+        for (uint32_t probe = 0; probe < cap; probe++) {
+            const unsigned long long old = atomicCAS(keys + h, EMPTY_KEY, key);
+            if (old == EMPTY_KEY || old == key) { vals[h] = i; break; }
+            h = (h + 1) & (cap - 1);
+        }
+    }
+}
+
+inline uint32_t streamBlocks(const DeviceWorld& w, uint64_t cap) {
+    const uint64_t want = (cap + 255) / 256;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)w.numCU * 16));
+}
+
+}  // namespace
+
+hipError_t launchNarrow(hipStream_t st, DeviceWorld& w) {
+    const Shapes s{ w.pose, w.shape, w.dims, w.verts, w.normals };
+    const uint32_t blocks = streamBlocks(w, w.pairCap);
+    hipLaunchKernelGGL(kNarrow, dim3(blocks), dim3(256), 0, st, w.pairs, w.ctr, w.pairCap, s, w.slots, w.flags);
+    hipLaunchKernelGGL(kClampPairs, dim3(1), dim3(1), 0, st, w.ctr, w.pairCap);
+    hipError_t e = launchScan(st, w.flags, w.flagScan, &w.ctr->scratch[1], &w.ctr->scratch[2], w.pairCap, w.scanBlock,
+                              w.scanBlockCap);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kCompact, dim3(blocks), dim3(256), 0, st, w.slots, w.flags, w.flagScan, w.ctr, w.man[w.cur],
+                       w.man[1 - w.cur], w.hashKeys, w.hashVals, w.colSlot);
+    hipLaunchKernelGGL(kSetManifolds, dim3(1), dim3(1), 0, st, w.ctr);
+    return hipGetLastError();
+}
+
+hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w) {
+    hipLaunchKernelGGL(kHashSize, dim3(1), dim3(1), 0, st, w.ctr, (uint32_t)w.hashCapMax);
+    const uint32_t blocks = streamBlocks(w, w.hashCapMax);
+    hipLaunchKernelGGL(kHashClear, dim3(blocks), dim3(256), 0, st, w.hashKeys, w.ctr);
+    hipLaunchKernelGGL(kHashInsert, dim3(streamBlocks(w, w.manCap)), dim3(256), 0, st, w.man[w.cur], w.ctr, w.hashKeys,
+                       w.hashVals, w.colSlot);
+    return hipGetLastError();
+}
+
+}  // namespace agx

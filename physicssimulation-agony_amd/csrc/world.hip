@@ -1,0 +1,785 @@
+// C ABI (include/agx.h): scene building, device world lifetime, step orchestration, read-outs.
+#include "agx_internal.h"
+#include "builder.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+struct agx_builder {
+    agx::Builder b;
+};
+
+struct agx_world {
+    agx::DeviceWorld d;
+    agx_settings settings{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<void*> allocations;
+    std::vector<float> mass, inertia, localComX, localComY;   // per body, constant
+    std::vector<uint32_t> colType, colMaterial, colSlot, colScene, polyStart, polyCount;
+    std::vector<float> colP0, colP1, colMinExt, colMaxExt;
+    uint32_t staticBodies = 0, staticColliders = 0;
+    float accumulator = 0.0f;
+    uint32_t lastStepCount = 1;
+    bool postUpdatePending = true;   // next step's CCD sweep starts from the post-update pose
+    bool inStep = false;             // phases of a step have run but FINISH has not
+    int nextPhase = 0;
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(agx_world* w, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (w) w->err = buf;
+    g_lastError = buf;
+    return code;
+}
+
+#define HIP_CHECK(w, call)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) return fail((w), AGX_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_));  \
+    } while (0)
+
+template <class T>
+hipError_t devAlloc(agx_world* w, T** p, size_t count) {
+    *p = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+    if (e == hipSuccess) w->allocations.push_back(*p);
+    return e;
+}
+
+template <class T>
+hipError_t upload(agx_world* w, T** p, const std::vector<T>& host) {
+    hipError_t e = devAlloc(w, p, host.size());
+    if (e != hipSuccess || host.empty()) return e;
+    return hipMemcpy(*p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+uint32_t ceilLog2(uint32_t v) {
+    uint32_t b = 0;
+    while ((1u << b) < v) b++;
+    return b;
+}
+
+// Allocates a tree over 'list' (ascending collider indices) grouped by scene.
+int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, const std::vector<uint32_t>& colScene,
+              uint32_t S) {
+    t.n = (uint32_t)list.size();
+    t.sortBlocks = std::max(1u, (t.n + agx::SORT_TILE - 1) / agx::SORT_TILE);
+    std::vector<uint32_t> sceneStart(S + 1, 0);
+    for (uint32_t c : list) sceneStart[colScene[c] + 1]++;
+    for (uint32_t s = 0; s < S; s++) sceneStart[s + 1] += sceneStart[s];
+    std::vector<uint32_t> chunkScene;
+    std::vector<uint2> chunkRange;
+    for (uint32_t s = 0; s < S; s++)
+        for (uint32_t lo = sceneStart[s]; lo < sceneStart[s + 1]; lo += agx::BOUNDS_CHUNK) {
+            chunkScene.push_back(s);
+            chunkRange.push_back(make_uint2(lo, std::min(sceneStart[s + 1], lo + agx::BOUNDS_CHUNK)));
+        }
+    t.chunks = (uint32_t)chunkScene.size();
+    const size_t n = t.n, nodes = n ? 2 * n - 1 : 1, internal = n > 1 ? n - 1 : 1;
+    HIP_CHECK(w, upload(w, &t.list, list));
+    HIP_CHECK(w, upload(w, &t.sceneStart, sceneStart));
+    HIP_CHECK(w, upload(w, &t.chunkScene, chunkScene));
+    HIP_CHECK(w, upload(w, &t.chunkRange, chunkRange));
+    HIP_CHECK(w, devAlloc(w, &t.keys, n));
+    HIP_CHECK(w, devAlloc(w, &t.vals, n));
+    HIP_CHECK(w, devAlloc(w, &t.keysAlt, n));
+    HIP_CHECK(w, devAlloc(w, &t.valsAlt, n));
+    HIP_CHECK(w, devAlloc(w, &t.hist, 256ull * t.sortBlocks));
+    HIP_CHECK(w, devAlloc(w, &t.nodeAabb, nodes));
+    HIP_CHECK(w, devAlloc(w, &t.child, internal));
+    HIP_CHECK(w, devAlloc(w, &t.range, internal));
+    HIP_CHECK(w, devAlloc(w, &t.parent, nodes));
+    HIP_CHECK(w, devAlloc(w, &t.flags, internal));
+    HIP_CHECK(w, devAlloc(w, &t.leafBody, n));
+    HIP_CHECK(w, devAlloc(w, &t.sceneBounds, S));
+    if (n) HIP_CHECK(w, hipMemcpy(t.vals, list.data(), n * 4, hipMemcpyHostToDevice));   // initial CCD order
+    return AGX_OK;
+}
+
+int allocPairBuffers(agx_world* w, uint64_t pairCap) {
+    agx::DeviceWorld& d = w->d;
+    d.pairCap = pairCap;
+    d.queryWaves = (d.nDyn + 63) / 64;
+    HIP_CHECK(w, devAlloc(w, &d.pairs, pairCap));
+    HIP_CHECK(w, devAlloc(w, &d.waveCount, 2ull * std::max(1u, d.queryWaves)));
+    HIP_CHECK(w, devAlloc(w, &d.waveOffset, 2ull * std::max(1u, d.queryWaves)));
+    d.scanBlockCap = 4096;
+    HIP_CHECK(w, devAlloc(w, &d.scanBlock, d.scanBlockCap));
+    HIP_CHECK(w, devAlloc(w, &d.ctr, 1));
+    HIP_CHECK(w, hipMemset(d.ctr, 0, sizeof(agx::Counters)));
+    return AGX_OK;
+}
+
+int beginStep(agx_world* w) {
+    agx::DeviceWorld& d = w->d;
+    const size_t off = offsetof(agx::Counters, pairs);
+    HIP_CHECK(w, agx::launchFillBytes(w->stream, reinterpret_cast<char*>(d.ctr) + off, sizeof(agx::Counters) - off));
+    return AGX_OK;
+}
+
+int runPhase(agx_world* w, uint32_t phase, float dt) {
+    agx::DeviceWorld& d = w->d;
+    const agx_settings& s = w->settings;
+    hipStream_t st = w->stream;
+    const bool ccd = s.ccd_enabled != 0;
+    switch (phase) {
+    case AGX_PHASE_INTEGRATE:
+        if (int r = beginStep(w)) return r;
+        w->inStep = true;
+        HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0));
+        break;
+    case AGX_PHASE_POSE:
+        HIP_CHECK(w, agx::launchPose(st, d, 0, w->postUpdatePending ? 1 : 0, ccd ? 1 : 0));
+        break;
+    case AGX_PHASE_CCD:
+        if (ccd) HIP_CHECK(w, agx::launchCcd(st, d, s.ccd_fast_factor, s.ccd_core_fraction));
+        break;
+    case AGX_PHASE_BROAD:
+        HIP_CHECK(w, agx::launchBuildTree(st, d, d.dyn));
+        HIP_CHECK(w, agx::launchPairs(st, d));
+        break;
+    case AGX_PHASE_NARROW:
+        HIP_CHECK(w, agx::launchNarrow(st, d));
+        break;
+    case AGX_PHASE_SOLVE:
+        if (!s.collide_only) {
+            HIP_CHECK(w, agx::launchColourPrep(st, d));
+            HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations, s.track_residual ? 1 : 0,
+                                          s.residual_target, s.track_errors ? 1 : 0));
+        }
+        break;
+    case AGX_PHASE_ISLANDS:
+        HIP_CHECK(w, agx::launchIslands(st, d));
+        break;
+    case AGX_PHASE_FINISH:
+        HIP_CHECK(w, agx::launchHashBuild(st, d));
+        d.cur ^= 1;
+        w->postUpdatePending = false;
+        w->inStep = false;
+        break;
+    default:
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown phase %u", phase);
+    }
+    w->nextPhase = (int)((phase + 1) % AGX_PHASE_COUNT);
+    return AGX_OK;
+}
+
+int postUpdate(agx_world* w) {
+    HIP_CHECK(w, agx::launchPose(w->stream, w->d, 0, 0, 0));
+    w->postUpdatePending = true;
+    return AGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void agx_settings_default(agx_settings* s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof *s);
+    s->gravity_x = 0.0f;
+    s->gravity_y = -9.81f;
+    s->velocity_iterations = 6;
+    s->position_iterations = 3;
+    s->ccd_enabled = 1;
+    s->ccd_fast_factor = 1.0f;
+    s->ccd_core_fraction = 0.5f;
+    s->update_interval = 1.0f / 300.0f;
+    s->max_delta_per_update = 1.0f / 20.0f;
+    s->time_scale = 1.0f;
+}
+
+const char* agx_last_error(void) { return g_lastError.c_str(); }
+
+int agx_device_count(int* out) {
+    if (!out) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "out is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *out = e == hipSuccess ? n : 0;
+    return AGX_OK;
+}
+
+int agx_builder_create(uint32_t num_scenes, agx_builder** out) {
+    if (!out || num_scenes == 0) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "need out pointer and >= 1 scene");
+    agx_builder* b = new (std::nothrow) agx_builder();
+    if (!b) return fail(nullptr, AGX_ERR_CAPACITY, "out of host memory");
+    b->b.scenes.resize(num_scenes);
+    *out = b;
+    return AGX_OK;
+}
+
+int agx_builder_destroy(agx_builder* b) {
+    delete b;
+    return AGX_OK;
+}
+
+int agx_builder_add_material(agx_builder* b, float e, float sf, float df, uint32_t* out_index) {
+    if (!b) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "builder is null");
+    // SimulationImpl/MaterialManager.h:31: elasticity clamped to [0, 1], frictions to >= 0.
+    b->b.materials.push_back({ std::fmin(std::fmax(e, 0.0f), 1.0f), std::fmax(sf, 0.0f), std::fmax(df, 0.0f) });
+    if (out_index) *out_index = (uint32_t)b->b.materials.size() - 1;
+    return AGX_OK;
+}
+
+int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uint8_t* type, const float* px,
+                           const float* py, const float* rot, const float* vx, const float* vy, const float* av,
+                           const float* mass, const uint32_t* material, const float* p0, const float* p1,
+                           const uint32_t* vstart, const uint32_t* vcount, const float* verts, uint32_t* out_index) {
+    if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    if (n && (!type || !px || !py || !rot || !vx || !vy || !av || !mass || !material || !p0 || !p1))
+        return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "null body array");
+    agx::SceneData& sd = b->b.scenes[scene];
+    sd.materialCount = (uint32_t)b->b.materials.size();
+    int rejected = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t r = UINT32_MAX;
+        if (type[i] == AGX_SHAPE_CIRCLE) r = sd.createCircle(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i]);
+        else if (type[i] == AGX_SHAPE_BOX) r = sd.createBox(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i], p1[i]);
+        else if (type[i] == AGX_SHAPE_POLYGON && vstart && vcount && verts)
+            r = sd.createPolygon(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i],
+                                 reinterpret_cast<const agx::Vec*>(verts) + vstart[i], vcount[i]);
+        if (r == UINT32_MAX) rejected++;
+        if (out_index) out_index[i] = r;
+    }
+    if (rejected) fail(nullptr, 0, "%d bodies rejected (invalid or non-convex polygon outline)", rejected);
+    return rejected;
+}
+
+int agx_builder_scene_counts(agx_builder* b, uint32_t scene, agx_counts* out) {
+    if (!b || !out || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    const agx::SceneData& sd = b->b.scenes[scene];
+    out->scenes = (uint32_t)b->b.scenes.size(); out->bodies = sd.bodyCount(); out->static_bodies = sd.staticBodies;
+    out->colliders = sd.colliderCount(); out->static_colliders = sd.staticColliders;
+    out->vertices = (uint32_t)sd.verts.size(); out->materials = (uint32_t)b->b.materials.size(); out->reserved = 0;
+    return AGX_OK;
+}
+
+int agx_builder_read_body_f32(agx_builder* b, uint32_t scene, uint32_t field, float* out, uint32_t n) {
+    if (!b || !out || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    const agx::SceneData& sd = b->b.scenes[scene];
+    if (n != sd.bodyCount()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "expected %u bodies", sd.bodyCount());
+    const std::vector<float>* v = nullptr;
+    switch (field) {
+    case AGX_BODY_COM_X: v = &sd.comX; break;       case AGX_BODY_COM_Y: v = &sd.comY; break;
+    case AGX_BODY_ROTATION: v = &sd.rot; break;     case AGX_BODY_VEL_X: v = &sd.velX; break;
+    case AGX_BODY_VEL_Y: v = &sd.velY; break;       case AGX_BODY_ANG_VEL: v = &sd.angVel; break;
+    case AGX_BODY_COS: v = &sd.cosv; break;         case AGX_BODY_SIN: v = &sd.sinv; break;
+    case AGX_BODY_INV_MASS: v = &sd.invMass; break; case AGX_BODY_INV_INERTIA: v = &sd.invInertia; break;
+    case AGX_BODY_MASS: v = &sd.mass; break;        case AGX_BODY_INERTIA: v = &sd.inertia; break;
+    case AGX_BODY_LOCAL_COM_X: v = &sd.localComX; break; case AGX_BODY_LOCAL_COM_Y: v = &sd.localComY; break;
+    default: return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "unknown body field %u", field);
+    }
+    std::memcpy(out, v->data(), n * 4u);
+    return AGX_OK;
+}
+
+int agx_builder_read_collider(agx_builder* b, uint32_t scene, uint32_t field, uint32_t as_u32, void* out, uint32_t n) {
+    if (!b || !out || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    const agx::SceneData& sd = b->b.scenes[scene];
+    if (n != sd.colliderCount()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "expected %u colliders", sd.colliderCount());
+    if (as_u32) {
+        uint32_t* o = static_cast<uint32_t*>(out);
+        switch (field) {
+        case AGX_COL_BODY: std::memcpy(o, sd.colBody.data(), n * 4u); return AGX_OK;
+        case AGX_COL_TYPE: for (uint32_t i = 0; i < n; i++) o[i] = sd.colType[i]; return AGX_OK;
+        case AGX_COL_MATERIAL: std::memcpy(o, sd.colMaterial.data(), n * 4u); return AGX_OK;
+        case AGX_COL_SLOT: std::memcpy(o, sd.colSlot.data(), n * 4u); return AGX_OK;
+        case AGX_COL_POLY_START: std::memcpy(o, sd.polyStart.data(), n * 4u); return AGX_OK;
+        case AGX_COL_POLY_COUNT: std::memcpy(o, sd.polyCount.data(), n * 4u); return AGX_OK;
+        default: return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "unknown collider field %u", field);
+        }
+    }
+    const std::vector<float>* v = nullptr;
+    switch (field) {
+    case AGX_COL_COM_OFF_X: v = &sd.comOffX; break;  case AGX_COL_COM_OFF_Y: v = &sd.comOffY; break;
+    case AGX_COL_LOCAL_ROT: v = &sd.localRot; break; case AGX_COL_P0: v = &sd.p0; break;
+    case AGX_COL_P1: v = &sd.p1; break;              case AGX_COL_MIN_EXTENT: v = &sd.minExtent; break;
+    case AGX_COL_MAX_EXTENT: v = &sd.maxExtent; break;
+    default: return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "unknown collider field %u", field);
+    }
+    std::memcpy(out, v->data(), n * 4u);
+    return AGX_OK;
+}
+
+int agx_builder_read_vertices(agx_builder* b, uint32_t scene, float* verts, float* normals, uint32_t n) {
+    if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    const agx::SceneData& sd = b->b.scenes[scene];
+    if (n != sd.verts.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "expected %zu vertices", sd.verts.size());
+    if (verts) std::memcpy(verts, sd.verts.data(), n * 8u);
+    if (normals) std::memcpy(normals, sd.normals.data(), n * 8u);
+    return AGX_OK;
+}
+
+int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world** out) {
+    if (!bld || !out) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "builder and out are required");
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) return fail(nullptr, AGX_ERR_NO_DEVICE, "no HIP device");
+    agx_world* w = new (std::nothrow) agx_world();
+    if (!w) return fail(nullptr, AGX_ERR_CAPACITY, "out of host memory");
+    if (settings) w->settings = *settings; else agx_settings_default(&w->settings);
+    agx::DeviceWorld& d = w->d;
+    const auto& scenes = bld->b.scenes;
+    d.S = (uint32_t)scenes.size();
+    d.sceneBits = ceilLog2(d.S);
+    if (d.sceneBits > 16) { delete w; return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "too many scenes"); }
+    d.mortonBits = std::min(30u, (32u - d.sceneBits) & ~1u);
+
+    // Concatenate scenes.
+    std::vector<float4> pos, vel, prev;
+    std::vector<float2> rcs, inv, comOff, dims, extent, verts, normals;
+    std::vector<uint32_t> bodyScene, colBody, colScene, colSlot, dynBodies, bodyToDyn, dynList, statList;
+    std::vector<float> localRot;
+    std::vector<uint4> shape;
+    for (uint32_t s = 0; s < d.S; s++) {
+        const agx::SceneData& sd = scenes[s];
+        const uint32_t bOff = (uint32_t)pos.size(), cOff = (uint32_t)colBody.size(), vOff = (uint32_t)verts.size();
+        for (uint32_t i = 0; i < sd.bodyCount(); i++) {
+            pos.push_back(make_float4(sd.comX[i], sd.comY[i], sd.rot[i], 0.0f));
+            vel.push_back(make_float4(sd.velX[i], sd.velY[i], sd.angVel[i], 0.0f));
+            prev.push_back(make_float4(sd.comX[i], sd.comY[i], sd.cosv[i], sd.sinv[i]));
+            rcs.push_back(make_float2(sd.cosv[i], sd.sinv[i]));
+            inv.push_back(make_float2(sd.invMass[i], sd.invInertia[i]));
+            bodyScene.push_back(s);
+            w->mass.push_back(sd.mass[i]); w->inertia.push_back(sd.inertia[i]);
+            w->localComX.push_back(sd.localComX[i]); w->localComY.push_back(sd.localComY[i]);
+            if (sd.invMass[i] != 0.0f) { bodyToDyn.push_back((uint32_t)dynBodies.size()); dynBodies.push_back(bOff + i); }
+            else bodyToDyn.push_back(UINT32_MAX);
+        }
+        for (uint32_t c = 0; c < sd.colliderCount(); c++) {
+            const uint32_t gb = bOff + sd.colBody[c];
+            colBody.push_back(gb);
+            colScene.push_back(s);
+            colSlot.push_back(cOff + sd.colSlot[c]);
+            comOff.push_back(make_float2(sd.comOffX[c], sd.comOffY[c]));
+            localRot.push_back(sd.localRot[c]);
+            const uint32_t ps = sd.colType[c] == AGX_SHAPE_POLYGON ? vOff + sd.polyStart[c] : 0u;
+            shape.push_back(make_uint4(sd.colType[c], ps, sd.polyCount[c], sd.colMaterial[c]));
+            dims.push_back(make_float2(sd.p0[c], sd.p1[c]));
+            extent.push_back(make_float2(sd.minExtent[c], sd.maxExtent[c]));
+            (sd.invMass[sd.colBody[c]] != 0.0f ? dynList : statList).push_back(cOff + c);
+            w->colType.push_back(sd.colType[c]); w->colMaterial.push_back(sd.colMaterial[c]);
+            w->colSlot.push_back(cOff + sd.colSlot[c]); w->colScene.push_back(s);
+            w->polyStart.push_back(ps); w->polyCount.push_back(sd.polyCount[c]);
+            w->colP0.push_back(sd.p0[c]); w->colP1.push_back(sd.p1[c]);
+            w->colMinExt.push_back(sd.minExtent[c]); w->colMaxExt.push_back(sd.maxExtent[c]);
+        }
+        for (size_t v = 0; v < sd.verts.size(); v++) {
+            verts.push_back(make_float2(sd.verts[v].x, sd.verts[v].y));
+            normals.push_back(make_float2(sd.normals[v].x, sd.normals[v].y));
+        }
+        w->staticBodies += sd.staticBodies;
+        w->staticColliders += sd.staticColliders;
+    }
+    d.N = (uint32_t)pos.size();
+    d.C = (uint32_t)colBody.size();
+    d.nDyn = (uint32_t)dynList.size();
+    d.nStat = (uint32_t)statList.size();
+    d.nDynBodies = (uint32_t)dynBodies.size();
+    d.nVerts = (uint32_t)verts.size();
+    std::vector<float4> mats;
+    for (const agx::Material& m : bld->b.materials) mats.push_back(make_float4(m.elasticity, m.staticFriction, m.dynamicFriction, 0.0f));
+    d.nMat = (uint32_t)mats.size();
+
+    hipDeviceProp_t prop;
+    int dev = 0;
+    int rc = AGX_OK;
+    auto bail = [&](int code) { agx_world_destroy(w); return code; };
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return bail(fail(nullptr, AGX_ERR_HIP, "hipGetDeviceProperties failed"));
+    d.numCU = prop.multiProcessorCount;
+    d.coopBlocks = d.numCU * std::min(2, agx::cooperativeBlocksPerCU());
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, AGX_ERR_HIP, "hipStreamCreate failed"));
+
+#define UP(p, v) do { if (upload(w, &(p), (v)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "upload " #p)); } while (0)
+#define AL(p, n) do { if (devAlloc(w, &(p), (n)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "alloc " #p)); } while (0)
+    UP(d.pos, pos); UP(d.vel, vel); UP(d.prev, prev); UP(d.rcs, rcs); UP(d.inv, inv);
+    UP(d.bodyScene, bodyScene); UP(d.dynBodies, dynBodies); UP(d.bodyToDyn, bodyToDyn);
+    UP(d.colBody, colBody); UP(d.colScene, colScene); UP(d.colSlot, colSlot); UP(d.comOff, comOff);
+    UP(d.localRot, localRot); UP(d.shape, shape); UP(d.dims, dims); UP(d.extent, extent);
+    UP(d.verts, verts); UP(d.normals, normals); UP(d.materials, mats);
+    AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N);
+    AL(d.pose, d.C); AL(d.wRot, d.C); AL(d.prevW, d.C); AL(d.aabb, d.C);
+    if ((rc = allocTree(w, d.dyn, dynList, colScene, d.S))) return bail(rc);
+    if ((rc = allocTree(w, d.stat, statList, colScene, d.S))) return bail(rc);
+    const uint64_t pairCap = std::max<uint64_t>(1u << 16, 16ull * d.nDyn + 4ull * d.nStat);
+    if ((rc = allocPairBuffers(w, pairCap))) return bail(rc);
+    d.manCap = pairCap;
+    AL(d.slots, pairCap); AL(d.flags, pairCap); AL(d.flagScan, pairCap);
+    AL(d.man[0], pairCap); AL(d.man[1], pairCap);
+    d.hashCapMax = 16;
+    while (d.hashCapMax < 2 * d.manCap) d.hashCapMax <<= 1;
+    AL(d.hashKeys, d.hashCapMax); AL(d.hashVals, d.hashCapMax);
+    AL(d.colourId, pairCap); AL(d.claim, d.N); AL(d.used, 2ull * d.N); AL(d.order, pairCap);
+    AL(d.cs.idx, pairCap); AL(d.cs.nrm, pairCap); AL(d.cs.inv, pairCap); AL(d.cs.r0, pairCap);
+    AL(d.cs.r1, pairCap); AL(d.cs.k0, pairCap); AL(d.cs.k1, pairCap); AL(d.cs.imp, pairCap);
+#undef UP
+#undef AL
+    // Initial state: island ids, CCD hit times, empty contact cache, refreshSpatialData, trees.
+    hipStream_t st = w->stream;
+    bool ok = agx::launchFillU32(st, d.islandId, UINT32_MAX, d.N) == hipSuccess &&
+              agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.hitTime), 0x3F800000u, d.N) == hipSuccess &&
+              agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.hashKeys), 0xFFFFFFFFu, 2 * d.hashCapMax) == hipSuccess &&
+              agx::launchRefreshCosSin(st, d) == hipSuccess &&
+              agx::launchPose(st, d, 1, 0, 0) == hipSuccess &&
+              agx::launchBuildTree(st, d, d.stat) == hipSuccess &&
+              agx::launchBuildTree(st, d, d.dyn) == hipSuccess &&
+              hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) return bail(fail(nullptr, AGX_ERR_HIP, "world initialisation failed: %s", hipGetErrorString(hipGetLastError())));
+    *out = w;
+    return AGX_OK;
+}
+
+int agx_world_destroy(agx_world* w) {
+    if (!w) return AGX_OK;
+    if (w->stream) { (void)hipStreamSynchronize(w->stream); (void)hipStreamDestroy(w->stream); }
+    for (void* p : w->allocations) (void)hipFree(p);
+    delete w;
+    return AGX_OK;
+}
+
+const char* agx_world_last_error(agx_world* w) { return w ? w->err.c_str() : g_lastError.c_str(); }
+
+int agx_world_set_settings(agx_world* w, const agx_settings* s) {
+    if (!w || !s) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->velocity_iterations > agx::MAX_SWEEPS) return fail(w, AGX_ERR_INVALID_ARGUMENT, "at most %u velocity iterations", agx::MAX_SWEEPS);
+    w->settings = *s;
+    return AGX_OK;
+}
+
+int agx_world_get_settings(agx_world* w, agx_settings* s) {
+    if (!w || !s) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    *s = w->settings;
+    return AGX_OK;
+}
+
+int agx_world_counts(agx_world* w, agx_counts* out) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    out->scenes = w->d.S; out->bodies = w->d.N; out->static_bodies = w->staticBodies;
+    out->colliders = w->d.C; out->static_colliders = w->staticColliders;
+    out->vertices = w->d.nVerts; out->materials = w->d.nMat; out->reserved = 0;
+    return AGX_OK;
+}
+
+int agx_world_run_phase(agx_world* w, uint32_t phase, float dt) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (phase >= AGX_PHASE_COUNT) return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown phase %u", phase);
+    if ((int)phase != w->nextPhase) return fail(w, AGX_ERR_STATE, "phase %u out of order (expected %d)", phase, w->nextPhase);
+    return runPhase(w, phase, dt);
+}
+
+int agx_world_post_update(agx_world* w) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    return postUpdate(w);
+}
+
+int agx_world_step(agx_world* w, float dt, uint32_t n_steps) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (!(dt > 0.0f)) return AGX_OK;
+    for (uint32_t i = 0; i < n_steps; i++)
+        for (uint32_t p = 0; p < AGX_PHASE_COUNT; p++)
+            if (int r = runPhase(w, p, dt)) return r;
+    return n_steps ? postUpdate(w) : AGX_OK;
+}
+
+int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps) {
+    return agx_world_step(w, dt, n_steps);
+}
+
+// Simulation::update + FixedClock::advance (SimulationImpl/FixedClock.h:35-63).
+int agx_world_update(agx_world* w, float dt, uint32_t* out_steps) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (out_steps) *out_steps = 0;
+    if (dt <= 0.0f) return AGX_OK;
+    const agx_settings& s = w->settings;
+    w->accumulator += std::fmin(dt, s.max_delta_per_update);
+    const uint32_t steps = (uint32_t)std::floor(w->accumulator / s.update_interval);
+    w->accumulator -= (float)steps * s.update_interval;
+    if (steps == 0) return AGX_OK;
+    w->lastStepCount = steps;
+    if (out_steps) *out_steps = steps;
+    return agx_world_step(w, s.update_interval * s.time_scale, steps);
+}
+
+int agx_world_sync(agx_world* w) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    return AGX_OK;
+}
+
+int agx_world_read_body_f32(agx_world* w, uint32_t field, float* out, uint32_t n) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    agx::DeviceWorld& d = w->d;
+    if (n != d.N) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u bodies, got %u", d.N, n);
+    if (field >= AGX_BODY_FIELD_COUNT) return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown body field %u", field);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    if (field == AGX_BODY_MASS || field == AGX_BODY_INERTIA || field == AGX_BODY_LOCAL_COM_X || field == AGX_BODY_LOCAL_COM_Y) {
+        const std::vector<float>& v = field == AGX_BODY_MASS ? w->mass : field == AGX_BODY_INERTIA ? w->inertia
+                                    : field == AGX_BODY_LOCAL_COM_X ? w->localComX : w->localComY;
+        std::memcpy(out, v.data(), n * 4u);
+        return AGX_OK;
+    }
+    if (field <= AGX_BODY_ANG_VEL) {
+        std::vector<float4> h(n);
+        HIP_CHECK(w, hipMemcpy(h.data(), field <= AGX_BODY_ROTATION ? d.pos : d.vel, n * sizeof(float4), hipMemcpyDeviceToHost));
+        const uint32_t lane = field <= AGX_BODY_ROTATION ? field : field - AGX_BODY_VEL_X;
+        for (uint32_t i = 0; i < n; i++) out[i] = lane == 0 ? h[i].x : lane == 1 ? h[i].y : h[i].z;
+        return AGX_OK;
+    }
+    std::vector<float2> h(n);
+    HIP_CHECK(w, hipMemcpy(h.data(), field <= AGX_BODY_SIN ? d.rcs : d.inv, n * sizeof(float2), hipMemcpyDeviceToHost));
+    const bool first = field == AGX_BODY_COS || field == AGX_BODY_INV_MASS;
+    for (uint32_t i = 0; i < n; i++) out[i] = first ? h[i].x : h[i].y;
+    return AGX_OK;
+}
+
+int agx_world_write_body_f32(agx_world* w, uint32_t field, const float* in, uint32_t n) {
+    if (!w || !in) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    agx::DeviceWorld& d = w->d;
+    if (n != d.N) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u bodies, got %u", d.N, n);
+    if (field > AGX_BODY_SIN) return fail(w, AGX_ERR_INVALID_ARGUMENT, "body field %u is not writable", field);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    if (field <= AGX_BODY_ANG_VEL) {
+        float4* p = field <= AGX_BODY_ROTATION ? d.pos : d.vel;
+        std::vector<float4> h(n);
+        HIP_CHECK(w, hipMemcpy(h.data(), p, n * sizeof(float4), hipMemcpyDeviceToHost));
+        const uint32_t lane = field <= AGX_BODY_ROTATION ? field : field - AGX_BODY_VEL_X;
+        for (uint32_t i = 0; i < n; i++) (lane == 0 ? h[i].x : lane == 1 ? h[i].y : h[i].z) = in[i];
+        HIP_CHECK(w, hipMemcpy(p, h.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        return AGX_OK;
+    }
+    std::vector<float2> h(n);
+    HIP_CHECK(w, hipMemcpy(h.data(), d.rcs, n * sizeof(float2), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) (field == AGX_BODY_COS ? h[i].x : h[i].y) = in[i];
+    HIP_CHECK(w, hipMemcpy(d.rcs, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    return AGX_OK;
+}
+
+int agx_world_read_collider_f32(agx_world* w, uint32_t field, float* out, uint32_t n) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    agx::DeviceWorld& d = w->d;
+    if (n != d.C) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u colliders, got %u", d.C, n);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    switch (field) {
+    case AGX_COL_WORLD_X: case AGX_COL_WORLD_Y: case AGX_COL_COS: case AGX_COL_SIN: {
+        std::vector<float4> h(n);
+        HIP_CHECK(w, hipMemcpy(h.data(), d.pose, n * sizeof(float4), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++)
+            out[i] = field == AGX_COL_WORLD_X ? h[i].x : field == AGX_COL_WORLD_Y ? h[i].y : field == AGX_COL_COS ? h[i].z : h[i].w;
+        return AGX_OK;
+    }
+    case AGX_COL_AABB_MIN_X: case AGX_COL_AABB_MIN_Y: case AGX_COL_AABB_MAX_X: case AGX_COL_AABB_MAX_Y: {
+        std::vector<float4> h(n);
+        HIP_CHECK(w, hipMemcpy(h.data(), d.aabb, n * sizeof(float4), hipMemcpyDeviceToHost));
+        const uint32_t k = field - AGX_COL_AABB_MIN_X;
+        for (uint32_t i = 0; i < n; i++) out[i] = k == 0 ? h[i].x : k == 1 ? h[i].y : k == 2 ? h[i].z : h[i].w;
+        return AGX_OK;
+    }
+    case AGX_COL_WORLD_ROT:
+        HIP_CHECK(w, hipMemcpy(out, d.wRot, n * 4u, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    case AGX_COL_PREV_X: case AGX_COL_PREV_Y: case AGX_COL_COM_OFF_X: case AGX_COL_COM_OFF_Y: {
+        std::vector<float2> h(n);
+        const bool pv = field <= AGX_COL_PREV_Y;
+        HIP_CHECK(w, hipMemcpy(h.data(), pv ? d.prevW : d.comOff, n * sizeof(float2), hipMemcpyDeviceToHost));
+        const bool x = field == AGX_COL_PREV_X || field == AGX_COL_COM_OFF_X;
+        for (uint32_t i = 0; i < n; i++) out[i] = x ? h[i].x : h[i].y;
+        return AGX_OK;
+    }
+    case AGX_COL_LOCAL_ROT:
+        HIP_CHECK(w, hipMemcpy(out, d.localRot, n * 4u, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    case AGX_COL_P0: std::memcpy(out, w->colP0.data(), n * 4u); return AGX_OK;
+    case AGX_COL_P1: std::memcpy(out, w->colP1.data(), n * 4u); return AGX_OK;
+    case AGX_COL_MIN_EXTENT: std::memcpy(out, w->colMinExt.data(), n * 4u); return AGX_OK;
+    case AGX_COL_MAX_EXTENT: std::memcpy(out, w->colMaxExt.data(), n * 4u); return AGX_OK;
+    default:
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown collider field %u", field);
+    }
+}
+
+int agx_world_read_collider_u32(agx_world* w, uint32_t field, uint32_t* out, uint32_t n) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != w->d.C) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u colliders, got %u", w->d.C, n);
+    switch (field) {
+    case AGX_COL_BODY: HIP_CHECK(w, hipMemcpy(out, w->d.colBody, n * 4u, hipMemcpyDeviceToHost)); return AGX_OK;
+    case AGX_COL_TYPE: std::memcpy(out, w->colType.data(), n * 4u); return AGX_OK;
+    case AGX_COL_MATERIAL: std::memcpy(out, w->colMaterial.data(), n * 4u); return AGX_OK;
+    case AGX_COL_SLOT: std::memcpy(out, w->colSlot.data(), n * 4u); return AGX_OK;
+    case AGX_COL_POLY_START: std::memcpy(out, w->polyStart.data(), n * 4u); return AGX_OK;
+    case AGX_COL_POLY_COUNT: std::memcpy(out, w->polyCount.data(), n * 4u); return AGX_OK;
+    case AGX_COL_SCENE: std::memcpy(out, w->colScene.data(), n * 4u); return AGX_OK;
+    default: return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown collider field %u", field);
+    }
+}
+
+int agx_world_read_pairs(agx_world* w, uint32_t* a, uint32_t* b, uint64_t cap, uint64_t* count) {
+    if (!w || !count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    agx::Counters c;
+    HIP_CHECK(w, hipMemcpy(&c, w->d.ctr, sizeof c, hipMemcpyDeviceToHost));
+    const uint64_t n = std::min<uint64_t>(c.pairs, w->d.pairCap);
+    *count = c.pairs;
+    if (n > cap) return fail(w, AGX_ERR_CAPACITY, "%llu pairs exceed cap %llu", (unsigned long long)n, (unsigned long long)cap);
+    if (n && (!a || !b)) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null pair arrays");
+    std::vector<uint2> h(n);
+    if (n) HIP_CHECK(w, hipMemcpy(h.data(), w->d.pairs, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; i++) { a[i] = h[i].x; b[i] = h[i].y; }
+    if (c.overflow & 1u) return fail(w, AGX_ERR_CAPACITY, "pair buffer overflow");
+    return AGX_OK;
+}
+
+int agx_world_read_manifolds(agx_world* w, uint32_t which, agx_manifold* out, uint64_t cap, uint64_t* count) {
+    if (!w || !count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    agx::Counters c;
+    HIP_CHECK(w, hipMemcpy(&c, w->d.ctr, sizeof c, hipMemcpyDeviceToHost));
+    const agx_manifold* src;
+    uint64_t n;
+    if (w->inStep && which == 0) { src = w->d.man[w->d.cur]; n = c.manifolds; }
+    else { src = w->d.man[1 - w->d.cur]; n = c.prevManifolds; }
+    *count = n;
+    if (n > cap) return fail(w, AGX_ERR_CAPACITY, "%llu manifolds exceed cap", (unsigned long long)n);
+    if (n && !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null output");
+    if (n) HIP_CHECK(w, hipMemcpy(out, src, n * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
+int agx_world_write_manifolds(agx_world* w, uint32_t which, const agx_manifold* in, uint64_t count) {
+    if (!w || (count && !in)) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (which != 1 || w->inStep) return fail(w, AGX_ERR_STATE, "only the warm-start source can be written, between steps");
+    if (count > w->d.manCap) return fail(w, AGX_ERR_CAPACITY, "too many manifolds");
+    agx::DeviceWorld& d = w->d;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    if (count) HIP_CHECK(w, hipMemcpy(d.man[1 - d.cur], in, count * sizeof(agx_manifold), hipMemcpyHostToDevice));
+    const uint32_t m = (uint32_t)count;
+    HIP_CHECK(w, hipMemcpy(&d.ctr->manifolds, &m, 4, hipMemcpyHostToDevice));
+    d.cur ^= 1;
+    HIP_CHECK(w, agx::launchHashBuild(w->stream, d));
+    d.cur ^= 1;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    return AGX_OK;
+}
+
+int agx_world_read_solve_order(agx_world* w, uint32_t* order, uint64_t cap, uint32_t* colour_start, uint32_t colour_cap,
+                               uint32_t* colours) {
+    if (!w || !colours) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    agx::Counters* c = new agx::Counters;
+    if (hipMemcpy(c, w->d.ctr, sizeof *c, hipMemcpyDeviceToHost) != hipSuccess) { delete c; return fail(w, AGX_ERR_HIP, "read counters"); }
+    const uint32_t K = c->colours;
+    const uint32_t total = K ? c->colourStart[K] : 0u;
+    *colours = K;
+    if (K + 1 > colour_cap) { delete c; return fail(w, AGX_ERR_CAPACITY, "%u colours exceed cap", K); }
+    if (colour_start) std::memcpy(colour_start, c->colourStart, (K + 1) * 4u);
+    delete c;
+    if (total > cap) return fail(w, AGX_ERR_CAPACITY, "order needs %u entries", total);
+    if (total && order) HIP_CHECK(w, hipMemcpy(order, w->d.order, total * 4u, hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
+int agx_world_read_islands(agx_world* w, uint32_t* island_id, uint32_t n, uint32_t* island_count) {
+    if (!w || !island_id || !island_count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != w->d.N) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u bodies", w->d.N);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    HIP_CHECK(w, hipMemcpy(island_id, w->d.islandId, n * 4u, hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(island_count, &w->d.ctr->islands, 4, hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
+int agx_world_read_stats(agx_world* w, agx_stats* out) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    agx::Counters* c = new agx::Counters;
+    if (hipMemcpy(c, w->d.ctr, sizeof *c, hipMemcpyDeviceToHost) != hipSuccess) { delete c; return fail(w, AGX_ERR_HIP, "read counters"); }
+    std::memset(out, 0, sizeof *out);
+    out->pairs = c->pairs;
+    out->manifolds = w->inStep ? c->manifolds : c->prevManifolds;
+    out->contacts = c->contacts;
+    out->colours = c->colours;
+    out->islands = c->islands;
+    out->ccd_hits = c->ccdHits;
+    out->velocity_sweeps = c->sweeps;
+    out->overflow = c->overflow;
+    out->velocity_error = c->velocityError;
+    out->position_error = c->positionError;
+    for (uint32_t k = 0; k < 64 && k < agx::MAX_SWEEPS; k++)
+        out->residual[k] = (float)(std::sqrt(c->residualNum[k]) / std::fmax(std::sqrt(c->residualDen[k]), 1e-12));
+    delete c;
+    return AGX_OK;
+}
+
+int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t* is_dynamic, const uint32_t* scene,
+                         uint32_t n, uint32_t* pairs_a, uint32_t* pairs_b, uint64_t cap, uint64_t* count) {
+    if (!count || (n && (!aabb || !owner || !is_dynamic))) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    *count = 0;
+    if (n == 0) return AGX_OK;
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices == 0) return fail(nullptr, AGX_ERR_NO_DEVICE, "no HIP device");
+    agx_world* w = new (std::nothrow) agx_world();
+    if (!w) return fail(nullptr, AGX_ERR_CAPACITY, "out of host memory");
+    auto bail = [&](int code) { const std::string e = w->err; agx_world_destroy(w); g_lastError = e; return code; };
+    agx::DeviceWorld& d = w->d;
+    uint32_t S = 1;
+    std::vector<uint32_t> sc(n, 0);
+    if (scene) for (uint32_t i = 0; i < n; i++) { sc[i] = scene[i]; S = std::max(S, scene[i] + 1); }
+    d.S = S;
+    d.sceneBits = ceilLog2(S);
+    if (d.sceneBits > 16) return bail(fail(w, AGX_ERR_INVALID_ARGUMENT, "too many scenes"));
+    d.mortonBits = std::min(30u, (32u - d.sceneBits) & ~1u);
+    d.C = n;
+    std::vector<uint32_t> dynList, statList, body(owner, owner + n);
+    for (uint32_t i = 0; i < n; i++) (is_dynamic[i] ? dynList : statList).push_back(i);
+    d.nDyn = (uint32_t)dynList.size();
+    d.nStat = (uint32_t)statList.size();
+    std::vector<float4> boxes(n);
+    for (uint32_t i = 0; i < n; i++) boxes[i] = make_float4(aabb[4 * i], aabb[4 * i + 1], aabb[4 * i + 2], aabb[4 * i + 3]);
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return bail(fail(w, AGX_ERR_HIP, "hipGetDeviceProperties failed"));
+    d.numCU = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(w, AGX_ERR_HIP, "stream"));
+    if (upload(w, &d.aabb, boxes) != hipSuccess || upload(w, &d.colBody, body) != hipSuccess ||
+        upload(w, &d.colScene, sc) != hipSuccess)
+        return bail(fail(w, AGX_ERR_HIP, "upload failed"));
+    int rc;
+    if ((rc = allocTree(w, d.dyn, dynList, sc, S))) return bail(rc);
+    if ((rc = allocTree(w, d.stat, statList, sc, S))) return bail(rc);
+    if ((rc = allocPairBuffers(w, std::max<uint64_t>(cap, 1)))) return bail(rc);
+    hipStream_t st = w->stream;
+    if (agx::launchBuildTree(st, d, d.stat) != hipSuccess || agx::launchBuildTree(st, d, d.dyn) != hipSuccess ||
+        agx::launchPairs(st, d) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return bail(fail(w, AGX_ERR_HIP, "broad phase failed: %s", hipGetErrorString(hipGetLastError())));
+    agx::Counters c;
+    if (hipMemcpy(&c, d.ctr, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return bail(fail(w, AGX_ERR_HIP, "read counters"));
+    *count = c.pairs;
+    if (c.overflow & 2u) return bail(fail(w, AGX_ERR_CAPACITY, "traversal stack overflow"));
+    if ((uint64_t)c.pairs > cap) return bail(fail(w, AGX_ERR_CAPACITY, "%u pairs exceed cap", c.pairs));
+    std::vector<uint2> h(c.pairs);
+    if (c.pairs && hipMemcpy(h.data(), d.pairs, c.pairs * sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
+        return bail(fail(w, AGX_ERR_HIP, "read pairs"));
+    for (uint32_t i = 0; i < c.pairs; i++) { pairs_a[i] = h[i].x; pairs_b[i] = h[i].y; }
+    agx_world_destroy(w);
+    return AGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+"""ctypes wrapper of oracle/liboracle.so (the CPU restatement used as the checker)."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = ROOT / "oracle" / "liboracle.so"
+
+MANIFOLD_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("nx", "<f4"), ("ny", "<f4"), ("depth", "<f4"),
+                           ("count", "<u4"), ("p0x", "<f4"), ("p0y", "<f4"), ("p1x", "<f4"), ("p1y", "<f4"),
+                           ("id0", "<u4"), ("id1", "<u4"), ("jn0", "<f4"), ("jt0", "<f4"), ("jn1", "<f4"),
+                           ("jt1", "<f4")])
+
+_P = C.c_void_p
+_SIGS = {
+    "orc_new": (_P, []), "orc_free": (None, [_P]),
+    "orc_set_settings": (None, [_P, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float]),
+    "orc_add_material": (C.c_uint32, [_P, C.c_float, C.c_float, C.c_float]),
+    "orc_add_bodies": (C.c_int, [_P, C.c_uint32] + [_P] * 15),
+    "orc_refresh": (None, [_P]), "orc_counts": (None, [_P, _P]),
+    "orc_get_f32": (C.c_int, [_P, C.c_char_p, _P, C.c_uint32]),
+    "orc_set_f32": (C.c_int, [_P, C.c_char_p, _P, C.c_uint32]),
+    "orc_get_u32": (C.c_int, [_P, C.c_char_p, _P, C.c_uint32]),
+    "orc_get_types": (C.c_int, [_P, _P, C.c_uint32]),
+    "orc_get_verts": (C.c_int, [_P, _P, _P, C.c_uint32]),
+    "orc_fast_cossin": (None, [_P, _P, _P, C.c_uint32]), "orc_wrap": (None, [_P, _P, C.c_uint32]),
+    "orc_snapshot": (None, [_P]), "orc_integrate": (None, [_P, C.c_float]), "orc_wrap_cossin": (None, [_P]),
+    "orc_poses": (None, [_P, C.c_int]), "orc_aabbs": (None, [_P, C.c_int]), "orc_ccd": (C.c_int, [_P, _P]),
+    "orc_broad_bvh": (C.c_int64, [_P, _P, _P, C.c_uint64]), "orc_broad_brute": (C.c_int64, [_P, _P, _P, C.c_uint64]),
+    "orc_narrow": (C.c_int64, [_P, _P, _P, C.c_uint64, _P, C.c_uint64]),
+    "orc_set_prev_manifolds": (None, [_P, _P, C.c_uint64]), "orc_warm_start": (None, [_P, _P, C.c_uint64]),
+    "orc_solve": (None, [_P, _P, C.c_uint64, _P, C.c_uint32, C.c_uint32, _P]),
+    "orc_islands": (C.c_uint32, [_P, _P, C.c_uint64]),
+    "orc_step": (None, [_P, C.c_float]), "orc_post_update": (None, [_P]),
+    "orc_update": (None, [_P, C.c_float, C.c_uint32]),
+    "orc_get_manifolds": (C.c_int64, [_P, _P, C.c_uint64]), "orc_errors": (None, [_P, _P]),
+}
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C oracle`")
+        h = C.CDLL(str(LIB_PATH))
+        for k, (r, a) in _SIGS.items():
+            f = getattr(h, k)
+            f.restype = r
+            f.argtypes = a
+        _LIB = h
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+BODY_F32 = ["comX", "comY", "velX", "velY", "rot", "angVel", "mass", "invMass", "inertia", "invInertia", "cos",
+            "sin", "localComX", "localComY", "boundRadius", "prevComX", "prevComY"]
+COL_F32 = ["localOffX", "localOffY", "comOffX", "comOffY", "localRot", "wX", "wY", "wRot", "wCos", "wSin", "minX",
+           "minY", "maxX", "maxY", "p0", "p1", "minExtent", "maxExtent", "prevWX", "prevWY"]
+
+
+class World:
+    """One oracle world (a single scene)."""
+
+    def __init__(self, settings=None):
+        self.h = lib().orc_new()
+        s = dict(gx=0.0, gy=-9.81, v=6, p=3, ccd=1, fast=1.0, core=0.5)
+        if settings:
+            s.update(settings)
+        lib().orc_set_settings(self.h, s["gx"], s["gy"], s["v"], s["p"], s["ccd"], s["fast"], s["core"])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_free(self.h)
+            self.h = None
+
+    def add_material(self, e, us, ud):
+        return lib().orc_add_material(self.h, e, us, ud)
+
+    def add_bodies(self, b):
+        n = len(b["type"])
+        f = lambda k, dt=np.float32: np.ascontiguousarray(b[k], dt)
+        arrs = [f("type", np.uint8), f("px"), f("py"), f("rot"), f("vx"), f("vy"), f("av"), f("mass"),
+                f("material", np.uint32), f("p0"), f("p1"), f("vstart", np.uint32), f("vcount", np.uint32)]
+        verts = np.ascontiguousarray(b["verts"] if len(b["verts"]) else np.zeros((1, 2)), np.float32)
+        out = np.empty(n, np.uint32)
+        rc = lib().orc_add_bodies(self.h, n, *[_p(a) for a in arrs], _p(verts), _p(out))
+        return out, rc
+
+    def refresh(self):
+        lib().orc_refresh(self.h)
+
+    def counts(self):
+        out = np.zeros(10, np.uint32)
+        lib().orc_counts(self.h, _p(out))
+        keys = ["bodies", "static_bodies", "colliders", "static_colliders", "vertices", "materials", "manifolds",
+                "pairs", "islands", "prev_manifolds"]
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def get(self, name):
+        c = self.counts()
+        if name in COL_F32:
+            n, dt = c["colliders"], np.float32
+        elif name in BODY_F32:
+            n, dt = c["bodies"], np.float32
+        elif name in ("colBody", "colMaterial", "colSlot", "polyStart", "polyCount"):
+            n, dt = c["colliders"], np.uint32
+        else:
+            n, dt = c["bodies"], np.uint32
+        out = np.empty(n, dt)
+        fn = lib().orc_get_f32 if dt == np.float32 else lib().orc_get_u32
+        rc = fn(self.h, name.encode(), _p(out), n)
+        if rc != 0:
+            raise KeyError(name)
+        return out
+
+    def set(self, name, values):
+        v = np.ascontiguousarray(values, np.float32)
+        if lib().orc_set_f32(self.h, name.encode(), _p(v), len(v)) != 0:
+            raise KeyError(name)
+
+    def types(self):
+        n = self.counts()["colliders"]
+        out = np.empty(n, np.uint8)
+        lib().orc_get_types(self.h, _p(out), n)
+        return out
+
+    def vertices(self):
+        n = self.counts()["vertices"]
+        v = np.empty((n, 2), np.float32)
+        nn = np.empty((n, 2), np.float32)
+        lib().orc_get_verts(self.h, _p(v), _p(nn), n)
+        return v, nn
+
+    # phases
+    def snapshot(self): lib().orc_snapshot(self.h)
+    def integrate(self, dt): lib().orc_integrate(self.h, dt)
+    def wrap_cossin(self): lib().orc_wrap_cossin(self.h)
+    def poses(self, all_=False): lib().orc_poses(self.h, int(all_))
+    def aabbs(self, all_=False): lib().orc_aabbs(self.h, int(all_))
+    def step(self, dt): lib().orc_step(self.h, dt)
+    def post_update(self): lib().orc_post_update(self.h)
+    def update(self, dt, steps=1): lib().orc_update(self.h, dt, steps)
+
+    def ccd(self):
+        t = np.empty(self.counts()["bodies"], np.float32)
+        hit = lib().orc_ccd(self.h, _p(t))
+        return bool(hit), t
+
+    def _pairs(self, fn):
+        cap = 1 << 16
+        while True:
+            a = np.empty(cap, np.uint32)
+            b = np.empty(cap, np.uint32)
+            n = fn(self.h, _p(a), _p(b), cap)
+            if n >= 0:
+                return a[:n], b[:n]
+            cap = -n
+
+    def broad_bvh(self): return self._pairs(lib().orc_broad_bvh)
+    def broad_brute(self): return self._pairs(lib().orc_broad_brute)
+
+    def narrow(self, pa, pb):
+        pa = np.ascontiguousarray(pa, np.uint32)
+        pb = np.ascontiguousarray(pb, np.uint32)
+        out = np.empty(max(len(pa), 1), MANIFOLD_DTYPE)
+        n = lib().orc_narrow(self.h, _p(pa), _p(pb), len(pa), _p(out), len(out))
+        return out[:n]
+
+    def set_prev_manifolds(self, m):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        lib().orc_set_prev_manifolds(self.h, _p(m), len(m))
+
+    def warm_start(self, m):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE).copy()
+        lib().orc_warm_start(self.h, _p(m), len(m))
+        return m
+
+    def solve(self, m, order=None, v_it=8, p_it=3, residual=False):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE).copy()
+        o = None if order is None else np.ascontiguousarray(order, np.uint32)
+        r = np.zeros(max(v_it, 1), np.float32) if residual else None
+        lib().orc_solve(self.h, _p(m), len(m), _p(o), v_it, p_it, _p(r))
+        return (m, r) if residual else m
+
+    def islands(self, m):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        k = lib().orc_islands(self.h, _p(m), len(m))
+        return self.get("islandId"), k
+
+    def manifolds(self):
+        cap = max(self.counts()["manifolds"], 1)
+        out = np.empty(cap, MANIFOLD_DTYPE)
+        n = lib().orc_get_manifolds(self.h, _p(out), cap)
+        return out[:n]
+
+    def errors(self):
+        out = np.zeros(2, np.float32)
+        lib().orc_errors(self.h, _p(out))
+        return out
+
+
+def fast_cossin(x):
+    x = np.ascontiguousarray(x, np.float32)
+    c = np.empty_like(x)
+    s = np.empty_like(x)
+    lib().orc_fast_cossin(_p(x), _p(c), _p(s), len(x))
+    return c, s
+
+
+def wrap(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_wrap(_p(x), _p(y), len(x))
+    return y
+
+
+def world_from_scene(scene, settings=None):
+    w = World(settings)
+    for m in scene.get("materials", []):
+        w.add_material(*m)
+    idx, rc = w.add_bodies(scene["bodies"])
+    w.refresh()
+    return w

@@ -1,0 +1,266 @@
+"""GPU parity: the HIP path against the oracle, phase by phase, on identical inputs.
+
+Bars: integration, poses and AABBs bit for bit (same operation order, IEEE division and
+square root, no contraction); broad-phase pair sets bit for bit; manifolds and warm-start
+impulses bit for bit (checked at the 1e-5 relative bar as well); solver impulses within 1e-4
+of the oracle stepped in the GPU's colour order; island partitions bit for bit.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 60.0
+
+
+def make_world(agx, scene, **settings):
+    b = agx.Builder(1)
+    agx.build_scene(b, scene)
+    s = agx.default_settings(**{**scene.get("settings", {}), **settings})
+    return agx.World(b, s)
+
+
+def oracle_from_gpu(oracle, scene, w, settings=None):
+    """Oracle world with the GPU world's current state (bodies, collider poses, warm-start source)."""
+    ow = oracle.world_from_scene(scene, settings)
+    for g, o in (("com_x", "comX"), ("com_y", "comY"), ("rotation", "rot"), ("vel_x", "velX"), ("vel_y", "velY"),
+                 ("ang_vel", "angVel"), ("cos", "cos"), ("sin", "sin")):
+        ow.set(o, w.body(g))
+    for g, o in (("world_x", "wX"), ("world_y", "wY"), ("cos", "wCos"), ("sin", "wSin"), ("world_rot", "wRot"),
+                 ("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY")):
+        ow.set(o, w.collider(g))
+    ow.set_prev_manifolds(w.manifolds(1))
+    return ow
+
+
+def sort_key(m):
+    return np.argsort((m["a"].astype(np.uint64) << np.uint64(32)) | m["b"].astype(np.uint64), kind="stable")
+
+
+def pair_set(a, b, is_dyn):
+    s = set()
+    for x, y in zip(a.tolist(), b.tolist()):
+        s.add((min(x, y), max(x, y)) if is_dyn[x] and is_dyn[y] else (x, y))
+    return s
+
+
+def brute_pairs(aabb, owner, dyn, scene=None):
+    """Vectorised exact pair set: strict overlap, different owner, at least one dynamic, same scene."""
+    n = len(owner)
+    order = np.argsort(aabb[:, 0], kind="stable")
+    mnx = aabb[order, 0]
+    out = set()
+    hi = np.searchsorted(mnx, aabb[order, 2], side="left")
+    for k in range(n):
+        i = order[k]
+        js = order[k + 1:hi[k]]
+        if len(js) == 0:
+            continue
+        ok = (aabb[i, 0] < aabb[js, 2]) & (aabb[i, 2] > aabb[js, 0]) & (aabb[i, 1] < aabb[js, 3]) & (aabb[i, 3] > aabb[js, 1])
+        ok &= owner[js] != owner[i]
+        ok &= dyn[js] | dyn[i]
+        if scene is not None:
+            ok &= scene[js] == scene[i]
+        for j in js[ok].tolist():
+            if dyn[i] and dyn[j]:
+                out.add((min(i, j), max(i, j)))
+            elif dyn[i]:
+                out.add((i, j))
+            else:
+                out.add((j, i))
+    return out
+
+
+@pytest.mark.parametrize("n,scenes_n", [(3000, 1), (20000, 1), (5000, 7)])
+def test_broadphase_pairs_match_brute_force(agx, n, scenes_n):
+    rng = np.random.default_rng(n + scenes_n)
+    size = np.sqrt(n) * 1.2
+    c = rng.uniform(0, size, (n, 2)).astype(np.float32)
+    e = rng.uniform(0.1, 1.2, (n, 2)).astype(np.float32)
+    aabb = np.concatenate([c - e, c + e], axis=1).astype(np.float32)
+    owner = np.arange(n, dtype=np.uint32)
+    owner[1::17] = owner[0::17][: len(owner[1::17])]          # some bodies own two colliders
+    dyn = (rng.random(n) > 0.1).astype(np.uint8)
+    scene = rng.integers(0, scenes_n, n).astype(np.uint32) if scenes_n > 1 else None
+    a, b = agx.broadphase_pairs(aabb, owner, dyn, scene)
+    got = pair_set(a, b, dyn.astype(bool))
+    assert len(got) == len(a), "duplicate pairs"
+    assert got == brute_pairs(aabb, owner, dyn.astype(bool), scene)
+
+
+def test_broadphase_edge_cases(agx):
+    # empty input, one collider, touching (not strictly overlapping) boxes, identical boxes.
+    a, b = agx.broadphase_pairs(np.zeros((0, 4), np.float32), np.zeros(0, np.uint32), np.zeros(0, np.uint8))
+    assert len(a) == 0
+    one = np.array([[0, 0, 1, 1]], np.float32)
+    a, b = agx.broadphase_pairs(one, np.zeros(1, np.uint32), np.ones(1, np.uint8))
+    assert len(a) == 0
+    boxes = np.array([[0, 0, 1, 1], [1, 0, 2, 1], [0, 0, 1, 1], [0, 0, 1, 1]], np.float32)
+    a, b = agx.broadphase_pairs(boxes, np.array([0, 1, 2, 2], np.uint32), np.ones(4, np.uint8))
+    assert pair_set(a, b, np.ones(4, bool)) == {(0, 2), (0, 3)}   # touching edges do not overlap; 2,3 share a body
+
+
+def test_step_phases_match_oracle(agx, oracle, scenes):
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene)
+    w.step(DT, 90)
+    ow = oracle_from_gpu(oracle, scene, w, dict(v=8, p=3))
+    ow.snapshot()
+    # integrate + wrap + cos/sin
+    w.run_phase(agx.PHASE_INTEGRATE, DT)
+    ow.integrate(DT)
+    ow.wrap_cossin()
+    for g, o in (("com_x", "comX"), ("com_y", "comY"), ("rotation", "rot"), ("vel_x", "velX"), ("vel_y", "velY"),
+                 ("ang_vel", "angVel"), ("cos", "cos"), ("sin", "sin")):
+        assert np.array_equal(w.body(g), ow.get(o)), g
+    # poses + AABBs (+ the CCD sweep origin)
+    w.run_phase(agx.PHASE_POSE, DT)
+    ow.poses()
+    ow.aabbs()
+    for g, o in (("world_x", "wX"), ("world_y", "wY"), ("world_rot", "wRot"), ("cos", "wCos"), ("sin", "wSin"),
+                 ("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY"),
+                 ("prev_x", "prevWX"), ("prev_y", "prevWY")):
+        gv, ov = w.collider(g), ow.get(o)
+        ns = ow.counts()["static_colliders"]
+        assert np.array_equal(gv[ns:], ov[ns:]), g
+    w.run_phase(agx.PHASE_CCD, DT)
+    hit, _ = ow.ccd()
+    assert w.collider("world_x").tobytes() == ow.get("wX").tobytes()
+    # broad phase: identical pair set
+    w.run_phase(agx.PHASE_BROAD, DT)
+    ga, gb = w.pairs()
+    oa, ob = ow.broad_brute()
+    is_dyn = np.zeros(ow.counts()["colliders"], bool)
+    is_dyn[ow.counts()["static_colliders"]:] = True
+    gset = pair_set(ga, gb, is_dyn)
+    assert len(gset) == len(ga)
+    assert gset == pair_set(oa, ob, is_dyn)
+    assert len(gset) > 1000
+    # narrow phase + warm start on the GPU's pairs
+    w.run_phase(agx.PHASE_NARROW, DT)
+    mg = w.manifolds(0)
+    mo = ow.warm_start(ow.narrow(ga, gb))
+    assert len(mg) == len(mo) > 1000
+    mg_s, mo_s = mg[sort_key(mg)], mo[sort_key(mo)]
+    for f in mg.dtype.names:
+        if f in ("p1x", "p1y", "id1", "jn1", "jt1"):
+            two = mg_s["count"] == 2
+            assert np.array_equal(mg_s[f][two], mo_s[f][two]), f
+        else:
+            assert np.array_equal(mg_s[f], mo_s[f]), f
+    assert np.count_nonzero(mg["jn0"]) > len(mg) // 2, "warm start found few previous impulses"
+    # solve in the GPU's colour order, from the same pre-solve state
+    w.run_phase(agx.PHASE_SOLVE, DT)
+    order, starts = w.solve_order()
+    assert len(order) == len(mg) and np.array_equal(np.sort(order), np.arange(len(mg)))
+    body_of = w.collider("body")
+    inv_mass = w.body("inv_mass")
+    for k in range(len(starts) - 1):
+        ms = mg[order[starts[k]:starts[k + 1]]]
+        bodies = np.concatenate([body_of[ms["a"]], body_of[ms["b"]]])
+        bodies = bodies[inv_mass[bodies] != 0]
+        assert len(np.unique(bodies)) == len(bodies), f"colour {k} shares a dynamic body"
+    solved_o = ow.solve(mg, order=order, v_it=8, p_it=3)
+    solved_g = w.manifolds(0)
+    for f in ("jn0", "jt0", "jn1", "jt1"):
+        two = solved_g["count"] == 2 if f in ("jn1", "jt1") else np.ones(len(solved_g), bool)
+        np.testing.assert_allclose(solved_g[f][two], solved_o[f][two], rtol=1e-4, atol=1e-4, err_msg=f)
+    for g, o in (("vel_x", "velX"), ("vel_y", "velY"), ("ang_vel", "angVel"), ("com_x", "comX"), ("com_y", "comY")):
+        np.testing.assert_allclose(w.body(g), ow.get(o), rtol=1e-4, atol=1e-4, err_msg=g)
+    # islands
+    w.run_phase(agx.PHASE_ISLANDS, DT)
+    ids_g, k_g = w.islands()
+    ids_o, k_o = ow.islands(solved_g)
+    assert k_g == k_o
+    canon = lambda lab: {frozenset(np.flatnonzero(lab == v).tolist()) for v in np.unique(lab)}
+    assert canon(ids_g) == canon(ids_o)
+    w.run_phase(agx.PHASE_FINISH, DT)
+    st = w.stats()
+    assert st.overflow == 0 and st.manifolds == len(mg)
+
+
+def test_long_run_is_stable(agx, scenes):
+    scene = scenes.pile_small(seed=3)
+    w = make_world(agx, scene, track_errors=1)
+    w.step(DT, 300)
+    y = w.body("com_y")
+    x = w.body("com_x")
+    assert np.all(np.isfinite(y)) and np.all(np.isfinite(x))
+    ns = w.counts.static_bodies
+    assert np.all(y[ns:] > -0.5) and np.all(np.abs(x[ns:]) < 33.0)   # nothing escapes the box
+    st = w.stats()
+    assert st.overflow == 0
+    assert st.position_error / max(st.manifolds, 1) < 0.02        # mean residual penetration per manifold
+
+
+def test_ccd_matches_oracle(agx, oracle, scenes):
+    # fast circles and polygons shot at thin static walls
+    acc = scenes._Acc()
+    acc.add(1, 20.0, 0.0, p0=0.3, p1=40.0)
+    acc.add(2, -20.0, 0.0, rot=0.3, verts=scenes.polygon_outline(np.random.default_rng(0), 6, 1.5))
+    rng = np.random.default_rng(4)
+    for i in range(64):
+        y = float(rng.uniform(-15, 15))
+        if i % 2:
+            acc.add(0, 0.0, y, vx=900.0 * (1 if i % 4 == 1 else -1), mass=1.0, p0=0.2)
+        else:
+            acc.add(2, 0.0, y, rot=float(rng.uniform(0, 6)), vx=-900.0 if i % 4 == 0 else 900.0, mass=1.0,
+                    verts=scenes.polygon_outline(rng, 5, 0.3))
+    scene = {"materials": [(0.0, 0.6, 0.6)], "bodies": acc.bodies(), "settings": {"gravity_y": 0.0}}
+    w = make_world(agx, scene)
+    w.step(DT, 1)
+    ow = oracle_from_gpu(oracle, scene, w, dict(gy=0.0))
+    ow.snapshot()
+    for p in (agx.PHASE_INTEGRATE, agx.PHASE_POSE):
+        w.run_phase(p, DT)
+    ow.integrate(DT)
+    ow.wrap_cossin()
+    ow.poses()
+    ow.aabbs()
+    w.run_phase(agx.PHASE_CCD, DT)
+    hit, t = ow.ccd()
+    assert hit
+    np.testing.assert_allclose(w.body("com_x"), ow.get("comX"), rtol=0, atol=1e-4)
+    np.testing.assert_allclose(w.collider("aabb_max_x"), ow.get("maxX"), rtol=0, atol=1e-4)
+    for p in range(agx.PHASE_BROAD, agx.PHASE_FINISH + 1):
+        w.run_phase(p, DT)
+    assert w.stats().ccd_hits == int(np.count_nonzero(t < 1.0))
+
+
+def test_batched_scenes_match_single_scenes(agx, oracle, scenes):
+    # S independent scenes in one world give each scene's own pairs and manifolds.
+    parts = [scenes.pile(120 + 40 * s, 30, 14.0, 14.0, seed=20 + s) for s in range(5)]
+    b = agx.Builder(len(parts))
+    for s, sc in enumerate(parts):
+        agx.build_scene(b, sc, s)
+    w = agx.World(b, agx.default_settings(velocity_iterations=8))
+    w.step(DT, 60)
+    for p in range(agx.PHASE_NARROW + 1):
+        w.run_phase(p, DT)
+    ga, gb = w.pairs()
+    scene_of = w.collider("scene")
+    assert np.all(scene_of[ga] == scene_of[gb])
+    mg = w.manifolds(0)
+    c_off = 0
+    for s, sc in enumerate(parts):
+        nc = int(np.count_nonzero(scene_of == s))
+        sel = (ga >= c_off) & (ga < c_off + nc)
+        ow = oracle.world_from_scene(sc)
+        for g, o in (("world_x", "wX"), ("world_y", "wY"), ("cos", "wCos"), ("sin", "wSin"),
+                     ("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY")):
+            ow.set(o, w.collider(g)[c_off:c_off + nc])
+        oa, ob = ow.broad_brute()
+        is_dyn = np.zeros(nc, bool)
+        is_dyn[ow.counts()["static_colliders"]:] = True
+        assert pair_set(ga[sel] - c_off, gb[sel] - c_off, is_dyn) == pair_set(oa, ob, is_dyn)
+        ms = mg[(mg["a"] >= c_off) & (mg["a"] < c_off + nc)].copy()
+        mo = ow.narrow(ga[sel] - c_off, gb[sel] - c_off)
+        ms["a"] -= c_off
+        ms["b"] -= c_off
+        assert len(ms) == len(mo)
+        ms, mo = ms[sort_key(ms)], mo[sort_key(mo)]
+        for f in ("a", "b", "count", "nx", "ny", "depth", "p0x", "p0y", "id0"):
+            assert np.array_equal(ms[f], mo[f]), (s, f)
+        c_off += nc
+    for p in range(agx.PHASE_SOLVE, agx.PHASE_FINISH + 1):
+        w.run_phase(p, DT)
+    assert w.stats().overflow == 0
