@@ -424,25 +424,33 @@ __global__ __launch_bounds__(256) void kUfUnite(const agx_manifold* __restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void kUfFlatten(uint32_t* parent, const uint32_t* __restrict__ list, uint32_t n,
-                                                  uint32_t* __restrict__ rootFlag) {
+// Read-only root lookup: no path splitting, so concurrent lookups never rewrite parents and
+// every body sees the final root (all unions finished in the previous kernel).
+__device__ inline uint32_t ufRootOf(const uint32_t* parent, uint32_t x) {
+    uint32_t p = parent[x];
+    while (p != x) { x = p; p = parent[x]; }
+    return x;
+}
+
+// root[b] (kept in islandId as scratch) and a root flag per dynamic body for the rank scan.
+__global__ __launch_bounds__(256) void kUfRoots(const uint32_t* __restrict__ parent, const uint32_t* __restrict__ list,
+                                                uint32_t n, uint32_t* __restrict__ rootFlag, uint32_t* __restrict__ rootOf) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t b = list[k];
-        const uint32_t r = ufFind(parent, b);
-        if (r != b) __hip_atomic_store(parent + b, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t r = ufRootOf(parent, b);
+        rootOf[b] = r;
         rootFlag[k] = (r == b) ? 1u : 0u;
     }
 }
 
-// Root rank -> island id for every dynamic body (dynamic list is ascending, so the rank of a
-// root in the list is its rank among roots); static bodies get UINT32_MAX.
-__global__ __launch_bounds__(256) void kIslandIds(const uint32_t* __restrict__ parent, const uint32_t* __restrict__ list,
-                                                  uint32_t n, const uint32_t* __restrict__ rootScan,
+// Root rank -> island id for every dynamic body (the dynamic list is ascending, so the rank of a
+// root in the list is its rank among roots); static bodies keep UINT32_MAX.
+__global__ __launch_bounds__(256) void kIslandIds(const uint32_t* __restrict__ list, uint32_t n,
+                                                  const uint32_t* __restrict__ rootScan,
                                                   const uint32_t* __restrict__ bodyToDyn, uint32_t* __restrict__ islandId) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t b = list[k];
-        const uint32_t r = parent[b];
-        islandId[b] = rootScan[bodyToDyn[r]];
+        islandId[b] = rootScan[bodyToDyn[islandId[b]]];
     }
 }
 
@@ -483,12 +491,12 @@ hipError_t launchIslands(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL(kUfUnite, dim3(blocksFor((uint32_t)std::min<uint64_t>(w.manCap, 1u << 22))), dim3(256), 0, st,
                        w.man[w.cur], w.ctr, w.colBody, w.inv, w.ufParent);
     if (w.nDynBodies == 0) return launchFillU32(st, &w.ctr->islands, 0u, 1);
-    hipLaunchKernelGGL(kUfFlatten, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.ufParent, w.dynBodies, w.nDynBodies,
-                       w.ufRoot);
+    hipLaunchKernelGGL(kUfRoots, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.ufParent, w.dynBodies, w.nDynBodies,
+                       w.ufRoot, w.islandId);
     hipError_t e = launchScan(st, w.ufRoot, w.ufRoot, nullptr, &w.ctr->scratch[3], w.nDynBodies, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kIslandIds, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.ufParent, w.dynBodies, w.nDynBodies,
-                       w.ufRoot, w.bodyToDyn, w.islandId);
+    hipLaunchKernelGGL(kIslandIds, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.ufRoot,
+                       w.bodyToDyn, w.islandId);
     hipLaunchKernelGGL(kIslandCount, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }
