@@ -194,6 +194,13 @@ int agx_world_read_solve_order(agx_world* w, uint32_t* order, uint64_t cap, uint
 int agx_world_read_islands(agx_world* w, uint32_t* island_id, uint32_t n, uint32_t* island_count);
 int agx_world_read_stats(agx_world* w, agx_stats* out);
 
+/* GPU time per step segment, measured with HIP events on the world's stream while profiling
+ * is on. Segments: see agx_profile_segment_name. agx_world_read_profile waits for the stream,
+ * returns per-segment total milliseconds and launch counts since the last read, and resets. */
+int agx_world_set_profiling(agx_world* w, uint32_t enabled);
+int agx_world_read_profile(agx_world* w, double* ms_total, uint32_t* launches, uint32_t cap, uint32_t* segments);
+const char* agx_profile_segment_name(uint32_t segment);
+
 /* Standalone broad phase over injected AABBs (BroadPhaseCollisionDetector::findCollisions,
  * Broad/BroadPhaseCollisionDetector.cpp:109-244): aabb holds n x (min_x, min_y, max_x, max_y),
  * owner[i] the owning body, is_dynamic[i] 1 for dynamic colliders, scene[i] (may be NULL) the

@@ -108,6 +108,9 @@ _SIGS = {
     "agx_world_read_islands": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "agx_broadphase_pairs": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_set_profiling": (C.c_int, [_P, C.c_uint32]),
+    "agx_world_read_profile": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "agx_profile_segment_name": (C.c_char_p, [C.c_uint32]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -325,6 +328,17 @@ class World:
         cnt = C.c_uint32()
         _check(lib().agx_world_read_islands(self._h, _ptr(ids), len(ids), C.byref(cnt)), self._h)
         return ids, cnt.value
+
+    def set_profiling(self, enabled: bool):
+        _check(lib().agx_world_set_profiling(self._h, int(enabled)), self._h)
+
+    def profile(self) -> dict:
+        """{segment: (total ms, launches)} since the last call (HIP events on the world's stream)."""
+        ms = np.zeros(32, np.float64)
+        n = np.zeros(32, np.uint32)
+        k = C.c_uint32()
+        _check(lib().agx_world_read_profile(self._h, _ptr(ms), _ptr(n), 32, C.byref(k)), self._h)
+        return {lib().agx_profile_segment_name(i).decode(): (float(ms[i]), int(n[i])) for i in range(k.value)}
 
     def stats(self) -> Stats:
         s = Stats()

@@ -15,6 +15,16 @@ struct agx_builder {
     agx::Builder b;
 };
 
+namespace {
+enum Segment : uint32_t {
+    SEG_INTEGRATE, SEG_POSE, SEG_CCD, SEG_TREE, SEG_PAIRS, SEG_NARROW, SEG_COLOUR, SEG_SOLVE, SEG_ISLANDS,
+    SEG_FINISH, SEG_POST_UPDATE, SEG_COUNT
+};
+const char* const kSegmentNames[SEG_COUNT] = { "integrate", "pose", "ccd", "tree_build", "pair_traversal", "narrow",
+                                               "colour_prep", "solve", "islands", "finish", "post_update" };
+struct EventPair { hipEvent_t a, b; uint32_t seg; };
+}  // namespace
+
 struct agx_world {
     agx::DeviceWorld d;
     agx_settings settings{};
@@ -30,6 +40,11 @@ struct agx_world {
     bool postUpdatePending = true;   // next step's CCD sweep starts from the post-update pose
     bool inStep = false;             // phases of a step have run but FINISH has not
     int nextPhase = 0;
+    bool profiling = false;
+    std::vector<EventPair> pending;  // recorded, not yet read
+    std::vector<hipEvent_t> eventPool;
+    double segMs[SEG_COUNT] = {};
+    uint32_t segLaunches[SEG_COUNT] = {};
 };
 
 namespace {
@@ -126,6 +141,31 @@ int allocPairBuffers(agx_world* w, uint64_t pairCap) {
     return AGX_OK;
 }
 
+hipEvent_t takeEvent(agx_world* w) {
+    if (!w->eventPool.empty()) { hipEvent_t e = w->eventPool.back(); w->eventPool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one segment's launches with events when profiling is on.
+struct SegmentTimer {
+    agx_world* w;
+    EventPair p{ nullptr, nullptr, 0 };
+    SegmentTimer(agx_world* world, uint32_t seg) : w(world) {
+        if (!w->profiling) return;
+        p.seg = seg;
+        p.a = takeEvent(w);
+        p.b = takeEvent(w);
+        if (p.a) (void)hipEventRecord(p.a, w->stream);
+    }
+    ~SegmentTimer() {
+        if (!w->profiling || !p.a || !p.b) return;
+        (void)hipEventRecord(p.b, w->stream);
+        w->pending.push_back(p);
+    }
+};
+
 int beginStep(agx_world* w) {
     agx::DeviceWorld& d = w->d;
     const size_t off = offsetof(agx::Counters, pairs);
@@ -139,36 +179,58 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
     hipStream_t st = w->stream;
     const bool ccd = s.ccd_enabled != 0;
     switch (phase) {
-    case AGX_PHASE_INTEGRATE:
+    case AGX_PHASE_INTEGRATE: {
         if (int r = beginStep(w)) return r;
         w->inStep = true;
+        SegmentTimer t(w, SEG_INTEGRATE);
         HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0));
         break;
-    case AGX_PHASE_POSE:
+    }
+    case AGX_PHASE_POSE: {
+        SegmentTimer t(w, SEG_POSE);
         HIP_CHECK(w, agx::launchPose(st, d, 0, w->postUpdatePending ? 1 : 0, ccd ? 1 : 0));
         break;
+    }
     case AGX_PHASE_CCD:
-        if (ccd) HIP_CHECK(w, agx::launchCcd(st, d, s.ccd_fast_factor, s.ccd_core_fraction));
+        if (ccd && d.nStat > 0) {
+            SegmentTimer t(w, SEG_CCD);
+            HIP_CHECK(w, agx::launchCcd(st, d, s.ccd_fast_factor, s.ccd_core_fraction));
+        }
         break;
-    case AGX_PHASE_BROAD:
-        HIP_CHECK(w, agx::launchBuildTree(st, d, d.dyn));
+    case AGX_PHASE_BROAD: {
+        {
+            SegmentTimer t(w, SEG_TREE);
+            HIP_CHECK(w, agx::launchBuildTree(st, d, d.dyn));
+        }
+        SegmentTimer t(w, SEG_PAIRS);
         HIP_CHECK(w, agx::launchPairs(st, d));
         break;
-    case AGX_PHASE_NARROW:
+    }
+    case AGX_PHASE_NARROW: {
+        SegmentTimer t(w, SEG_NARROW);
         HIP_CHECK(w, agx::launchNarrow(st, d));
         break;
+    }
     case AGX_PHASE_SOLVE:
         if (!s.collide_only) {
-            HIP_CHECK(w, agx::launchColourPrep(st, d));
+            {
+                SegmentTimer t(w, SEG_COLOUR);
+                HIP_CHECK(w, agx::launchColourPrep(st, d));
+            }
+            SegmentTimer t(w, SEG_SOLVE);
             HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations, s.track_residual ? 1 : 0,
                                           s.residual_target, s.track_errors ? 1 : 0));
         }
         break;
-    case AGX_PHASE_ISLANDS:
+    case AGX_PHASE_ISLANDS: {
+        SegmentTimer t(w, SEG_ISLANDS);
         HIP_CHECK(w, agx::launchIslands(st, d));
         break;
-    case AGX_PHASE_FINISH:
+    }
+    case AGX_PHASE_FINISH: {
+        SegmentTimer t(w, SEG_FINISH);
         HIP_CHECK(w, agx::launchHashBuild(st, d));
+    }
         d.cur ^= 1;
         w->postUpdatePending = false;
         w->inStep = false;
@@ -181,6 +243,7 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
 }
 
 int postUpdate(agx_world* w) {
+    SegmentTimer t(w, SEG_POST_UPDATE);
     HIP_CHECK(w, agx::launchPose(w->stream, w->d, 0, 0, 0));
     w->postUpdatePending = true;
     return AGX_OK;
@@ -449,6 +512,8 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
 int agx_world_destroy(agx_world* w) {
     if (!w) return AGX_OK;
     if (w->stream) { (void)hipStreamSynchronize(w->stream); (void)hipStreamDestroy(w->stream); }
+    for (const EventPair& p : w->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (hipEvent_t e : w->eventPool) (void)hipEventDestroy(e);
     for (void* p : w->allocations) (void)hipFree(p);
     delete w;
     return AGX_OK;
@@ -726,6 +791,34 @@ int agx_world_read_stats(agx_world* w, agx_stats* out) {
     delete c;
     return AGX_OK;
 }
+
+int agx_world_set_profiling(agx_world* w, uint32_t enabled) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    w->profiling = enabled != 0;
+    return AGX_OK;
+}
+
+int agx_world_read_profile(agx_world* w, double* ms_total, uint32_t* launches, uint32_t cap, uint32_t* segments) {
+    if (!w || !segments) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    for (const EventPair& p : w->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) { w->segMs[p.seg] += ms; w->segLaunches[p.seg]++; }
+        w->eventPool.push_back(p.a);
+        w->eventPool.push_back(p.b);
+    }
+    w->pending.clear();
+    *segments = SEG_COUNT;
+    for (uint32_t i = 0; i < SEG_COUNT && i < cap; i++) {
+        if (ms_total) ms_total[i] = w->segMs[i];
+        if (launches) launches[i] = w->segLaunches[i];
+        w->segMs[i] = 0.0;
+        w->segLaunches[i] = 0;
+    }
+    return AGX_OK;
+}
+
+const char* agx_profile_segment_name(uint32_t segment) { return segment < SEG_COUNT ? kSegmentNames[segment] : ""; }
 
 int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t* is_dynamic, const uint32_t* scene,
                          uint32_t n, uint32_t* pairs_a, uint32_t* pairs_b, uint64_t cap, uint64_t* count) {
