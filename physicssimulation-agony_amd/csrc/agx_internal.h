@@ -10,8 +10,10 @@
 namespace agx {
 
 constexpr uint32_t WAVE = 64;
-constexpr uint32_t MAX_COLOURS = 1280;     // 128 first-fit colours + one per overflow round
-constexpr uint32_t MAX_COLOUR_ROUNDS = 1024;
+constexpr uint32_t FIRST_FIT_COLOURS = 128;  // chosen first-fit from two 64-bit masks per body
+constexpr uint32_t MAX_COLOURS = FIRST_FIT_COLOURS + 1;   // + one shared overflow colour
+constexpr uint32_t COLOUR_BLOCKS = 512;     // blocks of the colour histogram / scatter kernels
+constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataflow kernels
 constexpr uint32_t MAX_SWEEPS = 64;
 constexpr uint32_t TRAVERSAL_STACK = 64;
 constexpr uint32_t SORT_TILE = 4096;        // keys per radix-sort block (256 threads x 16)
@@ -30,18 +32,21 @@ struct Counters {
     uint32_t islands;
     uint32_t ccdHits;
     uint32_t sweeps;
-    uint32_t overflow;         // bit 0 pairs, bit 1 traversal stack, bit 2 colouring, bit 3 hash
+    uint32_t overflow;         // OVERFLOW_* bits
     uint32_t scratch[8];       // scan totals
-    uint32_t colourRounds;
-    uint32_t remaining[MAX_COLOUR_ROUNDS + 1];
-    uint32_t colourCount[MAX_COLOURS + 1];
-    uint32_t colourStart[MAX_COLOURS + 2];
-    uint32_t colourFill[MAX_COLOURS + 1];
+    uint32_t colourStart[MAX_COLOURS + 1];
     double residualNum[MAX_SWEEPS];
     double residualDen[MAX_SWEEPS];
     float velocityError, positionError;
     uint32_t barrier[4];       // grid-barrier arrival counters (one per cooperative kernel)
 };
+
+constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)
+constexpr uint32_t OVERFLOW_STACK = 2u;       // traversal stack too small (error)
+constexpr uint32_t OVERFLOW_COLOURING = 4u;   // a constraint found no free first-fit colour (still exact)
+constexpr uint32_t OVERFLOW_HASH = 8u;        // warm-start hash full (error)
+constexpr uint32_t OVERFLOW_STALL = 16u;      // a dataflow kernel hit its time limit (error)
+constexpr uint32_t OVERFLOW_ITEMS = 32u;      // more constraints than the dataflow kernels hold (error)
 
 // One BVH over a fixed collider list (dynamic or static), Karras layout:
 // internal nodes [0, n-1), leaves [n-1, 2n-1); node 0 is the root (a leaf when n == 1).
@@ -67,9 +72,10 @@ struct Tree {
     float4* sceneBounds = nullptr;   // S: (minX, minY, maxX, maxY)
 };
 
-// Constraint planes, one float4 per constraint per plane, in colour order.
+// Constraint planes, one float4 per constraint per plane, in solve order (colour-major).
 struct Constraints {
     uint4* idx;      // bodyA, bodyB, count, manifold
+    uint4* dep;      // rankA, degreeA, rankB, degreeB (degree 0: static body, never waited on)
     float4* nrm;     // nx, ny, staticFriction, dynamicFriction
     float4* inv;     // invMassA, invMassB, invInertiaA, invInertiaB
     float4* r0;      // rAn0, rBn0, rAt0, rBt0
@@ -129,12 +135,22 @@ struct DeviceWorld {
     unsigned long long* hashKeys = nullptr;
     uint32_t* hashVals = nullptr;
     uint64_t hashCapMax = 0;
-    // solver
-    uint8_t* colour = nullptr;       // low 8 bits of the colour (first-fit)
-    uint16_t* colourId = nullptr;    // full colour id per manifold
-    unsigned long long* claim = nullptr;   // per body
-    unsigned long long* used = nullptr;    // per body, 2 words (128 colours)
-    uint32_t* order = nullptr;       // constraint slot -> manifold
+    // solver: per-body incidence lists (CSR over bodies) drive a dataflow colouring and solve
+    uint2* manBodies = nullptr;      // per manifold: body of A, body of B
+    uint32_t* degree = nullptr;      // per body: manifolds touching it (dynamic bodies only)
+    uint32_t* incStart = nullptr;    // per body: start of its list in inc (scan of degree)
+    uint32_t* incFill = nullptr;     // per body: fill cursor
+    uint32_t* inc = nullptr;         // 2 * manCap entries 2m + side, grouped by body
+    uint32_t* incSorted = nullptr;   // the same lists sorted by solve position
+    uint32_t* rank = nullptr;        // per entry 2m + side: rank in its body's list
+    uint32_t* flow = nullptr;        // per body: dataflow completion counter
+    unsigned long long* used = nullptr;    // per body, 2 words: first-fit colours taken
+    uint16_t* colourId = nullptr;    // per manifold
+    uint32_t* colourHist = nullptr;  // COLOUR_BLOCKS x MAX_COLOURS block histograms, then block bases
+    uint32_t* posOf = nullptr;       // manifold -> solve position
+    uint32_t* order = nullptr;       // solve position -> manifold
+    float* blockRes = nullptr;       // per solve block: residual partial sums (2 x MAX_SWEEPS)
+    float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
     Constraints cs{};
     // scan scratch
     uint32_t* scanBlock = nullptr;
@@ -142,7 +158,8 @@ struct DeviceWorld {
     Counters* ctr = nullptr;
     // launch shape
     int numCU = 0;
-    int coopBlocks = 0;              // co-resident blocks for cooperative kernels
+    int colourBlocks = 0;            // co-resident blocks of the dataflow colouring kernel
+    int solveBlocks = 0;             // co-resident blocks of the dataflow solve kernel
 };
 
 // ---------------------------------------------------------------- device math
@@ -188,21 +205,69 @@ __host__ __device__ inline float wrapRotation(float r) {
 }
 }  // namespace dm
 
-// ---------------------------------------------------------------- grid barrier
-// Cooperative kernels only (all blocks co-resident). Every wave waits for its own stores,
-// the block synchronises, thread 0 publishes with an agent-scope release (L2 write-back)
-// and waits with an agent-scope acquire (L2 invalidate) until all blocks have arrived.
-__device__ inline void gridBarrier(uint32_t* counter, uint32_t nblocks, uint32_t& generation) {
+// ---------------------------------------------------------------- coherent access
+// Data shared between waves of one running kernel on different XCDs (each XCD has its own L2)
+// goes through agent-scope relaxed atomics, which the hardware keeps coherent without L2
+// write-backs or invalidations. A writer publishes with orderStores() (all of its earlier
+// memory operations complete) before the flag store; a reader issues its data loads only
+// after observing the flag.
+__device__ inline uint32_t loadFlag(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void storeFlag(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long loadCoherent64(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void storeCoherent64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float2 loadCoherent2(const float2* p) {
+    const unsigned long long a = loadCoherent64(reinterpret_cast<const unsigned long long*>(p));
+    return make_float2(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)));
+}
+__device__ inline void storeCoherent2(float2* p, float2 v) {
+    storeCoherent64(reinterpret_cast<unsigned long long*>(p),
+                    ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x));
+}
+__device__ inline float4 loadCoherent(const float4* p) {
+    const float2* q = reinterpret_cast<const float2*>(p);
+    const float2 a = loadCoherent2(q), b = loadCoherent2(q + 1);
+    return make_float4(a.x, a.y, b.x, b.y);
+}
+__device__ inline void storeCoherent(float4* p, float4 v) {
+    float2* q = reinterpret_cast<float2*>(p);
+    storeCoherent2(q, make_float2(v.x, v.y));
+    storeCoherent2(q + 1, make_float2(v.z, v.w));
+}
+__device__ inline void orderStores() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// ---------------------------------------------------------------- grid barrier
+// Cooperative kernels only (all blocks co-resident). Every wave waits for its own memory
+// operations, the block synchronises, thread 0 arrives on the counter and polls it. Only data
+// written with coherent stores is visible to other blocks afterwards. Gives up (returns false)
+// after the deadline (wall clock, 100 MHz), so a broken invariant cannot hang the device.
+__device__ inline bool gridBarrier(uint32_t* counter, uint32_t nblocks, uint32_t& generation, uint64_t deadline) {
+    orderStores();
     __syncthreads();
+    __shared__ uint32_t ok;
     if (threadIdx.x == 0) {
+        ok = 1u;
         generation++;
         const uint32_t target = generation * nblocks;
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (loadFlag(counter) < target) {
+            if (wall_clock64() > deadline) { ok = 0u; break; }
             __builtin_amdgcn_s_sleep(1);
+        }
     }
     __syncthreads();
+    return ok != 0u;
 }
 
 // ---------------------------------------------------------------- host-side launchers
@@ -229,7 +294,8 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                        float residualTarget, int trackErrors);
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
-// Co-resident blocks per CU for the cooperative solver kernels at 256 threads.
-int cooperativeBlocksPerCU();
+constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
+// Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels.
+void dataflowBlocksPerCU(int& colour, int& solve);
 
 }  // namespace agx

@@ -207,19 +207,6 @@ __global__ __launch_bounds__(256) void kLeaves(const uint32_t* __restrict__ vals
     if (n == 1) parent[0] = 0xFFFFFFFFu;
 }
 
-__device__ inline void storeCoherent(float4* p, float4 v) {
-    uint64_t* q = reinterpret_cast<uint64_t*>(p);
-    __hip_atomic_store(q, ((uint64_t)__float_as_uint(v.y) << 32) | __float_as_uint(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, ((uint64_t)__float_as_uint(v.w) << 32) | __float_as_uint(v.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline float4 loadCoherent(const float4* p) {
-    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
-    const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)),
-                       __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
-}
-
 // Bottom-up refit (BvhTree.cpp:426-530 computes the same unions top-down in reverse order):
 // the second thread to reach a node merges its children. Node boxes written here use
 // agent-coherent stores, completed before the arrival counter is bumped.
@@ -231,7 +218,7 @@ __global__ __launch_bounds__(256) void kRefit(uint32_t n, const uint2* __restric
     while (true) {
         const uint32_t p = parent[node];
         if (p == 0xFFFFFFFFu) break;
-        __builtin_amdgcn_s_waitcnt(0);
+        orderStores();
         const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0) break;
         const uint2 ch = child[p];

@@ -1,17 +1,32 @@
 // Contact solver (Solving/BodyCollisionSolver.cpp) and islands (IslandManager.cpp).
 //
-// Colouring replaces SolvingPlanner's waves (Solving/SolvingPlanner.cpp:141-310): constraints
-// of one colour share no dynamic body (static bodies are never written), so a colour is solved
-// in parallel with the same result as any sequential order of it; colours run in sequence.
+// The reference solves its manifolds one after another (Gauss-Seidel) in the order of its planner
+// waves (Solving/SolvingPlanner.cpp:141-310). Here the order is colour-major: constraints of one
+// colour share no dynamic body, so every order inside a colour gives the same bits. The device
+// runs that sequential order as a dataflow instead of colour by colour behind grid barriers:
+// every dynamic body counts the constraints finished on it, and a constraint runs as soon as the
+// counters of both its bodies show that every constraint before it (by sweep, then by solve
+// position) touching them is done. The result equals a sequential sweep in the exported order
+// exactly, and later colours and sweeps start as soon as their own inputs are ready.
+//
+// The colouring is a dataflow too: greedy first fit in a fixed pseudo-random priority order
+// (each manifold takes the lowest colour not taken at its bodies by manifolds of higher
+// priority). It is deterministic and its dependency chains are short.
 #include "agx_internal.h"
 
 namespace agx {
 
 namespace {
 
-constexpr uint16_t UNCOLOURED = 0xFFFFu;
-constexpr uint32_t FIRST_FIT = 128;   // colours chosen first-fit from two 64-bit masks per body
+constexpr uint64_t STALL_TICKS = 200000000ull;   // 2 s of the 100 MHz wall clock
+constexpr uint32_t FINISH_BLOCKS = SOLVE_FINISH_BLOCKS;
 
+inline uint32_t blocksFor(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 8192u)); }
+inline uint32_t blocksForManifolds(const DeviceWorld& w) {
+    return blocksFor((uint32_t)std::min<uint64_t>(w.manCap, 1u << 22));
+}
+
+// Bijective on 32 bits, so priorities of distinct manifolds never tie.
 __device__ inline uint32_t mix32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
     return x;
@@ -19,97 +34,240 @@ __device__ inline uint32_t mix32(uint32_t x) {
 
 __device__ inline bool isDynamic(const float2* inv, uint32_t b) { return inv[b].x != 0.0f; }
 
-// ---------------------------------------------------------------- colouring + constraint prep
-// Round r: every uncoloured manifold claims its dynamic bodies with a priority stamped with r
-// (atomic max), so claims of earlier rounds always lose. A manifold holding all its claims is
-// alone at its bodies this round and takes the lowest colour free at both (first fit).
-__global__ __launch_bounds__(256) void kColourPrep(const agx_manifold* __restrict__ man, Counters* __restrict__ ctr,
-                                                   const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
-                                                   const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                   const uint4* __restrict__ shape, const float4* __restrict__ materials,
-                                                   uint16_t* __restrict__ colourId, unsigned long long* __restrict__ claim,
-                                                   unsigned long long* __restrict__ used, uint32_t* __restrict__ order,
-                                                   Constraints cs) {
-    uint32_t gen = 0;
-    uint32_t* bar = &ctr->barrier[0];
-    const uint32_t M = ctr->manifolds;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t m = tid; m < M; m += stride) {
-        colourId[m] = UNCOLOURED;
-        const uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
-        if (isDynamic(inv, a)) { claim[a] = 0; used[2 * a] = 0; used[2 * a + 1] = 0; }
-        if (isDynamic(inv, b)) { claim[b] = 0; used[2 * b] = 0; used[2 * b + 1] = 0; }
+// ---------------------------------------------------------------- incidence lists
+__global__ __launch_bounds__(256) void kFlowClear(uint32_t n, uint32_t* __restrict__ degree, uint32_t* __restrict__ fill,
+                                                  uint32_t* __restrict__ flow, unsigned long long* __restrict__ used) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        degree[i] = 0; fill[i] = 0; flow[i] = 0;
+        used[2 * i] = 0; used[2 * i + 1] = 0;
     }
-    gridBarrier(bar, gridDim.x, gen);
-    uint32_t rounds = 0;
-    for (uint32_t r = 0; r < MAX_COLOUR_ROUNDS; r++) {
-        rounds = r + 1;
-        for (uint32_t m = tid; m < M; m += stride) {
-            if (colourId[m] != UNCOLOURED) continue;
-            const unsigned long long prio = ((unsigned long long)(r + 1) << 48) |
-                                            ((unsigned long long)(mix32(m) & 0xFFFFu) << 32) | m;
-            const uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
-            if (isDynamic(inv, a)) atomicMax(claim + a, prio);
-            if (isDynamic(inv, b)) atomicMax(claim + b, prio);
+}
+
+__global__ __launch_bounds__(256) void kDegree(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
+                                               const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
+                                               uint2* __restrict__ manBodies, uint32_t* __restrict__ degree) {
+    const uint32_t M = ctr->manifolds;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
+        manBodies[m] = make_uint2(a, b);
+        if (isDynamic(inv, a)) atomicAdd(degree + a, 1u);
+        if (isDynamic(inv, b)) atomicAdd(degree + b, 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void kIncident(const Counters* __restrict__ ctr, const uint2* __restrict__ manBodies,
+                                                 const float2* __restrict__ inv, const uint32_t* __restrict__ incStart,
+                                                 uint32_t* __restrict__ fill, uint32_t* __restrict__ inc) {
+    const uint32_t M = ctr->manifolds;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint2 mb = manBodies[m];
+        if (isDynamic(inv, mb.x)) inc[incStart[mb.x] + atomicAdd(fill + mb.x, 1u)] = 2 * m;
+        if (isDynamic(inv, mb.y)) inc[incStart[mb.y] + atomicAdd(fill + mb.y, 1u)] = 2 * m + 1;
+    }
+}
+
+// Rank of every list entry among its body's entries by manifold priority.
+__global__ __launch_bounds__(256) void kRankByPriority(const uint32_t* __restrict__ dynBodies, uint32_t nDyn,
+                                                       const uint32_t* __restrict__ degree,
+                                                       const uint32_t* __restrict__ incStart,
+                                                       const uint32_t* __restrict__ inc, uint32_t* __restrict__ rank) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nDyn; k += gridDim.x * blockDim.x) {
+        const uint32_t b = dynBodies[k], s = incStart[b], d = degree[b];
+        for (uint32_t i = 0; i < d; i++) {
+            const uint32_t e = inc[s + i];
+            const uint32_t pe = mix32(e >> 1);
+            uint32_t r = 0;
+            for (uint32_t j = 0; j < d; j++) r += mix32(inc[s + j] >> 1) < pe ? 1u : 0u;
+            rank[e] = r;
         }
-        gridBarrier(bar, gridDim.x, gen);
-        uint32_t left = 0;
-        for (uint32_t m = tid; m < M; m += stride) {
-            if (colourId[m] != UNCOLOURED) continue;
-            const unsigned long long prio = ((unsigned long long)(r + 1) << 48) |
-                                            ((unsigned long long)(mix32(m) & 0xFFFFu) << 32) | m;
-            const uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
-            const bool da = isDynamic(inv, a), db = isDynamic(inv, b);
-            const bool win = (!da || claim[a] == prio) && (!db || claim[b] == prio);
-            if (!win) { left++; continue; }
-            const unsigned long long u0 = (da ? used[2 * a] : 0ull) | (db ? used[2 * b] : 0ull);
-            const unsigned long long u1 = (da ? used[2 * a + 1] : 0ull) | (db ? used[2 * b + 1] : 0ull);
-            uint32_t c;
-            if (~u0) c = (uint32_t)__ffsll((unsigned long long)~u0) - 1;
-            else if (~u1) c = 64 + (uint32_t)__ffsll((unsigned long long)~u1) - 1;
-            else c = FIRST_FIT + r;   // all first-fit colours taken: this round's winners are independent
-            colourId[m] = (uint16_t)c;
-            if (c < 64) {
-                const unsigned long long bit = 1ull << c;
-                if (da) used[2 * a] |= bit;
-                if (db) used[2 * b] |= bit;
-            } else if (c < FIRST_FIT) {
-                const unsigned long long bit = 1ull << (c - 64);
-                if (da) used[2 * a + 1] |= bit;
-                if (db) used[2 * b + 1] |= bit;
+    }
+}
+
+// ---------------------------------------------------------------- dataflow colouring
+// Lane t holds manifolds t, t + T, ... (T = grid threads) and takes them in priority order, so
+// the manifold of lowest priority still open anywhere is always the next one of its lane: with
+// every block resident the flow cannot deadlock. A manifold is ready when both of its bodies'
+// counters equal its rank there; it then reads their colour masks, takes the first free colour,
+// publishes the masks and advances the counters.
+__global__ __launch_bounds__(256) void kColourFlow(Counters* __restrict__ ctr, const uint2* __restrict__ manBodies,
+                                                   const float2* __restrict__ inv, const uint32_t* __restrict__ rank,
+                                                   uint32_t* flow, unsigned long long* used,
+                                                   uint16_t* __restrict__ colourId) {
+    const uint32_t M = ctr->manifolds;
+    const uint32_t T = gridDim.x * blockDim.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((uint64_t)M > (uint64_t)MAX_FLOW_ITEMS * T) {
+        if (tid == 0) atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
+        return;
+    }
+    unsigned long long pending = 0;
+    for (uint32_t k = 0; k < MAX_FLOW_ITEMS && (uint64_t)tid + (uint64_t)k * T < M; k++) pending |= 1ull << k;
+    auto pick = [&]() -> uint32_t {
+        uint32_t best = 0xFFFFFFFFu, bestK = MAX_FLOW_ITEMS;
+        for (unsigned long long q = pending; q; q &= q - 1) {
+            const uint32_t k = (uint32_t)__ffsll(q) - 1;
+            const uint32_t pr = mix32(tid + k * T);
+            if (pr < best) { best = pr; bestK = k; }
+        }
+        return bestK;
+    };
+    uint32_t k = pick();
+    uint32_t m = 0, a = 0, b = 0, ra = 0, rb = 0;
+    bool da = false, db = false;
+    auto load = [&]() {
+        m = tid + k * T;
+        const uint2 mb = manBodies[m];
+        a = mb.x; b = mb.y;
+        da = isDynamic(inv, a); db = isDynamic(inv, b);
+        ra = da ? rank[2 * m] : 0u;
+        rb = db ? rank[2 * m + 1] : 0u;
+    };
+    if (k < MAX_FLOW_ITEMS) load();
+    const uint64_t deadline = wall_clock64() + STALL_TICKS;
+    while (true) {
+        bool progressed = false;
+        if (k < MAX_FLOW_ITEMS) {
+            const uint32_t fa = da ? loadFlag(flow + a) : ra;
+            const uint32_t fb = db ? loadFlag(flow + b) : rb;
+            if (fa == ra && fb == rb) {
+                unsigned long long ua0 = 0, ua1 = 0, ub0 = 0, ub1 = 0;
+                if (da) { ua0 = loadCoherent64(used + 2 * a); ua1 = loadCoherent64(used + 2 * a + 1); }
+                if (db) { ub0 = loadCoherent64(used + 2 * b); ub1 = loadCoherent64(used + 2 * b + 1); }
+                const unsigned long long f0 = ~(ua0 | ub0), f1 = ~(ua1 | ub1);
+                uint32_t c;
+                if (f0) c = (uint32_t)__ffsll(f0) - 1;
+                else if (f1) c = 64 + (uint32_t)__ffsll(f1) - 1;
+                else c = FIRST_FIT_COLOURS;
+                colourId[m] = (uint16_t)c;
+                if (c < 64) {
+                    const unsigned long long bit = 1ull << c;
+                    if (da) storeCoherent64(used + 2 * a, ua0 | bit);
+                    if (db) storeCoherent64(used + 2 * b, ub0 | bit);
+                } else if (c < FIRST_FIT_COLOURS) {
+                    const unsigned long long bit = 1ull << (c - 64);
+                    if (da) storeCoherent64(used + 2 * a + 1, ua1 | bit);
+                    if (db) storeCoherent64(used + 2 * b + 1, ub1 | bit);
+                } else {
+                    atomicOr(&ctr->overflow, OVERFLOW_COLOURING);
+                }
+                orderStores();
+                if (da) storeFlag(flow + a, ra + 1);
+                if (db) storeFlag(flow + b, rb + 1);
+                pending &= ~(1ull << k);
+                k = pick();
+                if (k < MAX_FLOW_ITEMS) load();
+                progressed = true;
             }
         }
-        for (uint32_t off = 32; off > 0; off >>= 1) left += __shfl_down(left, off, 64);
-        if ((threadIdx.x & 63u) == 0 && left) atomicAdd(&ctr->remaining[r], left);
-        gridBarrier(bar, gridDim.x, gen);
-        if (__hip_atomic_load(&ctr->remaining[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        if (r + 1 == MAX_COLOUR_ROUNDS && tid == 0) atomicOr(&ctr->overflow, 4u);
+        if (__all(k >= MAX_FLOW_ITEMS)) break;
+        if (!__any(progressed)) {
+            if (wall_clock64() > deadline) { atomicOr(&ctr->overflow, OVERFLOW_STALL); break; }
+            __builtin_amdgcn_s_sleep(1);
+        }
     }
-    // Colour histogram, scan, scatter into colour order with constraint prep.
-    for (uint32_t m = tid; m < M; m += stride) {
-        const uint16_t c = colourId[m];
-        if (c == UNCOLOURED) continue;
-        atomicAdd(&ctr->colourCount[c], 1u);
-        atomicMax(&ctr->colours, (uint32_t)c + 1u);
-    }
-    gridBarrier(bar, gridDim.x, gen);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint32_t K = ctr->colours;
+}
+
+// ---------------------------------------------------------------- solve order
+// Block histograms over contiguous manifold chunks, one column scan, then a scatter that
+// reserves each block's slots from its scanned base.
+__device__ inline void chunkOf(uint32_t M, uint32_t& lo, uint32_t& hi) {
+    const uint32_t chunk = (M + gridDim.x - 1) / gridDim.x;
+    lo = blockIdx.x * chunk;
+    hi = min(M, lo + chunk);
+}
+
+__global__ __launch_bounds__(256) void kColourHist(const Counters* __restrict__ ctr, const uint16_t* __restrict__ colourId,
+                                                   uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[MAX_COLOURS];
+    for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    uint32_t lo, hi;
+    chunkOf(ctr->manifolds, lo, hi);
+    for (uint32_t m = lo + threadIdx.x; m < hi; m += blockDim.x) atomicAdd(&h[colourId[m]], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x) hist[blockIdx.x * MAX_COLOURS + i] = h[i];
+}
+
+// One block: turns the block histograms into each block's first slot per colour.
+__global__ __launch_bounds__(256) void kColourScan(Counters* __restrict__ ctr, uint32_t* __restrict__ hist, uint32_t blocks) {
+    __shared__ uint32_t total[MAX_COLOURS];
+    __shared__ uint32_t start[MAX_COLOURS];
+    const uint32_t c = threadIdx.x;
+    if (c < MAX_COLOURS) {
         uint32_t run = 0;
-        for (uint32_t k = 0; k < K; k++) { ctr->colourStart[k] = run; run += ctr->colourCount[k]; }
-        ctr->colourStart[K] = run;
-        ctr->colourRounds = rounds;
+        for (uint32_t j = 0; j < blocks; j++) {
+            const uint32_t v = hist[j * MAX_COLOURS + c];
+            hist[j * MAX_COLOURS + c] = run;
+            run += v;
+        }
+        total[c] = run;
     }
-    gridBarrier(bar, gridDim.x, gen);
-    // computeConstraintData (BodyCollisionSolver.cpp:229-390).
-    for (uint32_t m = tid; m < M; m += stride) {
-        const uint16_t col = colourId[m];
-        if (col == UNCOLOURED) continue;
-        const uint32_t slot = ctr->colourStart[col] + atomicAdd(&ctr->colourFill[col], 1u);
-        order[slot] = m;
+    __syncthreads();
+    if (c == 0) {
+        uint32_t s = 0, K = 0;
+        for (uint32_t i = 0; i < MAX_COLOURS; i++) {
+            start[i] = s;
+            ctr->colourStart[i] = s;
+            s += total[i];
+            if (total[i]) K = i + 1;
+        }
+        ctr->colourStart[MAX_COLOURS] = s;
+        ctr->colours = K;
+    }
+    __syncthreads();
+    if (c < MAX_COLOURS)
+        for (uint32_t j = 0; j < blocks; j++) hist[j * MAX_COLOURS + c] += start[c];
+}
+
+__global__ __launch_bounds__(256) void kColourScatter(const Counters* __restrict__ ctr, const uint16_t* __restrict__ colourId,
+                                                      const uint32_t* __restrict__ hist, uint32_t* __restrict__ order,
+                                                      uint32_t* __restrict__ posOf) {
+    __shared__ uint32_t cursor[MAX_COLOURS];
+    for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x) cursor[i] = hist[blockIdx.x * MAX_COLOURS + i];
+    __syncthreads();
+    uint32_t lo, hi;
+    chunkOf(ctr->manifolds, lo, hi);
+    for (uint32_t m = lo + threadIdx.x; m < hi; m += blockDim.x) {
+        const uint32_t p = atomicAdd(&cursor[colourId[m]], 1u);
+        order[p] = m;
+        posOf[m] = p;
+    }
+}
+
+// Rank of every list entry among its body's entries by solve position, the lists sorted by
+// position, and the counters reset for the solve.
+__global__ __launch_bounds__(256) void kSolveRank(const uint32_t* __restrict__ dynBodies, uint32_t nDyn,
+                                                  const uint32_t* __restrict__ degree, const uint32_t* __restrict__ incStart,
+                                                  const uint32_t* __restrict__ inc, const uint32_t* __restrict__ posOf,
+                                                  uint32_t* __restrict__ rank, uint32_t* __restrict__ incSorted,
+                                                  uint32_t* __restrict__ flow) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nDyn; k += gridDim.x * blockDim.x) {
+        const uint32_t b = dynBodies[k], s = incStart[b], d = degree[b];
+        for (uint32_t i = 0; i < d; i++) {
+            const uint32_t e = inc[s + i];
+            const uint32_t pe = posOf[e >> 1];
+            uint32_t r = 0;
+            for (uint32_t j = 0; j < d; j++) r += posOf[inc[s + j] >> 1] < pe ? 1u : 0u;
+            rank[e] = r;
+            incSorted[s + r] = e;
+        }
+        flow[b] = 0;
+    }
+}
+
+// computeConstraintData (BodyCollisionSolver.cpp:229-390), written at the solve position.
+__global__ __launch_bounds__(256) void kPrep(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
+                                             const uint2* __restrict__ manBodies, const uint32_t* __restrict__ posOf,
+                                             const uint32_t* __restrict__ degree, const uint32_t* __restrict__ rank,
+                                             const float2* __restrict__ inv, const float4* __restrict__ pos,
+                                             const float4* __restrict__ vel, const uint4* __restrict__ shape,
+                                             const float4* __restrict__ materials, Constraints cs) {
+    const uint32_t M = ctr->manifolds;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint32_t slot = posOf[m];
         const agx_manifold d = man[m];
-        const uint32_t bA = colBody[d.a], bB = colBody[d.b];
+        const uint2 mb = manBodies[m];
+        const uint32_t bA = mb.x, bB = mb.y;
         const float4 pA = pos[bA], pB = pos[bB];
         const float2 iA = inv[bA], iB = inv[bB];
         const float nx = d.normal_x, ny = d.normal_y;
@@ -136,7 +294,10 @@ __global__ __launch_bounds__(256) void kColourPrep(const agx_manifold* __restric
             pr[i][5] = td > 0.0f ? 1.0f / td : 0.0f;
             pr[i][6] = negVn > 0.0f ? e * negVn : 0.0f;
         }
+        const bool dA = iA.x != 0.0f, dB = iB.x != 0.0f;
         cs.idx[slot] = make_uint4(bA, bB, d.count, m);
+        cs.dep[slot] = make_uint4(dA ? rank[2 * m] : 0u, dA ? degree[bA] : 0u, dB ? rank[2 * m + 1] : 0u,
+                                  dB ? degree[bB] : 0u);
         cs.nrm[slot] = make_float4(nx, ny, sf, df);
         cs.inv[slot] = make_float4(iA.x, iB.x, iA.y, iB.y);
         cs.r0[slot] = make_float4(pr[0][0], pr[0][1], pr[0][2], pr[0][3]);
@@ -147,36 +308,58 @@ __global__ __launch_bounds__(256) void kColourPrep(const agx_manifold* __restric
     }
 }
 
-// ---------------------------------------------------------------- sweeps
-struct Vel { float vx, vy, w; };
+// applyWarmStarting (BodyCollisionSolver.cpp:416-490). A warm-start impulse depends on the
+// constraint alone, so each body adds its constraints' impulses in solve order: the same
+// operations, in the same order per body, as a sequential pass.
+__global__ __launch_bounds__(256) void kWarm(const uint32_t* __restrict__ dynBodies, uint32_t nDyn,
+                                             const uint32_t* __restrict__ degree, const uint32_t* __restrict__ incStart,
+                                             const uint32_t* __restrict__ incSorted, const uint32_t* __restrict__ posOf,
+                                             Constraints cs, float4* __restrict__ vel) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nDyn; k += gridDim.x * blockDim.x) {
+        const uint32_t b = dynBodies[k], s = incStart[b], d = degree[b];
+        if (d == 0) continue;
+        float4 v = vel[b];
+        for (uint32_t r = 0; r < d; r++) {
+            const uint32_t e = incSorted[s + r];
+            const uint32_t c = posOf[e >> 1];
+            const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], r1 = cs.r1[c], j = cs.imp[c];
+            const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
+            const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
+            const float sn = jn0 + jn1, st = jt0 + jt1;
+            const float sx = sn * nx + st * tx, sy = sn * ny + st * ty;
+            if ((e & 1u) == 0) {
+                v.x = v.x - sx * iv.x; v.y = v.y - sy * iv.x;
+                v.z -= (jn0 * r0.x + jt0 * r0.z + jn1 * r1.x + jt1 * r1.z) * iv.z;
+            } else {
+                v.x = v.x + sx * iv.y; v.y = v.y + sy * iv.y;
+                v.z += (jn0 * r0.y + jt0 * r0.w + jn1 * r1.y + jt1 * r1.w) * iv.w;
+            }
+        }
+        vel[b] = v;
+    }
+}
 
-// applyWarmStarting (BodyCollisionSolver.cpp:416-490).
-__device__ inline void warmOne(const Constraints& cs, uint32_t c, float4* __restrict__ vel) {
-    const uint4 id = cs.idx[c];
-    const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], r1 = cs.r1[c], j = cs.imp[c];
-    float4 a = vel[id.x], b = vel[id.y];
-    const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
-    const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
-    const float sn = jn0 + jn1, st = jt0 + jt1;
-    const float sx = sn * nx + st * tx, sy = sn * ny + st * ty;
-    a.x = a.x - sx * iv.x; a.y = a.y - sy * iv.x;
-    b.x = b.x + sx * iv.y; b.y = b.y + sy * iv.y;
-    a.z -= (jn0 * r0.x + jt0 * r0.z + jn1 * r1.x + jt1 * r1.z) * iv.z;
-    b.z += (jn0 * r0.y + jt0 * r0.w + jn1 * r1.y + jt1 * r1.w) * iv.w;
-    if (iv.x > 0.0f) vel[id.x] = a;
-    if (iv.y > 0.0f) vel[id.y] = b;
+// ---------------------------------------------------------------- dataflow sweeps
+struct Item {
+    uint4 id;     // bodyA, bodyB, count, manifold
+    uint4 dep;    // rankA, degreeA, rankB, degreeB
+    float4 nrm, iv, r0, r1, k0, k1, imp;
+};
+
+__device__ inline void loadItem(const Constraints& cs, uint32_t c, Item& it) {
+    it.id = cs.idx[c]; it.dep = cs.dep[c];
+    it.nrm = cs.nrm[c]; it.iv = cs.inv[c]; it.r0 = cs.r0[c]; it.r1 = cs.r1[c];
+    it.k0 = cs.k0[c]; it.k1 = cs.k1[c]; it.imp = cs.imp[c];
 }
 
 // solveVelocityConstraints (BodyCollisionSolver.cpp:492-928): one point = clamped normal step
 // (skipped while separating with zero impulse) + iterative friction; two points = 2x2 block LCP
-// (4 cases, iterative fallback) + iterative friction per point. Returns sum of squared
-// accumulator changes and of squared accumulators through dsq/lsq.
-__device__ inline void velocityOne(const Constraints& cs, uint32_t c, float4* __restrict__ vel, float& dsq, float& lsq) {
-    const uint4 id = cs.idx[c];
-    const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], k0 = cs.k0[c];
-    float4 j = cs.imp[c];
+// (4 cases, iterative fallback) + iterative friction per point. Adds the squared accumulator
+// changes and the squared accumulators to dsq and lsq.
+__device__ inline void velocityStep(Item& it, float4& A, float4& B, float& dsq, float& lsq) {
+    float4 j = it.imp;
     const float4 j0 = j;
-    float4 A = vel[id.x], B = vel[id.y];
+    const float4 nr = it.nrm, iv = it.iv, r0 = it.r0, k0 = it.k0;
     const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
     const float sF = nr.z, dF = nr.w;
     const float iMA = iv.x, iMB = iv.y, iIA = iv.z, iIB = iv.w;
@@ -192,25 +375,27 @@ __device__ inline void velocityOne(const Constraints& cs, uint32_t c, float4* __
         A.x = A.x - ix * iMA; A.y = A.y - iy * iMA; A.z -= dj * rAt * iIA;
         B.x = B.x + ix * iMB; B.y = B.y + iy * iMB; B.z += dj * rBt * iIB;
     };
-    if (id.z == 1) {
-        const float rAn = r0.x, rBn = r0.y;
-        float jnSingle = 0.0f;
+    auto normal = [&](float rAn, float rBn, float bias, float nMass, float& acc) {
         const float vn = ((B.x - A.x) * nx + (B.y - A.y) * ny) + rBn * B.z - rAn * A.z;
+        const float jn = (bias - vn) * nMass;
+        const float old = acc;
+        acc = fmaxf(0.0f, old + jn);
+        const float dj = acc - old;
+        const float ix = dj * nx, iy = dj * ny;
+        A.x = A.x - ix * iMA; A.y = A.y - iy * iMA; A.z -= dj * rAn * iIA;
+        B.x = B.x + ix * iMB; B.y = B.y + iy * iMB; B.z += dj * rBn * iIB;
+    };
+    if (it.id.z == 1) {
+        float jnSingle = 0.0f;
+        const float vn = ((B.x - A.x) * nx + (B.y - A.y) * ny) + r0.y * B.z - r0.x * A.z;
         if (!(vn > 0.0f && j.x == 0.0f)) {
-            const float jn = (k0.z - vn) * k0.x;
-            const float old = j.x;
-            j.x = fmaxf(0.0f, old + jn);
-            const float dj = j.x - old;
+            normal(r0.x, r0.y, k0.z, k0.x, j.x);
             jnSingle = j.x;
-            const float ix = dj * nx, iy = dj * ny;
-            A.x = A.x - ix * iMA; A.y = A.y - iy * iMA; A.z -= dj * rAn * iIA;
-            B.x = B.x + ix * iMB; B.y = B.y + iy * iMB; B.z += dj * rBn * iIB;
         }
         friction(r0.z, r0.w, k0.y, j.y, jnSingle);
     } else {
-        const float4 r1 = cs.r1[c], k1 = cs.k1[c];
+        const float4 r1 = it.r1, k1 = it.k1;
         const float effMass = iMA + iMB;
-        float jnArr0, jnArr1;
         const float dvn = (B.x - A.x) * nx + (B.y - A.y) * ny;
         const float vn0 = dvn + r0.y * B.z - r0.x * A.z;
         const float vn1 = dvn + r1.y * B.z - r1.x * A.z;
@@ -241,30 +426,13 @@ __device__ inline void velocityOne(const Constraints& cs, uint32_t c, float4* __
             }
             if (!ok && b0 <= EPS && b1 <= EPS) { x0 = 0.0f; x1 = 0.0f; ok = true; }
         }
+        float jnArr0, jnArr1;
         if (!ok) {
-            // Iterative normal fallback, point 0 then point 1.
-            {
-                const float vn = ((B.x - A.x) * nx + (B.y - A.y) * ny) + r0.y * B.z - r0.x * A.z;
-                const float jn = (k0.z - vn) * k0.x;
-                const float old = j.x;
-                j.x = fmaxf(0.0f, old + jn);
-                const float dj = j.x - old;
-                jnArr0 = j.x;
-                const float ix = dj * nx, iy = dj * ny;
-                A.x = A.x - ix * iMA; A.y = A.y - iy * iMA; A.z -= dj * r0.x * iIA;
-                B.x = B.x + ix * iMB; B.y = B.y + iy * iMB; B.z += dj * r0.y * iIB;
-            }
-            {
-                const float vn = ((B.x - A.x) * nx + (B.y - A.y) * ny) + r1.y * B.z - r1.x * A.z;
-                const float jn = (k1.y - vn) * k0.w;
-                const float old = j.z;
-                j.z = fmaxf(0.0f, old + jn);
-                const float dj = j.z - old;
-                jnArr1 = j.z;
-                const float ix = dj * nx, iy = dj * ny;
-                A.x = A.x - ix * iMA; A.y = A.y - iy * iMA; A.z -= dj * r1.x * iIA;
-                B.x = B.x + ix * iMB; B.y = B.y + iy * iMB; B.z += dj * r1.y * iIB;
-            }
+            // iterative normal fallback, point 0 then point 1
+            normal(r0.x, r0.y, k0.z, k0.x, j.x);
+            jnArr0 = j.x;
+            normal(r1.x, r1.y, k1.y, k0.w, j.z);
+            jnArr1 = j.z;
         } else {
             const float d0 = x0 - j.x, d1 = x1 - j.z;
             j.x = x0; j.z = x1;
@@ -279,95 +447,162 @@ __device__ inline void velocityOne(const Constraints& cs, uint32_t c, float4* __
         friction(r0.z, r0.w, k0.y, j.y, jnArr0);
         friction(r1.z, r1.w, k1.x, j.w, jnArr1);
     }
-    cs.imp[c] = j;
-    if (iMA > 0.0f) vel[id.x] = A;
-    if (iMB > 0.0f) vel[id.y] = B;
+    it.imp = j;
     const float e0 = j.x - j0.x, e1 = j.y - j0.y;
     dsq += e0 * e0 + e1 * e1;
     lsq += j.x * j.x + j.y * j.y;
-    if (id.z > 1) {
+    if (it.id.z > 1) {
         const float e2 = j.z - j0.z, e3 = j.w - j0.w;
         dsq += e2 * e2 + e3 * e3;
         lsq += j.z * j.z + j.w * j.w;
     }
 }
 
-// solvePositionConstraints (BodyCollisionSolver.cpp:930-980): linear only.
-__device__ inline void positionOne(const Constraints& cs, uint32_t c, float4* __restrict__ pos,
-                                   const float2* __restrict__ inv) {
-    const uint4 id = cs.idx[c];
-    const float4 nr = cs.nrm[c];
-    const float penBase = cs.k1[c].z;
-    float4 A = pos[id.x], B = pos[id.y];
-    const float pen = penBase - ((B.x - A.x) * nr.x + (B.y - A.y) * nr.y);
-    if (pen <= 0.0f) return;
-    const float iMA = inv[id.x].x, iMB = inv[id.y].x;
+// solvePositionConstraints (BodyCollisionSolver.cpp:930-980): linear only. Returns false when
+// the constraint is not penetrating (nothing moves).
+__device__ inline bool positionStep(const Item& it, float2& A, float2& B) {
+    const float pen = it.k1.z - ((B.x - A.x) * it.nrm.x + (B.y - A.y) * it.nrm.y);
+    if (pen <= 0.0f) return false;
+    const float iMA = it.iv.x, iMB = it.iv.y;
     const float total = pen / (iMA + iMB) * 0.5f;
     const float cA = iMA * total, cB = iMB * total;
-    if (iMA > 0.0f) { A.x -= nr.x * cA; A.y -= nr.y * cA; pos[id.x] = A; }
-    if (iMB > 0.0f) { B.x += nr.x * cB; B.y += nr.y * cB; pos[id.y] = B; }
+    A.x -= it.nrm.x * cA; A.y -= it.nrm.y * cA;
+    B.x += it.nrm.x * cB; B.y += it.nrm.y * cB;
+    return true;
 }
 
-__device__ inline void blockAddDouble(double* dst, float v) {
-    for (uint32_t off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63u) == 0 && v != 0.0f) atomicAdd(dst, (double)v);
+__device__ inline void storeCoherentF(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float loadCoherentF(const float* p) {
+    return __uint_as_float(loadFlag(reinterpret_cast<const uint32_t*>(p)));
 }
 
-// Persistent solve: warm start, velocity sweeps, position sweeps; colours in sequence with a
-// grid barrier after each; finally scatter the accumulators back to the manifolds.
-__global__ __launch_bounds__(256) void kSolve(Counters* __restrict__ ctr, Constraints cs, float4* __restrict__ vel,
-                                              float4* __restrict__ pos, const float2* __restrict__ inv,
-                                              const uint32_t* __restrict__ order, agx_manifold* __restrict__ man,
-                                              uint32_t vIt, uint32_t pIt, int trackResidual, float residualTarget,
-                                              int trackErrors) {
-    uint32_t gen = 0;
-    uint32_t* bar = &ctr->barrier[1];
-    const uint32_t K = ctr->colours;
+// Persistent dataflow over the passes: velocity sweeps [0, vEnd), then position sweeps. Lane t
+// holds solve positions t, t + T, ... and takes them in (pass, position) order, which is the
+// sequential order restricted to the lane, so the lowest open (pass, position) anywhere is always
+// its lane's next one and the flow cannot deadlock with every block resident. Counter targets:
+// before pass q a body with degree d has counted q * d, and a constraint of rank r at the body
+// waits for q * d + r. Static bodies (degree 0) are read, never waited on or written.
+// With a residual target the velocity sweeps run one at a time behind a grid barrier and every
+// block sums the same per-block partials in the same order, so all agree on when to stop.
+__global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Constraints cs, float4* vel, float4* pos,
+                                                  uint32_t* flow, float* __restrict__ blockRes, uint32_t vIt, uint32_t pIt,
+                                                  int trackResidual, float residualTarget) {
+    __shared__ float res[2 * MAX_SWEEPS];
+    __shared__ double red[2][4];
+    for (uint32_t i = threadIdx.x; i < 2 * MAX_SWEEPS; i += blockDim.x) res[i] = 0.0f;
+    __syncthreads();
+    const uint32_t M = ctr->manifolds;
+    const uint32_t T = gridDim.x * blockDim.x;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = 0; k < K; k++) {
-        const uint32_t lo = ctr->colourStart[k], hi = ctr->colourStart[k + 1];
-        for (uint32_t c = lo + tid; c < hi; c += stride) warmOne(cs, c, vel);
-        gridBarrier(bar, gridDim.x, gen);
+    if ((uint64_t)M > (uint64_t)MAX_FLOW_ITEMS * T) {
+        if (tid == 0) atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
+        return;
     }
-    uint32_t sweeps = 0;
-    for (uint32_t it = 0; it < vIt && it < MAX_SWEEPS; it++) {
-        float dsq = 0.0f, lsq = 0.0f;
-        for (uint32_t k = 0; k < K; k++) {
-            const uint32_t lo = ctr->colourStart[k], hi = ctr->colourStart[k + 1];
-            for (uint32_t c = lo + tid; c < hi; c += stride) velocityOne(cs, c, vel, dsq, lsq);
-            if (k + 1 < K) gridBarrier(bar, gridDim.x, gen);
+    const bool track = trackResidual || residualTarget > 0.0f;
+    const uint64_t deadline = wall_clock64() + STALL_TICKS;
+    Item it;
+    uint32_t loaded = 0xFFFFFFFFu;
+    uint32_t vEnd = vIt;
+    auto run = [&](uint32_t qBegin, uint32_t qEnd) -> bool {
+        uint32_t q = qBegin, k = 0;
+        bool active = qBegin < qEnd && tid < M;
+        while (true) {
+            bool progressed = false;
+            if (active) {
+                const uint32_t c = tid + k * T;
+                if (c != loaded) { loadItem(cs, c, it); loaded = c; }
+                const uint32_t eA = q * it.dep.y + it.dep.x, eB = q * it.dep.w + it.dep.z;
+                const uint32_t fA = it.dep.y ? loadFlag(flow + it.id.x) : eA;
+                const uint32_t fB = it.dep.w ? loadFlag(flow + it.id.y) : eB;
+                if (fA == eA && fB == eB) {
+                    if (q < vEnd) {
+                        float4 A = it.dep.y ? loadCoherent(vel + it.id.x) : vel[it.id.x];
+                        float4 B = it.dep.w ? loadCoherent(vel + it.id.y) : vel[it.id.y];
+                        float dsq = 0.0f, lsq = 0.0f;
+                        velocityStep(it, A, B, dsq, lsq);
+                        cs.imp[c] = it.imp;
+                        if (it.dep.y) storeCoherent(vel + it.id.x, A);
+                        if (it.dep.w) storeCoherent(vel + it.id.y, B);
+                        if (track) { atomicAdd(&res[q], dsq); atomicAdd(&res[MAX_SWEEPS + q], lsq); }
+                    } else {
+                        const float2* pA = reinterpret_cast<const float2*>(pos + it.id.x);
+                        const float2* pB = reinterpret_cast<const float2*>(pos + it.id.y);
+                        float2 A = it.dep.y ? loadCoherent2(pA) : *pA;
+                        float2 B = it.dep.w ? loadCoherent2(pB) : *pB;
+                        if (positionStep(it, A, B)) {
+                            if (it.dep.y) storeCoherent2(reinterpret_cast<float2*>(pos + it.id.x), A);
+                            if (it.dep.w) storeCoherent2(reinterpret_cast<float2*>(pos + it.id.y), B);
+                        }
+                    }
+                    orderStores();
+                    if (it.dep.y) storeFlag(flow + it.id.x, eA + 1);
+                    if (it.dep.w) storeFlag(flow + it.id.y, eB + 1);
+                    k++;
+                    if ((uint64_t)tid + (uint64_t)k * T >= M) { k = 0; q++; active = q < qEnd; }
+                    progressed = true;
+                }
+            }
+            if (__all(!active)) break;
+            if (!__any(progressed)) {
+                if (wall_clock64() > deadline) { atomicOr(&ctr->overflow, OVERFLOW_STALL); return false; }
+                __builtin_amdgcn_s_sleep(1);
+            }
         }
-        sweeps = it + 1;
-        if (trackResidual || residualTarget > 0.0f) {
-            blockAddDouble(&ctr->residualNum[it], dsq);
-            blockAddDouble(&ctr->residualDen[it], lsq);
+        return true;
+    };
+    float* myRes = blockRes + (size_t)blockIdx.x * 2 * MAX_SWEEPS;
+    if (residualTarget > 0.0f) {
+        uint32_t gen = 0;
+        uint32_t sweeps = vIt;
+        for (uint32_t s = 0; s < vIt; s++) {
+            const bool good = run(s, s + 1);
+            if (__syncthreads_or(!good)) break;
+            if (threadIdx.x == 0) { storeCoherentF(myRes + s, res[s]); storeCoherentF(myRes + MAX_SWEEPS + s, res[MAX_SWEEPS + s]); }
+            if (!gridBarrier(&ctr->barrier[1], gridDim.x, gen, deadline)) {
+                if (threadIdx.x == 0) atomicOr(&ctr->overflow, OVERFLOW_STALL);
+                break;
+            }
+            double num = 0.0, den = 0.0;
+            for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
+                num += (double)loadCoherentF(blockRes + (size_t)j * 2 * MAX_SWEEPS + s);
+                den += (double)loadCoherentF(blockRes + (size_t)j * 2 * MAX_SWEEPS + MAX_SWEEPS + s);
+            }
+            for (uint32_t off = 32; off > 0; off >>= 1) { num += __shfl_down(num, off, 64); den += __shfl_down(den, off, 64); }
+            if ((threadIdx.x & 63u) == 0) { red[0][threadIdx.x >> 6] = num; red[1][threadIdx.x >> 6] = den; }
+            __syncthreads();
+            num = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+            den = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+            __syncthreads();
+            if (sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12)) { sweeps = s + 1; break; }
         }
-        gridBarrier(bar, gridDim.x, gen);
-        if (residualTarget > 0.0f) {
-            const double num = __hip_atomic_load(&ctr->residualNum[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const double den = __hip_atomic_load(&ctr->residualDen[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12)) break;
-        }
+        vEnd = sweeps;
+        run(vEnd, vEnd + pIt);
+    } else {
+        run(0, vIt + pIt);
     }
-    if (tid == 0) ctr->sweeps = sweeps;
-    for (uint32_t it = 0; it < pIt; it++) {
-        for (uint32_t k = 0; k < K; k++) {
-            const uint32_t lo = ctr->colourStart[k], hi = ctr->colourStart[k + 1];
-            for (uint32_t c = lo + tid; c < hi; c += stride) positionOne(cs, c, pos, inv);
-            gridBarrier(bar, gridDim.x, gen);
-        }
-    }
-    const uint32_t total = K ? ctr->colourStart[K] : 0u;
+    __syncthreads();
+    if (track)
+        for (uint32_t q = threadIdx.x; q < vEnd; q += blockDim.x) { myRes[q] = res[q]; myRes[MAX_SWEEPS + q] = res[MAX_SWEEPS + q]; }
+    if (tid == 0) ctr->sweeps = vEnd;
+}
+
+// Accumulators back to the manifolds (the warm-start source of the next step) and
+// computeConstraintErrors (BodyCollisionSolver.cpp:132-183) as per-block partial sums.
+__global__ __launch_bounds__(256) void kSolveFinish(const Counters* __restrict__ ctr, Constraints cs,
+                                                    const float4* __restrict__ vel, const float4* __restrict__ pos,
+                                                    agx_manifold* __restrict__ man, float* __restrict__ errPartial,
+                                                    int trackErrors) {
+    const uint32_t M = ctr->manifolds;
     float app = 0.0f, pen = 0.0f;
-    for (uint32_t c = tid; c < total; c += stride) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < M; c += gridDim.x * blockDim.x) {
         const float4 j = cs.imp[c];
-        agx_manifold& m = man[order[c]];
+        const uint4 id = cs.idx[c];
+        agx_manifold& m = man[id.w];
         m.jn0 = j.x; m.jt0 = j.y;
-        if (m.count > 1) { m.jn1 = j.z; m.jt1 = j.w; }
+        if (id.z > 1) { m.jn1 = j.z; m.jt1 = j.w; }
         if (trackErrors) {
-            // computeConstraintErrors (BodyCollisionSolver.cpp:132-183).
-            const uint4 id = cs.idx[c];
             const float4 nr = cs.nrm[c];
             const float4 A = pos[id.x], B = pos[id.y];
             pen += fmaxf(cs.k1[c].z - ((B.x - A.x) * nr.x + (B.y - A.y) * nr.y), 0.0f);
@@ -379,9 +614,38 @@ __global__ __launch_bounds__(256) void kSolve(Counters* __restrict__ ctr, Constr
             if (id.z > 1) { const float a1 = -(dv + r1.y * vB.z - r1.x * vA.z); app += a1 * (a1 > 0.0f ? 1.0f : 0.0f); }
         }
     }
-    if (trackErrors) {
-        for (uint32_t off = 32; off > 0; off >>= 1) { app += __shfl_down(app, off, 64); pen += __shfl_down(pen, off, 64); }
-        if ((threadIdx.x & 63u) == 0) { atomicAdd(&ctr->velocityError, app); atomicAdd(&ctr->positionError, pen); }
+    if (!trackErrors) return;
+    __shared__ float sa[4], sp[4];
+    for (uint32_t off = 32; off > 0; off >>= 1) { app += __shfl_down(app, off, 64); pen += __shfl_down(pen, off, 64); }
+    if ((threadIdx.x & 63u) == 0) { sa[threadIdx.x >> 6] = app; sp[threadIdx.x >> 6] = pen; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        errPartial[2 * blockIdx.x] = sa[0] + sa[1] + sa[2] + sa[3];
+        errPartial[2 * blockIdx.x + 1] = sp[0] + sp[1] + sp[2] + sp[3];
+    }
+}
+
+// One block: residual per sweep and error totals from the per-block partials.
+__global__ __launch_bounds__(256) void kSolveReduce(Counters* __restrict__ ctr, const float* __restrict__ blockRes,
+                                                    uint32_t resBlocks, const float* __restrict__ errPartial,
+                                                    uint32_t errBlocks, int track, int trackErrors) {
+    const uint32_t sweeps = ctr->sweeps;
+    if (track) {
+        for (uint32_t q = threadIdx.x; q < sweeps && q < MAX_SWEEPS; q += blockDim.x) {
+            double num = 0.0, den = 0.0;
+            for (uint32_t j = 0; j < resBlocks; j++) {
+                num += (double)blockRes[(size_t)j * 2 * MAX_SWEEPS + q];
+                den += (double)blockRes[(size_t)j * 2 * MAX_SWEEPS + MAX_SWEEPS + q];
+            }
+            ctr->residualNum[q] = num;
+            ctr->residualDen[q] = den;
+        }
+    }
+    if (trackErrors && threadIdx.x == 0) {
+        double a = 0.0, p = 0.0;
+        for (uint32_t j = 0; j < errBlocks; j++) { a += errPartial[2 * j]; p += errPartial[2 * j + 1]; }
+        ctr->velocityError = (float)a;
+        ctr->positionError = (float)p;
     }
 }
 
@@ -456,34 +720,59 @@ __global__ __launch_bounds__(256) void kIslandIds(const uint32_t* __restrict__ l
 
 __global__ void kIslandCount(Counters* ctr) { ctr->islands = ctr->scratch[3]; }
 
-inline uint32_t blocksFor(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 8192u)); }
-
 }  // namespace
 
-int cooperativeBlocksPerCU() {
+void dataflowBlocksPerCU(int& colour, int& solve) {
     int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void*>(kColourPrep), 256, 0) != hipSuccess) a = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(kSolve), 256, 0) != hipSuccess) b = 1;
-    return std::max(1, std::min(a, b));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void*>(kColourFlow), 256, 0) != hipSuccess) a = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(kSolveFlow), 256, 0) != hipSuccess) b = 1;
+    colour = std::max(1, std::min(a, 4));
+    solve = std::max(1, std::min(b, 4));
 }
 
 hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
-    void* args[] = { (void*)&w.man[w.cur], (void*)&w.ctr, (void*)&w.colBody, (void*)&w.inv, (void*)&w.pos, (void*)&w.vel,
-                     (void*)&w.shape, (void*)&w.materials, (void*)&w.colourId, (void*)&w.claim, (void*)&w.used,
-                     (void*)&w.order, (void*)&w.cs };
-    hipError_t e = launchFillU32(st, &w.ctr->barrier[0], 0u, 1);
+    const uint32_t gm = blocksForManifolds(w);
+    const uint32_t gd = blocksFor(std::max(1u, w.nDynBodies));
+    hipLaunchKernelGGL(kFlowClear, dim3(blocksFor(w.N)), dim3(256), 0, st, w.N, w.degree, w.incFill, w.flow, w.used);
+    hipLaunchKernelGGL(kDegree, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody, w.inv, w.manBodies, w.degree);
+    hipError_t e = launchScan(st, w.degree, w.incStart, nullptr, &w.ctr->scratch[4], w.N, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kColourPrep), dim3(w.coopBlocks), dim3(256), args, 0, st);
+    hipLaunchKernelGGL(kIncident, dim3(gm), dim3(256), 0, st, w.ctr, w.manBodies, w.inv, w.incStart, w.incFill, w.inc);
+    hipLaunchKernelGGL(kRankByPriority, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.inc,
+                       w.rank);
+    void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.flow, (void*)&w.used,
+                     (void*)&w.colourId };
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kColourFlow), dim3(w.colourBlocks), dim3(256), args, 0, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
+    hipLaunchKernelGGL(kColourScan, dim3(1), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
+    hipLaunchKernelGGL(kColourScatter, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist, w.order,
+                       w.posOf);
+    hipLaunchKernelGGL(kSolveRank, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.inc,
+                       w.posOf, w.rank, w.incSorted, w.flow);
+    hipLaunchKernelGGL(kPrep, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
+                       w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
+    hipLaunchKernelGGL(kWarm, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.incSorted,
+                       w.posOf, w.cs, w.vel);
+    return hipGetLastError();
 }
 
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                        float residualTarget, int trackErrors) {
-    void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&w.inv, (void*)&w.order,
-                     (void*)&w.man[w.cur], (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget,
-                     (void*)&trackErrors };
+    vIt = std::min(vIt, MAX_SWEEPS);
+    void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&w.flow, (void*)&w.blockRes,
+                     (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
     hipError_t e = launchFillU32(st, &w.ctr->barrier[1], 0u, 1);
     if (e != hipSuccess) return e;
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kSolve), dim3(w.coopBlocks), dim3(256), args, 0, st);
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kSolveFlow), dim3(w.solveBlocks), dim3(256), args, 0, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kSolveFinish, dim3(FINISH_BLOCKS), dim3(256), 0, st, w.ctr, w.cs, w.vel, w.pos, w.man[w.cur],
+                       w.errPartial, trackErrors);
+    const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;
+    if (track || trackErrors)
+        hipLaunchKernelGGL(kSolveReduce, dim3(1), dim3(256), 0, st, w.ctr, w.blockRes, (uint32_t)w.solveBlocks, w.errPartial,
+                           FINISH_BLOCKS, track, trackErrors);
+    return hipGetLastError();
 }
 
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w) {

@@ -466,7 +466,12 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return bail(fail(nullptr, AGX_ERR_HIP, "hipGetDeviceProperties failed"));
     d.numCU = prop.multiProcessorCount;
-    d.coopBlocks = d.numCU * std::min(2, agx::cooperativeBlocksPerCU());
+    {
+        int perCuColour = 1, perCuSolve = 1;
+        agx::dataflowBlocksPerCU(perCuColour, perCuSolve);
+        d.colourBlocks = d.numCU * perCuColour;
+        d.solveBlocks = d.numCU * perCuSolve;
+    }
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(nullptr, AGX_ERR_HIP, "hipStreamCreate failed"));
 
@@ -489,8 +494,12 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     d.hashCapMax = 16;
     while (d.hashCapMax < 2 * d.manCap) d.hashCapMax <<= 1;
     AL(d.hashKeys, d.hashCapMax); AL(d.hashVals, d.hashCapMax);
-    AL(d.colourId, pairCap); AL(d.claim, d.N); AL(d.used, 2ull * d.N); AL(d.order, pairCap);
-    AL(d.cs.idx, pairCap); AL(d.cs.nrm, pairCap); AL(d.cs.inv, pairCap); AL(d.cs.r0, pairCap);
+    AL(d.manBodies, pairCap); AL(d.degree, d.N); AL(d.incStart, d.N); AL(d.incFill, d.N); AL(d.flow, d.N);
+    AL(d.inc, 2 * pairCap); AL(d.incSorted, 2 * pairCap); AL(d.rank, 2 * pairCap); AL(d.used, 2ull * d.N);
+    AL(d.colourId, pairCap); AL(d.colourHist, (uint64_t)agx::COLOUR_BLOCKS * agx::MAX_COLOURS);
+    AL(d.posOf, pairCap); AL(d.order, pairCap);
+    AL(d.blockRes, (uint64_t)d.solveBlocks * 2 * agx::MAX_SWEEPS); AL(d.errPartial, 2 * agx::SOLVE_FINISH_BLOCKS);
+    AL(d.cs.idx, pairCap); AL(d.cs.dep, pairCap); AL(d.cs.nrm, pairCap); AL(d.cs.inv, pairCap); AL(d.cs.r0, pairCap);
     AL(d.cs.r1, pairCap); AL(d.cs.k0, pairCap); AL(d.cs.k1, pairCap); AL(d.cs.imp, pairCap);
 #undef UP
 #undef AL
