@@ -14,6 +14,7 @@ constexpr uint32_t FIRST_FIT_COLOURS = 128;  // chosen first-fit from two 64-bit
 constexpr uint32_t MAX_COLOURS = FIRST_FIT_COLOURS + 1;   // + one shared overflow colour
 constexpr uint32_t COLOUR_BLOCKS = 512;     // blocks of the colour histogram / scatter kernels
 constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataflow kernels
+constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour start on a wave boundary
 constexpr uint32_t MAX_SWEEPS = 64;
 constexpr uint32_t TRAVERSAL_STACK = 64;
 constexpr uint32_t SORT_TILE = 4096;        // keys per radix-sort block (256 threads x 16)
@@ -34,7 +35,8 @@ struct Counters {
     uint32_t sweeps;
     uint32_t overflow;         // OVERFLOW_* bits
     uint32_t scratch[8];       // scan totals
-    uint32_t colourStart[MAX_COLOURS + 1];
+    uint32_t colourStart[MAX_COLOURS + 1];   // solve position of each colour's first slot (wave-aligned)
+    uint32_t colourCount[MAX_COLOURS];       // constraints of each colour (the rest of its slots are holes)
     double residualNum[MAX_SWEEPS];
     double residualDen[MAX_SWEEPS];
     float velocityError, positionError;
@@ -143,12 +145,17 @@ struct DeviceWorld {
     uint32_t* inc = nullptr;         // 2 * manCap entries 2m + side, grouped by body
     uint32_t* incSorted = nullptr;   // the same lists sorted by solve position
     uint32_t* rank = nullptr;        // per entry 2m + side: rank in its body's list
-    uint32_t* flow = nullptr;        // per body: dataflow completion counter
-    unsigned long long* used = nullptr;    // per body, 2 words: first-fit colours taken
+    // Per-body records of the dataflow kernels: 64-bit words (sequence << 32 | 32-bit payload).
+    // A word's sequence counts the operations finished on the body, so one atomic 64-bit load
+    // tells a waiting constraint both that its turn has come and the current value.
+    unsigned long long* colRec = nullptr;   // 4 words: first-fit colours taken (4 x 32 bits)
+    uint32_t* groupDone = nullptr;   // 2 x solveGroups: per lane-group progress of the velocity and position flows
+    uint32_t solveGroups = 0;
     uint16_t* colourId = nullptr;    // per manifold
     uint32_t* colourHist = nullptr;  // COLOUR_BLOCKS x MAX_COLOURS block histograms, then block bases
     uint32_t* posOf = nullptr;       // manifold -> solve position
-    uint32_t* order = nullptr;       // solve position -> manifold
+    uint32_t* order = nullptr;       // solve position -> manifold (UINT32_MAX in padding holes)
+    uint64_t solveCap = 0;           // solve positions: manCap + COLOUR_ALIGN * MAX_COLOURS
     float* blockRes = nullptr;       // per solve block: residual partial sums (2 x MAX_SWEEPS)
     float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
     Constraints cs{};

@@ -4,10 +4,11 @@
 // waves (Solving/SolvingPlanner.cpp:141-310). Here the order is colour-major: constraints of one
 // colour share no dynamic body, so every order inside a colour gives the same bits. The device
 // runs that sequential order as a dataflow instead of colour by colour behind grid barriers:
-// every dynamic body counts the constraints finished on it, and a constraint runs as soon as the
-// counters of both its bodies show that every constraint before it (by sweep, then by solve
-// position) touching them is done. The result equals a sequential sweep in the exported order
-// exactly, and later colours and sweeps start as soon as their own inputs are ready.
+// every dynamic body keeps records whose sequence counts the operations finished on it, and a
+// constraint runs as soon as the records of both its bodies show that every constraint before it
+// (by sweep, then by solve position) touching them is done. The result equals a sequential sweep
+// in the exported order exactly, and later colours and sweeps start as soon as their own inputs
+// are ready.
 //
 // The colouring is a dataflow too: greedy first fit in a fixed pseudo-random priority order
 // (each manifold takes the lowest colour not taken at its bodies by manifolds of higher
@@ -19,6 +20,10 @@ namespace agx {
 namespace {
 
 constexpr uint64_t STALL_TICKS = 200000000ull;   // 2 s of the 100 MHz wall clock
+// A waiting constraint whose body counter is d short of its target cannot run before d - 1 more
+// constraints finish there, each at least one coherent memory round trip (about 1 us): it polls
+// again only after that long, which keeps the polling traffic off the critical path.
+constexpr uint64_t BACKOFF_TICKS_PER_HOP = 100;   // 1 us
 constexpr uint32_t FINISH_BLOCKS = SOLVE_FINISH_BLOCKS;
 
 inline uint32_t blocksFor(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 8192u)); }
@@ -35,11 +40,17 @@ __device__ inline uint32_t mix32(uint32_t x) {
 __device__ inline bool isDynamic(const float2* inv, uint32_t b) { return inv[b].x != 0.0f; }
 
 // ---------------------------------------------------------------- incidence lists
+__device__ inline unsigned long long packWord(uint32_t seq, uint32_t bits) {
+    return ((unsigned long long)seq << 32) | bits;
+}
+__device__ inline uint32_t seqOf(unsigned long long w) { return (uint32_t)(w >> 32); }
+__device__ inline uint32_t bitsOf(unsigned long long w) { return (uint32_t)w; }
+
 __global__ __launch_bounds__(256) void kFlowClear(uint32_t n, uint32_t* __restrict__ degree, uint32_t* __restrict__ fill,
-                                                  uint32_t* __restrict__ flow, unsigned long long* __restrict__ used) {
+                                                  unsigned long long* __restrict__ colRec) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        degree[i] = 0; fill[i] = 0; flow[i] = 0;
-        used[2 * i] = 0; used[2 * i + 1] = 0;
+        degree[i] = 0; fill[i] = 0;
+        for (uint32_t k = 0; k < 4; k++) colRec[4 * i + k] = 0;
     }
 }
 
@@ -86,13 +97,26 @@ __global__ __launch_bounds__(256) void kRankByPriority(const uint32_t* __restric
 // ---------------------------------------------------------------- dataflow colouring
 // Lane t holds manifolds t, t + T, ... (T = grid threads) and takes them in priority order, so
 // the manifold of lowest priority still open anywhere is always the next one of its lane: with
-// every block resident the flow cannot deadlock. A manifold is ready when both of its bodies'
-// counters equal its rank there; it then reads their colour masks, takes the first free colour,
-// publishes the masks and advances the counters.
+// every block resident the flow cannot deadlock. A manifold is ready when the colour records of
+// both its bodies carry its rank there as sequence; it takes the first colour free at both and
+// writes the records back with the next sequence.
+__device__ inline bool readColours(const unsigned long long* rec, uint32_t expect, uint32_t* bits, uint32_t& dist) {
+    unsigned long long w[4];
+    for (uint32_t i = 0; i < 4; i++) w[i] = loadCoherent64(rec + i);
+    bool ok = true;
+    uint32_t lo = expect;
+    for (uint32_t i = 0; i < 4; i++) {
+        bits[i] = bitsOf(w[i]);
+        ok = ok && seqOf(w[i]) == expect;
+        lo = min(lo, seqOf(w[i]));
+    }
+    dist = max(dist, expect - lo);
+    return ok;
+}
+
 __global__ __launch_bounds__(256) void kColourFlow(Counters* __restrict__ ctr, const uint2* __restrict__ manBodies,
                                                    const float2* __restrict__ inv, const uint32_t* __restrict__ rank,
-                                                   uint32_t* flow, unsigned long long* used,
-                                                   uint16_t* __restrict__ colourId) {
+                                                   unsigned long long* colRec, uint16_t* __restrict__ colourId) {
     const uint32_t M = ctr->manifolds;
     const uint32_t T = gridDim.x * blockDim.x;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,35 +148,31 @@ __global__ __launch_bounds__(256) void kColourFlow(Counters* __restrict__ ctr, c
     };
     if (k < MAX_FLOW_ITEMS) load();
     const uint64_t deadline = wall_clock64() + STALL_TICKS;
+    uint64_t wake = 0;
     while (true) {
         bool progressed = false;
-        if (k < MAX_FLOW_ITEMS) {
-            const uint32_t fa = da ? loadFlag(flow + a) : ra;
-            const uint32_t fb = db ? loadFlag(flow + b) : rb;
-            if (fa == ra && fb == rb) {
-                unsigned long long ua0 = 0, ua1 = 0, ub0 = 0, ub1 = 0;
-                if (da) { ua0 = loadCoherent64(used + 2 * a); ua1 = loadCoherent64(used + 2 * a + 1); }
-                if (db) { ub0 = loadCoherent64(used + 2 * b); ub1 = loadCoherent64(used + 2 * b + 1); }
-                const unsigned long long f0 = ~(ua0 | ub0), f1 = ~(ua1 | ub1);
-                uint32_t c;
-                if (f0) c = (uint32_t)__ffsll(f0) - 1;
-                else if (f1) c = 64 + (uint32_t)__ffsll(f1) - 1;
-                else c = FIRST_FIT_COLOURS;
-                colourId[m] = (uint16_t)c;
-                if (c < 64) {
-                    const unsigned long long bit = 1ull << c;
-                    if (da) storeCoherent64(used + 2 * a, ua0 | bit);
-                    if (db) storeCoherent64(used + 2 * b, ub0 | bit);
-                } else if (c < FIRST_FIT_COLOURS) {
-                    const unsigned long long bit = 1ull << (c - 64);
-                    if (da) storeCoherent64(used + 2 * a + 1, ua1 | bit);
-                    if (db) storeCoherent64(used + 2 * b + 1, ub1 | bit);
-                } else {
-                    atomicOr(&ctr->overflow, OVERFLOW_COLOURING);
+        const uint64_t now = wall_clock64();
+        if (k < MAX_FLOW_ITEMS && now >= wake) {
+            uint32_t ba[4] = { 0, 0, 0, 0 }, bb[4] = { 0, 0, 0, 0 };
+            uint32_t dist = 1;
+            bool ok = true;
+            if (da) ok = readColours(colRec + 4 * (size_t)a, ra, ba, dist);
+            if (db) ok = readColours(colRec + 4 * (size_t)b, rb, bb, dist) && ok;
+            if (!ok) {
+                wake = now + (uint64_t)(dist - 1) * BACKOFF_TICKS_PER_HOP;
+            } else {
+                uint32_t c = FIRST_FIT_COLOURS;
+                for (uint32_t i = 0; i < 4; i++) {
+                    const uint32_t free = ~(ba[i] | bb[i]);
+                    if (free) { c = 32 * i + (uint32_t)__ffs((int)free) - 1; break; }
                 }
-                orderStores();
-                if (da) storeFlag(flow + a, ra + 1);
-                if (db) storeFlag(flow + b, rb + 1);
+                colourId[m] = (uint16_t)c;
+                if (c == FIRST_FIT_COLOURS) atomicOr(&ctr->overflow, OVERFLOW_COLOURING);
+                for (uint32_t i = 0; i < 4; i++) {
+                    const uint32_t bit = (c < FIRST_FIT_COLOURS && (c >> 5) == i) ? (1u << (c & 31u)) : 0u;
+                    if (da) storeCoherent64(colRec + 4 * (size_t)a + i, packWord(ra + 1, ba[i] | bit));
+                    if (db) storeCoherent64(colRec + 4 * (size_t)b + i, packWord(rb + 1, bb[i] | bit));
+                }
                 pending &= ~(1ull << k);
                 k = pick();
                 if (k < MAX_FLOW_ITEMS) load();
@@ -161,7 +181,7 @@ __global__ __launch_bounds__(256) void kColourFlow(Counters* __restrict__ ctr, c
         }
         if (__all(k >= MAX_FLOW_ITEMS)) break;
         if (!__any(progressed)) {
-            if (wall_clock64() > deadline) { atomicOr(&ctr->overflow, OVERFLOW_STALL); break; }
+            if (now > deadline) { atomicOr(&ctr->overflow, OVERFLOW_STALL); break; }
             __builtin_amdgcn_s_sleep(1);
         }
     }
@@ -188,42 +208,67 @@ __global__ __launch_bounds__(256) void kColourHist(const Counters* __restrict__ 
     for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x) hist[blockIdx.x * MAX_COLOURS + i] = h[i];
 }
 
-// One block: turns the block histograms into each block's first slot per colour.
-__global__ __launch_bounds__(256) void kColourScan(Counters* __restrict__ ctr, uint32_t* __restrict__ hist, uint32_t blocks) {
-    __shared__ uint32_t total[MAX_COLOURS];
-    __shared__ uint32_t start[MAX_COLOURS];
-    const uint32_t c = threadIdx.x;
-    if (c < MAX_COLOURS) {
-        uint32_t run = 0;
-        for (uint32_t j = 0; j < blocks; j++) {
-            const uint32_t v = hist[j * MAX_COLOURS + c];
-            hist[j * MAX_COLOURS + c] = run;
-            run += v;
+// One block per colour: exclusive scan of that colour's count over the histogram blocks (each
+// block's first slot within the colour) and the colour's total.
+__global__ __launch_bounds__(256) void kColourColumnScan(Counters* __restrict__ ctr, uint32_t* __restrict__ hist,
+                                                         uint32_t blocks) {
+    __shared__ uint32_t waveSum[4];
+    const uint32_t c = blockIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < blocks; base += blockDim.x) {
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t v = j < blocks ? hist[j * MAX_COLOURS + c] : 0u;
+        // inclusive scan inside the wave, then across the four waves
+        uint32_t x = v;
+        for (uint32_t off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, WAVE);
+            if ((threadIdx.x & (WAVE - 1)) >= off) x += y;
         }
-        total[c] = run;
+        if ((threadIdx.x & (WAVE - 1)) == WAVE - 1) waveSum[threadIdx.x / WAVE] = x;
+        __syncthreads();
+        uint32_t before = carry;
+        for (uint32_t k = 0; k < threadIdx.x / WAVE; k++) before += waveSum[k];
+        if (j < blocks) hist[j * MAX_COLOURS + c] = before + x - v;
+        carry += waveSum[0] + waveSum[1] + waveSum[2] + waveSum[3];
+        __syncthreads();
     }
-    __syncthreads();
-    if (c == 0) {
-        uint32_t s = 0, K = 0;
-        for (uint32_t i = 0; i < MAX_COLOURS; i++) {
-            start[i] = s;
-            ctr->colourStart[i] = s;
-            s += total[i];
-            if (total[i]) K = i + 1;
-        }
-        ctr->colourStart[MAX_COLOURS] = s;
-        ctr->colours = K;
+    if (threadIdx.x == 0) ctr->colourCount[c] = carry;
+}
+
+// Colour starts: every colour's slots start on a wave boundary (COLOUR_ALIGN), so a wave of the
+// solve never mixes colours; the slots past a colour's count are holes.
+__global__ void kColourStarts(Counters* __restrict__ ctr) {
+    if (threadIdx.x != 0) return;
+    uint32_t s = 0, K = 0;
+    for (uint32_t i = 0; i < MAX_COLOURS; i++) {
+        const uint32_t n = ctr->colourCount[i];
+        ctr->colourStart[i] = s;
+        s += (n + COLOUR_ALIGN - 1) / COLOUR_ALIGN * COLOUR_ALIGN;
+        if (n) K = i + 1;
     }
-    __syncthreads();
-    if (c < MAX_COLOURS)
-        for (uint32_t j = 0; j < blocks; j++) hist[j * MAX_COLOURS + c] += start[c];
+    ctr->colourStart[MAX_COLOURS] = s;
+    ctr->colours = K;
+}
+
+// Marks the padding slots at the end of every colour: no manifold (count 0), never waited on.
+__global__ __launch_bounds__(256) void kMarkHoles(const Counters* __restrict__ ctr, uint32_t* __restrict__ order,
+                                                  Constraints cs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = i / COLOUR_ALIGN, j = i % COLOUR_ALIGN;
+    if (k >= MAX_COLOURS) return;
+    const uint32_t p = ctr->colourStart[k] + ctr->colourCount[k] + j;
+    if (p >= ctr->colourStart[k + 1]) return;
+    order[p] = 0xFFFFFFFFu;
+    cs.idx[p] = make_uint4(0u, 0u, 0u, 0xFFFFFFFFu);
+    cs.dep[p] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ __launch_bounds__(256) void kColourScatter(const Counters* __restrict__ ctr, const uint16_t* __restrict__ colourId,
                                                       const uint32_t* __restrict__ hist, uint32_t* __restrict__ order,
                                                       uint32_t* __restrict__ posOf) {
     __shared__ uint32_t cursor[MAX_COLOURS];
-    for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x) cursor[i] = hist[blockIdx.x * MAX_COLOURS + i];
+    for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x)
+        cursor[i] = ctr->colourStart[i] + hist[blockIdx.x * MAX_COLOURS + i];
     __syncthreads();
     uint32_t lo, hi;
     chunkOf(ctr->manifolds, lo, hi);
@@ -234,13 +279,12 @@ __global__ __launch_bounds__(256) void kColourScatter(const Counters* __restrict
     }
 }
 
-// Rank of every list entry among its body's entries by solve position, the lists sorted by
-// position, and the counters reset for the solve.
+// Rank of every list entry among its body's entries by solve position, and the lists sorted by
+// position.
 __global__ __launch_bounds__(256) void kSolveRank(const uint32_t* __restrict__ dynBodies, uint32_t nDyn,
                                                   const uint32_t* __restrict__ degree, const uint32_t* __restrict__ incStart,
                                                   const uint32_t* __restrict__ inc, const uint32_t* __restrict__ posOf,
-                                                  uint32_t* __restrict__ rank, uint32_t* __restrict__ incSorted,
-                                                  uint32_t* __restrict__ flow) {
+                                                  uint32_t* __restrict__ rank, uint32_t* __restrict__ incSorted) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nDyn; k += gridDim.x * blockDim.x) {
         const uint32_t b = dynBodies[k], s = incStart[b], d = degree[b];
         for (uint32_t i = 0; i < d; i++) {
@@ -251,14 +295,27 @@ __global__ __launch_bounds__(256) void kSolveRank(const uint32_t* __restrict__ d
             rank[e] = r;
             incSorted[s + r] = e;
         }
-        flow[b] = 0;
     }
 }
 
+// Predecessor of a constraint at one of its dynamic bodies in the solve order: the body's previous
+// constraint in the same sweep, or, for the body's first constraint, its last constraint of the
+// previous sweep (flagged PRED_PREV_SWEEP). NO_PRED: a static body.
+constexpr uint32_t NO_PRED = 0xFFFFFFFFu;
+constexpr uint32_t PRED_PREV_SWEEP = 0x80000000u;
+
 // computeConstraintData (BodyCollisionSolver.cpp:229-390), written at the solve position.
+__device__ inline uint32_t predecessorOf(uint32_t body, uint32_t r, const uint32_t* degree, const uint32_t* incStart,
+                                         const uint32_t* incSorted, const uint32_t* posOf) {
+    const uint32_t s = incStart[body];
+    if (r > 0) return posOf[incSorted[s + r - 1] >> 1];
+    return posOf[incSorted[s + degree[body] - 1] >> 1] | PRED_PREV_SWEEP;
+}
+
 __global__ __launch_bounds__(256) void kPrep(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
                                              const uint2* __restrict__ manBodies, const uint32_t* __restrict__ posOf,
                                              const uint32_t* __restrict__ degree, const uint32_t* __restrict__ rank,
+                                             const uint32_t* __restrict__ incStart, const uint32_t* __restrict__ incSorted,
                                              const float2* __restrict__ inv, const float4* __restrict__ pos,
                                              const float4* __restrict__ vel, const uint4* __restrict__ shape,
                                              const float4* __restrict__ materials, Constraints cs) {
@@ -296,8 +353,9 @@ __global__ __launch_bounds__(256) void kPrep(const agx_manifold* __restrict__ ma
         }
         const bool dA = iA.x != 0.0f, dB = iB.x != 0.0f;
         cs.idx[slot] = make_uint4(bA, bB, d.count, m);
-        cs.dep[slot] = make_uint4(dA ? rank[2 * m] : 0u, dA ? degree[bA] : 0u, dB ? rank[2 * m + 1] : 0u,
-                                  dB ? degree[bB] : 0u);
+        cs.dep[slot] = make_uint4(dA ? predecessorOf(bA, rank[2 * m], degree, incStart, incSorted, posOf) : NO_PRED,
+                                  dB ? predecessorOf(bB, rank[2 * m + 1], degree, incStart, incSorted, posOf) : NO_PRED,
+                                  dA ? 1u : 0u, dB ? 1u : 0u);
         cs.nrm[slot] = make_float4(nx, ny, sf, df);
         cs.inv[slot] = make_float4(iA.x, iB.x, iA.y, iB.y);
         cs.r0[slot] = make_float4(pr[0][0], pr[0][1], pr[0][2], pr[0][3]);
@@ -342,7 +400,7 @@ __global__ __launch_bounds__(256) void kWarm(const uint32_t* __restrict__ dynBod
 // ---------------------------------------------------------------- dataflow sweeps
 struct Item {
     uint4 id;     // bodyA, bodyB, count, manifold
-    uint4 dep;    // rankA, degreeA, rankB, degreeB
+    uint4 dep;    // predecessor at A, predecessor at B, A dynamic, B dynamic
     float4 nrm, iv, r0, r1, k0, k1, imp;
 };
 
@@ -458,17 +516,70 @@ __device__ inline void velocityStep(Item& it, float4& A, float4& B, float& dsq, 
     }
 }
 
-// solvePositionConstraints (BodyCollisionSolver.cpp:930-980): linear only. Returns false when
-// the constraint is not penetrating (nothing moves).
-__device__ inline bool positionStep(const Item& it, float2& A, float2& B) {
-    const float pen = it.k1.z - ((B.x - A.x) * it.nrm.x + (B.y - A.y) * it.nrm.y);
-    if (pen <= 0.0f) return false;
-    const float iMA = it.iv.x, iMB = it.iv.y;
-    const float total = pen / (iMA + iMB) * 0.5f;
-    const float cA = iMA * total, cB = iMB * total;
-    A.x -= it.nrm.x * cA; A.y -= it.nrm.y * cA;
-    B.x += it.nrm.x * cB; B.y += it.nrm.y * cB;
-    return true;
+struct PosItem {
+    uint2 body;
+    uint32_t count;  // contact points; 0 in a padding hole
+    uint4 dep;       // predecessor at A, predecessor at B, A dynamic, B dynamic
+    float nx, ny, iMA, iMB, penBase;
+};
+
+__device__ inline void loadPosItem(const Constraints& cs, uint32_t c, PosItem& pi) {
+    const uint4 id = cs.idx[c];
+    const float4 nr = cs.nrm[c], iv = cs.inv[c];
+    pi.body = make_uint2(id.x, id.y);
+    pi.count = id.z;
+    pi.dep = cs.dep[c];
+    pi.nx = nr.x; pi.ny = nr.y; pi.iMA = iv.x; pi.iMB = iv.y;
+    pi.penBase = cs.k1[c].z;
+}
+
+// solvePositionConstraints (BodyCollisionSolver.cpp:930-980): linear only; nothing moves when
+// the constraint is not penetrating.
+__device__ inline void positionStep(const PosItem& pi, float2& A, float2& B) {
+    const float pen = pi.penBase - ((B.x - A.x) * pi.nx + (B.y - A.y) * pi.ny);
+    if (pen <= 0.0f) return;
+    const float total = pen / (pi.iMA + pi.iMB) * 0.5f;
+    const float cA = pi.iMA * total, cB = pi.iMB * total;
+    A.x -= pi.nx * cA; A.y -= pi.ny * cA;
+    B.x += pi.nx * cB; B.y += pi.ny * cB;
+}
+
+// Lanes run their constraints in groups of SOLVE_GROUP consecutive lanes (a divisor of the wave
+// size and of COLOUR_ALIGN, so a group's constraints of one step share one colour).
+constexpr uint32_t SOLVE_GROUP = 4;
+
+// True when the predicate holds on every lane of this lane's group.
+__device__ inline bool groupAll(bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t shift = threadIdx.x & (WAVE - SOLVE_GROUP);
+    const unsigned long long bits = SOLVE_GROUP == 64 ? ~0ull : ((1ull << SOLVE_GROUP) - 1);
+    return ((mask >> shift) & bits) == bits;
+}
+
+// Progress of a group in one flow: after its step at (sweep q, first position b) it publishes
+// q * P + min(b + SOLVE_GROUP, P), so every position of the group up to that step reads as done,
+// and no position of a later step does. A constraint's predecessor at (sweep q', position p) is
+// done once its group's progress reaches q' * P + p + 1.
+__device__ inline uint32_t predTarget(uint32_t pred, uint32_t q, uint32_t P, uint32_t T, uint32_t& wave) {
+    wave = 0;
+    if (pred == NO_PRED) return 0;
+    const uint32_t p = pred & ~PRED_PREV_SWEEP;
+    uint32_t qp = q;
+    if (pred & PRED_PREV_SWEEP) {
+        if (q == 0) return 0;
+        qp = q - 1;
+    }
+    wave = (p % T) / SOLVE_GROUP;
+    return qp * P + p + 1;
+}
+
+__device__ inline bool predecessorsDone(const uint32_t* done, uint32_t predA, uint32_t predB, uint32_t q, uint32_t P,
+                                        uint32_t T) {
+    uint32_t wa, wb;
+    const uint32_t ta = predTarget(predA, q, P, T, wa), tb = predTarget(predB, q, P, T, wb);
+    const uint32_t fa = ta ? loadFlag(done + wa) : 0u;
+    const uint32_t fb = tb ? loadFlag(done + wb) : 0u;
+    return fa >= ta && fb >= tb;
 }
 
 __device__ inline void storeCoherentF(float* p, float v) {
@@ -478,73 +589,105 @@ __device__ inline float loadCoherentF(const float* p) {
     return __uint_as_float(loadFlag(reinterpret_cast<const uint32_t*>(p)));
 }
 
-// Persistent dataflow over the passes: velocity sweeps [0, vEnd), then position sweeps. Lane t
-// holds solve positions t, t + T, ... and takes them in (pass, position) order, which is the
-// sequential order restricted to the lane, so the lowest open (pass, position) anywhere is always
-// its lane's next one and the flow cannot deadlock with every block resident. Counter targets:
-// before pass q a body with degree d has counted q * d, and a constraint of rank r at the body
-// waits for q * d + r. Static bodies (degree 0) are read, never waited on or written.
+// Persistent dataflow over the sweeps. Velocity sweeps read and write only velocities (and the
+// constraint's own accumulators), position sweeps only positions, so the two are independent
+// flows that run side by side; each alone is the sequential order.
+// Work goes by group steps: lane t holds solve positions t, t + T, ... (T = grid threads, a
+// multiple of the wave size), and a group of lanes runs a step, all its lanes together, once the
+// predecessors of every lane's constraint are done (a done predecessor stays done). Steps are
+// taken in (sweep, position) order and each group publishes its progress per flow. Colours start
+// on wave boundaries, so the constraints of one step share one sweep and one colour and depend
+// only on steps of earlier (sweep, colour): the steps form an acyclic graph, the lowest open step
+// is always ready, and with every block resident neither flow can deadlock. Running whole groups
+// keeps the SIMDs busy with many lanes at a time; small groups keep a step from waiting on many
+// unrelated predecessors. Body state of dynamic bodies goes through coherent loads and stores;
+// static bodies are only read.
 // With a residual target the velocity sweeps run one at a time behind a grid barrier and every
 // block sums the same per-block partials in the same order, so all agree on when to stop.
 __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Constraints cs, float4* vel, float4* pos,
-                                                  uint32_t* flow, float* __restrict__ blockRes, uint32_t vIt, uint32_t pIt,
-                                                  int trackResidual, float residualTarget) {
+                                                  uint32_t* vDone, uint32_t* pDone, float* __restrict__ blockRes,
+                                                  uint32_t vIt, uint32_t pIt, int trackResidual, float residualTarget) {
     __shared__ float res[2 * MAX_SWEEPS];
     __shared__ double red[2][4];
     for (uint32_t i = threadIdx.x; i < 2 * MAX_SWEEPS; i += blockDim.x) res[i] = 0.0f;
     __syncthreads();
-    const uint32_t M = ctr->manifolds;
+    const uint32_t P = ctr->colourStart[MAX_COLOURS];   // solve positions including holes
     const uint32_t T = gridDim.x * blockDim.x;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((uint64_t)M > (uint64_t)MAX_FLOW_ITEMS * T) {
+    const uint32_t groupBase = tid & ~(SOLVE_GROUP - 1);
+    const uint32_t groupId = tid / SOLVE_GROUP;
+    const bool leader = (threadIdx.x & (SOLVE_GROUP - 1)) == 0;
+    if ((uint64_t)P > (uint64_t)MAX_FLOW_ITEMS * T) {
         if (tid == 0) atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
         return;
     }
     const bool track = trackResidual || residualTarget > 0.0f;
     const uint64_t deadline = wall_clock64() + STALL_TICKS;
     Item it;
-    uint32_t loaded = 0xFFFFFFFFu;
-    uint32_t vEnd = vIt;
-    auto run = [&](uint32_t qBegin, uint32_t qEnd) -> bool {
-        uint32_t q = qBegin, k = 0;
-        bool active = qBegin < qEnd && tid < M;
+    PosItem pi;
+    uint32_t loaded = 0xFFFFFFFFu, pLoaded = 0xFFFFFFFFu;
+    auto run = [&](uint32_t vBegin, uint32_t vEnd, uint32_t pBegin, uint32_t pEnd) -> bool {
+        uint32_t qv = vBegin, kv = 0, qp = pBegin, kp = 0;   // group-uniform cursors
+        bool vAct = vBegin < vEnd && groupBase < P, pAct = pBegin < pEnd && groupBase < P;
+        bool vReady = false, pReady = false;
         while (true) {
             bool progressed = false;
-            if (active) {
-                const uint32_t c = tid + k * T;
-                if (c != loaded) { loadItem(cs, c, it); loaded = c; }
-                const uint32_t eA = q * it.dep.y + it.dep.x, eB = q * it.dep.w + it.dep.z;
-                const uint32_t fA = it.dep.y ? loadFlag(flow + it.id.x) : eA;
-                const uint32_t fB = it.dep.w ? loadFlag(flow + it.id.y) : eB;
-                if (fA == eA && fB == eB) {
-                    if (q < vEnd) {
-                        float4 A = it.dep.y ? loadCoherent(vel + it.id.x) : vel[it.id.x];
+            if (vAct) {
+                const uint32_t c = tid + kv * T;
+                bool has = c < P;
+                if (has && c != loaded) { loadItem(cs, c, it); loaded = c; }
+                has = has && it.id.z != 0;
+                if (has && !vReady) vReady = predecessorsDone(vDone, it.dep.x, it.dep.y, qv, P, T);
+                if (groupAll(!has || vReady)) {
+                    float dsq = 0.0f, lsq = 0.0f;
+                    if (has) {
+                        float4 A = it.dep.z ? loadCoherent(vel + it.id.x) : vel[it.id.x];
                         float4 B = it.dep.w ? loadCoherent(vel + it.id.y) : vel[it.id.y];
-                        float dsq = 0.0f, lsq = 0.0f;
                         velocityStep(it, A, B, dsq, lsq);
                         cs.imp[c] = it.imp;
-                        if (it.dep.y) storeCoherent(vel + it.id.x, A);
+                        if (it.dep.z) storeCoherent(vel + it.id.x, A);
                         if (it.dep.w) storeCoherent(vel + it.id.y, B);
-                        if (track) { atomicAdd(&res[q], dsq); atomicAdd(&res[MAX_SWEEPS + q], lsq); }
-                    } else {
-                        const float2* pA = reinterpret_cast<const float2*>(pos + it.id.x);
-                        const float2* pB = reinterpret_cast<const float2*>(pos + it.id.y);
-                        float2 A = it.dep.y ? loadCoherent2(pA) : *pA;
-                        float2 B = it.dep.w ? loadCoherent2(pB) : *pB;
-                        if (positionStep(it, A, B)) {
-                            if (it.dep.y) storeCoherent2(reinterpret_cast<float2*>(pos + it.id.x), A);
-                            if (it.dep.w) storeCoherent2(reinterpret_cast<float2*>(pos + it.id.y), B);
-                        }
                     }
                     orderStores();
-                    if (it.dep.y) storeFlag(flow + it.id.x, eA + 1);
-                    if (it.dep.w) storeFlag(flow + it.id.y, eB + 1);
-                    k++;
-                    if ((uint64_t)tid + (uint64_t)k * T >= M) { k = 0; q++; active = q < qEnd; }
+                    if (track) {
+                        for (uint32_t off = SOLVE_GROUP / 2; off > 0; off >>= 1) {
+                            dsq += __shfl_xor(dsq, off, SOLVE_GROUP);
+                            lsq += __shfl_xor(lsq, off, SOLVE_GROUP);
+                        }
+                        if (leader) { atomicAdd(&res[qv], dsq); atomicAdd(&res[MAX_SWEEPS + qv], lsq); }
+                    }
+                    if (leader) storeFlag(vDone + groupId, qv * P + min(groupBase + kv * T + SOLVE_GROUP, P));
+                    vReady = false;
+                    kv++;
+                    if ((uint64_t)groupBase + (uint64_t)kv * T >= P) { kv = 0; qv++; vAct = qv < vEnd; }
                     progressed = true;
                 }
             }
-            if (__all(!active)) break;
+            if (pAct) {
+                const uint32_t c = tid + kp * T;
+                bool has = c < P;
+                if (has && c != pLoaded) { loadPosItem(cs, c, pi); pLoaded = c; }
+                has = has && pi.count != 0;
+                if (has && !pReady) pReady = predecessorsDone(pDone, pi.dep.x, pi.dep.y, qp, P, T);
+                if (groupAll(!has || pReady)) {
+                    if (has) {
+                        float2* pA = reinterpret_cast<float2*>(pos + pi.body.x);
+                        float2* pB = reinterpret_cast<float2*>(pos + pi.body.y);
+                        float2 A = pi.dep.z ? loadCoherent2(pA) : *pA;
+                        float2 B = pi.dep.w ? loadCoherent2(pB) : *pB;
+                        positionStep(pi, A, B);
+                        if (pi.dep.z) storeCoherent2(pA, A);
+                        if (pi.dep.w) storeCoherent2(pB, B);
+                    }
+                    orderStores();
+                    if (leader) storeFlag(pDone + groupId, qp * P + min(groupBase + kp * T + SOLVE_GROUP, P));
+                    pReady = false;
+                    kp++;
+                    if ((uint64_t)groupBase + (uint64_t)kp * T >= P) { kp = 0; qp++; pAct = qp < pEnd; }
+                    progressed = true;
+                }
+            }
+            if (__all(!vAct && !pAct)) break;
             if (!__any(progressed)) {
                 if (wall_clock64() > deadline) { atomicOr(&ctr->overflow, OVERFLOW_STALL); return false; }
                 __builtin_amdgcn_s_sleep(1);
@@ -553,11 +696,11 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
         return true;
     };
     float* myRes = blockRes + (size_t)blockIdx.x * 2 * MAX_SWEEPS;
+    uint32_t sweepsDone = vIt;
     if (residualTarget > 0.0f) {
         uint32_t gen = 0;
-        uint32_t sweeps = vIt;
         for (uint32_t s = 0; s < vIt; s++) {
-            const bool good = run(s, s + 1);
+            const bool good = run(s, s + 1, 0, 0);
             if (__syncthreads_or(!good)) break;
             if (threadIdx.x == 0) { storeCoherentF(myRes + s, res[s]); storeCoherentF(myRes + MAX_SWEEPS + s, res[MAX_SWEEPS + s]); }
             if (!gridBarrier(&ctr->barrier[1], gridDim.x, gen, deadline)) {
@@ -575,17 +718,16 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
             num = red[0][0] + red[0][1] + red[0][2] + red[0][3];
             den = red[1][0] + red[1][1] + red[1][2] + red[1][3];
             __syncthreads();
-            if (sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12)) { sweeps = s + 1; break; }
+            if (sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12)) { sweepsDone = s + 1; break; }
         }
-        vEnd = sweeps;
-        run(vEnd, vEnd + pIt);
+        run(0, 0, 0, pIt);
     } else {
-        run(0, vIt + pIt);
+        run(0, vIt, 0, pIt);
     }
     __syncthreads();
     if (track)
-        for (uint32_t q = threadIdx.x; q < vEnd; q += blockDim.x) { myRes[q] = res[q]; myRes[MAX_SWEEPS + q] = res[MAX_SWEEPS + q]; }
-    if (tid == 0) ctr->sweeps = vEnd;
+        for (uint32_t q = threadIdx.x; q < sweepsDone; q += blockDim.x) { myRes[q] = res[q]; myRes[MAX_SWEEPS + q] = res[MAX_SWEEPS + q]; }
+    if (tid == 0) ctr->sweeps = sweepsDone;
 }
 
 // Accumulators back to the manifolds (the warm-start source of the next step) and
@@ -594,11 +736,12 @@ __global__ __launch_bounds__(256) void kSolveFinish(const Counters* __restrict__
                                                     const float4* __restrict__ vel, const float4* __restrict__ pos,
                                                     agx_manifold* __restrict__ man, float* __restrict__ errPartial,
                                                     int trackErrors) {
-    const uint32_t M = ctr->manifolds;
+    const uint32_t P = ctr->colourStart[MAX_COLOURS];
     float app = 0.0f, pen = 0.0f;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < M; c += gridDim.x * blockDim.x) {
-        const float4 j = cs.imp[c];
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < P; c += gridDim.x * blockDim.x) {
         const uint4 id = cs.idx[c];
+        if (id.z == 0) continue;
+        const float4 j = cs.imp[c];
         agx_manifold& m = man[id.w];
         m.jn0 = j.x; m.jt0 = j.y;
         if (id.z > 1) { m.jn1 = j.z; m.jt1 = j.w; }
@@ -625,27 +768,40 @@ __global__ __launch_bounds__(256) void kSolveFinish(const Counters* __restrict__
     }
 }
 
-// One block: residual per sweep and error totals from the per-block partials.
+// One block: residual per sweep and error totals from the per-block partials (fixed order).
+__device__ inline double blockSum(double v, double* scratch) {
+    for (uint32_t off = WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, WAVE);
+    __syncthreads();
+    if ((threadIdx.x & (WAVE - 1)) == 0) scratch[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (uint32_t k = 0; k < blockDim.x / WAVE; k++) t += scratch[k];
+    return t;
+}
+
 __global__ __launch_bounds__(256) void kSolveReduce(Counters* __restrict__ ctr, const float* __restrict__ blockRes,
                                                     uint32_t resBlocks, const float* __restrict__ errPartial,
                                                     uint32_t errBlocks, int track, int trackErrors) {
-    const uint32_t sweeps = ctr->sweeps;
+    __shared__ double scratch[4];
+    const uint32_t sweeps = min(ctr->sweeps, MAX_SWEEPS);
     if (track) {
-        for (uint32_t q = threadIdx.x; q < sweeps && q < MAX_SWEEPS; q += blockDim.x) {
+        for (uint32_t q = 0; q < sweeps; q++) {
             double num = 0.0, den = 0.0;
-            for (uint32_t j = 0; j < resBlocks; j++) {
+            for (uint32_t j = threadIdx.x; j < resBlocks; j += blockDim.x) {
                 num += (double)blockRes[(size_t)j * 2 * MAX_SWEEPS + q];
                 den += (double)blockRes[(size_t)j * 2 * MAX_SWEEPS + MAX_SWEEPS + q];
             }
-            ctr->residualNum[q] = num;
-            ctr->residualDen[q] = den;
+            num = blockSum(num, scratch);
+            den = blockSum(den, scratch);
+            if (threadIdx.x == 0) { ctr->residualNum[q] = num; ctr->residualDen[q] = den; }
         }
     }
-    if (trackErrors && threadIdx.x == 0) {
+    if (trackErrors) {
         double a = 0.0, p = 0.0;
-        for (uint32_t j = 0; j < errBlocks; j++) { a += errPartial[2 * j]; p += errPartial[2 * j + 1]; }
-        ctr->velocityError = (float)a;
-        ctr->positionError = (float)p;
+        for (uint32_t j = threadIdx.x; j < errBlocks; j += blockDim.x) { a += errPartial[2 * j]; p += errPartial[2 * j + 1]; }
+        a = blockSum(a, scratch);
+        p = blockSum(p, scratch);
+        if (threadIdx.x == 0) { ctr->velocityError = (float)a; ctr->positionError = (float)p; }
     }
 }
 
@@ -733,25 +889,27 @@ void dataflowBlocksPerCU(int& colour, int& solve) {
 hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
     const uint32_t gm = blocksForManifolds(w);
     const uint32_t gd = blocksFor(std::max(1u, w.nDynBodies));
-    hipLaunchKernelGGL(kFlowClear, dim3(blocksFor(w.N)), dim3(256), 0, st, w.N, w.degree, w.incFill, w.flow, w.used);
+    hipLaunchKernelGGL(kFlowClear, dim3(blocksFor(w.N)), dim3(256), 0, st, w.N, w.degree, w.incFill, w.colRec);
     hipLaunchKernelGGL(kDegree, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody, w.inv, w.manBodies, w.degree);
     hipError_t e = launchScan(st, w.degree, w.incStart, nullptr, &w.ctr->scratch[4], w.N, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kIncident, dim3(gm), dim3(256), 0, st, w.ctr, w.manBodies, w.inv, w.incStart, w.incFill, w.inc);
     hipLaunchKernelGGL(kRankByPriority, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.inc,
                        w.rank);
-    void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.flow, (void*)&w.used,
+    void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.colRec,
                      (void*)&w.colourId };
     e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kColourFlow), dim3(w.colourBlocks), dim3(256), args, 0, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
-    hipLaunchKernelGGL(kColourScan, dim3(1), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
+    hipLaunchKernelGGL(kColourColumnScan, dim3(MAX_COLOURS), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
+    hipLaunchKernelGGL(kColourStarts, dim3(1), dim3(64), 0, st, w.ctr);
     hipLaunchKernelGGL(kColourScatter, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist, w.order,
                        w.posOf);
+    hipLaunchKernelGGL(kMarkHoles, dim3((MAX_COLOURS * COLOUR_ALIGN + 255) / 256), dim3(256), 0, st, w.ctr, w.order, w.cs);
     hipLaunchKernelGGL(kSolveRank, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.inc,
-                       w.posOf, w.rank, w.incSorted, w.flow);
+                       w.posOf, w.rank, w.incSorted);
     hipLaunchKernelGGL(kPrep, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
-                       w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
+                       w.incStart, w.incSorted, w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
     hipLaunchKernelGGL(kWarm, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.incSorted,
                        w.posOf, w.cs, w.vel);
     return hipGetLastError();
@@ -760,9 +918,15 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                        float residualTarget, int trackErrors) {
     vIt = std::min(vIt, MAX_SWEEPS);
-    void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&w.flow, (void*)&w.blockRes,
-                     (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
+    // wave progress markers count sweep * positions: keep them within 32 bits
+    if ((uint64_t)(std::max(vIt, pIt) + 1) * w.solveCap >= (1ull << 32)) return hipErrorInvalidValue;
+    uint32_t* vDone = w.groupDone;
+    uint32_t* pDone = w.groupDone + w.solveGroups;
+    void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&vDone, (void*)&pDone,
+                     (void*)&w.blockRes, (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
     hipError_t e = launchFillU32(st, &w.ctr->barrier[1], 0u, 1);
+    if (e != hipSuccess) return e;
+    e = launchFillU32(st, w.groupDone, 0u, 2ull * w.solveGroups);
     if (e != hipSuccess) return e;
     e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kSolveFlow), dim3(w.solveBlocks), dim3(256), args, 0, st);
     if (e != hipSuccess) return e;

@@ -494,13 +494,18 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     d.hashCapMax = 16;
     while (d.hashCapMax < 2 * d.manCap) d.hashCapMax <<= 1;
     AL(d.hashKeys, d.hashCapMax); AL(d.hashVals, d.hashCapMax);
-    AL(d.manBodies, pairCap); AL(d.degree, d.N); AL(d.incStart, d.N); AL(d.incFill, d.N); AL(d.flow, d.N);
-    AL(d.inc, 2 * pairCap); AL(d.incSorted, 2 * pairCap); AL(d.rank, 2 * pairCap); AL(d.used, 2ull * d.N);
+    AL(d.manBodies, pairCap); AL(d.degree, d.N); AL(d.incStart, d.N); AL(d.incFill, d.N);
+    AL(d.inc, 2 * pairCap); AL(d.incSorted, 2 * pairCap); AL(d.rank, 2 * pairCap);
+    AL(d.colRec, 4ull * d.N);
+    d.solveGroups = (uint32_t)d.solveBlocks * 256u;   // enough for any group size down to one lane
+    AL(d.groupDone, 2ull * d.solveGroups);
     AL(d.colourId, pairCap); AL(d.colourHist, (uint64_t)agx::COLOUR_BLOCKS * agx::MAX_COLOURS);
-    AL(d.posOf, pairCap); AL(d.order, pairCap);
+    const uint64_t solveCap = pairCap + (uint64_t)agx::COLOUR_ALIGN * agx::MAX_COLOURS;
+    d.solveCap = solveCap;
+    AL(d.posOf, pairCap); AL(d.order, solveCap);
     AL(d.blockRes, (uint64_t)d.solveBlocks * 2 * agx::MAX_SWEEPS); AL(d.errPartial, 2 * agx::SOLVE_FINISH_BLOCKS);
-    AL(d.cs.idx, pairCap); AL(d.cs.dep, pairCap); AL(d.cs.nrm, pairCap); AL(d.cs.inv, pairCap); AL(d.cs.r0, pairCap);
-    AL(d.cs.r1, pairCap); AL(d.cs.k0, pairCap); AL(d.cs.k1, pairCap); AL(d.cs.imp, pairCap);
+    AL(d.cs.idx, solveCap); AL(d.cs.dep, solveCap); AL(d.cs.nrm, solveCap); AL(d.cs.inv, solveCap); AL(d.cs.r0, solveCap);
+    AL(d.cs.r1, solveCap); AL(d.cs.k0, solveCap); AL(d.cs.k1, solveCap); AL(d.cs.imp, solveCap);
 #undef UP
 #undef AL
     // Initial state: island ids, CCD hit times, empty contact cache, refreshSpatialData, trees.
@@ -760,13 +765,23 @@ int agx_world_read_solve_order(agx_world* w, uint32_t* order, uint64_t cap, uint
     agx::Counters* c = new agx::Counters;
     if (hipMemcpy(c, w->d.ctr, sizeof *c, hipMemcpyDeviceToHost) != hipSuccess) { delete c; return fail(w, AGX_ERR_HIP, "read counters"); }
     const uint32_t K = c->colours;
-    const uint32_t total = K ? c->colourStart[K] : 0u;
     *colours = K;
     if (K + 1 > colour_cap) { delete c; return fail(w, AGX_ERR_CAPACITY, "%u colours exceed cap", K); }
-    if (colour_start) std::memcpy(colour_start, c->colourStart, (K + 1) * 4u);
+    // Device positions are wave-aligned per colour; the exported order drops the padding holes.
+    std::vector<uint32_t> start(K + 1), count(K);
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < K; k++) { start[k] = total; count[k] = c->colourCount[k]; total += count[k]; }
+    start[K] = total;
+    const uint32_t padded = K ? c->colourStart[K] : 0u;
+    std::vector<uint32_t> deviceStart(c->colourStart, c->colourStart + K + 1);
     delete c;
+    if (colour_start) std::memcpy(colour_start, start.data(), (K + 1) * 4u);
     if (total > cap) return fail(w, AGX_ERR_CAPACITY, "order needs %u entries", total);
-    if (total && order) HIP_CHECK(w, hipMemcpy(order, w->d.order, total * 4u, hipMemcpyDeviceToHost));
+    if (total && order) {
+        std::vector<uint32_t> dev(padded);
+        HIP_CHECK(w, hipMemcpy(dev.data(), w->d.order, padded * 4u, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < K; k++) std::memcpy(order + start[k], dev.data() + deviceStart[k], count[k] * 4u);
+    }
     return AGX_OK;
 }
 
