@@ -54,13 +54,39 @@ def algorithmic_bytes(segment, st, counts, settings, mean_poly_verts):
         "tree_build": (24.0 + 16.0 * passes + 36.0 + 84.0) * n_dyn_cols,
         "pair_traversal": 16.0 * n_dyn_cols + 8.0 * p,
         "narrow": (48.0 + 2 * 8.0 * mean_poly_verts) * p + (64.0 + 32.0) * m,
-        "colour_prep": 290.0 * m,
-        "solve": m * (156.0 + 180.0 * sweeps + 64.0 * settings.position_iterations + 32.0),
+        # constraint prep 290 B + warm start 156 B per manifold; colouring 8 B per manifold and pass
+        "colour_prep": (290.0 + 156.0 + 8.0) * m,
+        # velocity sweeps 180 B and position sweeps 64 B per manifold and sweep
+        "solve": m * (180.0 * sweeps + 64.0 * settings.position_iterations),
+        "solve_finish": 32.0 * m,
         "islands": 16.0 * m + 8.0 * n_dyn_bodies,
         "finish": 48.0 * m,
         "post_update": 40.0 * n_dyn_cols,
     }
     return table.get(segment, 0.0)
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    n = 128 * 1024 * 1024                      # 512 MiB of float32
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copied {n * 4} bytes three times")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed PMC summary, if one exists."""
+    path = ROOT / "profiles" / f"{kernel}_pile65536_hbm_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def cpu_baseline(agx, scene, world, budget_s):
@@ -100,7 +126,12 @@ def main():
     ap.add_argument("--workload", default="pile65536", choices=["pile65536", "pile2560"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 512 MiB device copy three times (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
               f"in DESIGN.md and is not run.", file=sys.stderr)
@@ -123,13 +154,12 @@ def main():
     world.sync()
     log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.1f} s")
 
-    world.set_profiling(True)
-    world.profile()
     try:
         import torch
         sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync_torch = lambda: None  # noqa: E731
+    # timed region: plain stepping, no profiling events
     world.sync()
     sync_torch()
     t0 = time.perf_counter()
@@ -137,13 +167,20 @@ def main():
     world.sync()
     sync_torch()
     elapsed = time.perf_counter() - t0
-    prof = world.profile()
-    world.set_profiling(False)
     st = world.stats()
     n = world.counts.bodies
     value = n * args.steps / elapsed
     log(f"timed {args.steps} steps in {elapsed:.3f} s: {1e3 * elapsed / args.steps:.3f} ms/step, "
         f"{st.pairs} pairs, {st.manifolds} manifolds, {st.colours} colours, overflow={st.overflow}")
+    # profiled pass: per-segment GPU time from HIP events on the world's stream
+    prof_steps = max(10, min(args.steps, 50))
+    world.set_profiling(True)
+    world.profile()
+    world.step(dt, prof_steps)
+    world.sync()
+    prof = world.profile()
+    world.set_profiling(False)
+    st = world.stats()
 
     vcount = scene["bodies"]["vcount"]
     poly = vcount[vcount > 0]
@@ -154,7 +191,7 @@ def main():
     alg = algorithmic_bytes(dominant, st, world.counts, settings, mean_v)
     achieved = alg / (per_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant),
                 "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(per_launch_ms, 4)}
     residual = [float(st.residual[k]) for k in range(st.velocity_sweeps)]
     out = {
@@ -167,6 +204,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "segments_ms_per_step": {k: round(v, 4) for k, v in seg_ms.items()},
+        "segments_profiled_steps": prof_steps,
         "step_stats": {"pairs": int(st.pairs), "manifolds": int(st.manifolds), "contacts": int(st.contacts),
                        "colours": int(st.colours), "islands": int(st.islands),
                        "residual_per_sweep": [round(r, 6) for r in residual],

@@ -298,8 +298,12 @@ hipError_t launchNarrow(hipStream_t st, DeviceWorld& w);
 hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w);
 
 hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
+// Colouring, solve order, constraint prep, warm start (and the solve kernel's cleared flags).
+// launchSolve: the velocity and position sweeps (one kernel); launchSolveFinish: accumulators
+// back to the manifolds, residual and error totals.
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
-                       float residualTarget, int trackErrors);
+                       float residualTarget);
+hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors);
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
 // Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels.

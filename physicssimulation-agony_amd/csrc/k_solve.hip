@@ -912,24 +912,25 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
                        w.incStart, w.incSorted, w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
     hipLaunchKernelGGL(kWarm, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.incSorted,
                        w.posOf, w.cs, w.vel);
-    return hipGetLastError();
+    // cleared for the solve kernel: its grid-barrier counter and the group progress flags
+    e = launchFillU32(st, &w.ctr->barrier[1], 0u, 1);
+    if (e != hipSuccess) return e;
+    return launchFillU32(st, w.groupDone, 0u, 2ull * w.solveGroups);
 }
 
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
-                       float residualTarget, int trackErrors) {
+                       float residualTarget) {
     vIt = std::min(vIt, MAX_SWEEPS);
-    // wave progress markers count sweep * positions: keep them within 32 bits
+    // group progress markers count sweep * positions: keep them within 32 bits
     if ((uint64_t)(std::max(vIt, pIt) + 1) * w.solveCap >= (1ull << 32)) return hipErrorInvalidValue;
     uint32_t* vDone = w.groupDone;
     uint32_t* pDone = w.groupDone + w.solveGroups;
     void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&vDone, (void*)&pDone,
                      (void*)&w.blockRes, (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
-    hipError_t e = launchFillU32(st, &w.ctr->barrier[1], 0u, 1);
-    if (e != hipSuccess) return e;
-    e = launchFillU32(st, w.groupDone, 0u, 2ull * w.solveGroups);
-    if (e != hipSuccess) return e;
-    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kSolveFlow), dim3(w.solveBlocks), dim3(256), args, 0, st);
-    if (e != hipSuccess) return e;
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kSolveFlow), dim3(w.solveBlocks), dim3(256), args, 0, st);
+}
+
+hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors) {
     hipLaunchKernelGGL(kSolveFinish, dim3(FINISH_BLOCKS), dim3(256), 0, st, w.ctr, w.cs, w.vel, w.pos, w.man[w.cur],
                        w.errPartial, trackErrors);
     const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;

@@ -17,11 +17,12 @@ struct agx_builder {
 
 namespace {
 enum Segment : uint32_t {
-    SEG_INTEGRATE, SEG_POSE, SEG_CCD, SEG_TREE, SEG_PAIRS, SEG_NARROW, SEG_COLOUR, SEG_SOLVE, SEG_ISLANDS,
-    SEG_FINISH, SEG_POST_UPDATE, SEG_COUNT
+    SEG_INTEGRATE, SEG_POSE, SEG_CCD, SEG_TREE, SEG_PAIRS, SEG_NARROW, SEG_COLOUR, SEG_SOLVE, SEG_SOLVE_FINISH,
+    SEG_ISLANDS, SEG_FINISH, SEG_POST_UPDATE, SEG_COUNT
 };
 const char* const kSegmentNames[SEG_COUNT] = { "integrate", "pose", "ccd", "tree_build", "pair_traversal", "narrow",
-                                               "colour_prep", "solve", "islands", "finish", "post_update" };
+                                               "colour_prep", "solve", "solve_finish", "islands", "finish",
+                                               "post_update" };
 struct EventPair { hipEvent_t a, b; uint32_t seg; };
 }  // namespace
 
@@ -217,9 +218,14 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
                 SegmentTimer t(w, SEG_COLOUR);
                 HIP_CHECK(w, agx::launchColourPrep(st, d));
             }
-            SegmentTimer t(w, SEG_SOLVE);
-            HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations, s.track_residual ? 1 : 0,
-                                          s.residual_target, s.track_errors ? 1 : 0));
+            {
+                SegmentTimer t(w, SEG_SOLVE);
+                HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations,
+                                              s.track_residual ? 1 : 0, s.residual_target));
+            }
+            SegmentTimer t(w, SEG_SOLVE_FINISH);
+            HIP_CHECK(w, agx::launchSolveFinish(st, d, s.track_residual ? 1 : 0, s.residual_target,
+                                                s.track_errors ? 1 : 0));
         }
         break;
     case AGX_PHASE_ISLANDS: {

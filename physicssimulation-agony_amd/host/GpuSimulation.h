@@ -1,0 +1,270 @@
+// GpuSimulation.h - C++ mirror of PS_AGONY2D::Simulation and its ObjectAPI over the agx C ABI.
+//
+// Header-only; links against libagx.so. Names, parameter structs and error behaviour follow the
+// reference so that code written against PS_AGONY2D::Simulation ports by changing the namespace:
+//   Simulation::update / getSimulationSettings / createMaterial / objects()   (Simulation.h:148-203)
+//   ObjectAPI::createCircle / createBox / createPolygon (+ *H), getBodyData    (ObjectAPI.h:34-109)
+//   BodyCreateParams, CircleCreateParams, BoxCreateParams, PolygonCreateParams (ObjectCreateParams.h:14-95)
+//   Material                                                                  (Types/Material.h:6-11)
+// Bad user input returns std::nullopt / false and writes a message to std::cerr, like the
+// reference's ObjectManager (ObjectManager.cpp:92-97).
+//
+// Differences a caller must know: the scene is uploaded to the GPU at the first update() (or an
+// explicit upload()); objects are created before that point, and creation afterwards returns
+// std::nullopt. Read-outs copy device state to host vectors (the reference returns non-owning views).
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <iostream>
+#include <optional>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/agx.h"
+
+namespace PS_AGONY2D_GPU {
+
+using Real = float;
+using ObjectDenseIndex = uint32_t;
+using MaterialIndex = uint32_t;
+using Handle = uint32_t;   // the dense index at creation; bodies are never destroyed here
+
+struct Vec2 {
+    Real x = 0, y = 0;
+};
+
+// Types/Material.h:6-11 (default elasticity 1).
+struct Material {
+    Real elasticity = 1.0f;
+    Real staticFriction = 0.0f;
+    Real dynamicFriction = 0.0f;
+};
+
+// ObjectCreateParams.h:14-24. Mass 0 makes a static body; inertia and centre of mass come from the
+// shape (a caller-supplied inertia is not used on this path).
+struct BodyCreateParams {
+    Vec2 position{};
+    Vec2 velocity{};
+    Real rotation = 0;
+    Real angularVelocity = 0;
+    Real mass = 0;
+    Real inertia = 0;
+    MaterialIndex materialIndex = 0;
+};
+struct CircleCreateParams { BodyCreateParams base; Real radius = 0; };
+struct BoxCreateParams { BodyCreateParams base; Vec2 size{}; };
+struct PolygonCreateParams { BodyCreateParams base; std::vector<Vec2> localVertices; };
+
+// BodySoA fields of one body (SoA/BodySoA.h:14-36).
+struct BodyData {
+    Vec2 centerOfMass{}, velocity{}, localCenterOfMass{};
+    Real rotation = 0, angularVelocity = 0, mass = 0, inertia = 0, invMass = 0, invInertia = 0;
+};
+
+// Host copies of the body arrays, indexed by dense body index.
+struct BodyArrays {
+    std::vector<float> comX, comY, rotation, velX, velY, angularVelocity, invMass;
+};
+
+class Simulation;
+
+class ObjectAPI {
+public:
+    explicit ObjectAPI(Simulation& s) : sim(s) {}
+    ObjectAPI(const ObjectAPI&) = delete;
+    ObjectAPI& operator=(const ObjectAPI&) = delete;
+
+    std::optional<ObjectDenseIndex> createCircle(const CircleCreateParams& p);
+    std::optional<ObjectDenseIndex> createBox(const BoxCreateParams& p);
+    std::optional<ObjectDenseIndex> createPolygon(const PolygonCreateParams& p);
+    std::optional<Handle> createCircleH(const CircleCreateParams& p) { return createCircle(p); }
+    std::optional<Handle> createBoxH(const BoxCreateParams& p) { return createBox(p); }
+    std::optional<Handle> createPolygonH(const PolygonCreateParams& p) { return createPolygon(p); }
+
+    std::optional<BodyData> getBodyData(ObjectDenseIndex index) const;
+    BodyArrays getBodies() const;
+
+private:
+    std::optional<ObjectDenseIndex> add(uint8_t type, const BodyCreateParams& b, float p0, float p1,
+                                        const std::vector<Vec2>* verts);
+    Simulation& sim;
+};
+
+class Simulation {
+public:
+    // Simulation.h:50-76 defaults (6 velocity / 3 position iterations, CCD on, 1/300 s interval).
+    using SimulationSettings = agx_settings;
+
+    explicit Simulation(uint32_t scenes = 1) : api(*this) {
+        agx_settings_default(&settings);
+        if (agx_builder_create(scenes, &builder) != AGX_OK) fail("agx_builder_create", agx_last_error());
+    }
+    ~Simulation() {
+        if (world) agx_world_destroy(world);
+        if (builder) agx_builder_destroy(builder);
+    }
+    Simulation(const Simulation&) = delete;
+    Simulation& operator=(const Simulation&) = delete;
+
+    // Settings take effect at the next update().
+    SimulationSettings& getSimulationSettings() noexcept { return settings; }
+
+    std::optional<MaterialIndex> createMaterial(const Material& m) {
+        if (world) { fail("createMaterial", "the scene is already on the GPU"); return std::nullopt; }
+        uint32_t idx = 0;
+        if (agx_builder_add_material(builder, m.elasticity, m.staticFriction, m.dynamicFriction, &idx) != AGX_OK) {
+            fail("createMaterial", agx_last_error());
+            return std::nullopt;
+        }
+        return idx;
+    }
+
+    ObjectAPI& objects() noexcept { return api; }
+
+    // Uploads the scene (Simulation::refreshSpatialData runs here). Called by the first update().
+    bool upload() {
+        if (world) return true;
+        if (agx_world_create(builder, &settings, &world) != AGX_OK) {
+            fail("upload", agx_last_error());
+            world = nullptr;
+            return false;
+        }
+        return true;
+    }
+
+    // Simulation::update (Simulation.cpp:32-74): fixed-interval sub-steps, then the pose refresh.
+    uint32_t update(Real dt) {
+        if (!upload()) return 0;
+        agx_world_set_settings(world, &settings);
+        uint32_t steps = 0;
+        if (agx_world_update(world, dt, &steps) != AGX_OK) fail("update", agx_world_last_error(world));
+        return steps;
+    }
+
+    // Exactly n physics steps of dt (a clock that yields n steps), then the pose refresh.
+    bool step(Real dt, uint32_t n) {
+        if (!upload()) return false;
+        agx_world_set_settings(world, &settings);
+        if (agx_world_step(world, dt, n) != AGX_OK || agx_world_sync(world) != AGX_OK) {
+            fail("step", agx_world_last_error(world));
+            return false;
+        }
+        return true;
+    }
+
+    // Simulation::getBodyCollisionData (Simulation.h:186): the last step's solved manifolds.
+    std::vector<agx_manifold> getBodyCollisionData() const {
+        std::vector<agx_manifold> out;
+        if (!world) return out;
+        uint64_t count = 0;
+        agx_world_read_manifolds(world, 0, nullptr, 0, &count);
+        out.resize(count);
+        if (count && agx_world_read_manifolds(world, 0, out.data(), count, &count) != AGX_OK) out.clear();
+        return out;
+    }
+
+    // Simulation::islands (Simulation.h:203): island id per body (UINT32_MAX for static bodies).
+    std::pair<std::vector<uint32_t>, uint32_t> islands() const {
+        std::vector<uint32_t> ids;
+        uint32_t n = 0;
+        if (!world) return { ids, 0 };
+        ids.resize(counts().bodies);
+        agx_world_read_islands(world, ids.data(), (uint32_t)ids.size(), &n);
+        return { ids, n };
+    }
+
+    agx_stats getStats() const {
+        agx_stats st{};
+        if (world) agx_world_read_stats(world, &st);
+        return st;
+    }
+
+    agx_counts counts() const {
+        agx_counts c{};
+        if (world) agx_world_counts(world, &c);
+        else agx_builder_scene_counts(builder, 0, &c);
+        return c;
+    }
+
+    agx_world* nativeWorld() noexcept { return world; }
+    agx_builder* nativeBuilder() noexcept { return builder; }
+
+    static void fail(const char* what, const char* why) {
+        std::cerr << "PS_AGONY2D_GPU::" << what << ": " << (why ? why : "error") << '\n';
+    }
+
+private:
+    friend class ObjectAPI;
+    agx_settings settings{};
+    agx_builder* builder = nullptr;
+    agx_world* world = nullptr;
+    ObjectAPI api;
+};
+
+inline std::optional<ObjectDenseIndex> ObjectAPI::add(uint8_t type, const BodyCreateParams& b, float p0, float p1,
+                                                      const std::vector<Vec2>* verts) {
+    if (sim.world) { Simulation::fail("create", "the scene is already on the GPU"); return std::nullopt; }
+    const uint32_t vcount = verts ? (uint32_t)verts->size() : 0u, vstart = 0;
+    std::vector<float> flat;
+    if (verts) for (const Vec2& v : *verts) { flat.push_back(v.x); flat.push_back(v.y); }
+    uint32_t out = UINT32_MAX;
+    const int rc = agx_builder_add_bodies(sim.builder, 0, 1, &type, &b.position.x, &b.position.y, &b.rotation,
+                                          &b.velocity.x, &b.velocity.y, &b.angularVelocity, &b.mass,
+                                          &b.materialIndex, &p0, &p1, &vstart, &vcount,
+                                          flat.empty() ? nullptr : flat.data(), &out);
+    if (rc < 0 || out == UINT32_MAX) {
+        Simulation::fail("create", rc < 0 ? agx_last_error() : "invalid or non-convex polygon");
+        return std::nullopt;
+    }
+    return out;
+}
+
+inline std::optional<ObjectDenseIndex> ObjectAPI::createCircle(const CircleCreateParams& p) {
+    return add(AGX_SHAPE_CIRCLE, p.base, p.radius, 0.0f, nullptr);
+}
+inline std::optional<ObjectDenseIndex> ObjectAPI::createBox(const BoxCreateParams& p) {
+    return add(AGX_SHAPE_BOX, p.base, p.size.x, p.size.y, nullptr);
+}
+inline std::optional<ObjectDenseIndex> ObjectAPI::createPolygon(const PolygonCreateParams& p) {
+    return add(AGX_SHAPE_POLYGON, p.base, 0.0f, 0.0f, &p.localVertices);
+}
+
+inline BodyArrays ObjectAPI::getBodies() const {
+    BodyArrays a;
+    const uint32_t n = sim.counts().bodies;
+    auto read = [&](uint32_t field, std::vector<float>& v) {
+        v.resize(n);
+        if (sim.world) agx_world_read_body_f32(sim.world, field, v.data(), n);
+        else agx_builder_read_body_f32(sim.builder, 0, field, v.data(), n);
+    };
+    read(AGX_BODY_COM_X, a.comX); read(AGX_BODY_COM_Y, a.comY); read(AGX_BODY_ROTATION, a.rotation);
+    read(AGX_BODY_VEL_X, a.velX); read(AGX_BODY_VEL_Y, a.velY); read(AGX_BODY_ANG_VEL, a.angularVelocity);
+    read(AGX_BODY_INV_MASS, a.invMass);
+    return a;
+}
+
+inline std::optional<BodyData> ObjectAPI::getBodyData(ObjectDenseIndex index) const {
+    const uint32_t n = sim.counts().bodies;
+    if (index >= n) return std::nullopt;
+    auto field = [&](uint32_t f) {
+        std::vector<float> v(n);
+        if (sim.world) agx_world_read_body_f32(sim.world, f, v.data(), n);
+        else agx_builder_read_body_f32(sim.builder, 0, f, v.data(), n);
+        return v[index];
+    };
+    BodyData d;
+    d.centerOfMass = { field(AGX_BODY_COM_X), field(AGX_BODY_COM_Y) };
+    d.velocity = { field(AGX_BODY_VEL_X), field(AGX_BODY_VEL_Y) };
+    d.localCenterOfMass = { field(AGX_BODY_LOCAL_COM_X), field(AGX_BODY_LOCAL_COM_Y) };
+    d.rotation = field(AGX_BODY_ROTATION);
+    d.angularVelocity = field(AGX_BODY_ANG_VEL);
+    d.mass = field(AGX_BODY_MASS);
+    d.inertia = field(AGX_BODY_INERTIA);
+    d.invMass = field(AGX_BODY_INV_MASS);
+    d.invInertia = field(AGX_BODY_INV_INERTIA);
+    return d;
+}
+
+}  // namespace PS_AGONY2D_GPU

@@ -1,0 +1,72 @@
+// Steps a small pile through the C++ mirror (GpuSimulation.h) the way a PS_AGONY2D program drives
+// Simulation: create a material and bodies, call update(), read manifolds, islands and bodies.
+// Exit status 0 when the run is sane (bodies stay finite and inside the box, contacts form).
+#include "GpuSimulation.h"
+
+#include <cmath>
+#include <cstdio>
+
+using namespace PS_AGONY2D_GPU;
+
+int main() {
+    Simulation sim;
+    auto& settings = sim.getSimulationSettings();
+    settings.velocity_iterations = 8;
+    settings.update_interval = 1.0f / 60.0f;
+    const auto mat = sim.createMaterial({ 0.0f, 0.6f, 0.6f });
+    if (!mat) return 2;
+    auto& objects = sim.objects();
+    // static box of 20 x 20 m
+    const Vec2 walls[4][2] = { { { 0, -0.5f }, { 21, 1 } }, { { 0, 20.5f }, { 21, 1 } },
+                               { { -10.5f, 10 }, { 1, 21 } }, { { 10.5f, 10 }, { 1, 21 } } };
+    for (const auto& w : walls) {
+        BoxCreateParams p;
+        p.base.position = w[0];
+        p.base.materialIndex = *mat;
+        p.size = w[1];
+        if (!objects.createBox(p)) return 2;
+    }
+    int made = 0;
+    for (int iy = 0; iy < 12; iy++)
+        for (int ix = 0; ix < 16; ix++) {
+            BodyCreateParams b;
+            b.position = { -8.5f + 1.1f * ix + 0.05f * (iy & 1), 1.0f + 1.1f * iy };
+            b.rotation = 0.3f * (float)(ix + iy);
+            b.mass = 1.0f;
+            b.materialIndex = *mat;
+            bool ok;
+            switch ((ix + iy) % 3) {
+            case 0: { CircleCreateParams p; p.base = b; p.radius = 0.45f; ok = objects.createCircle(p).has_value(); break; }
+            case 1: { BoxCreateParams p; p.base = b; p.size = { 0.8f, 0.6f }; ok = objects.createBox(p).has_value(); break; }
+            default: {
+                PolygonCreateParams p;
+                p.base = b;
+                for (int k = 0; k < 5; k++) p.localVertices.push_back({ 0.45f * std::cos(1.2566f * k), 0.45f * std::sin(1.2566f * k) });
+                ok = objects.createPolygon(p).has_value();
+            }
+            }
+            made += ok ? 1 : 0;
+        }
+    // an invalid polygon (collinear) is rejected like the reference does
+    PolygonCreateParams bad;
+    bad.base.mass = 1.0f;
+    bad.localVertices = { { 0, 0 }, { 1, 0 }, { 2, 0 } };
+    const bool rejected = !objects.createPolygon(bad).has_value();
+
+    uint32_t steps = 0;
+    for (int frame = 0; frame < 180; frame++) steps += sim.update(1.0f / 60.0f);
+    const auto manifolds = sim.getBodyCollisionData();
+    const auto [ids, islands] = sim.islands();
+    const BodyArrays bodies = objects.getBodies();
+    bool finite = true, inside = true;
+    for (size_t i = 0; i < bodies.comX.size(); i++) {
+        finite = finite && std::isfinite(bodies.comX[i]) && std::isfinite(bodies.comY[i]);
+        inside = inside && std::fabs(bodies.comX[i]) < 11.0f && bodies.comY[i] > -1.0f && bodies.comY[i] < 21.0f;
+    }
+    const agx_stats st = sim.getStats();
+    std::printf("bodies %d steps %u manifolds %zu islands %u colours %u overflow %u rejected %d\n", made, steps,
+                manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0);
+    const bool ok = made == 192 && steps == 180 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
+                    st.overflow == 0;
+    return ok ? 0 : 1;
+}
