@@ -806,23 +806,39 @@ __global__ __launch_bounds__(256) void kSolveReduce(Counters* __restrict__ ctr, 
 }
 
 // ---------------------------------------------------------------- islands
-// Lock-free union-find over contact edges between dynamic bodies: the larger root is hooked
-// under the smaller by compare-and-swap, so each component's root is its smallest body index.
-// Island ids number the roots in ascending order (IslandManager.cpp:36-80 numbers DSU roots
-// in index order; the partition is the same).
-__device__ inline uint32_t ufFind(uint32_t* parent, uint32_t x) {
-    uint32_t p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Union-find over contact edges between dynamic bodies where every link points from a larger
+// body index to a smaller one, so each component's root is its smallest body index. Island ids
+// number the roots in ascending order (IslandManager.cpp:36-80 numbers DSU roots in index order;
+// the partition is the same).
+//
+// Parents are followed with plain loads. Links are never removed and only point downwards, so a
+// stale read can miss a recent link but never invents one: a walk always ends at a real ancestor,
+// and a root it takes for current is checked by the compare-and-swap that hooks it, which fails
+// and returns the newer parent when that root was already hooked.
+__device__ inline uint32_t ufFind(const uint32_t* parent, uint32_t x) {
+    uint32_t p = parent[x];
     while (p != x) {
-        const uint32_t gp = __hip_atomic_load(parent + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         x = p;
-        p = gp;
+        p = parent[x];
     }
     return x;
 }
 
 __global__ __launch_bounds__(256) void kUfInit(uint32_t* parent, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) parent[i] = i;
+}
+
+// First links without any walk: each body points at its smallest-index dynamic neighbour below it.
+__global__ __launch_bounds__(256) void kUfLinkNeighbours(const agx_manifold* __restrict__ man,
+                                                         const Counters* __restrict__ ctr,
+                                                         const uint32_t* __restrict__ colBody,
+                                                         const float2* __restrict__ inv, uint32_t* parent) {
+    const uint32_t M = ctr->manifolds;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
+        if (!isDynamic(inv, a) || !isDynamic(inv, b)) continue;
+        atomicMin(parent + max(a, b), min(a, b));
+    }
 }
 
 __global__ __launch_bounds__(256) void kUfUnite(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
@@ -832,14 +848,13 @@ __global__ __launch_bounds__(256) void kUfUnite(const agx_manifold* __restrict__
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
         uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
         if (!isDynamic(inv, a) || !isDynamic(inv, b)) continue;
-        while (true) {
-            a = ufFind(parent, a);
-            b = ufFind(parent, b);
-            if (a == b) break;
+        a = ufFind(parent, a);
+        b = ufFind(parent, b);
+        while (a != b) {
             if (a > b) { const uint32_t t = a; a = b; b = t; }
             const uint32_t old = atomicCAS(parent + b, b, a);
             if (old == b) break;
-            b = old;
+            b = ufFind(parent, old);
         }
     }
 }
@@ -942,6 +957,8 @@ hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, 
 
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL(kUfInit, dim3(blocksFor(w.N)), dim3(256), 0, st, w.ufParent, w.N);
+    hipLaunchKernelGGL(kUfLinkNeighbours, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody,
+                       w.inv, w.ufParent);
     hipLaunchKernelGGL(kUfUnite, dim3(blocksFor((uint32_t)std::min<uint64_t>(w.manCap, 1u << 22))), dim3(256), 0, st,
                        w.man[w.cur], w.ctr, w.colBody, w.inv, w.ufParent);
     if (w.nDynBodies == 0) return launchFillU32(st, &w.ctr->islands, 0u, 1);
