@@ -50,6 +50,20 @@ constexpr uint32_t OVERFLOW_HASH = 8u;        // warm-start hash full (error)
 constexpr uint32_t OVERFLOW_STALL = 16u;      // a dataflow kernel hit its time limit (error)
 constexpr uint32_t OVERFLOW_ITEMS = 32u;      // more constraints than the dataflow kernels hold (error)
 
+// Both children of an internal node in one 64-byte record, so a traversal step is one load.
+// A leaf child has first == last (its sorted position), ref = its collider and body = the
+// collider's body; an internal child has first < last and ref = its node index.
+struct ChildInfo {
+    float4 box;
+    uint32_t first, last, ref, body;
+};
+struct NodeRecord {
+    ChildInfo c[2];
+};
+static_assert(sizeof(NodeRecord) == 64, "node record is one 64-byte line");
+
+constexpr uint32_t STAGE_PAIRS = 1024;   // pairs a traversal wave stages before the gather
+
 // One BVH over a fixed collider list (dynamic or static), Karras layout:
 // internal nodes [0, n-1), leaves [n-1, 2n-1); node 0 is the root (a leaf when n == 1).
 struct Tree {
@@ -65,6 +79,7 @@ struct Tree {
     uint2* range = nullptr;        // n-1: first, last sorted leaf
     uint32_t* parent = nullptr;    // 2n-1
     uint32_t* flags = nullptr;     // n-1 refit arrival counters
+    NodeRecord* rec = nullptr;     // n-1 internal-node records (written by the refit)
     uint32_t* leafBody = nullptr;  // n
     uint32_t* sceneStart = nullptr;  // S+1: sorted leaf range of each scene
     uint32_t* list = nullptr;      // n: colliders of this tree (ascending index)
@@ -125,6 +140,7 @@ struct DeviceWorld {
     uint64_t pairCap = 0;
     uint2* pairs = nullptr;
     uint32_t* waveCount = nullptr;   // per query wave (self + cross passes)
+    uint2* stagedPairs = nullptr;    // per query wave: STAGE_PAIRS pairs
     uint32_t* waveOffset = nullptr;
     uint32_t queryWaves = 0;
     // narrow phase

@@ -208,13 +208,35 @@ __global__ __launch_bounds__(256) void kLeaves(const uint32_t* __restrict__ vals
 }
 
 // Bottom-up refit (BvhTree.cpp:426-530 computes the same unions top-down in reverse order):
-// the second thread to reach a node merges its children. Node boxes written here use
-// agent-coherent stores, completed before the arrival counter is bumped.
-__global__ __launch_bounds__(256) void kRefit(uint32_t n, const uint2* __restrict__ child, const uint32_t* __restrict__ parent,
-                                              uint32_t* __restrict__ flags, float4* __restrict__ nodeAabb) {
+// the second thread to reach a node merges its children and writes the node's record (both
+// children's boxes, ranges and references). Node boxes written here use agent-coherent stores,
+// completed before the arrival counter is bumped.
+__device__ inline ChildInfo childInfo(uint32_t c, uint32_t leafBase, const float4& box, const uint2* range,
+                                      const uint32_t* vals, const uint32_t* leafBody) {
+    ChildInfo ci;
+    ci.box = box;
+    if (c >= leafBase) {
+        const uint32_t j = c - leafBase;
+        ci.first = ci.last = j;
+        ci.ref = vals[j];
+        ci.body = leafBody[j];
+    } else {
+        const uint2 r = range[c];
+        ci.first = r.x; ci.last = r.y;
+        ci.ref = c;
+        ci.body = 0xFFFFFFFFu;
+    }
+    return ci;
+}
+
+__global__ __launch_bounds__(256) void kRefit(uint32_t n, const uint2* __restrict__ child, const uint2* __restrict__ range,
+                                              const uint32_t* __restrict__ parent, const uint32_t* __restrict__ vals,
+                                              const uint32_t* __restrict__ leafBody, uint32_t* __restrict__ flags,
+                                              float4* __restrict__ nodeAabb, NodeRecord* __restrict__ rec) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n || n < 2) return;
-    uint32_t node = n - 1 + j;
+    const uint32_t leafBase = n - 1;
+    uint32_t node = leafBase + j;
     while (true) {
         const uint32_t p = parent[node];
         if (p == 0xFFFFFFFFu) break;
@@ -225,42 +247,46 @@ __global__ __launch_bounds__(256) void kRefit(uint32_t n, const uint2* __restric
         const float4 a = loadCoherent(nodeAabb + ch.x);
         const float4 b = loadCoherent(nodeAabb + ch.y);
         storeCoherent(nodeAabb + p, make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w)));
+        NodeRecord r;
+        r.c[0] = childInfo(ch.x, leafBase, a, range, vals, leafBody);
+        r.c[1] = childInfo(ch.y, leafBase, b, range, vals, leafBody);
+        rec[p] = r;
         node = p;
     }
 }
 
 // ---------------------------------------------------------------- pair traversal
 // One query per lane, one wave-uniform stack of internal nodes in LDS (packet traversal):
-// queries are consecutive Morton-sorted dynamic leaves, so a wave's paths overlap. A child is
-// visited when any lane's strict overlap test passes (ballot); leaf children emit pairs for
-// the passing lanes. SELF restricts a query to leaves after it (each pair once) and skips
-// pairs of one body. COUNT: per-wave totals. WRITE: pairs at the scanned wave offsets, in the
-// same deterministic order.
+// queries are consecutive Morton-sorted dynamic leaves, so a wave's paths overlap. A step loads
+// one node record (both children) at a wave-uniform address; a child is visited when any
+// lane's strict overlap test passes (ballot), and leaf children emit pairs for the passing
+// lanes. SELF restricts a query to leaves after it (each pair once) and skips pairs of one body.
+// STAGED: pairs go to the wave's staging area (up to STAGE_PAIRS) and the wave's total to
+// waveCount; the gather then moves them to their scanned place. Otherwise (a second pass run
+// only by waves whose total exceeded the staging area) pairs go straight to the scanned offsets.
+// Both passes emit in the same deterministic order.
 struct Query { float4 box; uint32_t leaf, col, body, sf, sl; bool valid; };
 
 template <bool SELF>
-__device__ inline bool childActive(const Query& q, const Tree& t, uint32_t node, float4& box) {
-    const uint32_t leafBase = t.n - 1;
-    uint32_t first, last;
-    if (node >= leafBase) { first = last = node - leafBase; }
-    else { const uint2 r = t.range[node]; first = r.x; last = r.y; }
-    box = t.nodeAabb[node];
+__device__ inline bool childActive(const Query& q, const float4& box, uint32_t first, uint32_t last) {
     if (!q.valid) return false;
     if (first >= q.sl || last < q.sf) return false;
     if (SELF && last <= q.leaf) return false;
     return q.box.x < box.z && q.box.z > box.x && q.box.y < box.w && q.box.w > box.y;
 }
 
-template <bool SELF, bool WRITE>
+template <bool SELF, bool STAGED>
 __global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_t* __restrict__ colScene,
                                                  uint32_t* __restrict__ waveCount, const uint32_t* __restrict__ waveOffset,
                                                  uint32_t waveBaseIndex, uint2* __restrict__ pairs, uint64_t cap,
-                                                 Counters* ctr) {
+                                                 uint2* __restrict__ staged, Counters* ctr) {
     __shared__ uint32_t stackMem[4][TRAVERSAL_STACK];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     uint32_t* stack = stackMem[wid];
     const uint32_t waveGlobal = blockIdx.x * 4 + wid;
     if (waveGlobal * 64 >= qt.n) return;   // whole wave past the last query (wave-uniform exit)
+    const uint32_t region = waveBaseIndex + waveGlobal;
+    if (!STAGED && waveCount[region] <= STAGE_PAIRS) return;   // already gathered (wave-uniform)
     const uint32_t i = waveGlobal * 64 + lane;
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     Query q;
@@ -273,50 +299,64 @@ __global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_
     q.sf = q.valid ? tt.sceneStart[scene] : 0u;
     q.sl = q.valid ? tt.sceneStart[scene + 1] : 0u;
     uint32_t written = 0;
-    const uint64_t outBase = WRITE ? (uint64_t)waveOffset[waveBaseIndex + waveGlobal] : 0ull;
-    const uint32_t leafBase = tt.n - 1;
-    auto emitLeaf = [&](uint32_t node, bool act) {
-        const uint32_t j = node - leafBase;
-        const uint32_t other = tt.vals[j];
-        if (SELF) act = act && tt.leafBody[j] != q.body;
+    const uint64_t outBase = STAGED ? 0ull : (uint64_t)waveOffset[region];
+    uint2* stage = staged + (size_t)region * STAGE_PAIRS;
+    auto emit = [&](uint32_t other, uint32_t otherBody, bool act) {
+        if (SELF) act = act && otherBody != q.body;
         const uint64_t m = __ballot(act);
-        if (WRITE && act) {
-            const uint64_t dst = outBase + written + (uint64_t)__popcll(m & ltMask);
-            if (dst < cap) pairs[dst] = make_uint2(q.col, other);
+        if (act) {
+            const uint32_t k = written + (uint32_t)__popcll(m & ltMask);
+            if (STAGED) {
+                if (k < STAGE_PAIRS) stage[k] = make_uint2(q.col, other);
+            } else {
+                const uint64_t dst = outBase + k;
+                if (dst < cap) pairs[dst] = make_uint2(q.col, other);
+            }
         }
         written += (uint32_t)__popcll(m);
     };
     if (tt.n > 0 && __ballot(q.valid) != 0) {
         uint32_t sp = 0;
-        float4 box;
-        const bool rootAct = childActive<SELF>(q, tt, 0, box);
-        if (tt.n == 1) {
-            emitLeaf(0, rootAct);
-        } else if (__ballot(rootAct) != 0) {
-            stack[sp++] = 0;
-        }
+        const bool rootAct = childActive<SELF>(q, tt.nodeAabb[0], 0u, tt.n - 1);
+        if (tt.n == 1) emit(tt.vals[0], tt.leafBody[0], rootAct);
+        else if (__ballot(rootAct) != 0) stack[sp++] = 0;
         while (sp > 0) {
             const uint32_t node = __builtin_amdgcn_readfirstlane(stack[sp - 1]);
             sp--;
-            const uint2 ch = tt.child[node];
-            float4 bl, br;
-            const bool al = childActive<SELF>(q, tt, ch.x, bl);
-            const bool ar = childActive<SELF>(q, tt, ch.y, br);
-            const bool leafL = ch.x >= leafBase, leafR = ch.y >= leafBase;
-            if (leafL) emitLeaf(ch.x, al);
-            if (leafR) emitLeaf(ch.y, ar);
+            const NodeRecord r = tt.rec[node];
+            const bool al = childActive<SELF>(q, r.c[0].box, r.c[0].first, r.c[0].last);
+            const bool ar = childActive<SELF>(q, r.c[1].box, r.c[1].first, r.c[1].last);
+            const bool leafL = r.c[0].first == r.c[0].last, leafR = r.c[1].first == r.c[1].last;
+            if (leafL) emit(r.c[0].ref, r.c[0].body, al);
+            if (leafR) emit(r.c[1].ref, r.c[1].body, ar);
             const bool pushR = !leafR && __ballot(ar) != 0;
             const bool pushL = !leafL && __ballot(al) != 0;
             if (sp + (pushR ? 1 : 0) + (pushL ? 1 : 0) > TRAVERSAL_STACK) {
-                if (lane == 0) atomicOr(&ctr->overflow, 2u);
+                if (lane == 0) atomicOr(&ctr->overflow, OVERFLOW_STACK);
                 break;
             }
-            if (pushR) { if (lane == 0) stack[sp] = ch.y; sp++; }
-            if (pushL) { if (lane == 0) stack[sp] = ch.x; sp++; }
+            if (pushR) { if (lane == 0) stack[sp] = r.c[1].ref; sp++; }
+            if (pushL) { if (lane == 0) stack[sp] = r.c[0].ref; sp++; }
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if (!WRITE && lane == 0) waveCount[waveBaseIndex + waveGlobal] = written;
+    if (STAGED && lane == 0) waveCount[region] = written;
+}
+
+// One wave per staging area: copies a wave's staged pairs to their scanned place.
+__global__ __launch_bounds__(256) void kGatherPairs(const uint32_t* __restrict__ waveCount,
+                                                    const uint32_t* __restrict__ waveOffset, uint32_t regions,
+                                                    const uint2* __restrict__ staged, uint2* __restrict__ pairs,
+                                                    uint64_t cap) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t region = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (region >= regions) return;
+    const uint32_t n = waveCount[region];
+    if (n > STAGE_PAIRS) return;   // re-traversed straight into place
+    const uint64_t base = waveOffset[region];
+    const uint2* src = staged + (size_t)region * STAGE_PAIRS;
+    for (uint32_t k = lane; k < n; k += 64)
+        if (base + k < cap) pairs[base + k] = src[k];
 }
 
 __global__ void kPairTotal(Counters* ctr, uint64_t cap, const uint32_t* total) {
@@ -354,7 +394,8 @@ hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t) {
     hipLaunchKernelGGL(kLeaves, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.vals, t.n, w.aabb, w.colBody, t.nodeAabb,
                        t.leafBody, t.parent);
     if (t.n > 1)
-        hipLaunchKernelGGL(kRefit, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.n, t.child, t.parent, t.flags, t.nodeAabb);
+        hipLaunchKernelGGL(kRefit, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.n, t.child, t.range, t.parent, t.vals,
+                           t.leafBody, t.flags, t.nodeAabb, t.rec);
     return hipGetLastError();
 }
 
@@ -364,17 +405,19 @@ hipError_t launchPairs(hipStream_t st, DeviceWorld& w) {
     }
     const uint32_t waves = w.queryWaves;          // ceil(nDyn / 64)
     const uint32_t blocks = blocksFor(waves, 4);
-    hipLaunchKernelGGL((kTraverse<true, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
-                       w.waveOffset, 0u, w.pairs, w.pairCap, w.ctr);
-    hipLaunchKernelGGL((kTraverse<false, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
-                       w.waveOffset, waves, w.pairs, w.pairCap, w.ctr);
+    hipLaunchKernelGGL((kTraverse<true, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
+                       w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+    hipLaunchKernelGGL((kTraverse<false, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
+                       w.waveOffset, waves, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     uint32_t* total = &w.ctr->scratch[0];
     hipError_t e = launchScan(st, w.waveCount, w.waveOffset, nullptr, total, 2ull * waves, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((kTraverse<true, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
-                       w.waveOffset, 0u, w.pairs, w.pairCap, w.ctr);
-    hipLaunchKernelGGL((kTraverse<false, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
-                       w.waveOffset, waves, w.pairs, w.pairCap, w.ctr);
+    hipLaunchKernelGGL(kGatherPairs, dim3(blocksFor(2 * waves, 4)), dim3(256), 0, st, w.waveCount, w.waveOffset, 2 * waves,
+                       w.stagedPairs, w.pairs, w.pairCap);
+    hipLaunchKernelGGL((kTraverse<true, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
+                       w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+    hipLaunchKernelGGL((kTraverse<false, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
+                       w.waveOffset, waves, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     hipLaunchKernelGGL(kPairTotal, dim3(1), dim3(1), 0, st, w.ctr, w.pairCap, total);
     return hipGetLastError();
 }

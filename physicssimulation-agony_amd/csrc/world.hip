@@ -122,6 +122,7 @@ int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, con
     HIP_CHECK(w, devAlloc(w, &t.range, internal));
     HIP_CHECK(w, devAlloc(w, &t.parent, nodes));
     HIP_CHECK(w, devAlloc(w, &t.flags, internal));
+    HIP_CHECK(w, devAlloc(w, &t.rec, internal));
     HIP_CHECK(w, devAlloc(w, &t.leafBody, n));
     HIP_CHECK(w, devAlloc(w, &t.sceneBounds, S));
     if (n) HIP_CHECK(w, hipMemcpy(t.vals, list.data(), n * 4, hipMemcpyHostToDevice));   // initial CCD order
@@ -135,6 +136,7 @@ int allocPairBuffers(agx_world* w, uint64_t pairCap) {
     HIP_CHECK(w, devAlloc(w, &d.pairs, pairCap));
     HIP_CHECK(w, devAlloc(w, &d.waveCount, 2ull * std::max(1u, d.queryWaves)));
     HIP_CHECK(w, devAlloc(w, &d.waveOffset, 2ull * std::max(1u, d.queryWaves)));
+    HIP_CHECK(w, devAlloc(w, &d.stagedPairs, 2ull * std::max(1u, d.queryWaves) * agx::STAGE_PAIRS));
     d.scanBlockCap = 4096;
     HIP_CHECK(w, devAlloc(w, &d.scanBlock, d.scanBlockCap));
     HIP_CHECK(w, devAlloc(w, &d.ctr, 1));
