@@ -17,7 +17,7 @@ constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataf
 constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour start on a wave boundary
 constexpr uint32_t MAX_SWEEPS = 64;
 constexpr uint32_t TRAVERSAL_STACK = 64;
-constexpr uint32_t SORT_TILE = 4096;        // keys per radix-sort block (256 threads x 16)
+constexpr uint32_t SORT_TILE = 1024;        // keys per radix-sort block (256 threads x 4)
 constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one scene per chunk)
 
 // Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).
@@ -73,7 +73,7 @@ struct Tree {
     uint32_t* vals = nullptr;      // sorted collider indices
     uint32_t* keysAlt = nullptr;
     uint32_t* valsAlt = nullptr;
-    uint32_t* hist = nullptr;      // 256 * sortBlocks
+    uint32_t* hist = nullptr;      // radix sort scratch: 4 x (sortBlocks + 1) x 256 + 4 words
     float4* nodeAabb = nullptr;    // 2n-1
     uint2* child = nullptr;        // n-1: unified child indices
     uint2* range = nullptr;        // n-1: first, last sorted leaf

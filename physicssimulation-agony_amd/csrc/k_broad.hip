@@ -86,36 +86,73 @@ __global__ __launch_bounds__(256) void kMorton(const uint32_t* __restrict__ list
 }
 
 // ---------------------------------------------------------------- radix sort (8-bit digits)
-// Pass 1: per-block digit histogram into hist[digit * blocks + block].
-__global__ __launch_bounds__(256) void kRadixHist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
-                                                  uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+// One histogram pass for every digit, one scan of the 4 x 256 digit totals, then one stable
+// scatter per digit. A scatter tile finds the keys of its digit in earlier tiles by decoupled
+// look-back: it publishes its own count per digit at once, then walks back over earlier tiles'
+// published counts until it meets a tile that has published its inclusive prefix. Tiles take
+// their index from a ticket counter, so a tile only ever waits for tiles that are running.
+// Layout of the sort scratch (cleared once per sort): tile status [passes][tiles][256], digit
+// totals [passes][256], tickets [passes].
+constexpr uint32_t SORT_PASSES_MAX = 4;
+constexpr uint32_t STATUS_AGG = 1u << 30, STATUS_INC = 2u << 30, STATUS_VALUE = (1u << 30) - 1;
+constexpr uint64_t SORT_STALL_TICKS = 200000000ull;   // 2 s of the 100 MHz wall clock
+
+__global__ __launch_bounds__(256) void kRadixHistAll(const uint32_t* __restrict__ keys, uint32_t n, uint32_t passes,
+                                                     uint32_t* __restrict__ digitTotal) {
+    __shared__ uint32_t h[SORT_PASSES_MAX][256];
+    for (uint32_t p = 0; p < SORT_PASSES_MAX; p++) h[p][threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * SORT_TILE;
     for (uint32_t k = threadIdx.x; k < SORT_TILE; k += 256) {
         const uint32_t i = base + k;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+        if (i >= n) break;
+        const uint32_t key = keys[i];
+        for (uint32_t p = 0; p < passes; p++) atomicAdd(&h[p][(key >> (8 * p)) & 0xFFu], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+    for (uint32_t p = 0; p < passes; p++)
+        if (h[p][threadIdx.x]) atomicAdd(&digitTotal[p * 256 + threadIdx.x], h[p][threadIdx.x]);
 }
 
-// Pass 3: stable scatter. Each wave ranks 16 rounds of 64 consecutive keys with ballot
-// matching (8 ballots per round) and per-wave digit counters in LDS; waves are combined in
-// order, then added to the scanned global digit offsets.
+// One block: exclusive scan of each pass's 256 digit totals, in place.
+__global__ __launch_bounds__(256) void kRadixDigitScan(uint32_t* __restrict__ digitTotal, uint32_t passes) {
+    __shared__ uint32_t waveSum[4];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t v = digitTotal[p * 256 + threadIdx.x];
+        uint32_t x = v;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) waveSum[wid] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < wid; k++) before += waveSum[k];
+        digitTotal[p * 256 + threadIdx.x] = before + x - v;
+        __syncthreads();
+    }
+}
+
+// Stable scatter of one digit. Each wave ranks 16 rounds of 64 consecutive keys with ballot
+// matching (8 ballots per round) and per-wave digit counters in LDS; waves are combined in order,
+// earlier tiles by look-back, earlier digits by the scanned totals.
 __global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                     uint32_t n, uint32_t shift, const uint32_t* __restrict__ histScan,
+                                                     uint32_t n, uint32_t shift, const uint32_t* __restrict__ digitStart,
+                                                     uint32_t* status, uint32_t* ticket, uint32_t* overflow,
                                                      uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
-    constexpr uint32_t ROUNDS = SORT_TILE / 256;   // 16 keys per lane
+    constexpr uint32_t ROUNDS = SORT_TILE / 256;   // keys per lane
     __shared__ uint32_t cnt[4][256];
     __shared__ uint32_t base[4][256];
+    __shared__ uint32_t tileShared;
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) tileShared = atomicAdd(ticket, 1u);
     for (uint32_t w = 0; w < 4; w++) cnt[w][threadIdx.x] = 0;
     __syncthreads();
+    const uint32_t tile = tileShared;
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t rk[ROUNDS], dg[ROUNDS], ky[ROUNDS], vl[ROUNDS];
-    const uint32_t waveBase = blockIdx.x * SORT_TILE + wid * (SORT_TILE / 4);
+    const uint32_t waveBase = tile * SORT_TILE + wid * (SORT_TILE / 4);
     for (uint32_t r = 0; r < ROUNDS; r++) {
         const uint32_t i = waveBase + r * 64 + lane;
         const bool valid = i < n;
@@ -139,7 +176,38 @@ __global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict_
     __syncthreads();
     {
         const uint32_t d = threadIdx.x;
-        uint32_t run = histScan[d * gridDim.x + blockIdx.x];
+        const uint32_t mine = cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
+        uint32_t* own = status + tile * 256 + d;
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            storeFlag(own, STATUS_INC | mine);
+        } else {
+            storeFlag(own, STATUS_AGG | mine);
+            // look back LOOKBACK tiles at a time (all loads in flight together)
+            constexpr int32_t LOOKBACK = 16;
+            const uint64_t deadline = wall_clock64() + SORT_STALL_TICKS;
+            int32_t j = (int32_t)tile - 1;
+            while (j >= 0) {
+                uint32_t stv[LOOKBACK];
+                for (int32_t k = 0; k < LOOKBACK; k++) stv[k] = j - k >= 0 ? loadFlag(status + (j - k) * 256 + d) : 0u;
+                int32_t k = 0;
+                bool found = false;
+                for (; k < LOOKBACK && j - k >= 0; k++) {
+                    const uint32_t st = stv[k];
+                    if (st & STATUS_INC) { prefix += st & STATUS_VALUE; found = true; break; }
+                    if (!(st & STATUS_AGG)) break;   // not published yet: look again from here
+                    prefix += st & STATUS_VALUE;
+                }
+                if (found) break;
+                j -= k;
+                if (k == 0) {
+                    if (wall_clock64() > deadline) { atomicOr(overflow, OVERFLOW_STALL); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            storeFlag(own, STATUS_INC | (prefix + mine));
+        }
+        uint32_t run = digitStart[d] + prefix;
         for (uint32_t w = 0; w < 4; w++) { base[w][d] = run; run += cnt[w][d]; }
     }
     __syncthreads();
@@ -377,13 +445,17 @@ hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t) {
                        reinterpret_cast<const int4*>(t.sceneBounds), w.mortonBits, t.keys, t.vals);
     const uint32_t keyBits = w.mortonBits + w.sceneBits;
     const uint32_t passes = (keyBits + 7) / 8;
+    // sort scratch: tile status, digit totals, tickets (see kRadixScatter)
+    uint32_t* status = t.hist;
+    uint32_t* digitTotal = status + (size_t)SORT_PASSES_MAX * t.sortBlocks * 256;
+    uint32_t* tickets = digitTotal + SORT_PASSES_MAX * 256;
+    hipError_t e = launchFillU32(st, t.hist, 0u, (size_t)SORT_PASSES_MAX * (t.sortBlocks + 1) * 256 + SORT_PASSES_MAX);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kRadixHistAll, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.n, passes, digitTotal);
+    hipLaunchKernelGGL(kRadixDigitScan, dim3(1), dim3(256), 0, st, digitTotal, passes);
     for (uint32_t p = 0; p < passes; p++) {
-        const uint32_t shift = p * 8;
-        hipLaunchKernelGGL(kRadixHist, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.n, shift, t.hist);
-        const uint64_t histN = 256ull * t.sortBlocks;
-        hipError_t e = launchScan(st, t.hist, t.hist, nullptr, nullptr, histN, w.scanBlock, w.scanBlockCap);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kRadixScatter, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, shift, t.hist,
+        hipLaunchKernelGGL(kRadixScatter, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, p * 8,
+                           digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p, &w.ctr->overflow,
                            t.keysAlt, t.valsAlt);
         std::swap(t.keys, t.keysAlt);
         std::swap(t.vals, t.valsAlt);

@@ -116,7 +116,7 @@ int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, con
     HIP_CHECK(w, devAlloc(w, &t.vals, n));
     HIP_CHECK(w, devAlloc(w, &t.keysAlt, n));
     HIP_CHECK(w, devAlloc(w, &t.valsAlt, n));
-    HIP_CHECK(w, devAlloc(w, &t.hist, 256ull * t.sortBlocks));
+    HIP_CHECK(w, devAlloc(w, &t.hist, 4ull * (t.sortBlocks + 1) * 256 + 4));   // radix sort scratch
     HIP_CHECK(w, devAlloc(w, &t.nodeAabb, nodes));
     HIP_CHECK(w, devAlloc(w, &t.child, internal));
     HIP_CHECK(w, devAlloc(w, &t.range, internal));
