@@ -77,20 +77,30 @@ __global__ __launch_bounds__(256) void kIncident(const Counters* __restrict__ ct
     }
 }
 
-// Rank of every list entry among its body's entries by manifold priority.
-__global__ __launch_bounds__(256) void kRankByPriority(const uint32_t* __restrict__ dynBodies, uint32_t nDyn,
-                                                       const uint32_t* __restrict__ degree,
-                                                       const uint32_t* __restrict__ incStart,
-                                                       const uint32_t* __restrict__ inc, uint32_t* __restrict__ rank) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nDyn; k += gridDim.x * blockDim.x) {
-        const uint32_t b = dynBodies[k], s = incStart[b], d = degree[b];
-        for (uint32_t i = 0; i < d; i++) {
-            const uint32_t e = inc[s + i];
-            const uint32_t pe = mix32(e >> 1);
-            uint32_t r = 0;
-            for (uint32_t j = 0; j < d; j++) r += mix32(inc[s + j] >> 1) < pe ? 1u : 0u;
-            rank[e] = r;
+// Rank of every list entry among its body's entries by a key: the manifold priority
+// (BY_POSITION false, for the colouring) or the solve position (BY_POSITION true, which also
+// writes the lists sorted by position). One thread per list entry.
+template <bool BY_POSITION>
+__global__ __launch_bounds__(256) void kRankEntries(const Counters* __restrict__ ctr, const uint2* __restrict__ manBodies,
+                                                    const uint32_t* __restrict__ degree,
+                                                    const uint32_t* __restrict__ incStart, const uint32_t* __restrict__ inc,
+                                                    const uint32_t* __restrict__ posOf, uint32_t* __restrict__ rank,
+                                                    uint32_t* __restrict__ incSorted) {
+    const uint32_t total = ctr->scratch[4];   // list entries (scan total of the degrees)
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+        const uint32_t e = inc[k];
+        const uint32_t m = e >> 1;
+        const uint2 mb = manBodies[m];
+        const uint32_t b = (e & 1u) ? mb.y : mb.x;
+        const uint32_t s = incStart[b], d = degree[b];
+        const uint32_t key = BY_POSITION ? posOf[m] : mix32(m);
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < d; j++) {
+            const uint32_t o = inc[s + j] >> 1;
+            r += (BY_POSITION ? posOf[o] : mix32(o)) < key ? 1u : 0u;
         }
+        rank[e] = r;
+        if (BY_POSITION) incSorted[s + r] = e;
     }
 }
 
@@ -276,25 +286,6 @@ __global__ __launch_bounds__(256) void kColourScatter(const Counters* __restrict
         const uint32_t p = atomicAdd(&cursor[colourId[m]], 1u);
         order[p] = m;
         posOf[m] = p;
-    }
-}
-
-// Rank of every list entry among its body's entries by solve position, and the lists sorted by
-// position.
-__global__ __launch_bounds__(256) void kSolveRank(const uint32_t* __restrict__ dynBodies, uint32_t nDyn,
-                                                  const uint32_t* __restrict__ degree, const uint32_t* __restrict__ incStart,
-                                                  const uint32_t* __restrict__ inc, const uint32_t* __restrict__ posOf,
-                                                  uint32_t* __restrict__ rank, uint32_t* __restrict__ incSorted) {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nDyn; k += gridDim.x * blockDim.x) {
-        const uint32_t b = dynBodies[k], s = incStart[b], d = degree[b];
-        for (uint32_t i = 0; i < d; i++) {
-            const uint32_t e = inc[s + i];
-            const uint32_t pe = posOf[e >> 1];
-            uint32_t r = 0;
-            for (uint32_t j = 0; j < d; j++) r += posOf[inc[s + j] >> 1] < pe ? 1u : 0u;
-            rank[e] = r;
-            incSorted[s + r] = e;
-        }
     }
 }
 
@@ -909,8 +900,9 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
     hipError_t e = launchScan(st, w.degree, w.incStart, nullptr, &w.ctr->scratch[4], w.N, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kIncident, dim3(gm), dim3(256), 0, st, w.ctr, w.manBodies, w.inv, w.incStart, w.incFill, w.inc);
-    hipLaunchKernelGGL(kRankByPriority, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.inc,
-                       w.rank);
+    const uint32_t ge = blocksFor((uint32_t)std::min<uint64_t>(2 * w.manCap, 1u << 22));
+    hipLaunchKernelGGL(kRankEntries<false>, dim3(ge), dim3(256), 0, st, w.ctr, w.manBodies, w.degree, w.incStart, w.inc,
+                       w.posOf, w.rank, w.incSorted);
     void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.colRec,
                      (void*)&w.colourId };
     e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kColourFlow), dim3(w.colourBlocks), dim3(256), args, 0, st);
@@ -921,7 +913,7 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL(kColourScatter, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist, w.order,
                        w.posOf);
     hipLaunchKernelGGL(kMarkHoles, dim3((MAX_COLOURS * COLOUR_ALIGN + 255) / 256), dim3(256), 0, st, w.ctr, w.order, w.cs);
-    hipLaunchKernelGGL(kSolveRank, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.inc,
+    hipLaunchKernelGGL(kRankEntries<true>, dim3(ge), dim3(256), 0, st, w.ctr, w.manBodies, w.degree, w.incStart, w.inc,
                        w.posOf, w.rank, w.incSorted);
     hipLaunchKernelGGL(kPrep, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
                        w.incStart, w.incSorted, w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
