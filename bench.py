@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the collide-and-solve path on one MI355X.
 
-Prints one JSON line. A step is one full physics step (integrate, poses/AABBs, CCD, LBVH
-broad phase, SAT narrow phase with warm start, colouring + sequential-impulse solve, islands)
-plus the post-update pose pass, i.e. Simulation::update(1/60) with a 1/60 s fixed interval.
-Workload (default): the 65,536-body pile (70 % convex polygons with 3-8 vertices, 30 % circles,
-lognormal(ln 0.3, 0.3) radii, densities U[0.5, 2], 512 x 256 m static box, friction 0.6,
-restitution 0, dt 1/60, 8 velocity + 3 position iterations), settled by the warm-up steps.
-``value`` is body-steps/s with the bodies already resident in HBM.
+Prints one JSON line. A step is one full physics step (integrate, poses/AABBs, CCD, LBVH broad
+phase, SAT narrow phase with warm start, colouring + sequential-impulse solve, islands) plus the
+post-update pose pass, i.e. Simulation::update(1/60) with a 1/60 s fixed interval. Bodies are
+resident in HBM when a timed region starts.
+
+Headline (``value``): the batch of 1,024 independent scenes (log-uniform 64-4,096 bodies; box
+pyramids, random piles, funnels; about 1M bodies), settled, then stepped with velocity sweeps
+until the relative contact residual is <= 1e-3: body-steps/s of the learned warm start when it is
+available, else of the plain solver; both are reported with their sweep counts. Beside it, in
+their own objects: the 65,536-body pile with 8 velocity + 3 position sweeps (the plain GPU path
+against the single-threaded oracle) and the 1,048,576-collider scatter stepped collide-only with
+CCD (pairs/s).
 """
 import argparse
 import importlib.util
@@ -78,9 +83,9 @@ def calibrate_copy():
     log(f"copied {n * 4} bytes three times")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, workload):
     """HBM bytes per launch of a kernel from the committed PMC summary, if one exists."""
-    path = ROOT / "profiles" / f"{kernel}_pile65536_hbm_traffic.md"
+    path = ROOT / "profiles" / f"{kernel}_{workload}_hbm_traffic.md"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -89,42 +94,222 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(agx, scene, world, budget_s):
-    """The oracle (single-threaded CPU restatement) stepping the same settled pile."""
+
+def _oracle():
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as orc
-    s = scene["settings"]
-    ow = orc.world_from_scene(scene, dict(v=s["velocity_iterations"], p=s["position_iterations"]))
-    for g, o in (("com_x", "comX"), ("com_y", "comY"), ("rotation", "rot"), ("vel_x", "velX"), ("vel_y", "velY"),
-                 ("ang_vel", "angVel"), ("cos", "cos"), ("sin", "sin")):
-        ow.set(o, world.body(g))
-    ow.set("wX", world.collider("world_x"))
-    ow.set("wY", world.collider("world_y"))
-    ow.set_prev_manifolds(world.manifolds(1))
+    return orc
+
+
+BODY_STATE = (("com_x", "comX"), ("com_y", "comY"), ("rotation", "rot"), ("vel_x", "velX"), ("vel_y", "velY"),
+              ("ang_vel", "angVel"), ("cos", "cos"), ("sin", "sin"))
+
+
+def oracle_from_world(orc, scene, world, bsel, csel, v_it, p_it):
+    """Oracle world of one scene holding the GPU world's state for that scene's slices."""
+    ow = orc.world_from_scene(scene, dict(v=v_it, p=p_it))
+    for g, o in BODY_STATE:
+        ow.set(o, world.body(g)[bsel])
+    ow.set("wX", world.collider("world_x")[csel])
+    ow.set("wY", world.collider("world_y")[csel])
+    prev = world.manifolds(1)
+    c0, c1 = int(csel.start), int(csel.stop)
+    pm = prev[(prev["a"] >= c0) & (prev["a"] < c1)].copy()
+    pm["a"] -= c0
+    pm["b"] -= c0
+    ow.set_prev_manifolds(pm)
+    return ow
+
+
+def time_oracle(worlds_and_sizes, budget_s):
+    """Steps oracle worlds round-robin (one update(1/60) each) for about budget_s of CPU time."""
     dt = 1.0 / 60.0
+    bodies = steps = 0
     t0 = time.perf_counter()
-    ow.update(dt, 1)
-    first = time.perf_counter() - t0
-    steps = 1
-    extra = max(0, min(200, int(budget_s / max(first, 1e-6)) - 1))
-    t1 = time.perf_counter()
-    for _ in range(extra):
-        ow.update(dt, 1)
-    steps += extra
-    total = first + (time.perf_counter() - t1)
+    while True:
+        for ow, n in worlds_and_sizes:
+            ow.update(dt, 1)
+            bodies += n
+            steps += 1
+            if time.perf_counter() - t0 > budget_s:
+                return bodies / (time.perf_counter() - t0), steps, time.perf_counter() - t0
+
+
+def step_timed(world, steps, dt):
+    world.sync()
+    t0 = time.perf_counter()
+    world.step(dt, steps)
+    world.sync()
+    return time.perf_counter() - t0
+
+
+def profiled(world, steps, dt):
+    world.set_profiling(True)
+    world.profile()
+    world.step(dt, steps)
+    world.sync()
+    prof = world.profile()
+    world.set_profiling(False)
+    return {k: v[0] / max(v[1], 1) for k, v in prof.items() if v[1]}
+
+
+def roofline(seg_ms, st, counts, settings, mean_v, workload):
+    dominant = max(seg_ms, key=seg_ms.get)
+    per_launch_ms = seg_ms[dominant]
+    alg = algorithmic_bytes(dominant, st, counts, settings, mean_v)
+    achieved = alg / (per_launch_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant, workload),
+            "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(per_launch_ms, 4)}
+
+
+def mean_poly_verts(scenes):
+    v = np.concatenate([s["bodies"]["vcount"] for s in scenes])
+    v = v[v > 0]
+    return float(v.mean()) if len(v) else 0.0
+
+
+def build_world(agx, scenes, settings):
+    b = agx.Builder(len(scenes))
+    for i, sc in enumerate(scenes):
+        agx.build_scene(b, sc, i)
+    return agx.World(b, settings)
+
+
+def run_batch(agx, scenes_mod, args, dt):
+    t0 = time.perf_counter()
+    scenes = scenes_mod.batch(1024)
+    base = dict(position_iterations=3, track_residual=1, track_errors=1)
+    world = build_world(agx, scenes, agx.default_settings(velocity_iterations=8, **base))
     n = world.counts.bodies
-    return {"value": n * steps / total, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} oracle steps (update(1/60), single thread) of the same settled {n}-body pile; "
-                      f"{total:.1f} s of CPU time", "ms_per_step": 1e3 * total / steps}
+    log(f"batch: {len(scenes)} scenes, {n} bodies built in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    for k in range(0, args.warmup, 40):
+        world.step(dt, min(40, args.warmup - k))
+    world.sync()
+    log(f"batch: warm-up {args.warmup} steps in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    # plain solver, fixed 8 velocity sweeps
+    el = step_timed(world, args.steps, dt)
+    out["plain_fixed8"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
+                           "velocity_sweeps": 8}
+    # plain solver to the residual target
+    to_target = agx.default_settings(velocity_iterations=64, residual_target=1e-3, **base)
+    world.set_settings(to_target)
+    el = step_timed(world, args.steps, dt)
+    st = world.stats()
+    sweeps_plain = int(st.velocity_sweeps)
+    out["plain"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
+                    "velocity_sweeps_last_step": sweeps_plain,
+                    "residual_last_sweep": round(float(st.residual[max(sweeps_plain - 1, 0)]), 6)}
+    seg = profiled(world, max(5, min(args.steps, 20)), dt)
+    st = world.stats()
+    out["plain"]["segments_ms_per_step"] = {k: round(v, 4) for k, v in seg.items()}
+    out["roofline"] = roofline(seg, st, world.counts, to_target, mean_poly_verts(scenes), "batch1024")
+    out["stats"] = {"manifolds": int(st.manifolds), "pairs": int(st.pairs), "colours": int(st.colours),
+                    "islands": int(st.islands), "overflow": int(st.overflow)}
+    out["learned"] = None
+    out["bodies"] = n
+    out["world"], out["scenes"] = world, scenes
+    return out
+
+
+def batch_cpu_baseline(world, scenes, sweeps, budget_s):
+    orc = _oracle()
+    cs = world.collider("scene")
+    cb = world.collider("body")
+    bscene = np.empty(world.counts.bodies, np.int64)
+    bscene[cb] = cs
+    rng = np.random.default_rng(0)
+    pick = rng.permutation(len(scenes))[:64]
+    worlds = []
+    for i in pick:
+        bi = np.flatnonzero(bscene == i)
+        ci = np.flatnonzero(cs == i)
+        worlds.append((oracle_from_world(orc, scenes[i], world, slice(int(bi[0]), int(bi[-1]) + 1),
+                                         slice(int(ci[0]), int(ci[-1]) + 1), sweeps, 3), len(bi)))
+    value, steps, total = time_oracle(worlds, budget_s)
+    return {"value": round(value, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} oracle scene-steps (update(1/60), single thread, {sweeps} velocity + 3 position sweeps, "
+                      f"the plain GPU solver's sweep count at the 1e-3 target) over 64 of the batch's settled scenes; "
+                      f"{total:.1f} s of CPU time"}
+
+
+def run_pile(agx, scenes_mod, args, dt, cpu):
+    t0 = time.perf_counter()
+    scene = scenes_mod.pile_large(seed=2)
+    settings = agx.default_settings(**scene["settings"], track_residual=1, track_errors=1)
+    world = build_world(agx, [scene], settings)
+    n = world.counts.bodies
+    for k in range(0, 240, 40):
+        world.step(dt, 40)
+    world.sync()
+    steps = 100
+    el = step_timed(world, steps, dt)
+    st = world.stats()
+    seg = profiled(world, 50, dt)
+    st = world.stats()
+    out = {"workload": f"pile_large_{n}_bodies", "value": round(n * steps / el, 1), "unit": "body-steps/s",
+           "ms_per_step": round(1e3 * el / steps, 4), "velocity_iterations": 8, "position_iterations": 3,
+           "segments_ms_per_step": {k: round(v, 4) for k, v in seg.items()},
+           "roofline": roofline(seg, st, world.counts, settings, mean_poly_verts([scene]), "pile65536"),
+           "stats": {"pairs": int(st.pairs), "manifolds": int(st.manifolds), "colours": int(st.colours),
+                     "islands": int(st.islands), "overflow": int(st.overflow),
+                     "residual_per_sweep": [round(float(st.residual[k]), 6) for k in range(st.velocity_sweeps)]}}
+    log(f"pile: {out['ms_per_step']} ms/step in {time.perf_counter() - t0:.1f} s")
+    if cpu:
+        orc = _oracle()
+        ow = oracle_from_world(orc, scene, world, slice(0, n), slice(0, world.counts.colliders), 8, 3)
+        value, steps_c, total = time_oracle([(ow, n)], args.cpu_seconds)
+        out["cpu_baseline"] = {"value": round(value, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{steps_c} oracle steps (update(1/60), single thread) of the same settled "
+                                         f"{n}-body pile; {total:.1f} s of CPU time"}
+        out["speedup_vs_cpu_baseline"] = round(out["value"] / value, 2)
+    return out
+
+
+def run_scatter(agx, scenes_mod, args, dt, cpu):
+    t0 = time.perf_counter()
+    scene = scenes_mod.scatter_large(seed=3)
+    settings = agx.default_settings(gravity_y=0.0, collide_only=1)
+    world = build_world(agx, [scene], settings)
+    n = world.counts.bodies - world.counts.static_bodies
+    world.step(dt, 5)
+    steps = 20
+    el = step_timed(world, steps, dt)
+    st = world.stats()
+    seg = profiled(world, 10, dt)
+    st2 = world.stats()
+    out = {"workload": f"scatter_{n}_dynamic_colliders_{world.counts.static_bodies}_static", "unit": "pairs/s",
+           "value": round(st.pairs * steps / el, 1), "body_steps_per_s": round(n * steps / el, 1),
+           "ms_per_step": round(1e3 * el / steps, 4), "pairs_per_step": int(st.pairs),
+           "manifolds_per_step": int(st.manifolds), "ccd_hits_per_step": int(st2.ccd_hits),
+           "segments_ms_per_step": {k: round(v, 4) for k, v in seg.items()},
+           "roofline": roofline(seg, st2, world.counts, settings, mean_poly_verts([scene]), "scatter1m"),
+           "overflow": int(st2.overflow)}
+    log(f"scatter: {out['ms_per_step']} ms/step in {time.perf_counter() - t0:.1f} s")
+    if cpu:
+        orc = _oracle()
+        ow = orc.world_from_scene(scene, dict(v=8, p=3))
+        t1 = time.perf_counter()
+        ow.snapshot(); ow.integrate(dt); ow.wrap_cossin(); ow.poses(); ow.aabbs(); ow.ccd()
+        pa, pb = ow.broad_bvh()
+        ow.narrow(pa, pb)
+        total = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(len(pa) / total, 1), "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"one oracle collide-only step (integrate, poses, AABBs, CCD, BVH pairs, "
+                                         f"narrow phase) of the initial scatter, single thread; {total:.1f} s"}
+        out["speedup_vs_cpu_baseline"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=240)
-    ap.add_argument("--workload", default="pile65536", choices=["pile65536", "pile2560"])
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--steps", type=int, default=30, help="timed steps of the headline batch")
+    ap.add_argument("--warmup", type=int, default=240, help="settling steps of the headline batch")
+    ap.add_argument("--workload", default="all", choices=["all", "batch1024", "pile65536", "scatter1m"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device copy three times (PMC byte-counter calibration)")
@@ -133,87 +318,41 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
-              f"in DESIGN.md and is not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the path would shard is "
+              f"described in DESIGN.md and is not run.", file=sys.stderr)
         sys.exit(2)
-
     agx = load_package()
-    import agx.scenes as scenes  # noqa: F401
-    scenes = sys.modules["agx.scenes"]
-    t0 = time.perf_counter()
-    scene = scenes.pile_large(seed=2) if args.workload == "pile65536" else scenes.pile_small(seed=1)
-    builder = agx.Builder(1)
-    agx.build_scene(builder, scene)
-    settings = agx.default_settings(**scene["settings"], track_residual=1, track_errors=1)
-    world = agx.World(builder, settings)
-    log(f"scene {scene['name']}: {world.counts.bodies} bodies built in {time.perf_counter() - t0:.1f} s")
+    import agx.scenes  # noqa: F401
+    scenes_mod = sys.modules["agx.scenes"]
     dt = 1.0 / 60.0
-    t0 = time.perf_counter()
-    for k in range(0, args.warmup, 20):
-        world.step(dt, min(20, args.warmup - k))
-    world.sync()
-    log(f"warm-up {args.warmup} steps in {time.perf_counter() - t0:.1f} s")
-
-    try:
-        import torch
-        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync_torch = lambda: None  # noqa: E731
-    # timed region: plain stepping, no profiling events
-    world.sync()
-    sync_torch()
-    t0 = time.perf_counter()
-    world.step(dt, args.steps)
-    world.sync()
-    sync_torch()
-    elapsed = time.perf_counter() - t0
-    st = world.stats()
-    n = world.counts.bodies
-    value = n * args.steps / elapsed
-    log(f"timed {args.steps} steps in {elapsed:.3f} s: {1e3 * elapsed / args.steps:.3f} ms/step, "
-        f"{st.pairs} pairs, {st.manifolds} manifolds, {st.colours} colours, overflow={st.overflow}")
-    # profiled pass: per-segment GPU time from HIP events on the world's stream
-    prof_steps = max(10, min(args.steps, 50))
-    world.set_profiling(True)
-    world.profile()
-    world.step(dt, prof_steps)
-    world.sync()
-    prof = world.profile()
-    world.set_profiling(False)
-    st = world.stats()
-
-    vcount = scene["bodies"]["vcount"]
-    poly = vcount[vcount > 0]
-    mean_v = float(poly.mean()) if len(poly) else 0.0
-    seg_ms = {k: v[0] / max(v[1], 1) for k, v in prof.items() if v[1]}
-    dominant = max(seg_ms, key=seg_ms.get)
-    per_launch_ms = seg_ms[dominant]
-    alg = algorithmic_bytes(dominant, st, world.counts, settings, mean_v)
-    achieved = alg / (per_launch_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant),
-                "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(per_launch_ms, 4)}
-    residual = [float(st.residual[k]) for k in range(st.velocity_sweeps)]
+    cpu = not args.no_cpu
+    if args.workload in ("pile65536", "scatter1m"):
+        fn = run_pile if args.workload == "pile65536" else run_scatter
+        print(json.dumps(fn(agx, scenes_mod, args, dt, cpu)), flush=True)
+        return
+    b = run_batch(agx, scenes_mod, args, dt)
+    head = b["learned"] if b["learned"] else b["plain"]
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{scene['name']}_{n}_bodies_plain_solver", "bodies": n,
-                   "velocity_iterations": settings.velocity_iterations,
-                   "position_iterations": settings.position_iterations, "dt": dt, "warm_start": True,
+        "metric": METRIC, "value": head["value"], "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch1024_scenes_{b['bodies']}_bodies_to_residual_1e-3", "scenes": 1024,
+                   "bodies": b["bodies"], "residual_target": 1e-3, "max_velocity_sweeps": 64,
+                   "position_iterations": 3, "dt": dt, "warm_start": "learned" if b["learned"] else "cached",
                    "parallelism": "single GPU"},
-        "roofline": roofline,
-        "segments_ms_per_step": {k: round(v, 4) for k, v in seg_ms.items()},
-        "segments_profiled_steps": prof_steps,
-        "step_stats": {"pairs": int(st.pairs), "manifolds": int(st.manifolds), "contacts": int(st.contacts),
-                       "colours": int(st.colours), "islands": int(st.islands),
-                       "residual_per_sweep": [round(r, 6) for r in residual],
-                       "penetration_sum": float(st.position_error), "overflow": int(st.overflow)},
+        "roofline": b["roofline"],
+        "plain": b["plain"], "learned": b["learned"], "plain_fixed8": b["plain_fixed8"], "stats": b["stats"],
     }
-    if not args.no_cpu:
-        cb = cpu_baseline(agx, scene, world, args.cpu_seconds)
-        out["cpu_baseline"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in cb.items()}
-        out["speedup_vs_cpu_baseline"] = round(value / cb["value"], 2)
+    if b["learned"]:
+        out["learned_over_plain"] = round(b["learned"]["value"] / b["plain"]["value"], 3)
+    if cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(b["world"], b["scenes"], max(b["plain"]["velocity_sweeps_last_step"], 1),
+                                                 args.cpu_seconds)
+        out["speedup_vs_cpu_baseline"] = round(head["value"] / out["cpu_baseline"]["value"], 2)
+    del b
+    if args.workload == "all":
+        out["pile65536"] = run_pile(agx, scenes_mod, args, dt, cpu)
+        out["scatter1m"] = run_scatter(agx, scenes_mod, args, dt, cpu)
     print(json.dumps(out), flush=True)
 
 

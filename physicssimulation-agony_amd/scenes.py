@@ -197,39 +197,71 @@ def pile_large(seed: int = 2, n: int = 65536) -> dict:
 
 def scatter(n: int, size: float, seed: int, n_static: int = 0, bullet_fraction: float = 0.05,
             bullet_speed: float = 200.0, name: str = "scatter") -> dict:
-    """Bodies uniform in [-size/2, size/2]^2 with random orientation, bullets, static obstacles."""
+    """Bodies uniform in [-size/2, size/2]^2 with random orientation, lognormal(ln 0.3, 0.3) radii
+    (70 % polygons with 3-8 vertices, the rest circles), 5 % small bullets (radius x 0.25, circles)
+    at 200 m/s in random directions, and static boxes and polygons so that CCD has work. No
+    gravity; the world steps collide-only."""
     rng = np.random.default_rng(seed)
-    acc = _Acc()
-    for _ in range(n_static):
-        x, y = rng.uniform(-size / 2, size / 2, 2)
-        if rng.random() < 0.5:
-            acc.add(SHAPE_BOX, float(x), float(y), rot=float(rng.uniform(0, 2 * np.pi)),
-                    p0=float(rng.uniform(2, 8)), p1=float(rng.uniform(2, 8)))
-        else:
-            acc.add(SHAPE_POLYGON, float(x), float(y), rot=float(rng.uniform(0, 2 * np.pi)),
-                    verts=polygon_outline(rng, int(rng.integers(3, 9)), float(rng.uniform(1.5, 4.0))))
+    # static obstacles: boxes and polygons of 3-12 m
+    sx = rng.uniform(-size / 2, size / 2, n_static)
+    sy = rng.uniform(-size / 2, size / 2, n_static)
+    srot = np.minimum(rng.uniform(0, 2 * np.pi, n_static), 6.2831850)
+    sbox = rng.random(n_static) < 0.5
+    ns_poly = int((~sbox).sum())
+    _, sk, sverts = _polygons(rng, ns_poly, 1.5, 4.0)
+    sp0 = np.where(sbox, rng.uniform(2, 8, n_static), 0.0)
+    sp1 = np.where(sbox, rng.uniform(2, 8, n_static), 0.0)
+    svc = np.zeros(n_static, np.uint32)
+    svc[~sbox] = sk
+    # dynamic bodies
     radii = np.clip(rng.lognormal(math.log(0.3), 0.3, n), 0.1, 0.9)
     bullets = rng.random(n) < bullet_fraction
-    radii = np.where(bullets, radii * 0.25, radii)
-    xs = rng.uniform(-size / 2, size / 2, n)
-    ys = rng.uniform(-size / 2, size / 2, n)
-    rots = np.minimum(rng.uniform(0, 2 * np.pi, n), 6.2831850)
-    dirs = rng.uniform(0, 2 * np.pi, n)
-    kinds = rng.random(n) < 0.7
+    radii = np.where(bullets, radii * 0.25, radii).astype(np.float32)
+    x = rng.uniform(-size / 2, size / 2, n)
+    y = rng.uniform(-size / 2, size / 2, n)
+    rot = np.minimum(rng.uniform(0, 2 * np.pi, n), 6.2831850)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    is_poly = (rng.random(n) < 0.7) & ~bullets
     dens = rng.uniform(0.5, 2.0, n)
-    for i in range(n):
-        vx = bullet_speed * math.cos(dirs[i]) if bullets[i] else 0.0
-        vy = bullet_speed * math.sin(dirs[i]) if bullets[i] else 0.0
-        r = float(np.float32(radii[i]))
-        if kinds[i] and not bullets[i]:
-            v = polygon_outline(rng, int(rng.integers(3, 9)), r)
-            acc.add(SHAPE_POLYGON, float(xs[i]), float(ys[i]), rot=float(rots[i]), vx=vx, vy=vy,
-                    mass=float(np.float32(dens[i] * polygon_area(v))), verts=v)
-        else:
-            acc.add(SHAPE_CIRCLE, float(xs[i]), float(ys[i]), rot=float(rots[i]), vx=vx, vy=vy,
-                    mass=float(np.float32(dens[i] * math.pi * r * r)), p0=r)
-    return {"name": name, "materials": [(0.0, 0.6, 0.6)], "bodies": acc.bodies(),
+    npoly = int(is_poly.sum())
+    k = rng.integers(3, 9, npoly)
+    a8 = 2.0 * np.pi * (np.arange(8)[None, :] + rng.uniform(-0.3, 0.3, (npoly, 8))) / k[:, None]
+    mask = np.arange(8)[None, :] < k[:, None]
+    rp = radii[is_poly].astype(np.float64)[:, None]
+    dverts = np.stack([(rp * np.cos(a8))[mask], (rp * np.sin(a8))[mask]], axis=1).astype(np.float32)
+    mass = np.zeros(n)
+    mass[is_poly] = dens[is_poly] * _polygon_areas(dverts, k)
+    mass[~is_poly] = dens[~is_poly] * np.pi * radii[~is_poly].astype(np.float64) ** 2
+    vcount = np.zeros(n, np.uint32)
+    vcount[is_poly] = k
+    p0 = np.where(is_poly, 0.0, radii)
+    vx = np.where(bullets, bullet_speed * np.cos(ang), 0.0)
+    vy = np.where(bullets, bullet_speed * np.sin(ang), 0.0)
+    typ = np.where(is_poly, SHAPE_POLYGON, SHAPE_CIRCLE).astype(np.uint8)
+    vc_all = np.concatenate([svc, vcount])
+    vstart = np.zeros(len(vc_all), np.uint32)
+    has = vc_all > 0
+    vstart[has] = np.concatenate([[0], np.cumsum(vc_all[has])[:-1]]).astype(np.uint32)
+    body = {
+        "type": np.concatenate([np.where(sbox, SHAPE_BOX, SHAPE_POLYGON).astype(np.uint8), typ]),
+        "px": np.concatenate([sx, x]).astype(np.float32), "py": np.concatenate([sy, y]).astype(np.float32),
+        "rot": np.concatenate([srot, rot]).astype(np.float32),
+        "vx": np.concatenate([np.zeros(n_static), vx]).astype(np.float32),
+        "vy": np.concatenate([np.zeros(n_static), vy]).astype(np.float32),
+        "av": np.zeros(n_static + n, np.float32),
+        "mass": np.concatenate([np.zeros(n_static), mass]).astype(np.float32),
+        "material": np.ones(n_static + n, np.uint32),
+        "p0": np.concatenate([sp0, p0]).astype(np.float32), "p1": np.concatenate([sp1, np.zeros(n)]).astype(np.float32),
+        "vstart": vstart, "vcount": vc_all,
+        "verts": np.concatenate([sverts, dverts]).astype(np.float32),
+    }
+    return {"name": name, "materials": [(0.0, 0.6, 0.6)], "bodies": body,
             "settings": {"gravity_y": 0.0, "collide_only": 1}}
+
+
+def scatter_large(seed: int = 3) -> dict:
+    """1,048,576 dynamic colliders in a 4096 x 4096 m domain plus 16,384 static obstacles."""
+    return scatter(1 << 20, 4096.0, seed, n_static=16384, name="scatter_large")
 
 
 def pyramid(rows: int, seed: int, box: float = 1.0) -> dict:
@@ -245,3 +277,121 @@ def pyramid(rows: int, seed: int, box: float = 1.0) -> dict:
             acc.add(SHAPE_BOX, float(x), float(y), mass=box * box, p0=box, p1=box)
     return {"name": "pyramid", "materials": [(0.0, 0.6, 0.6)], "bodies": acc.bodies(),
             "settings": {"velocity_iterations": 8, "position_iterations": 3}}
+
+
+# ---------------------------------------------------------------- batch of independent scenes
+def _polygons(rng, n, r_lo, r_hi):
+    """n convex polygons with 3-8 vertices on jittered circles: (radius, vcount, verts (V, 2))."""
+    k = rng.integers(3, 9, n)
+    r = rng.uniform(r_lo, r_hi, n)
+    ang = 2.0 * np.pi * (np.arange(8)[None, :] + rng.uniform(-0.3, 0.3, (n, 8))) / k[:, None]
+    mask = np.arange(8)[None, :] < k[:, None]
+    vx = (r[:, None] * np.cos(ang))[mask]
+    vy = (r[:, None] * np.sin(ang))[mask]
+    return r, k, np.stack([vx, vy], axis=1).astype(np.float32)
+
+
+def _polygon_areas(verts, counts):
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    x = verts[:, 0].astype(np.float64)
+    y = verts[:, 1].astype(np.float64)
+    nxt = np.arange(len(verts)) + 1
+    ends = starts + counts
+    nxt[ends - 1] = starts                      # wrap each polygon
+    cross = x * y[nxt] - x[nxt] * y
+    return 0.5 * np.abs(np.add.reduceat(cross, starts))
+
+
+def _scene_arrays(static, dyn_type, dyn_x, dyn_y, dyn_rot, dyn_mass, dyn_p0, dyn_p1, dyn_vcount, dyn_verts):
+    """Body dict from static boxes [(x, y, rot, w, h)] and dynamic body arrays."""
+    ns, nd = len(static), len(dyn_type)
+    st = np.asarray(static, np.float64).reshape(ns, 5)
+    vstart = np.zeros(nd, np.uint32)
+    vc = np.asarray(dyn_vcount, np.uint32)
+    vstart[vc > 0] = np.concatenate([[0], np.cumsum(vc[vc > 0])[:-1]]).astype(np.uint32)
+    return {
+        "type": np.concatenate([np.full(ns, SHAPE_BOX, np.uint8), np.asarray(dyn_type, np.uint8)]),
+        "px": np.concatenate([st[:, 0], dyn_x]).astype(np.float32),
+        "py": np.concatenate([st[:, 1], dyn_y]).astype(np.float32),
+        "rot": np.concatenate([st[:, 2], dyn_rot]).astype(np.float32),
+        "vx": np.zeros(ns + nd, np.float32), "vy": np.zeros(ns + nd, np.float32), "av": np.zeros(ns + nd, np.float32),
+        "mass": np.concatenate([np.zeros(ns), dyn_mass]).astype(np.float32),
+        "material": np.ones(ns + nd, np.uint32),
+        "p0": np.concatenate([st[:, 3], dyn_p0]).astype(np.float32),
+        "p1": np.concatenate([st[:, 4], dyn_p1]).astype(np.float32),
+        "vstart": np.concatenate([np.zeros(ns, np.uint32), vstart]),
+        "vcount": np.concatenate([np.zeros(ns, np.uint32), vc]),
+        "verts": np.asarray(dyn_verts, np.float32).reshape(-1, 2),
+    }
+
+
+def _grid_bodies(rng, n, x0, y0, cols, cell=1.1, r_lo=0.2, r_hi=0.5):
+    """n bodies (70 % polygons, 30 % circles, density 1) on a jittered grid, row by row upwards."""
+    idx = np.arange(n)
+    x = x0 + cell * (idx % cols + 0.5) + rng.uniform(-0.04, 0.04, n)
+    y = y0 + cell * (idx // cols + 0.5) + rng.uniform(-0.04, 0.04, n)
+    is_poly = rng.random(n) < 0.7
+    npoly = int(is_poly.sum())
+    rp, kp, verts = _polygons(rng, npoly, r_lo, r_hi)
+    rc = rng.uniform(r_lo, r_hi, n - npoly)
+    typ = np.where(is_poly, SHAPE_POLYGON, SHAPE_CIRCLE).astype(np.uint8)
+    mass = np.zeros(n)
+    mass[is_poly] = _polygon_areas(verts, kp) if npoly else []
+    mass[~is_poly] = np.pi * rc.astype(np.float32).astype(np.float64) ** 2
+    p0 = np.zeros(n)
+    p0[~is_poly] = rc
+    vcount = np.zeros(n, np.uint32)
+    vcount[is_poly] = kp
+    rot = np.minimum(rng.uniform(0.0, 2.0 * np.pi, n), 6.2831850)
+    return typ, x, y, rot, mass, p0, np.zeros(n), vcount, verts
+
+
+def batch_scene(i: int, seed: int = 4, n_min: int = 64, n_max: int = 4096) -> dict:
+    """Scene i of the batch: body count log-uniform in [n_min, n_max]; layout by i mod 3 is a box
+    pyramid, a random pile in an open box, or a funnel of static boxes above a floor."""
+    rng = np.random.default_rng(seed + i)
+    n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+    layout = i % 3
+    if layout == 0:
+        rows = max(1, int((math.sqrt(8 * n + 1) - 1) / 2))
+        r = np.repeat(np.arange(rows), rows - np.arange(rows))
+        c = np.concatenate([np.arange(rows - k) for k in range(rows)])
+        count = rows - r
+        x = (c - (count - 1) / 2.0) * 1.02 + rng.uniform(-0.01, 0.01, len(r))
+        y = 0.5 + r * 1.001
+        m = len(r)
+        static = [(0.0, -1.0, 0.0, rows * 3.0 + 10.0, 2.0)]
+        body = _scene_arrays(static, np.full(m, SHAPE_BOX), x, y, np.zeros(m), np.ones(m), np.ones(m), np.ones(m),
+                             np.zeros(m, np.uint32), np.zeros((0, 2)))
+        name = "pyramid"
+    elif layout == 1:
+        cols = max(4, int(math.sqrt(n)))
+        width = cols * 1.1
+        height = (n // cols + 2) * 1.1 * 1.5
+        static = [(0.0, -1.0, 0.0, width + 4.0, 2.0), (-width / 2 - 1.0, height / 2, 0.0, 2.0, height),
+                  (width / 2 + 1.0, height / 2, 0.0, 2.0, height)]
+        body = _scene_arrays(static, *_grid_bodies(rng, n, -width / 2, 0.2, cols))
+        name = "pile"
+    else:
+        cols = max(4, int(math.sqrt(n)))
+        width = cols * 1.1
+        mouth = 3.0
+        slope = 0.5 * (width - mouth)
+        base = 6.0
+        rows_h = (n // cols + 2) * 1.1
+        top = base + slope + 1.0
+        length = slope * math.sqrt(2.0) + 1.0
+        static = [(0.0, -1.0, 0.0, width + 8.0, 2.0),
+                  (-(mouth / 2 + slope / 2) - 0.5, base + slope / 2, -math.pi / 4 + 2 * math.pi, length, 1.0),
+                  (mouth / 2 + slope / 2 + 0.5, base + slope / 2, math.pi / 4, length, 1.0),
+                  (-width / 2 - 1.5, (top + rows_h) / 2, 0.0, 1.0, top + rows_h + 2.0),
+                  (width / 2 + 1.5, (top + rows_h) / 2, 0.0, 1.0, top + rows_h + 2.0)]
+        body = _scene_arrays(static, *_grid_bodies(rng, n, -width / 2, top, cols))
+        name = "funnel"
+    return {"name": f"batch{i}_{name}", "materials": [(0.0, 0.6, 0.6)], "bodies": body,
+            "settings": {"velocity_iterations": 8, "position_iterations": 3, "gravity_y": -9.81}}
+
+
+def batch(num_scenes: int = 1024, seed: int = 4, n_min: int = 64, n_max: int = 4096) -> list:
+    """The scene batch: independent scenes of log-uniform body counts (one world, many scenes)."""
+    return [batch_scene(i, seed, n_min, n_max) for i in range(num_scenes)]

@@ -295,3 +295,115 @@ def test_batched_scenes_match_single_scenes(agx, oracle, scenes):
     for p in range(agx.PHASE_SOLVE, agx.PHASE_FINISH + 1):
         w.run_phase(p, DT)
     assert w.stats().overflow == 0
+
+
+def test_batch_scenes_match_oracle_scene_by_scene(agx, oracle, scenes):
+    # A batched world (pyramids, piles, funnels) against the oracle stepping each scene alone.
+    parts = [scenes.batch_scene(i, seed=4, n_min=64, n_max=400) for i in range(6)]
+    b = agx.Builder(len(parts))
+    for s, sc in enumerate(parts):
+        agx.build_scene(b, sc, s)
+    w = agx.World(b, agx.default_settings(velocity_iterations=8, position_iterations=3))
+    w.step(DT, 90)
+    cscene = w.collider("scene")
+    bscene = np.empty(w.counts.bodies, np.uint32)
+    bscene[w.collider("body")] = cscene          # one collider per body here
+    bodies0 = {f: w.body(f) for f in ("com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel", "cos", "sin")}
+    cols0 = {f: w.collider(f) for f in ("world_x", "world_y", "cos", "sin", "world_rot", "aabb_min_x", "aabb_min_y",
+                                         "aabb_max_x", "aabb_max_y")}
+    prev = w.manifolds(1)
+    for p in range(agx.PHASE_NARROW + 1):
+        w.run_phase(p, DT)
+    ga, gb = w.pairs()
+    mg = w.manifolds(0)
+    b1 = {f: w.body(f) for f in ("com_x", "com_y", "rotation", "cos")}
+    aabb1 = w.collider("aabb_min_x")
+    w.run_phase(agx.PHASE_SOLVE, DT)
+    order, _ = w.solve_order()
+    solved = w.manifolds(0)
+    b2 = {f: w.body(f) for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y")}
+    pos_of = np.empty(len(order), np.int64)
+    pos_of[order] = np.arange(len(order))
+    renames = {"com_x": "comX", "com_y": "comY", "rotation": "rot", "vel_x": "velX", "vel_y": "velY",
+               "ang_vel": "angVel", "cos": "cos", "sin": "sin"}
+    col_names = {"world_x": "wX", "world_y": "wY", "cos": "wCos", "sin": "wSin", "world_rot": "wRot",
+                 "aabb_min_x": "minX", "aabb_min_y": "minY", "aabb_max_x": "maxX", "aabb_max_y": "maxY"}
+    for s, sc in enumerate(parts):
+        bs = np.flatnonzero(bscene == s)
+        cs = np.flatnonzero(cscene == s)
+        b_off, c_off = int(bs[0]), int(cs[0])
+        ow = oracle.world_from_scene(sc, dict(v=8, p=3))
+        for f, o in renames.items():
+            ow.set(o, bodies0[f][bs])
+        for f, o in col_names.items():
+            ow.set(o, cols0[f][cs])
+        pm = prev[(prev["a"] >= c_off) & (prev["a"] < c_off + len(cs))].copy()
+        pm["a"] -= c_off
+        pm["b"] -= c_off
+        ow.set_prev_manifolds(pm)
+        ow.snapshot()
+        ow.integrate(DT)
+        ow.wrap_cossin()
+        ow.poses()
+        ow.aabbs()
+        for f in ("com_x", "com_y", "rotation", "cos"):
+            assert np.array_equal(b1[f][bs], ow.get(renames[f])), (s, f)
+        assert np.array_equal(aabb1[cs], ow.get("minX")), s
+        sel = (ga >= c_off) & (ga < c_off + len(cs))
+        pa, pb = ga[sel] - c_off, gb[sel] - c_off
+        oa, ob = ow.broad_brute()
+        is_dyn = np.zeros(len(cs), bool)
+        is_dyn[ow.counts()["static_colliders"]:] = True
+        assert pair_set(pa, pb, is_dyn) == pair_set(oa, ob, is_dyn), s
+        msel = np.flatnonzero((mg["a"] >= c_off) & (mg["a"] < c_off + len(cs)))
+        ms = mg[msel].copy()
+        ms["a"] -= c_off
+        ms["b"] -= c_off
+        mo = ow.warm_start(ow.narrow(pa, pb))
+        assert len(ms) == len(mo) > 0, s
+        ks, ko = ms[sort_key(ms)], mo[sort_key(mo)]
+        for f in ("a", "b", "count", "nx", "ny", "depth", "p0x", "p0y", "id0", "jn0", "jt0"):
+            assert np.array_equal(ks[f], ko[f]), (s, f)
+        # the GPU's solve order restricted to this scene
+        local_order = np.argsort(pos_of[msel], kind="stable").astype(np.uint32)
+        so = ow.solve(ms, order=local_order, v_it=8, p_it=3)
+        sg = solved[msel]
+        for f in ("jn0", "jt0"):
+            np.testing.assert_allclose(sg[f], so[f], rtol=1e-4, atol=1e-4, err_msg=f"{s} {f}")
+        for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y"):
+            np.testing.assert_allclose(b2[f][bs], ow.get(renames[f]), rtol=1e-4, atol=1e-4, err_msg=f"{s} {f}")
+    for p in range(agx.PHASE_ISLANDS, agx.PHASE_FINISH + 1):
+        w.run_phase(p, DT)
+    assert w.stats().overflow == 0
+
+
+def test_scatter_pairs_match_oracle_at_scale(agx, oracle, scenes):
+    # A quarter-size scatter (262,144 dynamic colliders, bullets, 4,096 statics): the GPU pair set
+    # after CCD equals the oracle's exact sweep-and-prune set on the same AABBs.
+    scene = scenes.scatter(1 << 18, 2048.0, seed=3, n_static=4096)
+    w = make_world(agx, scene)
+    w.step(DT, 2)
+    for p in range(agx.PHASE_BROAD + 1):
+        w.run_phase(p, DT)
+    ga, gb = w.pairs()
+    ow = oracle.world_from_scene(scene)
+    for g, o in (("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY")):
+        ow.set(o, w.collider(g))
+    oa, ob = ow.broad_brute()
+    ns = ow.counts()["static_colliders"]
+
+    def keys(a, b):
+        a = a.astype(np.uint64)
+        b = b.astype(np.uint64)
+        both_dyn = (a >= ns) & (b >= ns)
+        lo = np.where(both_dyn, np.minimum(a, b), a)
+        hi = np.where(both_dyn, np.maximum(a, b), b)
+        return np.sort((lo << np.uint64(32)) | hi)
+
+    kg, ko = keys(ga, gb), keys(oa, ob)
+    assert len(kg) > 1000 and len(np.unique(kg)) == len(kg)
+    assert np.array_equal(kg, ko)
+    for p in range(agx.PHASE_NARROW, agx.PHASE_FINISH + 1):
+        w.run_phase(p, DT)
+    st = w.stats()
+    assert st.overflow == 0 and st.manifolds > 0
