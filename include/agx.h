@@ -36,6 +36,7 @@ extern "C" {
 
 typedef struct agx_builder agx_builder;
 typedef struct agx_world agx_world;
+typedef struct agx_mlp agx_mlp;
 
 /* Simulation::SimulationSettings (Simulation.h:50-76), FixedClock::Settings
  * (SimulationImpl/FixedClock.h:12-17), Integrator::Settings (Integration/Integrator.h:14-19)
@@ -80,7 +81,7 @@ typedef struct agx_stats {
     uint32_t ccd_hits;             /* bodies moved back by CCD */
     uint32_t velocity_sweeps;      /* velocity sweeps actually run (adaptive mode may stop early) */
     uint32_t overflow;             /* nonzero: a capacity was exceeded, results are truncated */
-    uint32_t reserved;
+    uint32_t learned_warm_start;   /* 1: the step's warm start came from the network */
     float velocity_error;          /* computeConstraintErrors().x */
     float position_error;          /* computeConstraintErrors().y */
     float residual[64];            /* relative impulse change after velocity sweep k */
@@ -210,6 +211,65 @@ const char* agx_profile_segment_name(uint32_t segment);
 int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t* is_dynamic,
                          const uint32_t* scene, uint32_t n, uint32_t* pairs_a, uint32_t* pairs_b,
                          uint64_t cap, uint64_t* count);
+
+/* ---- learned warm start. No counterpart in the reference, which warm-starts every contact point
+ * from the cached accumulators of the previous step (NarrowPhaseCollisionDetector.cpp:753-765,
+ * BodyCollisionSolver.cpp:416-490). A network 24 -> 128 -> 128 -> 2 (ReLU, bf16 weights and
+ * activations, fp32 accumulation) maps per-point contact features to corrections of those cached
+ * accumulators; see k_learn.hip for the features and physicssimulation-agony_amd/csrc/k_mlp.hip for
+ * the forward, backward and Adam kernels.
+ * Parameters in host layout (AGX_MLP_PARAMS floats): W1[128][24], b1[128], W2[128][128], b2[128],
+ * W3[2][128], b3[2]; layer l computes h = W h_prev + b. */
+#define AGX_MLP_INPUTS 24
+#define AGX_MLP_HIDDEN 128
+#define AGX_MLP_OUTPUTS 2
+#define AGX_MLP_PARAMS 19970
+#define AGX_MLP_BATCH_MAX 8192
+
+/* He-initialised network (seeded) with a device buffer for sample_capacity training rows. */
+int agx_mlp_create(uint32_t seed, uint64_t sample_capacity, agx_mlp** out);
+int agx_mlp_destroy(agx_mlp* m);
+int agx_mlp_get_params(agx_mlp* m, float* out, uint64_t cap);
+int agx_mlp_set_params(agx_mlp* m, const float* in, uint64_t n);   /* also resets the optimiser */
+/* x: n x 24 features (rounded to bf16 on upload), y: n x 2 outputs. */
+int agx_mlp_forward(agx_mlp* m, const float* x, uint64_t n, float* y);
+/* Gradient of the mean squared error (1/n) sum_rows sum_outputs (y - t)^2 over one batch of n rows
+ * (a positive multiple of 16, at most AGX_MLP_BATCH_MAX), in host parameter layout. */
+int agx_mlp_gradients(agx_mlp* m, const float* x, const float* t, uint64_t n, float* grad, float* loss);
+/* Training rows: appended by agx_mlp_add_samples or by a world in AGX_LEARN_COLLECT mode. */
+int agx_mlp_add_samples(agx_mlp* m, const float* x, const float* t, uint64_t n);
+int agx_mlp_sample_count(agx_mlp* m, uint64_t* count);
+int agx_mlp_read_samples(agx_mlp* m, float* x, float* t, uint64_t offset, uint64_t n);
+int agx_mlp_clear_samples(agx_mlp* m);
+/* Adam (0.9, 0.999, 1e-8) over `iterations` random batches of the stored rows, all on the GPU;
+ * *mean_loss receives the mean batch loss. */
+int agx_mlp_train(agx_mlp* m, uint32_t iterations, uint32_t batch, float learning_rate, uint32_t seed, float* mean_loss);
+
+/* Modes of agx_world_set_learned (bit flags). PREDICT: warm start from the network, subject to the
+ * residual gate unless UNGATED; COLLECT: append every contact point of each step to the network's
+ * training rows (features, and as targets the corrections that reach the solver's final
+ * accumulators). The world keeps a reference to the network, which must outlive it. */
+#define AGX_LEARN_OFF 0u
+#define AGX_LEARN_PREDICT 1u
+#define AGX_LEARN_COLLECT 2u
+#define AGX_LEARN_UNGATED 4u
+int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode);
+
+/* Residual gate: exponential averages (weight 1/4 per step) of the cost of a step with the cached
+ * (cost_cached) and the learned warm start (cost_learned). Cost: velocity sweeps to reach the
+ * residual target, plus log2(residual / target) when the sweep cap ended the solve first; without
+ * a target, log10 of the last sweep's residual. The cheaper method runs, the other once every 16
+ * steps; a learned step that cost more than the cached average is followed by a cached step.
+ * agx_gate_update is the same update on the host (no device work). */
+typedef struct agx_gate {
+    float cost_cached, cost_learned;
+    uint32_t use_learned;          /* method of the next step */
+    uint32_t since_probe;
+    uint32_t steps_cached, steps_learned;
+} agx_gate;
+int agx_gate_update(agx_gate* g, uint32_t used_learned, float cost);
+int agx_world_read_gate(agx_world* w, agx_gate* out);
+int agx_world_reset_gate(agx_world* w);
 
 #ifdef __cplusplus
 }

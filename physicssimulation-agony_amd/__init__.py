@@ -60,8 +60,18 @@ class Settings(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("manifolds", C.c_uint64), ("contacts", C.c_uint64),
                 ("colours", C.c_uint32), ("islands", C.c_uint32), ("ccd_hits", C.c_uint32),
-                ("velocity_sweeps", C.c_uint32), ("overflow", C.c_uint32), ("reserved", C.c_uint32),
+                ("velocity_sweeps", C.c_uint32), ("overflow", C.c_uint32), ("learned_warm_start", C.c_uint32),
                 ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * 64)]
+
+
+class Gate(C.Structure):
+    """agx_gate: residual gate of the learned warm start."""
+    _fields_ = [("cost_cached", C.c_float), ("cost_learned", C.c_float), ("use_learned", C.c_uint32),
+                ("since_probe", C.c_uint32), ("steps_cached", C.c_uint32), ("steps_learned", C.c_uint32)]
+
+
+LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED = 0, 1, 2, 4
+MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS, MLP_BATCH_MAX = 24, 128, 2, 19970, 8192
 
 
 class Counts(C.Structure):
@@ -111,6 +121,21 @@ _SIGS = {
     "agx_world_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "agx_world_read_profile": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_profile_segment_name": (C.c_char_p, [C.c_uint32]),
+    "agx_mlp_create": (C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(_P)]),
+    "agx_mlp_destroy": (C.c_int, [_P]),
+    "agx_mlp_get_params": (C.c_int, [_P, _P, C.c_uint64]),
+    "agx_mlp_set_params": (C.c_int, [_P, _P, C.c_uint64]),
+    "agx_mlp_forward": (C.c_int, [_P, _P, C.c_uint64, _P]),
+    "agx_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_uint64, _P, C.POINTER(C.c_float)]),
+    "agx_mlp_add_samples": (C.c_int, [_P, _P, _P, C.c_uint64]),
+    "agx_mlp_sample_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "agx_mlp_read_samples": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint64]),
+    "agx_mlp_clear_samples": (C.c_int, [_P]),
+    "agx_mlp_train": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(C.c_float)]),
+    "agx_world_set_learned": (C.c_int, [_P, _P, C.c_uint32]),
+    "agx_gate_update": (C.c_int, [C.POINTER(Gate), C.c_uint32, C.c_float]),
+    "agx_world_read_gate": (C.c_int, [_P, C.POINTER(Gate)]),
+    "agx_world_reset_gate": (C.c_int, [_P]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -344,6 +369,92 @@ class World:
         s = Stats()
         _check(lib().agx_world_read_stats(self._h, C.byref(s)), self._h)
         return s
+
+    # -- learned warm start
+    def set_learned(self, mlp: "Mlp | None", mode: int):
+        """AGX_LEARN_* flags; the world keeps a reference to the network."""
+        self._mlp = mlp
+        _check(lib().agx_world_set_learned(self._h, mlp._h if mlp is not None else None, mode), self._h)
+
+    def gate(self) -> Gate:
+        g = Gate()
+        _check(lib().agx_world_read_gate(self._h, C.byref(g)), self._h)
+        return g
+
+    def reset_gate(self):
+        _check(lib().agx_world_reset_gate(self._h), self._h)
+
+
+class Mlp:
+    """The learned warm start's network (24 -> 128 -> 128 -> 2) with its training rows, on the GPU."""
+
+    def __init__(self, seed: int = 1, sample_capacity: int = 1 << 20):
+        self._h = C.c_void_p()
+        _check(lib().agx_mlp_create(seed, sample_capacity, C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().agx_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _LIB is not None:
+            self.close()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        _check(lib().agx_mlp_get_params(self._h, _ptr(out), MLP_PARAMS))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().agx_mlp_set_params(self._h, _ptr(p), len(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        y = np.empty((len(x), MLP_OUTPUTS), np.float32)
+        _check(lib().agx_mlp_forward(self._h, _ptr(x), len(x), _ptr(y)))
+        return y
+
+    def gradients(self, x, t):
+        """(loss, gradient in host parameter layout) of one batch."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_OUTPUTS)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(lib().agx_mlp_gradients(self._h, _ptr(x), _ptr(t), len(x), _ptr(g), C.byref(loss)))
+        return float(loss.value), g
+
+    def add_samples(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_INPUTS)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_OUTPUTS)
+        _check(lib().agx_mlp_add_samples(self._h, _ptr(x), _ptr(t), len(x)))
+
+    def sample_count(self) -> int:
+        n = C.c_uint64()
+        _check(lib().agx_mlp_sample_count(self._h, C.byref(n)))
+        return n.value
+
+    def samples(self, offset: int = 0, n: int | None = None):
+        n = self.sample_count() - offset if n is None else n
+        x = np.empty((n, MLP_INPUTS), np.float32)
+        t = np.empty((n, MLP_OUTPUTS), np.float32)
+        _check(lib().agx_mlp_read_samples(self._h, _ptr(x), _ptr(t), offset, n))
+        return x, t
+
+    def clear_samples(self):
+        _check(lib().agx_mlp_clear_samples(self._h))
+
+    def train(self, iterations: int, batch: int = 4096, learning_rate: float = 1e-3, seed: int = 1) -> float:
+        loss = C.c_float()
+        _check(lib().agx_mlp_train(self._h, iterations, batch, learning_rate, seed, C.byref(loss)))
+        return float(loss.value)
+
+
+def gate_update(g: Gate, used_learned: bool, cost: float) -> Gate:
+    """The residual gate's update on the host (the same code the device runs after each step)."""
+    _check(lib().agx_gate_update(C.byref(g), int(bool(used_learned)), cost))
+    return g
 
 
 def broadphase_pairs(aabb: np.ndarray, owner: np.ndarray, is_dynamic: np.ndarray, scene=None, cap=None):

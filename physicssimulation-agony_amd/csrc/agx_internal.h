@@ -21,10 +21,40 @@ constexpr uint32_t SORT_TILE = 1024;        // keys per radix-sort block (256 th
 constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one scene per chunk)
 
 // Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).
+// Residual gate of the learned warm start: exponential averages of what a step cost with the
+// cached warm start (0) and with the network's prediction (1). The cheaper method is used; every
+// GATE_PROBE_PERIOD steps the other one runs once so that its average stays current. A step whose
+// prediction cost more than the cached average is followed by a cached step at once, and a network
+// whose average falls behind loses the gate until a probe shows it cheaper again.
+constexpr uint32_t GATE_PROBE_PERIOD = 16;
+struct LearnGate {
+    float cost[2];
+    uint32_t use;            // method of the next step (1: learned)
+    uint32_t sinceProbe;
+    uint32_t steps[2];       // steps measured per method
+};
+
+__host__ __device__ inline void gateUpdate(LearnGate& g, uint32_t used, float cost) {
+    const uint32_t k = used ? 1u : 0u;
+    g.cost[k] = g.steps[k] == 0 ? cost : 0.75f * g.cost[k] + 0.25f * cost;
+    g.steps[k]++;
+    if (g.steps[1] == 0) { g.use = 1; return; }
+    if (g.steps[0] == 0) { g.use = 0; return; }
+    const uint32_t best = g.cost[1] < g.cost[0] ? 1u : 0u;
+    uint32_t next = best;
+    if (++g.sinceProbe >= GATE_PROBE_PERIOD) {
+        g.sinceProbe = 0;
+        next = 1u - best;
+    }
+    if (k == 1 && cost > g.cost[0]) next = 0;   // this prediction did worse than the cached average
+    g.use = next;
+}
+
 struct Counters {
     // persistent across steps (not cleared at step start)
     uint32_t prevManifolds;    // manifolds of the previous step (warm-start source)
     uint32_t hashCapacity;     // power of two >= 2 * prevManifolds (>= 16)
+    LearnGate gate;
     // per step
     uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
     uint32_t manifolds;
@@ -41,6 +71,10 @@ struct Counters {
     double residualDen[MAX_SWEEPS];
     float velocityError, positionError;
     uint32_t barrier[4];       // grid-barrier arrival counters (one per cooperative kernel)
+    uint32_t learnRows;        // feature rows of this step (2 per manifold; 0 when nothing is learned)
+    uint32_t learnPredictRows; // rows the network runs on (learnRows when the prediction is used, else 0)
+    uint32_t learnUsed;        // this step's warm start came from the network
+    uint32_t sampleBase;       // first training row this step claimed
 };
 
 constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)
@@ -183,6 +217,16 @@ struct DeviceWorld {
     int numCU = 0;
     int colourBlocks = 0;            // co-resident blocks of the dataflow colouring kernel
     int solveBlocks = 0;             // co-resident blocks of the dataflow solve kernel
+    // learned warm start: per contact point row 2m + p
+    uint16_t* learnX = nullptr;      // features (bf16, 32 per row)
+    float* learnY = nullptr;         // network outputs (2 per row)
+    // accumulator history: warmHist[c][m] holds the cached accumulators (jn0, jt0, jn1, jt1) manifold
+    // m of the step with cur == c started from; the narrow phase carries the matched previous
+    // manifold's entry into histOf (NaN for a point without one), i.e. the accumulators of two steps ago
+    float4* warmHist[2] = { nullptr, nullptr };
+    float4* histOf = nullptr;
+    const float4* histIn = nullptr;  // this step: warmHist of the previous step (null: not written)
+    float4* histOut = nullptr;       // this step: histOf (null: no learned warm start)
 };
 
 // ---------------------------------------------------------------- device math
@@ -313,8 +357,10 @@ hipError_t launchPairs(hipStream_t st, DeviceWorld& w);
 hipError_t launchNarrow(hipStream_t st, DeviceWorld& w);
 hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w);
 
+// Colouring, solve order and constraint prep; launchWarmStart then applies the warm-start
+// impulses (and clears the solve kernel's flags).
 hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
-// Colouring, solve order, constraint prep, warm start (and the solve kernel's cleared flags).
+hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // launchSolve: the velocity and position sweeps (one kernel); launchSolveFinish: accumulators
 // back to the manifolds, residual and error totals.
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
@@ -324,5 +370,7 @@ hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
 // Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels.
 void dataflowBlocksPerCU(int& colour, int& solve);
+// Message of the last failure outside a world (agx_last_error).
+void setLastError(const char* msg);
 
 }  // namespace agx

@@ -1,0 +1,267 @@
+// Learned warm start inside the step: contact features, the network's initial impulses, training
+// samples from the solver's own accumulators, and the residual gate.
+//
+// The reference warm-starts every contact point from the accumulators of the matching point of
+// the previous step (NarrowPhaseCollisionDetector.cpp:753-765, applied by applyWarmStarting,
+// BodyCollisionSolver.cpp:416-490). Here the network predicts a correction to that cached value:
+//   jn = max(0, jn_cached + S * y0),  jt = jt_cached + S * y1,
+// with S = normalMass * |g| * dt, the impulse that cancels one step of gravity along the normal.
+// An all-zero network therefore reproduces the cached warm start bit for bit, and the same
+// corrections, (jn_final - jn_cached) / S and (jt_final - jt_cached) / S, are the training targets.
+//
+// Features of contact point p of a manifold (all dimensionless, clamped to [-64, 64]):
+//   0, 1   cached jn, jt of the point / S          2, 3   cached jn, jt of the other point / S
+//   4      1 if the point had a cached impulse      5      1 if the manifold has two points
+//   6, 7   relative normal, tangent velocity / (|g| dt) before the warm start
+//   8      1 if the point has accumulators from two steps ago
+//   9      manifold depth / (|g| dt^2)              10, 11 normal along and across "up" (-g)
+//   12-15  invMassA, invMassB, rAn^2 invInertiaA, rBn^2 invInertiaB, each times normalMass
+//   16     tangentMass / normalMass
+//   17, 18 change of the cached jn, jt over the previous step / S (0 without history)
+//   19, 20 1 if body A, B is static                 21, 22 contacts of body A, B / 8
+//   23     relative normal velocity of the other point / (|g| dt)
+// Each manifold also records the cached accumulators it started from (warmHist), which the next
+// step's narrow phase carries over as the history of features 8, 17 and 18.
+#include "agx_internal.h"
+#include "mlp_internal.h"
+
+namespace agx {
+
+namespace {
+
+constexpr float FEATURE_CLAMP = 64.0f;
+
+struct Scale {
+    float gdt;        // |g| dt
+    float gdt2;       // |g| dt^2
+    float upX, upY;   // -g / |g|
+};
+
+__device__ inline float clampFeature(float v) {
+    if (!(v == v)) return 0.0f;
+    return fminf(fmaxf(v, -FEATURE_CLAMP), FEATURE_CLAMP);
+}
+
+// Normal mass of point p at a solve slot (0: a point that takes no impulse).
+__device__ inline float normalMassOf(const Constraints& cs, uint32_t slot, uint32_t p) {
+    const float4 k0 = cs.k0[slot];
+    return p == 0 ? k0.x : k0.w;
+}
+
+__global__ void kLearnRows(Counters* ctr, uint32_t predict, uint32_t gated, uint32_t collect) {
+    const uint32_t use = predict ? (gated ? ctr->gate.use : 1u) : 0u;
+    const uint32_t rows = 2u * ctr->manifolds;
+    ctr->learnUsed = use;
+    ctr->learnRows = (use || collect) ? rows : 0u;
+    ctr->learnPredictRows = use ? rows : 0u;
+}
+
+__global__ __launch_bounds__(256) void kLearnFeatures(const Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
+                                                      const uint32_t* __restrict__ posOf, const uint32_t* __restrict__ degree,
+                                                      const float4* __restrict__ vel, Constraints cs, Scale sc,
+                                                      const float4* __restrict__ histOf, float4* __restrict__ warmHist,
+                                                      uint16_t* __restrict__ x) {
+    // the starting accumulators are recorded on every step; features only when the network runs
+    // or rows are collected
+    const uint32_t M = ctr->manifolds;
+    const bool rows = ctr->learnRows != 0;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint32_t slot = posOf[m];
+        const float4 j = cs.imp[slot];
+        warmHist[m] = j;
+        if (!rows) continue;
+        const uint4 id = cs.idx[slot];
+        const float4 nr = cs.nrm[slot], iv = cs.inv[slot], r0 = cs.r0[slot], r1 = cs.r1[slot];
+        const float4 k0 = cs.k0[slot], k1 = cs.k1[slot];
+        const float4 vA = vel[id.x], vB = vel[id.y];
+        const float depth = man[m].depth;
+        const float4 hist = histOf[m];
+        const float hn[2] = { hist.x, hist.z }, ht[2] = { hist.y, hist.w };
+        const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
+        const float dvx = vB.x - vA.x, dvy = vB.y - vA.y;
+        const float dvn = dvx * nx + dvy * ny, dvt = dvx * tx + dvy * ty;
+        const float vn[2] = { dvn + r0.y * vB.z - r0.x * vA.z, dvn + r1.y * vB.z - r1.x * vA.z };
+        const float vt[2] = { dvt + r0.w * vB.z - r0.z * vA.z, dvt + r1.w * vB.z - r1.z * vA.z };
+        const float nMass[2] = { k0.x, k0.w }, tMass[2] = { k0.y, k1.x };
+        const float rAn[2] = { r0.x, r1.x }, rBn[2] = { r0.y, r1.y };
+        const float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w };
+        const float degA = iv.x != 0.0f ? (float)degree[id.x] : 0.0f;
+        const float degB = iv.y != 0.0f ? (float)degree[id.y] : 0.0f;
+        const float invG = 1.0f / sc.gdt;
+        for (uint32_t p = 0; p < 2; p++) {
+            float f[MLP_INP];
+            for (uint32_t k = 0; k < MLP_INP; k++) f[k] = 0.0f;
+            const float S = nMass[p] * sc.gdt;
+            if (p < id.z && S > 0.0f) {
+                const uint32_t o = 1u - p;
+                const float invS = 1.0f / S;
+                const bool two = id.z > 1;
+                f[0] = jn[p] * invS;
+                f[1] = jt[p] * invS;
+                f[2] = two ? jn[o] * invS : 0.0f;
+                f[3] = two ? jt[o] * invS : 0.0f;
+                f[4] = (jn[p] != 0.0f || jt[p] != 0.0f) ? 1.0f : 0.0f;
+                f[5] = two ? 1.0f : 0.0f;
+                f[6] = vn[p] * invG;
+                f[7] = vt[p] * invG;
+                const bool hasHist = hn[p] == hn[p] && ht[p] == ht[p];
+                f[8] = hasHist ? 1.0f : 0.0f;
+                f[9] = depth / sc.gdt2;
+                f[10] = nx * sc.upX + ny * sc.upY;
+                f[11] = nx * sc.upY - ny * sc.upX;
+                f[12] = iv.x * nMass[p];
+                f[13] = iv.y * nMass[p];
+                f[14] = rAn[p] * rAn[p] * iv.z * nMass[p];
+                f[15] = rBn[p] * rBn[p] * iv.w * nMass[p];
+                f[16] = tMass[p] / nMass[p];
+                f[17] = hasHist ? (jn[p] - hn[p]) * invS : 0.0f;
+                f[18] = hasHist ? (jt[p] - ht[p]) * invS : 0.0f;
+                f[19] = iv.x == 0.0f ? 1.0f : 0.0f;
+                f[20] = iv.y == 0.0f ? 1.0f : 0.0f;
+                f[21] = degA * 0.125f;
+                f[22] = degB * 0.125f;
+                f[23] = two ? vn[o] * invG : 0.0f;
+            }
+            uint32_t packed[MLP_INP / 2];
+            for (uint32_t k = 0; k < MLP_INP / 2; k++)
+                packed[k] = (uint32_t)bf16Bits(clampFeature(f[2 * k])) | ((uint32_t)bf16Bits(clampFeature(f[2 * k + 1])) << 16);
+            uint4* row = reinterpret_cast<uint4*>(x + (size_t)(2 * m + p) * MLP_INP);
+            for (uint32_t k = 0; k < MLP_INP / 8; k++)
+                row[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+        }
+    }
+}
+
+// Initial accumulators from the network's corrections (only on steps the gate gave to it).
+__global__ __launch_bounds__(256) void kLearnApply(const Counters* __restrict__ ctr, const uint32_t* __restrict__ posOf,
+                                                   Constraints cs, const float* __restrict__ y, float gdt) {
+    if (!ctr->learnUsed) return;
+    const uint32_t M = ctr->learnPredictRows / 2;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint32_t slot = posOf[m];
+        const uint32_t count = cs.idx[slot].z;
+        float4 j = cs.imp[slot];
+        float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w };
+        for (uint32_t p = 0; p < count && p < 2; p++) {
+            const float S = normalMassOf(cs, slot, p) * gdt;
+            const float y0 = y[(size_t)(2 * m + p) * MLP_OUT], y1 = y[(size_t)(2 * m + p) * MLP_OUT + 1];
+            if (!(S > 0.0f) || !isfinite(y0) || !isfinite(y1)) continue;
+            jn[p] = fmaxf(0.0f, jn[p] + S * y0);
+            jt[p] = jt[p] + S * y1;
+        }
+        cs.imp[slot] = make_float4(jn[0], jt[0], jn[1], jt[1]);
+    }
+}
+
+// Training rows from this step: features of every contact point and, as targets, the corrections
+// that would have turned the cached accumulators into the solver's final ones. Runs after the
+// sweeps and before the accumulators are written back to the manifolds (which still hold the
+// cached values). Rows are claimed per wave; rows past the buffer's capacity are dropped.
+__global__ __launch_bounds__(256) void kLearnCollect(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
+                                                     const uint32_t* __restrict__ posOf, Constraints cs,
+                                                     const uint16_t* __restrict__ x, float gdt, uint16_t* __restrict__ sx,
+                                                     float* __restrict__ st, uint32_t* __restrict__ sampleCount,
+                                                     uint64_t cap) {
+    const uint32_t M = ctr->learnRows / 2;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane of a wave runs the same number of rounds so the wave-wide claim stays uniform
+    for (uint32_t base = first - lane; base < M; base += stride) {
+        const uint32_t m = base + lane;
+        uint32_t count = 0, slot = 0;
+        if (m < M) {
+            slot = posOf[m];
+            count = min(cs.idx[slot].z, 2u);
+        }
+        uint32_t prefix = count;   // inclusive prefix over the wave
+        for (uint32_t off = 1; off < WAVE; off <<= 1) {
+            const uint32_t v = __shfl_up(prefix, off, WAVE);
+            if (lane >= off) prefix += v;
+        }
+        const uint32_t total = __shfl(prefix, WAVE - 1, WAVE);
+        uint32_t row0 = 0;
+        if (lane == WAVE - 1 && total) row0 = atomicAdd(sampleCount, total);
+        row0 = __shfl(row0, WAVE - 1, WAVE) + prefix - count;
+        if (m >= M || count == 0) continue;
+        const agx_manifold d = man[m];
+        const float4 j = cs.imp[slot];
+        const float jnC[2] = { d.jn0, d.jn1 }, jtC[2] = { d.jt0, d.jt1 };
+        const float jnF[2] = { j.x, j.z }, jtF[2] = { j.y, j.w };
+        for (uint32_t p = 0; p < count; p++) {
+            const uint64_t row = (uint64_t)row0 + p;
+            if (row >= cap) break;
+            const float S = normalMassOf(cs, slot, p) * gdt;
+            const float invS = S > 0.0f ? 1.0f / S : 0.0f;
+            const uint4* src = reinterpret_cast<const uint4*>(x + (size_t)(2 * m + p) * MLP_INP);
+            uint4* dst = reinterpret_cast<uint4*>(sx + row * MLP_INP);
+            for (uint32_t k = 0; k < MLP_INP / 8; k++) dst[k] = src[k];
+            st[row * MLP_OUT] = clampFeature((jnF[p] - jnC[p]) * invS);
+            st[row * MLP_OUT + 1] = clampFeature((jtF[p] - jtC[p]) * invS);
+        }
+    }
+}
+
+// The step's cost for the gate: velocity sweeps run to reach the residual target (plus a penalty
+// of log2(residual / target) when the sweep cap stopped it first), or, without a target, the
+// log10 of the last sweep's residual.
+__global__ void kGateUpdate(Counters* ctr, float residualTarget) {
+    const uint32_t sweeps = min(ctr->sweeps, MAX_SWEEPS);
+    if (sweeps == 0) return;
+    const double num = ctr->residualNum[sweeps - 1], den = ctr->residualDen[sweeps - 1];
+    const float res = (float)(sqrt(num) / fmax(sqrt(den), 1e-12));
+    float cost;
+    if (residualTarget > 0.0f)
+        cost = (float)sweeps + (res > residualTarget ? log2f(res / residualTarget) : 0.0f);
+    else
+        cost = log10f(fmaxf(res, 1e-12f));
+    LearnGate g = ctr->gate;
+    gateUpdate(g, ctr->learnUsed, cost);
+    ctr->gate = g;
+}
+
+inline uint32_t gridOf(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8192)); }
+
+Scale scaleOf(float gx, float gy, float dt) {
+    const float g = std::sqrt(gx * gx + gy * gy);
+    Scale s;
+    const float gs = std::max(g, 1.0f);
+    s.gdt = gs * dt;
+    s.gdt2 = gs * dt * dt;
+    s.upX = g > 0.0f ? -gx / g : 0.0f;
+    s.upY = g > 0.0f ? -gy / g : 1.0f;
+    return s;
+}
+
+}  // namespace
+
+hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice* mlp, uint32_t predict, uint32_t gated,
+                                 uint32_t collect, float gx, float gy, float dt) {
+    hipLaunchKernelGGL(kLearnRows, dim3(1), dim3(1), 0, st, w.ctr, predict && mlp ? 1u : 0u, gated, collect);
+    const Scale sc = scaleOf(gx, gy, dt);
+    const uint32_t grid = gridOf(std::min<uint64_t>(w.manCap, 1u << 22));
+    hipLaunchKernelGGL(kLearnFeatures, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.posOf, w.degree, w.vel, w.cs, sc,
+                       w.histOf, w.warmHist[w.cur], w.learnX);
+    if (predict && mlp) {
+        const uint32_t rows = (uint32_t)std::min<uint64_t>(2 * w.manCap, 0xFFFFFFF0u);
+        hipError_t e = launchMlpForward(st, *mlp, w.learnX, rows, w.learnY, nullptr, nullptr, &w.ctr->learnPredictRows);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kLearnApply, dim3(grid), dim3(256), 0, st, w.ctr, w.posOf, w.cs, w.learnY, sc.gdt);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, float gx, float gy, float dt) {
+    const Scale sc = scaleOf(gx, gy, dt);
+    const uint32_t grid = gridOf(std::min<uint64_t>(w.manCap, 1u << 22));
+    hipLaunchKernelGGL(kLearnCollect, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.posOf, w.cs, w.learnX, sc.gdt,
+                       smp.x, smp.t, smp.count, smp.cap);
+    return hipGetLastError();
+}
+
+hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w, float residualTarget) {
+    hipLaunchKernelGGL(kGateUpdate, dim3(1), dim3(1), 0, st, w.ctr, residualTarget);
+    return hipGetLastError();
+}
+
+}  // namespace agx

@@ -482,7 +482,8 @@ __global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__
                                                 const uint32_t* __restrict__ scan, Counters* __restrict__ ctr,
                                                 agx_manifold* __restrict__ cur, const agx_manifold* __restrict__ prev,
                                                 const unsigned long long* __restrict__ hashKeys,
-                                                const uint32_t* __restrict__ hashVals, const uint32_t* __restrict__ colSlot) {
+                                                const uint32_t* __restrict__ hashVals, const uint32_t* __restrict__ colSlot,
+                                                const float4* __restrict__ histIn, float4* __restrict__ histOut) {
     const uint32_t n = ctr->scratch[1];
     const uint32_t prevCount = ctr->prevManifolds;
     const uint32_t cap = ctr->hashCapacity;
@@ -492,6 +493,8 @@ __global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__
         if (!flags[p]) continue;
         agx_manifold m = slots[p];
         contacts += m.count;
+        const float none = __builtin_nanf("");
+        float4 hist = make_float4(none, none, none, none);   // accumulators of two steps ago, per point
         if (prevCount > 0 && cap > 0) {
             const unsigned long long key = pairKey(colSlot[m.a], colSlot[m.b]);
             uint32_t h = hashSlot(key, capLog2);
@@ -505,16 +508,18 @@ __global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__
             if (found != 0xFFFFFFFFu) {
                 const agx_manifold& q = prev[found];
                 if (q.a == m.a && q.b == m.b) {
-                    if (q.count > 0 && m.id0 == q.id0) { m.jn0 = q.jn0; m.jt0 = q.jt0; }
-                    else if (q.count > 1 && m.id0 == q.id1) { m.jn0 = q.jn1; m.jt0 = q.jt1; }
+                    const float4 qh = histIn ? histIn[found] : hist;
+                    if (q.count > 0 && m.id0 == q.id0) { m.jn0 = q.jn0; m.jt0 = q.jt0; hist.x = qh.x; hist.y = qh.y; }
+                    else if (q.count > 1 && m.id0 == q.id1) { m.jn0 = q.jn1; m.jt0 = q.jt1; hist.x = qh.z; hist.y = qh.w; }
                     if (m.count > 1) {
-                        if (q.count > 0 && m.id1 == q.id0) { m.jn1 = q.jn0; m.jt1 = q.jt0; }
-                        else if (q.count > 1 && m.id1 == q.id1) { m.jn1 = q.jn1; m.jt1 = q.jt1; }
+                        if (q.count > 0 && m.id1 == q.id0) { m.jn1 = q.jn0; m.jt1 = q.jt0; hist.z = qh.x; hist.w = qh.y; }
+                        else if (q.count > 1 && m.id1 == q.id1) { m.jn1 = q.jn1; m.jt1 = q.jt1; hist.z = qh.z; hist.w = qh.w; }
                     }
                 }
             }
         }
         cur[scan[p]] = m;
+        if (histOut) histOut[scan[p]] = hist;
     }
     for (uint32_t off = 32; off > 0; off >>= 1) contacts += __shfl_down(contacts, off, 64);
     if ((threadIdx.x & 63u) == 0 && contacts) atomicAdd(&ctr->contacts, contacts);
@@ -573,7 +578,7 @@ hipError_t launchNarrow(hipStream_t st, DeviceWorld& w) {
                               w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kCompact, dim3(blocks), dim3(256), 0, st, w.slots, w.flags, w.flagScan, w.ctr, w.man[w.cur],
-                       w.man[1 - w.cur], w.hashKeys, w.hashVals, w.colSlot);
+                       w.man[1 - w.cur], w.hashKeys, w.hashVals, w.colSlot, w.histIn, w.histOut);
     hipLaunchKernelGGL(kSetManifolds, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }

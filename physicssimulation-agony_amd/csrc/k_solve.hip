@@ -894,7 +894,6 @@ void dataflowBlocksPerCU(int& colour, int& solve) {
 
 hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
     const uint32_t gm = blocksForManifolds(w);
-    const uint32_t gd = blocksFor(std::max(1u, w.nDynBodies));
     hipLaunchKernelGGL(kFlowClear, dim3(blocksFor(w.N)), dim3(256), 0, st, w.N, w.degree, w.incFill, w.colRec);
     hipLaunchKernelGGL(kDegree, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody, w.inv, w.manBodies, w.degree);
     hipError_t e = launchScan(st, w.degree, w.incStart, nullptr, &w.ctr->scratch[4], w.N, w.scanBlock, w.scanBlockCap);
@@ -917,10 +916,15 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
                        w.posOf, w.rank, w.incSorted);
     hipLaunchKernelGGL(kPrep, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
                        w.incStart, w.incSorted, w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
+    return hipGetLastError();
+}
+
+hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w) {
+    const uint32_t gd = blocksFor(std::max(1u, w.nDynBodies));
     hipLaunchKernelGGL(kWarm, dim3(gd), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.degree, w.incStart, w.incSorted,
                        w.posOf, w.cs, w.vel);
     // cleared for the solve kernel: its grid-barrier counter and the group progress flags
-    e = launchFillU32(st, &w.ctr->barrier[1], 0u, 1);
+    hipError_t e = launchFillU32(st, &w.ctr->barrier[1], 0u, 1);
     if (e != hipSuccess) return e;
     return launchFillU32(st, w.groupDone, 0u, 2ull * w.solveGroups);
 }

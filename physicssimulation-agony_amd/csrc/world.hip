@@ -1,6 +1,7 @@
 // C ABI (include/agx.h): scene building, device world lifetime, step orchestration, read-outs.
 #include "agx_internal.h"
 #include "builder.h"
+#include "mlp_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -18,11 +19,11 @@ struct agx_builder {
 namespace {
 enum Segment : uint32_t {
     SEG_INTEGRATE, SEG_POSE, SEG_CCD, SEG_TREE, SEG_PAIRS, SEG_NARROW, SEG_COLOUR, SEG_SOLVE, SEG_SOLVE_FINISH,
-    SEG_ISLANDS, SEG_FINISH, SEG_POST_UPDATE, SEG_COUNT
+    SEG_ISLANDS, SEG_FINISH, SEG_POST_UPDATE, SEG_LEARN, SEG_COUNT
 };
 const char* const kSegmentNames[SEG_COUNT] = { "integrate", "pose", "ccd", "tree_build", "pair_traversal", "narrow",
                                                "colour_prep", "solve", "solve_finish", "islands", "finish",
-                                               "post_update" };
+                                               "post_update", "learned" };
 struct EventPair { hipEvent_t a, b; uint32_t seg; };
 }  // namespace
 
@@ -46,6 +47,9 @@ struct agx_world {
     std::vector<hipEvent_t> eventPool;
     double segMs[SEG_COUNT] = {};
     uint32_t segLaunches[SEG_COUNT] = {};
+    agx_mlp* mlp = nullptr;          // learned warm start (not owned)
+    uint32_t learnMode = 0;          // AGX_LEARN_* flags
+    bool histReady = false;          // the previous step recorded its manifolds' starting accumulators
 };
 
 namespace {
@@ -62,6 +66,14 @@ int fail(agx_world* w, int code, const char* fmt, ...) {
     g_lastError = buf;
     return code;
 }
+
+}  // namespace
+
+namespace agx {
+void setLastError(const char* msg) { g_lastError = msg ? msg : ""; }
+}  // namespace agx
+
+namespace {
 
 #define HIP_CHECK(w, call)                                                                              \
     do {                                                                                                \
@@ -210,24 +222,47 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
         break;
     }
     case AGX_PHASE_NARROW: {
+        const bool learned = w->mlp && !s.collide_only;
+        d.histOut = learned ? d.histOf : nullptr;
+        d.histIn = (learned && w->histReady) ? d.warmHist[1 - d.cur] : nullptr;
         SegmentTimer t(w, SEG_NARROW);
         HIP_CHECK(w, agx::launchNarrow(st, d));
         break;
     }
     case AGX_PHASE_SOLVE:
         if (!s.collide_only) {
+            const bool predict = w->mlp && (w->learnMode & AGX_LEARN_PREDICT);
+            const bool collect = w->mlp && (w->learnMode & AGX_LEARN_COLLECT);
+            const bool gated = predict && !(w->learnMode & AGX_LEARN_UNGATED);
+            // the gate judges each step by its residual, so a gated world always tracks it
+            const int trackResidual = (s.track_residual || gated) ? 1 : 0;
             {
                 SegmentTimer t(w, SEG_COLOUR);
                 HIP_CHECK(w, agx::launchColourPrep(st, d));
             }
+            if (predict || collect) {
+                SegmentTimer t(w, SEG_LEARN);
+                if (!d.histOut) return fail(w, AGX_ERR_STATE, "learned mode was enabled after this step's narrow phase");
+                HIP_CHECK(w, agx::launchLearnBeforeWarm(st, d, agx::mlpDevice(w->mlp), predict ? 1u : 0u, gated ? 1u : 0u,
+                                                        collect ? 1u : 0u, s.gravity_x, s.gravity_y, dt));
+            }
+            w->histReady = predict || collect;
+            {
+                SegmentTimer t(w, SEG_COLOUR);
+                HIP_CHECK(w, agx::launchWarmStart(st, d));
+            }
             {
                 SegmentTimer t(w, SEG_SOLVE);
-                HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations,
-                                              s.track_residual ? 1 : 0, s.residual_target));
+                HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations, trackResidual,
+                                              s.residual_target));
+            }
+            if (collect) {
+                SegmentTimer t(w, SEG_LEARN);
+                HIP_CHECK(w, agx::launchLearnCollect(st, d, *agx::mlpSamples(w->mlp), s.gravity_x, s.gravity_y, dt));
             }
             SegmentTimer t(w, SEG_SOLVE_FINISH);
-            HIP_CHECK(w, agx::launchSolveFinish(st, d, s.track_residual ? 1 : 0, s.residual_target,
-                                                s.track_errors ? 1 : 0));
+            HIP_CHECK(w, agx::launchSolveFinish(st, d, trackResidual, s.residual_target, s.track_errors ? 1 : 0));
+            if (gated) HIP_CHECK(w, agx::launchGateUpdate(st, d, s.residual_target));
         }
         break;
     case AGX_PHASE_ISLANDS: {
@@ -587,6 +622,61 @@ int agx_world_step(agx_world* w, float dt, uint32_t n_steps) {
     return n_steps ? postUpdate(w) : AGX_OK;
 }
 
+// ---- learned warm start
+int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (mode & ~(AGX_LEARN_PREDICT | AGX_LEARN_COLLECT | AGX_LEARN_UNGATED))
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown learned mode bits 0x%x", mode);
+    if ((mode & (AGX_LEARN_PREDICT | AGX_LEARN_COLLECT)) && !mlp)
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "learned mode needs a network");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    agx::DeviceWorld& d = w->d;
+    if ((mode & (AGX_LEARN_PREDICT | AGX_LEARN_COLLECT)) && !d.learnX) {
+        const size_t man = std::max<uint64_t>(d.manCap, 1), rows = 2 * man;
+        if (devAlloc(w, &d.learnX, rows * agx::MLP_INP) != hipSuccess || devAlloc(w, &d.learnY, rows * agx::MLP_OUT) != hipSuccess ||
+            devAlloc(w, &d.warmHist[0], man) != hipSuccess || devAlloc(w, &d.warmHist[1], man) != hipSuccess ||
+            devAlloc(w, &d.histOf, man) != hipSuccess)
+            return fail(w, AGX_ERR_HIP, "cannot allocate the learned warm-start buffers (%zu rows)", rows);
+    }
+    w->mlp = (mode & (AGX_LEARN_PREDICT | AGX_LEARN_COLLECT)) ? mlp : nullptr;
+    w->learnMode = w->mlp ? mode : 0u;
+    w->histReady = false;
+    return agx_world_reset_gate(w);
+}
+
+int agx_world_reset_gate(agx_world* w) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    agx::LearnGate g{};
+    g.use = 1;   // the first step tries the network
+    HIP_CHECK(w, hipMemcpyAsync(&w->d.ctr->gate, &g, sizeof g, hipMemcpyHostToDevice, w->stream));
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    return AGX_OK;
+}
+
+int agx_world_read_gate(agx_world* w, agx_gate* out) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    agx::LearnGate g{};
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    HIP_CHECK(w, hipMemcpy(&g, &w->d.ctr->gate, sizeof g, hipMemcpyDeviceToHost));
+    out->cost_cached = g.cost[0]; out->cost_learned = g.cost[1];
+    out->use_learned = g.use; out->since_probe = g.sinceProbe;
+    out->steps_cached = g.steps[0]; out->steps_learned = g.steps[1];
+    return AGX_OK;
+}
+
+int agx_gate_update(agx_gate* out, uint32_t used_learned, float cost) {
+    if (!out) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "gate is null");
+    agx::LearnGate g{};
+    g.cost[0] = out->cost_cached; g.cost[1] = out->cost_learned;
+    g.use = out->use_learned; g.sinceProbe = out->since_probe;
+    g.steps[0] = out->steps_cached; g.steps[1] = out->steps_learned;
+    agx::gateUpdate(g, used_learned, cost);
+    out->cost_cached = g.cost[0]; out->cost_learned = g.cost[1];
+    out->use_learned = g.use; out->since_probe = g.sinceProbe;
+    out->steps_cached = g.steps[0]; out->steps_learned = g.steps[1];
+    return AGX_OK;
+}
+
 int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps) {
     return agx_world_step(w, dt, n_steps);
 }
@@ -763,6 +853,7 @@ int agx_world_write_manifolds(agx_world* w, uint32_t which, const agx_manifold* 
     HIP_CHECK(w, agx::launchHashBuild(w->stream, d));
     d.cur ^= 1;
     HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    w->histReady = false;   // the new source has no recorded starting accumulators
     return AGX_OK;
 }
 
@@ -816,6 +907,7 @@ int agx_world_read_stats(agx_world* w, agx_stats* out) {
     out->ccd_hits = c->ccdHits;
     out->velocity_sweeps = c->sweeps;
     out->overflow = c->overflow;
+    out->learned_warm_start = c->learnUsed;
     out->velocity_error = c->velocityError;
     out->position_error = c->positionError;
     for (uint32_t k = 0; k < 64 && k < agx::MAX_SWEEPS; k++)

@@ -35,6 +35,9 @@ _SIGS = {
     "orc_step": (None, [_P, C.c_float]), "orc_post_update": (None, [_P]),
     "orc_update": (None, [_P, C.c_float, C.c_uint32]),
     "orc_get_manifolds": (C.c_int64, [_P, _P, C.c_uint64]), "orc_errors": (None, [_P, _P]),
+    "orc_mlp_forward": (None, [_P, _P, C.c_uint64, _P, C.c_int]),
+    "orc_mlp_gradients": (C.c_double, [_P, _P, _P, C.c_uint64, _P, C.c_int]),
+    "orc_mlp_loss": (C.c_double, [_P, _P, _P, _P, C.c_uint64]),
 }
 _LIB = None
 
@@ -226,3 +229,34 @@ def world_from_scene(scene, settings=None):
     idx, rc = w.add_bodies(scene["bodies"])
     w.refresh()
     return w
+
+
+# ---- network of the learned warm start (oracle/mlp.cpp)
+MLP_PARAMS = 19970
+
+
+def mlp_forward(params, x, rounding=True):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros((x.shape[0], 2), np.float32)
+    lib().orc_mlp_forward(_p(params), _p(x), x.shape[0], _p(y), 1 if rounding else 0)
+    return y
+
+
+def mlp_gradients(params, x, t, rounding=True):
+    """(loss, gradient in host parameter layout) of the mean squared error over the rows."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = lib().orc_mlp_gradients(_p(params), _p(x), _p(t), x.shape[0], _p(g), 1 if rounding else 0)
+    return float(loss), g
+
+
+def mlp_loss(params, x, t, delta=None):
+    """Exact (double, unrounded) loss with parameters params + delta."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    d = None if delta is None else np.ascontiguousarray(delta, np.float64)
+    return float(lib().orc_mlp_loss(_p(params), _p(d), _p(x), _p(t), x.shape[0]))

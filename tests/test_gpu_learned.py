@@ -1,0 +1,155 @@
+"""GPU checks of the learned warm start: the MFMA forward and backward kernels against the oracle
+network, training on the GPU, the world's feature/collect/predict hooks and the residual gate."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+IN, H, OUT = 24, 128, 2
+TENSORS = {"W1": (0, H * IN), "b1": (H * IN, H * IN + H), "W2": (H * IN + H, H * IN + H + H * H),
+           "b2": (H * IN + H + H * H, H * IN + 2 * H + H * H), "W3": (H * IN + 2 * H + H * H, H * IN + 2 * H + H * H + 2 * H),
+           "b3": (H * IN + 2 * H + H * H + 2 * H, 19970)}
+
+
+def features(rng, n):
+    x = rng.normal(0, 1, (n, IN)).astype(np.float32)
+    x[:, 4:6] = rng.integers(0, 2, (n, 2))          # flags
+    x[:, 0] = np.abs(rng.normal(0, 8, n))            # cached impulses span a wide range
+    return x
+
+
+def test_forward_matches_oracle(agx, oracle):
+    m = agx.Mlp(seed=3, sample_capacity=16)
+    x = features(np.random.default_rng(1), 1000)
+    y = m.forward(x)
+    yo = oracle.mlp_forward(m.params(), x, rounding=True)
+    scale = max(1.0, float(np.abs(yo).max()))
+    err = np.abs(y - yo)
+    # same bf16 rounding points; only fp32 summation order differs (it can flip a bf16 rounding)
+    assert err.max() <= 2e-2 * scale
+    assert np.sqrt((err ** 2).mean()) <= 2e-3 * scale
+
+
+def test_gradients_match_oracle(agx, oracle):
+    m = agx.Mlp(seed=5, sample_capacity=16)
+    rng = np.random.default_rng(2)
+    x = features(rng, 512)
+    t = rng.normal(0, 2, (512, OUT)).astype(np.float32)
+    loss, g = m.gradients(x, t)
+    lo, go = oracle.mlp_gradients(m.params(), x, t, rounding=True)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    for name, (a, b) in TENSORS.items():
+        rel = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-12)
+        assert rel < 2e-2, (name, rel)
+    # a batch that is not a multiple of 16 is rejected
+    with pytest.raises(agx.AgxError):
+        m.gradients(x[:100], t[:100])
+
+
+def test_training_on_gpu_fits_a_smooth_target(agx, oracle):
+    rng = np.random.default_rng(3)
+    x = features(rng, 1 << 16)
+    t = np.stack([0.5 * x[:, 0] + np.tanh(x[:, 6]), x[:, 7] * x[:, 10] - 0.3 * x[:, 2]], 1).astype(np.float32)
+    m = agx.Mlp(seed=7, sample_capacity=1 << 17)
+    m.add_samples(x, t)
+    assert m.sample_count() == len(x)
+    xr, tr = m.samples(100, 10)
+    np.testing.assert_array_equal(tr, t[100:110])
+    y0 = m.forward(x[:4096])
+    before = float(((y0 - t[:4096]) ** 2).sum(1).mean())
+    m.train(1500, 2048, 1e-3, seed=1)
+    m.train(500, 2048, 3e-4, seed=2)
+    y1 = m.forward(x[:4096])
+    after = float(((y1 - t[:4096]) ** 2).sum(1).mean())
+    assert after < before / 20, (before, after)
+    # the trained parameters still agree with the oracle's forward pass
+    yo = oracle.mlp_forward(m.params(), x[:2048], rounding=True)
+    assert np.abs(yo - y1[:2048]).max() <= 2e-2 * max(1.0, float(np.abs(yo).max()))
+
+
+def _world(agx, scenes, **settings):
+    scene = scenes.pile_small(seed=1)
+    b = agx.Builder()
+    agx.build_scene(b, scene)
+    s = agx.default_settings(**scene["settings"])
+    for k, v in settings.items():
+        setattr(s, k, v)
+    return agx.World(b, s)
+
+
+def _state(w):
+    return np.stack([w.body(f) for f in ("com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel")])
+
+
+def test_zero_network_reproduces_cached_warm_start(agx, scenes):
+    """Corrections of an all-zero network are exactly zero, so predict mode is bit-identical to the
+    cached warm start (fixed sweep counts keep both runs on the same schedule)."""
+    a = _world(agx, scenes)
+    b = _world(agx, scenes)
+    m = agx.Mlp(seed=1, sample_capacity=16)
+    m.set_params(np.zeros(agx.MLP_PARAMS, np.float32))
+    dt = 1.0 / 60.0
+    a.step(dt, 30)
+    b.step(dt, 30)
+    np.testing.assert_array_equal(_state(a), _state(b))
+    b.set_learned(m, agx.LEARN_PREDICT | agx.LEARN_UNGATED)
+    a.step(dt, 20)
+    b.step(dt, 20)
+    assert b.stats().learned_warm_start == 1 and a.stats().learned_warm_start == 0
+    np.testing.assert_array_equal(_state(a), _state(b))
+    # a nonzero correction does change the step (the prediction is applied)
+    p = np.zeros(agx.MLP_PARAMS, np.float32)
+    p[-2] = 0.5     # b3[0]: +0.5 S on every normal impulse
+    m.set_params(p)
+    a.step(dt, 1)
+    b.step(dt, 1)
+    assert not np.array_equal(_state(a), _state(b))
+
+
+def test_collect_rows_are_the_steps_contact_points(agx, scenes):
+    w = _world(agx, scenes)
+    dt = 1.0 / 60.0
+    w.step(dt, 40)
+    m = agx.Mlp(seed=1, sample_capacity=1 << 16)
+    w.set_learned(m, agx.LEARN_COLLECT)
+    w.step(dt, 1)
+    st = w.stats()
+    assert st.learned_warm_start == 0
+    assert m.sample_count() == st.contacts > 0
+    x, t = m.samples()
+    assert np.isfinite(x).all() and np.isfinite(t).all()
+    assert set(np.unique(x[:, 4])) <= {0.0, 1.0} and set(np.unique(x[:, 5])) <= {0.0, 1.0}
+    # the final normal impulse is never negative: target >= -(cached normal impulse) in units of S
+    assert (t[:, 0] + x[:, 0] >= -1e-2 * np.maximum(1.0, np.abs(x[:, 0]))).all()
+    # a settled pile mostly reuses cached impulses
+    assert x[:, 4].mean() > 0.5
+    w.step(dt, 1)
+    assert m.sample_count() == st.contacts + w.stats().contacts
+    w.set_learned(None, agx.LEARN_OFF)
+    n = m.sample_count()
+    w.step(dt, 1)
+    assert m.sample_count() == n
+
+
+def test_gate_rejects_a_harmful_network(agx, scenes):
+    """A network that cancels every cached normal impulse (a cold start) raises the residual: the
+    gate hands the warm start back to the cached impulses and only probes the network afterwards."""
+    # judged by the residual left after 8 sweeps (this pile does not reach 1e-3 within 64 sweeps)
+    w = _world(agx, scenes, velocity_iterations=8, track_residual=1)
+    dt = 1.0 / 60.0
+    w.step(dt, 40)
+    m = agx.Mlp(seed=1, sample_capacity=16)
+    p = np.zeros(agx.MLP_PARAMS, np.float32)
+    p[-2] = -1e4     # b3[0]: jn = max(0, jn_cached - 1e4 S) = 0
+    m.set_params(p)
+    w.set_learned(m, agx.LEARN_PREDICT)
+    used = []
+    for _ in range(48):
+        w.step(dt, 1)
+        used.append(int(w.stats().learned_warm_start))
+    g = w.gate()
+    info = (used, g.cost_learned, g.cost_cached)
+    assert g.cost_learned > g.cost_cached, info
+    assert g.use_learned == 0, info
+    assert sum(used[8:]) <= 4, used
+    assert g.steps_cached + g.steps_learned == 48
