@@ -176,41 +176,97 @@ def build_world(agx, scenes, settings):
     return agx.World(b, settings)
 
 
-def run_batch(agx, scenes_mod, args, dt):
+def collect_and_train(agx, mlp, learned_world, plain_world, dt, steps, every, mode, iterations, lr):
+    """Steps both worlds `steps` times in lockstep; the learned world runs in `mode` and adds the
+    training rows of every `every`-th step; then the network trains `iterations` batches on the GPU."""
+    for k in range(steps):
+        collect = k % every == 0
+        learned_world.set_learned(mlp, mode | (agx.LEARN_COLLECT if collect else 0))
+        learned_world.step(dt, 1)
+        plain_world.step(dt, 1)
+    learned_world.sync()
+    t0 = time.perf_counter()
+    loss = 0.0
+    for e in range(iterations // 1000):
+        loss = mlp.train(1000, 4096, lr if e < (iterations // 1000) * 7 // 10 else lr / 5, seed=mode * 100 + e + 1)
+    return loss, time.perf_counter() - t0
+
+
+def run_batch(agx, scenes_mod, args, dt, learned=True):
+    """Plain and learned warm starts on two copies of the batch stepped in lockstep, so both are
+    timed from the same settled state. The network is trained on the GPU on the solver's own
+    accumulators from the learned copy's untimed steps: first with the cached warm start, then
+    on its own predictions."""
     t0 = time.perf_counter()
     scenes = scenes_mod.batch(1024)
     base = dict(position_iterations=3, track_residual=1, track_errors=1)
-    world = build_world(agx, scenes, agx.default_settings(velocity_iterations=8, **base))
-    n = world.counts.bodies
+    fixed8 = agx.default_settings(velocity_iterations=8, **base)
+    to_target = agx.default_settings(velocity_iterations=64, residual_target=1e-3, **base)
+    worlds = [build_world(agx, scenes, fixed8) for _ in range(2 if learned else 1)]
+    plain_w = worlds[0]
+    n = plain_w.counts.bodies
     log(f"batch: {len(scenes)} scenes, {n} bodies built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    for k in range(0, args.warmup, 40):
-        world.step(dt, min(40, args.warmup - k))
-    world.sync()
+    for w in worlds:
+        for k in range(0, args.warmup, 40):
+            w.step(dt, min(40, args.warmup - k))
+        w.sync()
     log(f"batch: warm-up {args.warmup} steps in {time.perf_counter() - t0:.1f} s")
     out = {}
     # plain solver, fixed 8 velocity sweeps
-    el = step_timed(world, args.steps, dt)
+    el = step_timed(plain_w, args.steps, dt)
     out["plain_fixed8"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
                            "velocity_sweeps": 8}
-    # plain solver to the residual target
-    to_target = agx.default_settings(velocity_iterations=64, residual_target=1e-3, **base)
-    world.set_settings(to_target)
-    el = step_timed(world, args.steps, dt)
-    st = world.stats()
-    sweeps_plain = int(st.velocity_sweeps)
+    for w in worlds[1:]:
+        w.step(dt, args.steps)
+    for w in worlds:
+        w.set_settings(to_target)
+    if learned:
+        lw = worlds[1]
+        mlp = agx.Mlp(seed=1, sample_capacity=16 << 20)
+        t0 = time.perf_counter()
+        loss, tr = collect_and_train(agx, mlp, lw, plain_w, dt, 4, 1, agx.LEARN_OFF, 10000, 1e-3)
+        net = {"rows": mlp.sample_count(), "train_iterations": 10000, "batch": 4096, "final_batch_loss": round(loss, 4),
+               "train_s": round(tr, 2), "collect_and_train_s": round(time.perf_counter() - t0, 2),
+               "data": "every contact point of the learned copy's 4 untimed steps to the 1e-3 target with the cached "
+                       "warm start (features, and the corrections that reach the solver's final accumulators)"}
+        log(f"network: {mlp.sample_count()} rows, loss {loss:.3f}, train {tr:.1f} s")
+        lw.set_learned(mlp, agx.LEARN_PREDICT)
+    # timed: the residual target, plain then learned, each from the same step index
+    st0 = plain_w.stats()
+    el = step_timed(plain_w, args.steps, dt)
+    st = plain_w.stats()
+    mean_plain = (st.total_velocity_sweeps - st0.total_velocity_sweeps) / args.steps
     out["plain"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
-                    "velocity_sweeps_last_step": sweeps_plain,
-                    "residual_last_sweep": round(float(st.residual[max(sweeps_plain - 1, 0)]), 6)}
-    seg = profiled(world, max(5, min(args.steps, 20)), dt)
-    st = world.stats()
+                    "mean_velocity_sweeps": round(mean_plain, 2), "velocity_sweeps_last_step": int(st.velocity_sweeps),
+                    "residual_last_sweep": round(float(st.residual[max(int(st.velocity_sweeps) - 1, 0)]), 6)}
+    if learned:
+        st0 = lw.stats()
+        el = step_timed(lw, args.steps, dt)
+        st = lw.stats()
+        g = lw.gate()
+        mean_learned = (st.total_velocity_sweeps - st0.total_velocity_sweeps) / args.steps
+        out["learned"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
+                          "mean_velocity_sweeps": round(mean_learned, 2),
+                          "sweep_reduction": round(1.0 - mean_learned / max(mean_plain, 1e-9), 4),
+                          "gate": {"cost_cached": round(g.cost_cached, 3), "cost_learned": round(g.cost_learned, 3),
+                                   "steps_cached": int(g.steps_cached), "steps_learned": int(g.steps_learned)},
+                          "network": net}
+    else:
+        out["learned"] = None
+    seg = profiled(plain_w, max(5, min(args.steps, 20)), dt)
+    st = plain_w.stats()
     out["plain"]["segments_ms_per_step"] = {k: round(v, 4) for k, v in seg.items()}
-    out["roofline"] = roofline(seg, st, world.counts, to_target, mean_poly_verts(scenes), "batch1024")
+    out["roofline"] = roofline(seg, st, plain_w.counts, to_target, mean_poly_verts(scenes), "batch1024")
     out["stats"] = {"manifolds": int(st.manifolds), "pairs": int(st.pairs), "colours": int(st.colours),
                     "islands": int(st.islands), "overflow": int(st.overflow)}
-    out["learned"] = None
+    if learned:
+        lseg = profiled(lw, max(5, min(args.steps, 20)), dt)
+        out["learned"]["segments_ms_per_step"] = {k: round(v, 4) for k, v in lseg.items()}
+        for w in worlds[1:]:
+            w.close()
     out["bodies"] = n
-    out["world"], out["scenes"] = world, scenes
+    out["world"], out["scenes"] = plain_w, scenes
     return out
 
 
@@ -311,6 +367,7 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all", "batch1024", "pile65536", "scatter1m"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="plain warm start only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device copy three times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -330,7 +387,7 @@ def main():
         fn = run_pile if args.workload == "pile65536" else run_scatter
         print(json.dumps(fn(agx, scenes_mod, args, dt, cpu)), flush=True)
         return
-    b = run_batch(agx, scenes_mod, args, dt)
+    b = run_batch(agx, scenes_mod, args, dt, learned=not args.no_learned)
     head = b["learned"] if b["learned"] else b["plain"]
     out = {
         "metric": METRIC, "value": head["value"], "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -346,7 +403,7 @@ def main():
     if b["learned"]:
         out["learned_over_plain"] = round(b["learned"]["value"] / b["plain"]["value"], 3)
     if cpu:
-        out["cpu_baseline"] = batch_cpu_baseline(b["world"], b["scenes"], max(b["plain"]["velocity_sweeps_last_step"], 1),
+        out["cpu_baseline"] = batch_cpu_baseline(b["world"], b["scenes"], max(int(round(b["plain"]["mean_velocity_sweeps"])), 1),
                                                  args.cpu_seconds)
         out["speedup_vs_cpu_baseline"] = round(head["value"] / out["cpu_baseline"]["value"], 2)
     del b

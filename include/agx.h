@@ -7,9 +7,11 @@
  * physicssimulation-agony_amd/host/GpuSimulation.h and calls only these functions.
  *
  * Conventions: every function returns 0 (AGX_OK) or a negative AGX_ERR_* code and never
- * throws. Callers own host buffers; a world owns its device buffers and one HIP stream.
- * One world must be externally synchronised; distinct worlds may be used from distinct
- * threads. agx_world_last_error() describes the last failure on a world,
+ * throws. Callers own host buffers; a world owns its device buffers. All worlds and networks
+ * on one device share one HIP stream, so their device work runs in issue order (the solver's
+ * dataflow kernels must not run beside another world's). One world must be externally
+ * synchronised; distinct worlds may be used from distinct threads, and waiting on one world
+ * also waits for the others' queued work. agx_world_last_error() describes the last failure on a world,
  * agx_last_error() the last failure of a call without a world.
  */
 #ifndef AGX_H
@@ -85,6 +87,8 @@ typedef struct agx_stats {
     float velocity_error;          /* computeConstraintErrors().x */
     float position_error;          /* computeConstraintErrors().y */
     float residual[64];            /* relative impulse change after velocity sweep k */
+    uint64_t solved_steps;         /* steps solved since the world was created */
+    uint64_t total_velocity_sweeps; /* velocity sweeps over those steps */
 } agx_stats;
 
 /* World sizes. */
@@ -256,16 +260,19 @@ int agx_mlp_train(agx_mlp* m, uint32_t iterations, uint32_t batch, float learnin
 int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode);
 
 /* Residual gate: exponential averages (weight 1/4 per step) of the cost of a step with the cached
- * (cost_cached) and the learned warm start (cost_learned). Cost: velocity sweeps to reach the
- * residual target, plus log2(residual / target) when the sweep cap ended the solve first; without
- * a target, log10 of the last sweep's residual. The cheaper method runs, the other once every 16
- * steps; a learned step that cost more than the cached average is followed by a cached step.
- * agx_gate_update is the same update on the host (no device work). */
+ * (cost_cached) and the learned warm start (cost_learned). Cost: log10 of the relative residual
+ * after the 8th velocity sweep (after the last one when fewer run). The cheaper method runs; the other runs once as a
+ * probe every `period` steps (16 at first, doubled after each losing probe up to 1024, reset by a
+ * winning one) and its cost replaces that method's average; a learned step that cost more than the
+ * cached average is followed by a cached step. agx_gate_update is the same update on the host (no
+ * device work). */
 typedef struct agx_gate {
     float cost_cached, cost_learned;
     uint32_t use_learned;          /* method of the next step */
     uint32_t since_probe;
     uint32_t steps_cached, steps_learned;
+    uint32_t period;               /* steps between probes (0 before the first update) */
+    uint32_t probing;              /* the next step is a probe */
 } agx_gate;
 int agx_gate_update(agx_gate* g, uint32_t used_learned, float cost);
 int agx_world_read_gate(agx_world* w, agx_gate* out);

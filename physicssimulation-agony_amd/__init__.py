@@ -61,13 +61,15 @@ class Stats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("manifolds", C.c_uint64), ("contacts", C.c_uint64),
                 ("colours", C.c_uint32), ("islands", C.c_uint32), ("ccd_hits", C.c_uint32),
                 ("velocity_sweeps", C.c_uint32), ("overflow", C.c_uint32), ("learned_warm_start", C.c_uint32),
-                ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * 64)]
+                ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * 64),
+                ("solved_steps", C.c_uint64), ("total_velocity_sweeps", C.c_uint64)]
 
 
 class Gate(C.Structure):
     """agx_gate: residual gate of the learned warm start."""
     _fields_ = [("cost_cached", C.c_float), ("cost_learned", C.c_float), ("use_learned", C.c_uint32),
-                ("since_probe", C.c_uint32), ("steps_cached", C.c_uint32), ("steps_learned", C.c_uint32)]
+                ("since_probe", C.c_uint32), ("steps_cached", C.c_uint32), ("steps_learned", C.c_uint32),
+                ("period", C.c_uint32), ("probing", C.c_uint32)]
 
 
 LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED = 0, 1, 2, 4

@@ -22,31 +22,44 @@ constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one s
 
 // Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).
 // Residual gate of the learned warm start: exponential averages of what a step cost with the
-// cached warm start (0) and with the network's prediction (1). The cheaper method is used; every
-// GATE_PROBE_PERIOD steps the other one runs once so that its average stays current. A step whose
-// prediction cost more than the cached average is followed by a cached step at once, and a network
-// whose average falls behind loses the gate until a probe shows it cheaper again.
+// cached warm start (0) and with the network's prediction (1). The cheaper method is used, and the
+// other one runs once as a probe every `period` steps, and its cost replaces that method's stale
+// average; a probe that loses doubles the period (up to GATE_PROBE_MAX), one that wins resets it.
+// A step whose prediction cost more than the cached average is followed by a cached step at once.
 constexpr uint32_t GATE_PROBE_PERIOD = 16;
+constexpr uint32_t GATE_PROBE_MAX = 1024;
 struct LearnGate {
     float cost[2];
     uint32_t use;            // method of the next step (1: learned)
     uint32_t sinceProbe;
     uint32_t steps[2];       // steps measured per method
+    uint32_t period;         // steps between probes (0: GATE_PROBE_PERIOD)
+    uint32_t probing;        // the next step is a probe
 };
 
 __host__ __device__ inline void gateUpdate(LearnGate& g, uint32_t used, float cost) {
     const uint32_t k = used ? 1u : 0u;
-    g.cost[k] = g.steps[k] == 0 ? cost : 0.75f * g.cost[k] + 0.25f * cost;
+    // a probe replaces its method's average: the old one is stale by a whole period
+    g.cost[k] = (g.steps[k] == 0 || g.probing) ? cost : 0.75f * g.cost[k] + 0.25f * cost;
     g.steps[k]++;
+    if (g.period == 0) g.period = GATE_PROBE_PERIOD;
     if (g.steps[1] == 0) { g.use = 1; return; }
     if (g.steps[0] == 0) { g.use = 0; return; }
     const uint32_t best = g.cost[1] < g.cost[0] ? 1u : 0u;
+    if (g.probing) {
+        g.probing = 0;
+        g.period = best == k ? GATE_PROBE_PERIOD : (g.period * 2 < GATE_PROBE_MAX ? g.period * 2 : GATE_PROBE_MAX);
+    }
     uint32_t next = best;
-    if (++g.sinceProbe >= GATE_PROBE_PERIOD) {
+    if (++g.sinceProbe >= g.period) {
         g.sinceProbe = 0;
         next = 1u - best;
+        g.probing = 1;
     }
-    if (k == 1 && cost > g.cost[0]) next = 0;   // this prediction did worse than the cached average
+    if (k == 1 && cost > g.cost[0]) {   // this prediction did worse than the cached average
+        next = 0;
+        g.probing = 0;
+    }
     g.use = next;
 }
 
@@ -55,6 +68,8 @@ struct Counters {
     uint32_t prevManifolds;    // manifolds of the previous step (warm-start source)
     uint32_t hashCapacity;     // power of two >= 2 * prevManifolds (>= 16)
     LearnGate gate;
+    unsigned long long solveSteps;   // steps solved since the world was created
+    unsigned long long sweepsTotal;  // velocity sweeps run over those steps
     // per step
     uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
     uint32_t manifolds;
@@ -372,5 +387,10 @@ constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
 void dataflowBlocksPerCU(int& colour, int& solve);
 // Message of the last failure outside a world (agx_last_error).
 void setLastError(const char* msg);
+// The one stream every world and network of the current device runs on. The dataflow kernels
+// (colouring, sweeps, radix scatter) wait on other blocks of their own launch, which needs those
+// blocks to get CUs; two such kernels of different worlds side by side could each hold what the
+// other waits for. One stream per device keeps all launches in issue order.
+hipError_t deviceStream(hipStream_t* out);
 
 }  // namespace agx

@@ -146,8 +146,9 @@ __global__ __launch_bounds__(256) void kLearnApply(const Counters* __restrict__ 
             const float S = normalMassOf(cs, slot, p) * gdt;
             const float y0 = y[(size_t)(2 * m + p) * MLP_OUT], y1 = y[(size_t)(2 * m + p) * MLP_OUT + 1];
             if (!(S > 0.0f) || !isfinite(y0) || !isfinite(y1)) continue;
-            jn[p] = fmaxf(0.0f, jn[p] + S * y0);
-            jt[p] = jt[p] + S * y1;
+            // corrections beyond the range of the training targets are not trusted
+            jn[p] = fmaxf(0.0f, jn[p] + S * fminf(fmaxf(y0, -FEATURE_CLAMP), FEATURE_CLAMP));
+            jt[p] = jt[p] + S * fminf(fmaxf(y1, -FEATURE_CLAMP), FEATURE_CLAMP);
         }
         cs.imp[slot] = make_float4(jn[0], jt[0], jn[1], jt[1]);
     }
@@ -202,21 +203,19 @@ __global__ __launch_bounds__(256) void kLearnCollect(Counters* __restrict__ ctr,
     }
 }
 
-// The step's cost for the gate: velocity sweeps run to reach the residual target (plus a penalty
-// of log2(residual / target) when the sweep cap stopped it first), or, without a target, the
-// log10 of the last sweep's residual.
-__global__ void kGateUpdate(Counters* ctr, float residualTarget) {
+// The step's cost for the gate: log10 of the relative residual after velocity sweep GATE_SWEEP (or
+// after the last sweep when fewer ran). A better warm start leaves less for the early sweeps to
+// change; the early residual is far less noisy than the number of sweeps a step needs, which a
+// collapsing stack somewhere in the batch can drive to the cap whatever the warm start.
+constexpr uint32_t GATE_SWEEP = 8;
+__global__ void kGateUpdate(Counters* ctr) {
     const uint32_t sweeps = min(ctr->sweeps, MAX_SWEEPS);
     if (sweeps == 0) return;
-    const double num = ctr->residualNum[sweeps - 1], den = ctr->residualDen[sweeps - 1];
+    const uint32_t s = min(sweeps, GATE_SWEEP) - 1;
+    const double num = ctr->residualNum[s], den = ctr->residualDen[s];
     const float res = (float)(sqrt(num) / fmax(sqrt(den), 1e-12));
-    float cost;
-    if (residualTarget > 0.0f)
-        cost = (float)sweeps + (res > residualTarget ? log2f(res / residualTarget) : 0.0f);
-    else
-        cost = log10f(fmaxf(res, 1e-12f));
     LearnGate g = ctr->gate;
-    gateUpdate(g, ctr->learnUsed, cost);
+    gateUpdate(g, ctr->learnUsed, log10f(fmaxf(res, 1e-12f)));
     ctr->gate = g;
 }
 
@@ -259,8 +258,8 @@ hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& 
     return hipGetLastError();
 }
 
-hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w, float residualTarget) {
-    hipLaunchKernelGGL(kGateUpdate, dim3(1), dim3(1), 0, st, w.ctr, residualTarget);
+hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w) {
+    hipLaunchKernelGGL(kGateUpdate, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }
 

@@ -718,7 +718,11 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
     __syncthreads();
     if (track)
         for (uint32_t q = threadIdx.x; q < sweepsDone; q += blockDim.x) { myRes[q] = res[q]; myRes[MAX_SWEEPS + q] = res[MAX_SWEEPS + q]; }
-    if (tid == 0) ctr->sweeps = sweepsDone;
+    if (tid == 0) {
+        ctr->sweeps = sweepsDone;
+        ctr->solveSteps += 1;
+        ctr->sweepsTotal += sweepsDone;
+    }
 }
 
 // Accumulators back to the manifolds (the warm-start source of the next step) and

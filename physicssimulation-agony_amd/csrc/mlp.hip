@@ -104,7 +104,7 @@ int agx_mlp_create(uint32_t seed, uint64_t sample_capacity, agx_mlp** out) {
     agx_mlp* m = new (std::nothrow) agx_mlp();
     if (!m) return mfail(AGX_ERR_CAPACITY, "out of host memory");
     auto bail = [&](int rc) { agx_mlp_destroy(m); return rc; };
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) return bail(mfail(AGX_ERR_HIP, "stream"));
+    if (agx::deviceStream(&m->stream) != hipSuccess) return bail(mfail(AGX_ERR_HIP, "stream"));
     const uint32_t P = agx::MLP_PARAMS_DEV, B = agx::MLP_BATCH_MAX;
 #define MA(p, n) do { if (mAlloc(m, &(p), (n)) != hipSuccess) return bail(mfail(AGX_ERR_HIP, "alloc " #p)); } while (0)
     MA(m->p.master, P); MA(m->p.mirror, P); MA(m->g.all, P); MA(m->adam.m, P); MA(m->adam.v, P); MA(m->adam.trainable, P);
@@ -148,7 +148,7 @@ int agx_mlp_create(uint32_t seed, uint64_t sample_capacity, agx_mlp** out) {
 
 int agx_mlp_destroy(agx_mlp* m) {
     if (!m) return AGX_OK;
-    if (m->stream) { (void)hipStreamSynchronize(m->stream); (void)hipStreamDestroy(m->stream); }
+    if (m->stream) (void)hipStreamSynchronize(m->stream);   // the device's shared stream stays
     for (void* p : m->allocations) (void)hipFree(p);
     delete m;
     return AGX_OK;

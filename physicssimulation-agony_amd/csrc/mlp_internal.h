@@ -106,6 +106,6 @@ struct DeviceWorld;
 hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice* mlp, uint32_t predict, uint32_t gated,
                                  uint32_t collect, float gx, float gy, float dt);
 hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, float gx, float gy, float dt);
-hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w, float residualTarget);
+hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w);
 
 }  // namespace agx

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -71,6 +72,22 @@ int fail(agx_world* w, int code, const char* fmt, ...) {
 
 namespace agx {
 void setLastError(const char* msg) { g_lastError = msg ? msg : ""; }
+
+hipError_t deviceStream(hipStream_t* out) {
+    static std::mutex mu;
+    static hipStream_t streams[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!streams[dev]) {
+        e = hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+    }
+    *out = streams[dev];
+    return hipSuccess;
+}
 }  // namespace agx
 
 namespace {
@@ -262,7 +279,7 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
             }
             SegmentTimer t(w, SEG_SOLVE_FINISH);
             HIP_CHECK(w, agx::launchSolveFinish(st, d, trackResidual, s.residual_target, s.track_errors ? 1 : 0));
-            if (gated) HIP_CHECK(w, agx::launchGateUpdate(st, d, s.residual_target));
+            if (gated) HIP_CHECK(w, agx::launchGateUpdate(st, d));
         }
         break;
     case AGX_PHASE_ISLANDS: {
@@ -515,8 +532,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         d.colourBlocks = d.numCU * perCuColour;
         d.solveBlocks = d.numCU * perCuSolve;
     }
-    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(nullptr, AGX_ERR_HIP, "hipStreamCreate failed"));
+    if (agx::deviceStream(&w->stream) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "hipStreamCreate failed"));
 
 #define UP(p, v) do { if (upload(w, &(p), (v)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "upload " #p)); } while (0)
 #define AL(p, n) do { if (devAlloc(w, &(p), (n)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "alloc " #p)); } while (0)
@@ -568,7 +584,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
 
 int agx_world_destroy(agx_world* w) {
     if (!w) return AGX_OK;
-    if (w->stream) { (void)hipStreamSynchronize(w->stream); (void)hipStreamDestroy(w->stream); }
+    if (w->stream) (void)hipStreamSynchronize(w->stream);   // the device's shared stream stays
     for (const EventPair& p : w->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : w->eventPool) (void)hipEventDestroy(e);
     for (void* p : w->allocations) (void)hipFree(p);
@@ -638,9 +654,10 @@ int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode) {
             devAlloc(w, &d.histOf, man) != hipSuccess)
             return fail(w, AGX_ERR_HIP, "cannot allocate the learned warm-start buffers (%zu rows)", rows);
     }
+    // the accumulator history stays valid across mode changes: every step solved with a network
+    // set records it, and a step without one marks it missing
     w->mlp = (mode & (AGX_LEARN_PREDICT | AGX_LEARN_COLLECT)) ? mlp : nullptr;
     w->learnMode = w->mlp ? mode : 0u;
-    w->histReady = false;
     return agx_world_reset_gate(w);
 }
 
@@ -661,6 +678,7 @@ int agx_world_read_gate(agx_world* w, agx_gate* out) {
     out->cost_cached = g.cost[0]; out->cost_learned = g.cost[1];
     out->use_learned = g.use; out->since_probe = g.sinceProbe;
     out->steps_cached = g.steps[0]; out->steps_learned = g.steps[1];
+    out->period = g.period; out->probing = g.probing;
     return AGX_OK;
 }
 
@@ -670,10 +688,12 @@ int agx_gate_update(agx_gate* out, uint32_t used_learned, float cost) {
     g.cost[0] = out->cost_cached; g.cost[1] = out->cost_learned;
     g.use = out->use_learned; g.sinceProbe = out->since_probe;
     g.steps[0] = out->steps_cached; g.steps[1] = out->steps_learned;
+    g.period = out->period; g.probing = out->probing;
     agx::gateUpdate(g, used_learned, cost);
     out->cost_cached = g.cost[0]; out->cost_learned = g.cost[1];
     out->use_learned = g.use; out->since_probe = g.sinceProbe;
     out->steps_cached = g.steps[0]; out->steps_learned = g.steps[1];
+    out->period = g.period; out->probing = g.probing;
     return AGX_OK;
 }
 
@@ -908,6 +928,8 @@ int agx_world_read_stats(agx_world* w, agx_stats* out) {
     out->velocity_sweeps = c->sweeps;
     out->overflow = c->overflow;
     out->learned_warm_start = c->learnUsed;
+    out->solved_steps = c->solveSteps;
+    out->total_velocity_sweeps = c->sweepsTotal;
     out->velocity_error = c->velocityError;
     out->position_error = c->positionError;
     for (uint32_t k = 0; k < 64 && k < agx::MAX_SWEEPS; k++)
@@ -974,7 +996,7 @@ int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
         return bail(fail(w, AGX_ERR_HIP, "hipGetDeviceProperties failed"));
     d.numCU = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(w, AGX_ERR_HIP, "stream"));
+    if (agx::deviceStream(&w->stream) != hipSuccess) return bail(fail(w, AGX_ERR_HIP, "stream"));
     if (upload(w, &d.aabb, boxes) != hipSuccess || upload(w, &d.colBody, body) != hipSuccess ||
         upload(w, &d.colScene, sc) != hipSuccess)
         return bail(fail(w, AGX_ERR_HIP, "upload failed"));

@@ -110,18 +110,40 @@ def run_gate(agx, costs, steps=64):
     return g, used
 
 
+def probe_steps(used, method):
+    return [k for k, u in enumerate(used) if k >= 2 and u == method]
+
+
 def test_gate_prefers_learned_when_cheaper(agx):
-    g, used = run_gate(agx, {0: 20.0, 1: 12.0})
+    g, used = run_gate(agx, {0: 20.0, 1: 12.0}, steps=200)
     assert used[:2] == [1, 0]             # first step tries the network, then the cached warm start once
     assert g.cost_learned == pytest.approx(12.0) and g.cost_cached == pytest.approx(20.0)
-    # afterwards the cached warm start only runs as a probe, once every 16 steps
-    assert sum(1 for u in used[2:] if u == 0) == (len(used) - 2) // 16
+    # afterwards the cached warm start only runs as a probe; each losing probe doubles the period
+    probes = probe_steps(used, 0)
+    gaps = np.diff([1] + probes)
+    assert list(gaps) == [16, 32, 64], probes
 
 
 def test_gate_falls_back_when_prediction_raises_residual(agx):
-    g, used = run_gate(agx, {0: 12.0, 1: 20.0})
+    g, used = run_gate(agx, {0: 12.0, 1: 20.0}, steps=200)
     assert g.use_learned == 0
-    assert sum(used[2:]) == (len(used) - 2) // 16   # the network only runs as a probe
+    probes = probe_steps(used, 1)          # the network only runs as a probe, ever more rarely
+    assert list(np.diff([1] + probes)) == [16, 32, 64], probes
+    assert g.period == 128
+
+
+def test_gate_winning_probe_resets_period(agx):
+    g = agx.Gate()
+    g.use_learned = 1
+    cost = {0: 12.0, 1: 20.0}
+    used = []
+    for k in range(120):
+        u = int(g.use_learned)
+        used.append(u)
+        agx.gate_update(g, u, cost[u])
+        if k == 60:
+            cost[1] = 4.0                   # the network becomes much better
+    assert g.use_learned == 1 and g.period == 16
 
 
 def test_gate_switches_when_learned_degrades(agx):
