@@ -291,6 +291,10 @@ class World:
     def step(self, dt: float, n_steps: int = 1):
         _check(lib().agx_world_step(self._h, dt, n_steps), self._h)
 
+    def step_graph(self, dt: float, n_steps: int = 1):
+        """As step(), with every step after the first replayed from a captured HIP graph."""
+        _check(lib().agx_world_step_graph(self._h, dt, n_steps), self._h)
+
     def run_phase(self, phase: int, dt: float):
         _check(lib().agx_world_run_phase(self._h, phase, dt), self._h)
 

@@ -21,6 +21,17 @@ __device__ inline float4_t mfma16(short4_t a, short4_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// 16x16x32: lane (i, q) holds 8 k-slots of A row i and of B column i. Which k a slot stands for
+// depends only on (q, slot), so any assignment shared by A and B gives the same product; here slots
+// 0..3 are k = 4q .. 4q+3 of the first 16-block and slots 4..7 the same k of the second, which is
+// how two 16x16 output tiles of the previous layer already sit in the registers.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+__device__ inline float4_t mfma32(short4_t aLo, short4_t aHi, short4_t bLo, short4_t bHi, float4_t c) {
+    const bf16x8_t a = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(aLo, aHi, 0, 1, 2, 3, 4, 5, 6, 7));
+    const bf16x8_t b = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(bLo, bHi, 0, 1, 2, 3, 4, 5, 6, 7));
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 __device__ inline short4_t frag4(const uint16_t* p) {   // 4 consecutive bf16
     const uint2 v = *reinterpret_cast<const uint2*>(p);
     short4_t s;
@@ -93,27 +104,34 @@ __global__ __launch_bounds__(256) void kMlpForward(const uint16_t* __restrict__ 
         const bool valid = row < n;
         short4_t xb[2] = { xn[0], xn[1] };
         loadX(t + stride, xn);
-        // layer 1: h1^T [128 x 16] = W1 [128 x 32] x^T
+        // layer 1: h1^T [128 x 16] = W1 [128 x 32] x^T, one 16x16x32 product per 16 units
         short4_t h1[MLP_H / 16];
         for (uint32_t ob = 0; ob < MLP_H / 16; ob++) {
+            const uint16_t* wr = w1 + (ob * 16 + i) * W1_STRIDE + 4 * q;
             float4_t c = { 0.f, 0.f, 0.f, 0.f };
-            for (uint32_t kb = 0; kb < 2; kb++) c = mfma16(frag4(w1 + (ob * 16 + i) * W1_STRIDE + kb * 16 + 4 * q), xb[kb], c);
+            c = mfma32(frag4(wr), frag4(wr + 16), xb[0], xb[1], c);
             const float* b = bias + ob * 16 + 4 * q;
             h1[ob] = pack4(fmaxf(c[0] + b[0], 0.0f), fmaxf(c[1] + b[1], 0.0f), fmaxf(c[2] + b[2], 0.0f),
                            fmaxf(c[3] + b[3], 0.0f));
         }
-        // layer 2: h2^T = W2 h1^T
+        // layer 2: h2^T = W2 h1^T; k-block kb spans the unit blocks 2 kb and 2 kb + 1
         short4_t h2[MLP_H / 16];
         for (uint32_t ob = 0; ob < MLP_H / 16; ob++) {
+            const uint16_t* wr = w2 + (ob * 16 + i) * WH_STRIDE + 4 * q;
             float4_t c = { 0.f, 0.f, 0.f, 0.f };
-            for (uint32_t kb = 0; kb < MLP_H / 16; kb++) c = mfma16(frag4(w2 + (ob * 16 + i) * WH_STRIDE + kb * 16 + 4 * q), h1[kb], c);
+            for (uint32_t kb = 0; kb < MLP_H / 32; kb++)
+                c = mfma32(frag4(wr + 32 * kb), frag4(wr + 32 * kb + 16), h1[2 * kb], h1[2 * kb + 1], c);
             const float* b = bias + MLP_H + ob * 16 + 4 * q;
             h2[ob] = pack4(fmaxf(c[0] + b[0], 0.0f), fmaxf(c[1] + b[1], 0.0f), fmaxf(c[2] + b[2], 0.0f),
                            fmaxf(c[3] + b[3], 0.0f));
         }
         // layer 3: y^T [16 x 16] = W3 (rows 2..15 zero) h2^T; lane (i, q = 0) holds outputs 0, 1 of sample i
         float4_t c = { 0.f, 0.f, 0.f, 0.f };
-        for (uint32_t kb = 0; kb < MLP_H / 16; kb++) c = mfma16(frag4(w3 + i * WH_STRIDE + kb * 16 + 4 * q), h2[kb], c);
+        {
+            const uint16_t* wr = w3 + i * WH_STRIDE + 4 * q;
+            for (uint32_t kb = 0; kb < MLP_H / 32; kb++)
+                c = mfma32(frag4(wr + 32 * kb), frag4(wr + 32 * kb + 16), h2[2 * kb], h2[2 * kb + 1], c);
+        }
         if (q == 0 && valid)
             *reinterpret_cast<float2*>(y + (size_t)row * MLP_OUT) =
                 make_float2(c[0] + bias[2 * MLP_H], c[1] + bias[2 * MLP_H + 1]);

@@ -51,6 +51,13 @@ struct agx_world {
     agx_mlp* mlp = nullptr;          // learned warm start (not owned)
     uint32_t learnMode = 0;          // AGX_LEARN_* flags
     bool histReady = false;          // the previous step recorded its manifolds' starting accumulators
+    // step graphs (agx_world_step_graph): one per ping-pong parity of the manifold buffers
+    hipStream_t captureStream = nullptr;
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<unsigned char> key;   // what the captured launches depend on
+        bool histReadyAfter = false;
+    } graphs[2];
 };
 
 namespace {
@@ -306,6 +313,62 @@ int postUpdate(agx_world* w) {
     SegmentTimer t(w, SEG_POST_UPDATE);
     HIP_CHECK(w, agx::launchPose(w->stream, w->d, 0, 0, 0));
     w->postUpdatePending = true;
+    return AGX_OK;
+}
+
+// Everything a captured step's launches depend on besides device memory: the step size, the
+// settings, the learned mode and network, whether a history was recorded, and the parity.
+std::vector<unsigned char> graphKey(const agx_world* w, float dt) {
+    std::vector<unsigned char> k;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), c, c + n);
+    };
+    put(&dt, sizeof dt);
+    put(&w->settings, sizeof w->settings);
+    put(&w->mlp, sizeof w->mlp);
+    put(&w->learnMode, sizeof w->learnMode);
+    const unsigned char flags[3] = { (unsigned char)w->histReady, (unsigned char)w->postUpdatePending,
+                                     (unsigned char)w->d.cur };
+    put(flags, sizeof flags);
+    const void* bufs[3] = { w->d.learnX, w->d.warmHist[0], w->d.histOf };
+    put(bufs, sizeof bufs);
+    return k;
+}
+
+void dropGraph(agx_world::StepGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.key.clear();
+}
+
+// Records one full step (all phases) into a graph on the world's private capture stream, so that
+// launches other worlds queue on the shared stream meanwhile are not captured. The host-side
+// state advances as if the step had run.
+int captureStep(agx_world* w, float dt, agx_world::StepGraph& g) {
+    dropGraph(g);
+    std::vector<unsigned char> key = graphKey(w, dt);
+    if (!w->captureStream)
+        HIP_CHECK(w, hipStreamCreateWithFlags(&w->captureStream, hipStreamNonBlocking));
+    hipStream_t shared = w->stream;
+    w->stream = w->captureStream;
+    hipError_t e = hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal);
+    int rc = AGX_OK;
+    if (e == hipSuccess)
+        for (uint32_t p = 0; p < AGX_PHASE_COUNT && rc == AGX_OK; p++) rc = runPhase(w, p, dt);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(w->stream, &graph) : e;
+    w->stream = shared;
+    if (rc != AGX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess || e2 != hipSuccess) return fail(w, AGX_ERR_HIP, "step capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { g.exec = nullptr; return fail(w, AGX_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+    g.key = std::move(key);
+    g.histReadyAfter = w->histReady;
     return AGX_OK;
 }
 
@@ -585,6 +648,8 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
 int agx_world_destroy(agx_world* w) {
     if (!w) return AGX_OK;
     if (w->stream) (void)hipStreamSynchronize(w->stream);   // the device's shared stream stays
+    for (agx_world::StepGraph& g : w->graphs) dropGraph(g);
+    if (w->captureStream) (void)hipStreamDestroy(w->captureStream);
     for (const EventPair& p : w->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : w->eventPool) (void)hipEventDestroy(e);
     for (void* p : w->allocations) (void)hipFree(p);
@@ -697,8 +762,33 @@ int agx_gate_update(agx_gate* out, uint32_t used_learned, float cost) {
     return AGX_OK;
 }
 
+// Steps as agx_world_step does, with every step after the first replayed from a captured graph
+// (one per parity of the ping-pong manifold buffers, re-captured when what it depends on changed).
+// The first step runs as launches because it starts from the post-update pose. Profiling needs the
+// per-segment events of plain launches, so a profiled world steps without graphs.
 int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps) {
-    return agx_world_step(w, dt, n_steps);
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (!(dt > 0.0f) || n_steps == 0) return AGX_OK;
+    if (w->profiling) return agx_world_step(w, dt, n_steps);
+    for (uint32_t p = 0; p < AGX_PHASE_COUNT; p++)
+        if (int r = runPhase(w, p, dt)) return r;
+    for (uint32_t i = 1; i < n_steps; i++) {
+        agx_world::StepGraph& g = w->graphs[w->d.cur];
+        if (!g.exec || g.key != graphKey(w, dt)) {
+            if (int r = captureStep(w, dt, g)) return r;
+            // captured, not run: the host state already moved one step ahead; run the graph now
+            HIP_CHECK(w, hipGraphLaunch(g.exec, w->stream));
+            continue;
+        }
+        HIP_CHECK(w, hipGraphLaunch(g.exec, w->stream));
+        // the host-side effects of one step (runPhase INTEGRATE..FINISH)
+        w->d.cur ^= 1;
+        w->postUpdatePending = false;
+        w->histReady = g.histReadyAfter;
+        w->nextPhase = 0;
+    }
+    return postUpdate(w);
 }
 
 // Simulation::update + FixedClock::advance (SimulationImpl/FixedClock.h:35-63).

@@ -131,16 +131,24 @@ def test_collect_rows_are_the_steps_contact_points(agx, scenes):
     assert m.sample_count() == n
 
 
+def _batch_world(agx, scenes, n_scenes, **settings):
+    batch = scenes.batch(n_scenes)
+    b = agx.Builder(len(batch))
+    for i, sc in enumerate(batch):
+        agx.build_scene(b, sc, i)
+    return agx.World(b, agx.default_settings(**settings))
+
+
 def test_gate_rejects_a_harmful_network(agx, scenes):
-    """A network that cancels every cached normal impulse (a cold start) raises the residual: the
-    gate hands the warm start back to the cached impulses and only probes the network afterwards."""
-    # judged by the residual left after 8 sweeps (this pile does not reach 1e-3 within 64 sweeps)
-    w = _world(agx, scenes, velocity_iterations=8, track_residual=1)
+    """A network that cancels every cached normal impulse (a cold start) leaves a larger residual
+    after the early sweeps: the gate hands the warm start back to the cached impulses and probes the
+    network ever more rarely."""
+    w = _batch_world(agx, scenes, 64, velocity_iterations=16, position_iterations=3, track_residual=1)
     dt = 1.0 / 60.0
-    w.step(dt, 40)
+    w.step(dt, 240)
     m = agx.Mlp(seed=1, sample_capacity=16)
     p = np.zeros(agx.MLP_PARAMS, np.float32)
-    p[-2] = -1e4     # b3[0]: jn = max(0, jn_cached - 1e4 S) = 0
+    p[-2] = -1e4     # b3[0]: jn = max(0, jn_cached - 64 S) = 0 for every point in range
     m.set_params(p)
     w.set_learned(m, agx.LEARN_PREDICT)
     used = []
@@ -151,5 +159,35 @@ def test_gate_rejects_a_harmful_network(agx, scenes):
     info = (used, g.cost_learned, g.cost_cached)
     assert g.cost_learned > g.cost_cached, info
     assert g.use_learned == 0, info
-    assert sum(used[8:]) <= 4, used
+    assert used[:2] == [1, 0] and sum(used[2:]) <= 2, info     # probes after 16 and 32 more steps
     assert g.steps_cached + g.steps_learned == 48
+
+
+def test_trained_network_needs_fewer_sweeps_than_cached_warm_start(agx, scenes):
+    """What the trained network gains: two copies of a 128-scene batch in lockstep; the network is
+    trained on the GPU on 4 steps of one copy's own solves (cached warm start), then both step to
+    the 1e-3 residual target, one warm-started from the network (every step, no gate)."""
+    dt = 1.0 / 60.0
+    base = dict(position_iterations=3, track_residual=1)
+    worlds = [_batch_world(agx, scenes, 128, velocity_iterations=8, **base) for _ in range(2)]
+    for w in worlds:
+        w.step(dt, 240)
+        w.set_settings(agx.default_settings(velocity_iterations=64, residual_target=1e-3, **base))
+    plain, learned = worlds
+    m = agx.Mlp(seed=1, sample_capacity=4 << 20)
+    learned.set_learned(m, agx.LEARN_COLLECT)
+    for _ in range(4):
+        plain.step(dt, 1)
+        learned.step(dt, 1)
+    m.train(3000, 4096, 1e-3, seed=1)
+    m.train(1000, 4096, 2e-4, seed=2)
+    learned.set_learned(m, agx.LEARN_PREDICT | agx.LEARN_UNGATED)
+    s0 = [w.stats().total_velocity_sweeps for w in worlds]
+    for _ in range(16):
+        plain.step(dt, 1)
+        learned.step(dt, 1)
+    s1 = [w.stats().total_velocity_sweeps for w in worlds]
+    sweeps_plain, sweeps_learned = (s1[0] - s0[0]) / 16, (s1[1] - s0[1]) / 16
+    print(f"mean velocity sweeps to 1e-3: cached {sweeps_plain:.2f}, learned {sweeps_learned:.2f}")
+    assert learned.stats().learned_warm_start == 1
+    assert sweeps_learned < sweeps_plain
