@@ -223,6 +223,16 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
         w.step(dt, args.steps)
     for w in worlds:
         w.set_settings(to_target)
+    # launch shape of the dataflow kernels, measured on the plain copy and applied to both
+    n0 = plain_w.stats().solved_steps
+    tune = plain_w.autotune(dt, 2)
+    tuned_steps = plain_w.stats().solved_steps - n0
+    for w in worlds[1:]:
+        w.set_tuning(tune.solve_blocks_per_cu, tune.colour_blocks_per_cu)
+        w.step(dt, tuned_steps)
+    out["tuning"] = {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu,
+                     "from_cache": int(tune.from_cache), "steps": int(tuned_steps)}
+    log(f"batch: tuned {tune.solve_blocks_per_cu}/{tune.colour_blocks_per_cu} blocks per CU over {tuned_steps} steps")
     if learned:
         lw = worlds[1]
         mlp = agx.Mlp(seed=1, sample_capacity=16 << 20)
@@ -302,6 +312,7 @@ def run_pile(agx, scenes_mod, args, dt, cpu):
     for k in range(0, 240, 40):
         world.step(dt, 40)
     world.sync()
+    tune = world.autotune(dt, 4)
     steps = 100
     el = step_timed(world, steps, dt)
     st = world.stats()
@@ -310,6 +321,7 @@ def run_pile(agx, scenes_mod, args, dt, cpu):
     out = {"workload": f"pile_large_{n}_bodies", "value": round(n * steps / el, 1), "unit": "body-steps/s",
            "ms_per_step": round(1e3 * el / steps, 4), "velocity_iterations": 8, "position_iterations": 3,
            "segments_ms_per_step": {k: round(v, 4) for k, v in seg.items()},
+           "tuning": {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu},
            "roofline": roofline(seg, st, world.counts, settings, mean_poly_verts([scene]), "pile65536"),
            "stats": {"pairs": int(st.pairs), "manifolds": int(st.manifolds), "colours": int(st.colours),
                      "islands": int(st.islands), "overflow": int(st.overflow),
@@ -401,6 +413,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": b["roofline"],
         "plain": b["plain"], "learned": b["learned"], "plain_fixed8": b["plain_fixed8"], "stats": b["stats"],
+        "tuning": b["tuning"],
     }
     if b["learned"]:
         out["learned_over_plain"] = round(b["learned"]["value"] / b["plain"]["value"], 3)

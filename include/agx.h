@@ -206,6 +206,25 @@ int agx_world_set_profiling(agx_world* w, uint32_t enabled);
 int agx_world_read_profile(agx_world* w, double* ms_total, uint32_t* launches, uint32_t cap, uint32_t* segments);
 const char* agx_profile_segment_name(uint32_t segment);
 
+/* Launch-shape tuning. The reference partitions work by fixed sizes (COLLISION_COUNT_PER_WORKER,
+ * Solving/BodyCollisionSolver.cpp:211-212; DETERMINISTIC_CHUNK_SIZE,
+ * CollisionDetection/Narrow/NarrowPhaseCollisionDetector.h:153). Here the co-resident blocks per CU
+ * of the dataflow colouring and sweep kernels are measured instead: agx_world_autotune times
+ * graph-replayed steps for each shape (1, 2, 4 ... up to the maximum; two interleaved rounds, which
+ * advances the simulation) and keeps the fastest. Results are cached in the process per GPU, world
+ * size class and solver settings, and in the file named by the environment variable AGX_TUNE_CACHE
+ * when it is set; a cached shape is applied without stepping (from_cache = 1). */
+typedef struct agx_tuning {
+    uint32_t solve_blocks_per_cu;       /* sweep kernel */
+    uint32_t colour_blocks_per_cu;      /* colouring kernel */
+    float ms_per_step;                  /* mean step time measured with this shape (autotune only) */
+    uint32_t from_cache;
+    uint32_t max_solve_blocks_per_cu, max_colour_blocks_per_cu;   /* limits (get_tuning, autotune) */
+} agx_tuning;
+int agx_world_get_tuning(agx_world* w, agx_tuning* out);
+int agx_world_set_tuning(agx_world* w, const agx_tuning* in);
+int agx_world_autotune(agx_world* w, float dt, uint32_t steps_per_trial, agx_tuning* out);
+
 /* Standalone broad phase over injected AABBs (BroadPhaseCollisionDetector::findCollisions,
  * Broad/BroadPhaseCollisionDetector.cpp:109-244): aabb holds n x (min_x, min_y, max_x, max_y),
  * owner[i] the owning body, is_dynamic[i] 1 for dynamic colliders, scene[i] (may be NULL) the

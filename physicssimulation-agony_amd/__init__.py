@@ -72,6 +72,13 @@ class Gate(C.Structure):
                 ("period", C.c_uint32), ("probing", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    """agx_tuning: co-resident blocks per CU of the dataflow colouring and sweep kernels."""
+    _fields_ = [("solve_blocks_per_cu", C.c_uint32), ("colour_blocks_per_cu", C.c_uint32),
+                ("ms_per_step", C.c_float), ("from_cache", C.c_uint32),
+                ("max_solve_blocks_per_cu", C.c_uint32), ("max_colour_blocks_per_cu", C.c_uint32)]
+
+
 LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED = 0, 1, 2, 4
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS, MLP_BATCH_MAX = 24, 128, 2, 19970, 8192
 
@@ -138,6 +145,9 @@ _SIGS = {
     "agx_gate_update": (C.c_int, [C.POINTER(Gate), C.c_uint32, C.c_float]),
     "agx_world_read_gate": (C.c_int, [_P, C.POINTER(Gate)]),
     "agx_world_reset_gate": (C.c_int, [_P]),
+    "agx_world_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "agx_world_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "agx_world_autotune": (C.c_int, [_P, C.c_float, C.c_uint32, C.POINTER(Tuning)]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -389,6 +399,22 @@ class World:
 
     def reset_gate(self):
         _check(lib().agx_world_reset_gate(self._h), self._h)
+
+    # -- launch shapes
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().agx_world_get_tuning(self._h, C.byref(t)), self._h)
+        return t
+
+    def set_tuning(self, solve_blocks_per_cu: int, colour_blocks_per_cu: int):
+        t = Tuning(solve_blocks_per_cu=solve_blocks_per_cu, colour_blocks_per_cu=colour_blocks_per_cu)
+        _check(lib().agx_world_set_tuning(self._h, C.byref(t)), self._h)
+
+    def autotune(self, dt: float, steps_per_trial: int = 2) -> Tuning:
+        """Times graph-replayed steps per launch shape and keeps the fastest (advances the world)."""
+        t = Tuning()
+        _check(lib().agx_world_autotune(self._h, dt, steps_per_trial, C.byref(t)), self._h)
+        return t
 
 
 class Mlp:

@@ -383,8 +383,10 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
 hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors);
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
-// Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels.
+// Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels: the default
+// (at most 4) and what the occupancy calculator allows.
 void dataflowBlocksPerCU(int& colour, int& solve);
+void dataflowOccupancy(int& colour, int& solve);
 // Message of the last failure outside a world (agx_last_error).
 void setLastError(const char* msg);
 // The one stream every world and network of the current device runs on. The dataflow kernels

@@ -888,6 +888,14 @@ __global__ void kIslandCount(Counters* ctr) { ctr->islands = ctr->scratch[3]; }
 
 }  // namespace
 
+void dataflowOccupancy(int& colour, int& solve) {
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void*>(kColourFlow), 256, 0) != hipSuccess) a = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(kSolveFlow), 256, 0) != hipSuccess) b = 1;
+    colour = std::max(1, a);
+    solve = std::max(1, b);
+}
+
 void dataflowBlocksPerCU(int& colour, int& solve) {
     int a = 0, b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void*>(kColourFlow), 256, 0) != hipSuccess) a = 1;

@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
@@ -51,6 +53,7 @@ struct agx_world {
     agx_mlp* mlp = nullptr;          // learned warm start (not owned)
     uint32_t learnMode = 0;          // AGX_LEARN_* flags
     bool histReady = false;          // the previous step recorded its manifolds' starting accumulators
+    int colourPerCuMax = 1, solvePerCuMax = 1;   // launch-shape limits (occupancy, allocations)
     // step graphs (agx_world_step_graph): one per ping-pong parity of the manifold buffers
     hipStream_t captureStream = nullptr;
     struct StepGraph {
@@ -333,6 +336,8 @@ std::vector<unsigned char> graphKey(const agx_world* w, float dt) {
     put(flags, sizeof flags);
     const void* bufs[3] = { w->d.learnX, w->d.warmHist[0], w->d.histOf };
     put(bufs, sizeof bufs);
+    const int shape[2] = { w->d.colourBlocks, w->d.solveBlocks };
+    put(shape, sizeof shape);
     return k;
 }
 
@@ -590,10 +595,13 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         return bail(fail(nullptr, AGX_ERR_HIP, "hipGetDeviceProperties failed"));
     d.numCU = prop.multiProcessorCount;
     {
-        int perCuColour = 1, perCuSolve = 1;
+        int perCuColour = 1, perCuSolve = 1, occColour = 1, occSolve = 1;
         agx::dataflowBlocksPerCU(perCuColour, perCuSolve);
+        agx::dataflowOccupancy(occColour, occSolve);
         d.colourBlocks = d.numCU * perCuColour;
         d.solveBlocks = d.numCU * perCuSolve;
+        w->colourPerCuMax = occColour;
+        w->solvePerCuMax = perCuSolve;   // the sweep kernel's per-block buffers are sized for this many
     }
     if (agx::deviceStream(&w->stream) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "hipStreamCreate failed"));
 
@@ -759,6 +767,146 @@ int agx_gate_update(agx_gate* out, uint32_t used_learned, float cost) {
     out->use_learned = g.use; out->since_probe = g.sinceProbe;
     out->steps_cached = g.steps[0]; out->steps_learned = g.steps[1];
     out->period = g.period; out->probing = g.probing;
+    return AGX_OK;
+}
+
+}  // extern "C"
+
+// ---- launch-shape tuning
+namespace {
+
+std::mutex g_tuneMutex;
+std::map<std::string, agx_tuning> g_tuneCache;
+bool g_tuneFileLoaded = false;
+
+// The GPU, the world's size class (bodies to the power of two below, scenes) and the solver
+// settings a tuning was measured for.
+std::string tuneKey(agx_world* w) {
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipGetDeviceProperties(&prop, dev);
+    auto bucket = [](uint64_t v) { uint32_t b = 0; while (v > 1) { v >>= 1; b++; } return b; };
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s|%s|cu%d|bodies2^%u|scenes%u|iv%u|ip%u|target%d|learned%u", prop.name,
+             prop.gcnArchName, prop.multiProcessorCount, bucket(w->d.N), w->d.S, w->settings.velocity_iterations, w->settings.position_iterations, w->settings.residual_target > 0.0f ? 1 : 0,
+             w->learnMode);
+    for (char* c = buf; *c; c++) if (*c == ' ' || *c == '\n') *c = '_';
+    return buf;
+}
+
+// AGX_TUNE_CACHE names a text file of "key solve colour ms" lines shared by processes.
+void loadTuneFile() {
+    if (g_tuneFileLoaded) return;
+    g_tuneFileLoaded = true;
+    const char* path = std::getenv("AGX_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char key[512];
+    agx_tuning t{};
+    while (std::fscanf(f, "%511s %u %u %f", key, &t.solve_blocks_per_cu, &t.colour_blocks_per_cu, &t.ms_per_step) == 4)
+        g_tuneCache[key] = t;
+    std::fclose(f);
+}
+
+void storeTune(const std::string& key, const agx_tuning& t) {
+    g_tuneCache[key] = t;
+    const char* path = std::getenv("AGX_TUNE_CACHE");
+    if (!path || !*path) return;
+    if (FILE* f = std::fopen(path, "a")) {
+        std::fprintf(f, "%s %u %u %.6f\n", key.c_str(), t.solve_blocks_per_cu, t.colour_blocks_per_cu, t.ms_per_step);
+        std::fclose(f);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int agx_world_get_tuning(agx_world* w, agx_tuning* out) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(out, 0, sizeof *out);
+    out->solve_blocks_per_cu = (uint32_t)(w->d.solveBlocks / w->d.numCU);
+    out->colour_blocks_per_cu = (uint32_t)(w->d.colourBlocks / w->d.numCU);
+    out->max_solve_blocks_per_cu = (uint32_t)w->solvePerCuMax;
+    out->max_colour_blocks_per_cu = (uint32_t)w->colourPerCuMax;
+    return AGX_OK;
+}
+
+int agx_world_set_tuning(agx_world* w, const agx_tuning* in) {
+    if (!w || !in) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (in->solve_blocks_per_cu < 1 || (int)in->solve_blocks_per_cu > w->solvePerCuMax || in->colour_blocks_per_cu < 1 ||
+        (int)in->colour_blocks_per_cu > w->colourPerCuMax)
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "blocks per CU must be 1..%d (sweeps) and 1..%d (colouring)",
+                    w->solvePerCuMax, w->colourPerCuMax);
+    w->d.solveBlocks = w->d.numCU * (int)in->solve_blocks_per_cu;
+    w->d.colourBlocks = w->d.numCU * (int)in->colour_blocks_per_cu;
+    return AGX_OK;
+}
+
+// Times `steps_per_trial` graph-replayed steps for every launch shape in two interleaved rounds
+// (so that a scene that settles meanwhile favours no candidate) and keeps the fastest. A shape
+// measured before for the same GPU, size class and solver settings is reused without stepping.
+int agx_world_autotune(agx_world* w, float dt, uint32_t steps_per_trial, agx_tuning* out) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (!(dt > 0.0f) || steps_per_trial == 0) return fail(w, AGX_ERR_INVALID_ARGUMENT, "need dt > 0 and steps");
+    const std::string key = tuneKey(w);
+    {
+        std::lock_guard<std::mutex> lock(g_tuneMutex);
+        loadTuneFile();
+        auto it = g_tuneCache.find(key);
+        if (it != g_tuneCache.end()) {
+            agx_tuning t = it->second;
+            if ((int)t.solve_blocks_per_cu <= w->solvePerCuMax && (int)t.colour_blocks_per_cu <= w->colourPerCuMax) {
+                if (int r = agx_world_set_tuning(w, &t)) return r;
+                if (out) { *out = t; out->from_cache = 1; out->max_solve_blocks_per_cu = (uint32_t)w->solvePerCuMax;
+                           out->max_colour_blocks_per_cu = (uint32_t)w->colourPerCuMax; }
+                return AGX_OK;
+            }
+        }
+    }
+    std::vector<agx_tuning> cand;
+    for (uint32_t sb = 1; sb <= (uint32_t)w->solvePerCuMax; sb *= 2)
+        for (uint32_t cb = 1; cb <= (uint32_t)w->colourPerCuMax; cb *= 2) {
+            agx_tuning t{};
+            t.solve_blocks_per_cu = sb;
+            t.colour_blocks_per_cu = cb;
+            cand.push_back(t);
+        }
+    std::vector<double> ms(cand.size(), 0.0);
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_CHECK(w, hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return fail(w, AGX_ERR_HIP, "event"); }
+    int rc = AGX_OK;
+    for (int round = 0; round < 2 && rc == AGX_OK; round++)
+        for (size_t c = 0; c < cand.size() && rc == AGX_OK; c++) {
+            if ((rc = agx_world_set_tuning(w, &cand[c]))) break;
+            if (hipEventRecord(a, w->stream) != hipSuccess) { rc = fail(w, AGX_ERR_HIP, "event"); break; }
+            if ((rc = agx_world_step_graph(w, dt, steps_per_trial))) break;
+            float e = 0.0f;
+            if (hipEventRecord(b, w->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&e, a, b) != hipSuccess) { rc = fail(w, AGX_ERR_HIP, "event timing"); break; }
+            agx_stats st{};
+            if ((rc = agx_world_read_stats(w, &st))) break;
+            ms[c] += st.overflow ? 1e30 : (double)e;   // a shape that overflowed a capacity is never chosen
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != AGX_OK) return rc;
+    size_t best = 0;
+    for (size_t c = 1; c < cand.size(); c++) if (ms[c] < ms[best]) best = c;
+    agx_tuning t = cand[best];
+    t.ms_per_step = (float)(ms[best] / (2.0 * steps_per_trial));
+    if (int r = agx_world_set_tuning(w, &t)) return r;
+    {
+        std::lock_guard<std::mutex> lock(g_tuneMutex);
+        storeTune(key, t);
+    }
+    if (out) { *out = t; out->from_cache = 0; out->max_solve_blocks_per_cu = (uint32_t)w->solvePerCuMax;
+               out->max_colour_blocks_per_cu = (uint32_t)w->colourPerCuMax; }
     return AGX_OK;
 }
 

@@ -58,3 +58,28 @@ def test_graph_steps_with_learned_warm_start(agx, scenes):
     b.step_graph(dt, 9)
     _same(a, b)
     assert b.stats().learned_warm_start == 1
+
+
+def test_launch_shapes_give_identical_steps_and_autotune(agx, scenes):
+    """The colouring and the solve order inside a colour do not depend on how many blocks run the
+    dataflow kernels, so every launch shape steps to the same bits; the tuner picks one of them and
+    reuses it from its cache."""
+    a, b = _world(agx, scenes), _world(agx, scenes)
+    t = a.tuning()
+    assert t.solve_blocks_per_cu >= 1 and t.max_solve_blocks_per_cu >= t.solve_blocks_per_cu
+    b.set_tuning(1, 1)
+    dt = 1.0 / 60.0
+    a.step(dt, 20)
+    b.step_graph(dt, 20)
+    _same(a, b)
+    with pytest.raises(agx.AgxError):
+        b.set_tuning(t.max_solve_blocks_per_cu + 1, 1)
+    r = a.autotune(dt, 2)
+    assert r.from_cache == 0 and r.ms_per_step > 0
+    assert 1 <= r.solve_blocks_per_cu <= r.max_solve_blocks_per_cu
+    assert 1 <= r.colour_blocks_per_cu <= r.max_colour_blocks_per_cu
+    now = a.tuning()
+    assert (now.solve_blocks_per_cu, now.colour_blocks_per_cu) == (r.solve_blocks_per_cu, r.colour_blocks_per_cu)
+    again = a.autotune(dt, 2)
+    assert again.from_cache == 1
+    assert (again.solve_blocks_per_cu, again.colour_blocks_per_cu) == (r.solve_blocks_per_cu, r.colour_blocks_per_cu)
