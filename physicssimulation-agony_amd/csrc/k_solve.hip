@@ -888,6 +888,20 @@ __global__ void kIslandCount(Counters* ctr) { ctr->islands = ctr->scratch[3]; }
 
 }  // namespace
 
+// Launches a dataflow kernel whose blocks wait on one another, so all of them must be resident at
+// once. The grid never exceeds the kernel's occupancy (blocks per CU times CUs), and a cooperative
+// launch has the runtime check that. The runtime refuses a cooperative launch with
+// hipErrorIllegalState while another process on the device holds its cooperative-launch resource;
+// the same grid is then launched as an ordinary kernel. Without other work on the device every block
+// is resident; if another process's kernels keep some blocks out, every wait ends at its 2 s limit
+// and flags OVERFLOW_STALL, so the step reports an error instead of hanging.
+static hipError_t launchResident(const void* kernel, uint32_t blocks, void** args, hipStream_t st) {
+    hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(256), args, 0, st);
+    if (e != hipErrorIllegalState) return e;
+    (void)hipGetLastError();
+    return hipLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, st);
+}
+
 void dataflowOccupancy(int& colour, int& solve) {
     int a = 0, b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void*>(kColourFlow), 256, 0) != hipSuccess) a = 1;
@@ -916,7 +930,7 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
                        w.posOf, w.rank, w.incSorted);
     void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.colRec,
                      (void*)&w.colourId };
-    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kColourFlow), dim3(w.colourBlocks), dim3(256), args, 0, st);
+    e = launchResident(reinterpret_cast<const void*>(kColourFlow), (uint32_t)w.colourBlocks, args, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
     hipLaunchKernelGGL(kColourColumnScan, dim3(MAX_COLOURS), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
@@ -950,7 +964,7 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
     uint32_t* pDone = w.groupDone + w.solveGroups;
     void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&vDone, (void*)&pDone,
                      (void*)&w.blockRes, (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kSolveFlow), dim3(w.solveBlocks), dim3(256), args, 0, st);
+    return launchResident(reinterpret_cast<const void*>(kSolveFlow), (uint32_t)w.solveBlocks, args, st);
 }
 
 hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors) {
