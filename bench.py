@@ -7,12 +7,14 @@ post-update pose pass, i.e. Simulation::update(1/60) with a 1/60 s fixed interva
 resident in HBM when a timed region starts.
 
 Headline (``value``): the batch of 1,024 independent scenes (log-uniform 64-4,096 bodies; box
-pyramids, random piles, funnels; about 1M bodies), settled, then stepped with velocity sweeps
-until the relative contact residual is <= 1e-3: body-steps/s of the learned warm start when it is
-available, else of the plain solver; both are reported with their sweep counts. Beside it, in
-their own objects: the 65,536-body pile with 8 velocity + 3 position sweeps (the plain GPU path
-against the single-threaded oracle) and the 1,048,576-collider scatter stepped collide-only with
-CCD (pairs/s).
+pyramids, random piles, funnels; about 1M bodies), settled at the residual target, then stepped
+with velocity sweeps until the relative contact residual is <= 1e-3: body-steps/s of the batched
+solver with the learned warm start when it is available, else of the plain (cached) warm start.
+Reported beside it: the plain warm start on the same batch, the plain method without batching
+(a stratified sample of the scenes, each in a world of its own), the plain solver at a fixed 8
+sweeps, the 65,536-body pile with 8 velocity + 3 position sweeps (the plain GPU path against the
+single-threaded oracle) and the 1,048,576-collider scatter stepped collide-only with CCD
+(pairs/s).
 """
 import argparse
 import importlib.util
@@ -178,72 +180,116 @@ def build_world(agx, scenes, settings):
     return agx.World(b, settings)
 
 
-def collect_and_train(agx, mlp, learned_world, plain_world, dt, steps, every, mode, iterations, lr):
-    """Steps both worlds `steps` times in lockstep; the learned world runs in `mode` and adds the
-    training rows of every `every`-th step; then the network trains `iterations` batches on the GPU."""
-    for k in range(steps):
-        collect = k % every == 0
-        learned_world.set_learned(mlp, mode | (agx.LEARN_COLLECT if collect else 0))
-        learned_world.step(dt, 1)
-        plain_world.step(dt, 1)
-    learned_world.sync()
-    t0 = time.perf_counter()
-    loss = 0.0
-    for e in range(iterations // 1000):
-        loss = mlp.train(1000, 4096, lr if e < (iterations // 1000) * 7 // 10 else lr / 5, seed=mode * 100 + e + 1)
-    return loss, time.perf_counter() - t0
+def scene_ranges(world, num_scenes):
+    """Body and collider index ranges of every scene of a batch world (scenes are concatenated)."""
+    cs = world.collider("scene")
+    cb = world.collider("body")
+    bscene = np.empty(world.counts.bodies, np.int64)
+    bscene[cb] = cs
+    bstart = np.searchsorted(bscene, np.arange(num_scenes + 1))
+    cstart = np.searchsorted(cs, np.arange(num_scenes + 1))
+    return bstart, cstart
+
+
+def world_for_scene(agx, scene, batch_world, bsel, csel, settings, prev_manifolds):
+    """A one-scene world holding the batch world's current state of that scene: body state, the
+    warm-start manifolds (collider indices made local) and the post-update collider poses."""
+    w = build_world(agx, [scene], settings)
+    for g, _ in BODY_STATE:
+        w.set_body(g, batch_world.body(g)[bsel])
+    c0, c1 = int(csel.start), int(csel.stop)
+    pm = prev_manifolds[(prev_manifolds["a"] >= c0) & (prev_manifolds["a"] < c1)].copy()
+    pm["a"] -= c0
+    pm["b"] -= c0
+    w.set_warm_start_source(pm)
+    w.post_update()
+    return w
+
+
+def unbatched_sample(agx, world, scenes, settings, dt, stride, steps):
+    """The plain method without batching: every stride-th scene of the batch in a world of its own,
+    started from the batch world's current state and stepped `steps` times with ordinary launches
+    (Simulation::update per scene, one scene after the other). Returns the sample's bodies, summed
+    ms per step and body-weighted mean sweeps per step."""
+    bstart, cstart = scene_ranges(world, len(scenes))
+    prev = world.manifolds(1)
+    bodies = 0
+    ms = 0.0
+    sweeps = 0.0
+    for i in range(0, len(scenes), stride):
+        w = world_for_scene(agx, scenes[i], world, slice(int(bstart[i]), int(bstart[i + 1])),
+                            slice(int(cstart[i]), int(cstart[i + 1])), settings, prev)
+        n = w.counts.bodies
+        s0 = w.stats().total_velocity_sweeps
+        w.sync()
+        t0 = time.perf_counter()
+        w.step(dt, steps)
+        w.sync()
+        ms += 1e3 * (time.perf_counter() - t0) / steps
+        sweeps += n * (w.stats().total_velocity_sweeps - s0) / steps
+        bodies += n
+        w.close()
+    return bodies, ms, sweeps / max(bodies, 1)
 
 
 def run_batch(agx, scenes_mod, args, dt, learned=True):
-    """Plain and learned warm starts on two copies of the batch stepped in lockstep, so both are
-    timed from the same settled state. The network is trained on the GPU on the solver's own
-    accumulators from the learned copy's untimed steps: first with the cached warm start, then
-    on its own predictions."""
+    """Plain and learned warm starts on two copies of the batch stepped in lockstep to the 1e-3
+    residual target from the same settled state. The network is trained on the GPU on the learned
+    copy's untimed steps (rows of contact points without a cached impulse, with the corrections the
+    solver itself reached as targets) and then warm-starts those points, behind the residual gate.
+    Beside them: the plain method without batching on a stratified sample of the scenes, and the
+    plain solver at a fixed 8 velocity sweeps."""
     t0 = time.perf_counter()
     scenes = scenes_mod.batch(1024)
     base = dict(position_iterations=3, track_residual=1, track_errors=1)
     fixed8 = agx.default_settings(velocity_iterations=8, **base)
     to_target = agx.default_settings(velocity_iterations=64, residual_target=1e-3, **base)
-    worlds = [build_world(agx, scenes, fixed8) for _ in range(2 if learned else 1)]
+    worlds = [build_world(agx, scenes, to_target) for _ in range(2 if learned else 1)]
     plain_w = worlds[0]
     n = plain_w.counts.bodies
     log(f"batch: {len(scenes)} scenes, {n} bodies built in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     for w in worlds:
         for k in range(0, args.warmup, 40):
-            w.step(dt, min(40, args.warmup - k))
+            w.step_graph(dt, min(40, args.warmup - k))
         w.sync()
-    log(f"batch: warm-up {args.warmup} steps in {time.perf_counter() - t0:.1f} s")
+    log(f"batch: warm-up {args.warmup} steps to the residual target in {time.perf_counter() - t0:.1f} s")
     out = {}
-    # plain solver, fixed 8 velocity sweeps
-    el = step_timed(plain_w, args.steps, dt)
-    out["plain_fixed8"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
-                           "velocity_sweeps": 8}
-    for w in worlds[1:]:
-        w.step(dt, args.steps)
-    for w in worlds:
-        w.set_settings(to_target)
     # launch shape of the dataflow kernels, measured on the plain copy and applied to both
     n0 = plain_w.stats().solved_steps
     tune = plain_w.autotune(dt, 2)
     tuned_steps = plain_w.stats().solved_steps - n0
     for w in worlds[1:]:
         w.set_tuning(tune.solve_blocks_per_cu, tune.colour_blocks_per_cu)
-        w.step(dt, tuned_steps)
+        w.step_graph(dt, tuned_steps)
     out["tuning"] = {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu,
                      "from_cache": int(tune.from_cache), "steps": int(tuned_steps)}
     log(f"batch: tuned {tune.solve_blocks_per_cu}/{tune.colour_blocks_per_cu} blocks per CU over {tuned_steps} steps")
+    mode = agx.LEARN_PREDICT | agx.LEARN_NEW_CONTACTS
     if learned:
         lw = worlds[1]
         mlp = agx.Mlp(seed=1, sample_capacity=16 << 20)
         t0 = time.perf_counter()
-        loss, tr = collect_and_train(agx, mlp, lw, plain_w, dt, 4, 1, agx.LEARN_OFF, 10000, 1e-3)
-        net = {"rows": mlp.sample_count(), "train_iterations": 10000, "batch": 4096, "final_batch_loss": round(loss, 4),
-               "train_s": round(tr, 2), "collect_and_train_s": round(time.perf_counter() - t0, 2),
-               "data": "every contact point of the learned copy's 4 untimed steps to the 1e-3 target with the cached "
-                       "warm start (features, and the corrections that reach the solver's final accumulators)"}
+        collect_steps = 12
+        lw.set_learned(mlp, agx.LEARN_COLLECT | agx.LEARN_NEW_CONTACTS)
+        for _ in range(collect_steps):
+            lw.step(dt, 1)
+            plain_w.step(dt, 1)
+        lw.set_learned(None, agx.LEARN_OFF)
+        lw.sync()
+        t1 = time.perf_counter()
+        iterations, loss = 10000, 0.0
+        for e in range(iterations // 1000):
+            loss = mlp.train(1000, 4096, 1e-3 if e < 7 else 2e-4, seed=e + 1)
+        tr = time.perf_counter() - t1
+        net = {"rows": mlp.sample_count(), "train_iterations": iterations, "batch": 4096,
+               "final_batch_loss": round(loss, 4), "train_s": round(tr, 2),
+               "collect_and_train_s": round(time.perf_counter() - t0, 2), "mode": "new contacts, gated",
+               "data": f"every contact point without a cached impulse in the learned copy's {collect_steps} untimed "
+                       f"steps to the 1e-3 target (features, and the impulses the solver reached, in units of "
+                       f"the point's weight impulse)"}
         log(f"network: {mlp.sample_count()} rows, loss {loss:.3f}, train {tr:.1f} s")
-        lw.set_learned(mlp, agx.LEARN_PREDICT)
+        lw.set_learned(mlp, mode)
     # timed: the residual target, plain then learned, each from the same step index
     st0 = plain_w.stats()
     el = step_timed(plain_w, args.steps, dt)
@@ -251,7 +297,7 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
     mean_plain = (st.total_velocity_sweeps - st0.total_velocity_sweeps) / args.steps
     out["plain"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
                     "mean_velocity_sweeps": round(mean_plain, 2), "velocity_sweeps_last_step": int(st.velocity_sweeps),
-                    "residual_last_sweep": round(float(st.residual[max(int(st.velocity_sweeps) - 1, 0)]), 6)}
+                    "residual_per_sweep_last_step": [round(float(st.residual[k]), 6) for k in range(st.velocity_sweeps)]}
     if learned:
         st0 = lw.stats()
         el = step_timed(lw, args.steps, dt)
@@ -277,6 +323,25 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
         out["learned"]["segments_ms_per_step"] = {k: round(v, 4) for k, v in lseg.items()}
         for w in worlds[1:]:
             w.close()
+    # the plain method without batching, from the same state
+    out["plain_unbatched"] = None
+    if args.unbatched_stride > 0:
+        t0 = time.perf_counter()
+        sb, sms, ssw = unbatched_sample(agx, plain_w, scenes, to_target, dt, args.unbatched_stride, 10)
+        out["plain_unbatched"] = {
+            "value": round(sb / (sms * 1e-3), 1), "ms_per_batch_step_of_sample": round(sms, 3),
+            "body_weighted_mean_velocity_sweeps": round(ssw, 2),
+            "sample": f"every {args.unbatched_stride}th scene ({len(range(0, len(scenes), args.unbatched_stride))} "
+                      f"scenes, {sb} bodies), each in a world of its own started from the batch's state, 10 steps "
+                      f"each with ordinary launches, one scene after the other"}
+        log(f"unbatched sample: {sb} bodies, {sms:.1f} ms per step of all sampled scenes, in {time.perf_counter() - t0:.1f} s")
+    # the plain solver at the reference's fixed 8 velocity sweeps (last: it leaves the state less converged)
+    plain_w.set_settings(fixed8)
+    plain_w.step(dt, 2)
+    el = step_timed(plain_w, args.steps, dt)
+    out["plain_fixed8"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
+                           "velocity_sweeps": 8}
+    plain_w.set_settings(to_target)
     out["bodies"] = n
     out["world"], out["scenes"] = plain_w, scenes
     return out
@@ -380,6 +445,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=240, help="settling steps of the headline batch")
     ap.add_argument("--workload", default="all", choices=["all", "batch1024", "pile65536", "scatter1m"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--unbatched-stride", type=int, default=16,
+                    help="every n-th scene of the batch is stepped in a world of its own (plain, unbatched); 0: none")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="plain warm start only")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -412,11 +479,13 @@ def main():
                    "position_iterations": 3, "dt": dt, "warm_start": "learned" if b["learned"] else "cached",
                    "parallelism": "single GPU"},
         "roofline": b["roofline"],
-        "plain": b["plain"], "learned": b["learned"], "plain_fixed8": b["plain_fixed8"], "stats": b["stats"],
-        "tuning": b["tuning"],
+        "plain": b["plain"], "learned": b["learned"], "plain_unbatched": b["plain_unbatched"],
+        "plain_fixed8": b["plain_fixed8"], "stats": b["stats"], "tuning": b["tuning"],
     }
+    if b["plain_unbatched"]:
+        out["batched_over_unbatched_plain"] = round(head["value"] / b["plain_unbatched"]["value"], 2)
     if b["learned"]:
-        out["learned_over_plain"] = round(b["learned"]["value"] / b["plain"]["value"], 3)
+        out["learned_over_plain_batched"] = round(b["learned"]["value"] / b["plain"]["value"], 3)
     if cpu:
         out["cpu_baseline"] = batch_cpu_baseline(b["world"], b["scenes"], max(int(round(b["plain"]["mean_velocity_sweeps"])), 1),
                                                  args.cpu_seconds)

@@ -276,6 +276,9 @@ int agx_mlp_train(agx_mlp* m, uint32_t iterations, uint32_t batch, float learnin
 #define AGX_LEARN_PREDICT 1u
 #define AGX_LEARN_COLLECT 2u
 #define AGX_LEARN_UNGATED 4u
+/* Apply the network only at contact points without a cached impulse (new contacts); points with one
+   keep the cached warm start. With AGX_LEARN_COLLECT, only such points give training rows. */
+#define AGX_LEARN_NEW_CONTACTS 8u
 int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode);
 
 /* Residual gate: exponential averages (weight 1/4 per step) of the cost of a step with the cached

@@ -79,7 +79,7 @@ class Tuning(C.Structure):
                 ("max_solve_blocks_per_cu", C.c_uint32), ("max_colour_blocks_per_cu", C.c_uint32)]
 
 
-LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED = 0, 1, 2, 4
+LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED, LEARN_NEW_CONTACTS = 0, 1, 2, 4, 8
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS, MLP_BATCH_MAX = 24, 128, 2, 19970, 8192
 
 

@@ -90,6 +90,7 @@ struct Counters {
     uint32_t learnPredictRows; // rows the network runs on (learnRows when the prediction is used, else 0)
     uint32_t learnUsed;        // this step's warm start came from the network
     uint32_t sampleBase;       // first training row this step claimed
+    uint32_t learnCompactRows; // new-contacts mode: rows written (one per contact point without a cached impulse)
 };
 
 constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)
@@ -235,6 +236,7 @@ struct DeviceWorld {
     // learned warm start: per contact point row 2m + p
     uint16_t* learnX = nullptr;      // features (bf16, 32 per row)
     float* learnY = nullptr;         // network outputs (2 per row)
+    uint32_t* learnMap = nullptr;    // new-contacts mode: row -> 2 * manifold + point
     // accumulator history: warmHist[c][m] holds the cached accumulators (jn0, jt0, jn1, jt1) manifold
     // m of the step with cur == c started from; the narrow phase carries the matched previous
     // manifold's entry into histOf (NaN for a point without one), i.e. the accumulators of two steps ago

@@ -56,80 +56,147 @@ __global__ void kLearnRows(Counters* ctr, uint32_t predict, uint32_t gated, uint
     ctr->learnPredictRows = use ? rows : 0u;
 }
 
-__global__ __launch_bounds__(256) void kLearnFeatures(const Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
+// Features of contact point p of manifold m (see the list above); false for a point that takes no
+// impulse (the row stays all zero).
+__device__ inline bool pointFeatures(const Constraints& cs, uint32_t slot, uint32_t p, const float4* __restrict__ vel,
+                                     const uint32_t* __restrict__ degree, float depth, float4 hist, const Scale& sc,
+                                     float (&f)[MLP_INP]) {
+    for (uint32_t k = 0; k < MLP_INP; k++) f[k] = 0.0f;
+    const uint4 id = cs.idx[slot];
+    const float4 k0 = cs.k0[slot];
+    const float nMass[2] = { k0.x, k0.w };
+    const float S = nMass[p] * sc.gdt;
+    if (!(p < id.z && S > 0.0f)) return false;
+    const float4 j = cs.imp[slot];
+    const float4 nr = cs.nrm[slot], iv = cs.inv[slot], r0 = cs.r0[slot], r1 = cs.r1[slot];
+    const float4 k1 = cs.k1[slot];
+    const float4 vA = vel[id.x], vB = vel[id.y];
+    const float hn[2] = { hist.x, hist.z }, ht[2] = { hist.y, hist.w };
+    const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
+    const float dvx = vB.x - vA.x, dvy = vB.y - vA.y;
+    const float dvn = dvx * nx + dvy * ny, dvt = dvx * tx + dvy * ty;
+    const float vn[2] = { dvn + r0.y * vB.z - r0.x * vA.z, dvn + r1.y * vB.z - r1.x * vA.z };
+    const float vt[2] = { dvt + r0.w * vB.z - r0.z * vA.z, dvt + r1.w * vB.z - r1.z * vA.z };
+    const float tMass[2] = { k0.y, k1.x };
+    const float rAn[2] = { r0.x, r1.x }, rBn[2] = { r0.y, r1.y };
+    const float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w };
+    const float degA = iv.x != 0.0f ? (float)degree[id.x] : 0.0f;
+    const float degB = iv.y != 0.0f ? (float)degree[id.y] : 0.0f;
+    const float invG = 1.0f / sc.gdt;
+    const uint32_t o = 1u - p;
+    const float invS = 1.0f / S;
+    const bool two = id.z > 1;
+    f[0] = jn[p] * invS;
+    f[1] = jt[p] * invS;
+    f[2] = two ? jn[o] * invS : 0.0f;
+    f[3] = two ? jt[o] * invS : 0.0f;
+    f[4] = (jn[p] != 0.0f || jt[p] != 0.0f) ? 1.0f : 0.0f;
+    f[5] = two ? 1.0f : 0.0f;
+    f[6] = vn[p] * invG;
+    f[7] = vt[p] * invG;
+    const bool hasHist = hn[p] == hn[p] && ht[p] == ht[p];
+    f[8] = hasHist ? 1.0f : 0.0f;
+    f[9] = depth / sc.gdt2;
+    f[10] = nx * sc.upX + ny * sc.upY;
+    f[11] = nx * sc.upY - ny * sc.upX;
+    f[12] = iv.x * nMass[p];
+    f[13] = iv.y * nMass[p];
+    f[14] = rAn[p] * rAn[p] * iv.z * nMass[p];
+    f[15] = rBn[p] * rBn[p] * iv.w * nMass[p];
+    f[16] = tMass[p] / nMass[p];
+    f[17] = hasHist ? (jn[p] - hn[p]) * invS : 0.0f;
+    f[18] = hasHist ? (jt[p] - ht[p]) * invS : 0.0f;
+    f[19] = iv.x == 0.0f ? 1.0f : 0.0f;
+    f[20] = iv.y == 0.0f ? 1.0f : 0.0f;
+    f[21] = degA * 0.125f;
+    f[22] = degB * 0.125f;
+    f[23] = two ? vn[o] * invG : 0.0f;
+    return true;
+}
+
+__device__ inline void storeFeatureRow(uint16_t* __restrict__ x, uint64_t row, const float (&f)[MLP_INP]) {
+    uint32_t packed[MLP_INP / 2];
+    for (uint32_t k = 0; k < MLP_INP / 2; k++)
+        packed[k] = (uint32_t)bf16Bits(clampFeature(f[2 * k])) | ((uint32_t)bf16Bits(clampFeature(f[2 * k + 1])) << 16);
+    uint4* dst = reinterpret_cast<uint4*>(x + row * MLP_INP);
+    for (uint32_t k = 0; k < MLP_INP / 8; k++)
+        dst[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+}
+
+// Wave-wide exclusive prefix of `count` plus one atomic claim of the wave's total on `counter`:
+// the first of this lane's rows. Every lane of the wave must call it.
+__device__ inline uint32_t claimRows(uint32_t count, uint32_t* counter) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    uint32_t prefix = count;   // inclusive prefix over the wave
+    for (uint32_t off = 1; off < WAVE; off <<= 1) {
+        const uint32_t v = __shfl_up(prefix, off, WAVE);
+        if (lane >= off) prefix += v;
+    }
+    const uint32_t total = __shfl(prefix, WAVE - 1, WAVE);
+    uint32_t row0 = 0;
+    if (lane == WAVE - 1 && total) row0 = atomicAdd(counter, total);
+    return __shfl(row0, WAVE - 1, WAVE) + prefix - count;
+}
+
+// Records every manifold's starting accumulators (the next step's history) and, when the network
+// runs or rows are collected, the feature rows: two per manifold at rows 2m and 2m + 1, or with
+// newOnly one per contact point without a cached impulse, in claimed rows with map[row] = 2m + p.
+__global__ __launch_bounds__(256) void kLearnFeatures(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
                                                       const uint32_t* __restrict__ posOf, const uint32_t* __restrict__ degree,
                                                       const float4* __restrict__ vel, Constraints cs, Scale sc,
                                                       const float4* __restrict__ histOf, float4* __restrict__ warmHist,
-                                                      uint16_t* __restrict__ x) {
-    // the starting accumulators are recorded on every step; features only when the network runs
-    // or rows are collected
+                                                      uint32_t newOnly, uint16_t* __restrict__ x, uint32_t* __restrict__ map) {
     const uint32_t M = ctr->manifolds;
     const bool rows = ctr->learnRows != 0;
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
-        const uint32_t slot = posOf[m];
-        const float4 j = cs.imp[slot];
-        warmHist[m] = j;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    // wave-uniform rounds, so that the row claims of newOnly stay wave-wide
+    for (uint32_t base = first - lane; base < M; base += stride) {
+        const uint32_t m = base + lane;
+        uint32_t slot = 0;
+        float4 j = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (m < M) {
+            slot = posOf[m];
+            j = cs.imp[slot];
+            warmHist[m] = j;
+        }
         if (!rows) continue;
-        const uint4 id = cs.idx[slot];
-        const float4 nr = cs.nrm[slot], iv = cs.inv[slot], r0 = cs.r0[slot], r1 = cs.r1[slot];
-        const float4 k0 = cs.k0[slot], k1 = cs.k1[slot];
-        const float4 vA = vel[id.x], vB = vel[id.y];
+        float f[MLP_INP];
+        if (!newOnly) {
+            if (m >= M) continue;
+            const float depth = man[m].depth;
+            const float4 hist = histOf[m];
+            for (uint32_t p = 0; p < 2; p++) {
+                pointFeatures(cs, slot, p, vel, degree, depth, hist, sc, f);
+                storeFeatureRow(x, 2ull * m + p, f);
+            }
+            continue;
+        }
+        uint32_t keep = 0;
+        if (m < M) {
+            const uint32_t count = min(cs.idx[slot].z, 2u);
+            const float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w };
+            for (uint32_t p = 0; p < count; p++)
+                if (jn[p] == 0.0f && jt[p] == 0.0f && normalMassOf(cs, slot, p) > 0.0f) keep |= 1u << p;
+        }
+        uint32_t row = claimRows(__builtin_popcount(keep), &ctr->learnCompactRows);
+        if (!keep) continue;
         const float depth = man[m].depth;
         const float4 hist = histOf[m];
-        const float hn[2] = { hist.x, hist.z }, ht[2] = { hist.y, hist.w };
-        const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
-        const float dvx = vB.x - vA.x, dvy = vB.y - vA.y;
-        const float dvn = dvx * nx + dvy * ny, dvt = dvx * tx + dvy * ty;
-        const float vn[2] = { dvn + r0.y * vB.z - r0.x * vA.z, dvn + r1.y * vB.z - r1.x * vA.z };
-        const float vt[2] = { dvt + r0.w * vB.z - r0.z * vA.z, dvt + r1.w * vB.z - r1.z * vA.z };
-        const float nMass[2] = { k0.x, k0.w }, tMass[2] = { k0.y, k1.x };
-        const float rAn[2] = { r0.x, r1.x }, rBn[2] = { r0.y, r1.y };
-        const float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w };
-        const float degA = iv.x != 0.0f ? (float)degree[id.x] : 0.0f;
-        const float degB = iv.y != 0.0f ? (float)degree[id.y] : 0.0f;
-        const float invG = 1.0f / sc.gdt;
         for (uint32_t p = 0; p < 2; p++) {
-            float f[MLP_INP];
-            for (uint32_t k = 0; k < MLP_INP; k++) f[k] = 0.0f;
-            const float S = nMass[p] * sc.gdt;
-            if (p < id.z && S > 0.0f) {
-                const uint32_t o = 1u - p;
-                const float invS = 1.0f / S;
-                const bool two = id.z > 1;
-                f[0] = jn[p] * invS;
-                f[1] = jt[p] * invS;
-                f[2] = two ? jn[o] * invS : 0.0f;
-                f[3] = two ? jt[o] * invS : 0.0f;
-                f[4] = (jn[p] != 0.0f || jt[p] != 0.0f) ? 1.0f : 0.0f;
-                f[5] = two ? 1.0f : 0.0f;
-                f[6] = vn[p] * invG;
-                f[7] = vt[p] * invG;
-                const bool hasHist = hn[p] == hn[p] && ht[p] == ht[p];
-                f[8] = hasHist ? 1.0f : 0.0f;
-                f[9] = depth / sc.gdt2;
-                f[10] = nx * sc.upX + ny * sc.upY;
-                f[11] = nx * sc.upY - ny * sc.upX;
-                f[12] = iv.x * nMass[p];
-                f[13] = iv.y * nMass[p];
-                f[14] = rAn[p] * rAn[p] * iv.z * nMass[p];
-                f[15] = rBn[p] * rBn[p] * iv.w * nMass[p];
-                f[16] = tMass[p] / nMass[p];
-                f[17] = hasHist ? (jn[p] - hn[p]) * invS : 0.0f;
-                f[18] = hasHist ? (jt[p] - ht[p]) * invS : 0.0f;
-                f[19] = iv.x == 0.0f ? 1.0f : 0.0f;
-                f[20] = iv.y == 0.0f ? 1.0f : 0.0f;
-                f[21] = degA * 0.125f;
-                f[22] = degB * 0.125f;
-                f[23] = two ? vn[o] * invG : 0.0f;
-            }
-            uint32_t packed[MLP_INP / 2];
-            for (uint32_t k = 0; k < MLP_INP / 2; k++)
-                packed[k] = (uint32_t)bf16Bits(clampFeature(f[2 * k])) | ((uint32_t)bf16Bits(clampFeature(f[2 * k + 1])) << 16);
-            uint4* row = reinterpret_cast<uint4*>(x + (size_t)(2 * m + p) * MLP_INP);
-            for (uint32_t k = 0; k < MLP_INP / 8; k++)
-                row[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+            if (!(keep >> p & 1u)) continue;
+            pointFeatures(cs, slot, p, vel, degree, depth, hist, sc, f);
+            storeFeatureRow(x, row, f);
+            map[row] = 2u * m + p;
+            row++;
         }
     }
+}
+
+// New-contacts mode: the network runs on the claimed rows only, on steps the gate gave to it.
+__global__ void kLearnCompactRows(Counters* ctr) {
+    ctr->learnPredictRows = ctr->learnUsed ? ctr->learnCompactRows : 0u;
 }
 
 // Initial accumulators from the network's corrections (only on steps the gate gave to it).
@@ -154,51 +221,76 @@ __global__ __launch_bounds__(256) void kLearnApply(const Counters* __restrict__ 
     }
 }
 
+// New-contacts mode: the correction of each claimed row's point. Each row owns one half of its
+// slot's accumulators (jn, jt of one point), so rows of the same manifold never share a store.
+__global__ __launch_bounds__(256) void kLearnApplyRows(const Counters* __restrict__ ctr, const uint32_t* __restrict__ posOf,
+                                                       const uint32_t* __restrict__ map, Constraints cs,
+                                                       const float* __restrict__ y, float gdt) {
+    const uint32_t n = ctr->learnPredictRows;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const uint32_t pt = map[r], m = pt >> 1, p = pt & 1u;
+        const uint32_t slot = posOf[m];
+        const float S = normalMassOf(cs, slot, p) * gdt;
+        const float y0 = y[(size_t)r * MLP_OUT], y1 = y[(size_t)r * MLP_OUT + 1];
+        if (!(S > 0.0f) || !isfinite(y0) || !isfinite(y1)) continue;
+        float2* half = reinterpret_cast<float2*>(cs.imp + slot) + p;
+        const float2 j = *half;
+        *half = make_float2(fmaxf(0.0f, j.x + S * fminf(fmaxf(y0, -FEATURE_CLAMP), FEATURE_CLAMP)),
+                            j.y + S * fminf(fmaxf(y1, -FEATURE_CLAMP), FEATURE_CLAMP));
+    }
+}
+
 // Training rows from this step: features of every contact point and, as targets, the corrections
 // that would have turned the cached accumulators into the solver's final ones. Runs after the
 // sweeps and before the accumulators are written back to the manifolds (which still hold the
 // cached values). Rows are claimed per wave; rows past the buffer's capacity are dropped.
 __global__ __launch_bounds__(256) void kLearnCollect(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
                                                      const uint32_t* __restrict__ posOf, Constraints cs,
-                                                     const uint16_t* __restrict__ x, float gdt, uint16_t* __restrict__ sx,
+                                                     const uint16_t* __restrict__ x, const uint32_t* __restrict__ map,
+                                                     float gdt, uint32_t newOnly, uint16_t* __restrict__ sx,
                                                      float* __restrict__ st, uint32_t* __restrict__ sampleCount,
                                                      uint64_t cap) {
-    const uint32_t M = ctr->learnRows / 2;
+    // newOnly: one row per claimed feature row (points without a cached impulse); else the points
+    // of every manifold, features at rows 2m + p
+    const uint32_t n = newOnly ? ctr->learnCompactRows : ctr->learnRows / 2;
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
     // every lane of a wave runs the same number of rounds so the wave-wide claim stays uniform
-    for (uint32_t base = first - lane; base < M; base += stride) {
-        const uint32_t m = base + lane;
-        uint32_t count = 0, slot = 0;
-        if (m < M) {
-            slot = posOf[m];
-            count = min(cs.idx[slot].z, 2u);
+    for (uint32_t base = first - lane; base < n; base += stride) {
+        const uint32_t k = base + lane;
+        uint32_t m = 0, slot = 0, keep = 0;   // bit p: point p of manifold m gives a row
+        if (k < n) {
+            if (newOnly) {
+                m = map[k] >> 1;
+                keep = 1u << (map[k] & 1u);
+                slot = posOf[m];
+            } else {
+                m = k;
+                slot = posOf[m];
+                keep = (1u << min(cs.idx[slot].z, 2u)) - 1u;
+            }
         }
-        uint32_t prefix = count;   // inclusive prefix over the wave
-        for (uint32_t off = 1; off < WAVE; off <<= 1) {
-            const uint32_t v = __shfl_up(prefix, off, WAVE);
-            if (lane >= off) prefix += v;
-        }
-        const uint32_t total = __shfl(prefix, WAVE - 1, WAVE);
-        uint32_t row0 = 0;
-        if (lane == WAVE - 1 && total) row0 = atomicAdd(sampleCount, total);
-        row0 = __shfl(row0, WAVE - 1, WAVE) + prefix - count;
-        if (m >= M || count == 0) continue;
+        const uint32_t count = __builtin_popcount(keep);
+        const uint32_t row0 = claimRows(count, sampleCount);
+        if (count == 0) continue;
         const agx_manifold d = man[m];
         const float4 j = cs.imp[slot];
         const float jnC[2] = { d.jn0, d.jn1 }, jtC[2] = { d.jt0, d.jt1 };
         const float jnF[2] = { j.x, j.z }, jtF[2] = { j.y, j.w };
-        for (uint32_t p = 0; p < count; p++) {
-            const uint64_t row = (uint64_t)row0 + p;
+        uint64_t row = row0;
+        for (uint32_t p = 0; p < 2; p++) {
+            if (!(keep >> p & 1u)) continue;
             if (row >= cap) break;
             const float S = normalMassOf(cs, slot, p) * gdt;
             const float invS = S > 0.0f ? 1.0f / S : 0.0f;
-            const uint4* src = reinterpret_cast<const uint4*>(x + (size_t)(2 * m + p) * MLP_INP);
+            const uint64_t srcRow = newOnly ? k : 2ull * m + p;
+            const uint4* src = reinterpret_cast<const uint4*>(x + srcRow * MLP_INP);
             uint4* dst = reinterpret_cast<uint4*>(sx + row * MLP_INP);
-            for (uint32_t k = 0; k < MLP_INP / 8; k++) dst[k] = src[k];
+            for (uint32_t q = 0; q < MLP_INP / 8; q++) dst[q] = src[q];
             st[row * MLP_OUT] = clampFeature((jnF[p] - jnC[p]) * invS);
             st[row * MLP_OUT + 1] = clampFeature((jtF[p] - jtC[p]) * invS);
+            row++;
         }
     }
 }
@@ -235,26 +327,31 @@ Scale scaleOf(float gx, float gy, float dt) {
 }  // namespace
 
 hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice* mlp, uint32_t predict, uint32_t gated,
-                                 uint32_t collect, float gx, float gy, float dt) {
+                                 uint32_t collect, uint32_t newOnly, float gx, float gy, float dt) {
     hipLaunchKernelGGL(kLearnRows, dim3(1), dim3(1), 0, st, w.ctr, predict && mlp ? 1u : 0u, gated, collect);
     const Scale sc = scaleOf(gx, gy, dt);
     const uint32_t grid = gridOf(std::min<uint64_t>(w.manCap, 1u << 22));
     hipLaunchKernelGGL(kLearnFeatures, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.posOf, w.degree, w.vel, w.cs, sc,
-                       w.histOf, w.warmHist[w.cur], w.learnX);
+                       w.histOf, w.warmHist[w.cur], newOnly, w.learnX, w.learnMap);
+    if (newOnly) hipLaunchKernelGGL(kLearnCompactRows, dim3(1), dim3(1), 0, st, w.ctr);
     if (predict && mlp) {
         const uint32_t rows = (uint32_t)std::min<uint64_t>(2 * w.manCap, 0xFFFFFFF0u);
         hipError_t e = launchMlpForward(st, *mlp, w.learnX, rows, w.learnY, nullptr, nullptr, &w.ctr->learnPredictRows);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(kLearnApply, dim3(grid), dim3(256), 0, st, w.ctr, w.posOf, w.cs, w.learnY, sc.gdt);
+        if (newOnly)
+            hipLaunchKernelGGL(kLearnApplyRows, dim3(grid), dim3(256), 0, st, w.ctr, w.posOf, w.learnMap, w.cs, w.learnY, sc.gdt);
+        else
+            hipLaunchKernelGGL(kLearnApply, dim3(grid), dim3(256), 0, st, w.ctr, w.posOf, w.cs, w.learnY, sc.gdt);
     }
     return hipGetLastError();
 }
 
-hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, float gx, float gy, float dt) {
+hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, uint32_t newOnly, float gx,
+                              float gy, float dt) {
     const Scale sc = scaleOf(gx, gy, dt);
     const uint32_t grid = gridOf(std::min<uint64_t>(w.manCap, 1u << 22));
-    hipLaunchKernelGGL(kLearnCollect, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.posOf, w.cs, w.learnX, sc.gdt,
-                       smp.x, smp.t, smp.count, smp.cap);
+    hipLaunchKernelGGL(kLearnCollect, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.posOf, w.cs, w.learnX, w.learnMap,
+                       sc.gdt, newOnly, smp.x, smp.t, smp.count, smp.cap);
     return hipGetLastError();
 }
 

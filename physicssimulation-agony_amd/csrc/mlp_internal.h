@@ -104,8 +104,9 @@ hipStream_t mlpStream(const agx_mlp* m);
 // between the sweeps and the write-back of the accumulators).
 struct DeviceWorld;
 hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice* mlp, uint32_t predict, uint32_t gated,
-                                 uint32_t collect, float gx, float gy, float dt);
-hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, float gx, float gy, float dt);
+                                 uint32_t collect, uint32_t newOnly, float gx, float gy, float dt);
+hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, uint32_t newOnly, float gx,
+                              float gy, float dt);
 hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w);
 
 }  // namespace agx

@@ -270,8 +270,9 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
             if (predict || collect) {
                 SegmentTimer t(w, SEG_LEARN);
                 if (!d.histOut) return fail(w, AGX_ERR_STATE, "learned mode was enabled after this step's narrow phase");
+                const uint32_t newOnly = (w->learnMode & AGX_LEARN_NEW_CONTACTS) ? 1u : 0u;
                 HIP_CHECK(w, agx::launchLearnBeforeWarm(st, d, agx::mlpDevice(w->mlp), predict ? 1u : 0u, gated ? 1u : 0u,
-                                                        collect ? 1u : 0u, s.gravity_x, s.gravity_y, dt));
+                                                        collect ? 1u : 0u, newOnly, s.gravity_x, s.gravity_y, dt));
             }
             w->histReady = predict || collect;
             {
@@ -285,7 +286,9 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
             }
             if (collect) {
                 SegmentTimer t(w, SEG_LEARN);
-                HIP_CHECK(w, agx::launchLearnCollect(st, d, *agx::mlpSamples(w->mlp), s.gravity_x, s.gravity_y, dt));
+                HIP_CHECK(w, agx::launchLearnCollect(st, d, *agx::mlpSamples(w->mlp),
+                                                     (w->learnMode & AGX_LEARN_NEW_CONTACTS) ? 1u : 0u, s.gravity_x,
+                                                     s.gravity_y, dt));
             }
             SegmentTimer t(w, SEG_SOLVE_FINISH);
             HIP_CHECK(w, agx::launchSolveFinish(st, d, trackResidual, s.residual_target, s.track_errors ? 1 : 0));
@@ -714,7 +717,7 @@ int agx_world_step(agx_world* w, float dt, uint32_t n_steps) {
 // ---- learned warm start
 int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
-    if (mode & ~(AGX_LEARN_PREDICT | AGX_LEARN_COLLECT | AGX_LEARN_UNGATED))
+    if (mode & ~(AGX_LEARN_PREDICT | AGX_LEARN_COLLECT | AGX_LEARN_UNGATED | AGX_LEARN_NEW_CONTACTS))
         return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown learned mode bits 0x%x", mode);
     if ((mode & (AGX_LEARN_PREDICT | AGX_LEARN_COLLECT)) && !mlp)
         return fail(w, AGX_ERR_INVALID_ARGUMENT, "learned mode needs a network");
@@ -723,6 +726,7 @@ int agx_world_set_learned(agx_world* w, agx_mlp* mlp, uint32_t mode) {
     if ((mode & (AGX_LEARN_PREDICT | AGX_LEARN_COLLECT)) && !d.learnX) {
         const size_t man = std::max<uint64_t>(d.manCap, 1), rows = 2 * man;
         if (devAlloc(w, &d.learnX, rows * agx::MLP_INP) != hipSuccess || devAlloc(w, &d.learnY, rows * agx::MLP_OUT) != hipSuccess ||
+            devAlloc(w, &d.learnMap, rows) != hipSuccess ||
             devAlloc(w, &d.warmHist[0], man) != hipSuccess || devAlloc(w, &d.warmHist[1], man) != hipSuccess ||
             devAlloc(w, &d.histOf, man) != hipSuccess)
             return fail(w, AGX_ERR_HIP, "cannot allocate the learned warm-start buffers (%zu rows)", rows);

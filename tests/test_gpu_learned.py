@@ -191,3 +191,39 @@ def test_trained_network_needs_fewer_sweeps_than_cached_warm_start(agx, scenes):
     print(f"mean velocity sweeps to 1e-3: cached {sweeps_plain:.2f}, learned {sweeps_learned:.2f}")
     assert learned.stats().learned_warm_start == 1
     assert sweeps_learned < sweeps_plain
+
+
+def test_new_contacts_mode_rows_are_the_uncached_points(agx, scenes):
+    """In new-contacts mode a world collects one row per contact point without a cached impulse:
+    exactly the rows with feature 4 (has a cached impulse) equal to 0 that full collection gives on
+    an identical world; a network whose correction is +0.5 S then changes the step."""
+    dt = 1.0 / 60.0
+    full, new = _world(agx, scenes), _world(agx, scenes)
+    for w in (full, new):
+        w.step(dt, 12)      # still settling: new contacts every step
+    mf = agx.Mlp(seed=1, sample_capacity=1 << 16)
+    mn = agx.Mlp(seed=1, sample_capacity=1 << 16)
+    full.set_learned(mf, agx.LEARN_COLLECT)
+    new.set_learned(mn, agx.LEARN_COLLECT | agx.LEARN_NEW_CONTACTS)
+    full.step(dt, 1)
+    new.step(dt, 1)
+    np.testing.assert_array_equal(_state(full), _state(new))
+    xf, tf = mf.samples()
+    xn, tn = mn.samples()
+    assert 0 < len(xn) < len(xf)
+    assert (xn[:, 4] == 0).all() and (xn[:, 0] == 0).all() and (xn[:, 1] == 0).all()
+    sel = xf[:, 4] == 0
+    key = lambda x, t: np.lexsort(np.concatenate([x, t], 1).T[::-1])
+    a, b = np.concatenate([xf[sel], tf[sel]], 1), np.concatenate([xn, tn], 1)
+    np.testing.assert_array_equal(a[key(xf[sel], tf[sel])], b[key(xn, tn)])
+    # prediction in new-contacts mode: a constant +0.5 S normal correction reaches only new points
+    p = np.zeros(agx.MLP_PARAMS, np.float32)
+    p[-2] = 0.5
+    mp = agx.Mlp(seed=1, sample_capacity=1 << 16)
+    mp.set_params(p)
+    full.set_learned(None, agx.LEARN_OFF)
+    new.set_learned(mp, agx.LEARN_PREDICT | agx.LEARN_UNGATED | agx.LEARN_NEW_CONTACTS)
+    full.step(dt, 1)
+    new.step(dt, 1)
+    assert new.stats().learned_warm_start == 1
+    assert not np.array_equal(_state(full), _state(new))
