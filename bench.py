@@ -47,12 +47,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(segment, st, counts, settings, mean_poly_verts):
-    """Bytes a segment must move per launch (each array element counted once per pass)."""
+def algorithmic_bytes(segment, st, counts, settings, mean_poly_verts, sweeps):
+    """Bytes a segment must move per launch (each array element counted once per pass); `sweeps`
+    is the mean number of velocity sweeps of the launches timed."""
     n_dyn_bodies = counts.bodies - counts.static_bodies
     n_dyn_cols = counts.colliders - counts.static_colliders
     m, p = float(st.manifolds), float(st.pairs)
-    sweeps = max(st.velocity_sweeps, 0)
     key_bits = 30
     passes = math.ceil(key_bits / 8)
     table = {
@@ -137,30 +137,45 @@ def time_oracle(worlds_and_sizes, budget_s):
                 return bodies / (time.perf_counter() - t0), steps, time.perf_counter() - t0
 
 
+USE_GRAPHS = True   # --no-graphs: every step as ordinary launches (for kernel tracing)
+
+
+def advance(world, dt, steps):
+    """`steps` steps replayed from captured step graphs (the first step of a call runs as ordinary
+    launches), or all as ordinary launches without graphs."""
+    if USE_GRAPHS:
+        world.step_graph(dt, steps)
+    else:
+        world.step(dt, steps)
+
+
 def step_timed(world, steps, dt):
-    """Wall time of `steps` steps replayed from captured step graphs (the first step of the call
-    runs as plain launches)."""
+    """Wall time of `steps` steps."""
     world.sync()
     t0 = time.perf_counter()
-    world.step_graph(dt, steps)
+    advance(world, dt, steps)
     world.sync()
     return time.perf_counter() - t0
 
 
 def profiled(world, steps, dt):
+    """Per-segment GPU ms per launch (HIP events on the world's stream) over `steps` steps, and the
+    mean velocity sweeps of those steps."""
     world.set_profiling(True)
     world.profile()
+    s0 = world.stats().total_velocity_sweeps
     world.step(dt, steps)
     world.sync()
     prof = world.profile()
+    sweeps = (world.stats().total_velocity_sweeps - s0) / steps
     world.set_profiling(False)
-    return {k: v[0] / max(v[1], 1) for k, v in prof.items() if v[1]}
+    return {k: v[0] / max(v[1], 1) for k, v in prof.items() if v[1]}, sweeps
 
 
-def roofline(seg_ms, st, counts, settings, mean_v, workload):
+def roofline(seg_ms, st, counts, settings, mean_v, workload, sweeps):
     dominant = max(seg_ms, key=seg_ms.get)
     per_launch_ms = seg_ms[dominant]
-    alg = algorithmic_bytes(dominant, st, counts, settings, mean_v)
+    alg = algorithmic_bytes(dominant, st, counts, settings, mean_v, sweeps)
     achieved = alg / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant, workload),
@@ -251,20 +266,27 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
     t0 = time.perf_counter()
     for w in worlds:
         for k in range(0, args.warmup, 40):
-            w.step_graph(dt, min(40, args.warmup - k))
+            advance(w, dt, min(40, args.warmup - k))
         w.sync()
     log(f"batch: warm-up {args.warmup} steps to the residual target in {time.perf_counter() - t0:.1f} s")
     out = {}
     # launch shape of the dataflow kernels, measured on the plain copy and applied to both
-    n0 = plain_w.stats().solved_steps
-    tune = plain_w.autotune(dt, 2)
-    tuned_steps = plain_w.stats().solved_steps - n0
-    for w in worlds[1:]:
-        w.set_tuning(tune.solve_blocks_per_cu, tune.colour_blocks_per_cu)
-        w.step_graph(dt, tuned_steps)
-    out["tuning"] = {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu,
-                     "from_cache": int(tune.from_cache), "steps": int(tuned_steps)}
-    log(f"batch: tuned {tune.solve_blocks_per_cu}/{tune.colour_blocks_per_cu} blocks per CU over {tuned_steps} steps")
+    if args.tuning:
+        sb, cb = (int(v) for v in args.tuning.split(","))
+        for w in worlds:
+            w.set_tuning(sb, cb)
+        out["tuning"] = {"solve_blocks_per_cu": sb, "colour_blocks_per_cu": cb, "from_cache": 0, "steps": 0, "given": 1}
+    else:
+        n0 = plain_w.stats().solved_steps
+        tune = plain_w.autotune(dt, 2)
+        tuned_steps = plain_w.stats().solved_steps - n0
+        for w in worlds[1:]:
+            w.set_tuning(tune.solve_blocks_per_cu, tune.colour_blocks_per_cu)
+            w.step_graph(dt, tuned_steps)
+        out["tuning"] = {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu,
+                         "from_cache": int(tune.from_cache), "steps": int(tuned_steps)}
+    log(f"batch: {out['tuning']['solve_blocks_per_cu']}/{out['tuning']['colour_blocks_per_cu']} blocks per CU "
+        f"(sweeps/colouring), {out['tuning']['steps']} tuning steps")
     mode = agx.LEARN_PREDICT | agx.LEARN_NEW_CONTACTS
     if learned:
         lw = worlds[1]
@@ -312,14 +334,15 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
                           "network": net}
     else:
         out["learned"] = None
-    seg = profiled(plain_w, max(5, min(args.steps, 20)), dt)
+    seg, sweeps = profiled(plain_w, max(5, min(args.steps, 20)), dt)
     st = plain_w.stats()
     out["plain"]["segments_ms_per_step"] = {k: round(v, 4) for k, v in seg.items()}
-    out["roofline"] = roofline(seg, st, plain_w.counts, to_target, mean_poly_verts(scenes), "batch1024")
+    out["roofline"] = roofline(seg, st, plain_w.counts, to_target, mean_poly_verts(scenes), "batch1024", sweeps)
+    out["roofline"]["mean_velocity_sweeps"] = round(sweeps, 2)
     out["stats"] = {"manifolds": int(st.manifolds), "pairs": int(st.pairs), "colours": int(st.colours),
                     "islands": int(st.islands), "overflow": int(st.overflow)}
     if learned:
-        lseg = profiled(lw, max(5, min(args.steps, 20)), dt)
+        lseg, _ = profiled(lw, max(5, min(args.steps, 20)), dt)
         out["learned"]["segments_ms_per_step"] = {k: round(v, 4) for k, v in lseg.items()}
         for w in worlds[1:]:
             w.close()
@@ -377,17 +400,17 @@ def run_pile(agx, scenes_mod, args, dt, cpu):
     for k in range(0, 240, 40):
         world.step(dt, 40)
     world.sync()
-    tune = world.autotune(dt, 4)
+    tune = world.autotune(dt, 4) if USE_GRAPHS else world.tuning()
     steps = 100
     el = step_timed(world, steps, dt)
     st = world.stats()
-    seg = profiled(world, 50, dt)
+    seg, sweeps = profiled(world, 50, dt)
     st = world.stats()
     out = {"workload": f"pile_large_{n}_bodies", "value": round(n * steps / el, 1), "unit": "body-steps/s",
            "ms_per_step": round(1e3 * el / steps, 4), "velocity_iterations": 8, "position_iterations": 3,
            "segments_ms_per_step": {k: round(v, 4) for k, v in seg.items()},
            "tuning": {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu},
-           "roofline": roofline(seg, st, world.counts, settings, mean_poly_verts([scene]), "pile65536"),
+           "roofline": roofline(seg, st, world.counts, settings, mean_poly_verts([scene]), "pile65536", sweeps),
            "stats": {"pairs": int(st.pairs), "manifolds": int(st.manifolds), "colours": int(st.colours),
                      "islands": int(st.islands), "overflow": int(st.overflow),
                      "residual_per_sweep": [round(float(st.residual[k]), 6) for k in range(st.velocity_sweeps)]}}
@@ -413,14 +436,14 @@ def run_scatter(agx, scenes_mod, args, dt, cpu):
     steps = 20
     el = step_timed(world, steps, dt)
     st = world.stats()
-    seg = profiled(world, 10, dt)
+    seg, _ = profiled(world, 10, dt)
     st2 = world.stats()
     out = {"workload": f"scatter_{n}_dynamic_colliders_{world.counts.static_bodies}_static", "unit": "pairs/s",
            "value": round(st.pairs * steps / el, 1), "body_steps_per_s": round(n * steps / el, 1),
            "ms_per_step": round(1e3 * el / steps, 4), "pairs_per_step": int(st.pairs),
            "manifolds_per_step": int(st.manifolds), "ccd_hits_per_step": int(st2.ccd_hits),
            "segments_ms_per_step": {k: round(v, 4) for k, v in seg.items()},
-           "roofline": roofline(seg, st2, world.counts, settings, mean_poly_verts([scene]), "scatter1m"),
+           "roofline": roofline(seg, st2, world.counts, settings, mean_poly_verts([scene]), "scatter1m", 0),
            "overflow": int(st2.overflow)}
     log(f"scatter: {out['ms_per_step']} ms/step in {time.perf_counter() - t0:.1f} s")
     if cpu:
@@ -449,6 +472,9 @@ def main():
                     help="every n-th scene of the batch is stepped in a world of its own (plain, unbatched); 0: none")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="plain warm start only")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="step with ordinary launches instead of captured step graphs (for kernel tracing)")
+    ap.add_argument("--tuning", default="", help="blocks per CU 'sweeps,colouring' for the batch instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device copy three times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -459,6 +485,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); how the path would shard is "
               f"described in DESIGN.md and is not run.", file=sys.stderr)
         sys.exit(2)
+    global USE_GRAPHS
+    USE_GRAPHS = not args.no_graphs
+    if args.no_graphs and not args.tuning:
+        args.tuning = "1,2"
     agx = load_package()
     import agx.scenes  # noqa: F401
     scenes_mod = sys.modules["agx.scenes"]

@@ -138,55 +138,81 @@ __device__ inline uint32_t claimRows(uint32_t count, uint32_t* counter) {
     return __shfl(row0, WAVE - 1, WAVE) + prefix - count;
 }
 
+// Block-wide claim of rows on `counter`: the first of this thread's `count` rows. Every thread of
+// the (256-thread) block must call it; one atomic per block keeps the counter uncontended.
+__device__ inline uint32_t claimRowsBlock(uint32_t count, uint32_t* counter) {
+    __shared__ uint32_t waveTotal[4];
+    __shared__ uint32_t blockBase;
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    uint32_t prefix = count;   // inclusive prefix over the wave
+    for (uint32_t off = 1; off < WAVE; off <<= 1) {
+        const uint32_t v = __shfl_up(prefix, off, WAVE);
+        if (lane >= off) prefix += v;
+    }
+    if (lane == WAVE - 1) waveTotal[wid] = prefix;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = waveTotal[0] + waveTotal[1] + waveTotal[2] + waveTotal[3];
+        blockBase = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = blockBase;
+    for (uint32_t k = 0; k < wid; k++) base += waveTotal[k];
+    __syncthreads();   // waveTotal and blockBase are reused by the next call
+    return base + prefix - count;
+}
+
 // Records every manifold's starting accumulators (the next step's history) and, when the network
 // runs or rows are collected, the feature rows: two per manifold at rows 2m and 2m + 1, or with
 // newOnly one per contact point without a cached impulse, in claimed rows with map[row] = 2m + p.
+// Runs over solve positions, so the constraint planes are read in order (holes are skipped).
 __global__ __launch_bounds__(256) void kLearnFeatures(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
-                                                      const uint32_t* __restrict__ posOf, const uint32_t* __restrict__ degree,
-                                                      const float4* __restrict__ vel, Constraints cs, Scale sc,
-                                                      const float4* __restrict__ histOf, float4* __restrict__ warmHist,
-                                                      uint32_t newOnly, uint16_t* __restrict__ x, uint32_t* __restrict__ map) {
-    const uint32_t M = ctr->manifolds;
+                                                      const uint32_t* __restrict__ degree, const float4* __restrict__ vel,
+                                                      Constraints cs, Scale sc, const float4* __restrict__ histOf,
+                                                      float4* __restrict__ warmHist, uint32_t newOnly,
+                                                      uint16_t* __restrict__ x, uint32_t* __restrict__ map) {
+    const uint32_t P = ctr->colourStart[MAX_COLOURS];
     const bool rows = ctr->learnRows != 0;
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
-    // wave-uniform rounds, so that the row claims of newOnly stay wave-wide
-    for (uint32_t base = first - lane; base < M; base += stride) {
-        const uint32_t m = base + lane;
-        uint32_t slot = 0;
+    // block-uniform rounds, so that the row claims of newOnly stay block-wide
+    for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
+        const uint32_t c = base + threadIdx.x;
+        uint4 id = make_uint4(0u, 0u, 0u, 0u);
         float4 j = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (m < M) {
-            slot = posOf[m];
-            j = cs.imp[slot];
-            warmHist[m] = j;
+        if (c < P) {
+            id = cs.idx[c];
+            if (id.z != 0) {
+                j = cs.imp[c];
+                warmHist[id.w] = j;
+            }
         }
         if (!rows) continue;
+        const uint32_t m = id.w;
         float f[MLP_INP];
         if (!newOnly) {
-            if (m >= M) continue;
+            if (id.z == 0) continue;
             const float depth = man[m].depth;
             const float4 hist = histOf[m];
             for (uint32_t p = 0; p < 2; p++) {
-                pointFeatures(cs, slot, p, vel, degree, depth, hist, sc, f);
+                pointFeatures(cs, c, p, vel, degree, depth, hist, sc, f);
                 storeFeatureRow(x, 2ull * m + p, f);
             }
             continue;
         }
         uint32_t keep = 0;
-        if (m < M) {
-            const uint32_t count = min(cs.idx[slot].z, 2u);
-            const float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w };
+        if (id.z != 0) {
+            const uint32_t count = min(id.z, 2u);
+            const float4 k0 = cs.k0[c];
+            const float jn[2] = { j.x, j.z }, jt[2] = { j.y, j.w }, nm[2] = { k0.x, k0.w };
             for (uint32_t p = 0; p < count; p++)
-                if (jn[p] == 0.0f && jt[p] == 0.0f && normalMassOf(cs, slot, p) > 0.0f) keep |= 1u << p;
+                if (jn[p] == 0.0f && jt[p] == 0.0f && nm[p] > 0.0f) keep |= 1u << p;
         }
-        uint32_t row = claimRows(__builtin_popcount(keep), &ctr->learnCompactRows);
+        uint32_t row = claimRowsBlock(__builtin_popcount(keep), &ctr->learnCompactRows);
         if (!keep) continue;
         const float depth = man[m].depth;
         const float4 hist = histOf[m];
         for (uint32_t p = 0; p < 2; p++) {
             if (!(keep >> p & 1u)) continue;
-            pointFeatures(cs, slot, p, vel, degree, depth, hist, sc, f);
+            pointFeatures(cs, c, p, vel, degree, depth, hist, sc, f);
             storeFeatureRow(x, row, f);
             map[row] = 2u * m + p;
             row++;
@@ -331,7 +357,7 @@ hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice
     hipLaunchKernelGGL(kLearnRows, dim3(1), dim3(1), 0, st, w.ctr, predict && mlp ? 1u : 0u, gated, collect);
     const Scale sc = scaleOf(gx, gy, dt);
     const uint32_t grid = gridOf(std::min<uint64_t>(w.manCap, 1u << 22));
-    hipLaunchKernelGGL(kLearnFeatures, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.posOf, w.degree, w.vel, w.cs, sc,
+    hipLaunchKernelGGL(kLearnFeatures, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.degree, w.vel, w.cs, sc,
                        w.histOf, w.warmHist[w.cur], newOnly, w.learnX, w.learnMap);
     if (newOnly) hipLaunchKernelGGL(kLearnCompactRows, dim3(1), dim3(1), 0, st, w.ctr);
     if (predict && mlp) {
