@@ -464,7 +464,7 @@ def run_scatter(agx, scenes_mod, args, dt, cpu):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed steps of the headline batch")
+    ap.add_argument("--steps", type=int, default=60, help="timed steps of the headline batch")
     ap.add_argument("--warmup", type=int, default=240, help="settling steps of the headline batch")
     ap.add_argument("--workload", default="all", choices=["all", "batch1024", "pile65536", "scatter1m"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -474,7 +474,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="plain warm start only")
     ap.add_argument("--no-graphs", action="store_true",
                     help="step with ordinary launches instead of captured step graphs (for kernel tracing)")
-    ap.add_argument("--tuning", default="", help="blocks per CU 'sweeps,colouring' for the batch instead of autotuning")
+    ap.add_argument("--tuning", default="",
+                    help="'sweep blocks per CU,colouring blocks per CU' for the batch instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device copy three times (PMC byte-counter calibration)")
     args = ap.parse_args()

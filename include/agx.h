@@ -211,7 +211,8 @@ const char* agx_profile_segment_name(uint32_t segment);
  * CollisionDetection/Narrow/NarrowPhaseCollisionDetector.h:153). Here the co-resident blocks per CU
  * of the dataflow colouring and sweep kernels are measured instead: agx_world_autotune times
  * graph-replayed steps for each shape (1, 2, 4 ... up to the maximum; two interleaved rounds, which
- * advances the simulation) and keeps the fastest. Results are cached in the process per GPU, world
+ * advances the simulation) and keeps the fastest, per sweep run when a residual target makes the
+ * sweep count vary. Every shape steps to the same bits. Results are cached in the process per GPU, world
  * size class and solver settings, and in the file named by the environment variable AGX_TUNE_CACHE
  * when it is set; a cached shape is applied without stepping (from_cache = 1). */
 typedef struct agx_tuning {

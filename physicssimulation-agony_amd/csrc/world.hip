@@ -880,7 +880,10 @@ int agx_world_autotune(agx_world* w, float dt, uint32_t steps_per_trial, agx_tun
             t.colour_blocks_per_cu = cb;
             cand.push_back(t);
         }
-    std::vector<double> ms(cand.size(), 0.0);
+    // With a residual target the sweep count varies from step to step, so a trial's time is
+    // divided by the sweeps it ran (velocity and position) to compare launch shapes fairly.
+    const bool perSweep = w->settings.residual_target > 0.0f && !w->settings.collide_only;
+    std::vector<double> ms(cand.size(), 0.0), work(cand.size(), 0.0);
     hipEvent_t a = nullptr, b = nullptr;
     HIP_CHECK(w, hipEventCreate(&a));
     if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return fail(w, AGX_ERR_HIP, "event"); }
@@ -888,6 +891,8 @@ int agx_world_autotune(agx_world* w, float dt, uint32_t steps_per_trial, agx_tun
     for (int round = 0; round < 2 && rc == AGX_OK; round++)
         for (size_t c = 0; c < cand.size() && rc == AGX_OK; c++) {
             if ((rc = agx_world_set_tuning(w, &cand[c]))) break;
+            agx_stats s0{};
+            if ((rc = agx_world_read_stats(w, &s0))) break;
             if (hipEventRecord(a, w->stream) != hipSuccess) { rc = fail(w, AGX_ERR_HIP, "event"); break; }
             if ((rc = agx_world_step_graph(w, dt, steps_per_trial))) break;
             float e = 0.0f;
@@ -896,12 +901,16 @@ int agx_world_autotune(agx_world* w, float dt, uint32_t steps_per_trial, agx_tun
             agx_stats st{};
             if ((rc = agx_world_read_stats(w, &st))) break;
             ms[c] += st.overflow ? 1e30 : (double)e;   // a shape that overflowed a capacity is never chosen
+            work[c] += perSweep ? (double)(st.total_velocity_sweeps - s0.total_velocity_sweeps) +
+                                      (double)steps_per_trial * w->settings.position_iterations
+                                : (double)steps_per_trial;
         }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc != AGX_OK) return rc;
     size_t best = 0;
-    for (size_t c = 1; c < cand.size(); c++) if (ms[c] < ms[best]) best = c;
+    auto cost = [&](size_t c) { return ms[c] / std::max(work[c], 1.0); };
+    for (size_t c = 1; c < cand.size(); c++) if (cost(c) < cost(best)) best = c;
     agx_tuning t = cand[best];
     t.ms_per_step = (float)(ms[best] / (2.0 * steps_per_trial));
     if (int r = agx_world_set_tuning(w, &t)) return r;
