@@ -74,6 +74,13 @@ typedef struct agx_manifold {
 } agx_manifold;
 
 /* Per-step counters of the last step. */
+/* Bits of agx_stats.overflow. */
+#define AGX_OVERFLOW_PAIRS 1u      /* pair buffer too small */
+#define AGX_OVERFLOW_STACK 2u      /* BVH traversal stack too small */
+#define AGX_OVERFLOW_COLOURING 4u  /* a constraint found no free first-fit colour (the solve is still exact) */
+#define AGX_OVERFLOW_HASH 8u       /* warm-start hash table full */
+#define AGX_OVERFLOW_STALL 16u     /* a dataflow kernel's wait hit its 2 s limit */
+#define AGX_OVERFLOW_ITEMS 32u     /* more constraints than the dataflow kernels hold */
 typedef struct agx_stats {
     uint64_t pairs;                /* broad-phase pairs */
     uint64_t manifolds;            /* narrow-phase manifolds */
@@ -82,7 +89,7 @@ typedef struct agx_stats {
     uint32_t islands;              /* islands of dynamic bodies */
     uint32_t ccd_hits;             /* bodies moved back by CCD */
     uint32_t velocity_sweeps;      /* velocity sweeps actually run (adaptive mode may stop early) */
-    uint32_t overflow;             /* nonzero: a capacity was exceeded, results are truncated */
+    uint32_t overflow;             /* AGX_OVERFLOW_* bits; nonzero except COLOURING: results are not valid */
     uint32_t learned_warm_start;   /* 1: the step's warm start came from the network */
     float velocity_error;          /* computeConstraintErrors().x */
     float position_error;          /* computeConstraintErrors().y */
