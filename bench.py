@@ -158,16 +158,26 @@ def step_timed(world, steps, dt):
     return time.perf_counter() - t0
 
 
+def dynamic_bodies(world):
+    return max(world.counts.bodies - world.counts.static_bodies, 1)
+
+
+def mean_sweeps(world, stats0, steps):
+    """Velocity sweeps per dynamic body and step since `stats0` (scenes solved apart may stop at
+    different sweeps; each body counts its scene's)."""
+    return (world.stats().total_body_sweeps - stats0.total_body_sweeps) / (steps * dynamic_bodies(world))
+
+
 def profiled(world, steps, dt):
     """Per-segment GPU ms per launch (HIP events on the world's stream) over `steps` steps, and the
-    mean velocity sweeps of those steps."""
+    mean velocity sweeps per dynamic body of those steps."""
     world.set_profiling(True)
     world.profile()
-    s0 = world.stats().total_velocity_sweeps
+    s0 = world.stats().total_body_sweeps
     world.step(dt, steps)
     world.sync()
     prof = world.profile()
-    sweeps = (world.stats().total_velocity_sweeps - s0) / steps
+    sweeps = (world.stats().total_body_sweeps - s0) / (steps * dynamic_bodies(world))
     world.set_profiling(False)
     return {k: v[0] / max(v[1], 1) for k, v in prof.items() if v[1]}, sweeps
 
@@ -225,26 +235,27 @@ def unbatched_sample(agx, world, scenes, settings, dt, stride, steps):
     """The plain method without batching: every stride-th scene of the batch in a world of its own,
     started from the batch world's current state and stepped `steps` times with ordinary launches
     (Simulation::update per scene, one scene after the other). Returns the sample's bodies, summed
-    ms per step and body-weighted mean sweeps per step."""
+    ms per step and velocity sweeps per dynamic body and step."""
     bstart, cstart = scene_ranges(world, len(scenes))
     prev = world.manifolds(1)
-    bodies = 0
+    bodies = dyn = 0
     ms = 0.0
     sweeps = 0.0
     for i in range(0, len(scenes), stride):
         w = world_for_scene(agx, scenes[i], world, slice(int(bstart[i]), int(bstart[i + 1])),
                             slice(int(cstart[i]), int(cstart[i + 1])), settings, prev)
         n = w.counts.bodies
-        s0 = w.stats().total_velocity_sweeps
+        s0 = w.stats().total_body_sweeps
         w.sync()
         t0 = time.perf_counter()
         w.step(dt, steps)
         w.sync()
         ms += 1e3 * (time.perf_counter() - t0) / steps
-        sweeps += n * (w.stats().total_velocity_sweeps - s0) / steps
+        sweeps += (w.stats().total_body_sweeps - s0) / steps
         bodies += n
+        dyn += dynamic_bodies(w)
         w.close()
-    return bodies, ms, sweeps / max(bodies, 1)
+    return bodies, ms, sweeps / max(dyn, 1)
 
 
 def run_batch(agx, scenes_mod, args, dt, learned=True):
@@ -316,7 +327,7 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
     st0 = plain_w.stats()
     el = step_timed(plain_w, args.steps, dt)
     st = plain_w.stats()
-    mean_plain = (st.total_velocity_sweeps - st0.total_velocity_sweeps) / args.steps
+    mean_plain = mean_sweeps(plain_w, st0, args.steps)
     out["plain"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
                     "mean_velocity_sweeps": round(mean_plain, 2), "velocity_sweeps_last_step": int(st.velocity_sweeps),
                     "residual_per_sweep_last_step": [round(float(st.residual[k]), 6) for k in range(st.velocity_sweeps)]}
@@ -325,7 +336,7 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
         el = step_timed(lw, args.steps, dt)
         st = lw.stats()
         g = lw.gate()
-        mean_learned = (st.total_velocity_sweeps - st0.total_velocity_sweeps) / args.steps
+        mean_learned = mean_sweeps(lw, st0, args.steps)
         out["learned"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
                           "mean_velocity_sweeps": round(mean_learned, 2),
                           "sweep_reduction": round(1.0 - mean_learned / max(mean_plain, 1e-9), 4),

@@ -58,6 +58,11 @@ typedef struct agx_settings {
     uint32_t track_residual;               /* relative impulse change after every velocity sweep */
     float residual_target;                 /* > 0: stop velocity sweeps once residual <= target */
     uint32_t collide_only;                 /* 1: integrate + collide, skip the solver (broad/narrow timing) */
+    uint32_t solve_per_scene;              /* 0: one dataflow over all scenes (one residual for the batch);
+                                              1 (default): each scene solved by one workgroup with its bodies in
+                                              LDS, stopping at its own residual, when the world has several
+                                              scenes and every scene fits; 2: also for a single scene.
+                                              Fixed sweep counts give the same bits either way. */
 } agx_settings;
 
 /* One manifold record (64 bytes): NarrowPhaseCollisionDetector.h:26-61 with the collider
@@ -73,7 +78,6 @@ typedef struct agx_manifold {
     float jn0, jt0, jn1, jt1;
 } agx_manifold;
 
-/* Per-step counters of the last step. */
 /* Bits of agx_stats.overflow. */
 #define AGX_OVERFLOW_PAIRS 1u      /* pair buffer too small */
 #define AGX_OVERFLOW_STACK 2u      /* BVH traversal stack too small */
@@ -81,6 +85,8 @@ typedef struct agx_manifold {
 #define AGX_OVERFLOW_HASH 8u       /* warm-start hash table full */
 #define AGX_OVERFLOW_STALL 16u     /* a dataflow kernel's wait hit its 2 s limit */
 #define AGX_OVERFLOW_ITEMS 32u     /* more constraints than the dataflow kernels hold */
+
+/* Per-step counters of the last step. */
 typedef struct agx_stats {
     uint64_t pairs;                /* broad-phase pairs */
     uint64_t manifolds;            /* narrow-phase manifolds */
@@ -95,7 +101,10 @@ typedef struct agx_stats {
     float position_error;          /* computeConstraintErrors().y */
     float residual[64];            /* relative impulse change after velocity sweep k */
     uint64_t solved_steps;         /* steps solved since the world was created */
-    uint64_t total_velocity_sweeps; /* velocity sweeps over those steps */
+    uint64_t total_velocity_sweeps; /* velocity sweeps over those steps (per step: the most any scene ran) */
+    uint64_t total_body_sweeps;    /* over those steps: dynamic bodies x the velocity sweeps of their scene */
+    float mean_velocity_sweeps;    /* last step: velocity sweeps per dynamic body (scenes may stop apart) */
+    uint32_t solved_per_scene;     /* 1: the last step was solved scene by scene */
 } agx_stats;
 
 /* World sizes. */
@@ -205,6 +214,9 @@ int agx_world_read_solve_order(agx_world* w, uint32_t* order, uint64_t cap, uint
                                uint32_t colour_cap, uint32_t* colours);
 int agx_world_read_islands(agx_world* w, uint32_t* island_id, uint32_t n, uint32_t* island_count);
 int agx_world_read_stats(agx_world* w, agx_stats* out);
+/* Velocity sweeps each scene ran in the last step solved scene by scene (stats.solved_per_scene);
+ * n must be the scene count. No counterpart in the reference (one simulation per scene there). */
+int agx_world_read_scene_sweeps(agx_world* w, uint32_t* out, uint32_t n);
 
 /* GPU time per step segment, measured with HIP events on the world's stream while profiling
  * is on. Segments: see agx_profile_segment_name. agx_world_read_profile waits for the stream,

@@ -54,7 +54,7 @@ class Settings(C.Structure):
                 ("ccd_enabled", C.c_uint32), ("ccd_fast_factor", C.c_float), ("ccd_core_fraction", C.c_float),
                 ("update_interval", C.c_float), ("max_delta_per_update", C.c_float), ("time_scale", C.c_float),
                 ("track_errors", C.c_uint32), ("track_residual", C.c_uint32), ("residual_target", C.c_float),
-                ("collide_only", C.c_uint32)]
+                ("collide_only", C.c_uint32), ("solve_per_scene", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -62,7 +62,9 @@ class Stats(C.Structure):
                 ("colours", C.c_uint32), ("islands", C.c_uint32), ("ccd_hits", C.c_uint32),
                 ("velocity_sweeps", C.c_uint32), ("overflow", C.c_uint32), ("learned_warm_start", C.c_uint32),
                 ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * 64),
-                ("solved_steps", C.c_uint64), ("total_velocity_sweeps", C.c_uint64)]
+                ("solved_steps", C.c_uint64), ("total_velocity_sweeps", C.c_uint64),
+                ("total_body_sweeps", C.c_uint64), ("mean_velocity_sweeps", C.c_float),
+                ("solved_per_scene", C.c_uint32)]
 
 
 class Gate(C.Structure):
@@ -126,6 +128,7 @@ _SIGS = {
     "agx_world_read_solve_order": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_islands": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "agx_world_read_scene_sweeps": (C.c_int, [_P, _P, C.c_uint32]),
     "agx_broadphase_pairs": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "agx_world_read_profile": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -385,6 +388,12 @@ class World:
         s = Stats()
         _check(lib().agx_world_read_stats(self._h, C.byref(s)), self._h)
         return s
+
+    def scene_sweeps(self) -> np.ndarray:
+        """Velocity sweeps each scene ran in the last step solved scene by scene."""
+        out = np.empty(self.counts.scenes, np.uint32)
+        _check(lib().agx_world_read_scene_sweeps(self._h, _ptr(out), len(out)), self._h)
+        return out
 
     # -- learned warm start
     def set_learned(self, mlp: "Mlp | None", mode: int):

@@ -69,7 +69,8 @@ struct Counters {
     uint32_t hashCapacity;     // power of two >= 2 * prevManifolds (>= 16)
     LearnGate gate;
     unsigned long long solveSteps;   // steps solved since the world was created
-    unsigned long long sweepsTotal;  // velocity sweeps run over those steps
+    unsigned long long sweepsTotal;  // velocity sweeps run over those steps (per step: the most any scene ran)
+    unsigned long long bodySweepsTotal;   // sum over those steps of dynamic bodies x the sweeps of their scene
     // per step
     uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
     uint32_t manifolds;
@@ -91,6 +92,9 @@ struct Counters {
     uint32_t learnUsed;        // this step's warm start came from the network
     uint32_t sampleBase;       // first training row this step claimed
     uint32_t learnCompactRows; // new-contacts mode: rows written (one per contact point without a cached impulse)
+    uint32_t sceneCursor;      // per-scene solve: scenes taken by its blocks
+    uint32_t perScene;         // this step was solved scene by scene
+    unsigned long long bodySweeps;   // this step: dynamic bodies x the sweeps of their scene
 };
 
 constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)
@@ -227,6 +231,15 @@ struct DeviceWorld {
     uint32_t* order = nullptr;       // solve position -> manifold (UINT32_MAX in padding holes)
     uint64_t solveCap = 0;           // solve positions: manCap + COLOUR_ALIGN * MAX_COLOURS
     float* blockRes = nullptr;       // per solve block: residual partial sums (2 x MAX_SWEEPS)
+    // per-scene solve (scenes whose bodies fit one block's LDS)
+    uint32_t* sceneBodyStart = nullptr;   // S + 1: first body of each scene (scenes are concatenated)
+    uint32_t* sceneOrder = nullptr;       // scenes by decreasing body count (the blocks' work order)
+    uint32_t* sceneDyn = nullptr;         // dynamic bodies per scene
+    uint2* sceneRange = nullptr;          // S x MAX_COLOURS: first and one past the last slot of the scene in a colour
+    float2* sceneRes = nullptr;           // S x MAX_SWEEPS: the scene's squared accumulator change and accumulators
+    uint32_t* sceneSweeps = nullptr;      // velocity sweeps each scene ran this step
+    uint32_t sceneLdsBytes = 0;           // dynamic LDS of the per-scene solve (0: some scene does not fit)
+    int sceneBlocks = 0;                  // resident blocks of the per-scene solve
     float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
     Constraints cs{};
     // scan scratch
@@ -384,9 +397,15 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
 hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // launchSolve: the velocity and position sweeps (one kernel); launchSolveFinish: accumulators
 // back to the manifolds, residual and error totals.
+// Per-scene solve: LDS bytes for scenes of up to maxBodies bodies (0 when they do not fit) and the
+// resident blocks for that size.
+uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks);
+hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
+                            float residualTarget);
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                        float residualTarget);
-hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors);
+hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors,
+                             int perScene);
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
 // Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels: the default

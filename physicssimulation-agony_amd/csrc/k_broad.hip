@@ -343,17 +343,21 @@ __device__ inline bool childActive(const Query& q, const float4& box, uint32_t f
     return q.box.x < box.z && q.box.z > box.x && q.box.y < box.w && q.box.w > box.y;
 }
 
+// Query wave w writes its pairs with the query tree itself (part 0) and with the other tree
+// (part 1) to region 2w + part, so the pair list runs wave by wave in query (scene-major Morton)
+// order: the pairs, and the manifolds compacted from them, are grouped by scene except inside a
+// wave whose queries straddle two scenes.
 template <bool SELF, bool STAGED>
 __global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_t* __restrict__ colScene,
                                                  uint32_t* __restrict__ waveCount, const uint32_t* __restrict__ waveOffset,
-                                                 uint32_t waveBaseIndex, uint2* __restrict__ pairs, uint64_t cap,
+                                                 uint32_t part, uint2* __restrict__ pairs, uint64_t cap,
                                                  uint2* __restrict__ staged, Counters* ctr) {
     __shared__ uint32_t stackMem[4][TRAVERSAL_STACK];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     uint32_t* stack = stackMem[wid];
     const uint32_t waveGlobal = blockIdx.x * 4 + wid;
     if (waveGlobal * 64 >= qt.n) return;   // whole wave past the last query (wave-uniform exit)
-    const uint32_t region = waveBaseIndex + waveGlobal;
+    const uint32_t region = 2 * waveGlobal + part;
     if (!STAGED && waveCount[region] <= STAGE_PAIRS) return;   // already gathered (wave-uniform)
     const uint32_t i = waveGlobal * 64 + lane;
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -480,7 +484,7 @@ hipError_t launchPairs(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL((kTraverse<true, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
                        w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     hipLaunchKernelGGL((kTraverse<false, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
-                       w.waveOffset, waves, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+                       w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     uint32_t* total = &w.ctr->scratch[0];
     hipError_t e = launchScan(st, w.waveCount, w.waveOffset, nullptr, total, 2ull * waves, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
@@ -489,7 +493,7 @@ hipError_t launchPairs(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL((kTraverse<true, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
                        w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     hipLaunchKernelGGL((kTraverse<false, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
-                       w.waveOffset, waves, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+                       w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     hipLaunchKernelGGL(kPairTotal, dim3(1), dim3(1), 0, st, w.ctr, w.pairCap, total);
     return hipGetLastError();
 }

@@ -322,18 +322,26 @@ __global__ __launch_bounds__(256) void kLearnCollect(Counters* __restrict__ ctr,
 }
 
 // The step's cost for the gate: log10 of the relative residual after velocity sweep GATE_SWEEP (or
-// after the last sweep when fewer ran). A better warm start leaves less for the early sweeps to
-// change; the early residual is far less noisy than the number of sweeps a step needs, which a
-// collapsing stack somewhere in the batch can drive to the cap whatever the warm start.
+// after the last sweep when fewer ran); a better warm start leaves less for the early sweeps to
+// change, and the early residual is far less noisy than the number of sweeps a step needs, which a
+// collapsing stack somewhere in a batch solved as one can drive to the cap whatever the warm start.
+// A batch solved scene by scene to a residual target is judged by what the warm start is for: the
+// mean over dynamic bodies of the sweeps their scene needed, which one collapsing scene moves little.
 constexpr uint32_t GATE_SWEEP = 8;
-__global__ void kGateUpdate(Counters* ctr) {
+__global__ void kGateUpdate(Counters* ctr, uint32_t nDynBodies, uint32_t toTarget) {
     const uint32_t sweeps = min(ctr->sweeps, MAX_SWEEPS);
     if (sweeps == 0) return;
-    const uint32_t s = min(sweeps, GATE_SWEEP) - 1;
-    const double num = ctr->residualNum[s], den = ctr->residualDen[s];
-    const float res = (float)(sqrt(num) / fmax(sqrt(den), 1e-12));
+    float cost;
+    if (ctr->perScene && toTarget) {
+        cost = (float)((double)ctr->bodySweeps / (double)max(nDynBodies, 1u));
+    } else {
+        const uint32_t s = min(sweeps, GATE_SWEEP) - 1;
+        const double num = ctr->residualNum[s], den = ctr->residualDen[s];
+        const float res = (float)(sqrt(num) / fmax(sqrt(den), 1e-12));
+        cost = log10f(fmaxf(res, 1e-12f));
+    }
     LearnGate g = ctr->gate;
-    gateUpdate(g, ctr->learnUsed, log10f(fmaxf(res, 1e-12f)));
+    gateUpdate(g, ctr->learnUsed, cost);
     ctr->gate = g;
 }
 
@@ -381,8 +389,8 @@ hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& 
     return hipGetLastError();
 }
 
-hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w) {
-    hipLaunchKernelGGL(kGateUpdate, dim3(1), dim3(1), 0, st, w.ctr);
+hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w, uint32_t toTarget) {
+    hipLaunchKernelGGL(kGateUpdate, dim3(1), dim3(1), 0, st, w.ctr, w.nDynBodies, toTarget);
     return hipGetLastError();
 }
 

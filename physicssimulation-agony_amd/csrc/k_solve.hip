@@ -273,19 +273,48 @@ __global__ __launch_bounds__(256) void kMarkHoles(const Counters* __restrict__ c
     cs.dep[p] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// Stable: a colour's slots follow manifold order (blocks own contiguous chunks in order, a block
+// takes its chunk in rounds of 256, waves in order, lanes in order), so the manifolds of one scene
+// occupy a contiguous run of each colour's slots up to the few manifolds of a pair wave that
+// straddles two scenes.
 __global__ __launch_bounds__(256) void kColourScatter(const Counters* __restrict__ ctr, const uint16_t* __restrict__ colourId,
                                                       const uint32_t* __restrict__ hist, uint32_t* __restrict__ order,
                                                       uint32_t* __restrict__ posOf) {
+    constexpr uint32_t COLOUR_BITS = 8;
+    static_assert(MAX_COLOURS <= (1u << COLOUR_BITS), "colour ids fit the matched bits");
     __shared__ uint32_t cursor[MAX_COLOURS];
+    __shared__ uint32_t waveCount[4][MAX_COLOURS];
     for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x)
         cursor[i] = ctr->colourStart[i] + hist[blockIdx.x * MAX_COLOURS + i];
-    __syncthreads();
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const uint64_t ltMask = lane == 0 ? 0ull : (~0ull >> (WAVE - lane));
     uint32_t lo, hi;
     chunkOf(ctr->manifolds, lo, hi);
-    for (uint32_t m = lo + threadIdx.x; m < hi; m += blockDim.x) {
-        const uint32_t p = atomicAdd(&cursor[colourId[m]], 1u);
-        order[p] = m;
-        posOf[m] = p;
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {   // block-uniform rounds
+        for (uint32_t i = threadIdx.x; i < 4 * MAX_COLOURS; i += blockDim.x) (&waveCount[0][0])[i] = 0;
+        __syncthreads();
+        const uint32_t m = base + threadIdx.x;
+        const bool valid = m < hi;
+        const uint32_t c = valid ? colourId[m] : 0u;
+        // lanes of this wave with the same colour
+        uint64_t same = __ballot(valid);
+        for (uint32_t b = 0; b < COLOUR_BITS; b++) {
+            const uint64_t ones = __ballot(valid && ((c >> b) & 1u));
+            same &= ((c >> b) & 1u) ? ones : ~ones;
+        }
+        const uint32_t rank = (uint32_t)__popcll(same & ltMask);
+        if (valid && (same >> lane) == 1ull) waveCount[wid][c] = (uint32_t)__popcll(same);   // the group's last lane
+        __syncthreads();
+        if (valid) {
+            uint32_t p = cursor[c] + rank;
+            for (uint32_t k = 0; k < wid; k++) p += waveCount[k][c];
+            order[p] = m;
+            posOf[m] = p;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < MAX_COLOURS; i += blockDim.x)
+            cursor[i] += waveCount[0][i] + waveCount[1][i] + waveCount[2][i] + waveCount[3][i];
+        __syncthreads();   // before the next round clears waveCount
     }
 }
 
@@ -725,6 +754,199 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
     }
 }
 
+// ---------------------------------------------------------------- per-scene solve
+// Scenes of a batch share no body, so each can be solved by one workgroup on its own, with the
+// scene's body records in LDS: the same sequential colour-major order as the dataflow (within a
+// colour the constraints share no dynamic body, so the block's lanes run them at once, and a block
+// barrier separates colours), hence the same bits, without device-wide flags or barriers. With a
+// residual target each scene stops at its own sweep, as a simulation of that scene alone would.
+
+__global__ __launch_bounds__(256) void kSceneRangeClear(uint2* __restrict__ range, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        range[i] = make_uint2(0xFFFFFFFFu, 0u);
+}
+
+// range[scene * MAX_COLOURS + colour] = (first slot, one past the last slot) of the scene's
+// constraints in that colour. The colour scatter keeps manifold order and manifolds come grouped
+// by scene, so a range holds few slots of other scenes (the solve skips them by body index).
+__global__ __launch_bounds__(256) void kSceneRanges(const Counters* __restrict__ ctr, Constraints cs,
+                                                    const uint32_t* __restrict__ bodyScene, uint2* __restrict__ range) {
+    __shared__ uint32_t starts[MAX_COLOURS + 1];
+    for (uint32_t i = threadIdx.x; i <= MAX_COLOURS; i += blockDim.x) starts[i] = ctr->colourStart[i];
+    __syncthreads();
+    const uint32_t P = starts[MAX_COLOURS];
+    const uint32_t K = ctr->colours;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
+        const uint32_t c = base + threadIdx.x;
+        uint32_t key = 0xFFFFFFFFu;
+        if (c < P) {
+            const uint4 id = cs.idx[c];
+            if (id.z != 0) {
+                uint32_t lo = 0, hi = K;   // starts[lo] <= c < starts[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (starts[mid] <= c) lo = mid; else hi = mid;
+                }
+                key = bodyScene[id.x] * MAX_COLOURS + lo;
+            }
+        }
+        // one update per wave when the wave's slots share a key (the common case)
+        const uint32_t first = __shfl(key, 0, WAVE);
+        const bool uniform = __all(key == first);
+        if (uniform) {
+            if (first == 0xFFFFFFFFu) continue;
+            uint32_t mn = c, mx = c + 1;
+            for (uint32_t off = WAVE / 2; off > 0; off >>= 1) {
+                mn = min(mn, __shfl_xor(mn, off, WAVE));
+                mx = max(mx, __shfl_xor(mx, off, WAVE));
+            }
+            if (lane == 0) {
+                atomicMin(&range[first].x, mn);
+                atomicMax(&range[first].y, mx);
+            }
+        } else if (key != 0xFFFFFFFFu) {
+            atomicMin(&range[key].x, c);
+            atomicMax(&range[key].y, c + 1);
+        }
+    }
+}
+
+constexpr uint32_t SCENE_THREADS = 512;
+
+__device__ inline double sceneBlockSum(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = SCENE_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double t = red[0];
+    __syncthreads();
+    return t;
+}
+
+// Blocks take scenes largest first from a shared cursor until none is left; a scene's results do
+// not depend on which block solves it.
+__global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restrict__ ctr, Constraints cs, float4* __restrict__ vel,
+                                                             float4* __restrict__ pos, const uint32_t* __restrict__ sceneBodyStart,
+                                                             const uint32_t* __restrict__ sceneOrder,
+                                                             const uint32_t* __restrict__ sceneDyn, const uint2* __restrict__ range,
+                                                             float2* __restrict__ sceneRes, uint32_t* __restrict__ sceneSweeps,
+                                                             uint32_t S, uint32_t vIt, uint32_t pIt, int track,
+                                                             float residualTarget) {
+    extern __shared__ float4 body[];   // the scene's body velocities, then positions (x, y)
+    __shared__ uint2 rng[MAX_COLOURS];
+    __shared__ double red[SCENE_THREADS];
+    __shared__ uint32_t taken;
+    __shared__ int stop;
+    const uint32_t K = ctr->colours;
+    const uint32_t t = threadIdx.x;
+    while (true) {
+        if (t == 0) taken = atomicAdd(&ctr->sceneCursor, 1u);
+        __syncthreads();
+        const uint32_t k0 = taken;
+        if (k0 >= S) break;
+        const uint32_t s = sceneOrder[k0];
+        const uint32_t b0 = sceneBodyStart[s], b1 = sceneBodyStart[s + 1];
+        for (uint32_t k = t; k < K; k += SCENE_THREADS) rng[k] = range[(size_t)s * MAX_COLOURS + k];
+        for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) body[b - b0] = vel[b];
+        __syncthreads();
+        uint32_t done = vIt;
+        for (uint32_t q = 0; q < vIt; q++) {
+            float dsq = 0.0f, lsq = 0.0f;
+            // This thread's slots, colour by colour: rng[k].x + t, + SCENE_THREADS, ... below rng[k].y.
+            // The constraint after the current one is loaded before the current one is solved, also
+            // across the barrier that ends a colour: only this thread writes a constraint's
+            // accumulators, and nothing else in a constraint's record changes during the solve.
+            uint32_t pk = 0, pc = K ? rng[0].x + t : 0u;
+            auto seek = [&]() {
+                while (pk < K && pc >= rng[pk].y) {
+                    pk++;
+                    if (pk < K) pc = rng[pk].x + t;
+                }
+            };
+            seek();
+            Item next;
+            if (pk < K) loadItem(cs, pc, next);
+            for (uint32_t k = 0; k < K; k++) {
+                while (pk == k) {
+                    Item it = next;
+                    const uint32_t c = pc;
+                    pc += SCENE_THREADS;
+                    seek();
+                    if (pk < K) loadItem(cs, pc, next);
+                    if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) continue;   // a hole or another scene
+                    if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
+                        atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
+                        continue;
+                    }
+                    float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
+                    velocityStep(it, A, B, dsq, lsq);
+                    cs.imp[c] = it.imp;
+                    if (it.dep.z) body[it.id.x - b0] = A;
+                    if (it.dep.w) body[it.id.y - b0] = B;
+                }
+                __syncthreads();
+            }
+            const double num = sceneBlockSum((double)dsq, red);
+            const double den = sceneBlockSum((double)lsq, red);
+            if (t == 0) {
+                if (track) sceneRes[(size_t)s * MAX_SWEEPS + q] = make_float2((float)num, (float)den);
+                stop = residualTarget > 0.0f && sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12);
+            }
+            __syncthreads();
+            if (stop) { done = q + 1; break; }
+        }
+        for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) vel[b] = body[b - b0];
+        __syncthreads();
+        if (pIt) {
+            float2* bp = reinterpret_cast<float2*>(body);
+            for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) {
+                const float4 x = pos[b];
+                bp[b - b0] = make_float2(x.x, x.y);
+            }
+            __syncthreads();
+            for (uint32_t q = 0; q < pIt; q++)
+                for (uint32_t k = 0; k < K; k++) {
+                    const uint2 r = rng[k];
+                    for (uint32_t c = r.x + t; c < r.y; c += SCENE_THREADS) {
+                        const uint4 id = cs.idx[c];
+                        if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
+                        PosItem pi;
+                        loadPosItem(cs, c, pi);
+                        float2 A = bp[id.x - b0], B = bp[id.y - b0];
+                        positionStep(pi, A, B);
+                        if (pi.dep.z) bp[id.x - b0] = A;
+                        if (pi.dep.w) bp[id.y - b0] = B;
+                    }
+                    __syncthreads();
+                }
+            for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) reinterpret_cast<float2*>(pos + b)[0] = bp[b - b0];
+        }
+        if (t == 0) {
+            sceneSweeps[s] = done;
+            atomicMax(&ctr->sweeps, done);
+            atomicAdd(&ctr->bodySweeps, (unsigned long long)sceneDyn[s] * done);
+        }
+        __syncthreads();   // before the next scene reuses the LDS
+    }
+}
+
+// Step totals of the per-scene solve.
+__global__ void kSceneTotals(Counters* ctr) {
+    ctr->perScene = 1u;
+    ctr->solveSteps += 1;
+    ctr->sweepsTotal += ctr->sweeps;
+    ctr->bodySweepsTotal += ctr->bodySweeps;
+}
+
+// Step totals of the dataflow solve (all dynamic bodies ran the same sweeps).
+__global__ void kFlowTotals(Counters* ctr, uint32_t nDynBodies) {
+    ctr->bodySweeps = (unsigned long long)nDynBodies * ctr->sweeps;
+    ctr->bodySweepsTotal += ctr->bodySweeps;
+}
+
 // Accumulators back to the manifolds (the warm-start source of the next step) and
 // computeConstraintErrors (BodyCollisionSolver.cpp:132-183) as per-block partial sums.
 __global__ __launch_bounds__(256) void kSolveFinish(const Counters* __restrict__ ctr, Constraints cs,
@@ -772,6 +994,25 @@ __device__ inline double blockSum(double v, double* scratch) {
     double t = 0.0;
     for (uint32_t k = 0; k < blockDim.x / WAVE; k++) t += scratch[k];
     return t;
+}
+
+// Residual after each velocity sweep over all scenes still running at it (one block per sweep;
+// scenes summed in index order).
+__global__ __launch_bounds__(256) void kSceneReduce(Counters* __restrict__ ctr, const float2* __restrict__ sceneRes,
+                                                    const uint32_t* __restrict__ sceneSweeps, uint32_t S) {
+    __shared__ double scratch[4];
+    const uint32_t q = blockIdx.x;
+    if (q >= ctr->sweeps) return;
+    double num = 0.0, den = 0.0;
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x)
+        if (sceneSweeps[s] > q) {
+            const float2 v = sceneRes[(size_t)s * MAX_SWEEPS + q];
+            num += (double)v.x;
+            den += (double)v.y;
+        }
+    num = blockSum(num, scratch);
+    den = blockSum(den, scratch);
+    if (threadIdx.x == 0) { ctr->residualNum[q] = num; ctr->residualDen[q] = den; }
 }
 
 __global__ __launch_bounds__(256) void kSolveReduce(Counters* __restrict__ ctr, const float* __restrict__ blockRes,
@@ -964,16 +1205,63 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
     uint32_t* pDone = w.groupDone + w.solveGroups;
     void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&vDone, (void*)&pDone,
                      (void*)&w.blockRes, (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
-    return launchResident(reinterpret_cast<const void*>(kSolveFlow), (uint32_t)w.solveBlocks, args, st);
+    hipError_t e = launchResident(reinterpret_cast<const void*>(kSolveFlow), (uint32_t)w.solveBlocks, args, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kFlowTotals, dim3(1), dim3(1), 0, st, w.ctr, w.nDynBodies);
+    return hipGetLastError();
 }
 
-hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors) {
+uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks) {
+    blocks = 0;
+    const size_t bytes = std::max<size_t>(maxBodies, 1) * sizeof(float4);
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    const size_t staticBytes = MAX_COLOURS * sizeof(uint2) + SCENE_THREADS * sizeof(double) + 16;
+    const size_t limit = std::max<size_t>(prop.sharedMemPerBlock, (size_t)prop.maxSharedMemoryPerMultiProcessor);
+    if (bytes + staticBytes > limit) return 0;
+    if (bytes > 48 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneSolve), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    int perCu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(kSceneSolve), SCENE_THREADS,
+                                                     bytes) != hipSuccess || perCu < 1) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    blocks = numCU * perCu;
+    return (uint32_t)bytes;
+}
+
+hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
+                            float residualTarget) {
+    vIt = std::min(vIt, MAX_SWEEPS);
+    const uint32_t ranges = w.S * MAX_COLOURS;
+    hipLaunchKernelGGL(kSceneRangeClear, dim3(blocksFor(ranges)), dim3(256), 0, st, w.sceneRange, ranges);
+    hipLaunchKernelGGL(kSceneRanges, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.ctr, w.cs, w.bodyScene, w.sceneRange);
+    const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;
+    const uint32_t blocks = (uint32_t)std::max(1, std::min<int>(w.sceneBlocks, (int)w.S));
+    hipLaunchKernelGGL(kSceneSolve, dim3(blocks), dim3(SCENE_THREADS), w.sceneLdsBytes, st, w.ctr, w.cs, w.vel, w.pos,
+                       w.sceneBodyStart, w.sceneOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S, vIt, pIt,
+                       track, residualTarget);
+    hipLaunchKernelGGL(kSceneTotals, dim3(1), dim3(1), 0, st, w.ctr);
+    return hipGetLastError();
+}
+
+hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors,
+                             int perScene) {
     hipLaunchKernelGGL(kSolveFinish, dim3(FINISH_BLOCKS), dim3(256), 0, st, w.ctr, w.cs, w.vel, w.pos, w.man[w.cur],
                        w.errPartial, trackErrors);
     const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;
-    if (track || trackErrors)
+    if (perScene && track)
+        hipLaunchKernelGGL(kSceneReduce, dim3(MAX_SWEEPS), dim3(256), 0, st, w.ctr, w.sceneRes, w.sceneSweeps, w.S);
+    const int flowTrack = perScene ? 0 : track;
+    if (flowTrack || trackErrors)
         hipLaunchKernelGGL(kSolveReduce, dim3(1), dim3(256), 0, st, w.ctr, w.blockRes, (uint32_t)w.solveBlocks, w.errPartial,
-                           FINISH_BLOCKS, track, trackErrors);
+                           FINISH_BLOCKS, flowTrack, trackErrors);
     return hipGetLastError();
 }
 

@@ -107,6 +107,6 @@ hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice
                                  uint32_t collect, uint32_t newOnly, float gx, float gy, float dt);
 hipError_t launchLearnCollect(hipStream_t st, DeviceWorld& w, const MlpSamples& smp, uint32_t newOnly, float gx,
                               float gy, float dt);
-hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w);
+hipError_t launchGateUpdate(hipStream_t st, DeviceWorld& w, uint32_t toTarget);
 
 }  // namespace agx

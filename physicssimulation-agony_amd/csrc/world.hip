@@ -279,10 +279,16 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
                 SegmentTimer t(w, SEG_COLOUR);
                 HIP_CHECK(w, agx::launchWarmStart(st, d));
             }
+            const bool perScene = d.sceneLdsBytes > 0 &&
+                                  (s.solve_per_scene == 2 || (s.solve_per_scene == 1 && d.S > 1));
             {
                 SegmentTimer t(w, SEG_SOLVE);
-                HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations, trackResidual,
-                                              s.residual_target));
+                if (perScene)
+                    HIP_CHECK(w, agx::launchSceneSolve(st, d, s.velocity_iterations, s.position_iterations, trackResidual,
+                                                       s.residual_target));
+                else
+                    HIP_CHECK(w, agx::launchSolve(st, d, s.velocity_iterations, s.position_iterations, trackResidual,
+                                                  s.residual_target));
             }
             if (collect) {
                 SegmentTimer t(w, SEG_LEARN);
@@ -291,8 +297,9 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
                                                      s.gravity_y, dt));
             }
             SegmentTimer t(w, SEG_SOLVE_FINISH);
-            HIP_CHECK(w, agx::launchSolveFinish(st, d, trackResidual, s.residual_target, s.track_errors ? 1 : 0));
-            if (gated) HIP_CHECK(w, agx::launchGateUpdate(st, d));
+            HIP_CHECK(w, agx::launchSolveFinish(st, d, trackResidual, s.residual_target, s.track_errors ? 1 : 0,
+                                                perScene ? 1 : 0));
+            if (gated) HIP_CHECK(w, agx::launchGateUpdate(st, d, s.residual_target > 0.0f ? 1u : 0u));
         }
         break;
     case AGX_PHASE_ISLANDS: {
@@ -397,6 +404,7 @@ void agx_settings_default(agx_settings* s) {
     s->update_interval = 1.0f / 300.0f;
     s->max_delta_per_update = 1.0f / 20.0f;
     s->time_scale = 1.0f;
+    s->solve_per_scene = 1;
 }
 
 const char* agx_last_error(void) { return g_lastError.c_str(); }
@@ -538,11 +546,14 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     std::vector<float4> pos, vel, prev;
     std::vector<float2> rcs, inv, comOff, dims, extent, verts, normals;
     std::vector<uint32_t> bodyScene, colBody, colScene, colSlot, dynBodies, bodyToDyn, dynList, statList;
+    std::vector<uint32_t> sceneBodyStart, sceneDyn;
     std::vector<float> localRot;
     std::vector<uint4> shape;
     for (uint32_t s = 0; s < d.S; s++) {
         const agx::SceneData& sd = scenes[s];
         const uint32_t bOff = (uint32_t)pos.size(), cOff = (uint32_t)colBody.size(), vOff = (uint32_t)verts.size();
+        sceneBodyStart.push_back(bOff);
+        sceneDyn.push_back(sd.bodyCount() - sd.staticBodies);
         for (uint32_t i = 0; i < sd.bodyCount(); i++) {
             pos.push_back(make_float4(sd.comX[i], sd.comY[i], sd.rot[i], 0.0f));
             vel.push_back(make_float4(sd.velX[i], sd.velY[i], sd.angVel[i], 0.0f));
@@ -581,6 +592,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         w->staticColliders += sd.staticColliders;
     }
     d.N = (uint32_t)pos.size();
+    sceneBodyStart.push_back(d.N);
     d.C = (uint32_t)colBody.size();
     d.nDyn = (uint32_t)dynList.size();
     d.nStat = (uint32_t)statList.size();
@@ -607,6 +619,17 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         w->solvePerCuMax = perCuSolve;   // the sweep kernel's per-block buffers are sized for this many
     }
     if (agx::deviceStream(&w->stream) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "hipStreamCreate failed"));
+    // per-scene solve: scenes largest first, LDS for the largest scene's bodies
+    std::vector<uint32_t> sceneOrder(d.S);
+    uint32_t maxSceneBodies = 0;
+    for (uint32_t s = 0; s < d.S; s++) {
+        sceneOrder[s] = s;
+        maxSceneBodies = std::max(maxSceneBodies, sceneBodyStart[s + 1] - sceneBodyStart[s]);
+    }
+    std::stable_sort(sceneOrder.begin(), sceneOrder.end(), [&](uint32_t a, uint32_t b) {
+        return sceneBodyStart[a + 1] - sceneBodyStart[a] > sceneBodyStart[b + 1] - sceneBodyStart[b];
+    });
+    d.sceneLdsBytes = agx::sceneSolveSetup(maxSceneBodies, d.numCU, d.sceneBlocks);
 
 #define UP(p, v) do { if (upload(w, &(p), (v)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "upload " #p)); } while (0)
 #define AL(p, n) do { if (devAlloc(w, &(p), (n)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "alloc " #p)); } while (0)
@@ -615,6 +638,9 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     UP(d.colBody, colBody); UP(d.colScene, colScene); UP(d.colSlot, colSlot); UP(d.comOff, comOff);
     UP(d.localRot, localRot); UP(d.shape, shape); UP(d.dims, dims); UP(d.extent, extent);
     UP(d.verts, verts); UP(d.normals, normals); UP(d.materials, mats);
+    UP(d.sceneBodyStart, sceneBodyStart); UP(d.sceneOrder, sceneOrder); UP(d.sceneDyn, sceneDyn);
+    AL(d.sceneRange, (uint64_t)d.S * agx::MAX_COLOURS); AL(d.sceneRes, (uint64_t)d.S * agx::MAX_SWEEPS);
+    AL(d.sceneSweeps, d.S);
     AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N);
     AL(d.pose, d.C); AL(d.wRot, d.C); AL(d.prevW, d.C); AL(d.aabb, d.C);
     if ((rc = allocTree(w, d.dyn, dynList, colScene, d.S))) return bail(rc);
@@ -1181,11 +1207,22 @@ int agx_world_read_stats(agx_world* w, agx_stats* out) {
     out->learned_warm_start = c->learnUsed;
     out->solved_steps = c->solveSteps;
     out->total_velocity_sweeps = c->sweepsTotal;
+    out->total_body_sweeps = c->bodySweepsTotal;
+    out->mean_velocity_sweeps = (float)((double)c->bodySweeps / (double)std::max(w->d.nDynBodies, 1u));
+    out->solved_per_scene = c->perScene;
     out->velocity_error = c->velocityError;
     out->position_error = c->positionError;
     for (uint32_t k = 0; k < 64 && k < agx::MAX_SWEEPS; k++)
         out->residual[k] = (float)(std::sqrt(c->residualNum[k]) / std::fmax(std::sqrt(c->residualDen[k]), 1e-12));
     delete c;
+    return AGX_OK;
+}
+
+int agx_world_read_scene_sweeps(agx_world* w, uint32_t* out, uint32_t n) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != w->d.S) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u scenes, got %u", w->d.S, n);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    HIP_CHECK(w, hipMemcpy(out, w->d.sceneSweeps, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return AGX_OK;
 }
 

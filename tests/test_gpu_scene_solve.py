@@ -1,0 +1,109 @@
+"""The per-scene solve (each scene of a batch solved by one workgroup with its bodies in LDS)
+against the dataflow solve over the whole batch: the same sequential colour-major order, so fixed
+sweep counts give identical bits; with a residual target every scene stops at its own sweep,
+exactly as the same scene solved alone."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel")
+DT = 1.0 / 60.0
+
+
+def _world(agx, parts, **settings):
+    b = agx.Builder(len(parts))
+    for i, sc in enumerate(parts):
+        agx.build_scene(b, sc, i)
+    return agx.World(b, agx.default_settings(**settings))
+
+
+def _same(a, b):
+    for f in FIELDS:
+        np.testing.assert_array_equal(a.body(f), b.body(f), err_msg=f)
+    np.testing.assert_array_equal(a.manifolds(), b.manifolds())
+
+
+def test_per_scene_solve_matches_dataflow_bit_for_bit(agx, scenes):
+    parts = scenes.batch(24)
+    base = dict(velocity_iterations=8, position_iterations=3, track_residual=1, track_errors=1)
+    flow = _world(agx, parts, solve_per_scene=0, **base)
+    per = _world(agx, parts, solve_per_scene=1, **base)
+    print("bodies", flow.counts.bodies, flush=True)
+    for n in (1, 10, 29):
+        flow.step(DT, n)
+        per.step_graph(DT, n)
+        _same(flow, per)
+    sf, sp = flow.stats(), per.stats()
+    assert sf.solved_per_scene == 0 and sp.solved_per_scene == 1
+    assert sf.overflow == 0 and sp.overflow == 0 and sp.manifolds > 0
+    assert sp.velocity_sweeps == 8 and sp.mean_velocity_sweeps == pytest.approx(8.0)
+    assert sp.total_body_sweeps == sf.total_body_sweeps
+    np.testing.assert_allclose(sp.residual[:8], sf.residual[:8], rtol=1e-4)
+    assert sp.velocity_error == pytest.approx(sf.velocity_error, rel=1e-4)
+
+
+def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes):
+    """One step with a 1e-3 residual target, solved scene by scene: every scene ran the sweeps
+    the oracle needs to reach the target on that scene alone (replayed in the GPU's order), and
+    ended in the oracle's state after exactly those sweeps."""
+    parts = [scenes.batch_scene(i, seed=4, n_min=64, n_max=600) for i in range(9)]
+    target = 1e-3
+    w = _world(agx, parts, velocity_iterations=64, residual_target=target, position_iterations=3, track_residual=1)
+    w.step(DT, 45)
+    cscene = w.collider("scene")
+    bscene = np.empty(w.counts.bodies, np.uint32)
+    bscene[w.collider("body")] = cscene
+    for p in range(agx.PHASE_SOLVE):
+        w.run_phase(p, DT)
+    mg = w.manifolds(0)
+    state = {f: w.body(f) for f in ("com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel", "cos", "sin")}
+    w.run_phase(agx.PHASE_SOLVE, DT)
+    st = w.stats()
+    assert st.solved_per_scene == 1 and st.overflow == 0
+    sweeps = w.scene_sweeps()
+    order, _ = w.solve_order()
+    after = {f: w.body(f) for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y")}
+    pos_of = np.empty(len(order), np.int64)
+    pos_of[order] = np.arange(len(order))
+    renames = {"com_x": "comX", "com_y": "comY", "rotation": "rot", "vel_x": "velX", "vel_y": "velY",
+               "ang_vel": "angVel", "cos": "cos", "sin": "sin"}
+    checked = 0
+    for s_, sc in enumerate(parts):
+        bs = np.flatnonzero(bscene == s_)
+        cs = np.flatnonzero(cscene == s_)
+        c_off = int(cs[0])
+        msel = np.flatnonzero((mg["a"] >= c_off) & (mg["a"] < c_off + len(cs)))
+        ms = mg[msel].copy()
+        ms["a"] -= c_off
+        ms["b"] -= c_off
+        local = np.argsort(pos_of[msel], kind="stable").astype(np.uint32)
+
+        def fresh():
+            ow = oracle.world_from_scene(sc, dict(v=64, p=3))
+            for f, o in renames.items():
+                ow.set(o, state[f][bs])
+            return ow
+        _, res = fresh().solve(ms, order=local, v_it=64, p_it=3, residual=True)
+        below = np.flatnonzero(res <= target)
+        k = int(below[0]) + 1 if len(below) else 64
+        # a residual within rounding of the target may fall either side on the GPU
+        near = abs(res[k - 1] - target) < 1e-4 * target or (k > 1 and abs(res[k - 2] - target) < 1e-4 * target)
+        if near:
+            continue
+        assert sweeps[s_] == k, (s_, int(sweeps[s_]), k, res[:k + 1])
+        ow = fresh()
+        ow.solve(ms, order=local, v_it=k, p_it=3)
+        for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y"):
+            np.testing.assert_allclose(after[f][bs], ow.get(renames[f]), rtol=1e-4, atol=1e-4, err_msg=f"{s_} {f}")
+        checked += 1
+    assert checked >= 7
+    assert st.velocity_sweeps == sweeps.max()
+    assert len(set(sweeps.tolist())) > 1          # scenes stopped apart
+    assert st.mean_velocity_sweeps < st.velocity_sweeps
+
+
+def test_single_scene_defaults_to_the_dataflow(agx, scenes):
+    w = _world(agx, [scenes.pile_small(seed=1)], velocity_iterations=8)
+    w.step(DT, 3)
+    assert w.stats().solved_per_scene == 0
