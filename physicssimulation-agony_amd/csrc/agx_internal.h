@@ -150,7 +150,7 @@ struct Tree {
 // Constraint planes, one float4 per constraint per plane, in solve order (colour-major).
 struct Constraints {
     uint4* idx;      // bodyA, bodyB, count, manifold
-    uint4* dep;      // rankA, degreeA, rankB, degreeB (degree 0: static body, never waited on)
+    uint4* dep;      // predecessor slot at A and at B (dataflow only), A dynamic, B dynamic
     float4* nrm;     // nx, ny, staticFriction, dynamicFriction
     float4* inv;     // invMassA, invMassB, invInertiaA, invInertiaB
     float4* r0;      // rAn0, rBn0, rAt0, rBt0
@@ -238,6 +238,9 @@ struct DeviceWorld {
     uint2* sceneRange = nullptr;          // S x MAX_COLOURS: first and one past the last slot of the scene in a colour
     float2* sceneRes = nullptr;           // S x MAX_SWEEPS: the scene's squared accumulator change and accumulators
     uint32_t* sceneSweeps = nullptr;      // velocity sweeps each scene ran this step
+    uint32_t* sceneIslands = nullptr;     // islands per scene (per-scene islands)
+    uint32_t* sceneIslandOffset = nullptr;   // islands of all earlier scenes
+    uint32_t sceneMaxBodies = 0;          // bodies of the largest scene
     uint32_t sceneLdsBytes = 0;           // dynamic LDS of the per-scene solve (0: some scene does not fit)
     int sceneBlocks = 0;                  // resident blocks of the per-scene solve
     float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
@@ -402,6 +405,8 @@ hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks);
 hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                             float residualTarget);
+// Islands from the slot ranges of the last per-scene solve (one workgroup per scene).
+hipError_t launchSceneIslands(hipStream_t st, DeviceWorld& w);
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                        float residualTarget);
 hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors,

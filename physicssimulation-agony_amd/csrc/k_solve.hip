@@ -1127,6 +1127,104 @@ __global__ __launch_bounds__(256) void kIslandIds(const uint32_t* __restrict__ l
 
 __global__ void kIslandCount(Counters* ctr) { ctr->islands = ctr->scratch[3]; }
 
+// ---------------------------------------------------------------- per-scene islands
+// After a per-scene solve, each scene's islands come from one workgroup: union-find over the
+// scene's dynamic contacts in LDS (a root is hooked under the smaller root, so every root is its
+// component's smallest body), then the roots ranked in body order. A scan over the scenes'
+// counts turns the local ranks into the ids the device-wide union-find gives: islands numbered by
+// ascending root body.
+constexpr uint32_t ISLAND_THREADS = 512;
+
+__device__ inline uint32_t ldsFind(uint32_t* par, uint32_t x) {
+    uint32_t p = par[x];
+    while (p != x) {
+        const uint32_t g = par[p];
+        par[x] = g;   // path halving: both values are ancestors of x
+        x = p;
+        p = g;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(ISLAND_THREADS) void kSceneIslands(Constraints cs, const float2* __restrict__ inv,
+                                                                const uint32_t* __restrict__ sceneBodyStart,
+                                                                const uint2* __restrict__ range, const Counters* __restrict__ ctr,
+                                                                uint32_t* __restrict__ islandId,
+                                                                uint32_t* __restrict__ sceneIslands) {
+    extern __shared__ uint32_t par[];   // the scene's bodies, then the root ranks
+    __shared__ uint32_t waveSum[ISLAND_THREADS / WAVE];
+    const uint32_t s = blockIdx.x;
+    const uint32_t b0 = sceneBodyStart[s], b1 = sceneBodyStart[s + 1], n = b1 - b0;
+    const uint32_t t = threadIdx.x;
+    const uint32_t K = ctr->colours;
+    for (uint32_t i = t; i < n; i += ISLAND_THREADS) par[i] = i;
+    __syncthreads();
+    for (uint32_t k = 0; k < K; k++) {
+        const uint2 r = range[(size_t)s * MAX_COLOURS + k];
+        for (uint32_t c = r.x + t; c < r.y; c += ISLAND_THREADS) {
+            const uint4 id = cs.idx[c];
+            if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
+            const uint4 dep = cs.dep[c];
+            if (!dep.z || !dep.w) continue;   // contacts with a static body join nothing
+            uint32_t a = ldsFind(par, id.x - b0), b = ldsFind(par, id.y - b0);
+            while (a != b) {
+                if (a > b) { const uint32_t x = a; a = b; b = x; }
+                const uint32_t old = atomicCAS(par + b, b, a);
+                if (old == b) break;
+                b = ldsFind(par, old);
+            }
+        }
+    }
+    __syncthreads();
+    // roots of dynamic bodies, ranked in body order: each thread takes a contiguous run of bodies
+    const uint32_t per = (n + ISLAND_THREADS - 1) / ISLAND_THREADS;
+    const uint32_t lo = min(n, t * per), hi = min(n, lo + per);
+    uint32_t roots = 0;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t x = i, p = par[x];
+        while (p != x) { x = p; p = par[x]; }
+        roots += (x == i && inv[b0 + i].x != 0.0f) ? 1u : 0u;
+    }
+    uint32_t incl = roots;
+    const uint32_t lane = t & (WAVE - 1), wid = t / WAVE;
+    for (uint32_t off = 1; off < WAVE; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += v;
+    }
+    if (lane == WAVE - 1) waveSum[wid] = incl;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (uint32_t w = 0; w < ISLAND_THREADS / WAVE; w++) {
+        if (w < wid) base += waveSum[w];
+        total += waveSum[w];
+    }
+    // root ranks go to the second half of the LDS array (par stays intact for the lookups)
+    uint32_t* rankOf = par + n;
+    uint32_t next = base + incl - roots;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t x = i, p = par[x];
+        while (p != x) { x = p; p = par[x]; }
+        if (x == i && inv[b0 + i].x != 0.0f) rankOf[i] = next++;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < n; i += ISLAND_THREADS) {
+        uint32_t x = i, p = par[x];
+        while (p != x) { x = p; p = par[x]; }
+        islandId[b0 + i] = inv[b0 + i].x != 0.0f ? rankOf[x] : 0xFFFFFFFFu;
+    }
+    if (t == 0) sceneIslands[s] = total;
+}
+
+// Local island ranks plus the island count of all earlier scenes.
+__global__ __launch_bounds__(256) void kSceneIslandOffsets(const uint32_t* __restrict__ bodyScene,
+                                                           const uint32_t* __restrict__ offset, uint32_t n,
+                                                           uint32_t* __restrict__ islandId) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+        const uint32_t id = islandId[b];
+        if (id != 0xFFFFFFFFu) islandId[b] = id + offset[bodyScene[b]];
+    }
+}
+
 }  // namespace
 
 // Launches a dataflow kernel whose blocks wait on one another, so all of them must be resident at
@@ -1262,6 +1360,18 @@ hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, 
     if (flowTrack || trackErrors)
         hipLaunchKernelGGL(kSolveReduce, dim3(1), dim3(256), 0, st, w.ctr, w.blockRes, (uint32_t)w.solveBlocks, w.errPartial,
                            FINISH_BLOCKS, flowTrack, trackErrors);
+    return hipGetLastError();
+}
+
+hipError_t launchSceneIslands(hipStream_t st, DeviceWorld& w) {
+    const uint32_t bytes = 2 * std::max<uint32_t>(w.sceneMaxBodies, 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(kSceneIslands, dim3(w.S), dim3(ISLAND_THREADS), bytes, st, w.cs, w.inv, w.sceneBodyStart, w.sceneRange,
+                       w.ctr, w.islandId, w.sceneIslands);
+    hipError_t e = launchScan(st, w.sceneIslands, w.sceneIslandOffset, nullptr, &w.ctr->islands, w.S, w.scanBlock,
+                              w.scanBlockCap);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kSceneIslandOffsets, dim3(blocksFor(w.N)), dim3(256), 0, st, w.bodyScene, w.sceneIslandOffset, w.N,
+                       w.islandId);
     return hipGetLastError();
 }
 

@@ -44,6 +44,7 @@ struct agx_world {
     uint32_t lastStepCount = 1;
     bool postUpdatePending = true;   // next step's CCD sweep starts from the post-update pose
     bool inStep = false;             // phases of a step have run but FINISH has not
+    bool solvedPerScene = false;     // this step's SOLVE phase ran the per-scene solve
     int nextPhase = 0;
     bool profiling = false;
     std::vector<EventPair> pending;  // recorded, not yet read
@@ -224,6 +225,7 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
     case AGX_PHASE_INTEGRATE: {
         if (int r = beginStep(w)) return r;
         w->inStep = true;
+        w->solvedPerScene = false;
         SegmentTimer t(w, SEG_INTEGRATE);
         HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0));
         break;
@@ -281,6 +283,7 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
             }
             const bool perScene = d.sceneLdsBytes > 0 &&
                                   (s.solve_per_scene == 2 || (s.solve_per_scene == 1 && d.S > 1));
+            w->solvedPerScene = perScene;
             {
                 SegmentTimer t(w, SEG_SOLVE);
                 if (perScene)
@@ -304,7 +307,8 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
         break;
     case AGX_PHASE_ISLANDS: {
         SegmentTimer t(w, SEG_ISLANDS);
-        HIP_CHECK(w, agx::launchIslands(st, d));
+        if (w->solvedPerScene && !s.collide_only) HIP_CHECK(w, agx::launchSceneIslands(st, d));
+        else HIP_CHECK(w, agx::launchIslands(st, d));
         break;
     }
     case AGX_PHASE_FINISH: {
@@ -630,6 +634,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         return sceneBodyStart[a + 1] - sceneBodyStart[a] > sceneBodyStart[b + 1] - sceneBodyStart[b];
     });
     d.sceneLdsBytes = agx::sceneSolveSetup(maxSceneBodies, d.numCU, d.sceneBlocks);
+    d.sceneMaxBodies = maxSceneBodies;
 
 #define UP(p, v) do { if (upload(w, &(p), (v)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "upload " #p)); } while (0)
 #define AL(p, n) do { if (devAlloc(w, &(p), (n)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "alloc " #p)); } while (0)
@@ -640,7 +645,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     UP(d.verts, verts); UP(d.normals, normals); UP(d.materials, mats);
     UP(d.sceneBodyStart, sceneBodyStart); UP(d.sceneOrder, sceneOrder); UP(d.sceneDyn, sceneDyn);
     AL(d.sceneRange, (uint64_t)d.S * agx::MAX_COLOURS); AL(d.sceneRes, (uint64_t)d.S * agx::MAX_SWEEPS);
-    AL(d.sceneSweeps, d.S);
+    AL(d.sceneSweeps, d.S); AL(d.sceneIslands, d.S); AL(d.sceneIslandOffset, d.S);
     AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N);
     AL(d.pose, d.C); AL(d.wRot, d.C); AL(d.prevW, d.C); AL(d.aabb, d.C);
     if ((rc = allocTree(w, d.dyn, dynList, colScene, d.S))) return bail(rc);

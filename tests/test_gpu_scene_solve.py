@@ -1,7 +1,8 @@
 """The per-scene solve (each scene of a batch solved by one workgroup with its bodies in LDS)
 against the dataflow solve over the whole batch: the same sequential colour-major order, so fixed
-sweep counts give identical bits; with a residual target every scene stops at its own sweep,
-exactly as the same scene solved alone."""
+sweep counts give identical bits (and the per-scene islands the same ids as the device-wide
+union-find); with a residual target every scene stops where the oracle, replaying that scene
+alone in the GPU's order, reaches the target."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,11 @@ def _same(a, b):
     for f in FIELDS:
         np.testing.assert_array_equal(a.body(f), b.body(f), err_msg=f)
     np.testing.assert_array_equal(a.manifolds(), b.manifolds())
+    # islands: device-wide union-find (dataflow solve) and per-scene union-find in LDS agree exactly
+    ia, ka = a.islands()
+    ib, kb = b.islands()
+    assert ka == kb
+    np.testing.assert_array_equal(ia, ib)
 
 
 def test_per_scene_solve_matches_dataflow_bit_for_bit(agx, scenes):
