@@ -58,11 +58,11 @@ typedef struct agx_settings {
     uint32_t track_residual;               /* relative impulse change after every velocity sweep */
     float residual_target;                 /* > 0: stop velocity sweeps once residual <= target */
     uint32_t collide_only;                 /* 1: integrate + collide, skip the solver (broad/narrow timing) */
-    uint32_t solve_per_scene;              /* 0: one dataflow over all scenes (one residual for the batch);
-                                              1 (default): each scene solved by one workgroup with its bodies in
-                                              LDS, stopping at its own residual, when the world has several
-                                              scenes and every scene fits; 2: also for a single scene.
-                                              Fixed sweep counts give the same bits either way. */
+    uint32_t solve_per_scene;              /* 1 (default): each scene solved by one workgroup with its bodies
+                                              in LDS, stopping at its own residual, whenever every scene fits
+                                              (about 10,000 bodies); 0: one dataflow over all scenes (one
+                                              residual for the batch), also used for larger scenes. Fixed
+                                              sweep counts give the same bits either way. */
 } agx_settings;
 
 /* One manifold record (64 bytes): NarrowPhaseCollisionDetector.h:26-61 with the collider

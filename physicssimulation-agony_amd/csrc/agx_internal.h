@@ -238,10 +238,13 @@ struct DeviceWorld {
     uint2* sceneRange = nullptr;          // S x MAX_COLOURS: first and one past the last slot of the scene in a colour
     float2* sceneRes = nullptr;           // S x MAX_SWEEPS: the scene's squared accumulator change and accumulators
     uint32_t* sceneSweeps = nullptr;      // velocity sweeps each scene ran this step
+    uint2* sceneManRange = nullptr;       // S: first manifold and one past the last of the scene
     uint32_t* sceneIslands = nullptr;     // islands per scene (per-scene islands)
     uint32_t* sceneIslandOffset = nullptr;   // islands of all earlier scenes
     uint32_t sceneMaxBodies = 0;          // bodies of the largest scene
     uint32_t sceneLdsBytes = 0;           // dynamic LDS of the per-scene solve (0: some scene does not fit)
+    uint32_t sceneColourLdsBytes = 0;     // dynamic LDS of the per-scene colouring
+    uint32_t sceneColourCap = 0xFFFFu;    // most manifolds of a scene the colouring keeps in LDS
     int sceneBlocks = 0;                  // resident blocks of the per-scene solve
     float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
     Constraints cs{};
@@ -402,9 +405,11 @@ hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // back to the manifolds, residual and error totals.
 // Per-scene solve: LDS bytes for scenes of up to maxBodies bodies (0 when they do not fit) and the
 // resident blocks for that size.
-uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks);
+uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks, uint32_t& colourLdsBytes);
 hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                             float residualTarget);
+// Colouring (one workgroup per scene, in LDS) and constraint data without predecessor slots.
+hipError_t launchColourPrepScene(hipStream_t st, DeviceWorld& w);
 // Islands from the slot ranges of the last per-scene solve (one workgroup per scene).
 hipError_t launchSceneIslands(hipStream_t st, DeviceWorld& w);
 hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,

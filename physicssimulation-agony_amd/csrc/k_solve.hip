@@ -197,6 +197,158 @@ __global__ __launch_bounds__(256) void kColourFlow(Counters* __restrict__ ctr, c
     }
 }
 
+// ---------------------------------------------------------------- per-scene colouring
+// The same greedy first fit in priority order as kColourFlow, one workgroup per scene, as rounds
+// in LDS: a manifold takes its colour once it has the smallest priority among the uncoloured
+// manifolds at each of its dynamic bodies. Then every higher-priority manifold at those bodies is
+// coloured and no lower-priority one is, so the colours free at both bodies are exactly the
+// sequential first fit's, and at most one manifold per body takes a colour in a round.
+constexpr uint32_t COLOUR_THREADS = 512;
+constexpr uint16_t UNCOLOURED = 0xFFFFu;
+
+__global__ __launch_bounds__(256) void kSceneManClear(uint2* __restrict__ range, uint32_t S) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x)
+        range[i] = make_uint2(0xFFFFFFFFu, 0u);
+}
+
+// First manifold and one past the last of every scene (manifolds come grouped by scene up to a
+// pair wave that straddles two scenes).
+__global__ __launch_bounds__(256) void kSceneManRanges(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
+                                                       const uint32_t* __restrict__ colScene, uint2* __restrict__ range) {
+    const uint32_t M = ctr->manifolds;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < M; base += gridDim.x * blockDim.x) {
+        const uint32_t m = base + threadIdx.x;
+        const uint32_t key = m < M ? colScene[man[m].a] : 0xFFFFFFFFu;
+        const uint32_t first = __shfl(key, 0, WAVE);
+        if (__all(key == first)) {
+            if (first == 0xFFFFFFFFu) continue;
+            uint32_t mn = m, mx = m + 1;
+            for (uint32_t off = WAVE / 2; off > 0; off >>= 1) {
+                mn = min(mn, __shfl_xor(mn, off, WAVE));
+                mx = max(mx, __shfl_xor(mx, off, WAVE));
+            }
+            if (lane == 0) { atomicMin(&range[first].x, mn); atomicMax(&range[first].y, mx); }
+        } else if (key != 0xFFFFFFFFu) {
+            atomicMin(&range[key].x, m);
+            atomicMax(&range[key].y, m + 1);
+        }
+    }
+}
+
+// manBodies of every manifold cleared before the scenes fill in their own: a scene's manifold
+// range can hold a few of a neighbouring scene's manifolds, which it recognises by bodies outside
+// its own, whether their owner has written them yet or not.
+__global__ __launch_bounds__(256) void kClearManBodies(const Counters* __restrict__ ctr, uint2* __restrict__ manBodies) {
+    const uint32_t M = ctr->manifolds;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x)
+        manBodies[m] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+}
+
+// Also writes manBodies (the manifold's bodies) and degree (contacts per dynamic body) for the
+// scene's manifolds and bodies, as kDegree does for the dataflow. The rounds work on a copy of the
+// scene's manifolds (bodies packed with their dynamic flags, colour) in LDS when it fits after the
+// body records, else on the same copy in global scratch (`spill`, `spillColour`).
+__device__ inline uint32_t packManifold(uint32_t la, bool da, uint32_t lb, bool db) {
+    return (la | (da ? 0x8000u : 0u)) | ((lb | (db ? 0x8000u : 0u)) << 16);
+}
+
+__global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifold* __restrict__ man, Counters* __restrict__ ctr,
+                                                               const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
+                                                               const uint32_t* __restrict__ sceneBodyStart,
+                                                               const uint2* __restrict__ manRange, uint32_t ldsManifolds,
+                                                               uint2* __restrict__ manBodies, uint32_t* __restrict__ degree,
+                                                               uint32_t* __restrict__ spill, uint16_t* __restrict__ colourId) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int left;
+    const uint32_t s = blockIdx.x, t = threadIdx.x;
+    const uint32_t b0 = sceneBodyStart[s], b1 = sceneBodyStart[s + 1], nb = b1 - b0;
+    const uint32_t maxBodies = ldsManifolds >> 16;   // LDS layout: body records for the largest scene first
+    const uint32_t capM = ldsManifolds & 0xFFFFu;
+    uint32_t* minPrio = lds;                         // nb: smallest priority of an uncoloured manifold at the body
+    uint32_t* mask = lds + maxBodies;                // nb x 4: colours taken at the body
+    uint32_t* packedL = lds + 5 * maxBodies;         // capM: the scene's manifolds (bodies, flags)
+    uint16_t* colourL = reinterpret_cast<uint16_t*>(packedL + capM);   // capM colours
+    const uint2 r = manRange[s];
+    const uint32_t nm = r.y > r.x ? r.y - r.x : 0u;
+    const bool inLds = nm <= capM;
+    uint32_t* packed = inLds ? packedL : spill + r.x;
+    uint16_t* col = inLds ? colourL : colourId + r.x;
+    // a manifold of this scene: marked in LDS, or in the spill case recognised by its bodies
+    // (manBodies was cleared before, and only a manifold's own scene writes its entry)
+    auto own = [&](uint32_t i) -> bool {
+        return inLds ? packed[i] != 0xFFFFFFFFu : manBodies[r.x + i].x - b0 < nb;
+    };
+    for (uint32_t i = t; i < nb; i += COLOUR_THREADS) {
+        minPrio[i] = 0xFFFFFFFFu;
+        for (uint32_t k = 0; k < 4; k++) mask[4 * i + k] = 0u;
+    }
+    __syncthreads();
+    // the scene's manifolds: bodies, degrees (counted in the first mask word), uncoloured
+    for (uint32_t i = t; i < nm; i += COLOUR_THREADS) {
+        const uint32_t m = r.x + i;
+        const agx_manifold d = man[m];
+        const uint32_t a = colBody[d.a], b = colBody[d.b];
+        if (a < b0 || a >= b1) {   // another scene's manifold (its global slot is its owner's)
+            if (inLds) packed[i] = 0xFFFFFFFFu;
+            continue;
+        }
+        manBodies[m] = make_uint2(a, b);
+        const bool da = inv[a].x != 0.0f, db = inv[b].x != 0.0f;
+        packed[i] = packManifold(a - b0, da, b - b0, db);
+        col[i] = UNCOLOURED;
+        if (da) atomicAdd(&mask[4 * (a - b0)], 1u);
+        if (db) atomicAdd(&mask[4 * (b - b0)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < nb; i += COLOUR_THREADS) {
+        degree[b0 + i] = mask[4 * i];   // 0 for a static body: none of its contacts counted it
+        mask[4 * i] = 0u;
+    }
+    __syncthreads();
+    while (true) {
+        if (t == 0) left = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < nm; i += COLOUR_THREADS) {
+            if (!own(i)) continue;
+            const uint32_t pk = packed[i];
+            if (col[i] != UNCOLOURED) continue;
+            const uint32_t pr = mix32(r.x + i);
+            if (pk & 0x8000u) atomicMin(&minPrio[pk & 0x7FFFu], pr);
+            if (pk & 0x80000000u) atomicMin(&minPrio[(pk >> 16) & 0x7FFFu], pr);
+            left = 1;
+        }
+        __syncthreads();
+        if (!left) break;
+        for (uint32_t i = t; i < nm; i += COLOUR_THREADS) {
+            if (!own(i)) continue;
+            const uint32_t pk = packed[i];
+            if (col[i] != UNCOLOURED) continue;
+            const uint32_t pr = mix32(r.x + i);
+            const uint32_t la = pk & 0x7FFFu, lb = (pk >> 16) & 0x7FFFu;
+            const bool da = (pk & 0x8000u) != 0, db = (pk & 0x80000000u) != 0;
+            if ((da && minPrio[la] != pr) || (db && minPrio[lb] != pr)) continue;
+            uint32_t c = FIRST_FIT_COLOURS;
+            for (uint32_t k = 0; k < 4; k++) {
+                const uint32_t used = (da ? mask[4 * la + k] : 0u) | (db ? mask[4 * lb + k] : 0u);
+                if (~used) { c = 32 * k + (uint32_t)__ffs((int)~used) - 1; break; }
+            }
+            col[i] = (uint16_t)c;
+            if (c == FIRST_FIT_COLOURS) { atomicOr(&ctr->overflow, OVERFLOW_COLOURING); continue; }
+            // no other manifold of this round reads these bodies' masks (it would need the
+            // smallest priority here), so the bits can be set at once
+            if (da) mask[4 * la + (c >> 5)] |= 1u << (c & 31u);
+            if (db) mask[4 * lb + (c >> 5)] |= 1u << (c & 31u);
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < nb; i += COLOUR_THREADS) minPrio[i] = 0xFFFFFFFFu;
+        __syncthreads();
+    }
+    if (inLds)
+        for (uint32_t i = t; i < nm; i += COLOUR_THREADS)
+            if (own(i)) colourId[r.x + i] = col[i];
+}
+
 // ---------------------------------------------------------------- solve order
 // Block histograms over contiguous manifold chunks, one column scan, then a scatter that
 // reserves each block's slots from its scanned base.
@@ -332,6 +484,8 @@ __device__ inline uint32_t predecessorOf(uint32_t body, uint32_t r, const uint32
     return posOf[incSorted[s + degree[body] - 1] >> 1] | PRED_PREV_SWEEP;
 }
 
+// PRED: the predecessor slots the dataflow solve waits on (the per-scene solve needs none).
+template <bool PRED>
 __global__ __launch_bounds__(256) void kPrep(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
                                              const uint2* __restrict__ manBodies, const uint32_t* __restrict__ posOf,
                                              const uint32_t* __restrict__ degree, const uint32_t* __restrict__ rank,
@@ -373,8 +527,8 @@ __global__ __launch_bounds__(256) void kPrep(const agx_manifold* __restrict__ ma
         }
         const bool dA = iA.x != 0.0f, dB = iB.x != 0.0f;
         cs.idx[slot] = make_uint4(bA, bB, d.count, m);
-        cs.dep[slot] = make_uint4(dA ? predecessorOf(bA, rank[2 * m], degree, incStart, incSorted, posOf) : NO_PRED,
-                                  dB ? predecessorOf(bB, rank[2 * m + 1], degree, incStart, incSorted, posOf) : NO_PRED,
+        cs.dep[slot] = make_uint4(PRED && dA ? predecessorOf(bA, rank[2 * m], degree, incStart, incSorted, posOf) : NO_PRED,
+                                  PRED && dB ? predecessorOf(bB, rank[2 * m + 1], degree, incStart, incSorted, posOf) : NO_PRED,
                                   dA ? 1u : 0u, dB ? 1u : 0u);
         cs.nrm[slot] = make_float4(nx, ny, sf, df);
         cs.inv[slot] = make_float4(iA.x, iB.x, iA.y, iB.y);
@@ -852,6 +1006,34 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
         for (uint32_t k = t; k < K; k += SCENE_THREADS) rng[k] = range[(size_t)s * MAX_COLOURS + k];
         for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) body[b - b0] = vel[b];
         __syncthreads();
+        // applyWarmStarting (BodyCollisionSolver.cpp:416-490) colour by colour: every body adds its
+        // constraints' impulses in solve order, with the operations of kWarm
+        for (uint32_t k = 0; k < K; k++) {
+            const uint2 r = rng[k];
+            for (uint32_t c = r.x + t; c < r.y; c += SCENE_THREADS) {
+                const uint4 id = cs.idx[c];
+                if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
+                const uint4 dep = cs.dep[c];
+                const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], r1 = cs.r1[c], j = cs.imp[c];
+                const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
+                const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
+                const float sn = jn0 + jn1, st = jt0 + jt1;
+                const float sx = sn * nx + st * tx, sy = sn * ny + st * ty;
+                if (dep.z) {
+                    float4 v = body[id.x - b0];
+                    v.x = v.x - sx * iv.x; v.y = v.y - sy * iv.x;
+                    v.z -= (jn0 * r0.x + jt0 * r0.z + jn1 * r1.x + jt1 * r1.z) * iv.z;
+                    body[id.x - b0] = v;
+                }
+                if (dep.w) {
+                    float4 v = body[id.y - b0];
+                    v.x = v.x + sx * iv.y; v.y = v.y + sy * iv.y;
+                    v.z += (jn0 * r0.y + jt0 * r0.w + jn1 * r1.y + jt1 * r1.w) * iv.w;
+                    body[id.y - b0] = v;
+                }
+            }
+            __syncthreads();
+        }
         uint32_t done = vIt;
         for (uint32_t q = 0; q < vIt; q++) {
             float dsq = 0.0f, lsq = 0.0f;
@@ -1279,7 +1461,31 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL(kMarkHoles, dim3((MAX_COLOURS * COLOUR_ALIGN + 255) / 256), dim3(256), 0, st, w.ctr, w.order, w.cs);
     hipLaunchKernelGGL(kRankEntries<true>, dim3(ge), dim3(256), 0, st, w.ctr, w.manBodies, w.degree, w.incStart, w.inc,
                        w.posOf, w.rank, w.incSorted);
-    hipLaunchKernelGGL(kPrep, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
+    hipLaunchKernelGGL(kPrep<true>, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
+                       w.incStart, w.incSorted, w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
+    return hipGetLastError();
+}
+
+// Colouring and constraint data for the per-scene solve: each scene coloured by one workgroup in
+// LDS, then the same stable colour scatter and constraint data (without predecessor slots).
+hipError_t launchColourPrepScene(hipStream_t st, DeviceWorld& w) {
+    const uint32_t gm = blocksForManifolds(w);
+    hipLaunchKernelGGL(kSceneManClear, dim3(blocksFor(w.S)), dim3(256), 0, st, w.sceneManRange, w.S);
+    hipLaunchKernelGGL(kSceneManRanges, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colScene, w.sceneManRange);
+    hipLaunchKernelGGL(kClearManBodies, dim3(gm), dim3(256), 0, st, w.ctr, w.manBodies);
+    const uint32_t maxBodies = std::max<uint32_t>(w.sceneMaxBodies, 1);
+    const uint32_t lds = w.sceneColourLdsBytes;
+    const uint32_t capM = (lds - 5 * maxBodies * (uint32_t)sizeof(uint32_t)) / 6;
+    const uint32_t layout = (maxBodies << 16) | std::min<uint32_t>(std::min<uint32_t>(capM, w.sceneColourCap), 0xFFFFu);
+    hipLaunchKernelGGL(kSceneColour, dim3(w.S), dim3(COLOUR_THREADS), lds, st, w.man[w.cur], w.ctr, w.colBody, w.inv,
+                       w.sceneBodyStart, w.sceneManRange, layout, w.manBodies, w.degree, w.rank, w.colourId);
+    hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
+    hipLaunchKernelGGL(kColourColumnScan, dim3(MAX_COLOURS), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
+    hipLaunchKernelGGL(kColourStarts, dim3(1), dim3(64), 0, st, w.ctr);
+    hipLaunchKernelGGL(kColourScatter, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist, w.order,
+                       w.posOf);
+    hipLaunchKernelGGL(kMarkHoles, dim3((MAX_COLOURS * COLOUR_ALIGN + 255) / 256), dim3(256), 0, st, w.ctr, w.order, w.cs);
+    hipLaunchKernelGGL(kPrep<false>, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.manBodies, w.posOf, w.degree, w.rank,
                        w.incStart, w.incSorted, w.inv, w.pos, w.vel, w.shape, w.materials, w.cs);
     return hipGetLastError();
 }
@@ -1309,8 +1515,9 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
     return hipGetLastError();
 }
 
-uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks) {
+uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks, uint32_t& colourLdsBytes) {
     blocks = 0;
+    colourLdsBytes = 0;
     const size_t bytes = std::max<size_t>(maxBodies, 1) * sizeof(float4);
     hipDeviceProp_t prop{};
     int dev = 0;
@@ -1321,6 +1528,19 @@ uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks) {
     if (bytes > 48 * 1024 &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneSolve), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    // the per-scene colouring: 20 bytes per body of the largest scene, then as many manifolds
+    // (6 bytes each) as the rest of the LDS holds
+    if (maxBodies >= 0x8000u) return 0;   // local body indices are packed in 15 bits
+    const size_t colourBodies = std::max<size_t>(maxBodies, 1) * 5 * sizeof(uint32_t);
+    if (colourBodies + 6 * 256 + 64 > limit) return 0;
+    const size_t colourBytes = std::min<size_t>(limit - 64, colourBodies + 6 * 0xFFFFull) & ~(size_t)15;
+    colourLdsBytes = (uint32_t)colourBytes;
+    if (colourBytes > 48 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneColour), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)colourBytes) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }

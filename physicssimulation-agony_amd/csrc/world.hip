@@ -265,9 +265,12 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
             const bool gated = predict && !(w->learnMode & AGX_LEARN_UNGATED);
             // the gate judges each step by its residual, so a gated world always tracks it
             const int trackResidual = (s.track_residual || gated) ? 1 : 0;
+            const bool perScene = d.sceneLdsBytes > 0 && s.solve_per_scene != 0;
+            w->solvedPerScene = perScene;
             {
                 SegmentTimer t(w, SEG_COLOUR);
-                HIP_CHECK(w, agx::launchColourPrep(st, d));
+                if (perScene) HIP_CHECK(w, agx::launchColourPrepScene(st, d));
+                else HIP_CHECK(w, agx::launchColourPrep(st, d));
             }
             if (predict || collect) {
                 SegmentTimer t(w, SEG_LEARN);
@@ -277,13 +280,10 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
                                                         collect ? 1u : 0u, newOnly, s.gravity_x, s.gravity_y, dt));
             }
             w->histReady = predict || collect;
-            {
+            if (!perScene) {   // the per-scene solve applies the warm start itself
                 SegmentTimer t(w, SEG_COLOUR);
                 HIP_CHECK(w, agx::launchWarmStart(st, d));
             }
-            const bool perScene = d.sceneLdsBytes > 0 &&
-                                  (s.solve_per_scene == 2 || (s.solve_per_scene == 1 && d.S > 1));
-            w->solvedPerScene = perScene;
             {
                 SegmentTimer t(w, SEG_SOLVE);
                 if (perScene)
@@ -633,8 +633,11 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     std::stable_sort(sceneOrder.begin(), sceneOrder.end(), [&](uint32_t a, uint32_t b) {
         return sceneBodyStart[a + 1] - sceneBodyStart[a] > sceneBodyStart[b + 1] - sceneBodyStart[b];
     });
-    d.sceneLdsBytes = agx::sceneSolveSetup(maxSceneBodies, d.numCU, d.sceneBlocks);
+    d.sceneLdsBytes = agx::sceneSolveSetup(maxSceneBodies, d.numCU, d.sceneBlocks, d.sceneColourLdsBytes);
     d.sceneMaxBodies = maxSceneBodies;
+    // AGX_COLOUR_LDS_CAP lowers how many of a scene's manifolds the per-scene colouring keeps in
+    // LDS (larger scenes colour from global scratch instead); tests use it to reach that path
+    if (const char* cap = std::getenv("AGX_COLOUR_LDS_CAP")) d.sceneColourCap = (uint32_t)std::strtoul(cap, nullptr, 10);
 
 #define UP(p, v) do { if (upload(w, &(p), (v)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "upload " #p)); } while (0)
 #define AL(p, n) do { if (devAlloc(w, &(p), (n)) != hipSuccess) return bail(fail(nullptr, AGX_ERR_HIP, "alloc " #p)); } while (0)
@@ -645,7 +648,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     UP(d.verts, verts); UP(d.normals, normals); UP(d.materials, mats);
     UP(d.sceneBodyStart, sceneBodyStart); UP(d.sceneOrder, sceneOrder); UP(d.sceneDyn, sceneDyn);
     AL(d.sceneRange, (uint64_t)d.S * agx::MAX_COLOURS); AL(d.sceneRes, (uint64_t)d.S * agx::MAX_SWEEPS);
-    AL(d.sceneSweeps, d.S); AL(d.sceneIslands, d.S); AL(d.sceneIslandOffset, d.S);
+    AL(d.sceneSweeps, d.S); AL(d.sceneIslands, d.S); AL(d.sceneIslandOffset, d.S); AL(d.sceneManRange, d.S);
     AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N);
     AL(d.pose, d.C); AL(d.wRot, d.C); AL(d.prevW, d.C); AL(d.aabb, d.C);
     if ((rc = allocTree(w, d.dyn, dynList, colScene, d.S))) return bail(rc);

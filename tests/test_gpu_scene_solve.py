@@ -40,6 +40,11 @@ def test_per_scene_solve_matches_dataflow_bit_for_bit(agx, scenes):
         flow.step(DT, n)
         per.step_graph(DT, n)
         _same(flow, per)
+        # the per-scene colouring (rounds in LDS) gives the dataflow colouring's colours exactly
+        of, cf = flow.solve_order()
+        op, cp = per.solve_order()
+        np.testing.assert_array_equal(cf, cp)
+        np.testing.assert_array_equal(of, op)
     sf, sp = flow.stats(), per.stats()
     assert sf.solved_per_scene == 0 and sp.solved_per_scene == 1
     assert sf.overflow == 0 and sp.overflow == 0 and sp.manifolds > 0
@@ -47,6 +52,22 @@ def test_per_scene_solve_matches_dataflow_bit_for_bit(agx, scenes):
     assert sp.total_body_sweeps == sf.total_body_sweeps
     np.testing.assert_allclose(sp.residual[:8], sf.residual[:8], rtol=1e-4)
     assert sp.velocity_error == pytest.approx(sf.velocity_error, rel=1e-4)
+
+
+def test_per_scene_colouring_from_global_scratch(agx, scenes, monkeypatch):
+    """Scenes with more manifolds than the colouring keeps in LDS colour from global scratch
+    (forced here with a tiny LDS cap): still the dataflow colouring's colours and bits."""
+    parts = scenes.batch(16)
+    base = dict(velocity_iterations=6, position_iterations=2)
+    flow = _world(agx, parts, solve_per_scene=0, **base)
+    monkeypatch.setenv("AGX_COLOUR_LDS_CAP", "64")
+    per = _world(agx, parts, solve_per_scene=1, **base)
+    for n in (1, 6):
+        flow.step(DT, n)
+        per.step(DT, n)
+        _same(flow, per)
+        np.testing.assert_array_equal(flow.solve_order()[0], per.solve_order()[0])
+    assert per.stats().solved_per_scene == 1
 
 
 def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes):
@@ -109,7 +130,11 @@ def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes):
     assert st.mean_velocity_sweeps < st.velocity_sweeps
 
 
-def test_single_scene_defaults_to_the_dataflow(agx, scenes):
-    w = _world(agx, [scenes.pile_small(seed=1)], velocity_iterations=8)
-    w.step(DT, 3)
-    assert w.stats().solved_per_scene == 0
+def test_scene_too_large_for_lds_uses_the_dataflow(agx, scenes):
+    small = _world(agx, [scenes.pile_small(seed=1)], velocity_iterations=8)
+    small.step(DT, 3)
+    assert small.stats().solved_per_scene == 1
+    big = _world(agx, [scenes.pile(9000, 3000, 140.0, 140.0, seed=3)], velocity_iterations=8)
+    assert big.counts.bodies > 11000
+    big.step(DT, 3)
+    assert big.stats().solved_per_scene == 0 and big.stats().overflow == 0
