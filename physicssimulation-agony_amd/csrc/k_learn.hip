@@ -164,7 +164,8 @@ __device__ inline uint32_t claimRowsBlock(uint32_t count, uint32_t* counter) {
 
 // Records every manifold's starting accumulators (the next step's history) and, when the network
 // runs or rows are collected, the feature rows: two per manifold at rows 2m and 2m + 1, or with
-// newOnly one per contact point without a cached impulse, in claimed rows with map[row] = 2m + p.
+// newOnly (no history kept) one per contact point without a cached impulse, in claimed rows with
+// map[row] = 2m + p.
 // Runs over solve positions, so the constraint planes are read in order (holes are skipped).
 __global__ __launch_bounds__(256) void kLearnFeatures(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
                                                       const uint32_t* __restrict__ degree, const float4* __restrict__ vel,
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(256) void kLearnFeatures(Counters* __restrict__ ctr
             id = cs.idx[c];
             if (id.z != 0) {
                 j = cs.imp[c];
-                warmHist[id.w] = j;
+                if (!newOnly) warmHist[id.w] = j;   // new-contacts mode keeps no history
             }
         }
         if (!rows) continue;
@@ -209,7 +210,8 @@ __global__ __launch_bounds__(256) void kLearnFeatures(Counters* __restrict__ ctr
         uint32_t row = claimRowsBlock(__builtin_popcount(keep), &ctr->learnCompactRows);
         if (!keep) continue;
         const float depth = man[m].depth;
-        const float4 hist = histOf[m];
+        const float none = __builtin_nanf("");
+        const float4 hist = make_float4(none, none, none, none);
         for (uint32_t p = 0; p < 2; p++) {
             if (!(keep >> p & 1u)) continue;
             pointFeatures(cs, c, p, vel, degree, depth, hist, sc, f);

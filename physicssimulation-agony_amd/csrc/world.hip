@@ -54,6 +54,7 @@ struct agx_world {
     agx_mlp* mlp = nullptr;          // learned warm start (not owned)
     uint32_t learnMode = 0;          // AGX_LEARN_* flags
     bool histReady = false;          // the previous step recorded its manifolds' starting accumulators
+    bool learnedNarrow = false;      // this step's narrow phase ran with a network set
     int colourPerCuMax = 1, solvePerCuMax = 1;   // launch-shape limits (occupancy, allocations)
     // step graphs (agx_world_step_graph): one per ping-pong parity of the manifold buffers
     hipStream_t captureStream = nullptr;
@@ -252,8 +253,12 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
     }
     case AGX_PHASE_NARROW: {
         const bool learned = w->mlp && !s.collide_only;
-        d.histOut = learned ? d.histOf : nullptr;
-        d.histIn = (learned && w->histReady) ? d.warmHist[1 - d.cur] : nullptr;
+        // accumulator history (features 8, 17, 18) only for the network at every point: a point
+        // without a cached impulse has practically never had one two steps ago
+        const bool hist = learned && !(w->learnMode & AGX_LEARN_NEW_CONTACTS);
+        d.histOut = hist ? d.histOf : nullptr;
+        d.histIn = (hist && w->histReady) ? d.warmHist[1 - d.cur] : nullptr;
+        w->learnedNarrow = learned;
         SegmentTimer t(w, SEG_NARROW);
         HIP_CHECK(w, agx::launchNarrow(st, d));
         break;
@@ -274,12 +279,12 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
             }
             if (predict || collect) {
                 SegmentTimer t(w, SEG_LEARN);
-                if (!d.histOut) return fail(w, AGX_ERR_STATE, "learned mode was enabled after this step's narrow phase");
+                if (!w->learnedNarrow) return fail(w, AGX_ERR_STATE, "learned mode was enabled after this step's narrow phase");
                 const uint32_t newOnly = (w->learnMode & AGX_LEARN_NEW_CONTACTS) ? 1u : 0u;
                 HIP_CHECK(w, agx::launchLearnBeforeWarm(st, d, agx::mlpDevice(w->mlp), predict ? 1u : 0u, gated ? 1u : 0u,
                                                         collect ? 1u : 0u, newOnly, s.gravity_x, s.gravity_y, dt));
             }
-            w->histReady = predict || collect;
+            w->histReady = (predict || collect) && !(w->learnMode & AGX_LEARN_NEW_CONTACTS);
             if (!perScene) {   // the per-scene solve applies the warm start itself
                 SegmentTimer t(w, SEG_COLOUR);
                 HIP_CHECK(w, agx::launchWarmStart(st, d));

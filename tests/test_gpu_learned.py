@@ -201,7 +201,8 @@ def test_trained_network_needs_fewer_sweeps_than_cached_warm_start(agx, scenes):
 def test_new_contacts_mode_rows_are_the_uncached_points(agx, scenes):
     """In new-contacts mode a world collects one row per contact point without a cached impulse:
     exactly the rows with feature 4 (has a cached impulse) equal to 0 that full collection gives on
-    an identical world; a network whose correction is +0.5 S then changes the step."""
+    an identical world, except that the mode keeps no accumulator history (features 8, 17 and 18
+    read "none"); a network whose correction is +0.5 S then changes the step."""
     dt = 1.0 / 60.0
     full, new = _world(agx, scenes), _world(agx, scenes)
     for w in (full, new):
@@ -217,7 +218,11 @@ def test_new_contacts_mode_rows_are_the_uncached_points(agx, scenes):
     xn, tn = mn.samples()
     assert 0 < len(xn) < len(xf)
     assert (xn[:, 4] == 0).all() and (xn[:, 0] == 0).all() and (xn[:, 1] == 0).all()
+    hist = [8, 17, 18]
+    assert (xn[:, hist] == 0).all()
     sel = xf[:, 4] == 0
+    xf = xf.copy()
+    xf[:, hist] = 0
     key = lambda x, t: np.lexsort(np.concatenate([x, t], 1).T[::-1])
     a, b = np.concatenate([xf[sel], tf[sel]], 1), np.concatenate([xn, tn], 1)
     np.testing.assert_array_equal(a[key(xf[sel], tf[sel])], b[key(xn, tn)])
