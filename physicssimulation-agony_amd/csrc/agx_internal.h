@@ -92,7 +92,8 @@ struct Counters {
     uint32_t learnUsed;        // this step's warm start came from the network
     uint32_t sampleBase;       // first training row this step claimed
     uint32_t learnCompactRows; // new-contacts mode: rows written (one per contact point without a cached impulse)
-    uint32_t sceneCursor;      // per-scene solve: scenes taken by its blocks
+    uint32_t sceneCursor;      // per-scene solve: scenes taken by whole blocks
+    uint32_t sceneCursor2;     // per-scene solve: scenes taken by lane groups
     uint32_t perScene;         // this step was solved scene by scene
     unsigned long long bodySweeps;   // this step: dynamic bodies x the sweeps of their scene
 };
@@ -158,6 +159,15 @@ struct Constraints {
     float4* k0;      // normalMass0, tangentMass0, bias0, normalMass1
     float4* k1;      // tangentMass1, bias1, penetrationBase, unused
     float4* imp;     // jn0, jt0, jn1, jt1
+    float4* blk;     // two points: K00, K11, K01 of the 2x2 normal block and 1/det (NaN when the
+                     // block solve is skipped as ill-conditioned); zero for one point
+};
+
+// Launch shape of the per-scene solve: `blocks` resident blocks (one per CU) first take the `whole`
+// largest scenes one each, then split into `groups` lane groups of `groupBodies` LDS body records.
+struct SceneLaunch {
+    int blocks = 0;
+    uint32_t whole = 0, groups = 1, groupBodies = 0;
 };
 
 struct DeviceWorld {
@@ -245,7 +255,7 @@ struct DeviceWorld {
     uint32_t sceneLdsBytes = 0;           // dynamic LDS of the per-scene solve (0: some scene does not fit)
     uint32_t sceneColourLdsBytes = 0;     // dynamic LDS of the per-scene colouring
     uint32_t sceneColourCap = 0xFFFFu;    // most manifolds of a scene the colouring keeps in LDS
-    int sceneBlocks = 0;                  // resident blocks of the per-scene solve
+    SceneLaunch sceneLaunch;              // launch shape of the per-scene solve
     float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
     Constraints cs{};
     // scan scratch
@@ -405,7 +415,8 @@ hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // back to the manifolds, residual and error totals.
 // Per-scene solve: LDS bytes for scenes of up to maxBodies bodies (0 when they do not fit) and the
 // resident blocks for that size.
-uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks, uint32_t& colourLdsBytes);
+uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, uint32_t wholeCap, SceneLaunch& out,
+                         uint32_t& colourLdsBytes);
 hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                             float residualTarget);
 // Colouring (one workgroup per scene, in LDS) and constraint data without predecessor slots.

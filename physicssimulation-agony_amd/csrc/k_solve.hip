@@ -470,6 +470,14 @@ __global__ __launch_bounds__(256) void kColourScatter(const Counters* __restrict
     }
 }
 
+// 1/det of a two-point contact's 2x2 normal block, or NaN when the block is too ill-conditioned for
+// the direct solve (BodyCollisionSolver.cpp block solver: maxK^2 < 1000 det).
+__device__ inline float blockInvDet(float K00, float K11, float K01) {
+    const float det = K00 * K11 - K01 * K01;
+    const float maxK = fmaxf(K00, K11);
+    return maxK * maxK < 1000.0f * det ? 1.0f / det : __builtin_nanf("");
+}
+
 // Predecessor of a constraint at one of its dynamic bodies in the solve order: the body's previous
 // constraint in the same sweep, or, for the body's first constraint, its last constraint of the
 // previous sweep (flagged PRED_PREV_SWEEP). NO_PRED: a static body.
@@ -537,6 +545,15 @@ __global__ __launch_bounds__(256) void kPrep(const agx_manifold* __restrict__ ma
         cs.k0[slot] = make_float4(pr[0][4], pr[0][5], pr[0][6], pr[1][4]);
         cs.k1[slot] = make_float4(pr[1][5], pr[1][6], penBase, 0.0f);
         cs.imp[slot] = make_float4(d.jn0, d.jt0, d.count > 1 ? d.jn1 : 0.0f, d.count > 1 ? d.jt1 : 0.0f);
+        float4 blk = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (d.count > 1) {
+            // the operations of the block solve in velocityStep, on the same values: the same bits
+            const float K00 = effMass + pr[0][0] * pr[0][0] * iA.y + pr[0][1] * pr[0][1] * iB.y;
+            const float K11 = effMass + pr[1][0] * pr[1][0] * iA.y + pr[1][1] * pr[1][1] * iB.y;
+            const float K01 = effMass + pr[0][0] * pr[1][0] * iA.y + pr[0][1] * pr[1][1] * iB.y;
+            blk = make_float4(K00, K11, K01, blockInvDet(K00, K11, K01));
+        }
+        cs.blk[slot] = blk;
     }
 }
 
@@ -574,8 +591,9 @@ __global__ __launch_bounds__(256) void kWarm(const uint32_t* __restrict__ dynBod
 // ---------------------------------------------------------------- dataflow sweeps
 struct Item {
     uint4 id;     // bodyA, bodyB, count, manifold
-    uint4 dep;    // predecessor at A, predecessor at B, A dynamic, B dynamic
+    uint4 dep;    // predecessor at A, predecessor at B, A dynamic, B dynamic (dataflow)
     float4 nrm, iv, r0, r1, k0, k1, imp;
+    float4 blk;   // the precomputed 2x2 block (per-scene solve)
 };
 
 __device__ inline void loadItem(const Constraints& cs, uint32_t c, Item& it) {
@@ -584,10 +602,20 @@ __device__ inline void loadItem(const Constraints& cs, uint32_t c, Item& it) {
     it.k0 = cs.k0[c]; it.k1 = cs.k1[c]; it.imp = cs.imp[c];
 }
 
+// The per-scene solve's record: the block instead of the predecessors (a body is dynamic iff its
+// inverse mass is not zero, which is what the dynamic flags of dep hold).
+__device__ inline void loadSceneItem(const Constraints& cs, uint32_t c, Item& it) {
+    it.id = cs.idx[c]; it.blk = cs.blk[c];
+    it.nrm = cs.nrm[c]; it.iv = cs.inv[c]; it.r0 = cs.r0[c]; it.r1 = cs.r1[c];
+    it.k0 = cs.k0[c]; it.k1 = cs.k1[c]; it.imp = cs.imp[c];
+}
+
 // solveVelocityConstraints (BodyCollisionSolver.cpp:492-928): one point = clamped normal step
 // (skipped while separating with zero impulse) + iterative friction; two points = 2x2 block LCP
 // (4 cases, iterative fallback) + iterative friction per point. Adds the squared accumulator
-// changes and the squared accumulators to dsq and lsq.
+// changes and the squared accumulators to dsq and lsq. BLK: the block's K and 1/det come from
+// it.blk (computed by kPrep with the same operations) instead of being recomputed.
+template <bool BLK>
 __device__ inline void velocityStep(Item& it, float4& A, float4& B, float& dsq, float& lsq) {
     float4 j = it.imp;
     const float4 j0 = j;
@@ -631,19 +659,17 @@ __device__ inline void velocityStep(Item& it, float4& A, float4& B, float& dsq, 
         const float dvn = (B.x - A.x) * nx + (B.y - A.y) * ny;
         const float vn0 = dvn + r0.y * B.z - r0.x * A.z;
         const float vn1 = dvn + r1.y * B.z - r1.x * A.z;
-        const float K00 = effMass + r0.x * r0.x * iIA + r0.y * r0.y * iIB;
-        const float K11 = effMass + r1.x * r1.x * iIA + r1.y * r1.y * iIB;
-        const float K01 = effMass + r0.x * r1.x * iIA + r0.y * r1.y * iIB;
+        const float K00 = BLK ? it.blk.x : effMass + r0.x * r0.x * iIA + r0.y * r0.y * iIB;
+        const float K11 = BLK ? it.blk.y : effMass + r1.x * r1.x * iIA + r1.y * r1.y * iIB;
+        const float K01 = BLK ? it.blk.z : effMass + r0.x * r1.x * iIA + r0.y * r1.y * iIB;
         const float b0 = k0.z - vn0 + K00 * j.x + K01 * j.z;
         const float b1 = k1.y - vn1 + K01 * j.x + K11 * j.z;
         float x0 = 0.0f, x1 = 0.0f;
         bool ok = false;
         {
             const float EPS = 1e-6f;
-            const float det = K00 * K11 - K01 * K01;
-            const float maxK = fmaxf(K00, K11);
-            if (maxK * maxK < 1000.0f * det) {
-                const float invDet = 1.0f / det;
+            const float invDet = BLK ? it.blk.w : blockInvDet(K00, K11, K01);
+            if (invDet == invDet) {
                 const float s0 = (K11 * b0 - K01 * b1) * invDet;
                 const float s1 = (K00 * b1 - K01 * b0) * invDet;
                 if (s0 >= -EPS && s1 >= -EPS) { x0 = fmaxf(s0, 0.0f); x1 = fmaxf(s1, 0.0f); ok = true; }
@@ -817,7 +843,7 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
                     if (has) {
                         float4 A = it.dep.z ? loadCoherent(vel + it.id.x) : vel[it.id.x];
                         float4 B = it.dep.w ? loadCoherent(vel + it.id.y) : vel[it.id.y];
-                        velocityStep(it, A, B, dsq, lsq);
+                        velocityStep<false>(it, A, B, dsq, lsq);
                         cs.imp[c] = it.imp;
                         if (it.dep.z) storeCoherent(vel + it.id.x, A);
                         if (it.dep.w) storeCoherent(vel + it.id.y, B);
@@ -966,152 +992,216 @@ __global__ __launch_bounds__(256) void kSceneRanges(const Counters* __restrict__
     }
 }
 
-constexpr uint32_t SCENE_THREADS = 512;
+constexpr uint32_t SCENE_THREADS = 1024;   // one block per CU
+constexpr uint32_t SCENE_GROUPS = 4;       // lane groups of a block for the smaller scenes
 
-__device__ inline double sceneBlockSum(double v, double* red) {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t off = SCENE_THREADS / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
+// The lanes that solve one scene: the whole block (block barriers) for the largest scenes, else a
+// group of SCENE_THREADS / groups lanes, whose waves meet at an arrival counter in LDS.
+struct SceneLanes {
+    uint32_t t;        // lane in the group
+    uint32_t T;        // lanes in the group
+    uint32_t g;        // group (0 for the whole block)
+    uint32_t waves;    // waves in the group
+    bool whole;
+    uint32_t* bar;     // the group's arrival counter
+    uint32_t* abort;   // the block's flag: a wait timed out, no sync waits any more
+    uint32_t target;   // arrivals the next sync waits for
+    Counters* ctr;
+
+    __device__ void sync() {
+        if (whole) { __syncthreads(); return; }
+        target += waves;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if ((threadIdx.x & (WAVE - 1)) == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // bounded: after about 2 s of 64-cycle sleeps the step is flagged invalid and every later
+        // sync of the block returns at once, so the kernel ends
+        for (uint32_t spins = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target;) {
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins == (1u << 26)) {
+                __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                atomicOr(&ctr->overflow, OVERFLOW_STALL);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    const double t = red[0];
-    __syncthreads();
-    return t;
+};
+
+// Sum of (num, den) over the group's lanes: a butterfly within each wave (every lane ends with
+// the same bits), then the waves' partials in wave order; every lane returns the total.
+__device__ inline double2 sceneSum(SceneLanes& L, double num, double den, double2* part) {
+    for (uint32_t off = WAVE / 2; off > 0; off >>= 1) {
+        num += __shfl_xor(num, off, WAVE);
+        den += __shfl_xor(den, off, WAVE);
+    }
+    const uint32_t w = L.t / WAVE;
+    if ((L.t & (WAVE - 1)) == 0) part[w] = make_double2(num, den);
+    L.sync();
+    double2 tot = part[0];
+    for (uint32_t k = 1; k < L.waves; k++) { tot.x += part[k].x; tot.y += part[k].y; }
+    L.sync();   // before the partials are written again
+    return tot;
 }
 
-// Blocks take scenes largest first from a shared cursor until none is left; a scene's results do
-// not depend on which block solves it.
+// One scene (bodies [b0, b1)) by the lanes L, body records in `body`: warm start, velocity sweeps
+// until the residual target or vIt, position sweeps; returns the velocity sweeps run.
+__device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __restrict__ ctr, const Constraints& cs,
+                                      float4* __restrict__ vel, float4* __restrict__ pos, uint32_t b0, uint32_t b1,
+                                      const uint2* __restrict__ range, float2* __restrict__ sceneRes, uint32_t K,
+                                      uint32_t vIt, uint32_t pIt, int track, float residualTarget, float4* body,
+                                      uint2* rng, double2* part) {
+    const uint32_t t = L.t, T = L.T;
+    for (uint32_t k = t; k < K; k += T) rng[k] = range[(size_t)s * MAX_COLOURS + k];
+    for (uint32_t b = b0 + t; b < b1; b += T) body[b - b0] = vel[b];
+    L.sync();
+    // applyWarmStarting (BodyCollisionSolver.cpp:416-490) colour by colour: every body adds its
+    // constraints' impulses in solve order, with the operations of kWarm
+    for (uint32_t k = 0; k < K; k++) {
+        const uint2 r = rng[k];
+        for (uint32_t c = r.x + t; c < r.y; c += T) {
+            const uint4 id = cs.idx[c];
+            if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
+            const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], r1 = cs.r1[c], j = cs.imp[c];
+            const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
+            const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
+            const float sn = jn0 + jn1, st = jt0 + jt1;
+            const float sx = sn * nx + st * tx, sy = sn * ny + st * ty;
+            if (iv.x != 0.0f) {
+                float4 v = body[id.x - b0];
+                v.x = v.x - sx * iv.x; v.y = v.y - sy * iv.x;
+                v.z -= (jn0 * r0.x + jt0 * r0.z + jn1 * r1.x + jt1 * r1.z) * iv.z;
+                body[id.x - b0] = v;
+            }
+            if (iv.y != 0.0f) {
+                float4 v = body[id.y - b0];
+                v.x = v.x + sx * iv.y; v.y = v.y + sy * iv.y;
+                v.z += (jn0 * r0.y + jt0 * r0.w + jn1 * r1.y + jt1 * r1.w) * iv.w;
+                body[id.y - b0] = v;
+            }
+        }
+        L.sync();
+    }
+    uint32_t done = vIt;
+    for (uint32_t q = 0; q < vIt; q++) {
+        float dsq = 0.0f, lsq = 0.0f;
+        // This lane's slots, colour by colour: rng[k].x + t, + T, ... below rng[k].y. The constraint
+        // after the current one is loaded before the current one is solved, also across the sync
+        // that ends a colour: only this lane writes a constraint's accumulators, and nothing else in
+        // a constraint's record changes during the solve.
+        uint32_t pk = 0, pc = K ? rng[0].x + t : 0u;
+        auto seek = [&]() {
+            while (pk < K && pc >= rng[pk].y) {
+                pk++;
+                if (pk < K) pc = rng[pk].x + t;
+            }
+        };
+        seek();
+        Item next;
+        if (pk < K) loadSceneItem(cs, pc, next);
+        for (uint32_t k = 0; k < K; k++) {
+            while (pk == k) {
+                Item it = next;
+                const uint32_t c = pc;
+                pc += T;
+                seek();
+                if (pk < K) loadSceneItem(cs, pc, next);
+                if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) continue;   // a hole or another scene
+                if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
+                    atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
+                    continue;
+                }
+                float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
+                velocityStep<true>(it, A, B, dsq, lsq);
+                cs.imp[c] = it.imp;
+                if (it.iv.x != 0.0f) body[it.id.x - b0] = A;
+                if (it.iv.y != 0.0f) body[it.id.y - b0] = B;
+            }
+            L.sync();
+        }
+        const double2 r = sceneSum(L, (double)dsq, (double)lsq, part);
+        if (t == 0 && track) sceneRes[(size_t)s * MAX_SWEEPS + q] = make_float2((float)r.x, (float)r.y);
+        if (residualTarget > 0.0f && sqrt(r.x) <= (double)residualTarget * fmax(sqrt(r.y), 1e-12)) { done = q + 1; break; }
+    }
+    for (uint32_t b = b0 + t; b < b1; b += T) vel[b] = body[b - b0];
+    L.sync();
+    if (pIt) {
+        float2* bp = reinterpret_cast<float2*>(body);
+        for (uint32_t b = b0 + t; b < b1; b += T) {
+            const float4 x = pos[b];
+            bp[b - b0] = make_float2(x.x, x.y);
+        }
+        L.sync();
+        for (uint32_t q = 0; q < pIt; q++)
+            for (uint32_t k = 0; k < K; k++) {
+                const uint2 r = rng[k];
+                for (uint32_t c = r.x + t; c < r.y; c += T) {
+                    const uint4 id = cs.idx[c];
+                    if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
+                    PosItem pi;
+                    loadPosItem(cs, c, pi);
+                    float2 A = bp[id.x - b0], B = bp[id.y - b0];
+                    positionStep(pi, A, B);
+                    if (pi.iMA != 0.0f) bp[id.x - b0] = A;
+                    if (pi.iMB != 0.0f) bp[id.y - b0] = B;
+                }
+                L.sync();
+            }
+        for (uint32_t b = b0 + t; b < b1; b += T) reinterpret_cast<float2*>(pos + b)[0] = bp[b - b0];
+    }
+    L.sync();   // before the next scene reuses the LDS
+    return done;
+}
+
+// Scenes are taken largest first: the first nWhole (one per block) by whole blocks, the rest by the
+// blocks' lane groups (groups per block, each with groupBodies body records of LDS) from a second
+// cursor. A scene's results do not depend on which block or group solves it.
 __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restrict__ ctr, Constraints cs, float4* __restrict__ vel,
                                                              float4* __restrict__ pos, const uint32_t* __restrict__ sceneBodyStart,
                                                              const uint32_t* __restrict__ sceneOrder,
                                                              const uint32_t* __restrict__ sceneDyn, const uint2* __restrict__ range,
                                                              float2* __restrict__ sceneRes, uint32_t* __restrict__ sceneSweeps,
-                                                             uint32_t S, uint32_t vIt, uint32_t pIt, int track,
-                                                             float residualTarget) {
-    extern __shared__ float4 body[];   // the scene's body velocities, then positions (x, y)
-    __shared__ uint2 rng[MAX_COLOURS];
-    __shared__ double red[SCENE_THREADS];
-    __shared__ uint32_t taken;
-    __shared__ int stop;
+                                                             uint32_t S, uint32_t nWhole, uint32_t groups, uint32_t groupBodies,
+                                                             uint32_t vIt, uint32_t pIt, int track, float residualTarget) {
+    extern __shared__ float4 body[];   // the scene's body velocities, then positions (x, y); per group
+    __shared__ uint2 rng[SCENE_GROUPS][MAX_COLOURS];
+    __shared__ double2 part[SCENE_THREADS / WAVE];
+    __shared__ uint32_t taken[SCENE_GROUPS];
+    __shared__ uint32_t bar[SCENE_GROUPS];
+    __shared__ uint32_t abort;
     const uint32_t K = ctr->colours;
-    const uint32_t t = threadIdx.x;
-    while (true) {
-        if (t == 0) taken = atomicAdd(&ctr->sceneCursor, 1u);
-        __syncthreads();
-        const uint32_t k0 = taken;
+    if (threadIdx.x < SCENE_GROUPS) bar[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) abort = 0u;
+    SceneLanes L{ threadIdx.x, SCENE_THREADS, 0u, SCENE_THREADS / WAVE, true, nullptr, &abort, 0u, ctr };
+    float4* lds = body;
+    bool grouped = false;
+    while (true) {   // one call site of solveScene for both phases
+        if (L.t == 0) taken[L.g] = grouped ? nWhole + atomicAdd(&ctr->sceneCursor2, 1u) : atomicAdd(&ctr->sceneCursor, 1u);
+        L.sync();
+        const uint32_t k0 = taken[L.g];
+        if (!grouped && k0 >= nWhole) {
+            __syncthreads();   // every wave has read taken[0] before the groups reuse it
+            if (nWhole >= S) break;
+            const uint32_t T = SCENE_THREADS / groups;
+            L.g = threadIdx.x / T;
+            L.t = threadIdx.x % T;
+            L.T = T;
+            L.waves = T / WAVE;
+            L.whole = groups == 1;
+            L.bar = &bar[L.g];
+            lds = body + (size_t)L.g * groupBodies;
+            grouped = true;
+            continue;
+        }
         if (k0 >= S) break;
         const uint32_t s = sceneOrder[k0];
-        const uint32_t b0 = sceneBodyStart[s], b1 = sceneBodyStart[s + 1];
-        for (uint32_t k = t; k < K; k += SCENE_THREADS) rng[k] = range[(size_t)s * MAX_COLOURS + k];
-        for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) body[b - b0] = vel[b];
-        __syncthreads();
-        // applyWarmStarting (BodyCollisionSolver.cpp:416-490) colour by colour: every body adds its
-        // constraints' impulses in solve order, with the operations of kWarm
-        for (uint32_t k = 0; k < K; k++) {
-            const uint2 r = rng[k];
-            for (uint32_t c = r.x + t; c < r.y; c += SCENE_THREADS) {
-                const uint4 id = cs.idx[c];
-                if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
-                const uint4 dep = cs.dep[c];
-                const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], r1 = cs.r1[c], j = cs.imp[c];
-                const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
-                const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
-                const float sn = jn0 + jn1, st = jt0 + jt1;
-                const float sx = sn * nx + st * tx, sy = sn * ny + st * ty;
-                if (dep.z) {
-                    float4 v = body[id.x - b0];
-                    v.x = v.x - sx * iv.x; v.y = v.y - sy * iv.x;
-                    v.z -= (jn0 * r0.x + jt0 * r0.z + jn1 * r1.x + jt1 * r1.z) * iv.z;
-                    body[id.x - b0] = v;
-                }
-                if (dep.w) {
-                    float4 v = body[id.y - b0];
-                    v.x = v.x + sx * iv.y; v.y = v.y + sy * iv.y;
-                    v.z += (jn0 * r0.y + jt0 * r0.w + jn1 * r1.y + jt1 * r1.w) * iv.w;
-                    body[id.y - b0] = v;
-                }
-            }
-            __syncthreads();
-        }
-        uint32_t done = vIt;
-        for (uint32_t q = 0; q < vIt; q++) {
-            float dsq = 0.0f, lsq = 0.0f;
-            // This thread's slots, colour by colour: rng[k].x + t, + SCENE_THREADS, ... below rng[k].y.
-            // The constraint after the current one is loaded before the current one is solved, also
-            // across the barrier that ends a colour: only this thread writes a constraint's
-            // accumulators, and nothing else in a constraint's record changes during the solve.
-            uint32_t pk = 0, pc = K ? rng[0].x + t : 0u;
-            auto seek = [&]() {
-                while (pk < K && pc >= rng[pk].y) {
-                    pk++;
-                    if (pk < K) pc = rng[pk].x + t;
-                }
-            };
-            seek();
-            Item next;
-            if (pk < K) loadItem(cs, pc, next);
-            for (uint32_t k = 0; k < K; k++) {
-                while (pk == k) {
-                    Item it = next;
-                    const uint32_t c = pc;
-                    pc += SCENE_THREADS;
-                    seek();
-                    if (pk < K) loadItem(cs, pc, next);
-                    if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) continue;   // a hole or another scene
-                    if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
-                        atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
-                        continue;
-                    }
-                    float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
-                    velocityStep(it, A, B, dsq, lsq);
-                    cs.imp[c] = it.imp;
-                    if (it.dep.z) body[it.id.x - b0] = A;
-                    if (it.dep.w) body[it.id.y - b0] = B;
-                }
-                __syncthreads();
-            }
-            const double num = sceneBlockSum((double)dsq, red);
-            const double den = sceneBlockSum((double)lsq, red);
-            if (t == 0) {
-                if (track) sceneRes[(size_t)s * MAX_SWEEPS + q] = make_float2((float)num, (float)den);
-                stop = residualTarget > 0.0f && sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12);
-            }
-            __syncthreads();
-            if (stop) { done = q + 1; break; }
-        }
-        for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) vel[b] = body[b - b0];
-        __syncthreads();
-        if (pIt) {
-            float2* bp = reinterpret_cast<float2*>(body);
-            for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) {
-                const float4 x = pos[b];
-                bp[b - b0] = make_float2(x.x, x.y);
-            }
-            __syncthreads();
-            for (uint32_t q = 0; q < pIt; q++)
-                for (uint32_t k = 0; k < K; k++) {
-                    const uint2 r = rng[k];
-                    for (uint32_t c = r.x + t; c < r.y; c += SCENE_THREADS) {
-                        const uint4 id = cs.idx[c];
-                        if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
-                        PosItem pi;
-                        loadPosItem(cs, c, pi);
-                        float2 A = bp[id.x - b0], B = bp[id.y - b0];
-                        positionStep(pi, A, B);
-                        if (pi.dep.z) bp[id.x - b0] = A;
-                        if (pi.dep.w) bp[id.y - b0] = B;
-                    }
-                    __syncthreads();
-                }
-            for (uint32_t b = b0 + t; b < b1; b += SCENE_THREADS) reinterpret_cast<float2*>(pos + b)[0] = bp[b - b0];
-        }
-        if (t == 0) {
+        const uint32_t done = solveScene(L, s, ctr, cs, vel, pos, sceneBodyStart[s], sceneBodyStart[s + 1], range, sceneRes, K,
+                                         vIt, pIt, track, residualTarget, lds, rng[L.g], part + L.g * L.waves);
+        if (L.t == 0) {
             sceneSweeps[s] = done;
             atomicMax(&ctr->sweeps, done);
             atomicAdd(&ctr->bodySweeps, (unsigned long long)sceneDyn[s] * done);
         }
-        __syncthreads();   // before the next scene reuses the LDS
     }
 }
 
@@ -1515,18 +1605,23 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
     return hipGetLastError();
 }
 
-uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks, uint32_t& colourLdsBytes) {
-    blocks = 0;
+uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, uint32_t wholeCap, SceneLaunch& out,
+                         uint32_t& colourLdsBytes) {
+    out = SceneLaunch{};
     colourLdsBytes = 0;
-    const size_t bytes = std::max<size_t>(maxBodies, 1) * sizeof(float4);
+    const uint32_t maxBodies = S ? orderedBodies[0] : 0u;
     hipDeviceProp_t prop{};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-    const size_t staticBytes = MAX_COLOURS * sizeof(uint2) + SCENE_THREADS * sizeof(double) + 16;
+    const size_t staticBytes = SCENE_GROUPS * MAX_COLOURS * sizeof(uint2) + (SCENE_THREADS / WAVE) * sizeof(double2) +
+                               2 * SCENE_GROUPS * sizeof(uint32_t) + 256;
     const size_t limit = std::max<size_t>(prop.sharedMemPerBlock, (size_t)prop.maxSharedMemoryPerMultiProcessor);
-    if (bytes + staticBytes > limit) return 0;
-    if (bytes > 48 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneSolve), hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (limit <= staticBytes) return 0;
+    // all the LDS a block may have (the block holds the CU alone): the largest scene's body records
+    // for a whole block, or `groups` smaller scenes side by side
+    const size_t bytes = (limit - staticBytes) & ~(size_t)255;
+    if (std::max<size_t>(maxBodies, 1) * sizeof(float4) > bytes) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneSolve), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bytes) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
@@ -1550,7 +1645,14 @@ uint32_t sceneSolveSetup(uint32_t maxBodies, int numCU, int& blocks, uint32_t& c
         (void)hipGetLastError();
         return 0;
     }
-    blocks = numCU * perCu;
+    out.blocks = numCU * perCu;
+    out.whole = std::min<uint32_t>(std::min<uint32_t>(S, (uint32_t)out.blocks), wholeCap);
+    out.groups = 1;
+    if (out.whole < S) {
+        const size_t next = std::max<size_t>(orderedBodies[out.whole], 1) * sizeof(float4);
+        while (out.groups < SCENE_GROUPS && 2 * out.groups * next <= bytes) out.groups *= 2;
+    }
+    out.groupBodies = (uint32_t)(bytes / out.groups / sizeof(float4));
     return (uint32_t)bytes;
 }
 
@@ -1561,10 +1663,12 @@ hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32
     hipLaunchKernelGGL(kSceneRangeClear, dim3(blocksFor(ranges)), dim3(256), 0, st, w.sceneRange, ranges);
     hipLaunchKernelGGL(kSceneRanges, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.ctr, w.cs, w.bodyScene, w.sceneRange);
     const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;
-    const uint32_t blocks = (uint32_t)std::max(1, std::min<int>(w.sceneBlocks, (int)w.S));
+    const SceneLaunch& sl = w.sceneLaunch;
+    // every block takes a whole scene first; with fewer scenes than blocks, no more blocks than scenes
+    const uint32_t blocks = (uint32_t)std::max(1, std::min<int>(sl.blocks, (int)std::max(w.S, sl.whole)));
     hipLaunchKernelGGL(kSceneSolve, dim3(blocks), dim3(SCENE_THREADS), w.sceneLdsBytes, st, w.ctr, w.cs, w.vel, w.pos,
-                       w.sceneBodyStart, w.sceneOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S, vIt, pIt,
-                       track, residualTarget);
+                       w.sceneBodyStart, w.sceneOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S, sl.whole,
+                       sl.groups, sl.groupBodies, vIt, pIt, track, residualTarget);
     hipLaunchKernelGGL(kSceneTotals, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }

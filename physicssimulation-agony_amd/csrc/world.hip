@@ -638,7 +638,13 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     std::stable_sort(sceneOrder.begin(), sceneOrder.end(), [&](uint32_t a, uint32_t b) {
         return sceneBodyStart[a + 1] - sceneBodyStart[a] > sceneBodyStart[b + 1] - sceneBodyStart[b];
     });
-    d.sceneLdsBytes = agx::sceneSolveSetup(maxSceneBodies, d.numCU, d.sceneBlocks, d.sceneColourLdsBytes);
+    // AGX_SCENE_WHOLE_CAP lowers how many of the largest scenes whole blocks solve (the rest go to
+    // lane groups); tests use it to reach the groups with few scenes
+    uint32_t wholeCap = UINT32_MAX;
+    if (const char* cap = std::getenv("AGX_SCENE_WHOLE_CAP")) wholeCap = (uint32_t)std::strtoul(cap, nullptr, 10);
+    std::vector<uint32_t> orderedBodies(d.S);
+    for (uint32_t k = 0; k < d.S; k++) orderedBodies[k] = sceneBodyStart[sceneOrder[k] + 1] - sceneBodyStart[sceneOrder[k]];
+    d.sceneLdsBytes = agx::sceneSolveSetup(orderedBodies.data(), d.S, d.numCU, wholeCap, d.sceneLaunch, d.sceneColourLdsBytes);
     d.sceneMaxBodies = maxSceneBodies;
     // AGX_COLOUR_LDS_CAP lowers how many of a scene's manifolds the per-scene colouring keeps in
     // LDS (larger scenes colour from global scratch instead); tests use it to reach that path
@@ -678,6 +684,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     AL(d.blockRes, (uint64_t)d.solveBlocks * 2 * agx::MAX_SWEEPS); AL(d.errPartial, 2 * agx::SOLVE_FINISH_BLOCKS);
     AL(d.cs.idx, solveCap); AL(d.cs.dep, solveCap); AL(d.cs.nrm, solveCap); AL(d.cs.inv, solveCap); AL(d.cs.r0, solveCap);
     AL(d.cs.r1, solveCap); AL(d.cs.k0, solveCap); AL(d.cs.k1, solveCap); AL(d.cs.imp, solveCap);
+    AL(d.cs.blk, solveCap);
 #undef UP
 #undef AL
     // Initial state: island ids, CCD hit times, empty contact cache, refreshSpatialData, trees.

@@ -54,6 +54,26 @@ def test_per_scene_solve_matches_dataflow_bit_for_bit(agx, scenes):
     assert sp.velocity_error == pytest.approx(sf.velocity_error, rel=1e-4)
 
 
+@pytest.mark.parametrize("whole_cap", ["0", "5"])
+def test_lane_groups_match_dataflow_bit_for_bit(agx, scenes, monkeypatch, whole_cap):
+    """Scenes past the first `whole_cap` (largest first) are solved by lane groups of a block that
+    meet at LDS arrival counters instead of block barriers: still the dataflow's bits."""
+    parts = scenes.batch(40, n_max=1500)
+    base = dict(velocity_iterations=8, position_iterations=3, track_residual=1)
+    flow = _world(agx, parts, solve_per_scene=0, **base)
+    monkeypatch.setenv("AGX_SCENE_WHOLE_CAP", whole_cap)
+    per = _world(agx, parts, solve_per_scene=1, **base)
+    print("bodies", flow.counts.bodies, "whole", whole_cap, flush=True)
+    for n in (1, 12):
+        flow.step(DT, n)
+        per.step(DT, n)
+        _same(flow, per)
+    sf, sp = flow.stats(), per.stats()
+    assert sp.solved_per_scene == 1 and sp.overflow == 0 and sp.manifolds > 0
+    assert sp.total_body_sweeps == sf.total_body_sweeps
+    np.testing.assert_allclose(sp.residual[:8], sf.residual[:8], rtol=1e-4)
+
+
 def test_per_scene_colouring_from_global_scratch(agx, scenes, monkeypatch):
     """Scenes with more manifolds than the colouring keeps in LDS colour from global scratch
     (forced here with a tiny LDS cap): still the dataflow colouring's colours and bits."""
@@ -70,10 +90,14 @@ def test_per_scene_colouring_from_global_scratch(agx, scenes, monkeypatch):
     assert per.stats().solved_per_scene == 1
 
 
-def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes):
-    """One step with a 1e-3 residual target, solved scene by scene: every scene ran the sweeps
-    the oracle needs to reach the target on that scene alone (replayed in the GPU's order), and
-    ended in the oracle's state after exactly those sweeps."""
+@pytest.mark.parametrize("whole_cap", [None, "0"])
+def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes, monkeypatch, whole_cap):
+    """One step with a 1e-3 residual target, solved scene by scene (by whole blocks, or with
+    whole_cap 0 by lane groups): every scene ran the sweeps the oracle needs to reach the target on
+    that scene alone (replayed in the GPU's order), and ended in the oracle's state after exactly
+    those sweeps."""
+    if whole_cap is not None:
+        monkeypatch.setenv("AGX_SCENE_WHOLE_CAP", whole_cap)
     parts = [scenes.batch_scene(i, seed=4, n_min=64, n_max=600) for i in range(9)]
     target = 1e-3
     w = _world(agx, parts, velocity_iterations=64, residual_target=target, position_iterations=3, track_residual=1)
