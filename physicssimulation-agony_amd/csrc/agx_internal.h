@@ -94,6 +94,7 @@ struct Counters {
     uint32_t learnCompactRows; // new-contacts mode: rows written (one per contact point without a cached impulse)
     uint32_t sceneCursor;      // per-scene solve: scenes taken by whole blocks
     uint32_t sceneCursor2;     // per-scene solve: scenes taken by lane groups
+    uint32_t sceneWhole;       // per-scene solve: scenes of this step's order that whole blocks take
     uint32_t perScene;         // this step was solved scene by scene
     unsigned long long bodySweeps;   // this step: dynamic bodies x the sweeps of their scene
 };
@@ -163,8 +164,9 @@ struct Constraints {
                      // block solve is skipped as ill-conditioned); zero for one point
 };
 
-// Launch shape of the per-scene solve: `blocks` resident blocks (one per CU) first take the `whole`
-// largest scenes one each, then split into `groups` lane groups of `groupBodies` LDS body records.
+// Launch shape of the per-scene solve: `blocks` resident blocks (one per CU) first take whole
+// scenes (at least `whole` of them), then split into `groups` lane groups of `groupBodies` LDS body
+// records.
 struct SceneLaunch {
     int blocks = 0;
     uint32_t whole = 0, groups = 1, groupBodies = 0;
@@ -256,6 +258,7 @@ struct DeviceWorld {
     uint32_t sceneColourLdsBytes = 0;     // dynamic LDS of the per-scene colouring
     uint32_t sceneColourCap = 0xFFFFu;    // most manifolds of a scene the colouring keeps in LDS
     SceneLaunch sceneLaunch;              // launch shape of the per-scene solve
+    uint32_t* sceneWorkOrder = nullptr;   // this step's scene order of the per-scene solve
     float* errPartial = nullptr;     // per finish block: velocity and position error partial sums
     Constraints cs{};
     // scan scratch
@@ -413,8 +416,9 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
 hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // launchSolve: the velocity and position sweeps (one kernel); launchSolveFinish: accumulators
 // back to the manifolds, residual and error totals.
-// Per-scene solve: LDS bytes for scenes of up to maxBodies bodies (0 when they do not fit) and the
-// resident blocks for that size.
+// Per-scene solve: its dynamic LDS bytes (0 when the largest scene does not fit) and launch shape,
+// for scenes whose body counts, largest first, are orderedBodies; wholeCap: how many scenes whole
+// blocks take (UINT32_MAX: one per resident block).
 uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, uint32_t wholeCap, SceneLaunch& out,
                          uint32_t& colourLdsBytes);
 hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,

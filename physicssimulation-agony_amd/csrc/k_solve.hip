@@ -994,9 +994,13 @@ __global__ __launch_bounds__(256) void kSceneRanges(const Counters* __restrict__
 
 constexpr uint32_t SCENE_THREADS = 1024;   // one block per CU
 constexpr uint32_t SCENE_GROUPS = 4;       // lane groups of a block for the smaller scenes
+constexpr uint32_t WORK_SORT_MAX = 4096;   // scenes the per-step work order sorts in LDS
 
 // The lanes that solve one scene: the whole block (block barriers) for the largest scenes, else a
 // group of SCENE_THREADS / groups lanes, whose waves meet at an arrival counter in LDS.
+// A value every lane of the wave holds, in a scalar register.
+__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 struct SceneLanes {
     uint32_t t;        // lane in the group
     uint32_t T;        // lanes in the group
@@ -1085,37 +1089,47 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
     uint32_t done = vIt;
     for (uint32_t q = 0; q < vIt; q++) {
         float dsq = 0.0f, lsq = 0.0f;
-        // This lane's slots, colour by colour: rng[k].x + t, + T, ... below rng[k].y. The constraint
-        // after the current one is loaded before the current one is solved, also across the sync
-        // that ends a colour: only this lane writes a constraint's accumulators, and nothing else in
-        // a constraint's record changes during the solve.
-        uint32_t pk = 0, pc = K ? rng[0].x + t : 0u;
-        auto seek = [&]() {
-            while (pk < K && pc >= rng[pk].y) {
-                pk++;
-                if (pk < K) pc = rng[pk].x + t;
+        // Rounds of the group's lanes, colour by colour: lane t takes slot base + t of the rounds
+        // base = rng[k].x, + T, ... below rng[k].y. The next round's constraint is loaded before the
+        // current one is solved (also across the sync that ends a colour), into the other of two
+        // records, so nothing is copied: only this lane writes a constraint's accumulators, and
+        // nothing else in a constraint's record changes during the solve.
+        auto solveOne = [&](Item& it, uint32_t c) {
+            if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) return;   // a hole or another scene
+            if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
+                atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
+                return;
             }
+            float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
+            velocityStep<true>(it, A, B, dsq, lsq);
+            cs.imp[c] = it.imp;
+            if (it.iv.x != 0.0f) body[it.id.x - b0] = A;
+            if (it.iv.y != 0.0f) body[it.id.y - b0] = B;
         };
-        seek();
-        Item next;
-        if (pk < K) loadSceneItem(cs, pc, next);
+        uint32_t nk = 0, nb = 0;   // the next round: colour and base (uniform)
+        auto firstRoundFrom = [&](uint32_t k) {
+            nk = k;
+            while (nk < K && uniform(rng[nk].x) >= uniform(rng[nk].y)) nk++;
+            if (nk < K) nb = uniform(rng[nk].x);
+        };
+        Item I0, I1;
+        firstRoundFrom(0);
+        if (nk < K && nb + t < uniform(rng[nk].y)) loadSceneItem(cs, nb + t, I0);
+        bool odd = false;   // the current round's record is I1
         for (uint32_t k = 0; k < K; k++) {
-            while (pk == k) {
-                Item it = next;
-                const uint32_t c = pc;
-                pc += T;
-                seek();
-                if (pk < K) loadSceneItem(cs, pc, next);
-                if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) continue;   // a hole or another scene
-                if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
-                    atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
-                    continue;
+            const uint32_t end = uniform(rng[k].y);
+            for (uint32_t base = uniform(rng[k].x); base < end; base += T) {
+                if (base + T < end) { nk = k; nb = base + T; } else firstRoundFrom(k + 1);
+                const bool loadNext = nk < K && nb + t < uniform(rng[nk].y);
+                const bool mine = base + t < end;
+                if (odd) {
+                    if (loadNext) loadSceneItem(cs, nb + t, I0);
+                    if (mine) solveOne(I1, base + t);
+                } else {
+                    if (loadNext) loadSceneItem(cs, nb + t, I1);
+                    if (mine) solveOne(I0, base + t);
                 }
-                float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
-                velocityStep<true>(it, A, B, dsq, lsq);
-                cs.imp[c] = it.imp;
-                if (it.iv.x != 0.0f) body[it.id.x - b0] = A;
-                if (it.iv.y != 0.0f) body[it.id.y - b0] = B;
+                odd = !odd;
             }
             L.sync();
         }
@@ -1153,15 +1167,16 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
     return done;
 }
 
-// Scenes are taken largest first: the first nWhole (one per block) by whole blocks, the rest by the
-// blocks' lane groups (groups per block, each with groupBodies body records of LDS) from a second
-// cursor. A scene's results do not depend on which block or group solves it.
+// Scenes are taken in the step's order (kSceneWorkOrder): the first ctr->sceneWhole by whole
+// blocks, the rest (none larger than groupBodies bodies) by the blocks' lane groups (groups per block, each
+// with groupBodies body records of LDS) from a second cursor. A scene's results do not depend on
+// which block or group solves it.
 __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restrict__ ctr, Constraints cs, float4* __restrict__ vel,
                                                              float4* __restrict__ pos, const uint32_t* __restrict__ sceneBodyStart,
                                                              const uint32_t* __restrict__ sceneOrder,
                                                              const uint32_t* __restrict__ sceneDyn, const uint2* __restrict__ range,
                                                              float2* __restrict__ sceneRes, uint32_t* __restrict__ sceneSweeps,
-                                                             uint32_t S, uint32_t nWhole, uint32_t groups, uint32_t groupBodies,
+                                                             uint32_t S, uint32_t groups, uint32_t groupBodies,
                                                              uint32_t vIt, uint32_t pIt, int track, float residualTarget) {
     extern __shared__ float4 body[];   // the scene's body velocities, then positions (x, y); per group
     __shared__ uint2 rng[SCENE_GROUPS][MAX_COLOURS];
@@ -1170,6 +1185,7 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
     __shared__ uint32_t bar[SCENE_GROUPS];
     __shared__ uint32_t abort;
     const uint32_t K = ctr->colours;
+    const uint32_t nWhole = ctr->sceneWhole;
     if (threadIdx.x < SCENE_GROUPS) bar[threadIdx.x] = 0u;
     if (threadIdx.x == 0) abort = 0u;
     SceneLanes L{ threadIdx.x, SCENE_THREADS, 0u, SCENE_THREADS / WAVE, true, nullptr, &abort, 0u, ctr };
@@ -1203,6 +1219,56 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
             atomicAdd(&ctr->bodySweeps, (unsigned long long)sceneDyn[s] * done);
         }
     }
+}
+
+// This step's order of the per-scene solve and how many of its first scenes whole blocks take:
+// the scenes a lane group cannot hold (more than groupBodies bodies), then by expected work, most
+// first (dynamic bodies x the velocity sweeps the scene ran in the previous step; ties in size
+// order); at least minWhole scenes go to whole blocks, so the heaviest get a CU each instead of a
+// quarter of one. With more than WORK_SORT_MAX scenes: size order and minWhole whole scenes.
+__global__ __launch_bounds__(1024) void kSceneWorkOrder(Counters* __restrict__ ctr, const uint32_t* __restrict__ sceneOrder,
+                                                        const uint32_t* __restrict__ sceneBodyStart,
+                                                        const uint32_t* __restrict__ sceneSweeps,
+                                                        const uint32_t* __restrict__ sceneDyn, uint32_t S, uint32_t minWhole,
+                                                        uint32_t groupBodies, uint32_t* __restrict__ out) {
+    __shared__ unsigned long long key[WORK_SORT_MAX];
+    __shared__ uint32_t must;
+    const uint32_t t = threadIdx.x;
+    if (S > WORK_SORT_MAX) {
+        for (uint32_t i = t; i < S; i += blockDim.x) out[i] = sceneOrder[i];
+        if (t == 0) ctr->sceneWhole = minWhole;
+        return;
+    }
+    if (t == 0) must = 0u;
+    uint32_t n2 = 1;
+    while (n2 < S) n2 <<= 1;
+    __syncthreads();
+    for (uint32_t i = t; i < n2; i += blockDim.x) {
+        unsigned long long k = 0ull;   // padding sorts last
+        if (i < S) {
+            const uint32_t s = sceneOrder[i];
+            const bool big = sceneBodyStart[s + 1] - sceneBodyStart[s] > groupBodies;
+            if (big) atomicAdd(&must, 1u);
+            const unsigned long long work = min((unsigned long long)max(sceneDyn[s], 1u) * max(sceneSweeps[s], 1u),
+                                                0x7FFFFFFFull);
+            k = ((unsigned long long)big << 63) | (work << 32) | (0xFFFFFFFFu - i);
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= n2; k <<= 1)   // bitonic sort, descending
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < n2; i += blockDim.x) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const unsigned long long a = key[i], b = key[p];
+                    if (((i & k) == 0) ? a < b : a > b) { key[i] = b; key[p] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = t; i < S; i += blockDim.x) out[i] = sceneOrder[0xFFFFFFFFu - (uint32_t)key[i]];
+    if (t == 0) ctr->sceneWhole = max(must, minWhole);
 }
 
 // Step totals of the per-scene solve.
@@ -1646,11 +1712,15 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         return 0;
     }
     out.blocks = numCU * perCu;
-    out.whole = std::min<uint32_t>(std::min<uint32_t>(S, (uint32_t)out.blocks), wholeCap);
+    out.whole = std::min<uint32_t>(S, wholeCap == UINT32_MAX ? (uint32_t)out.blocks : wholeCap);
     out.groups = 1;
     if (out.whole < S) {
-        const size_t next = std::max<size_t>(orderedBodies[out.whole], 1) * sizeof(float4);
-        while (out.groups < SCENE_GROUPS && 2 * out.groups * next <= bytes) out.groups *= 2;
+        if (S <= WORK_SORT_MAX) {
+            out.groups = SCENE_GROUPS;   // scenes larger than a group holds go to whole blocks
+        } else {                         // size order: the scenes after the whole ones fit
+            const size_t next = std::max<size_t>(orderedBodies[out.whole], 1) * sizeof(float4);
+            while (out.groups < SCENE_GROUPS && 2 * out.groups * next <= bytes) out.groups *= 2;
+        }
     }
     out.groupBodies = (uint32_t)(bytes / out.groups / sizeof(float4));
     return (uint32_t)bytes;
@@ -1664,10 +1734,12 @@ hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32
     hipLaunchKernelGGL(kSceneRanges, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.ctr, w.cs, w.bodyScene, w.sceneRange);
     const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;
     const SceneLaunch& sl = w.sceneLaunch;
+    hipLaunchKernelGGL(kSceneWorkOrder, dim3(1), dim3(1024), 0, st, w.ctr, w.sceneOrder, w.sceneBodyStart, w.sceneSweeps,
+                       w.sceneDyn, w.S, sl.whole, sl.groupBodies, w.sceneWorkOrder);
     // every block takes a whole scene first; with fewer scenes than blocks, no more blocks than scenes
     const uint32_t blocks = (uint32_t)std::max(1, std::min<int>(sl.blocks, (int)std::max(w.S, sl.whole)));
     hipLaunchKernelGGL(kSceneSolve, dim3(blocks), dim3(SCENE_THREADS), w.sceneLdsBytes, st, w.ctr, w.cs, w.vel, w.pos,
-                       w.sceneBodyStart, w.sceneOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S, sl.whole,
+                       w.sceneBodyStart, w.sceneWorkOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S,
                        sl.groups, sl.groupBodies, vIt, pIt, track, residualTarget);
     hipLaunchKernelGGL(kSceneTotals, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();

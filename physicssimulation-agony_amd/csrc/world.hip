@@ -638,8 +638,8 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     std::stable_sort(sceneOrder.begin(), sceneOrder.end(), [&](uint32_t a, uint32_t b) {
         return sceneBodyStart[a + 1] - sceneBodyStart[a] > sceneBodyStart[b + 1] - sceneBodyStart[b];
     });
-    // AGX_SCENE_WHOLE_CAP lowers how many of the largest scenes whole blocks solve (the rest go to
-    // lane groups); tests use it to reach the groups with few scenes
+    // AGX_SCENE_WHOLE_CAP sets how many of the largest scenes whole blocks solve (default: one per
+    // resident block; the rest go to lane groups); tests use it to reach the groups with few scenes
     uint32_t wholeCap = UINT32_MAX;
     if (const char* cap = std::getenv("AGX_SCENE_WHOLE_CAP")) wholeCap = (uint32_t)std::strtoul(cap, nullptr, 10);
     std::vector<uint32_t> orderedBodies(d.S);
@@ -659,7 +659,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     UP(d.verts, verts); UP(d.normals, normals); UP(d.materials, mats);
     UP(d.sceneBodyStart, sceneBodyStart); UP(d.sceneOrder, sceneOrder); UP(d.sceneDyn, sceneDyn);
     AL(d.sceneRange, (uint64_t)d.S * agx::MAX_COLOURS); AL(d.sceneRes, (uint64_t)d.S * agx::MAX_SWEEPS);
-    AL(d.sceneSweeps, d.S); AL(d.sceneIslands, d.S); AL(d.sceneIslandOffset, d.S); AL(d.sceneManRange, d.S);
+    AL(d.sceneSweeps, d.S); AL(d.sceneWorkOrder, d.S); AL(d.sceneIslands, d.S); AL(d.sceneIslandOffset, d.S); AL(d.sceneManRange, d.S);
     AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N);
     AL(d.pose, d.C); AL(d.wRot, d.C); AL(d.prevW, d.C); AL(d.aabb, d.C);
     if ((rc = allocTree(w, d.dyn, dynList, colScene, d.S))) return bail(rc);
@@ -690,6 +690,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     // Initial state: island ids, CCD hit times, empty contact cache, refreshSpatialData, trees.
     hipStream_t st = w->stream;
     bool ok = agx::launchFillU32(st, d.islandId, UINT32_MAX, d.N) == hipSuccess &&
+              agx::launchFillU32(st, d.sceneSweeps, 0u, d.S) == hipSuccess &&
               agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.hitTime), 0x3F800000u, d.N) == hipSuccess &&
               agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.hashKeys), 0xFFFFFFFFu, 2 * d.hashCapMax) == hipSuccess &&
               agx::launchRefreshCosSin(st, d) == hipSuccess &&
