@@ -95,6 +95,8 @@ struct Counters {
     uint32_t sceneCursor;      // per-scene solve: scenes taken by whole blocks
     uint32_t sceneCursor2;     // per-scene solve: scenes taken by lane groups
     uint32_t sceneWhole;       // per-scene solve: scenes of this step's order that whole blocks take
+    uint32_t colourCursor;     // per-scene colouring: scenes taken by whole blocks
+    uint32_t colourCursor2;    // per-scene colouring: scenes taken by lane groups
     uint32_t perScene;         // this step was solved scene by scene
     unsigned long long bodySweeps;   // this step: dynamic bodies x the sweeps of their scene
 };
@@ -167,9 +169,13 @@ struct Constraints {
 // Launch shape of the per-scene solve: `blocks` resident blocks (one per CU) first take whole
 // scenes (at least `whole` of them), then split into `groups` lane groups of `groupBodies` LDS body
 // records.
+// The per-scene colouring likewise: colourWhole whole scenes (largest first), then colourGroups
+// groups of colourGroupWords LDS words; layouts (bodies << 16) | manifolds kept in LDS.
 struct SceneLaunch {
     int blocks = 0;
     uint32_t whole = 0, groups = 1, groupBodies = 0;
+    int colourBlocks = 0;
+    uint32_t colourWhole = 0, colourGroups = 1, colourLayoutWhole = 0, colourLayoutGroup = 0, colourGroupWords = 0;
 };
 
 struct DeviceWorld {

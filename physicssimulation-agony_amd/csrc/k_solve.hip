@@ -197,13 +197,51 @@ __global__ __launch_bounds__(256) void kColourFlow(Counters* __restrict__ ctr, c
     }
 }
 
+// ---------------------------------------------------------------- lanes of a scene
+constexpr uint32_t SCENE_GROUPS = 4;       // lane groups of a block for the smaller scenes
+constexpr uint32_t WORK_SORT_MAX = 4096;   // scenes the per-step work order sorts in LDS
+
+// A value every lane of the wave holds, in a scalar register.
+__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+struct SceneLanes {
+    uint32_t t;        // lane in the group
+    uint32_t T;        // lanes in the group
+    uint32_t g;        // group (0 for the whole block)
+    uint32_t waves;    // waves in the group
+    bool whole;
+    uint32_t* bar;     // the group's arrival counter
+    uint32_t* abort;   // the block's flag: a wait timed out, no sync waits any more
+    uint32_t target;   // arrivals the next sync waits for
+    Counters* ctr;
+
+    __device__ void sync() {
+        if (whole) { __syncthreads(); return; }
+        target += waves;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if ((threadIdx.x & (WAVE - 1)) == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // bounded: after about 2 s of 64-cycle sleeps the step is flagged invalid and every later
+        // sync of the block returns at once, so the kernel ends
+        for (uint32_t spins = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target;) {
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins == (1u << 26)) {
+                __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                atomicOr(&ctr->overflow, OVERFLOW_STALL);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
+
 // ---------------------------------------------------------------- per-scene colouring
 // The same greedy first fit in priority order as kColourFlow, one workgroup per scene, as rounds
 // in LDS: a manifold takes its colour once it has the smallest priority among the uncoloured
 // manifolds at each of its dynamic bodies. Then every higher-priority manifold at those bodies is
 // coloured and no lower-priority one is, so the colours free at both bodies are exactly the
 // sequential first fit's, and at most one manifold per body takes a colour in a round.
-constexpr uint32_t COLOUR_THREADS = 512;
+constexpr uint32_t COLOUR_THREADS = 1024;   // one block per CU
 constexpr uint16_t UNCOLOURED = 0xFFFFu;
 
 __global__ __launch_bounds__(256) void kSceneManClear(uint2* __restrict__ range, uint32_t S) {
@@ -253,15 +291,14 @@ __device__ inline uint32_t packManifold(uint32_t la, bool da, uint32_t lb, bool 
     return (la | (da ? 0x8000u : 0u)) | ((lb | (db ? 0x8000u : 0u)) << 16);
 }
 
-__global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifold* __restrict__ man, Counters* __restrict__ ctr,
-                                                               const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
-                                                               const uint32_t* __restrict__ sceneBodyStart,
-                                                               const uint2* __restrict__ manRange, uint32_t ldsManifolds,
-                                                               uint2* __restrict__ manBodies, uint32_t* __restrict__ degree,
-                                                               uint32_t* __restrict__ spill, uint16_t* __restrict__ colourId) {
-    extern __shared__ uint32_t lds[];
-    __shared__ int left;
-    const uint32_t s = blockIdx.x, t = threadIdx.x;
+// The colouring of scene s by the lanes L, in the LDS at `lds` laid out as layout =
+// (maxBodies << 16) | capM: body records for up to maxBodies bodies, then up to capM manifolds.
+__device__ inline void colourScene(SceneLanes& L, uint32_t s, const agx_manifold* __restrict__ man, Counters* __restrict__ ctr,
+                                   const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
+                                   const uint32_t* __restrict__ sceneBodyStart, const uint2* __restrict__ manRange,
+                                   uint32_t ldsManifolds, uint2* __restrict__ manBodies, uint32_t* __restrict__ degree,
+                                   uint32_t* __restrict__ spill, uint16_t* __restrict__ colourId, uint32_t* lds, int* left) {
+    const uint32_t t = L.t, T = L.T;
     const uint32_t b0 = sceneBodyStart[s], b1 = sceneBodyStart[s + 1], nb = b1 - b0;
     const uint32_t maxBodies = ldsManifolds >> 16;   // LDS layout: body records for the largest scene first
     const uint32_t capM = ldsManifolds & 0xFFFFu;
@@ -279,13 +316,13 @@ __global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifol
     auto own = [&](uint32_t i) -> bool {
         return inLds ? packed[i] != 0xFFFFFFFFu : manBodies[r.x + i].x - b0 < nb;
     };
-    for (uint32_t i = t; i < nb; i += COLOUR_THREADS) {
+    for (uint32_t i = t; i < nb; i += T) {
         minPrio[i] = 0xFFFFFFFFu;
         for (uint32_t k = 0; k < 4; k++) mask[4 * i + k] = 0u;
     }
-    __syncthreads();
+    L.sync();
     // the scene's manifolds: bodies, degrees (counted in the first mask word), uncoloured
-    for (uint32_t i = t; i < nm; i += COLOUR_THREADS) {
+    for (uint32_t i = t; i < nm; i += T) {
         const uint32_t m = r.x + i;
         const agx_manifold d = man[m];
         const uint32_t a = colBody[d.a], b = colBody[d.b];
@@ -300,27 +337,27 @@ __global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifol
         if (da) atomicAdd(&mask[4 * (a - b0)], 1u);
         if (db) atomicAdd(&mask[4 * (b - b0)], 1u);
     }
-    __syncthreads();
-    for (uint32_t i = t; i < nb; i += COLOUR_THREADS) {
+    L.sync();
+    for (uint32_t i = t; i < nb; i += T) {
         degree[b0 + i] = mask[4 * i];   // 0 for a static body: none of its contacts counted it
         mask[4 * i] = 0u;
     }
-    __syncthreads();
+    L.sync();
     while (true) {
-        if (t == 0) left = 0;
-        __syncthreads();
-        for (uint32_t i = t; i < nm; i += COLOUR_THREADS) {
+        if (t == 0) *left = 0;
+        L.sync();
+        for (uint32_t i = t; i < nm; i += T) {
             if (!own(i)) continue;
             const uint32_t pk = packed[i];
             if (col[i] != UNCOLOURED) continue;
             const uint32_t pr = mix32(r.x + i);
             if (pk & 0x8000u) atomicMin(&minPrio[pk & 0x7FFFu], pr);
             if (pk & 0x80000000u) atomicMin(&minPrio[(pk >> 16) & 0x7FFFu], pr);
-            left = 1;
+            *left = 1;
         }
-        __syncthreads();
-        if (!left) break;
-        for (uint32_t i = t; i < nm; i += COLOUR_THREADS) {
+        L.sync();
+        if (!*left) break;
+        for (uint32_t i = t; i < nm; i += T) {
             if (!own(i)) continue;
             const uint32_t pk = packed[i];
             if (col[i] != UNCOLOURED) continue;
@@ -340,13 +377,61 @@ __global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifol
             if (da) mask[4 * la + (c >> 5)] |= 1u << (c & 31u);
             if (db) mask[4 * lb + (c >> 5)] |= 1u << (c & 31u);
         }
-        __syncthreads();
-        for (uint32_t i = t; i < nb; i += COLOUR_THREADS) minPrio[i] = 0xFFFFFFFFu;
-        __syncthreads();
+        L.sync();
+        for (uint32_t i = t; i < nb; i += T) minPrio[i] = 0xFFFFFFFFu;
+        L.sync();
     }
     if (inLds)
-        for (uint32_t i = t; i < nm; i += COLOUR_THREADS)
+        for (uint32_t i = t; i < nm; i += T)
             if (own(i)) colourId[r.x + i] = col[i];
+}
+
+// Blocks (one per CU) take scenes largest first from a cursor: the first nWhole whole, the rest
+// by `groups` lane groups of groupWords words of LDS each (layoutGroup); a scene's colours do not
+// depend on which block or group colours it.
+__global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifold* __restrict__ man, Counters* __restrict__ ctr,
+                                                               const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
+                                                               const uint32_t* __restrict__ sceneBodyStart,
+                                                               const uint32_t* __restrict__ sceneOrder,
+                                                               const uint2* __restrict__ manRange, uint32_t S, uint32_t nWhole,
+                                                               uint32_t groups, uint32_t layoutWhole, uint32_t layoutGroup,
+                                                               uint32_t groupWords, uint2* __restrict__ manBodies,
+                                                               uint32_t* __restrict__ degree, uint32_t* __restrict__ spill,
+                                                               uint16_t* __restrict__ colourId) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int left[SCENE_GROUPS];
+    __shared__ uint32_t taken[SCENE_GROUPS];
+    __shared__ uint32_t bar[SCENE_GROUPS];
+    __shared__ uint32_t abort;
+    if (threadIdx.x < SCENE_GROUPS) bar[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) abort = 0u;
+    SceneLanes L{ threadIdx.x, COLOUR_THREADS, 0u, COLOUR_THREADS / WAVE, true, nullptr, &abort, 0u, ctr };
+    uint32_t* region = lds;
+    uint32_t layout = layoutWhole;
+    bool grouped = false;
+    while (true) {
+        if (L.t == 0) taken[L.g] = grouped ? nWhole + atomicAdd(&ctr->colourCursor2, 1u) : atomicAdd(&ctr->colourCursor, 1u);
+        L.sync();
+        const uint32_t k0 = taken[L.g];
+        if (!grouped && k0 >= nWhole) {
+            __syncthreads();   // every wave has read taken[0] before the groups reuse it
+            if (nWhole >= S) break;
+            const uint32_t T = COLOUR_THREADS / groups;
+            L.g = threadIdx.x / T;
+            L.t = threadIdx.x % T;
+            L.T = T;
+            L.waves = T / WAVE;
+            L.whole = groups == 1;
+            L.bar = &bar[L.g];
+            region = lds + (size_t)L.g * groupWords;
+            layout = layoutGroup;
+            grouped = true;
+            continue;
+        }
+        if (k0 >= S) break;
+        colourScene(L, sceneOrder[k0], man, ctr, colBody, inv, sceneBodyStart, manRange, layout, manBodies, degree, spill,
+                    colourId, region, &left[L.g]);
+    }
 }
 
 // ---------------------------------------------------------------- solve order
@@ -993,45 +1078,9 @@ __global__ __launch_bounds__(256) void kSceneRanges(const Counters* __restrict__
 }
 
 constexpr uint32_t SCENE_THREADS = 1024;   // one block per CU
-constexpr uint32_t SCENE_GROUPS = 4;       // lane groups of a block for the smaller scenes
-constexpr uint32_t WORK_SORT_MAX = 4096;   // scenes the per-step work order sorts in LDS
 
 // The lanes that solve one scene: the whole block (block barriers) for the largest scenes, else a
 // group of SCENE_THREADS / groups lanes, whose waves meet at an arrival counter in LDS.
-// A value every lane of the wave holds, in a scalar register.
-__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-struct SceneLanes {
-    uint32_t t;        // lane in the group
-    uint32_t T;        // lanes in the group
-    uint32_t g;        // group (0 for the whole block)
-    uint32_t waves;    // waves in the group
-    bool whole;
-    uint32_t* bar;     // the group's arrival counter
-    uint32_t* abort;   // the block's flag: a wait timed out, no sync waits any more
-    uint32_t target;   // arrivals the next sync waits for
-    Counters* ctr;
-
-    __device__ void sync() {
-        if (whole) { __syncthreads(); return; }
-        target += waves;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if ((threadIdx.x & (WAVE - 1)) == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        // bounded: after about 2 s of 64-cycle sleeps the step is flagged invalid and every later
-        // sync of the block returns at once, so the kernel ends
-        for (uint32_t spins = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target;) {
-            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins == (1u << 26)) {
-                __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                atomicOr(&ctr->overflow, OVERFLOW_STALL);
-                break;
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-};
-
 // Sum of (num, den) over the group's lanes: a butterfly within each wave (every lane ends with
 // the same bits), then the waves' partials in wave order; every lane returns the total.
 __device__ inline double2 sceneSum(SceneLanes& L, double num, double den, double2* part) {
@@ -1629,12 +1678,15 @@ hipError_t launchColourPrepScene(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL(kSceneManClear, dim3(blocksFor(w.S)), dim3(256), 0, st, w.sceneManRange, w.S);
     hipLaunchKernelGGL(kSceneManRanges, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colScene, w.sceneManRange);
     hipLaunchKernelGGL(kClearManBodies, dim3(gm), dim3(256), 0, st, w.ctr, w.manBodies);
-    const uint32_t maxBodies = std::max<uint32_t>(w.sceneMaxBodies, 1);
-    const uint32_t lds = w.sceneColourLdsBytes;
-    const uint32_t capM = (lds - 5 * maxBodies * (uint32_t)sizeof(uint32_t)) / 6;
-    const uint32_t layout = (maxBodies << 16) | std::min<uint32_t>(std::min<uint32_t>(capM, w.sceneColourCap), 0xFFFFu);
-    hipLaunchKernelGGL(kSceneColour, dim3(w.S), dim3(COLOUR_THREADS), lds, st, w.man[w.cur], w.ctr, w.colBody, w.inv,
-                       w.sceneBodyStart, w.sceneManRange, layout, w.manBodies, w.degree, w.rank, w.colourId);
+    const SceneLaunch& sl = w.sceneLaunch;
+    auto capped = [&](uint32_t layout) {   // at most sceneColourCap manifolds in LDS
+        return (layout & 0xFFFF0000u) | std::min<uint32_t>(layout & 0xFFFFu, w.sceneColourCap);
+    };
+    const uint32_t blocks = (uint32_t)std::max(1, std::min<int>(sl.colourBlocks, (int)w.S));
+    hipLaunchKernelGGL(kSceneColour, dim3(blocks), dim3(COLOUR_THREADS), w.sceneColourLdsBytes, st, w.man[w.cur], w.ctr,
+                       w.colBody, w.inv, w.sceneBodyStart, w.sceneOrder, w.sceneManRange, w.S, sl.colourWhole, sl.colourGroups,
+                       capped(sl.colourLayoutWhole), capped(sl.colourLayoutGroup), sl.colourGroupWords, w.manBodies,
+                       w.degree, w.rank, w.colourId);
     hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
     hipLaunchKernelGGL(kColourColumnScan, dim3(MAX_COLOURS), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
     hipLaunchKernelGGL(kColourStarts, dim3(1), dim3(64), 0, st, w.ctr);
@@ -1692,19 +1744,23 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         (void)hipGetLastError();
         return 0;
     }
-    // the per-scene colouring: 20 bytes per body of the largest scene, then as many manifolds
-    // (6 bytes each) as the rest of the LDS holds
+    // the per-scene colouring, also one block per CU with all the LDS: per scene 20 bytes per body,
+    // then as many of its manifolds (6 bytes each) as the rest of the region holds (a scene with
+    // more colours them in global scratch)
     if (maxBodies >= 0x8000u) return 0;   // local body indices are packed in 15 bits
-    const size_t colourBodies = std::max<size_t>(maxBodies, 1) * 5 * sizeof(uint32_t);
-    if (colourBodies + 6 * 256 + 64 > limit) return 0;
-    const size_t colourBytes = std::min<size_t>(limit - 64, colourBodies + 6 * 0xFFFFull) & ~(size_t)15;
-    colourLdsBytes = (uint32_t)colourBytes;
-    if (colourBytes > 48 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneColour), hipFuncAttributeMaxDynamicSharedMemorySize,
+    const size_t colourStatic = 4 * SCENE_GROUPS * sizeof(uint32_t) + 256;
+    const size_t colourBytes = (limit - colourStatic) & ~(size_t)255;
+    auto capOf = [](size_t region, uint32_t bodies) -> size_t {   // manifolds after the body records
+        const size_t bodyBytes = (size_t)std::max(bodies, 1u) * 5 * sizeof(uint32_t) + 8;   // + rounding of the colours
+        return region > bodyBytes ? std::min<size_t>((region - bodyBytes) / 6, 0xFFFF) : 0;
+    };
+    if (capOf(colourBytes, maxBodies) < 64) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneColour), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)colourBytes) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
+    colourLdsBytes = (uint32_t)colourBytes;
     int perCu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(kSceneSolve), SCENE_THREADS,
                                                      bytes) != hipSuccess || perCu < 1) {
@@ -1723,6 +1779,27 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         }
     }
     out.groupBodies = (uint32_t)(bytes / out.groups / sizeof(float4));
+    // colouring: the same whole scenes, then as many groups (at most 4) as leave room for three
+    // manifolds per body of the largest remaining scene
+    int colourPerCu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&colourPerCu, reinterpret_cast<const void*>(kSceneColour),
+                                                     COLOUR_THREADS, colourBytes) != hipSuccess || colourPerCu < 1) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    out.colourBlocks = numCU * colourPerCu;
+    out.colourWhole = std::min<uint32_t>(S, wholeCap == UINT32_MAX ? (uint32_t)out.colourBlocks : wholeCap);
+    out.colourLayoutWhole = (std::max(maxBodies, 1u) << 16) | (uint32_t)capOf(colourBytes, maxBodies);
+    out.colourGroups = 1;
+    out.colourLayoutGroup = out.colourLayoutWhole;
+    auto groupWords = [&](uint32_t groups) { return (uint32_t)(colourBytes / groups / sizeof(uint32_t)) & ~3u; };
+    if (out.colourWhole < S) {
+        const uint32_t next = std::max(orderedBodies[out.colourWhole], 1u);
+        out.colourGroups = SCENE_GROUPS;
+        while (out.colourGroups > 1 && capOf(4ull * groupWords(out.colourGroups), next) < 3ull * next) out.colourGroups /= 2;
+        out.colourLayoutGroup = (next << 16) | (uint32_t)capOf(4ull * groupWords(out.colourGroups), next);
+    }
+    out.colourGroupWords = groupWords(out.colourGroups);
     return (uint32_t)bytes;
 }
 

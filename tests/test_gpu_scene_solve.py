@@ -56,8 +56,9 @@ def test_per_scene_solve_matches_dataflow_bit_for_bit(agx, scenes):
 
 @pytest.mark.parametrize("whole_cap", ["0", "5"])
 def test_lane_groups_match_dataflow_bit_for_bit(agx, scenes, monkeypatch, whole_cap):
-    """Scenes past the first `whole_cap` (largest first) are solved by lane groups of a block that
-    meet at LDS arrival counters instead of block barriers: still the dataflow's bits."""
+    """Scenes past the first `whole_cap` (largest first) are coloured and solved by lane groups of a
+    block that meet at LDS arrival counters instead of block barriers: still the dataflow's colours
+    and bits."""
     parts = scenes.batch(40, n_max=1500)
     base = dict(velocity_iterations=8, position_iterations=3, track_residual=1)
     flow = _world(agx, parts, solve_per_scene=0, **base)
@@ -68,6 +69,11 @@ def test_lane_groups_match_dataflow_bit_for_bit(agx, scenes, monkeypatch, whole_
         flow.step(DT, n)
         per.step(DT, n)
         _same(flow, per)
+        # the colouring by lane groups gives the dataflow colouring's colours too
+        of, cf = flow.solve_order()
+        op, cp = per.solve_order()
+        np.testing.assert_array_equal(cf, cp)
+        np.testing.assert_array_equal(of, op)
     sf, sp = flow.stats(), per.stats()
     assert sp.solved_per_scene == 1 and sp.overflow == 0 and sp.manifolds > 0
     assert sp.total_body_sweeps == sf.total_body_sweeps
