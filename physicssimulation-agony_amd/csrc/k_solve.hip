@@ -1097,6 +1097,55 @@ __device__ inline double2 sceneSum(SceneLanes& L, double num, double den, double
     return tot;
 }
 
+// Colour-major rounds of the lanes L over a scene's constraint ranges rng[0..K): lane t takes slot
+// base + t of the rounds base = rng[k].x, + T, ... below rng[k].y, and the lanes sync after each
+// colour. The next round's record is loaded (load(record, slot)) before the current one is run
+// (run(record, slot)), also across the sync that ends a colour, into the other of two records, so
+// nothing is copied. This needs that only the lane that runs a slot writes to its constraint, and
+// that nothing a record holds changes while the rounds run.
+template <class Rec, class Load, class Run>
+__device__ inline void colourRounds(SceneLanes& L, const uint2* rng, uint32_t K, Load load, Run run) {
+    const uint32_t t = L.t, T = L.T;
+    uint32_t nk = 0, nb = 0;   // the next round: colour and base (uniform)
+    auto firstRoundFrom = [&](uint32_t k) {
+        nk = k;
+        while (nk < K && uniform(rng[nk].x) >= uniform(rng[nk].y)) nk++;
+        if (nk < K) nb = uniform(rng[nk].x);
+    };
+    Rec I0, I1;
+    firstRoundFrom(0);
+    if (nk < K && nb + t < uniform(rng[nk].y)) load(I0, nb + t);
+    bool odd = false;   // the current round's record is I1
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t end = uniform(rng[k].y);
+        for (uint32_t base = uniform(rng[k].x); base < end; base += T) {
+            if (base + T < end) { nk = k; nb = base + T; } else firstRoundFrom(k + 1);
+            const bool loadNext = nk < K && nb + t < uniform(rng[nk].y);
+            const bool mine = base + t < end;
+            if (odd) {
+                if (loadNext) load(I0, nb + t);
+                if (mine) run(I1, base + t);
+            } else {
+                if (loadNext) load(I1, nb + t);
+                if (mine) run(I0, base + t);
+            }
+            odd = !odd;
+        }
+        L.sync();
+    }
+}
+
+// What the warm start and the position sweeps of the per-scene solve read of a constraint.
+struct WarmRec {
+    uint4 id;
+    float4 nrm, iv, r0, r1, imp;
+};
+struct PosRec {
+    uint4 id;
+    float2 n, im;
+    float penBase;
+};
+
 // One scene (bodies [b0, b1)) by the lanes L, body records in `body`: warm start, velocity sweeps
 // until the residual target or vIt, position sweeps; returns the velocity sweeps run.
 __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __restrict__ ctr, const Constraints& cs,
@@ -1108,80 +1157,53 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
     for (uint32_t k = t; k < K; k += T) rng[k] = range[(size_t)s * MAX_COLOURS + k];
     for (uint32_t b = b0 + t; b < b1; b += T) body[b - b0] = vel[b];
     L.sync();
+    // a constraint of this scene (not a hole, nor one of a neighbouring scene's in the range)
+    auto ours = [&](const uint4& id) { return id.z != 0 && id.x >= b0 && id.x < b1 && id.y >= b0 && id.y < b1; };
     // applyWarmStarting (BodyCollisionSolver.cpp:416-490) colour by colour: every body adds its
     // constraints' impulses in solve order, with the operations of kWarm
-    for (uint32_t k = 0; k < K; k++) {
-        const uint2 r = rng[k];
-        for (uint32_t c = r.x + t; c < r.y; c += T) {
-            const uint4 id = cs.idx[c];
-            if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
-            const float4 nr = cs.nrm[c], iv = cs.inv[c], r0 = cs.r0[c], r1 = cs.r1[c], j = cs.imp[c];
+    colourRounds<WarmRec>(
+        L, rng, K,
+        [&](WarmRec& w, uint32_t c) {
+            w.id = cs.idx[c]; w.nrm = cs.nrm[c]; w.iv = cs.inv[c]; w.r0 = cs.r0[c]; w.r1 = cs.r1[c]; w.imp = cs.imp[c];
+        },
+        [&](WarmRec& w, uint32_t) {
+            if (!ours(w.id)) return;
+            const float4 nr = w.nrm, iv = w.iv, r0 = w.r0, r1 = w.r1, j = w.imp;
             const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
             const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
             const float sn = jn0 + jn1, st = jt0 + jt1;
             const float sx = sn * nx + st * tx, sy = sn * ny + st * ty;
             if (iv.x != 0.0f) {
-                float4 v = body[id.x - b0];
+                float4 v = body[w.id.x - b0];
                 v.x = v.x - sx * iv.x; v.y = v.y - sy * iv.x;
                 v.z -= (jn0 * r0.x + jt0 * r0.z + jn1 * r1.x + jt1 * r1.z) * iv.z;
-                body[id.x - b0] = v;
+                body[w.id.x - b0] = v;
             }
             if (iv.y != 0.0f) {
-                float4 v = body[id.y - b0];
+                float4 v = body[w.id.y - b0];
                 v.x = v.x + sx * iv.y; v.y = v.y + sy * iv.y;
                 v.z += (jn0 * r0.y + jt0 * r0.w + jn1 * r1.y + jt1 * r1.w) * iv.w;
-                body[id.y - b0] = v;
+                body[w.id.y - b0] = v;
             }
-        }
-        L.sync();
-    }
+        });
     uint32_t done = vIt;
     for (uint32_t q = 0; q < vIt; q++) {
         float dsq = 0.0f, lsq = 0.0f;
-        // Rounds of the group's lanes, colour by colour: lane t takes slot base + t of the rounds
-        // base = rng[k].x, + T, ... below rng[k].y. The next round's constraint is loaded before the
-        // current one is solved (also across the sync that ends a colour), into the other of two
-        // records, so nothing is copied: only this lane writes a constraint's accumulators, and
-        // nothing else in a constraint's record changes during the solve.
-        auto solveOne = [&](Item& it, uint32_t c) {
-            if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) return;   // a hole or another scene
-            if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
-                atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
-                return;
-            }
-            float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
-            velocityStep<true>(it, A, B, dsq, lsq);
-            cs.imp[c] = it.imp;
-            if (it.iv.x != 0.0f) body[it.id.x - b0] = A;
-            if (it.iv.y != 0.0f) body[it.id.y - b0] = B;
-        };
-        uint32_t nk = 0, nb = 0;   // the next round: colour and base (uniform)
-        auto firstRoundFrom = [&](uint32_t k) {
-            nk = k;
-            while (nk < K && uniform(rng[nk].x) >= uniform(rng[nk].y)) nk++;
-            if (nk < K) nb = uniform(rng[nk].x);
-        };
-        Item I0, I1;
-        firstRoundFrom(0);
-        if (nk < K && nb + t < uniform(rng[nk].y)) loadSceneItem(cs, nb + t, I0);
-        bool odd = false;   // the current round's record is I1
-        for (uint32_t k = 0; k < K; k++) {
-            const uint32_t end = uniform(rng[k].y);
-            for (uint32_t base = uniform(rng[k].x); base < end; base += T) {
-                if (base + T < end) { nk = k; nb = base + T; } else firstRoundFrom(k + 1);
-                const bool loadNext = nk < K && nb + t < uniform(rng[nk].y);
-                const bool mine = base + t < end;
-                if (odd) {
-                    if (loadNext) loadSceneItem(cs, nb + t, I0);
-                    if (mine) solveOne(I1, base + t);
-                } else {
-                    if (loadNext) loadSceneItem(cs, nb + t, I1);
-                    if (mine) solveOne(I0, base + t);
+        // only the lane that solves a constraint writes its accumulators (the same lane every sweep)
+        colourRounds<Item>(
+            L, rng, K, [&](Item& it, uint32_t c) { loadSceneItem(cs, c, it); },
+            [&](Item& it, uint32_t c) {
+                if (it.id.z == 0 || it.id.x < b0 || it.id.x >= b1) return;   // a hole or another scene
+                if (it.id.y < b0 || it.id.y >= b1) {   // both bodies of a contact share a scene
+                    atomicOr(&ctr->overflow, OVERFLOW_ITEMS);
+                    return;
                 }
-                odd = !odd;
-            }
-            L.sync();
-        }
+                float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
+                velocityStep<true>(it, A, B, dsq, lsq);
+                cs.imp[c] = it.imp;
+                if (it.iv.x != 0.0f) body[it.id.x - b0] = A;
+                if (it.iv.y != 0.0f) body[it.id.y - b0] = B;
+            });
         const double2 r = sceneSum(L, (double)dsq, (double)lsq, part);
         if (t == 0 && track) sceneRes[(size_t)s * MAX_SWEEPS + q] = make_float2((float)r.x, (float)r.y);
         if (residualTarget > 0.0f && sqrt(r.x) <= (double)residualTarget * fmax(sqrt(r.y), 1e-12)) { done = q + 1; break; }
@@ -1195,21 +1217,26 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
             bp[b - b0] = make_float2(x.x, x.y);
         }
         L.sync();
+        // solvePositionConstraints (BodyCollisionSolver.cpp:930-980), the operations of positionStep
         for (uint32_t q = 0; q < pIt; q++)
-            for (uint32_t k = 0; k < K; k++) {
-                const uint2 r = rng[k];
-                for (uint32_t c = r.x + t; c < r.y; c += T) {
-                    const uint4 id = cs.idx[c];
-                    if (id.z == 0 || id.x < b0 || id.x >= b1 || id.y < b0 || id.y >= b1) continue;
+            colourRounds<PosRec>(
+                L, rng, K,
+                [&](PosRec& pr, uint32_t c) {
+                    pr.id = cs.idx[c];
+                    const float4 nr = cs.nrm[c], iv = cs.inv[c];
+                    pr.n = make_float2(nr.x, nr.y);
+                    pr.im = make_float2(iv.x, iv.y);
+                    pr.penBase = cs.k1[c].z;
+                },
+                [&](PosRec& pr, uint32_t) {
+                    if (!ours(pr.id)) return;
                     PosItem pi;
-                    loadPosItem(cs, c, pi);
-                    float2 A = bp[id.x - b0], B = bp[id.y - b0];
+                    pi.nx = pr.n.x; pi.ny = pr.n.y; pi.iMA = pr.im.x; pi.iMB = pr.im.y; pi.penBase = pr.penBase;
+                    float2 A = bp[pr.id.x - b0], B = bp[pr.id.y - b0];
                     positionStep(pi, A, B);
-                    if (pi.iMA != 0.0f) bp[id.x - b0] = A;
-                    if (pi.iMB != 0.0f) bp[id.y - b0] = B;
-                }
-                L.sync();
-            }
+                    if (pi.iMA != 0.0f) bp[pr.id.x - b0] = A;
+                    if (pi.iMB != 0.0f) bp[pr.id.y - b0] = B;
+                });
         for (uint32_t b = b0 + t; b < b1; b += T) reinterpret_cast<float2*>(pos + b)[0] = bp[b - b0];
     }
     L.sync();   // before the next scene reuses the LDS
