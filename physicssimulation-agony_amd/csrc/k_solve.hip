@@ -249,29 +249,31 @@ __global__ __launch_bounds__(256) void kSceneManClear(uint2* __restrict__ range,
         range[i] = make_uint2(0xFFFFFFFFu, 0u);
 }
 
+// Ranges of keys over an index space: range[key] = (smallest index, one past the largest) holding
+// that key. The indices come in runs of equal keys, so only a run's first and last index update
+// the range (an atomic min and max per run, not per index); keyAt(i) is NO_KEY beyond the end.
+// Every lane of the block's waves must call it with base uniform over the wave.
+constexpr uint32_t NO_KEY = 0xFFFFFFFFu;
+template <class KeyAt>
+__device__ inline void runRanges(uint32_t i, KeyAt keyAt, uint2* __restrict__ range) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t key = keyAt(i);
+    uint32_t prev = __shfl_up(key, 1, WAVE), next = __shfl_down(key, 1, WAVE);
+    if (lane == 0) prev = i > 0 ? keyAt(i - 1) : NO_KEY;
+    if (lane == WAVE - 1) next = keyAt(i + 1);
+    if (key == NO_KEY) return;
+    if (prev != key) atomicMin(&range[key].x, i);
+    if (next != key) atomicMax(&range[key].y, i + 1);
+}
+
 // First manifold and one past the last of every scene (manifolds come grouped by scene up to a
 // pair wave that straddles two scenes).
 __global__ __launch_bounds__(256) void kSceneManRanges(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
                                                        const uint32_t* __restrict__ colScene, uint2* __restrict__ range) {
     const uint32_t M = ctr->manifolds;
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < M; base += gridDim.x * blockDim.x) {
-        const uint32_t m = base + threadIdx.x;
-        const uint32_t key = m < M ? colScene[man[m].a] : 0xFFFFFFFFu;
-        const uint32_t first = __shfl(key, 0, WAVE);
-        if (__all(key == first)) {
-            if (first == 0xFFFFFFFFu) continue;
-            uint32_t mn = m, mx = m + 1;
-            for (uint32_t off = WAVE / 2; off > 0; off >>= 1) {
-                mn = min(mn, __shfl_xor(mn, off, WAVE));
-                mx = max(mx, __shfl_xor(mx, off, WAVE));
-            }
-            if (lane == 0) { atomicMin(&range[first].x, mn); atomicMax(&range[first].y, mx); }
-        } else if (key != 0xFFFFFFFFu) {
-            atomicMin(&range[key].x, m);
-            atomicMax(&range[key].y, m + 1);
-        }
-    }
+    auto keyAt = [&](uint32_t m) { return m < M ? colScene[man[m].a] : NO_KEY; };
+    for (uint32_t base = blockIdx.x * blockDim.x; base < M; base += gridDim.x * blockDim.x)
+        runRanges(base + threadIdx.x, keyAt, range);
 }
 
 // manBodies of every manifold cleared before the scenes fill in their own: a scene's manifold
@@ -688,10 +690,11 @@ __device__ inline void loadItem(const Constraints& cs, uint32_t c, Item& it) {
 }
 
 // The per-scene solve's record: the block instead of the predecessors (a body is dynamic iff its
-// inverse mass is not zero, which is what the dynamic flags of dep hold).
+// inverse mass is not zero, which is what the dynamic flags of dep hold), and no inverse masses
+// (the solve keeps the bodies' own in LDS: the same values).
 __device__ inline void loadSceneItem(const Constraints& cs, uint32_t c, Item& it) {
     it.id = cs.idx[c]; it.blk = cs.blk[c];
-    it.nrm = cs.nrm[c]; it.iv = cs.inv[c]; it.r0 = cs.r0[c]; it.r1 = cs.r1[c];
+    it.nrm = cs.nrm[c]; it.r0 = cs.r0[c]; it.r1 = cs.r1[c];
     it.k0 = cs.k0[c]; it.k1 = cs.k1[c]; it.imp = cs.imp[c];
 }
 
@@ -1041,40 +1044,19 @@ __global__ __launch_bounds__(256) void kSceneRanges(const Counters* __restrict__
     __syncthreads();
     const uint32_t P = starts[MAX_COLOURS];
     const uint32_t K = ctr->colours;
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
-        const uint32_t c = base + threadIdx.x;
-        uint32_t key = 0xFFFFFFFFu;
-        if (c < P) {
-            const uint4 id = cs.idx[c];
-            if (id.z != 0) {
-                uint32_t lo = 0, hi = K;   // starts[lo] <= c < starts[hi]
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (starts[mid] <= c) lo = mid; else hi = mid;
-                }
-                key = bodyScene[id.x] * MAX_COLOURS + lo;
-            }
+    auto keyAt = [&](uint32_t c) -> uint32_t {   // scene x colour of the constraint at slot c
+        if (c >= P) return NO_KEY;
+        const uint4 id = cs.idx[c];
+        if (id.z == 0) return NO_KEY;
+        uint32_t lo = 0, hi = K;   // starts[lo] <= c < starts[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (starts[mid] <= c) lo = mid; else hi = mid;
         }
-        // one update per wave when the wave's slots share a key (the common case)
-        const uint32_t first = __shfl(key, 0, WAVE);
-        const bool uniform = __all(key == first);
-        if (uniform) {
-            if (first == 0xFFFFFFFFu) continue;
-            uint32_t mn = c, mx = c + 1;
-            for (uint32_t off = WAVE / 2; off > 0; off >>= 1) {
-                mn = min(mn, __shfl_xor(mn, off, WAVE));
-                mx = max(mx, __shfl_xor(mx, off, WAVE));
-            }
-            if (lane == 0) {
-                atomicMin(&range[first].x, mn);
-                atomicMax(&range[first].y, mx);
-            }
-        } else if (key != 0xFFFFFFFFu) {
-            atomicMin(&range[key].x, c);
-            atomicMax(&range[key].y, c + 1);
-        }
-    }
+        return bodyScene[id.x] * MAX_COLOURS + lo;
+    };
+    for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x)
+        runRanges(base + threadIdx.x, keyAt, range);
 }
 
 constexpr uint32_t SCENE_THREADS = 1024;   // one block per CU
@@ -1138,11 +1120,11 @@ __device__ inline void colourRounds(SceneLanes& L, const uint2* rng, uint32_t K,
 // What the warm start and the position sweeps of the per-scene solve read of a constraint.
 struct WarmRec {
     uint4 id;
-    float4 nrm, iv, r0, r1, imp;
+    float4 nrm, r0, r1, imp;
 };
 struct PosRec {
     uint4 id;
-    float2 n, im;
+    float2 n;
     float penBase;
 };
 
@@ -1152,11 +1134,19 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
                                       float4* __restrict__ vel, float4* __restrict__ pos, uint32_t b0, uint32_t b1,
                                       const uint2* __restrict__ range, float2* __restrict__ sceneRes, uint32_t K,
                                       uint32_t vIt, uint32_t pIt, int track, float residualTarget, float4* body,
-                                      uint2* rng, double2* part) {
+                                      float2* binv, const float2* __restrict__ inv, uint2* rng, double2* part) {
     const uint32_t t = L.t, T = L.T;
     for (uint32_t k = t; k < K; k += T) rng[k] = range[(size_t)s * MAX_COLOURS + k];
-    for (uint32_t b = b0 + t; b < b1; b += T) body[b - b0] = vel[b];
+    for (uint32_t b = b0 + t; b < b1; b += T) {
+        body[b - b0] = vel[b];
+        binv[b - b0] = inv[b];   // inverse mass, inverse inertia: what the constraints' inv plane holds
+    }
     L.sync();
+    // the constraint's inverse masses and inertias (A, B, A, B), as kPrep writes them
+    auto invOf = [&](const uint4& id) {
+        const float2 a = binv[id.x - b0], b = binv[id.y - b0];
+        return make_float4(a.x, b.x, a.y, b.y);
+    };
     // a constraint of this scene (not a hole, nor one of a neighbouring scene's in the range)
     auto ours = [&](const uint4& id) { return id.z != 0 && id.x >= b0 && id.x < b1 && id.y >= b0 && id.y < b1; };
     // applyWarmStarting (BodyCollisionSolver.cpp:416-490) colour by colour: every body adds its
@@ -1164,11 +1154,11 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
     colourRounds<WarmRec>(
         L, rng, K,
         [&](WarmRec& w, uint32_t c) {
-            w.id = cs.idx[c]; w.nrm = cs.nrm[c]; w.iv = cs.inv[c]; w.r0 = cs.r0[c]; w.r1 = cs.r1[c]; w.imp = cs.imp[c];
+            w.id = cs.idx[c]; w.nrm = cs.nrm[c]; w.r0 = cs.r0[c]; w.r1 = cs.r1[c]; w.imp = cs.imp[c];
         },
         [&](WarmRec& w, uint32_t) {
             if (!ours(w.id)) return;
-            const float4 nr = w.nrm, iv = w.iv, r0 = w.r0, r1 = w.r1, j = w.imp;
+            const float4 nr = w.nrm, iv = invOf(w.id), r0 = w.r0, r1 = w.r1, j = w.imp;
             const float nx = nr.x, ny = nr.y, tx = -ny, ty = nx;
             const float jn0 = j.x, jt0 = j.y, jn1 = j.z, jt1 = j.w;
             const float sn = jn0 + jn1, st = jt0 + jt1;
@@ -1199,6 +1189,7 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
                     return;
                 }
                 float4 A = body[it.id.x - b0], B = body[it.id.y - b0];
+                it.iv = invOf(it.id);
                 velocityStep<true>(it, A, B, dsq, lsq);
                 cs.imp[c] = it.imp;
                 if (it.iv.x != 0.0f) body[it.id.x - b0] = A;
@@ -1223,15 +1214,15 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
                 L, rng, K,
                 [&](PosRec& pr, uint32_t c) {
                     pr.id = cs.idx[c];
-                    const float4 nr = cs.nrm[c], iv = cs.inv[c];
+                    const float4 nr = cs.nrm[c];
                     pr.n = make_float2(nr.x, nr.y);
-                    pr.im = make_float2(iv.x, iv.y);
                     pr.penBase = cs.k1[c].z;
                 },
                 [&](PosRec& pr, uint32_t) {
                     if (!ours(pr.id)) return;
                     PosItem pi;
-                    pi.nx = pr.n.x; pi.ny = pr.n.y; pi.iMA = pr.im.x; pi.iMB = pr.im.y; pi.penBase = pr.penBase;
+                    pi.nx = pr.n.x; pi.ny = pr.n.y; pi.penBase = pr.penBase;
+                    pi.iMA = binv[pr.id.x - b0].x; pi.iMB = binv[pr.id.y - b0].x;
                     float2 A = bp[pr.id.x - b0], B = bp[pr.id.y - b0];
                     positionStep(pi, A, B);
                     if (pi.iMA != 0.0f) bp[pr.id.x - b0] = A;
@@ -1244,15 +1235,17 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
 }
 
 // Scenes are taken in the step's order (kSceneWorkOrder): the first ctr->sceneWhole by whole
-// blocks, the rest (none larger than groupBodies bodies) by the blocks' lane groups (groups per block, each
-// with groupBodies body records of LDS) from a second cursor. A scene's results do not depend on
+// blocks, the rest (none larger than groupBodies bodies) by the blocks' lane groups (groups per
+// block, each with room for groupBodies bodies) from a second cursor. LDS per body: its velocity
+// (then position) record and its inverse mass and inertia. A scene's results do not depend on
 // which block or group solves it.
 __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restrict__ ctr, Constraints cs, float4* __restrict__ vel,
-                                                             float4* __restrict__ pos, const uint32_t* __restrict__ sceneBodyStart,
+                                                             float4* __restrict__ pos, const float2* __restrict__ inv,
+                                                             const uint32_t* __restrict__ sceneBodyStart,
                                                              const uint32_t* __restrict__ sceneOrder,
                                                              const uint32_t* __restrict__ sceneDyn, const uint2* __restrict__ range,
                                                              float2* __restrict__ sceneRes, uint32_t* __restrict__ sceneSweeps,
-                                                             uint32_t S, uint32_t groups, uint32_t groupBodies,
+                                                             uint32_t S, uint32_t maxBodies, uint32_t groups, uint32_t groupBodies,
                                                              uint32_t vIt, uint32_t pIt, int track, float residualTarget) {
     extern __shared__ float4 body[];   // the scene's body velocities, then positions (x, y); per group
     __shared__ uint2 rng[SCENE_GROUPS][MAX_COLOURS];
@@ -1266,6 +1259,7 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
     if (threadIdx.x == 0) abort = 0u;
     SceneLanes L{ threadIdx.x, SCENE_THREADS, 0u, SCENE_THREADS / WAVE, true, nullptr, &abort, 0u, ctr };
     float4* lds = body;
+    uint32_t cap = maxBodies;   // bodies of the current LDS region
     bool grouped = false;
     while (true) {   // one call site of solveScene for both phases
         if (L.t == 0) taken[L.g] = grouped ? nWhole + atomicAdd(&ctr->sceneCursor2, 1u) : atomicAdd(&ctr->sceneCursor, 1u);
@@ -1281,14 +1275,16 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
             L.waves = T / WAVE;
             L.whole = groups == 1;
             L.bar = &bar[L.g];
-            lds = body + (size_t)L.g * groupBodies;
+            lds = reinterpret_cast<float4*>(reinterpret_cast<char*>(body) + (size_t)L.g * groupBodies * 24u);
+            cap = groupBodies;
             grouped = true;
             continue;
         }
         if (k0 >= S) break;
         const uint32_t s = sceneOrder[k0];
         const uint32_t done = solveScene(L, s, ctr, cs, vel, pos, sceneBodyStart[s], sceneBodyStart[s + 1], range, sceneRes, K,
-                                         vIt, pIt, track, residualTarget, lds, rng[L.g], part + L.g * L.waves);
+                                         vIt, pIt, track, residualTarget, lds, reinterpret_cast<float2*>(lds + cap), inv,
+                                         rng[L.g], part + L.g * L.waves);
         if (L.t == 0) {
             sceneSweeps[s] = done;
             atomicMax(&ctr->sweeps, done);
@@ -1765,7 +1761,8 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
     // all the LDS a block may have (the block holds the CU alone): the largest scene's body records
     // for a whole block, or `groups` smaller scenes side by side
     const size_t bytes = (limit - staticBytes) & ~(size_t)255;
-    if (std::max<size_t>(maxBodies, 1) * sizeof(float4) > bytes) return 0;
+    constexpr size_t BODY_LDS = sizeof(float4) + sizeof(float2);   // velocity or position record, inverse mass/inertia
+    if (std::max<size_t>(maxBodies, 1) * BODY_LDS > bytes) return 0;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(kSceneSolve), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -1801,11 +1798,11 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         if (S <= WORK_SORT_MAX) {
             out.groups = SCENE_GROUPS;   // scenes larger than a group holds go to whole blocks
         } else {                         // size order: the scenes after the whole ones fit
-            const size_t next = std::max<size_t>(orderedBodies[out.whole], 1) * sizeof(float4);
+            const size_t next = std::max<size_t>(orderedBodies[out.whole], 1) * BODY_LDS;
             while (out.groups < SCENE_GROUPS && 2 * out.groups * next <= bytes) out.groups *= 2;
         }
     }
-    out.groupBodies = (uint32_t)(bytes / out.groups / sizeof(float4));
+    out.groupBodies = (uint32_t)(bytes / out.groups / BODY_LDS) & ~3u;   // regions 16-byte aligned
     // colouring: the same whole scenes, then as many groups (at most 4) as leave room for three
     // manifolds per body of the largest remaining scene
     int colourPerCu = 0;
@@ -1843,8 +1840,8 @@ hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32
     // every block takes a whole scene first; with fewer scenes than blocks, no more blocks than scenes
     const uint32_t blocks = (uint32_t)std::max(1, std::min<int>(sl.blocks, (int)std::max(w.S, sl.whole)));
     hipLaunchKernelGGL(kSceneSolve, dim3(blocks), dim3(SCENE_THREADS), w.sceneLdsBytes, st, w.ctr, w.cs, w.vel, w.pos,
-                       w.sceneBodyStart, w.sceneWorkOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S,
-                       sl.groups, sl.groupBodies, vIt, pIt, track, residualTarget);
+                       w.inv, w.sceneBodyStart, w.sceneWorkOrder, w.sceneDyn, w.sceneRange, w.sceneRes, w.sceneSweeps, w.S,
+                       std::max(w.sceneMaxBodies, 1u), sl.groups, sl.groupBodies, vIt, pIt, track, residualTarget);
     hipLaunchKernelGGL(kSceneTotals, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }
