@@ -17,7 +17,7 @@ constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataf
 constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour start on a wave boundary
 constexpr uint32_t MAX_SWEEPS = 64;
 constexpr uint32_t TRAVERSAL_STACK = 64;
-constexpr uint32_t SORT_TILE = 1024;        // keys per radix-sort block (256 threads x 4)
+constexpr uint32_t SORT_TILE = 4096;        // keys per radix-sort block (256 threads x 16)
 constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one scene per chunk)
 
 // Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void kRadixDigitScan(uint32_t* __restrict__ di
     }
 }
 
-// Stable scatter of one digit. Each wave ranks 16 rounds of 64 consecutive keys with ballot
+// Stable scatter of one digit. Each wave ranks SORT_TILE / 256 rounds of 64 consecutive keys with ballot
 // matching (8 ballots per round) and per-wave digit counters in LDS; waves are combined in order,
 // earlier tiles by look-back, earlier digits by the scanned totals.
 __global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,

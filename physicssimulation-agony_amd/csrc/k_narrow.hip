@@ -521,8 +521,15 @@ __global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__
         cur[scan[p]] = m;
         if (histOut) histOut[scan[p]] = hist;
     }
+    // one update of the shared counter per block (per wave, the updates queue on its address)
+    __shared__ uint32_t waveContacts[256 / 64];
     for (uint32_t off = 32; off > 0; off >>= 1) contacts += __shfl_down(contacts, off, 64);
-    if ((threadIdx.x & 63u) == 0 && contacts) atomicAdd(&ctr->contacts, contacts);
+    if ((threadIdx.x & 63u) == 0) waveContacts[threadIdx.x / 64] = contacts;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = waveContacts[0] + waveContacts[1] + waveContacts[2] + waveContacts[3];
+        if (total) atomicAdd(&ctr->contacts, total);
+    }
 }
 
 __global__ void kSetManifolds(Counters* ctr) { ctr->manifolds = ctr->scratch[2]; }
