@@ -438,27 +438,83 @@ __device__ bool boxPolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold
 }
 
 // NarrowPhaseCollisionDetector.cpp:653-667 canonical order, then SymmetricMatrix dispatch.
+// A block takes NARROW_CHUNK pairs at a time, bins them in LDS by shape combination and then runs
+// the bins in order, so the lanes of a wave mostly take the same collision method (a wave runs
+// every method any of its lanes needs). Results go to the pair's own slot, so the binning order
+// does not show in them.
+constexpr uint32_t NARROW_CHUNK = 2048;
+constexpr uint32_t NARROW_PER_LANE = NARROW_CHUNK / 256;
+constexpr uint32_t NO_METHOD = 7u;
 __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, Counters* __restrict__ ctr, uint64_t cap,
                                                Shapes s, agx_manifold* __restrict__ slots, uint32_t* __restrict__ flags) {
+    __shared__ uint2 binAB[NARROW_CHUNK];       // canonical (A, B) of the chunk's pairs, by method
+    __shared__ uint32_t binP[NARROW_CHUNK];     // pair index | method << 29
+    __shared__ uint32_t count[8], start[8];
     const uint32_t n = (uint32_t)min((uint64_t)ctr->pairs, cap);
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t a = pairs[p].x, b = pairs[p].y;
-        uint32_t ta = s.shape[a].x, tb = s.shape[b].x;
-        if (ta > tb) { const uint32_t t = a; a = b; b = t; const uint32_t u = ta; ta = tb; tb = u; }
-        else if (ta == tb && b > a) { const uint32_t t = a; a = b; b = t; }
-        agx_manifold m;
-        bool hit;
-        const uint32_t slot = tb * (tb + 1) / 2 + ta;
-        switch (slot) {
-        case 0: hit = circleCircle(s, a, b, m); break;
-        case 1: hit = circleBox(s, a, b, m); break;
-        case 2: hit = boxBox(s, a, b, m); break;
-        case 3: hit = circlePolygon(s, a, b, m); break;
-        case 4: hit = boxPolygon(s, a, b, m); break;
-        default: hit = polygonPolygon(s, a, b, m); break;
+    const uint32_t t = threadIdx.x, lane = t & (WAVE - 1);
+    const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;
+    for (uint32_t base = blockIdx.x * NARROW_CHUNK; base < n; base += gridDim.x * NARROW_CHUNK) {   // block-uniform
+        if (t < 8) count[t] = 0u;
+        __syncthreads();
+        uint32_t method[NARROW_PER_LANE], rank[NARROW_PER_LANE];
+        uint2 ab[NARROW_PER_LANE];
+        for (uint32_t r = 0; r < NARROW_PER_LANE; r++) {
+            const uint32_t p = base + r * 256 + t;
+            method[r] = NO_METHOD;
+            ab[r] = make_uint2(0u, 0u);
+            if (p < n) {
+                uint32_t a = pairs[p].x, b = pairs[p].y;
+                uint32_t ta = s.shape[a].x, tb = s.shape[b].x;
+                if (ta > tb) { const uint32_t x = a; a = b; b = x; const uint32_t u = ta; ta = tb; tb = u; }
+                else if (ta == tb && b > a) { const uint32_t x = a; a = b; b = x; }
+                method[r] = tb * (tb + 1) / 2 + ta;   // 0..5
+                ab[r] = make_uint2(a, b);
+            }
+            // rank within the method's bin: one LDS atomic per method present in the wave
+            rank[r] = 0u;
+            for (uint32_t c = 0; c < 6; c++) {
+                const uint64_t m = __ballot(method[r] == c);
+                if (!m) continue;
+                const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+                uint32_t first = 0;
+                if (lane == leader) first = atomicAdd(&count[c], (uint32_t)__popcll(m));
+                first = __shfl(first, leader, WAVE);
+                if (method[r] == c) rank[r] = first + (uint32_t)__popcll(m & below);
+            }
         }
-        if (hit) slots[p] = m;
-        flags[p] = hit ? 1u : 0u;
+        __syncthreads();
+        if (t == 0) {
+            uint32_t run = 0;
+            for (uint32_t c = 0; c < 6; c++) { start[c] = run; run += count[c]; }
+            start[6] = run;
+        }
+        __syncthreads();
+        for (uint32_t r = 0; r < NARROW_PER_LANE; r++) {
+            if (method[r] == NO_METHOD) continue;
+            const uint32_t e = start[method[r]] + rank[r];
+            binAB[e] = ab[r];
+            binP[e] = (base + r * 256 + t) | (method[r] << 29);
+        }
+        __syncthreads();
+        const uint32_t total = start[6];
+        for (uint32_t e = t; e < total; e += 256) {
+            const uint2 pab = binAB[e];
+            const uint32_t pm = binP[e];
+            const uint32_t p = pm & ((1u << 29) - 1u), c = pm >> 29;
+            agx_manifold m;
+            bool hit;
+            switch (c) {
+            case 0: hit = circleCircle(s, pab.x, pab.y, m); break;
+            case 1: hit = circleBox(s, pab.x, pab.y, m); break;
+            case 2: hit = boxBox(s, pab.x, pab.y, m); break;
+            case 3: hit = circlePolygon(s, pab.x, pab.y, m); break;
+            case 4: hit = boxPolygon(s, pab.x, pab.y, m); break;
+            default: hit = polygonPolygon(s, pab.x, pab.y, m); break;
+            }
+            if (hit) slots[p] = m;
+            flags[p] = hit ? 1u : 0u;
+        }
+        __syncthreads();   // before the next round reuses the bins
     }
 }
 

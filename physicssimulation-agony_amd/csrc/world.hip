@@ -172,6 +172,8 @@ int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, con
 
 int allocPairBuffers(agx_world* w, uint64_t pairCap) {
     agx::DeviceWorld& d = w->d;
+    // the narrow phase packs a pair index into 29 bits
+    if (pairCap >= (1ull << 29)) return fail(w, AGX_ERR_CAPACITY, "pair capacity above 2^29 pairs");
     d.pairCap = pairCap;
     d.queryWaves = (d.nDyn + 63) / 64;
     HIP_CHECK(w, devAlloc(w, &d.pairs, pairCap));
