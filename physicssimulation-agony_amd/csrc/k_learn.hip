@@ -139,9 +139,11 @@ __device__ inline uint32_t claimRows(uint32_t count, uint32_t* counter) {
 }
 
 // Block-wide claim of rows on `counter`: the first of this thread's `count` rows. Every thread of
-// the (256-thread) block must call it; one atomic per block keeps the counter uncontended.
+// the (FEATURE_THREADS-thread) block must call it; one atomic per block, since updates of one
+// address from many blocks queue behind each other.
+constexpr uint32_t FEATURE_THREADS = 1024;
 __device__ inline uint32_t claimRowsBlock(uint32_t count, uint32_t* counter) {
-    __shared__ uint32_t waveTotal[4];
+    __shared__ uint32_t waveTotal[FEATURE_THREADS / WAVE];
     __shared__ uint32_t blockBase;
     const uint32_t lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     uint32_t prefix = count;   // inclusive prefix over the wave
@@ -152,7 +154,8 @@ __device__ inline uint32_t claimRowsBlock(uint32_t count, uint32_t* counter) {
     if (lane == WAVE - 1) waveTotal[wid] = prefix;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t total = waveTotal[0] + waveTotal[1] + waveTotal[2] + waveTotal[3];
+        uint32_t total = 0;
+        for (uint32_t k = 0; k < FEATURE_THREADS / WAVE; k++) total += waveTotal[k];
         blockBase = total ? atomicAdd(counter, total) : 0u;
     }
     __syncthreads();
@@ -167,7 +170,7 @@ __device__ inline uint32_t claimRowsBlock(uint32_t count, uint32_t* counter) {
 // newOnly (no history kept) one per contact point without a cached impulse, in claimed rows with
 // map[row] = 2m + p.
 // Runs over solve positions, so the constraint planes are read in order (holes are skipped).
-__global__ __launch_bounds__(256) void kLearnFeatures(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
+__global__ __launch_bounds__(FEATURE_THREADS) void kLearnFeatures(Counters* __restrict__ ctr, const agx_manifold* __restrict__ man,
                                                       const uint32_t* __restrict__ degree, const float4* __restrict__ vel,
                                                       Constraints cs, Scale sc, const float4* __restrict__ histOf,
                                                       float4* __restrict__ warmHist, uint32_t newOnly,
@@ -366,8 +369,8 @@ hipError_t launchLearnBeforeWarm(hipStream_t st, DeviceWorld& w, const MlpDevice
                                  uint32_t collect, uint32_t newOnly, float gx, float gy, float dt) {
     hipLaunchKernelGGL(kLearnRows, dim3(1), dim3(1), 0, st, w.ctr, predict && mlp ? 1u : 0u, gated, collect);
     const Scale sc = scaleOf(gx, gy, dt);
-    const uint32_t grid = gridOf(std::min<uint64_t>(w.manCap, 1u << 22));
-    hipLaunchKernelGGL(kLearnFeatures, dim3(grid), dim3(256), 0, st, w.ctr, w.man[w.cur], w.degree, w.vel, w.cs, sc,
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((w.manCap + FEATURE_THREADS - 1) / FEATURE_THREADS, 4096));
+    hipLaunchKernelGGL(kLearnFeatures, dim3(grid), dim3(FEATURE_THREADS), 0, st, w.ctr, w.man[w.cur], w.degree, w.vel, w.cs, sc,
                        w.histOf, w.warmHist[w.cur], newOnly, w.learnX, w.learnMap);
     if (newOnly) hipLaunchKernelGGL(kLearnCompactRows, dim3(1), dim3(1), 0, st, w.ctr);
     if (predict && mlp) {
