@@ -17,7 +17,9 @@ constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataf
 constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour start on a wave boundary
 constexpr uint32_t MAX_SWEEPS = 64;
 constexpr uint32_t TRAVERSAL_STACK = 64;
-constexpr uint32_t SORT_TILE = 4096;        // keys per radix-sort block (256 threads x 16)
+constexpr uint32_t SORT_TILE_SMALL = 1024;  // keys per radix-sort block (256 threads x 4) ...
+constexpr uint32_t SORT_TILE_LARGE = 4096;  // ... and for trees of at least SORT_LARGE_KEYS keys (x 16):
+constexpr uint32_t SORT_LARGE_KEYS = 1u << 19;   // fewer tiles shorten the look-back, more keep CUs busy
 constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one scene per chunk)
 
 // Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).
@@ -130,6 +132,7 @@ constexpr uint32_t STAGE_PAIRS = 1024;   // pairs a traversal wave stages before
 // internal nodes [0, n-1), leaves [n-1, 2n-1); node 0 is the root (a leaf when n == 1).
 struct Tree {
     uint32_t n = 0;
+    uint32_t sortTile = 0;         // keys per radix-sort block (SORT_TILE_SMALL or SORT_TILE_LARGE)
     uint32_t sortBlocks = 0;
     uint32_t* keys = nullptr;      // sorted (scene << mortonBits | morton)
     uint32_t* vals = nullptr;      // sorted collider indices

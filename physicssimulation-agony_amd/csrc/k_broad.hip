@@ -98,12 +98,12 @@ constexpr uint32_t STATUS_AGG = 1u << 30, STATUS_INC = 2u << 30, STATUS_VALUE = 
 constexpr uint64_t SORT_STALL_TICKS = 200000000ull;   // 2 s of the 100 MHz wall clock
 
 __global__ __launch_bounds__(256) void kRadixHistAll(const uint32_t* __restrict__ keys, uint32_t n, uint32_t passes,
-                                                     uint32_t* __restrict__ digitTotal) {
+                                                     uint32_t tileKeys, uint32_t* __restrict__ digitTotal) {
     __shared__ uint32_t h[SORT_PASSES_MAX][256];
     for (uint32_t p = 0; p < SORT_PASSES_MAX; p++) h[p][threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * SORT_TILE;
-    for (uint32_t k = threadIdx.x; k < SORT_TILE; k += 256) {
+    const uint32_t base = blockIdx.x * tileKeys;
+    for (uint32_t k = threadIdx.x; k < tileKeys; k += 256) {
         const uint32_t i = base + k;
         if (i >= n) break;
         const uint32_t key = keys[i];
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(256) void kRadixDigitScan(uint32_t* __restrict__ di
 // Stable scatter of one digit. Each wave ranks SORT_TILE / 256 rounds of 64 consecutive keys with ballot
 // matching (8 ballots per round) and per-wave digit counters in LDS; waves are combined in order,
 // earlier tiles by look-back, earlier digits by the scanned totals.
+template <uint32_t SORT_TILE>
 __global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                      uint32_t n, uint32_t shift, const uint32_t* __restrict__ digitStart,
                                                      uint32_t* status, uint32_t* ticket, uint32_t* overflow,
@@ -455,12 +456,17 @@ hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t) {
     uint32_t* tickets = digitTotal + SORT_PASSES_MAX * 256;
     hipError_t e = launchFillU32(st, t.hist, 0u, (size_t)SORT_PASSES_MAX * (t.sortBlocks + 1) * 256 + SORT_PASSES_MAX);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kRadixHistAll, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.n, passes, digitTotal);
+    hipLaunchKernelGGL(kRadixHistAll, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.n, passes, t.sortTile, digitTotal);
     hipLaunchKernelGGL(kRadixDigitScan, dim3(1), dim3(256), 0, st, digitTotal, passes);
     for (uint32_t p = 0; p < passes; p++) {
-        hipLaunchKernelGGL(kRadixScatter, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, p * 8,
-                           digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p, &w.ctr->overflow,
-                           t.keysAlt, t.valsAlt);
+        if (t.sortTile == SORT_TILE_LARGE)
+            hipLaunchKernelGGL(kRadixScatter<SORT_TILE_LARGE>, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, p * 8,
+                               digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p, &w.ctr->overflow,
+                               t.keysAlt, t.valsAlt);
+        else
+            hipLaunchKernelGGL(kRadixScatter<SORT_TILE_SMALL>, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, p * 8,
+                               digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p, &w.ctr->overflow,
+                               t.keysAlt, t.valsAlt);
         std::swap(t.keys, t.keysAlt);
         std::swap(t.vals, t.valsAlt);
     }

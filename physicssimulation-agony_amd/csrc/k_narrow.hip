@@ -438,12 +438,14 @@ __device__ bool boxPolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold
 }
 
 // NarrowPhaseCollisionDetector.cpp:653-667 canonical order, then SymmetricMatrix dispatch.
-// A block takes NARROW_CHUNK pairs at a time, bins them in LDS by shape combination and then runs
-// the bins in order, so the lanes of a wave mostly take the same collision method (a wave runs
-// every method any of its lanes needs). Results go to the pair's own slot, so the binning order
-// does not show in them.
+// A block takes up to NARROW_CHUNK pairs at a time, bins them in LDS by shape combination and then
+// runs the bins in order, so the lanes of a wave mostly take the same collision method (a wave runs
+// every method any of its lanes needs). Chunks shrink when there are too few pairs to keep about
+// NARROW_BUSY_BLOCKS blocks working. Results go to the pair's own slot, so the binning order does
+// not show in them.
 constexpr uint32_t NARROW_CHUNK = 2048;
 constexpr uint32_t NARROW_PER_LANE = NARROW_CHUNK / 256;
+constexpr uint32_t NARROW_BUSY_BLOCKS = 1024;
 constexpr uint32_t NO_METHOD = 7u;
 __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, Counters* __restrict__ ctr, uint64_t cap,
                                                Shapes s, agx_manifold* __restrict__ slots, uint32_t* __restrict__ flags) {
@@ -453,16 +455,20 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
     const uint32_t n = (uint32_t)min((uint64_t)ctr->pairs, cap);
     const uint32_t t = threadIdx.x, lane = t & (WAVE - 1);
     const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;
-    for (uint32_t base = blockIdx.x * NARROW_CHUNK; base < n; base += gridDim.x * NARROW_CHUNK) {   // block-uniform
+    const uint32_t busy = min(gridDim.x, NARROW_BUSY_BLOCKS);
+    const uint32_t perLane = max(1u, min(NARROW_PER_LANE, (n + busy * 256 - 1) / (busy * 256)));
+    const uint32_t chunk = 256 * perLane;
+    for (uint32_t base = blockIdx.x * chunk; base < n; base += gridDim.x * chunk) {   // block-uniform
         if (t < 8) count[t] = 0u;
         __syncthreads();
         uint32_t method[NARROW_PER_LANE], rank[NARROW_PER_LANE];
         uint2 ab[NARROW_PER_LANE];
+#pragma unroll
         for (uint32_t r = 0; r < NARROW_PER_LANE; r++) {
             const uint32_t p = base + r * 256 + t;
             method[r] = NO_METHOD;
             ab[r] = make_uint2(0u, 0u);
-            if (p < n) {
+            if (r < perLane && p < n) {
                 uint32_t a = pairs[p].x, b = pairs[p].y;
                 uint32_t ta = s.shape[a].x, tb = s.shape[b].x;
                 if (ta > tb) { const uint32_t x = a; a = b; b = x; const uint32_t u = ta; ta = tb; tb = u; }
@@ -489,6 +495,7 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
             start[6] = run;
         }
         __syncthreads();
+#pragma unroll
         for (uint32_t r = 0; r < NARROW_PER_LANE; r++) {
             if (method[r] == NO_METHOD) continue;
             const uint32_t e = start[method[r]] + rank[r];

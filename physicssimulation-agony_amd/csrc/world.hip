@@ -136,7 +136,8 @@ uint32_t ceilLog2(uint32_t v) {
 int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, const std::vector<uint32_t>& colScene,
               uint32_t S) {
     t.n = (uint32_t)list.size();
-    t.sortBlocks = std::max(1u, (t.n + agx::SORT_TILE - 1) / agx::SORT_TILE);
+    t.sortTile = t.n >= agx::SORT_LARGE_KEYS ? agx::SORT_TILE_LARGE : agx::SORT_TILE_SMALL;
+    t.sortBlocks = std::max(1u, (t.n + t.sortTile - 1) / t.sortTile);
     std::vector<uint32_t> sceneStart(S + 1, 0);
     for (uint32_t c : list) sceneStart[colScene[c] + 1]++;
     for (uint32_t s = 0; s < S; s++) sceneStart[s + 1] += sceneStart[s];
