@@ -204,6 +204,9 @@ constexpr uint32_t WORK_SORT_MAX = 4096;   // scenes the per-step work order sor
 // A value every lane of the wave holds, in a scalar register.
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The lanes that work on one scene: the whole block (block barriers) for the largest scenes, else
+// a group of the block's lanes, whose waves meet at an arrival counter in LDS. Every lane of the
+// group must call sync() the same number of times.
 struct SceneLanes {
     uint32_t t;        // lane in the group
     uint32_t T;        // lanes in the group
@@ -1061,8 +1064,6 @@ __global__ __launch_bounds__(256) void kSceneRanges(const Counters* __restrict__
 
 constexpr uint32_t SCENE_THREADS = 1024;   // one block per CU
 
-// The lanes that solve one scene: the whole block (block barriers) for the largest scenes, else a
-// group of SCENE_THREADS / groups lanes, whose waves meet at an arrival counter in LDS.
 // Sum of (num, den) over the group's lanes: a butterfly within each wave (every lane ends with
 // the same bits), then the waves' partials in wave order; every lane returns the total.
 __device__ inline double2 sceneSum(SceneLanes& L, double num, double den, double2* part) {
