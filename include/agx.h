@@ -60,7 +60,7 @@ typedef struct agx_settings {
     uint32_t collide_only;                 /* 1: integrate + collide, skip the solver (broad/narrow timing) */
     uint32_t solve_per_scene;              /* 1 (default): each scene solved by one workgroup with its bodies
                                               in LDS, stopping at its own residual, whenever every scene fits
-                                              (about 10,000 bodies); 0: one dataflow over all scenes (one
+                                              (about 6,500 bodies); 0: one dataflow over all scenes (one
                                               residual for the batch), also used for larger scenes. Fixed
                                               sweep counts give the same bits either way. */
 } agx_settings;
