@@ -254,10 +254,15 @@ __global__ __launch_bounds__(256) void kCcd(const uint32_t* __restrict__ order, 
         stack[sp++] = ch.y;
         stack[sp++] = ch.x;
     }
+    bool first = false;   // this cast gave the body its first hit of the step
     if (best < 1.0f) {
         const uint32_t old = atomicMin(reinterpret_cast<uint32_t*>(hitTime) + body, __float_as_uint(best));
-        if (__uint_as_float(old) >= 1.0f) atomicAdd(&ctr->ccdHits, 1u);
+        first = __uint_as_float(old) >= 1.0f;
     }
+    // one update of the hit counter per wave (updates of one address from every lane queue there)
+    const uint64_t hits = __ballot(first);
+    if (first && (threadIdx.x & (WAVE - 1)) == (uint32_t)(__ffsll((unsigned long long)hits) - 1))
+        atomicAdd(&ctr->ccdHits, (uint32_t)__popcll(hits));
 }
 
 // CCD/ContinuousCollisionDetector.cpp:513-529: move hit colliders back along their body's
