@@ -427,7 +427,7 @@ hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // back to the manifolds, residual and error totals.
 // Per-scene solve: its dynamic LDS bytes (0 when the largest scene does not fit) and launch shape,
 // for scenes whose body counts, largest first, are orderedBodies; wholeCap: how many scenes whole
-// blocks take (UINT32_MAX: one per resident block).
+// blocks take (UINT32_MAX: one and a half per resident block).
 uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, uint32_t wholeCap, SceneLaunch& out,
                          uint32_t& colourLdsBytes);
 hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
