@@ -192,6 +192,12 @@ def roofline(seg_ms, st, counts, settings, mean_v, workload, sweeps):
             "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(per_launch_ms, 4)}
 
 
+def errors_of(st):
+    """computeConstraintErrors of the last step (Simulation.cpp:506-515): summed approach velocity
+    and penetration over all contact points (the drift the sweeps left)."""
+    return {"velocity": round(float(st.velocity_error), 6), "penetration": round(float(st.position_error), 6)}
+
+
 def mean_poly_verts(scenes):
     v = np.concatenate([s["bodies"]["vcount"] for s in scenes])
     v = v[v > 0]
@@ -330,6 +336,7 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
     mean_plain = mean_sweeps(plain_w, st0, args.steps)
     out["plain"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
                     "mean_velocity_sweeps": round(mean_plain, 2), "velocity_sweeps_last_step": int(st.velocity_sweeps),
+                    "constraint_errors_last_step": errors_of(st),
                     "residual_per_sweep_last_step": [round(float(st.residual[k]), 6) for k in range(st.velocity_sweeps)]}
     if learned:
         st0 = lw.stats()
@@ -339,6 +346,7 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
         mean_learned = mean_sweeps(lw, st0, args.steps)
         out["learned"] = {"value": round(n * args.steps / el, 1), "ms_per_step": round(1e3 * el / args.steps, 4),
                           "mean_velocity_sweeps": round(mean_learned, 2), "overflow": int(st.overflow),
+                          "constraint_errors_last_step": errors_of(st),
                           "sweep_reduction": round(1.0 - mean_learned / max(mean_plain, 1e-9), 4),
                           "gate": {"cost_cached": round(g.cost_cached, 3), "cost_learned": round(g.cost_learned, 3),
                                    "steps_cached": int(g.steps_cached), "steps_learned": int(g.steps_learned)},
@@ -422,6 +430,7 @@ def run_pile(agx, scenes_mod, args, dt, cpu):
            "segments_ms_per_step": {k: round(v, 4) for k, v in seg.items()},
            "tuning": {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu},
            "roofline": roofline(seg, st, world.counts, settings, mean_poly_verts([scene]), "pile65536", sweeps),
+           "constraint_errors_last_step": errors_of(st),
            "stats": {"pairs": int(st.pairs), "manifolds": int(st.manifolds), "colours": int(st.colours),
                      "islands": int(st.islands), "overflow": int(st.overflow),
                      "residual_per_sweep": [round(float(st.residual[k]), 6) for k in range(st.velocity_sweeps)]}}
