@@ -307,9 +307,12 @@ __device__ inline void colourScene(SceneLanes& L, uint32_t s, const agx_manifold
     const uint32_t b0 = sceneBodyStart[s], b1 = sceneBodyStart[s + 1], nb = b1 - b0;
     const uint32_t maxBodies = ldsManifolds >> 16;   // LDS layout: body records for the largest scene first
     const uint32_t capM = ldsManifolds & 0xFFFFu;
-    uint32_t* minPrio = lds;                         // nb: smallest priority of an uncoloured manifold at the body
-    uint32_t* mask = lds + maxBodies;                // nb x 4: colours taken at the body
-    uint32_t* packedL = lds + 5 * maxBodies;         // capM: the scene's manifolds (bodies, flags)
+    // nb: the round's smallest priority of an uncoloured manifold at the body, tagged with the
+    // round (~round in the high word, so an entry of an earlier round loses to any of this round's
+    // and needs no clearing)
+    unsigned long long* minPrio = reinterpret_cast<unsigned long long*>(lds);
+    uint32_t* mask = lds + 2 * maxBodies;            // nb x 4: colours taken at the body
+    uint32_t* packedL = lds + 6 * maxBodies;         // capM: the scene's manifolds (bodies, flags)
     uint16_t* colourL = reinterpret_cast<uint16_t*>(packedL + capM);   // capM colours
     const uint2 r = manRange[s];
     const uint32_t nm = r.y > r.x ? r.y - r.x : 0u;
@@ -322,9 +325,10 @@ __device__ inline void colourScene(SceneLanes& L, uint32_t s, const agx_manifold
         return inLds ? packed[i] != 0xFFFFFFFFu : manBodies[r.x + i].x - b0 < nb;
     };
     for (uint32_t i = t; i < nb; i += T) {
-        minPrio[i] = 0xFFFFFFFFu;
+        minPrio[i] = ~0ull;
         for (uint32_t k = 0; k < 4; k++) mask[4 * i + k] = 0u;
     }
+    if (t == 0) { left[0] = 0; left[1] = 0; }
     L.sync();
     // the scene's manifolds: bodies, degrees (counted in the first mask word), uncoloured
     for (uint32_t i = t; i < nm; i += T) {
@@ -348,25 +352,28 @@ __device__ inline void colourScene(SceneLanes& L, uint32_t s, const agx_manifold
         mask[4 * i] = 0u;
     }
     L.sync();
-    while (true) {
-        if (t == 0) *left = 0;
-        L.sync();
+    // round q: every uncoloured manifold posts its priority at its dynamic bodies and flags
+    // left[q & 1]; after a sync the ones holding both minima take their colours. left[(q + 1) & 1]
+    // is cleared during round q, which reads only left[q & 1].
+    for (uint32_t q = 0;; q++) {
+        const unsigned long long tag = (unsigned long long)(0xFFFFFFFFu - q) << 32;
         for (uint32_t i = t; i < nm; i += T) {
             if (!own(i)) continue;
             const uint32_t pk = packed[i];
             if (col[i] != UNCOLOURED) continue;
-            const uint32_t pr = mix32(r.x + i);
+            const unsigned long long pr = tag | mix32(r.x + i);
             if (pk & 0x8000u) atomicMin(&minPrio[pk & 0x7FFFu], pr);
             if (pk & 0x80000000u) atomicMin(&minPrio[(pk >> 16) & 0x7FFFu], pr);
-            *left = 1;
+            left[q & 1] = 1;
         }
+        if (t == 0) left[(q + 1) & 1] = 0;
         L.sync();
-        if (!*left) break;
+        if (!left[q & 1]) break;
         for (uint32_t i = t; i < nm; i += T) {
             if (!own(i)) continue;
             const uint32_t pk = packed[i];
             if (col[i] != UNCOLOURED) continue;
-            const uint32_t pr = mix32(r.x + i);
+            const unsigned long long pr = tag | mix32(r.x + i);
             const uint32_t la = pk & 0x7FFFu, lb = (pk >> 16) & 0x7FFFu;
             const bool da = (pk & 0x8000u) != 0, db = (pk & 0x80000000u) != 0;
             if ((da && minPrio[la] != pr) || (db && minPrio[lb] != pr)) continue;
@@ -382,8 +389,6 @@ __device__ inline void colourScene(SceneLanes& L, uint32_t s, const agx_manifold
             if (da) mask[4 * la + (c >> 5)] |= 1u << (c & 31u);
             if (db) mask[4 * lb + (c >> 5)] |= 1u << (c & 31u);
         }
-        L.sync();
-        for (uint32_t i = t; i < nb; i += T) minPrio[i] = 0xFFFFFFFFu;
         L.sync();
     }
     if (inLds)
@@ -403,8 +408,11 @@ __global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifol
                                                                uint32_t groupWords, uint2* __restrict__ manBodies,
                                                                uint32_t* __restrict__ degree, uint32_t* __restrict__ spill,
                                                                uint16_t* __restrict__ colourId) {
-    extern __shared__ uint32_t lds[];
-    __shared__ int left[SCENE_GROUPS];
+    // 16-byte aligned: the static arrays above it total 68 bytes, and the round-tagged minima are
+    // 64-bit LDS atomics, which need 8-byte alignment
+    extern __shared__ uint4 ldsBase[];
+    uint32_t* lds = reinterpret_cast<uint32_t*>(ldsBase);
+    __shared__ int left[SCENE_GROUPS][2];
     __shared__ uint32_t taken[SCENE_GROUPS];
     __shared__ uint32_t bar[SCENE_GROUPS];
     __shared__ uint32_t abort;
@@ -435,7 +443,7 @@ __global__ __launch_bounds__(COLOUR_THREADS) void kSceneColour(const agx_manifol
         }
         if (k0 >= S) break;
         colourScene(L, sceneOrder[k0], man, ctr, colBody, inv, sceneBodyStart, manRange, layout, manBodies, degree, spill,
-                    colourId, region, &left[L.g]);
+                    colourId, region, left[L.g]);
     }
 }
 
@@ -1769,14 +1777,14 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         (void)hipGetLastError();
         return 0;
     }
-    // the per-scene colouring, also one block per CU with all the LDS: per scene 20 bytes per body,
+    // the per-scene colouring, also one block per CU with all the LDS: per scene 24 bytes per body,
     // then as many of its manifolds (6 bytes each) as the rest of the region holds (a scene with
     // more colours them in global scratch)
     if (maxBodies >= 0x8000u) return 0;   // local body indices are packed in 15 bits
     const size_t colourStatic = 4 * SCENE_GROUPS * sizeof(uint32_t) + 256;
     const size_t colourBytes = (limit - colourStatic) & ~(size_t)255;
     auto capOf = [](size_t region, uint32_t bodies) -> size_t {   // manifolds after the body records
-        const size_t bodyBytes = (size_t)std::max(bodies, 1u) * 5 * sizeof(uint32_t) + 8;   // + rounding of the colours
+        const size_t bodyBytes = (size_t)std::max(bodies, 1u) * 6 * sizeof(uint32_t) + 8;   // + rounding of the colours
         return region > bodyBytes ? std::min<size_t>((region - bodyBytes) / 6, 0xFFFF) : 0;
     };
     if (capOf(colourBytes, maxBodies) < 64) return 0;
