@@ -80,6 +80,22 @@ def test_lane_groups_match_dataflow_bit_for_bit(agx, scenes, monkeypatch, whole_
     np.testing.assert_allclose(sp.residual[:8], sf.residual[:8], rtol=1e-4)
 
 
+def test_many_small_scenes_keep_size_order(agx, scenes):
+    """More scenes than the per-step work order sorts (4,096): the scenes go in size order, most of
+    them to lane groups; still the dataflow's colours and bits."""
+    parts = [scenes.pile(3, 2, 6.0, 6.0, seed=100 + i) for i in range(4200)]
+    base = dict(velocity_iterations=8, position_iterations=3)
+    flow = _world(agx, parts, solve_per_scene=0, **base)
+    per = _world(agx, parts, solve_per_scene=1, **base)
+    print("scenes", len(parts), "bodies", flow.counts.bodies, flush=True)
+    for n in (1, 20):
+        flow.step(DT, n)
+        per.step(DT, n)
+        _same(flow, per)
+        np.testing.assert_array_equal(flow.solve_order()[1], per.solve_order()[1])
+    assert per.stats().solved_per_scene == 1 and per.stats().overflow == 0 and per.stats().manifolds > 0
+
+
 def test_per_scene_colouring_from_global_scratch(agx, scenes, monkeypatch):
     """Scenes with more manifolds than the colouring keeps in LDS colour from global scratch
     (forced here with a tiny LDS cap): still the dataflow colouring's colours and bits."""
