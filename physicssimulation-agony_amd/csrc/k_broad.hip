@@ -420,7 +420,11 @@ __global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_
 __global__ __launch_bounds__(256) void kGatherPairs(const uint32_t* __restrict__ waveCount,
                                                     const uint32_t* __restrict__ waveOffset, uint32_t regions,
                                                     const uint2* __restrict__ staged, uint2* __restrict__ pairs,
-                                                    uint64_t cap) {
+                                                    uint64_t cap, Counters* ctr, const uint32_t* __restrict__ total) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the scanned pair total; no kernel of this phase reads it
+        ctr->pairs = *total;
+        if ((uint64_t)*total > cap) atomicOr(&ctr->overflow, 1u);
+    }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t region = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (region >= regions) return;
@@ -430,11 +434,6 @@ __global__ __launch_bounds__(256) void kGatherPairs(const uint32_t* __restrict__
     const uint2* src = staged + (size_t)region * STAGE_PAIRS;
     for (uint32_t k = lane; k < n; k += 64)
         if (base + k < cap) pairs[base + k] = src[k];
-}
-
-__global__ void kPairTotal(Counters* ctr, uint64_t cap, const uint32_t* total) {
-    ctr->pairs = *total;
-    if ((uint64_t)*total > cap) ctr->overflow |= 1u;
 }
 
 inline uint32_t blocksFor(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
@@ -495,12 +494,11 @@ hipError_t launchPairs(hipStream_t st, DeviceWorld& w) {
     hipError_t e = launchScan(st, w.waveCount, w.waveOffset, nullptr, total, 2ull * waves, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kGatherPairs, dim3(blocksFor(2 * waves, 4)), dim3(256), 0, st, w.waveCount, w.waveOffset, 2 * waves,
-                       w.stagedPairs, w.pairs, w.pairCap);
+                       w.stagedPairs, w.pairs, w.pairCap, w.ctr, total);
     hipLaunchKernelGGL((kTraverse<true, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
                        w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
     hipLaunchKernelGGL((kTraverse<false, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
                        w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
-    hipLaunchKernelGGL(kPairTotal, dim3(1), dim3(1), 0, st, w.ctr, w.pairCap, total);
     return hipGetLastError();
 }
 

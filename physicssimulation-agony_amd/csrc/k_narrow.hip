@@ -453,6 +453,7 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
     __shared__ uint32_t binP[NARROW_CHUNK];     // pair index | method << 29
     __shared__ uint32_t count[8], start[8];
     const uint32_t n = (uint32_t)min((uint64_t)ctr->pairs, cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->scratch[1] = n;   // the pairs the flag scan covers
     const uint32_t t = threadIdx.x, lane = t & (WAVE - 1);
     const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;
     const uint32_t busy = min(gridDim.x, NARROW_BUSY_BLOCKS);
@@ -525,10 +526,6 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
     }
 }
 
-__global__ void kClampPairs(Counters* ctr, uint64_t cap) {
-    ctr->scratch[1] = (uint32_t)min((uint64_t)ctr->pairs, cap);
-}
-
 __device__ inline unsigned long long pairKey(uint32_t sa, uint32_t sb) {   // NarrowPhaseCollisionDetector.h:215-218
     const unsigned long long lo = min(sa, sb), hi = max(sa, sb);
     return (lo << 32) | hi;
@@ -548,6 +545,7 @@ __global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__
                                                 const uint32_t* __restrict__ hashVals, const uint32_t* __restrict__ colSlot,
                                                 const float4* __restrict__ histIn, float4* __restrict__ histOut) {
     const uint32_t n = ctr->scratch[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->manifolds = ctr->scratch[2];   // the flag scan's total
     const uint32_t prevCount = ctr->prevManifolds;
     const uint32_t cap = ctr->hashCapacity;
     const uint32_t capLog2 = cap ? (uint32_t)__builtin_ctz(cap) : 0u;
@@ -595,20 +593,24 @@ __global__ __launch_bounds__(256) void kCompact(const agx_manifold* __restrict__
     }
 }
 
-__global__ void kSetManifolds(Counters* ctr) { ctr->manifolds = ctr->scratch[2]; }
 
 // rebuildContactCache (NarrowPhaseCollisionDetector.cpp:272-297): table sized to the manifold
 // count, only manifolds with a nonzero accumulator are keyed.
-__global__ void kHashSize(Counters* ctr, uint32_t capMax) {
-    const uint32_t m = ctr->manifolds;
+// Power of two, at least twice the key count, capped by the allocation.
+__device__ __forceinline__ uint32_t hashCapacityFor(uint32_t m, uint32_t capMax) {
     uint32_t cap = 16;
     while (cap < 2u * m && cap < capMax) cap <<= 1;
-    ctr->hashCapacity = cap;
-    ctr->prevManifolds = m;
+    return cap;
 }
 
-__global__ __launch_bounds__(256) void kHashClear(unsigned long long* keys, const Counters* ctr) {
-    const uint32_t cap = ctr->hashCapacity;
+// Sizes the table for this step's manifolds (the warm-start source of the next step) and clears it.
+__global__ __launch_bounds__(256) void kHashClear(unsigned long long* keys, Counters* ctr, uint32_t capMax) {
+    const uint32_t m = ctr->manifolds;
+    const uint32_t cap = hashCapacityFor(m, capMax);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // no thread of this kernel reads them
+        ctr->hashCapacity = cap;
+        ctr->prevManifolds = m;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) keys[i] = EMPTY_KEY;
 }
 
@@ -643,20 +645,17 @@ hipError_t launchNarrow(hipStream_t st, DeviceWorld& w) {
     const Shapes s{ w.pose, w.shape, w.dims, w.verts, w.normals };
     const uint32_t blocks = streamBlocks(w, w.pairCap);
     hipLaunchKernelGGL(kNarrow, dim3(blocks), dim3(256), 0, st, w.pairs, w.ctr, w.pairCap, s, w.slots, w.flags);
-    hipLaunchKernelGGL(kClampPairs, dim3(1), dim3(1), 0, st, w.ctr, w.pairCap);
     hipError_t e = launchScan(st, w.flags, w.flagScan, &w.ctr->scratch[1], &w.ctr->scratch[2], w.pairCap, w.scanBlock,
                               w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kCompact, dim3(blocks), dim3(256), 0, st, w.slots, w.flags, w.flagScan, w.ctr, w.man[w.cur],
                        w.man[1 - w.cur], w.hashKeys, w.hashVals, w.colSlot, w.histIn, w.histOut);
-    hipLaunchKernelGGL(kSetManifolds, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }
 
 hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w) {
-    hipLaunchKernelGGL(kHashSize, dim3(1), dim3(1), 0, st, w.ctr, (uint32_t)w.hashCapMax);
     const uint32_t blocks = streamBlocks(w, w.hashCapMax);
-    hipLaunchKernelGGL(kHashClear, dim3(blocks), dim3(256), 0, st, w.hashKeys, w.ctr);
+    hipLaunchKernelGGL(kHashClear, dim3(blocks), dim3(256), 0, st, w.hashKeys, w.ctr, (uint32_t)w.hashCapMax);
     hipLaunchKernelGGL(kHashInsert, dim3(streamBlocks(w, w.manCap)), dim3(256), 0, st, w.man[w.cur], w.ctr, w.hashKeys,
                        w.hashVals, w.colSlot);
     return hipGetLastError();

@@ -247,9 +247,15 @@ struct SceneLanes {
 constexpr uint32_t COLOUR_THREADS = 1024;   // one block per CU
 constexpr uint16_t UNCOLOURED = 0xFFFFu;
 
-__global__ __launch_bounds__(256) void kSceneManClear(uint2* __restrict__ range, uint32_t S) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < S; i += gridDim.x * blockDim.x)
-        range[i] = make_uint2(0xFFFFFFFFu, 0u);
+// Empties the scenes' manifold ranges (S) and their colour ranges (S x MAX_COLOURS) before the
+// atomic min/max fill them.
+__global__ __launch_bounds__(256) void kSceneRangesClear(uint2* __restrict__ manRange, uint2* __restrict__ colourRange,
+                                                         uint32_t S) {
+    const uint32_t n = S * (MAX_COLOURS + 1);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < S) manRange[i] = make_uint2(0xFFFFFFFFu, 0u);
+        else colourRange[i - S] = make_uint2(0xFFFFFFFFu, 0u);
+    }
 }
 
 // Ranges of keys over an index space: range[key] = (smallest index, one past the largest) holding
@@ -270,22 +276,19 @@ __device__ inline void runRanges(uint32_t i, KeyAt keyAt, uint2* __restrict__ ra
 }
 
 // First manifold and one past the last of every scene (manifolds come grouped by scene up to a
-// pair wave that straddles two scenes).
+// pair wave that straddles two scenes). Also clears manBodies of every manifold before the scenes
+// fill in their own: a scene's manifold range can hold a few of a neighbouring scene's manifolds,
+// which it recognises by bodies outside its own, whether their owner has written them yet or not.
 __global__ __launch_bounds__(256) void kSceneManRanges(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
-                                                       const uint32_t* __restrict__ colScene, uint2* __restrict__ range) {
+                                                       const uint32_t* __restrict__ colScene, uint2* __restrict__ range,
+                                                       uint2* __restrict__ manBodies) {
     const uint32_t M = ctr->manifolds;
     auto keyAt = [&](uint32_t m) { return m < M ? colScene[man[m].a] : NO_KEY; };
-    for (uint32_t base = blockIdx.x * blockDim.x; base < M; base += gridDim.x * blockDim.x)
-        runRanges(base + threadIdx.x, keyAt, range);
-}
-
-// manBodies of every manifold cleared before the scenes fill in their own: a scene's manifold
-// range can hold a few of a neighbouring scene's manifolds, which it recognises by bodies outside
-// its own, whether their owner has written them yet or not.
-__global__ __launch_bounds__(256) void kClearManBodies(const Counters* __restrict__ ctr, uint2* __restrict__ manBodies) {
-    const uint32_t M = ctr->manifolds;
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x)
-        manBodies[m] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < M; base += gridDim.x * blockDim.x) {
+        const uint32_t m = base + threadIdx.x;
+        if (m < M) manBodies[m] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+        runRanges(m, keyAt, range);
+    }
 }
 
 // Also writes manBodies (the manifold's bodies) and degree (contacts per dynamic body) for the
@@ -1040,11 +1043,6 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
 // barrier separates colours), hence the same bits, without device-wide flags or barriers. With a
 // residual target each scene stops at its own sweep, as a simulation of that scene alone would.
 
-__global__ __launch_bounds__(256) void kSceneRangeClear(uint2* __restrict__ range, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        range[i] = make_uint2(0xFFFFFFFFu, 0u);
-}
-
 // range[scene * MAX_COLOURS + colour] = (first slot, one past the last slot) of the scene's
 // constraints in that colour. The colour scatter keeps manifold order and manifolds come grouped
 // by scene, so a range holds few slots of other scenes (the solve skips them by body index).
@@ -1707,9 +1705,11 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
 // LDS, then the same stable colour scatter and constraint data (without predecessor slots).
 hipError_t launchColourPrepScene(hipStream_t st, DeviceWorld& w) {
     const uint32_t gm = blocksForManifolds(w);
-    hipLaunchKernelGGL(kSceneManClear, dim3(blocksFor(w.S)), dim3(256), 0, st, w.sceneManRange, w.S);
-    hipLaunchKernelGGL(kSceneManRanges, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colScene, w.sceneManRange);
-    hipLaunchKernelGGL(kClearManBodies, dim3(gm), dim3(256), 0, st, w.ctr, w.manBodies);
+    // the colour ranges are filled by the scene solve that always follows in the same phase
+    hipLaunchKernelGGL(kSceneRangesClear, dim3(blocksFor(w.S * (MAX_COLOURS + 1))), dim3(256), 0, st, w.sceneManRange,
+                       w.sceneRange, w.S);
+    hipLaunchKernelGGL(kSceneManRanges, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colScene, w.sceneManRange,
+                       w.manBodies);
     const SceneLaunch& sl = w.sceneLaunch;
     auto capped = [&](uint32_t layout) {   // at most sceneColourCap manifolds in LDS
         return (layout & 0xFFFF0000u) | std::min<uint32_t>(layout & 0xFFFFu, w.sceneColourCap);
@@ -1841,8 +1841,7 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
 hipError_t launchSceneSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pIt, int trackResidual,
                             float residualTarget) {
     vIt = std::min(vIt, MAX_SWEEPS);
-    const uint32_t ranges = w.S * MAX_COLOURS;
-    hipLaunchKernelGGL(kSceneRangeClear, dim3(blocksFor(ranges)), dim3(256), 0, st, w.sceneRange, ranges);
+    // sceneRange was emptied by the per-scene colouring of this phase
     hipLaunchKernelGGL(kSceneRanges, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.ctr, w.cs, w.bodyScene, w.sceneRange);
     const int track = (trackResidual || residualTarget > 0.0f) ? 1 : 0;
     const SceneLaunch& sl = w.sceneLaunch;
