@@ -226,6 +226,83 @@ def test_polygon_polygon_matches_box_box(oracle, scenes):
     assert mx["depth"][0] == pytest.approx(mb["depth"][0], abs=2e-6)
 
 
+def _convex_polygon(rng):
+    """3-8 vertices on an ellipse, counter-clockwise, no sliver edges."""
+    n = int(rng.integers(3, 9))
+    while True:
+        ang = np.sort(rng.uniform(0.0, 2 * np.pi, n))
+        gaps = np.diff(np.r_[ang, ang[0] + 2 * np.pi])
+        if gaps.min() > 0.35 and gaps.max() < np.pi - 0.2:
+            break
+    ax, ay = rng.uniform(0.6, 1.4, 2)
+    return np.stack([ax * np.cos(ang), ay * np.sin(ang)], 1).astype(np.float32)
+
+
+def _world_outlines(w):
+    """World vertices (counter-clockwise) of colliders 0 and 1, boxes from their half extents."""
+    v, _ = w.vertices()
+    start, count, typ = w.get("polyStart"), w.get("polyCount"), w.types()
+    wx, wy, c, s = w.get("wX"), w.get("wY"), w.get("wCos"), w.get("wSin")
+    hx, hy = w.get("p0"), w.get("p1")
+    out = []
+    for k in range(2):
+        if typ[k] == 2:
+            local = v[start[k]:start[k] + count[k]].astype(np.float64)
+        else:
+            local = np.array([[-hx[k], -hy[k]], [hx[k], -hy[k]], [hx[k], hy[k]], [-hx[k], hy[k]]], np.float64)
+        rot = np.array([[c[k], -s[k]], [s[k], c[k]]], np.float64)
+        out.append(local @ rot.T + [wx[k], wy[k]])
+    return out
+
+
+def _best_face(p, q):
+    """Separating-axis brute force: the face of p whose outward normal separates q the most."""
+    best, normal = -np.inf, None
+    for i in range(len(p)):
+        e = p[(i + 1) % len(p)] - p[i]
+        n = np.array([e[1], -e[0]]) / np.linalg.norm(e)
+        sep = float(np.min((q - p[i]) @ n))
+        if sep > best:
+            best, normal = sep, n
+    return best, normal
+
+
+def test_polygon_faces_match_separating_axis_brute_force(oracle, scenes):
+    """Box-box, box-polygon and polygon-polygon in general position (the two bodies' best face
+    separations at least 0.05 apart, so no tie-break decides): the manifold normal is the outward
+    normal of the face with the larger separation, oriented from A to B, and the depth is minus
+    that separation (BoxBox.cpp:13-228, BoxPolygon.cpp:96-172, PolygonPolygon.cpp:93-147)."""
+    rng = np.random.default_rng(11)
+    checked = 0
+    for _ in range(800):
+        kinds = rng.integers(1, 3, 2)
+        bodies = []
+        for k in range(2):
+            d = dict(px=float(k * rng.uniform(0.8, 1.8)), py=float(k * rng.uniform(-0.8, 0.8)),
+                     rot=float(rng.uniform(0.0, 6.28)), mass=1.0)
+            if kinds[k] == 2:
+                d.update(t=2, verts=_convex_polygon(rng))
+            else:
+                d.update(t=1, p0=float(rng.uniform(1.0, 2.5)), p1=float(rng.uniform(1.0, 2.5)))
+            bodies.append(d)
+        w = two_body(oracle, scenes, *bodies)
+        w.poses(True)
+        m = w.narrow([0], [1])
+        p0, p1 = _world_outlines(w)
+        s0, n0 = _best_face(p0, p1)
+        s1, n1 = _best_face(p1, p0)
+        if max(s0, s1) > -0.02 or abs(s0 - s1) < 0.05:
+            continue
+        assert len(m) == 1
+        normal = n0 if s0 > s1 else -n1          # from collider 0 to collider 1
+        if m["a"][0] == 1:
+            normal = -normal
+        assert np.allclose([m["nx"][0], m["ny"][0]], normal, atol=2e-5)
+        assert m["depth"][0] == pytest.approx(-max(s0, s1), abs=2e-5)
+        checked += 1
+    assert checked > 300
+
+
 def test_circle_box_and_polygon_regions(oracle, scenes):
     sq = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], np.float32)
     for shape in (dict(t=1, p0=2.0, p1=2.0), dict(t=2, verts=sq)):
