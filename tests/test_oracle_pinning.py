@@ -528,3 +528,60 @@ def test_ccd_circle_into_static_box(oracle, scenes):
     # core radius 0.1 reaches the wall face x = 4.9 at x = 4.8; travel 10 m per step
     assert t[1] == pytest.approx(0.48, abs=1e-5)
     assert w.get("comX")[1] == pytest.approx(4.8, abs=1e-4)
+
+
+def _distance_to_convex(p, x):
+    """Distance from point x to the convex outline p (counter-clockwise); 0 inside."""
+    inside = True
+    best = np.inf
+    for i in range(len(p)):
+        a, b = p[i], p[(i + 1) % len(p)]
+        e = b - a
+        if e[0] * (x[1] - a[1]) - e[1] * (x[0] - a[0]) < 0:
+            inside = False
+        t = np.clip((x - a) @ e / (e @ e), 0.0, 1.0)
+        best = min(best, float(np.linalg.norm(x - (a + t * e))))
+    return 0.0 if inside else best
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_ccd_circle_into_rotated_static_polygon_and_box(oracle, scenes, seed):
+    """A fast circle crossing a rotated static polygon or box within one step stops where its core
+    circle (half the radius, as in the circle-into-box case) first touches the outline; the time of
+    impact is found here by dense sampling and bisection of the exact point-to-outline distance
+    along the straight path (CCD/ContinuousCollisionDetector.cpp:129-220)."""
+    rng = np.random.default_rng(seed)
+    static = dict(px=5.0, py=0.0, mass=0.0, rot=float(rng.uniform(0.0, 6.28)))
+    if seed % 2:
+        static.update(t=1, p0=float(rng.uniform(0.6, 1.5)), p1=float(rng.uniform(2.0, 4.0)))
+    else:
+        static.update(t=2, verts=(_convex_polygon(rng) * 1.5).astype(np.float32))
+    vy = float(rng.uniform(-60.0, 60.0))
+    b = acc_bodies(scenes, [static, dict(t=0, px=0.0, py=float(rng.uniform(-0.5, 0.5)), vx=600.0, vy=vy,
+                                         mass=1.0, p0=0.2)])
+    w = scene_of(oracle, b, settings=dict(gy=0.0))
+    w.snapshot()
+    c0 = np.array([w.get("comX")[1], w.get("comY")[1]], np.float64)
+    w.integrate(DT)
+    w.wrap_cossin()
+    w.poses()
+    w.aabbs()
+    c1 = np.array([w.get("comX")[1], w.get("comY")[1]], np.float64)
+    outline = _world_outlines(w)[0]
+    core = 0.1
+    ts = np.linspace(0.0, 1.0, 4001)
+    dist = np.array([_distance_to_convex(outline, c0 + t * (c1 - c0)) for t in ts])
+    k = int(np.argmax(dist <= core))
+    assert dist[k] <= core and k > 0          # the path reaches the outline within the step
+    lo, hi = ts[k - 1], ts[k]
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if _distance_to_convex(outline, c0 + mid * (c1 - c0)) <= core:
+            hi = mid
+        else:
+            lo = mid
+    hit, t = w.ccd()
+    assert hit
+    assert t[1] == pytest.approx(hi, abs=2e-5)
+    assert w.get("comX")[1] == pytest.approx(c0[0] + hi * (c1[0] - c0[0]), abs=2e-4)
+    assert w.get("comY")[1] == pytest.approx(c0[1] + hi * (c1[1] - c0[1]), abs=2e-4)
