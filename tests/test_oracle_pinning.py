@@ -585,3 +585,55 @@ def test_ccd_circle_into_rotated_static_polygon_and_box(oracle, scenes, seed):
     assert t[1] == pytest.approx(hi, abs=2e-5)
     assert w.get("comX")[1] == pytest.approx(c0[0] + hi * (c1[0] - c0[0]), abs=2e-4)
     assert w.get("comY")[1] == pytest.approx(c0[1] + hi * (c1[1] - c0[1]), abs=2e-4)
+
+
+@pytest.mark.parametrize("mover", ["box", "polygon"])
+def test_ccd_moving_box_and_polygon_core(oracle, scenes, mover):
+    """A fast box or polygon sweeps its core circle (half its in-radius about the collider origin,
+    ContinuousCollisionDetector.cpp:353-372) along the origin's path; the in-radius is computed
+    here from the outline and the time of impact by bisection against the rotated static box."""
+    rng = np.random.default_rng(5)
+    static = dict(t=1, px=6.0, py=0.0, mass=0.0, rot=0.4, p0=1.0, p1=4.0)
+    if mover == "box":
+        body = dict(t=1, p0=0.6, p1=0.3)
+    else:
+        body = dict(t=2, verts=(_convex_polygon(rng) * 0.4).astype(np.float32))
+    body.update(px=0.0, py=0.2, rot=0.3, vx=600.0, vy=12.0, mass=1.0)
+    w = scene_of(oracle, acc_bodies(scenes, [static, body]), settings=dict(gy=0.0))
+    w.snapshot()
+    w.poses(True)
+    o0 = np.array([w.get("wX")[1], w.get("wY")[1]], np.float64)
+    com0 = np.array([w.get("comX")[1], w.get("comY")[1]], np.float64)
+    w.integrate(DT)
+    w.wrap_cossin()
+    w.poses()
+    w.aabbs()
+    o1 = np.array([w.get("wX")[1], w.get("wY")[1]], np.float64)
+    com1 = np.array([w.get("comX")[1], w.get("comY")[1]], np.float64)
+    # in-radius about the collider origin, from the local outline
+    if mover == "box":
+        inradius = min(w.get("p0")[1], w.get("p1")[1])
+    else:
+        v, _ = w.vertices()
+        s0, c = w.get("polyStart")[1], w.get("polyCount")[1]
+        loc = v[s0:s0 + c].astype(np.float64)
+        edges = [(loc[i], loc[(i + 1) % c] - loc[i]) for i in range(c)]
+        inradius = min(abs(e[0] * a[1] - e[1] * a[0]) / np.linalg.norm(e) for a, e in edges)
+    assert w.get("minExtent")[1] == pytest.approx(inradius, rel=1e-5)
+    core = 0.5 * inradius
+    outline = _world_outlines(w)[0]
+    ts = np.linspace(0.0, 1.0, 4001)
+    dist = np.array([_distance_to_convex(outline, o0 + t * (o1 - o0)) for t in ts])
+    k = int(np.argmax(dist <= core))
+    assert dist[k] <= core and k > 0
+    lo, hi = ts[k - 1], ts[k]
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if _distance_to_convex(outline, o0 + mid * (o1 - o0)) <= core:
+            hi = mid
+        else:
+            lo = mid
+    hit, t = w.ccd()
+    assert hit
+    assert t[1] == pytest.approx(hi, abs=2e-5)
+    assert w.get("comX")[1] == pytest.approx(com0[0] + hi * (com1[0] - com0[0]), abs=2e-4)
