@@ -85,6 +85,7 @@ typedef struct agx_manifold {
 #define AGX_OVERFLOW_HASH 8u       /* warm-start hash table full */
 #define AGX_OVERFLOW_STALL 16u     /* a dataflow kernel's wait hit its 2 s limit */
 #define AGX_OVERFLOW_ITEMS 32u     /* more constraints than the dataflow kernels hold */
+#define AGX_OVERFLOW_LDS 64u       /* a scene larger than its per-scene LDS region was not solved */
 
 /* Per-step counters of the last step. */
 typedef struct agx_stats {

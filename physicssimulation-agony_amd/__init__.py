@@ -11,8 +11,10 @@ runs whole physics steps (``Simulation::physicsStep``, Simulation.cpp:255-351), 
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -170,6 +172,25 @@ def lib() -> C.CDLL:
     return _LIB
 
 
+# Live worlds and networks, released newest first when the interpreter exits (before the HIP
+# runtime's own exit handlers), so no device object is freed from a finaliser after teardown began.
+_LIVE: "weakref.WeakValueDictionary[int, object]" = weakref.WeakValueDictionary()
+_NEXT_ID = [0]
+
+
+def _track(obj) -> None:
+    _NEXT_ID[0] += 1
+    _LIVE[_NEXT_ID[0]] = obj
+
+
+@atexit.register
+def _close_all() -> None:
+    for key in sorted(list(_LIVE.keys()), reverse=True):
+        obj = _LIVE.get(key)
+        if obj is not None:
+            obj.close()
+
+
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
@@ -271,6 +292,7 @@ class World:
         c = Counts()
         _check(lib().agx_world_counts(self._h, C.byref(c)), self._h)
         self.counts = c
+        _track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -432,6 +454,7 @@ class Mlp:
     def __init__(self, seed: int = 1, sample_capacity: int = 1 << 20):
         self._h = C.c_void_p()
         _check(lib().agx_mlp_create(seed, sample_capacity, C.byref(self._h)))
+        _track(self)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -109,9 +109,11 @@ constexpr uint32_t OVERFLOW_COLOURING = 4u;   // a constraint found no free firs
 constexpr uint32_t OVERFLOW_HASH = 8u;        // warm-start hash full (error)
 constexpr uint32_t OVERFLOW_STALL = 16u;      // a dataflow kernel hit its time limit (error)
 constexpr uint32_t OVERFLOW_ITEMS = 32u;      // more constraints than the dataflow kernels hold (error)
+constexpr uint32_t OVERFLOW_LDS = 64u;        // a scene larger than its per-scene LDS region was not solved (error)
 static_assert(OVERFLOW_PAIRS == AGX_OVERFLOW_PAIRS && OVERFLOW_STACK == AGX_OVERFLOW_STACK &&
                   OVERFLOW_COLOURING == AGX_OVERFLOW_COLOURING && OVERFLOW_HASH == AGX_OVERFLOW_HASH &&
-                  OVERFLOW_STALL == AGX_OVERFLOW_STALL && OVERFLOW_ITEMS == AGX_OVERFLOW_ITEMS,
+                  OVERFLOW_STALL == AGX_OVERFLOW_STALL && OVERFLOW_ITEMS == AGX_OVERFLOW_ITEMS &&
+                  OVERFLOW_LDS == AGX_OVERFLOW_LDS,
               "overflow bits match the public header");
 
 // Both children of an internal node in one 64-byte record, so a traversal step is one load.

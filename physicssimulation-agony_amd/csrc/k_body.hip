@@ -100,96 +100,98 @@ __global__ __launch_bounds__(256) void kPose(const float4* __restrict__ pos, con
 }
 
 // ---------------------------------------------------------------- CCD
-// CCD/ContinuousCollisionDetector.cpp:48-273 (casts of the core circle against inflated shapes).
+// The core circle (radius rho) of a fast dynamic collider sweeps the segment from its previous
+// to its current origin; it hits a static collider when it comes within rho of its outline, i.e.
+// when its centre enters the outline grown by rho (the Minkowski sum with a disc). The time of
+// impact is found in the static collider's frame by feature candidates: the boundary of a convex
+// outline grown by rho consists of each edge's line pushed out by rho (between the edge's end
+// normals) and of discs of radius rho around the vertices, so for a start outside the grown
+// outline the first entry is the earliest of those crossings. Times are fractions of the sweep
+// in [0, 1); NO_HIT marks none before the caller's limit. Follows the behaviour of
+// CCD/ContinuousCollisionDetector.cpp:48-273 (starts inside the grown outline never hit).
 constexpr float NO_HIT = 2.0f;
 
-__device__ inline float castCircle(float mx, float my, float dx, float dy, float dd, float radius, float best) {
-    const float c = mx * mx + my * my - radius * radius;
-    const float b = mx * dx + my * dy;
-    if (c <= 0.0f || b >= 0.0f) return NO_HIT;
-    if (c >= -2.0f * b * best) return NO_HIT;
-    const float disc = b * b - dd * c;
+// First time t in [0, limit) at which |o + t m| = rho for a point offset o from a disc centre moving
+// by m per sweep (mm = |m|^2); NO_HIT when it starts inside or on the disc, moves away, misses or
+// arrives at or after limit. The first root of mm t^2 + 2 k t + g is taken as g / (sqrt(k^2 - mm g) - k),
+// which has no cancellation for k < 0.
+__host__ __device__ inline float sweepDisc(float ox, float oy, float mx, float my, float mm, float rho, float limit) {
+    const float g = ox * ox + oy * oy - rho * rho;
+    const float k = ox * mx + oy * my;
+    if (!(g > 0.0f) || !(k < 0.0f)) return NO_HIT;
+    const float disc = k * k - mm * g;
     if (disc < 0.0f) return NO_HIT;
-    return c / (sqrtf(disc) - b);
+    const float t = g / (sqrtf(disc) - k);
+    return t < limit ? t : NO_HIT;
 }
 
-__device__ inline float castBox(float q0x, float q0y, float dlx, float dly, float dd, float hw, float hh,
-                                float radius, float best) {
-    const float sx = copysignf(1.0f, q0x), sy = copysignf(1.0f, q0y);
-    const float qx = q0x * sx, qy = q0y * sy;
-    const float vx = dlx * sx, vy = dly * sy;
-    const float w = hw + radius, h = hh + radius;
-    const float gapX = qx - w, gapY = qy - h;
-    if (gapX <= 0.0f && gapY <= 0.0f) return NO_HIT;
-    float tx = 0.0f;
-    if (gapX > 0.0f) {
-        if (vx >= 0.0f || gapX >= best * -vx) return NO_HIT;
-        tx = gapX / -vx;
-    }
-    float ty = 0.0f;
-    if (gapY > 0.0f) {
-        if (vy >= 0.0f || gapY >= best * -vy) return NO_HIT;
-        ty = gapY / -vy;
-    }
-    const float t = fmaxf(tx, ty);
-    const float px = qx + vx * t, py = qy + vy * t;
-    if (tx >= ty ? fabsf(py) > h : fabsf(px) > w) return NO_HIT;
-    if (fabsf(px) > hw && fabsf(py) > hh)
-        return castCircle(qx - copysignf(hw, px), qy - copysignf(hh, py), vx, vy, dd, radius, best);
-    return t;
-}
-
-__device__ inline float castPolygon(float q0x, float q0y, float dlx, float dly, float dd, const float2* __restrict__ v,
-                                    const float2* __restrict__ nrm, uint32_t count, float radius, float best) {
-    float tEnter = 0.0f, tExit = best;
-    uint32_t enterEdge = 0;
+// Convex outline given by corner(i) and outward unit edge normal faceNormal(i) of edge
+// corner(i) -> corner(i+1), in the collider's frame; q, m: the core's start and displacement in
+// that frame.
+template <class Corner, class FaceNormal>
+__host__ __device__ inline float sweepOutline(float qx, float qy, float mx, float my, float mm, float rho, float limit,
+                                     uint32_t count, Corner corner, FaceNormal faceNormal) {
+    float hit = NO_HIT;
+    bool beyondGrown = false;   // the start lies outside some edge's pushed-out line
+    bool outsideHull = false;   // the start lies outside some edge's own line
+    bool nearFeature = false;   // the start is within rho of an edge span or a vertex
     for (uint32_t i = 0; i < count; i++) {
-        const float2 n = nrm[i];
-        const float speed = n.x * dlx + n.y * dly;
-        const float gap = (n.x * (q0x - v[i].x) + n.y * (q0y - v[i].y)) - radius;
-        if (speed < 0.0f) {
-            if (gap > 0.0f) {
-                const float approach = -speed;
-                if (gap >= tExit * approach) return NO_HIT;
-                const float t = gap / approach;
-                if (t > tEnter) { tEnter = t; enterEdge = i; }
+        const float2 a = corner(i);
+        const float2 nrm = faceNormal(i);
+        const float ax = qx - a.x, ay = qy - a.y;
+        const float height = nrm.x * ax + nrm.y * ay;          // start's distance beyond edge i's line
+        const float rate = nrm.x * mx + nrm.y * my;            // its change over the whole sweep
+        if (height > rho) {
+            beyondGrown = true;
+            if (!(rate < 0.0f)) return NO_HIT;                 // never crosses this pushed-out line: separated
+            const float t = (height - rho) / -rate;
+            if (t < fminf(hit, limit)) {
+                const float2 b = corner(i + 1 == count ? 0 : i + 1);
+                const float ex = b.x - a.x, ey = b.y - a.y;
+                const float along = (ax + mx * t) * ex + (ay + my * t) * ey;
+                if (along >= 0.0f && along <= ex * ex + ey * ey) hit = t;
             }
-        } else if (gap > 0.0f) {
-            return NO_HIT;
-        } else if (speed > 0.0f) {
-            tExit = fminf(tExit, -gap / speed);
+        } else if (height > 0.0f) {
+            outsideHull = true;
+            const float2 b = corner(i + 1 == count ? 0 : i + 1);
+            const float ex = b.x - a.x, ey = b.y - a.y;
+            const float along = ax * ex + ay * ey;
+            if (along >= 0.0f && along <= ex * ex + ey * ey) nearFeature = true;
         }
+        if (ax * ax + ay * ay <= rho * rho) nearFeature = true;
+        hit = fminf(hit, sweepDisc(ax, ay, mx, my, mm, rho, fminf(hit, limit)));
     }
-    if (tEnter >= tExit) return NO_HIT;
-    uint32_t edge = enterEdge;
-    const bool startedInside = tEnter == 0.0f;
-    if (startedInside) {
-        float maxGap = -FLT_MAX;
-        for (uint32_t i = 0; i < count; i++) {
-            const float gap = (nrm[i].x * (q0x - v[i].x) + nrm[i].y * (q0y - v[i].y)) - radius;
-            if (gap > maxGap) { maxGap = gap; edge = i; }
-        }
-    }
-    const float px = q0x + dlx * tEnter, py = q0y + dly * tEnter;
-    const uint32_t next = edge + 1 == count ? 0 : edge + 1;
-    const float2 a = v[edge], b = v[next];
-    const float abx = b.x - a.x, aby = b.y - a.y;
-    const float u = (px - a.x) * abx + (py - a.y) * aby;
-    if (u >= 0.0f && u <= abx * abx + aby * aby) return startedInside ? NO_HIT : tEnter;
-    const float2 corner = u < 0.0f ? a : b;
-    return castCircle(q0x - corner.x, q0y - corner.y, dlx, dly, dd, radius, tExit);
+    const bool startsInside = !beyondGrown && (!outsideHull || nearFeature);
+    return startsInside ? NO_HIT : hit;
 }
 
-__device__ inline float castCore(uint4 sh, float2 dims, float4 ps, const float2* __restrict__ verts,
-                                 const float2* __restrict__ normals, float sx, float sy, float dx, float dy,
-                                 float dd, float core, float best) {
-    const float cs = ps.z, sn = ps.w;
-    const float rx = sx - ps.x, ry = sy - ps.y;
-    const float q0x = cs * rx + sn * ry, q0y = cs * ry - sn * rx;
-    const float dlx = cs * dx + sn * dy, dly = cs * dy - sn * dx;
-    if (sh.x == AGX_SHAPE_CIRCLE) return castCircle(q0x, q0y, dlx, dly, dd, dims.x + core, best);
-    if (sh.x == AGX_SHAPE_BOX) return castBox(q0x, q0y, dlx, dly, dd, dims.x, dims.y, core, best);
-    return castPolygon(q0x, q0y, dlx, dly, dd, verts + sh.y, normals + sh.y, sh.z, core, best);
+// Earliest time of impact (< limit) of the core with centre s, displacement m and radius rho
+// against static collider pose ps (x, y, cos, sin), or NO_HIT.
+__host__ __device__ inline float sweepCore(uint4 sh, float2 dims, float4 ps, const float2* __restrict__ verts,
+                                  const float2* __restrict__ normals, float sx, float sy, float mx, float my,
+                                  float mm, float rho, float limit) {
+    const float ox = sx - ps.x, oy = sy - ps.y;
+    if (sh.x == AGX_SHAPE_CIRCLE) return sweepDisc(ox, oy, mx, my, mm, dims.x + rho, limit);
+    // into the collider's frame: rotate by the inverse of its rotation
+    const float c = ps.z, sn = ps.w;
+    const float qx = c * ox + sn * oy, qy = c * oy - sn * ox;
+    const float lx = c * mx + sn * my, ly = c * my - sn * mx;
+    if (sh.x == AGX_SHAPE_BOX) {
+        const float hw = dims.x, hh = dims.y;
+        // corners counter-clockwise from (+hw, +hh); edge i's normal is +y, -x, -y, +x
+        return sweepOutline(qx, qy, lx, ly, mm, rho, limit, 4u,
+                            [=](uint32_t i) {
+                                return make_float2((i == 0 || i == 3) ? hw : -hw, i < 2 ? hh : -hh);
+                            },
+                            [](uint32_t i) {
+                                return make_float2(i == 1 ? -1.0f : (i == 3 ? 1.0f : 0.0f),
+                                                   i == 0 ? 1.0f : (i == 2 ? -1.0f : 0.0f));
+                            });
+    }
+    const float2* v = verts + sh.y;
+    const float2* nr = normals + sh.y;
+    return sweepOutline(qx, qy, lx, ly, mm, rho, limit, sh.z, [=](uint32_t i) { return v[i]; },
+                        [=](uint32_t i) { return nr[i]; });
 }
 
 // CCD/ContinuousCollisionDetector.cpp:307-440. Queries run in the previous step's Morton order
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(256) void kCcd(const uint32_t* __restrict__ order, 
             if (j < sf || j >= sl) continue;
             if (!(sMinX < nb.z && sMaxX > nb.x && sMinY < nb.w && sMaxY > nb.y)) continue;
             const uint32_t sc = st.vals[j];
-            best = fminf(best, castCore(shape[sc], dims[sc], pose[sc], verts, normals, s.x, s.y, dx, dy, dd, core, best));
+            best = fminf(best, sweepCore(shape[sc], dims[sc], pose[sc], verts, normals, s.x, s.y, dx, dy, dd, core, best));
             continue;
         }
         const uint2 rg = st.range[node];

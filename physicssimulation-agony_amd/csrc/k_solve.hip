@@ -1289,6 +1289,14 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
         }
         if (k0 >= S) break;
         const uint32_t s = sceneOrder[k0];
+        if (sceneBodyStart[s + 1] - sceneBodyStart[s] > cap) {   // larger than this LDS region: not solved
+            if (L.t == 0) {
+                sceneSweeps[s] = 0u;
+                atomicOr(&ctr->overflow, OVERFLOW_LDS);
+            }
+            L.sync();   // every wave has read taken[] before the next claim rewrites it
+            continue;
+        }
         const uint32_t done = solveScene(L, s, ctr, cs, vel, pos, sceneBodyStart[s], sceneBodyStart[s + 1], range, sceneRes, K,
                                          vIt, pIt, track, residualTarget, lds, reinterpret_cast<float2*>(lds + cap), inv,
                                          rng[L.g], part + L.g * L.waves);
@@ -1650,8 +1658,14 @@ __global__ __launch_bounds__(256) void kSceneIslandOffsets(const uint32_t* __res
 // hipErrorIllegalState while another process on the device holds its cooperative-launch resource;
 // the same grid is then launched as an ordinary kernel. Without other work on the device every block
 // is resident; if another process's kernels keep some blocks out, every wait ends at its 2 s limit
-// and flags OVERFLOW_STALL, so the step reports an error instead of hanging.
-static hipError_t launchResident(const void* kernel, uint32_t blocks, void** args, hipStream_t st) {
+// and flags OVERFLOW_STALL, so the step reports an error instead of hanging. Inside a stream capture
+// (step graphs) the ordinary launch is what gets recorded, so the grid is checked against the
+// occupancy here, before either launch: co-residency then rests on that bound alone.
+static hipError_t launchResident(const void* kernel, uint32_t blocks, int numCU, void** args, hipStream_t st) {
+    int perCu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, 256, 0) != hipSuccess) return hipErrorInvalidValue;
+    if (blocks == 0 || (uint64_t)blocks > (uint64_t)std::max(perCu, 0) * (uint64_t)std::max(numCU, 0))
+        return hipErrorInvalidConfiguration;
     hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(256), args, 0, st);
     if (e != hipErrorIllegalState) return e;
     (void)hipGetLastError();
@@ -1686,7 +1700,7 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
                        w.posOf, w.rank, w.incSorted);
     void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.colRec,
                      (void*)&w.colourId };
-    e = launchResident(reinterpret_cast<const void*>(kColourFlow), (uint32_t)w.colourBlocks, args, st);
+    e = launchResident(reinterpret_cast<const void*>(kColourFlow), (uint32_t)w.colourBlocks, w.numCU, args, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
     hipLaunchKernelGGL(kColourColumnScan, dim3(MAX_COLOURS), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
@@ -1749,10 +1763,16 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
     uint32_t* pDone = w.groupDone + w.solveGroups;
     void* args[] = { (void*)&w.ctr, (void*)&w.cs, (void*)&w.vel, (void*)&w.pos, (void*)&vDone, (void*)&pDone,
                      (void*)&w.blockRes, (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
-    hipError_t e = launchResident(reinterpret_cast<const void*>(kSolveFlow), (uint32_t)w.solveBlocks, args, st);
+    hipError_t e = launchResident(reinterpret_cast<const void*>(kSolveFlow), (uint32_t)w.solveBlocks, w.numCU, args, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kFlowTotals, dim3(1), dim3(1), 0, st, w.ctr, w.nDynBodies);
     return hipGetLastError();
+}
+
+// Body records each of `groups` equal LDS regions holds, rounded down to a multiple of four so that
+// every region starts 16-byte aligned.
+static uint32_t groupBodiesOf(size_t bytes, uint32_t groups) {
+    return (uint32_t)(bytes / groups / (sizeof(float4) + sizeof(float2))) & ~3u;
 }
 
 uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, uint32_t wholeCap, SceneLaunch& out,
@@ -1809,11 +1829,11 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         if (S <= WORK_SORT_MAX) {
             out.groups = SCENE_GROUPS;   // scenes larger than a group holds go to whole blocks
         } else {                         // size order: the scenes after the whole ones fit
-            const size_t next = std::max<size_t>(orderedBodies[out.whole], 1) * BODY_LDS;
-            while (out.groups < SCENE_GROUPS && 2 * out.groups * next <= bytes) out.groups *= 2;
+            const uint32_t next = std::max(orderedBodies[out.whole], 1u);
+            while (out.groups < SCENE_GROUPS && groupBodiesOf(bytes, 2 * out.groups) >= next) out.groups *= 2;
         }
     }
-    out.groupBodies = (uint32_t)(bytes / out.groups / BODY_LDS) & ~3u;   // regions 16-byte aligned
+    out.groupBodies = groupBodiesOf(bytes, out.groups);
     // colouring: the same whole scenes, then as many groups (at most 4) as leave room for three
     // manifolds per body of the largest remaining scene
     int colourPerCu = 0;
