@@ -205,6 +205,14 @@ int agx_world_write_body_f32(agx_world* w, uint32_t field, const float* in, uint
 int agx_world_read_collider_f32(agx_world* w, uint32_t field, float* out, uint32_t n);
 int agx_world_read_collider_u32(agx_world* w, uint32_t field, uint32_t* out, uint32_t n);
 int agx_world_read_pairs(agx_world* w, uint32_t* a, uint32_t* b, uint64_t cap, uint64_t* count);
+/* Sorted keys of the last broad-phase tree build (BvhTree::computeMortonCodes and
+ * sortByMortonCodes, Broad/BvhTree.cpp:252-354): which = 0 the dynamic colliders' tree, 1 the
+ * static one; n must be that tree's collider count. keys[i] = scene << morton_bits | Morton code of
+ * the AABB centre of collider colliders[i] (morton_bits / 2 bits per axis, quantised to the scene's
+ * bounds; the reference uses 16 bits per axis and one scene), ascending, equal keys in ascending
+ * collider order. Either array may be NULL. */
+int agx_world_read_broadphase_order(agx_world* w, uint32_t which, uint32_t* keys, uint32_t* colliders, uint32_t n,
+                                    uint32_t* morton_bits);
 /* which = 0: manifolds of the current step (pre-solve after NARROW, solved after SOLVE);
  * which = 1: the previous step's manifolds (the warm-start source). */
 int agx_world_read_manifolds(agx_world* w, uint32_t which, agx_manifold* out, uint64_t cap, uint64_t* count);

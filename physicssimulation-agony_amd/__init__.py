@@ -125,6 +125,7 @@ _SIGS = {
     "agx_world_read_collider_f32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
     "agx_world_read_collider_u32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
     "agx_world_read_pairs": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_read_broadphase_order": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_write_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64]),
     "agx_world_read_solve_order": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -367,6 +368,16 @@ class World:
         b = np.empty(max(n, 1), np.uint32)
         _check(lib().agx_world_read_pairs(self._h, _ptr(a), _ptr(b), n, C.byref(cnt)), self._h)
         return a[:n], b[:n]
+
+    def broadphase_order(self, which: int = 0):
+        """(keys, colliders, morton_bits) of the last build of the dynamic (0) or static (1) tree."""
+        c = self.counts
+        n = (c.colliders - c.static_colliders) if which == 0 else c.static_colliders
+        keys = np.empty(max(n, 1), np.uint32)
+        cols = np.empty(max(n, 1), np.uint32)
+        bits = C.c_uint32()
+        _check(lib().agx_world_read_broadphase_order(self._h, which, _ptr(keys), _ptr(cols), n, C.byref(bits)), self._h)
+        return keys[:n], cols[:n], bits.value
 
     def manifolds(self, which: int = 0) -> np.ndarray:
         cnt = C.c_uint64()

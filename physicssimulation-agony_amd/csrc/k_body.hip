@@ -103,12 +103,13 @@ __global__ __launch_bounds__(256) void kPose(const float4* __restrict__ pos, con
 // The core circle (radius rho) of a fast dynamic collider sweeps the segment from its previous
 // to its current origin; it hits a static collider when it comes within rho of its outline, i.e.
 // when its centre enters the outline grown by rho (the Minkowski sum with a disc). The time of
-// impact is found in the static collider's frame by feature candidates: the boundary of a convex
-// outline grown by rho consists of each edge's line pushed out by rho (between the edge's end
-// normals) and of discs of radius rho around the vertices, so for a start outside the grown
-// outline the first entry is the earliest of those crossings. Times are fractions of the sweep
-// in [0, 1); NO_HIT marks none before the caller's limit. Follows the behaviour of
-// CCD/ContinuousCollisionDetector.cpp:48-273 (starts inside the grown outline never hit).
+// impact is found in the static collider's frame. For a polygon, by feature candidates: the
+// boundary of a convex outline grown by rho consists of each edge's line pushed out by rho (between
+// the edge's end normals) and of discs of radius rho around the vertices, so for a start outside the
+// grown outline the first entry is the earliest of those crossings. A box is a rounded rectangle,
+// entered through its bounding rectangle. Times are fractions of the sweep in [0, 1); NO_HIT marks
+// none before the caller's limit. Follows the behaviour of CCD/ContinuousCollisionDetector.cpp:48-273
+// (starts inside the grown outline never hit; for boxes, starts inside its bounding rectangle).
 constexpr float NO_HIT = 2.0f;
 
 // First time t in [0, limit) at which |o + t m| = rho for a point offset o from a disc centre moving
@@ -165,29 +166,54 @@ __host__ __device__ inline float sweepOutline(float qx, float qy, float mx, floa
     return startsInside ? NO_HIT : hit;
 }
 
+// Box of half extents (hw, hh) grown by rho: a rounded rectangle inside the rectangle of half
+// extents (W, H) = (hw + rho, hh + rho). A start anywhere within that rectangle, the corner squares
+// outside the rounding included, counts as inside and never hits (CCD/ContinuousCollisionDetector.cpp:
+// 63-106 behaves so). Otherwise the start enters the rectangle when it has closed the last of its
+// per-axis excesses (the slab entry); an entry point beyond both the box's own extents lies in a corner
+// square, where only the disc of that corner can be met before the sweep leaves the square again.
+__host__ __device__ inline float sweepRoundedBox(float qx, float qy, float lx, float ly, float mm, float hw, float hh,
+                                                 float rho, float limit) {
+    const float W = hw + rho, H = hh + rho;
+    const float ax = fabsf(qx), ay = fabsf(qy);
+    const float excessX = ax - W, excessY = ay - H;       // start's distance beyond the rectangle's faces
+    if (!(excessX > 0.0f) && !(excessY > 0.0f)) return NO_HIT;
+    // the sweep mirrored into the start's quadrant, where the box lies towards -x and -y
+    const float sx = copysignf(1.0f, qx), sy = copysignf(1.0f, qy);
+    const float ux = lx * sx, uy = ly * sy;
+    const float closeX = -ux, closeY = -uy;               // speeds towards the box along each axis
+    float enterX = 0.0f, enterY = 0.0f;
+    if (excessX > 0.0f) {
+        if (!(closeX > 0.0f)) return NO_HIT;
+        enterX = excessX / closeX;
+        if (!(enterX < limit)) return NO_HIT;
+    }
+    if (excessY > 0.0f) {
+        if (!(closeY > 0.0f)) return NO_HIT;
+        enterY = excessY / closeY;
+        if (!(enterY < limit)) return NO_HIT;
+    }
+    const bool throughX = enterX >= enterY;               // the x faces are crossed last
+    const float t = throughX ? enterX : enterY;
+    const float ex = ax + ux * t, ey = ay + uy * t;       // entry point (mirrored)
+    if (throughX ? fabsf(ey) > H : fabsf(ex) > W) return NO_HIT;   // passes beside the rectangle
+    if (fabsf(ex) > hw && fabsf(ey) > hh)
+        return sweepDisc(ax - copysignf(hw, ex), ay - copysignf(hh, ey), ux, uy, mm, rho, limit);
+    return t;
+}
+
 // Earliest time of impact (< limit) of the core with centre s, displacement m and radius rho
-// against static collider pose ps (x, y, cos, sin), or NO_HIT.
+// against static collider pose ps (x, y, cos, sin), or NO_HIT. Worked in the collider's frame.
 __host__ __device__ inline float sweepCore(uint4 sh, float2 dims, float4 ps, const float2* __restrict__ verts,
                                   const float2* __restrict__ normals, float sx, float sy, float mx, float my,
                                   float mm, float rho, float limit) {
     const float ox = sx - ps.x, oy = sy - ps.y;
-    if (sh.x == AGX_SHAPE_CIRCLE) return sweepDisc(ox, oy, mx, my, mm, dims.x + rho, limit);
     // into the collider's frame: rotate by the inverse of its rotation
     const float c = ps.z, sn = ps.w;
     const float qx = c * ox + sn * oy, qy = c * oy - sn * ox;
     const float lx = c * mx + sn * my, ly = c * my - sn * mx;
-    if (sh.x == AGX_SHAPE_BOX) {
-        const float hw = dims.x, hh = dims.y;
-        // corners counter-clockwise from (+hw, +hh); edge i's normal is +y, -x, -y, +x
-        return sweepOutline(qx, qy, lx, ly, mm, rho, limit, 4u,
-                            [=](uint32_t i) {
-                                return make_float2((i == 0 || i == 3) ? hw : -hw, i < 2 ? hh : -hh);
-                            },
-                            [](uint32_t i) {
-                                return make_float2(i == 1 ? -1.0f : (i == 3 ? 1.0f : 0.0f),
-                                                   i == 0 ? 1.0f : (i == 2 ? -1.0f : 0.0f));
-                            });
-    }
+    if (sh.x == AGX_SHAPE_CIRCLE) return sweepDisc(qx, qy, lx, ly, mm, dims.x + rho, limit);
+    if (sh.x == AGX_SHAPE_BOX) return sweepRoundedBox(qx, qy, lx, ly, mm, dims.x, dims.y, rho, limit);
     const float2* v = verts + sh.y;
     const float2* nr = normals + sh.y;
     return sweepOutline(qx, qy, lx, ly, mm, rho, limit, sh.z, [=](uint32_t i) { return v[i]; },

@@ -1145,6 +1145,19 @@ int agx_world_read_pairs(agx_world* w, uint32_t* a, uint32_t* b, uint64_t cap, u
     return AGX_OK;
 }
 
+int agx_world_read_broadphase_order(agx_world* w, uint32_t which, uint32_t* keys, uint32_t* colliders, uint32_t n,
+                                    uint32_t* morton_bits) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (which > 1) return fail(w, AGX_ERR_INVALID_ARGUMENT, "which must be 0 (dynamic) or 1 (static)");
+    const agx::Tree& t = which ? w->d.stat : w->d.dyn;
+    if (n != t.n) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u entries", t.n);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    if (n && keys) HIP_CHECK(w, hipMemcpy(keys, t.keys, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (n && colliders) HIP_CHECK(w, hipMemcpy(colliders, t.vals, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (morton_bits) *morton_bits = w->d.mortonBits;
+    return AGX_OK;
+}
+
 int agx_world_read_manifolds(agx_world* w, uint32_t which, agx_manifold* out, uint64_t cap, uint64_t* count) {
     if (!w || !count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
     HIP_CHECK(w, hipStreamSynchronize(w->stream));

@@ -8,6 +8,8 @@ of the oracle stepped in the GPU's colour order; island partitions bit for bit.
 import numpy as np
 import pytest
 
+from parity_steps import assert_manifolds_equal, check_step
+
 pytestmark = pytest.mark.gpu
 DT = 1.0 / 60.0
 
@@ -101,81 +103,10 @@ def test_broadphase_edge_cases(agx):
 
 def test_step_phases_match_oracle(agx, oracle, scenes):
     scene = scenes.pile_small(seed=1)
-    w = make_world(agx, scene)
+    w = make_world(agx, scene, track_errors=1)
     w.step(DT, 90)
-    ow = oracle_from_gpu(oracle, scene, w, dict(v=8, p=3))
-    ow.snapshot()
-    # integrate + wrap + cos/sin
-    w.run_phase(agx.PHASE_INTEGRATE, DT)
-    ow.integrate(DT)
-    ow.wrap_cossin()
-    for g, o in (("com_x", "comX"), ("com_y", "comY"), ("rotation", "rot"), ("vel_x", "velX"), ("vel_y", "velY"),
-                 ("ang_vel", "angVel"), ("cos", "cos"), ("sin", "sin")):
-        assert np.array_equal(w.body(g), ow.get(o)), g
-    # poses + AABBs (+ the CCD sweep origin)
-    w.run_phase(agx.PHASE_POSE, DT)
-    ow.poses()
-    ow.aabbs()
-    for g, o in (("world_x", "wX"), ("world_y", "wY"), ("world_rot", "wRot"), ("cos", "wCos"), ("sin", "wSin"),
-                 ("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY"),
-                 ("prev_x", "prevWX"), ("prev_y", "prevWY")):
-        gv, ov = w.collider(g), ow.get(o)
-        ns = ow.counts()["static_colliders"]
-        assert np.array_equal(gv[ns:], ov[ns:]), g
-    w.run_phase(agx.PHASE_CCD, DT)
-    hit, _ = ow.ccd()
-    assert w.collider("world_x").tobytes() == ow.get("wX").tobytes()
-    # broad phase: identical pair set
-    w.run_phase(agx.PHASE_BROAD, DT)
-    ga, gb = w.pairs()
-    oa, ob = ow.broad_brute()
-    is_dyn = np.zeros(ow.counts()["colliders"], bool)
-    is_dyn[ow.counts()["static_colliders"]:] = True
-    gset = pair_set(ga, gb, is_dyn)
-    assert len(gset) == len(ga)
-    assert gset == pair_set(oa, ob, is_dyn)
-    assert len(gset) > 1000
-    # narrow phase + warm start on the GPU's pairs
-    w.run_phase(agx.PHASE_NARROW, DT)
-    mg = w.manifolds(0)
-    mo = ow.warm_start(ow.narrow(ga, gb))
-    assert len(mg) == len(mo) > 1000
-    mg_s, mo_s = mg[sort_key(mg)], mo[sort_key(mo)]
-    for f in mg.dtype.names:
-        if f in ("p1x", "p1y", "id1", "jn1", "jt1"):
-            two = mg_s["count"] == 2
-            assert np.array_equal(mg_s[f][two], mo_s[f][two]), f
-        else:
-            assert np.array_equal(mg_s[f], mo_s[f]), f
-    assert np.count_nonzero(mg["jn0"]) > len(mg) // 2, "warm start found few previous impulses"
-    # solve in the GPU's colour order, from the same pre-solve state
-    w.run_phase(agx.PHASE_SOLVE, DT)
-    order, starts = w.solve_order()
-    assert len(order) == len(mg) and np.array_equal(np.sort(order), np.arange(len(mg)))
-    body_of = w.collider("body")
-    inv_mass = w.body("inv_mass")
-    for k in range(len(starts) - 1):
-        ms = mg[order[starts[k]:starts[k + 1]]]
-        bodies = np.concatenate([body_of[ms["a"]], body_of[ms["b"]]])
-        bodies = bodies[inv_mass[bodies] != 0]
-        assert len(np.unique(bodies)) == len(bodies), f"colour {k} shares a dynamic body"
-    solved_o = ow.solve(mg, order=order, v_it=8, p_it=3)
-    solved_g = w.manifolds(0)
-    for f in ("jn0", "jt0", "jn1", "jt1"):
-        two = solved_g["count"] == 2 if f in ("jn1", "jt1") else np.ones(len(solved_g), bool)
-        np.testing.assert_allclose(solved_g[f][two], solved_o[f][two], rtol=1e-4, atol=1e-4, err_msg=f)
-    for g, o in (("vel_x", "velX"), ("vel_y", "velY"), ("ang_vel", "angVel"), ("com_x", "comX"), ("com_y", "comY")):
-        np.testing.assert_allclose(w.body(g), ow.get(o), rtol=1e-4, atol=1e-4, err_msg=g)
-    # islands
-    w.run_phase(agx.PHASE_ISLANDS, DT)
-    ids_g, k_g = w.islands()
-    ids_o, k_o = ow.islands(solved_g)
-    assert k_g == k_o
-    canon = lambda lab: {frozenset(np.flatnonzero(lab == v).tolist()) for v in np.unique(lab)}
-    assert canon(ids_g) == canon(ids_o)
-    w.run_phase(agx.PHASE_FINISH, DT)
-    st = w.stats()
-    assert st.overflow == 0 and st.manifolds == len(mg)
+    out = check_step(agx, oracle, scene, w, DT, 8, 3)
+    assert out["pairs"] > 1000 and out["two_point"] > 100
 
 
 def test_long_run_matches_oracle_statistics(agx, oracle, scenes):
@@ -287,10 +218,11 @@ def test_batched_scenes_match_single_scenes(agx, oracle, scenes):
         mo = ow.narrow(ga[sel] - c_off, gb[sel] - c_off)
         ms["a"] -= c_off
         ms["b"] -= c_off
-        assert len(ms) == len(mo)
         ms, mo = ms[sort_key(ms)], mo[sort_key(mo)]
-        for f in ("a", "b", "count", "nx", "ny", "depth", "p0x", "p0y", "id0"):
-            assert np.array_equal(ms[f], mo[f]), (s, f)
+        assert len(ms) == len(mo)
+        for f in ("jn0", "jt0", "jn1", "jt1"):   # warm-start impulses are checked elsewhere
+            ms[f] = mo[f]
+        assert_manifolds_equal(ms, mo, f"scene {s}")
         c_off += nc
     for p in range(agx.PHASE_SOLVE, agx.PHASE_FINISH + 1):
         w.run_phase(p, DT)
@@ -326,6 +258,7 @@ def test_batch_scenes_match_oracle_scene_by_scene(agx, oracle, scenes):
     pos_of[order] = np.arange(len(order))
     renames = {"com_x": "comX", "com_y": "comY", "rotation": "rot", "vel_x": "velX", "vel_y": "velY",
                "ang_vel": "angVel", "cos": "cos", "sin": "sin"}
+    two_point = 0
     col_names = {"world_x": "wX", "world_y": "wY", "cos": "wCos", "sin": "wSin", "world_rot": "wRot",
                  "aabb_min_x": "minX", "aabb_min_y": "minY", "aabb_max_x": "maxX", "aabb_max_y": "maxY"}
     for s, sc in enumerate(parts):
@@ -361,17 +294,18 @@ def test_batch_scenes_match_oracle_scene_by_scene(agx, oracle, scenes):
         ms["b"] -= c_off
         mo = ow.warm_start(ow.narrow(pa, pb))
         assert len(ms) == len(mo) > 0, s
-        ks, ko = ms[sort_key(ms)], mo[sort_key(mo)]
-        for f in ("a", "b", "count", "nx", "ny", "depth", "p0x", "p0y", "id0", "jn0", "jt0"):
-            assert np.array_equal(ks[f], ko[f]), (s, f)
+        two_point += assert_manifolds_equal(ms, mo, f"scene {s}")
         # the GPU's solve order restricted to this scene
         local_order = np.argsort(pos_of[msel], kind="stable").astype(np.uint32)
         so = ow.solve(ms, order=local_order, v_it=8, p_it=3)
         sg = solved[msel]
-        for f in ("jn0", "jt0"):
-            np.testing.assert_allclose(sg[f], so[f], rtol=1e-4, atol=1e-4, err_msg=f"{s} {f}")
+        two = sg["count"] == 2
+        for f in ("jn0", "jt0", "jn1", "jt1"):
+            sel = two if f in ("jn1", "jt1") else np.ones(len(sg), bool)
+            np.testing.assert_allclose(sg[f][sel], so[f][sel], rtol=1e-4, atol=1e-4, err_msg=f"{s} {f}")
         for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y"):
             np.testing.assert_allclose(b2[f][bs], ow.get(renames[f]), rtol=1e-4, atol=1e-4, err_msg=f"{s} {f}")
+    assert two_point > 50   # box stacks: both points of box-box manifolds checked
     for p in range(agx.PHASE_ISLANDS, agx.PHASE_FINISH + 1):
         w.run_phase(p, DT)
     assert w.stats().overflow == 0

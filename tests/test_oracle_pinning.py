@@ -637,3 +637,156 @@ def test_ccd_moving_box_and_polygon_core(oracle, scenes, mover):
     assert hit
     assert t[1] == pytest.approx(hi, abs=2e-5)
     assert w.get("comX")[1] == pytest.approx(com0[0] + hi * (com1[0] - com0[0]), abs=2e-4)
+
+
+# ---------------------------------------------------------------- collider AABBs
+def test_aabbs_match_rotated_outline_extremes(oracle, scenes):
+    """Box, polygon and circle AABBs (ColliderAABBBuilder.cpp:8-136) equal the extremes of the
+    collider's outline rotated with the collider's own cos/sin, computed here corner by corner in
+    float64; at angle 0 a box's AABB is exactly its centre +- (half width, half height)."""
+    rng = np.random.default_rng(21)
+    items = []
+    for k, ang in enumerate([0.0, np.pi / 2, np.pi / 4, np.pi / 6, 2.0, 3.5, 5.0, 6.0]):
+        items.append(dict(t=1, px=3.0 * k, py=1.0, rot=float(ang), mass=1.0, p0=3.0, p1=1.0))
+    for k in range(12):
+        items.append(dict(t=2, px=3.0 * k, py=8.0, rot=float(rng.uniform(0, 6.28)), mass=1.0,
+                          verts=scenes.polygon_outline(rng, int(rng.integers(3, 9)), float(rng.uniform(0.3, 1.2)))
+                          + rng.uniform(-0.2, 0.2, 2).astype(np.float32)))
+    items.append(dict(t=0, px=-4.0, py=-4.0, mass=1.0, p0=0.7))
+    w = scene_of(oracle, acc_bodies(scenes, items))
+    w.poses(True)
+    w.aabbs(True)
+    box = [w.get(k) for k in ("minX", "minY", "maxX", "maxY")]
+    wx, wy, c, s = w.get("wX"), w.get("wY"), w.get("wCos"), w.get("wSin")
+    v, _ = w.vertices()
+    st, cnt, typ, p0, p1 = w.get("polyStart"), w.get("polyCount"), w.types(), w.get("p0"), w.get("p1")
+    for k in range(len(items)):
+        if typ[k] == 0:
+            local = p0[k] * np.array([[1.0, 0.0], [-1.0, 0.0], [0.0, 1.0], [0.0, -1.0]])
+            rot = np.eye(2)
+        else:
+            if typ[k] == 1:
+                local = np.array([[p0[k], p1[k]], [-p0[k], p1[k]], [-p0[k], -p1[k]], [p0[k], -p1[k]]], np.float64)
+            else:
+                local = v[st[k]:st[k] + cnt[k]].astype(np.float64)
+            rot = np.array([[c[k], -s[k]], [s[k], c[k]]], np.float64)
+        world = local @ rot.T + [float(wx[k]), float(wy[k])]
+        expect = [world[:, 0].min(), world[:, 1].min(), world[:, 0].max(), world[:, 1].max()]
+        got = [float(b[k]) for b in box]
+        assert np.allclose(got, expect, atol=3e-6), (k, got, expect)
+    # the unrotated box: exact, and the half width lies along x
+    assert [float(b[0]) for b in box] == [-1.5, 0.5, 1.5, 1.5]
+    # a quarter turn swaps the box's extents (fast cos of pi/2 is within 2e-3 of 0)
+    assert float(box[2][1] - box[0][1]) == pytest.approx(1.0, abs=5e-3)
+    assert float(box[3][1] - box[1][1]) == pytest.approx(3.0, abs=5e-3)
+
+
+# ---------------------------------------------------------------- friction law
+FLOOR = dict(t=1, px=0.0, py=0.0, mass=0.0, p0=80.0, p1=1.0, material=1)
+
+
+def _slide(oracle, scenes, v0, gravity, steps, mats):
+    """A 1 x 1 box resting on a wide static floor (top at y = 0.5), started at velocity v0 along x."""
+    box = dict(t=1, px=0.0, py=1.0, vx=float(v0), mass=1.0, p0=1.0, p1=1.0, material=2)
+    w = scene_of(oracle, acc_bodies(scenes, [FLOOR, box]), materials=mats,
+                 settings=dict(gx=float(gravity[0]), gy=float(gravity[1]), v=8, p=3))
+    vx, jt = [], []
+    for _ in range(steps):
+        w.step(DT)
+        vx.append(float(w.get("velX")[1]))
+        m = w.manifolds()
+        jt.append(float(np.sum(m["jt0"] + np.where(m["count"] == 2, m["jt1"], 0.0))) if len(m) else 0.0)
+    return np.array(vx), np.array(jt), w
+
+
+# static and dynamic coefficients of the box and of the floor; combined by the geometric mean
+# (BodyCollisionSolver.cpp:304-305): static sqrt(0.81 * 0.25) = 0.45, dynamic sqrt(0.49 * 0.01) = 0.07
+FRICTION_MATS = ((0.0, 0.25, 0.01), (0.0, 0.81, 0.49))
+MU_S, MU_D = 0.45, 0.07
+
+
+@pytest.mark.parametrize("direction", [1.0, -1.0])
+def test_sliding_box_decelerates_at_dynamic_friction(oracle, scenes, direction):
+    """A box sliding on a level floor loses mu_d * g * dt of speed per step (Coulomb friction at the
+    dynamic coefficient; a swapped static/dynamic pair or another combination rule fails), and its
+    friction accumulator carries the sign of the slip along x: the tangent is (-n.y, n.x) and the
+    impulse +jt t acts on B (BodyCollisionSolver.cpp:311, :469, :561-582), so jt > 0 when the box
+    slides towards +x whichever of floor and box is A."""
+    vx, jt, _ = _slide(oracle, scenes, 3.0 * direction, (0.0, -9.81), 60, FRICTION_MATS)
+    t = np.arange(1, 61) * DT
+    expect = direction * (3.0 - MU_D * 9.81 * t)
+    assert np.allclose(vx[5:], expect[5:], atol=2e-3)
+    assert np.all(np.sign(jt[5:]) == direction)
+
+
+# combined static sqrt(0.9 * 0.9) = 0.9, dynamic sqrt(0.49 * 0.01) = 0.07
+INCLINE_MATS = ((0.0, 0.9, 0.01), (0.0, 0.9, 0.49))
+
+
+def test_box_on_incline_sticks_below_static_friction(oracle, scenes):
+    """Gravity tilted by atan(0.3) along the floor. The static test is made per contact point
+    against that point's whole slip (BodyCollisionSolver.cpp:564-580, :733-752), so a box on two
+    points sticks while one point can hold the slope (0.3 < 0.9 / 2): a box at rest stays at rest;
+    a box already sliding downhill speeds up at g (sin a - mu_d cos a), the dynamic coefficient
+    (0.07) being below the slope. With the two coefficients swapped the sliding box would stop."""
+    a = np.arctan(0.3)
+    g = 9.81 * np.array([np.sin(a), -np.cos(a)])
+    vx, _, w = _slide(oracle, scenes, 0.0, g, 90, INCLINE_MATS)
+    assert np.max(np.abs(vx[10:])) < 2e-3
+    vx, _, _ = _slide(oracle, scenes, 2.0, g, 60, INCLINE_MATS)
+    t = np.arange(1, 61) * DT
+    acc = 9.81 * (np.sin(a) - MU_D * np.cos(a))
+    assert np.allclose(vx[5:], 2.0 + acc * t[5:], atol=5e-3)
+
+
+# ---------------------------------------------------------------- CCD of spinning bodies
+def test_ccd_spinning_body_casts_the_straight_origin_path(oracle, scenes):
+    """A fast polygon that turns by 0.75 rad in the step. Its collider origin is the centroid, which
+    is the body's centre of mass (ObjectManager.cpp:312-349: a single convex piece is recentred on
+    its centroid), so the core circle (half the in-radius about the origin) is cast along the
+    straight segment between the origin's previous and current positions whatever the spin, and a
+    hit moves the body back by translation only, keeping the new rotation
+    (ContinuousCollisionDetector.cpp:353-372, :513-529). Time of impact here: dense sampling plus
+    bisection of the exact distance to the rotated static box along that segment."""
+    rng = np.random.default_rng(9)
+    verts = (scenes.polygon_outline(rng, 6, 0.5) + np.array([0.15, -0.1], np.float32)).astype(np.float32)
+    static = dict(t=1, px=7.0, py=0.3, mass=0.0, rot=0.25, p0=1.0, p1=5.0)
+    body = dict(t=2, px=0.0, py=0.0, rot=0.4, vx=640.0, vy=15.0, av=45.0, mass=1.0, verts=verts)
+    w = scene_of(oracle, acc_bodies(scenes, [static, body]), settings=dict(gy=0.0))
+    assert abs(float(w.get("comOffX")[1])) < 1e-6 and abs(float(w.get("comOffY")[1])) < 1e-6
+    w.snapshot()
+    w.poses(True)
+    o0 = np.array([w.get("wX")[1], w.get("wY")[1]], np.float64)
+    com0 = np.array([w.get("comX")[1], w.get("comY")[1]], np.float64)
+    rot0 = float(w.get("rot")[1])
+    w.integrate(DT)
+    w.wrap_cossin()
+    w.poses()
+    w.aabbs()
+    o1 = np.array([w.get("wX")[1], w.get("wY")[1]], np.float64)
+    com1 = np.array([w.get("comX")[1], w.get("comY")[1]], np.float64)
+    rot1 = float(w.get("rot")[1])
+    assert abs(rot1 - rot0) > 0.5           # the body turns by 0.75 rad in the step
+    v, _ = w.vertices()
+    s0, c = w.get("polyStart")[1], w.get("polyCount")[1]
+    loc = v[s0:s0 + c].astype(np.float64)
+    inradius = min(abs((loc[(i + 1) % c] - loc[i])[0] * loc[i][1] - (loc[(i + 1) % c] - loc[i])[1] * loc[i][0])
+                   / np.linalg.norm(loc[(i + 1) % c] - loc[i]) for i in range(c))
+    assert w.get("minExtent")[1] == pytest.approx(inradius, rel=1e-5)
+    core = 0.5 * inradius
+    outline = _world_outlines(w)[0]
+    path = lambda t: o0 + t * (o1 - o0)
+    ts = np.linspace(0.0, 1.0, 8001)
+    dist = np.array([_distance_to_convex(outline, path(t)) for t in ts])
+    k = int(np.argmax(dist <= core))
+    assert dist[k] <= core and k > 0
+    lo, hi = ts[k - 1], ts[k]
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (lo, mid) if _distance_to_convex(outline, path(mid)) <= core else (mid, hi)
+    hit, t = w.ccd()
+    assert hit
+    assert t[1] == pytest.approx(hi, abs=2e-5)
+    assert w.get("comX")[1] == pytest.approx(com0[0] + hi * (com1[0] - com0[0]), abs=2e-4)
+    assert w.get("comY")[1] == pytest.approx(com0[1] + hi * (com1[1] - com0[1]), abs=2e-4)
+    assert float(w.get("rot")[1]) == rot1   # rotation is kept
