@@ -1472,14 +1472,23 @@ __global__ __launch_bounds__(256) void kSolveReduce(Counters* __restrict__ ctr, 
 // number the roots in ascending order (IslandManager.cpp:36-80 numbers DSU roots in index order;
 // the partition is the same).
 //
-// Parents are followed with plain loads. Links are never removed and only point downwards, so a
-// stale read can miss a recent link but never invents one: a walk always ends at a real ancestor,
-// and a root it takes for current is checked by the compare-and-swap that hooks it, which fails
-// and returns the newer parent when that root was already hooked.
-__device__ inline uint32_t ufFind(const uint32_t* parent, uint32_t x) {
+// Parents are followed with plain loads. Every link points from a body to a smaller body of the
+// same set, so a stale read can miss a recent link but never invents one: a walk always ends at a
+// real ancestor, and a root it takes for current is checked by the compare-and-swap that hooks it,
+// which fails and returns the newer parent when that root was already hooked.
+//
+// Path halving: a walk re-points each body it passes at its grandparent with a plain store. The
+// grandparent is an ancestor, so the link still points downwards into the same set, and only
+// bodies that already have a parent are re-pointed (roots change only by the compare-and-swap).
+// Racing stores to one body each leave some ancestor there, so walks stay short on the long
+// parent chains of large islands.
+__device__ inline uint32_t ufFind(uint32_t* parent, uint32_t x) {
     uint32_t p = parent[x];
     while (p != x) {
-        x = p;
+        const uint32_t g = parent[p];
+        if (g == p) return p;
+        parent[x] = g;
+        x = g;
         p = parent[x];
     }
     return x;
