@@ -264,6 +264,22 @@ int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t
                          const uint32_t* scene, uint32_t n, uint32_t* pairs_a, uint32_t* pairs_b,
                          uint64_t cap, uint64_t* count);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc): agx_broadphase_pairs over DEVICE buffers (aabb 16-byte
+ * aligned; scene may be NULL for one scene; num_scenes must exceed every scene index) writing the
+ * same pairs, in the same order, to the device arrays pairs_a/pairs_b; *count is a host pointer.
+ * Every intermediate array lives in one static device array of the library (about 36 MB per
+ * device, held until the process ends); the call runs on the null stream, synchronizes, and
+ * allocates nothing. Limits: n <= AGX_NOALLOC_MAX_COLLIDERS, num_scenes <= AGX_NOALLOC_MAX_SCENES,
+ * at most AGX_NOALLOC_MAX_PAIRS pairs (AGX_ERR_CAPACITY beyond). Calls from one process run one
+ * at a time (a mutex); each call starts from cleared counters whatever the previous call left. */
+#define AGX_NOALLOC_MAX_COLLIDERS 65536u
+#define AGX_NOALLOC_MAX_SCENES 1024u
+#define AGX_NOALLOC_MAX_PAIRS 1048576u
+int agx_broadphase_pairs_noalloc(const float* aabb, const uint32_t* owner, const uint8_t* is_dynamic,
+                                 const uint32_t* scene, uint32_t n, uint32_t num_scenes, uint32_t* pairs_a,
+                                 uint32_t* pairs_b, uint64_t cap, uint64_t* count);
+
 /* ---- learned warm start. No counterpart in the reference, which warm-starts every contact point
  * from the cached accumulators of the previous step (NarrowPhaseCollisionDetector.cpp:753-765,
  * BodyCollisionSolver.cpp:416-490). A network 24 -> 128 -> 128 -> 2 (ReLU, bf16 weights and

@@ -133,6 +133,8 @@ _SIGS = {
     "agx_world_read_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "agx_world_read_scene_sweeps": (C.c_int, [_P, _P, C.c_uint32]),
     "agx_broadphase_pairs": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_broadphase_pairs_noalloc": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint64,
+                                               C.POINTER(C.c_uint64)]),
     "agx_world_set_profiling": (C.c_int, [_P, C.c_uint32]),
     "agx_world_read_profile": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_profile_segment_name": (C.c_char_p, [C.c_uint32]),

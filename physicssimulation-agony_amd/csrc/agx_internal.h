@@ -154,6 +154,24 @@ struct Tree {
     uint2* chunkRange = nullptr;
     uint32_t chunks = 0;
     float4* sceneBounds = nullptr;   // S: (minX, minY, maxX, maxY)
+    bool boundsReady = false;        // sceneBounds already computed (no chunked bounds pass)
+};
+
+// Device-side set-up of a broad phase over device inputs (one block of BROAD_PREP_THREADS):
+// dynamic and static collider lists in ascending index order, each tree's per-scene sorted ranges
+// (S + 1 entries) and per-scene bounds (order-preserving int encoding, as the tree build keeps
+// them); header[0..2] = dynamic count, static count, 1 when a scene index was >= S. sceneZero,
+// when not null, receives n zeros (the scene of every collider when the caller gives none).
+constexpr uint32_t BROAD_PREP_THREADS = 1024;
+struct BroadPrep {
+    uint32_t* dynList;
+    uint32_t* statList;
+    uint32_t* dynStart;
+    uint32_t* statStart;
+    int4* dynBounds;
+    int4* statBounds;
+    uint32_t* header;
+    uint32_t* sceneZero;
 };
 
 // Constraint planes, one float4 per constraint per plane, in solve order (colour-major).
@@ -417,6 +435,10 @@ hipError_t launchCcd(hipStream_t st, DeviceWorld& w, float fastFactor, float cor
 
 hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t);
 hipError_t launchPairs(hipStream_t st, DeviceWorld& w);
+// S <= BROAD_PREP_THREADS scenes; 40 S bytes of LDS.
+hipError_t launchBroadPrep(hipStream_t st, const float4* aabb, const uint8_t* isDynamic, const uint32_t* scene,
+                           uint32_t n, uint32_t S, const BroadPrep& out);
+hipError_t launchSplitPairs(hipStream_t st, const uint2* pairs, uint32_t n, uint32_t* a, uint32_t* b);
 
 hipError_t launchNarrow(hipStream_t st, DeviceWorld& w);
 hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w);

@@ -438,13 +438,130 @@ __global__ __launch_bounds__(256) void kGatherPairs(const uint32_t* __restrict__
 
 inline uint32_t blocksFor(uint32_t n, uint32_t per) { return (n + per - 1) / per; }
 
+// Exclusive scan of one value per thread over a block of BROAD_PREP_THREADS; *total gets the sum.
+__device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t* waveSums, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) waveSums[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < BROAD_PREP_THREADS / 64; k++) {
+            const uint32_t s = waveSums[k];
+            waveSums[k] = run;
+            run += s;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    const uint32_t r = waveSums[wave] + inc - v;
+    __syncthreads();
+    return r;
+}
+
+// One block: the dynamic and static collider lists (ascending index), each tree's per-scene
+// sorted ranges (from per-scene counts) and per-scene bounds (BvhTree.cpp:148-188), from device
+// inputs. Bounds are exact min/max, so they equal those of the chunked bounds pass. Dynamic LDS:
+// 8 S ints of bounds and 2 S counts.
+__global__ __launch_bounds__(BROAD_PREP_THREADS) void kBroadPrep(const float4* __restrict__ aabb,
+                                                                 const uint8_t* __restrict__ isDyn,
+                                                                 const uint32_t* __restrict__ scene, uint32_t n,
+                                                                 uint32_t S, BroadPrep o) {
+    extern __shared__ int prepLds[];
+    int* bnd = prepLds;                                   // [tree][scene][4]: tree 0 dynamic, 1 static
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(prepLds + 8 * S);   // [tree][scene]
+    __shared__ uint32_t waveSums[BROAD_PREP_THREADS / 64];
+    __shared__ uint32_t nDynTotal, sumA, sumB, bad;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = t; k < 2 * S; k += BROAD_PREP_THREADS) {
+        int* b = bnd + 4 * k;
+        b[0] = b[1] = floatOrder(FLT_MAX);
+        b[2] = b[3] = floatOrder(-FLT_MAX);
+        cnt[k] = 0;
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    const uint32_t per = (n + BROAD_PREP_THREADS - 1) / BROAD_PREP_THREADS;
+    const uint32_t lo = min(n, t * per), hi = min(n, lo + per);
+    uint32_t dynHere = 0;
+    for (uint32_t i = lo; i < hi; i++) {
+        const uint32_t d = isDyn[i] ? 1u : 0u;
+        dynHere += d;
+        const uint32_t s = scene ? scene[i] : 0u;
+        if (o.sceneZero) o.sceneZero[i] = 0u;
+        if (s >= S) { atomicOr(&bad, 1u); continue; }
+        const uint32_t k = (d ? 0u : S) + s;
+        const float4 a = aabb[i];
+        atomicMin(&bnd[4 * k + 0], floatOrder(a.x));
+        atomicMin(&bnd[4 * k + 1], floatOrder(a.y));
+        atomicMax(&bnd[4 * k + 2], floatOrder(a.z));
+        atomicMax(&bnd[4 * k + 3], floatOrder(a.w));
+        atomicAdd(&cnt[k], 1u);
+    }
+    // stable partition: this thread's colliders go after those of lower threads
+    uint32_t dOff = blockExclusiveScan(dynHere, waveSums, &nDynTotal);
+    uint32_t sOff = lo - dOff;
+    for (uint32_t i = lo; i < hi; i++) {
+        if (isDyn[i]) o.dynList[dOff++] = i;
+        else o.statList[sOff++] = i;
+    }
+    // per-scene sorted ranges: exclusive scans of the counts, S <= BROAD_PREP_THREADS
+    const uint32_t cd = t < S ? cnt[t] : 0u, cs = t < S ? cnt[S + t] : 0u;
+    const uint32_t startD = blockExclusiveScan(cd, waveSums, &sumA);
+    const uint32_t startS = blockExclusiveScan(cs, waveSums, &sumB);
+    if (t < S) {
+        o.dynStart[t] = startD;
+        o.statStart[t] = startS;
+        o.dynBounds[t] = make_int4(bnd[4 * t], bnd[4 * t + 1], bnd[4 * t + 2], bnd[4 * t + 3]);
+        const int* b = bnd + 4 * (S + t);
+        o.statBounds[t] = make_int4(b[0], b[1], b[2], b[3]);
+    }
+    if (t == 0) {
+        o.dynStart[S] = sumA;
+        o.statStart[S] = sumB;
+        o.header[0] = nDynTotal;
+        o.header[1] = n - nDynTotal;
+        o.header[2] = bad;
+    }
+}
+
+// Pairs into the caller's two index arrays.
+__global__ __launch_bounds__(256) void kSplitPairs(const uint2* __restrict__ pairs, uint32_t n, uint32_t* __restrict__ a,
+                                                   uint32_t* __restrict__ b) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint2 p = pairs[i];
+        a[i] = p.x;
+        b[i] = p.y;
+    }
+}
+
 }  // namespace
+
+hipError_t launchBroadPrep(hipStream_t st, const float4* aabb, const uint8_t* isDynamic, const uint32_t* scene,
+                           uint32_t n, uint32_t S, const BroadPrep& out) {
+    if (S == 0 || S > BROAD_PREP_THREADS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kBroadPrep, dim3(1), dim3(BROAD_PREP_THREADS), 40 * S, st, aabb, isDynamic, scene, n, S, out);
+    return hipGetLastError();
+}
+
+hipError_t launchSplitPairs(hipStream_t st, const uint2* pairs, uint32_t n, uint32_t* a, uint32_t* b) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kSplitPairs, dim3(std::min(blocksFor(n, 256), 4096u)), dim3(256), 0, st, pairs, n, a, b);
+    return hipGetLastError();
+}
 
 hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t) {
     if (t.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kBoundsInit, dim3(blocksFor(w.S, 256)), dim3(256), 0, st, reinterpret_cast<int4*>(t.sceneBounds), w.S);
-    hipLaunchKernelGGL(kBounds, dim3(t.chunks), dim3(256), 0, st, t.list, w.aabb, t.chunkScene, t.chunkRange,
-                       reinterpret_cast<int4*>(t.sceneBounds));
+    if (!t.boundsReady) {
+        hipLaunchKernelGGL(kBoundsInit, dim3(blocksFor(w.S, 256)), dim3(256), 0, st, reinterpret_cast<int4*>(t.sceneBounds),
+                           w.S);
+        hipLaunchKernelGGL(kBounds, dim3(t.chunks), dim3(256), 0, st, t.list, w.aabb, t.chunkScene, t.chunkRange,
+                           reinterpret_cast<int4*>(t.sceneBounds));
+    }
     hipLaunchKernelGGL(kMorton, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.list, t.n, w.aabb, w.colScene,
                        reinterpret_cast<const int4*>(t.sceneBounds), w.mortonBits, t.keys, t.vals);
     const uint32_t keyBits = w.mortonBits + w.sceneBits;

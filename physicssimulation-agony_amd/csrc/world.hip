@@ -1312,6 +1312,10 @@ int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t
     d.C = n;
     std::vector<uint32_t> dynList, statList, body(owner, owner + n);
     for (uint32_t i = 0; i < n; i++) (is_dynamic[i] ? dynList : statList).push_back(i);
+    // scene by scene (ascending index within a scene), so every bounds chunk holds one scene
+    auto byScene = [&](uint32_t x, uint32_t y) { return sc[x] < sc[y]; };
+    std::stable_sort(dynList.begin(), dynList.end(), byScene);
+    std::stable_sort(statList.begin(), statList.end(), byScene);
     d.nDyn = (uint32_t)dynList.size();
     d.nStat = (uint32_t)statList.size();
     std::vector<float4> boxes(n);
