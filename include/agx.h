@@ -87,6 +87,9 @@ typedef struct agx_manifold {
 #define AGX_OVERFLOW_ITEMS 32u     /* more constraints than the dataflow kernels hold */
 #define AGX_OVERFLOW_LDS 64u       /* a scene larger than its per-scene LDS region was not solved */
 
+/* Most velocity sweeps of one step (velocity_iterations at most this). */
+#define AGX_MAX_SWEEPS 256
+
 /* Per-step counters of the last step. */
 typedef struct agx_stats {
     uint64_t pairs;                /* broad-phase pairs */
@@ -100,12 +103,15 @@ typedef struct agx_stats {
     uint32_t learned_warm_start;   /* 1: the step's warm start came from the network */
     float velocity_error;          /* computeConstraintErrors().x */
     float position_error;          /* computeConstraintErrors().y */
-    float residual[64];            /* relative impulse change after velocity sweep k */
+    float residual[AGX_MAX_SWEEPS]; /* relative impulse change after velocity sweep k */
     uint64_t solved_steps;         /* steps solved since the world was created */
     uint64_t total_velocity_sweeps; /* velocity sweeps over those steps (per step: the most any scene ran) */
     uint64_t total_body_sweeps;    /* over those steps: dynamic bodies x the velocity sweeps of their scene */
     float mean_velocity_sweeps;    /* last step: velocity sweeps per dynamic body (scenes may stop apart) */
     uint32_t solved_per_scene;     /* 1: the last step was solved scene by scene */
+    uint32_t converged_scenes;     /* last step: scenes whose velocity sweeps met residual_target */
+    uint64_t converged_bodies;     /* last step: dynamic bodies of those scenes */
+    uint64_t total_converged_bodies; /* over the solved steps: the sum of converged_bodies */
 } agx_stats;
 
 /* World sizes. */
@@ -226,6 +232,10 @@ int agx_world_read_stats(agx_world* w, agx_stats* out);
 /* Velocity sweeps each scene ran in the last step solved scene by scene (stats.solved_per_scene);
  * n must be the scene count. No counterpart in the reference (one simulation per scene there). */
 int agx_world_read_scene_sweeps(agx_world* w, uint32_t* out, uint32_t n);
+/* Each scene's relative residual (the norm of its impulse change over the norm of its impulses)
+ * after its last velocity sweep in the last step solved scene by scene with track_residual set;
+ * n must be the scene count. A scene has met residual_target when its value is at most the target. */
+int agx_world_read_scene_residual(agx_world* w, float* out, uint32_t n);
 
 /* GPU time per step segment, measured with HIP events on the world's stream while profiling
  * is on. Segments: see agx_profile_segment_name. agx_world_read_profile waits for the stream,

@@ -21,6 +21,7 @@ import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libagx.so"
+MAX_SWEEPS = 256   # AGX_MAX_SWEEPS
 
 SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON = 0, 1, 2
 
@@ -63,10 +64,11 @@ class Stats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("manifolds", C.c_uint64), ("contacts", C.c_uint64),
                 ("colours", C.c_uint32), ("islands", C.c_uint32), ("ccd_hits", C.c_uint32),
                 ("velocity_sweeps", C.c_uint32), ("overflow", C.c_uint32), ("learned_warm_start", C.c_uint32),
-                ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * 64),
+                ("velocity_error", C.c_float), ("position_error", C.c_float), ("residual", C.c_float * MAX_SWEEPS),
                 ("solved_steps", C.c_uint64), ("total_velocity_sweeps", C.c_uint64),
                 ("total_body_sweeps", C.c_uint64), ("mean_velocity_sweeps", C.c_float),
-                ("solved_per_scene", C.c_uint32)]
+                ("solved_per_scene", C.c_uint32), ("converged_scenes", C.c_uint32),
+                ("converged_bodies", C.c_uint64), ("total_converged_bodies", C.c_uint64)]
 
 
 class Gate(C.Structure):
@@ -131,6 +133,7 @@ _SIGS = {
     "agx_world_read_solve_order": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_islands": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "agx_world_read_scene_residual": (C.c_int, [_P, _P, C.c_uint32]),
     "agx_world_read_scene_sweeps": (C.c_int, [_P, _P, C.c_uint32]),
     "agx_broadphase_pairs": (C.c_int, [_P, _P, _P, _P, C.c_uint32, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_broadphase_pairs_noalloc": (C.c_int, [_P, _P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, C.c_uint64,
@@ -423,6 +426,12 @@ class World:
         s = Stats()
         _check(lib().agx_world_read_stats(self._h, C.byref(s)), self._h)
         return s
+
+    def scene_residual(self) -> np.ndarray:
+        """Relative residual of each scene after its last velocity sweep of the last per-scene solve."""
+        out = np.empty(self.counts.scenes, np.float32)
+        _check(lib().agx_world_read_scene_residual(self._h, _ptr(out), len(out)), self._h)
+        return out
 
     def scene_sweeps(self) -> np.ndarray:
         """Velocity sweeps each scene ran in the last step solved scene by scene."""

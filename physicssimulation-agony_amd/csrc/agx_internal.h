@@ -15,7 +15,7 @@ constexpr uint32_t MAX_COLOURS = FIRST_FIT_COLOURS + 1;   // + one shared overfl
 constexpr uint32_t COLOUR_BLOCKS = 512;     // blocks of the colour histogram / scatter kernels
 constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataflow kernels
 constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour start on a wave boundary
-constexpr uint32_t MAX_SWEEPS = 64;
+constexpr uint32_t MAX_SWEEPS = AGX_MAX_SWEEPS;
 constexpr uint32_t TRAVERSAL_STACK = 64;
 constexpr uint32_t SORT_TILE_SMALL = 1024;  // keys per radix-sort block (256 threads x 4) ...
 constexpr uint32_t SORT_TILE_LARGE = 4096;  // ... and for trees of at least SORT_LARGE_KEYS keys (x 16):
@@ -73,6 +73,7 @@ struct Counters {
     unsigned long long solveSteps;   // steps solved since the world was created
     unsigned long long sweepsTotal;  // velocity sweeps run over those steps (per step: the most any scene ran)
     unsigned long long bodySweepsTotal;   // sum over those steps of dynamic bodies x the sweeps of their scene
+    unsigned long long convergedBodiesTotal;   // sum over those steps of convergedBodies
     // per step
     uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
     uint32_t manifolds;
@@ -101,6 +102,9 @@ struct Counters {
     uint32_t colourCursor2;    // per-scene colouring: scenes taken by lane groups
     uint32_t perScene;         // this step was solved scene by scene
     unsigned long long bodySweeps;   // this step: dynamic bodies x the sweeps of their scene
+    unsigned long long convergedBodies;   // this step: dynamic bodies of scenes that met the residual target
+    uint32_t convergedScenes;  // this step: scenes that met the residual target
+    uint32_t flowMet;          // this step: the dataflow solve met the residual target
 };
 
 constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)

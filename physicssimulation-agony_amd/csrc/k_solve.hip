@@ -999,6 +999,7 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
     };
     float* myRes = blockRes + (size_t)blockIdx.x * 2 * MAX_SWEEPS;
     uint32_t sweepsDone = vIt;
+    bool met = false;
     if (residualTarget > 0.0f) {
         uint32_t gen = 0;
         for (uint32_t s = 0; s < vIt; s++) {
@@ -1020,7 +1021,11 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
             num = red[0][0] + red[0][1] + red[0][2] + red[0][3];
             den = red[1][0] + red[1][1] + red[1][2] + red[1][3];
             __syncthreads();
-            if (sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12)) { sweepsDone = s + 1; break; }
+            if (sqrt(num) <= (double)residualTarget * fmax(sqrt(den), 1e-12)) {
+                sweepsDone = s + 1;
+                met = true;
+                break;
+            }
         }
         run(0, 0, 0, pIt);
     } else {
@@ -1031,6 +1036,7 @@ __global__ __launch_bounds__(256) void kSolveFlow(Counters* __restrict__ ctr, Co
         for (uint32_t q = threadIdx.x; q < sweepsDone; q += blockDim.x) { myRes[q] = res[q]; myRes[MAX_SWEEPS + q] = res[MAX_SWEEPS + q]; }
     if (tid == 0) {
         ctr->sweeps = sweepsDone;
+        ctr->flowMet = met ? 1u : 0u;
         ctr->solveSteps += 1;
         ctr->sweepsTotal += sweepsDone;
     }
@@ -1141,8 +1147,10 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
                                       float4* __restrict__ vel, float4* __restrict__ pos, uint32_t b0, uint32_t b1,
                                       const uint2* __restrict__ range, float2* __restrict__ sceneRes, uint32_t K,
                                       uint32_t vIt, uint32_t pIt, int track, float residualTarget, float4* body,
-                                      float2* binv, const float2* __restrict__ inv, uint2* rng, double2* part) {
+                                      float2* binv, const float2* __restrict__ inv, uint2* rng, double2* part,
+                                      bool& met) {
     const uint32_t t = L.t, T = L.T;
+    met = false;
     for (uint32_t k = t; k < K; k += T) rng[k] = range[(size_t)s * MAX_COLOURS + k];
     for (uint32_t b = b0 + t; b < b1; b += T) {
         body[b - b0] = vel[b];
@@ -1204,7 +1212,11 @@ __device__ inline uint32_t solveScene(SceneLanes& L, uint32_t s, Counters* __res
             });
         const double2 r = sceneSum(L, (double)dsq, (double)lsq, part);
         if (t == 0 && track) sceneRes[(size_t)s * MAX_SWEEPS + q] = make_float2((float)r.x, (float)r.y);
-        if (residualTarget > 0.0f && sqrt(r.x) <= (double)residualTarget * fmax(sqrt(r.y), 1e-12)) { done = q + 1; break; }
+        if (residualTarget > 0.0f && sqrt(r.x) <= (double)residualTarget * fmax(sqrt(r.y), 1e-12)) {
+            done = q + 1;
+            met = true;
+            break;
+        }
     }
     for (uint32_t b = b0 + t; b < b1; b += T) vel[b] = body[b - b0];
     L.sync();
@@ -1297,13 +1309,18 @@ __global__ __launch_bounds__(SCENE_THREADS) void kSceneSolve(Counters* __restric
             L.sync();   // every wave has read taken[] before the next claim rewrites it
             continue;
         }
+        bool met;
         const uint32_t done = solveScene(L, s, ctr, cs, vel, pos, sceneBodyStart[s], sceneBodyStart[s + 1], range, sceneRes, K,
                                          vIt, pIt, track, residualTarget, lds, reinterpret_cast<float2*>(lds + cap), inv,
-                                         rng[L.g], part + L.g * L.waves);
+                                         rng[L.g], part + L.g * L.waves, met);
         if (L.t == 0) {
             sceneSweeps[s] = done;
             atomicMax(&ctr->sweeps, done);
             atomicAdd(&ctr->bodySweeps, (unsigned long long)sceneDyn[s] * done);
+            if (met) {
+                atomicAdd(&ctr->convergedBodies, (unsigned long long)sceneDyn[s]);
+                atomicAdd(&ctr->convergedScenes, 1u);
+            }
         }
     }
 }
@@ -1364,12 +1381,19 @@ __global__ void kSceneTotals(Counters* ctr) {
     ctr->solveSteps += 1;
     ctr->sweepsTotal += ctr->sweeps;
     ctr->bodySweepsTotal += ctr->bodySweeps;
+    ctr->convergedBodiesTotal += ctr->convergedBodies;
 }
 
-// Step totals of the dataflow solve (all dynamic bodies ran the same sweeps).
-__global__ void kFlowTotals(Counters* ctr, uint32_t nDynBodies) {
+// Step totals of the dataflow solve (all dynamic bodies ran the same sweeps and share one
+// residual: every scene met the target or none did).
+__global__ void kFlowTotals(Counters* ctr, uint32_t nDynBodies, uint32_t S) {
     ctr->bodySweeps = (unsigned long long)nDynBodies * ctr->sweeps;
     ctr->bodySweepsTotal += ctr->bodySweeps;
+    if (ctr->flowMet) {
+        ctr->convergedBodies = nDynBodies;
+        ctr->convergedScenes = S;
+        ctr->convergedBodiesTotal += nDynBodies;
+    }
 }
 
 // Accumulators back to the manifolds (the warm-start source of the next step) and
@@ -1774,7 +1798,7 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
                      (void*)&w.blockRes, (void*)&vIt, (void*)&pIt, (void*)&trackResidual, (void*)&residualTarget };
     hipError_t e = launchResident(reinterpret_cast<const void*>(kSolveFlow), (uint32_t)w.solveBlocks, w.numCU, args, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kFlowTotals, dim3(1), dim3(1), 0, st, w.ctr, w.nDynBodies);
+    hipLaunchKernelGGL(kFlowTotals, dim3(1), dim3(1), 0, st, w.ctr, w.nDynBodies, w.S);
     return hipGetLastError();
 }
 

@@ -1247,11 +1247,33 @@ int agx_world_read_stats(agx_world* w, agx_stats* out) {
     out->total_body_sweeps = c->bodySweepsTotal;
     out->mean_velocity_sweeps = (float)((double)c->bodySweeps / (double)std::max(w->d.nDynBodies, 1u));
     out->solved_per_scene = c->perScene;
+    out->converged_scenes = c->convergedScenes;
+    out->converged_bodies = c->convergedBodies;
+    out->total_converged_bodies = c->convergedBodiesTotal;
     out->velocity_error = c->velocityError;
     out->position_error = c->positionError;
-    for (uint32_t k = 0; k < 64 && k < agx::MAX_SWEEPS; k++)
+    for (uint32_t k = 0; k < agx::MAX_SWEEPS; k++)
         out->residual[k] = (float)(std::sqrt(c->residualNum[k]) / std::fmax(std::sqrt(c->residualDen[k]), 1e-12));
     delete c;
+    return AGX_OK;
+}
+
+int agx_world_read_scene_residual(agx_world* w, float* out, uint32_t n) {
+    if (!w || !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (n != w->d.S) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u scenes, got %u", w->d.S, n);
+    if (!w->settings.track_residual) return fail(w, AGX_ERR_STATE, "residuals are tracked only with track_residual");
+    if (!w->d.sceneRes || !w->d.sceneSweeps) return fail(w, AGX_ERR_STATE, "the world does not solve scene by scene");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    std::vector<uint32_t> sweeps(n);
+    std::vector<float2> res((size_t)n * agx::MAX_SWEEPS);
+    HIP_CHECK(w, hipMemcpy(sweeps.data(), w->d.sceneSweeps, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(res.data(), w->d.sceneRes, res.size() * sizeof(float2), hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < n; s++) {
+        const uint32_t q = std::min(sweeps[s], agx::MAX_SWEEPS);
+        if (q == 0) { out[s] = 0.0f; continue; }
+        const float2 r = res[(size_t)s * agx::MAX_SWEEPS + q - 1];
+        out[s] = (float)(std::sqrt((double)r.x) / std::fmax(std::sqrt((double)r.y), 1e-12));
+    }
     return AGX_OK;
 }
 

@@ -135,6 +135,7 @@ def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes, mon
     st = w.stats()
     assert st.solved_per_scene == 1 and st.overflow == 0
     sweeps = w.scene_sweeps()
+    gres = w.scene_residual()
     order, _ = w.solve_order()
     after = {f: w.body(f) for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y")}
     pos_of = np.empty(len(order), np.int64)
@@ -165,12 +166,22 @@ def test_per_scene_target_matches_oracle_scene_by_scene(agx, oracle, scenes, mon
         if near:
             continue
         assert sweeps[s_] == k, (s_, int(sweeps[s_]), k, res[:k + 1])
+        assert gres[s_] == pytest.approx(res[k - 1], rel=1e-3), (s_, float(gres[s_]), float(res[k - 1]))
         ow = fresh()
         ow.solve(ms, order=local, v_it=k, p_it=3)
         for f in ("vel_x", "vel_y", "ang_vel", "com_x", "com_y"):
             np.testing.assert_allclose(after[f][bs], ow.get(renames[f]), rtol=1e-4, atol=1e-4, err_msg=f"{s_} {f}")
         checked += 1
     assert checked >= 7
+    # the convergence counts: every scene that stopped before the cap met the target; at the cap,
+    # those whose last residual is at most the target
+    dyn = np.bincount(bscene[w.body("inv_mass") != 0], minlength=len(parts))
+    early = sweeps < 64
+    assert np.all(gres[early] <= target * (1 + 1e-4))
+    at_cap_met = (sweeps == 64) & (gres <= target * (1 + 1e-4))
+    assert int(early.sum()) <= st.converged_scenes <= int(early.sum() + at_cap_met.sum())
+    assert int(dyn[early].sum()) <= st.converged_bodies <= int(dyn[early | at_cap_met].sum())
+    assert st.converged_bodies > 0
     assert st.velocity_sweeps == sweeps.max()
     assert len(set(sweeps.tolist())) > 1          # scenes stopped apart
     assert st.mean_velocity_sweeps < st.velocity_sweeps
@@ -184,3 +195,23 @@ def test_scene_too_large_for_lds_uses_the_dataflow(agx, scenes):
     assert big.counts.bodies > 11000
     big.step(DT, 3)
     assert big.stats().solved_per_scene == 0 and big.stats().overflow == 0
+
+
+def test_dataflow_solve_counts_convergence(agx, scenes):
+    """One scene solved by the device-wide dataflow with a residual target: the step's sweeps stop
+    at the target for all its bodies together, so either every dynamic body counts as converged or
+    none does, and the counts add up over the steps."""
+    w = _world(agx, [scenes.pile_small(seed=1)], velocity_iterations=agx.MAX_SWEEPS, residual_target=1e-3,
+               track_residual=1, solve_per_scene=0)
+    ndyn = w.counts.bodies - w.counts.static_bodies
+    total0 = w.stats().total_converged_bodies
+    steps = 0
+    for _ in range(6):
+        w.step(DT, 5)
+        steps += 5
+        st = w.stats()
+        assert st.solved_per_scene == 0 and st.overflow == 0
+        if st.velocity_sweeps < agx.MAX_SWEEPS:
+            assert st.converged_scenes == 1 and st.converged_bodies == ndyn
+        assert st.converged_bodies in (0, ndyn)
+    assert 0 < st.total_converged_bodies - total0 <= steps * ndyn
