@@ -17,9 +17,11 @@ single-threaded oracle) and the 1,048,576-collider scatter stepped collide-only 
 (pairs/s).
 """
 import argparse
+import concurrent.futures
 import importlib.util
 import json
 import math
+import os
 import sys
 import time
 from pathlib import Path
@@ -101,9 +103,46 @@ def measured_traffic(kernel, workload):
 
 
 
+def _cpu_flags():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+_CPU_INFO = None
+
+
+def cpu_info():
+    """Host CPU model, the CPUs this process may use, and the threads the CPU baseline runs on (at
+    most 16: the host-core share of one GPU here); read once, before any pool pins a thread."""
+    global _CPU_INFO
+    if _CPU_INFO is not None:
+        return _CPU_INFO
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0))
+    _CPU_INFO = {"model": model, "cpus_allowed": allowed, "threads": max(1, min(16, allowed))}
+    return _CPU_INFO
+
+
 def _oracle():
+    """The oracle as the CPU baseline: the -O3 AVX2/FMA build when the host has both (the same
+    results as the checker build), with the threaded step of the reference's structure."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as orc
+    fast = ROOT / "oracle" / "liboracle_fast.so"
+    if orc._LIB is None and fast.exists() and {"avx2", "fma"} <= _cpu_flags():
+        orc.LIB_PATH = fast
     return orc
 
 
@@ -127,18 +166,46 @@ def oracle_from_world(orc, scene, world, bsel, csel, v_it, p_it):
     return ow
 
 
-def time_oracle(worlds_and_sizes, budget_s):
-    """Steps oracle worlds round-robin (one update(1/60) each) for about budget_s of CPU time."""
+def time_oracle(worlds_and_sizes, budget_s, pool=None):
+    """Steps oracle worlds round-robin (one update(1/60) each, through the threaded step when a
+    pool is given) for about budget_s of wall time."""
     dt = 1.0 / 60.0
     bodies = steps = 0
     t0 = time.perf_counter()
     while True:
         for ow, n in worlds_and_sizes:
-            ow.update(dt, 1)
+            if pool is None:
+                ow.update(dt, 1)
+            else:
+                ow.update_threaded(pool, dt, 1)
             bodies += n
             steps += 1
             if time.perf_counter() - t0 > budget_s:
                 return bodies / (time.perf_counter() - t0), steps, time.perf_counter() - t0
+
+
+def time_oracle_scenes(worlds_and_sizes, budget_s, threads):
+    """Independent scenes stepped side by side: each of `threads` host threads steps its own share
+    of the worlds round-robin (the oracle's calls release the interpreter lock), for about budget_s
+    of wall time; body-steps/s over all threads."""
+    dt = 1.0 / 60.0
+    shares = [worlds_and_sizes[k::threads] for k in range(threads)]
+    deadline = time.perf_counter() + budget_s
+
+    def run(share):
+        bodies = steps = 0
+        while time.perf_counter() < deadline:
+            for ow, n in share:
+                ow.update(dt, 1)
+                bodies += n
+                steps += 1
+        return bodies, steps
+
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(run, [s for s in shares if s]))
+    el = time.perf_counter() - t0
+    return sum(r[0] for r in res) / el, sum(r[1] for r in res), el
 
 
 USE_GRAPHS = True   # --no-graphs: every step as ordinary launches (for kernel tracing)
@@ -417,6 +484,10 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
 
 
 def batch_cpu_baseline(world, scenes, sweeps, budget_s):
+    """The oracle on the host cores: 64 of the batch's settled scenes (a fixed random pick), each
+    stepped with update(1/60) at the plain GPU solver's mean sweep count, first on one thread, then
+    the scenes side by side on every host thread this process may use (the reference has no batch;
+    independent simulations on separate threads are how its CPU path would step many scenes)."""
     orc = _oracle()
     cs = world.collider("scene")
     cb = world.collider("body")
@@ -430,11 +501,16 @@ def batch_cpu_baseline(world, scenes, sweeps, budget_s):
         ci = np.flatnonzero(cs == i)
         worlds.append((oracle_from_world(orc, scenes[i], world, slice(int(bi[0]), int(bi[-1]) + 1),
                                          slice(int(ci[0]), int(ci[-1]) + 1), sweeps, 3), len(bi)))
-    value, steps, total = time_oracle(worlds, budget_s)
-    return {"value": round(value, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} oracle scene-steps (update(1/60), single thread, {sweeps} velocity + 3 position sweeps, "
-                      f"the plain GPU solver's sweep count at the 1e-3 target) over 64 of the batch's settled scenes; "
-                      f"{total:.1f} s of CPU time"}
+    info = cpu_info()
+    one, steps1, t1 = time_oracle(worlds, budget_s / 3)
+    value, steps, total = time_oracle_scenes(worlds, budget_s, info["threads"])
+    return {"value": round(value, 1), "unit": "body-steps/s", "cores": info["threads"], "kind": "port",
+            "one_thread_value": round(one, 1), "cpu": info["model"], "cpus_allowed": info["cpus_allowed"],
+            "build": orc.LIB_PATH.name,
+            "sample": f"{steps} oracle scene-steps (update(1/60), {sweeps} velocity + 3 position sweeps, the plain GPU "
+                      f"solver's mean sweep count at the 1e-3 target) over 64 of the batch's settled scenes, the "
+                      f"scenes side by side on {info['threads']} threads, {total:.1f} s; one thread: {steps1} "
+                      f"scene-steps in {t1:.1f} s"}
 
 
 def run_pile(agx, scenes_mod, args, dt, cpu):
@@ -464,12 +540,97 @@ def run_pile(agx, scenes_mod, args, dt, cpu):
     log(f"pile: {out['ms_per_step']} ms/step in {time.perf_counter() - t0:.1f} s")
     if cpu:
         orc = _oracle()
+        info = cpu_info()
         ow = oracle_from_world(orc, scene, world, slice(0, n), slice(0, world.counts.colliders), 8, 3)
-        value, steps_c, total = time_oracle([(ow, n)], args.cpu_seconds)
-        out["cpu_baseline"] = {"value": round(value, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{steps_c} oracle steps (update(1/60), single thread) of the same settled "
-                                         f"{n}-body pile; {total:.1f} s of CPU time"}
+        one, steps1, t1 = time_oracle([(ow, n)], args.cpu_seconds / 3)
+        ow = oracle_from_world(orc, scene, world, slice(0, n), slice(0, world.counts.colliders), 8, 3)
+        pool = orc.Pool(info["threads"])
+        value, steps_c, total = time_oracle([(ow, n)], args.cpu_seconds, pool)
+        pool.close()
+        out["cpu_baseline"] = {"value": round(value, 1), "unit": "body-steps/s", "cores": info["threads"],
+                               "kind": "port", "one_thread_value": round(one, 1), "cpu": info["model"],
+                               "cpus_allowed": info["cpus_allowed"], "build": orc.LIB_PATH.name,
+                               "sample": f"{steps_c} oracle steps (update(1/60)) of the same settled {n}-body pile "
+                                         f"with the reference's threading (narrow-phase chunks, wave-planned "
+                                         f"solver) on {info['threads']} threads, {total:.1f} s; one thread: "
+                                         f"{steps1} steps in {t1:.1f} s"}
         out["speedup_vs_cpu_baseline"] = round(out["value"] / value, 2)
+        out["speedup_vs_one_cpu_thread"] = round(out["value"] / one, 2)
+    return out
+
+
+def _state(mass, vx, vy, depth, errors):
+    """Momentum, kinetic energy and penetration of one state (dynamic bodies)."""
+    dyn = mass > 0
+    m, vx, vy = mass[dyn].astype(np.float64), vx[dyn].astype(np.float64), vy[dyn].astype(np.float64)
+    d = depth.astype(np.float64)
+    return {"px": float(np.sum(m * vx)), "py": float(np.sum(m * vy)), "kinetic": float(0.5 * np.sum(m * (vx * vx + vy * vy))),
+            "abs_momentum": float(np.sum(m * np.hypot(vx, vy))), "manifolds": int(len(d)),
+            "max_depth": float(d.max()) if len(d) else 0.0, "mean_depth": float(d.mean()) if len(d) else 0.0,
+            "velocity_error": float(errors[0]), "position_error": float(errors[1])}
+
+
+def _gpu_state(world):
+    st = world.stats()
+    return _state(world.body("mass"), world.body("vel_x"), world.body("vel_y"), world.manifolds(0)["depth"],
+                  (st.velocity_error, st.position_error))
+
+
+def _oracle_state(ow):
+    return _state(ow.get("mass"), ow.get("velX"), ow.get("velY"), ow.manifolds()["depth"], ow.errors())
+
+
+def _drift(series):
+    """Linear momentum drift relative to the run's initial momentum magnitude sum."""
+    p0 = series[0]
+    scale = max(p0["abs_momentum"], 1e-12)
+    return [round(math.hypot(s["px"] - p0["px"], s["py"] - p0["py"]) / scale, 9) for s in series]
+
+
+def run_small(agx, scenes_mod, args, dt, cpu):
+    """The 2,560-body pile (2,048 polygons, 512 circles in a 64 x 64 m box), 600 steps from the
+    drop at 8 + 3 sweeps, timed in blocks of 50 steps; momentum, kinetic energy and penetration
+    after each block. Beside it the same shapes as a free swarm without walls or gravity, where the
+    solver must conserve linear momentum: its drift over 600 steps. Both also on the host CPU."""
+    out = {"workload": "pile_small_2560_bodies_600_steps", "unit": "body-steps/s", "steps": 600}
+    for label, scene in (("pile", scenes_mod.pile_small(seed=1)), ("free_swarm", scenes_mod.free_swarm())):
+        settings = agx.default_settings(**scene["settings"], track_errors=1)
+        world = build_world(agx, [scene], settings)
+        ndyn = dynamic_bodies(world)
+        series = [_gpu_state(world)]
+        total = 0.0
+        for _ in range(12):
+            total += step_timed(world, 50, dt)
+            series.append(_gpu_state(world))
+        res = {"value": round(ndyn * 600 / total, 1), "ms_per_step": round(1e3 * total / 600, 4),
+               "series_every_50_steps": series}
+        if label == "free_swarm":
+            res["momentum_drift"] = _drift(series)
+        world.close()
+        if cpu:
+            orc = _oracle()
+            info = cpu_info()
+            pool = orc.Pool(info["threads"])
+            ow = orc.world_from_scene(scene, dict(v=8, p=3, gy=float(scene["settings"]["gravity_y"])))
+            cser = [_oracle_state(ow)]
+            ctotal = 0.0
+            for _ in range(12):
+                t0 = time.perf_counter()
+                ow.update_threaded(pool, dt, 50)
+                ctotal += time.perf_counter() - t0
+                cser.append(_oracle_state(ow))
+            pool.close()
+            res["cpu_baseline"] = {"value": round(ndyn * 600 / ctotal, 1), "unit": "body-steps/s",
+                                   "cores": info["threads"], "kind": "port", "cpu": info["model"],
+                                   "build": orc.LIB_PATH.name,
+                                   "sample": f"the same 600 steps with the oracle's threaded step ({ctotal:.1f} s)",
+                                   "series_every_50_steps": cser}
+            if label == "free_swarm":
+                res["cpu_baseline"]["momentum_drift"] = _drift(cser)
+            res["speedup_vs_cpu_baseline"] = round(res["value"] / res["cpu_baseline"]["value"], 2)
+        out[label] = res
+        log(f"{label}: {res['ms_per_step']} ms/step")
+    out["value"] = out["pile"]["value"]
     return out
 
 
@@ -495,27 +656,42 @@ def run_scatter(agx, scenes_mod, args, dt, cpu):
     log(f"scatter: {out['ms_per_step']} ms/step in {time.perf_counter() - t0:.1f} s")
     if cpu:
         orc = _oracle()
-        ow = orc.world_from_scene(scene, dict(v=8, p=3))
-        t1 = time.perf_counter()
-        ow.snapshot(); ow.integrate(dt); ow.wrap_cossin(); ow.poses(); ow.aabbs(); ow.ccd()
-        pa, pb = ow.broad_bvh()
-        ow.narrow(pa, pb)
-        total = time.perf_counter() - t1
-        out["cpu_baseline"] = {"value": round(len(pa) / total, 1), "unit": "pairs/s", "cores": 1, "kind": "port",
-                               "sample": f"one oracle collide-only step (integrate, poses, AABBs, CCD, BVH pairs, "
-                                         f"narrow phase) of the initial scatter, single thread; {total:.1f} s"}
+        info = cpu_info()
+        pool = orc.Pool(info["threads"])
+        res = {}
+        for label, p in (("one", None), ("all", pool)):
+            ow = orc.world_from_scene(scene, dict(v=8, p=3))
+            t1 = time.perf_counter()
+            ow.snapshot(); ow.integrate(dt); ow.wrap_cossin(); ow.poses(); ow.aabbs(); ow.ccd()
+            pa, pb = ow.broad_bvh()
+            if p is None:
+                ow.narrow(pa, pb)
+            else:
+                ow.narrow_threaded(p, pa, pb)
+            res[label] = (len(pa), time.perf_counter() - t1)
+            del ow
+        pool.close()
+        (pairs, total), (_, t_one) = res["all"], res["one"]
+        out["cpu_baseline"] = {"value": round(pairs / total, 1), "unit": "pairs/s", "cores": info["threads"],
+                               "kind": "port", "one_thread_value": round(pairs / t_one, 1), "cpu": info["model"],
+                               "cpus_allowed": info["cpus_allowed"], "build": orc.LIB_PATH.name,
+                               "sample": f"one oracle collide-only step (integrate, poses, AABBs, CCD, BVH pairs single-"
+                                         f"threaded as in the reference, narrow phase in 1,024-pair chunks on "
+                                         f"{info['threads']} threads) of the initial scatter, {total:.1f} s; one "
+                                         f"thread {t_one:.1f} s"}
         out["speedup_vs_cpu_baseline"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
     return out
 
 
 def main():
+    cpu_info()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60, help="timed steps of the headline batch")
     ap.add_argument("--settle", type=int, default=240,
                     help="settling steps of the headline batch (part of the workload: the settled batch)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps right before each timed region")
-    ap.add_argument("--workload", default="all", choices=["all", "batch1024", "pile65536", "scatter1m"])
+    ap.add_argument("--workload", default="all", choices=["all", "batch1024", "pile65536", "scatter1m", "pile2560"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--unbatched-stride", type=int, default=16,
                     help="every n-th scene of the batch is stepped in a world of its own (plain, unbatched); 0: none")
@@ -544,8 +720,8 @@ def main():
     scenes_mod = sys.modules["agx.scenes"]
     dt = 1.0 / 60.0
     cpu = not args.no_cpu
-    if args.workload in ("pile65536", "scatter1m"):
-        fn = run_pile if args.workload == "pile65536" else run_scatter
+    if args.workload in ("pile65536", "scatter1m", "pile2560"):
+        fn = {"pile65536": run_pile, "scatter1m": run_scatter, "pile2560": run_small}[args.workload]
         print(json.dumps(fn(agx, scenes_mod, args, dt, cpu)), flush=True)
         return
     # the pile and the scatter first: measured after the learned batch in the same process, the
@@ -555,6 +731,7 @@ def main():
     if args.workload == "all":
         side["pile65536"] = run_pile(agx, scenes_mod, args, dt, cpu)
         side["scatter1m"] = run_scatter(agx, scenes_mod, args, dt, cpu)
+        side["pile2560"] = run_small(agx, scenes_mod, args, dt, cpu)
     b = run_batch(agx, scenes_mod, args, dt, learned=not args.no_learned)
     head = b["learned"] if b["learned"] else b["plain"]
     out = {

@@ -1,5 +1,6 @@
 // C API of the oracle for ctypes (test infrastructure only; see oracle.h).
 #include "oracle.h"
+#include "solver_internal.h"
 
 #include <cstring>
 #include <string>
@@ -197,5 +198,40 @@ int64_t orc_get_manifolds(void* h, Manifold* out, uint64_t cap) {
     return (int64_t)w.manifolds.size();
 }
 void orc_errors(void* h, float* out) { out[0] = W(h).velocityError; out[1] = W(h).positionError; }
+
+// The threaded step (oracle/threaded.cpp): a pool of `participants` (pinned threads + the caller).
+void* orc_pool_new(uint32_t participants) { return poolCreate(participants); }
+void orc_pool_free(void* p) { poolDestroy(static_cast<Pool*>(p)); }
+uint32_t orc_pool_size(void* p) { return poolSize(static_cast<Pool*>(p)); }
+void orc_update_pool(void* h, void* pool, float dt, uint32_t steps) {
+    for (uint32_t i = 0; i < steps; i++) step(W(h), dt, static_cast<Pool*>(pool));
+    postUpdate(W(h));
+}
+// The solver's wave plan of manifolds m for `workers` workers: flat order (n entries), then the
+// passes' starts and sizes (wave-major) into passStart/passSize up to cap; returns the pass count.
+int64_t orc_plan_waves(void* h, const Manifold* m, uint64_t n, uint32_t workers, uint32_t* flat, uint32_t* passStart,
+                       uint32_t* passSize, uint64_t cap) {
+    const WavePlan plan = planWaves(W(h), std::vector<Manifold>(m, m + n), workers);
+    if (plan.passStart.size() > cap) return -(int64_t)plan.passStart.size();
+    std::memcpy(flat, plan.flat.data(), plan.flat.size() * sizeof(uint32_t));
+    std::memcpy(passStart, plan.passStart.data(), plan.passStart.size() * sizeof(uint32_t));
+    std::memcpy(passSize, plan.passSize.data(), plan.passSize.size() * sizeof(uint32_t));
+    return (int64_t)plan.passStart.size();
+}
+// BodyCollisionSolver::solveCollisions with its threaded path on the pool (m solved in place).
+void orc_solve_threaded(void* h, void* pool, Manifold* m, uint64_t n, uint32_t vIt, uint32_t pIt) {
+    std::vector<Manifold> ms(m, m + n);
+    solveThreaded(W(h), ms, *static_cast<Pool*>(pool), vIt, pIt);
+    std::memcpy(m, ms.data(), n * sizeof(Manifold));
+}
+// NarrowPhaseCollisionDetector::findCollisions with its threaded path on the pool.
+int64_t orc_narrow_threaded(void* h, void* pool, const uint32_t* pa, const uint32_t* pb, uint64_t n, Manifold* out,
+                            uint64_t cap) {
+    std::vector<Manifold> ms;
+    narrowPhaseThreaded(W(h), pa, pb, n, ms, *static_cast<Pool*>(pool));
+    if (ms.size() > cap) return -(int64_t)ms.size();
+    std::memcpy(out, ms.data(), ms.size() * sizeof(Manifold));
+    return (int64_t)ms.size();
+}
 
 }  // extern "C"

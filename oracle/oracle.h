@@ -126,7 +126,17 @@ void warmStartFromCache(const World& w, std::vector<Manifold>& cur);  // :272-29
 void solve(World& w, std::vector<Manifold>& m, const uint32_t* order,
            uint32_t vIt, uint32_t pIt, float* residualOut);   // Solving/BodyCollisionSolver.cpp:38-980
 void islands(World& w, const std::vector<Manifold>& m);       // IslandManager.cpp:7-116
-void step(World& w, float dt);                                // Simulation.cpp:255-351
+// ---- threaded step (the CPU baseline's structure; oracle/threaded.cpp) ----
+struct Pool;                                                  // pinned std::threads + the caller
+Pool* poolCreate(uint32_t participants);
+void poolDestroy(Pool* p);
+uint32_t poolSize(const Pool* p);
+void narrowPhaseThreaded(World& w, const uint32_t* pa, const uint32_t* pb, size_t n, std::vector<Manifold>& out,
+                         Pool& pool);
+void solveThreaded(World& w, std::vector<Manifold>& m, Pool& pool, uint32_t vIt, uint32_t pIt);
+
+// Simulation.cpp:255-351; with a pool, the narrow phase and the solver take its threaded paths.
+void step(World& w, float dt, Pool* pool = nullptr);
 void postUpdate(World& w);                                    // Simulation.cpp:640-655
 
 }  // namespace orc

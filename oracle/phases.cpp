@@ -341,7 +341,7 @@ void postUpdate(World& w) {
 }
 
 // Simulation.cpp:255-351 with sleeping, joints, the mouse holder and events off.
-void step(World& w, float dt) {
+void step(World& w, float dt, Pool* pool) {
     if (w.bodyCount() == 0) { w.prevManifolds.clear(); w.manifolds.clear(); return; }
     if (w.settings.ccd) snapshotPoses(w);
     integrate(w, dt);
@@ -359,10 +359,13 @@ void step(World& w, float dt) {
         if (w.pairA.empty()) {
             w.prevManifolds.clear();
         } else {
-            narrowPhase(w, w.pairA.data(), w.pairB.data(), w.pairA.size(), w.manifolds);
+            if (pool) narrowPhaseThreaded(w, w.pairA.data(), w.pairB.data(), w.pairA.size(), w.manifolds, *pool);
+            else narrowPhase(w, w.pairA.data(), w.pairB.data(), w.pairA.size(), w.manifolds);
             warmStartFromCache(w, w.manifolds);
-            if (!w.manifolds.empty())
-                solve(w, w.manifolds, nullptr, w.settings.velocityIterations, w.settings.positionIterations, nullptr);
+            if (!w.manifolds.empty()) {
+                if (pool) solveThreaded(w, w.manifolds, *pool, w.settings.velocityIterations, w.settings.positionIterations);
+                else solve(w, w.manifolds, nullptr, w.settings.velocityIterations, w.settings.positionIterations, nullptr);
+            }
             w.prevManifolds = w.manifolds;
         }
     } else {

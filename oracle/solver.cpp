@@ -3,6 +3,7 @@
 // solves in wave order; any order in which no two simultaneous constraints share a dynamic
 // body is equivalent to a sequential order).
 #include "oracle.h"
+#include "solver_internal.h"
 
 #include <cmath>
 #include <vector>
@@ -10,17 +11,6 @@
 namespace orc {
 
 namespace {
-
-struct PointData { float rAn, rBn, rAt, rBt, normalMass, tangentMass, bias; };
-struct Constraint {                // VelocityHotData + VelocityConstraintData + FrictionData + PositionHotData
-    uint32_t bodyA, bodyB, count, manifold;
-    float nx, ny;
-    float imp[4];                  // jn0, jt0, jn1, jt1
-    float invMassA, invMassB, invInertiaA, invInertiaB;
-    PointData p[2];
-    float staticFriction, dynamicFriction;
-    float penetrationBase;
-};
 
 constexpr float SLOP = 0.001f;            // BodyCollisionSolver.h:71-72
 constexpr float BAUMGARTE = 0.5f;
@@ -30,11 +20,8 @@ inline float clampf(float x, float lo, float hi) { return std::fmin(std::fmax(x,
 
 }  // namespace
 
-void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t vIt, uint32_t pIt, float* residualOut) {
-    const size_t M = ms.size();
-    std::vector<Constraint> cs(M);
-    // computeConstraintData (:229-390).
-    for (size_t c = 0; c < M; c++) {
+void Solver::prepare(const uint32_t* order, size_t lo, size_t hi) {
+    for (size_t c = lo; c < hi; c++) {
         const uint32_t mi = order ? order[c] : (uint32_t)c;
         const Manifold& d = ms[mi];
         Constraint& k = cs[c];
@@ -71,12 +58,10 @@ void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t 
             p.bias = (negVn > 0.0f) ? e * negVn : 0.0f;
         }
     }
-    auto load = [&w](uint32_t b, V2& v, float& av) { v = { w.velX[b], w.velY[b] }; av = w.angVel[b]; };
-    auto store = [&w](uint32_t b, V2 v, float av, float invMass) {
-        if (invMass > 0.0f) { w.velX[b] = v.x; w.velY[b] = v.y; w.angVel[b] = av; }
-    };
-    // applyWarmStarting (:416-490).
-    for (size_t c = 0; c < M; c++) {
+}
+
+void Solver::warmStart(size_t lo, size_t hi) {
+    for (size_t c = lo; c < hi; c++) {
         const Constraint& k = cs[c];
         V2 vA, vB; float wA, wB;
         load(k.bodyA, vA, wA); load(k.bodyB, vB, wB);
@@ -93,14 +78,11 @@ void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t 
         store(k.bodyA, vA, wA, k.invMassA);
         store(k.bodyB, vB, wB, k.invMassB);
     }
-    // solveVelocityConstraints (:492-928); block normal, iterative friction (.h:67-68).
-    std::vector<float> prev;
-    for (uint32_t it = 0; it < vIt; it++) {
-        if (residualOut) {
-            prev.resize(4 * M);
-            for (size_t c = 0; c < M; c++) for (int q = 0; q < 4; q++) prev[4 * c + q] = cs[c].imp[q];
-        }
-        for (size_t c = 0; c < M; c++) {
+}
+
+// Block normal, iterative friction (.h:67-68).
+void Solver::velocity(size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; c++) {
             Constraint& k = cs[c];
             V2 vA, vB; float wA, wB;
             load(k.bodyA, vA, wA); load(k.bodyB, vB, wB);
@@ -206,20 +188,10 @@ void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t 
             store(k.bodyA, vA, wA, iMA);
             store(k.bodyB, vB, wB, iMB);
         }
-        if (residualOut) {
-            double dsq = 0.0, lsq = 0.0;
-            for (size_t c = 0; c < M; c++)
-                for (uint32_t q = 0; q < 2 * cs[c].count; q++) {
-                    const double d = (double)cs[c].imp[q] - (double)prev[4 * c + q];
-                    dsq += d * d;
-                    lsq += (double)cs[c].imp[q] * (double)cs[c].imp[q];
-                }
-            residualOut[it] = (float)(std::sqrt(dsq) / std::fmax(std::sqrt(lsq), 1e-12));
-        }
-    }
-    // solvePositionConstraints (:930-980).
-    for (uint32_t it = 0; it < pIt; it++) {
-        for (size_t c = 0; c < M; c++) {
+}
+
+void Solver::position(size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; c++) {
             const Constraint& k = cs[c];
             const uint32_t bA = k.bodyA, bB = k.bodyB;
             const V2 pA{ w.comX[bA], w.comY[bA] }, pB{ w.comX[bB], w.comY[bB] };
@@ -233,8 +205,10 @@ void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t 
             if (iMA > 0) { w.comX[bA] -= vcA.x; w.comY[bA] -= vcA.y; }
             if (iMB > 0) { w.comX[bB] += vcB.x; w.comY[bB] += vcB.y; }
         }
-    }
-    // scatterImpulses (:392-414).
+}
+
+void Solver::finish() {
+    const size_t M = cs.size();
     for (size_t c = 0; c < M; c++) {
         Manifold& d = ms[cs[c].manifold];
         d.jn0 = cs[c].imp[0]; d.jt0 = cs[c].imp[1]; d.jn1 = cs[c].imp[2]; d.jt1 = cs[c].imp[3];
@@ -255,6 +229,33 @@ void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t 
     }
     w.velocityError = tApp;
     w.positionError = tPen;
+}
+
+void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t vIt, uint32_t pIt, float* residualOut) {
+    Solver sv(w, ms);
+    const size_t M = ms.size();
+    sv.prepare(order, 0, M);
+    sv.warmStart(0, M);
+    std::vector<float> prev;
+    for (uint32_t it = 0; it < vIt; it++) {
+        if (residualOut) {
+            prev.resize(4 * M);
+            for (size_t c = 0; c < M; c++) for (int q = 0; q < 4; q++) prev[4 * c + q] = sv.cs[c].imp[q];
+        }
+        sv.velocity(0, M);
+        if (residualOut) {
+            double dsq = 0.0, lsq = 0.0;
+            for (size_t c = 0; c < M; c++)
+                for (uint32_t q = 0; q < 2 * sv.cs[c].count; q++) {
+                    const double d = (double)sv.cs[c].imp[q] - (double)prev[4 * c + q];
+                    dsq += d * d;
+                    lsq += (double)sv.cs[c].imp[q] * (double)sv.cs[c].imp[q];
+                }
+            residualOut[it] = (float)(std::sqrt(dsq) / std::fmax(std::sqrt(lsq), 1e-12));
+        }
+    }
+    for (uint32_t it = 0; it < pIt; it++) sv.position(0, M);
+    sv.finish();
 }
 
 }  // namespace orc

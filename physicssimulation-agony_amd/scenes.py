@@ -182,6 +182,27 @@ def pile(n_poly: int, n_circle: int, width: float, height: float, seed: int, r_p
             "settings": {"velocity_iterations": 8, "position_iterations": 3, "gravity_y": -9.81}}
 
 
+def free_swarm(n_poly: int = 2048, n_circle: int = 512, size: float = 64.0, speed: float = 3.0, seed: int = 6,
+               name: str = "free_swarm") -> dict:
+    """The small pile's shapes without walls or gravity, started at random velocities (uniform in
+    a disc of radius `speed`) and spins: every contact is between two dynamic bodies, so total
+    linear momentum is conserved by the solver's equal and opposite impulses."""
+    rng = np.random.default_rng(seed)
+    acc = _Acc()
+    shapes = _shapes(rng, n_poly, n_circle, ("uniform", 0.2, 0.6), ("uniform", 0.2, 0.5), ("const", 1.0))
+    pos = _place_uniform(rng, shapes, -size / 2, -size / 2, size / 2, size / 2)
+    for (t, r, v, m), (x, y) in zip(shapes, pos):
+        ang, mag = rng.uniform(0.0, 2.0 * np.pi), speed * math.sqrt(rng.uniform())
+        kw = dict(rot=float(rng.uniform(0.0, 6.28)), vx=float(mag * math.cos(ang)), vy=float(mag * math.sin(ang)),
+                  av=float(rng.uniform(-1.0, 1.0)), mass=m)
+        if t == SHAPE_CIRCLE:
+            acc.add(t, x, y, p0=r, **kw)
+        else:
+            acc.add(t, x, y, verts=v, **kw)
+    return {"name": name, "materials": [(0.0, 0.6, 0.6)], "bodies": acc.bodies(),
+            "settings": {"velocity_iterations": 8, "position_iterations": 3, "gravity_y": 0.0}}
+
+
 def pile_small(seed: int = 1) -> dict:
     """2,048 polygons + 512 circles in a 64 x 64 m box."""
     return pile(2048, 512, 64.0, 64.0, seed, name="pile_small")

@@ -35,6 +35,11 @@ _SIGS = {
     "orc_step": (None, [_P, C.c_float]), "orc_post_update": (None, [_P]),
     "orc_update": (None, [_P, C.c_float, C.c_uint32]),
     "orc_get_manifolds": (C.c_int64, [_P, _P, C.c_uint64]), "orc_errors": (None, [_P, _P]),
+    "orc_pool_new": (_P, [C.c_uint32]), "orc_pool_free": (None, [_P]), "orc_pool_size": (C.c_uint32, [_P]),
+    "orc_update_pool": (None, [_P, _P, C.c_float, C.c_uint32]),
+    "orc_plan_waves": (C.c_int64, [_P, _P, C.c_uint64, C.c_uint32, _P, _P, _P, C.c_uint64]),
+    "orc_solve_threaded": (None, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_narrow_threaded": (C.c_int64, [_P, _P, _P, _P, C.c_uint64, _P, C.c_uint64]),
     "orc_mlp_forward": (None, [_P, _P, C.c_uint64, _P, C.c_int]),
     "orc_mlp_gradients": (C.c_double, [_P, _P, _P, C.c_uint64, _P, C.c_int]),
     "orc_mlp_loss": (C.c_double, [_P, _P, _P, _P, C.c_uint64]),
@@ -148,6 +153,33 @@ class World:
     def step(self, dt): lib().orc_step(self.h, dt)
     def post_update(self): lib().orc_post_update(self.h)
     def update(self, dt, steps=1): lib().orc_update(self.h, dt, steps)
+    def update_threaded(self, pool, dt, steps=1): lib().orc_update_pool(self.h, pool.h, dt, steps)
+
+    def plan_waves(self, m, workers):
+        """The solver's wave plan: (flat order, pass starts, pass sizes), passes wave-major."""
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        cap = max(1024, 4 * workers)
+        while True:
+            flat = np.empty(max(len(m), 1), np.uint32)
+            ps = np.empty(cap, np.uint32)
+            sz = np.empty(cap, np.uint32)
+            k = lib().orc_plan_waves(self.h, _p(m), len(m), workers, _p(flat), _p(ps), _p(sz), cap)
+            if k >= 0:
+                return flat[:len(m)], ps[:k], sz[:k]
+            cap = -k
+
+    def solve_threaded(self, pool, m, v_it=8, p_it=3):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE).copy()
+        lib().orc_solve_threaded(self.h, pool.h, _p(m), len(m), v_it, p_it)
+        return m
+
+    def narrow_threaded(self, pool, pa, pb):
+        pa = np.ascontiguousarray(pa, np.uint32)
+        pb = np.ascontiguousarray(pb, np.uint32)
+        cap = max(len(pa), 1)
+        out = np.empty(cap, MANIFOLD_DTYPE)
+        n = lib().orc_narrow_threaded(self.h, pool.h, _p(pa), _p(pb), len(pa), _p(out), cap)
+        return out[:n]
 
     def ccd(self):
         t = np.empty(self.counts()["bodies"], np.float32)
@@ -260,3 +292,22 @@ def mlp_loss(params, x, t, delta=None):
     t = np.ascontiguousarray(t, np.float32)
     d = None if delta is None else np.ascontiguousarray(delta, np.float64)
     return float(lib().orc_mlp_loss(_p(params), _p(d), _p(x), _p(t), x.shape[0]))
+
+
+class Pool:
+    """Pinned worker threads plus the calling thread for the oracle's threaded step."""
+
+    def __init__(self, participants):
+        self.h = lib().orc_pool_new(int(participants))
+
+    @property
+    def size(self):
+        return int(lib().orc_pool_size(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_pool_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
