@@ -207,6 +207,37 @@ int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps);
  * Simulation::getBodyCollisionData (Simulation.h:186), getBroadPhaseCollisionDetector
  * (Simulation.h:183), islands() (Simulation.h:203), getDebugData (Simulation.h:189). */
 int agx_world_read_body_f32(agx_world* w, uint32_t field, float* out, uint32_t n);
+
+/* Deferred modifications: ObjectAPI::setBodyPosition / setBodyRotation / setBodyVelocity /
+ * setBodyAngularVelocity / setBodyMass / setBodyInertia (SimulationAPIs/ObjectAPI.h:76-81),
+ * queued for body index `body` and applied in call order at the start of the next
+ * agx_world_update / _step / _step_graph / run_phase(INTEGRATE), or by agx_world_flush
+ * (ObjectManager::flushCommands, SimulationImpl/ObjectManager.cpp:638-865):
+ *   POSITION: (a, b) = the body origin; the centre of mass is placed at origin + R(rotation) local COM;
+ *   ROTATION: a, wrapped to [0, 2 pi); cos/sin refreshed;
+ *   VELOCITY: (a, b); ANGULAR_VELOCITY: a;
+ *   MASS: a > 0 on a dynamic body (the inertia scales with the mass; moving a body between the
+ *         static and dynamic groups is not supported and is refused);
+ *   INERTIA: max(0, a) on a dynamic body (ignored on a static one).
+ * As in the reference, collider poses follow at the next step's pose pass, so that step's CCD
+ * sweep starts from the pose before the edit. */
+#define AGX_SET_POSITION 0
+#define AGX_SET_ROTATION 1
+#define AGX_SET_VELOCITY 2
+#define AGX_SET_ANGULAR_VELOCITY 3
+#define AGX_SET_MASS 4
+#define AGX_SET_INERTIA 5
+int agx_world_queue_body(agx_world* w, uint32_t kind, uint32_t body, float a, float b);
+
+/* NarrowPhaseCollisionDetector::findCollisions on injected pairs
+ * (Narrow/NarrowPhaseCollisionDetector.cpp:101-151, :638-767): the manifolds of collider pairs
+ * (pairs_a[i], pairs_b[i]) at the world's current collider poses, in canonical A/B with contact
+ * ids, warm-started from the previous step's manifolds, in pair order. Between steps only; the
+ * next step's broad and narrow phases are not affected. AGX_ERR_CAPACITY when more than cap
+ * manifolds (*count receives the total). */
+int agx_world_narrowphase(agx_world* w, const uint32_t* pairs_a, const uint32_t* pairs_b, uint64_t n, agx_manifold* out,
+                          uint64_t cap, uint64_t* count);
+int agx_world_flush(agx_world* w);
 int agx_world_write_body_f32(agx_world* w, uint32_t field, const float* in, uint32_t n);
 int agx_world_read_collider_f32(agx_world* w, uint32_t field, float* out, uint32_t n);
 int agx_world_read_collider_u32(agx_world* w, uint32_t field, uint32_t* out, uint32_t n);

@@ -188,6 +188,7 @@ uint32_t orc_islands(void* h, const Manifold* m, uint64_t n) {
 void orc_step(void* h, float dt) { step(W(h), dt); }
 void orc_post_update(void* h) { postUpdate(W(h)); }
 void orc_update(void* h, float dt, uint32_t steps) {
+    flushCommands(W(h));   // Simulation::update flushes first (Simulation.cpp:38-39)
     for (uint32_t i = 0; i < steps; i++) step(W(h), dt);
     postUpdate(W(h));
 }
@@ -199,11 +200,15 @@ int64_t orc_get_manifolds(void* h, Manifold* out, uint64_t cap) {
 }
 void orc_errors(void* h, float* out) { out[0] = W(h).velocityError; out[1] = W(h).positionError; }
 
+void orc_queue_body(void* h, uint32_t kind, uint32_t body, float a, float b) { W(h).commands.push_back({ kind, body, a, b }); }
+void orc_flush(void* h) { flushCommands(W(h)); }
+
 // The threaded step (oracle/threaded.cpp): a pool of `participants` (pinned threads + the caller).
 void* orc_pool_new(uint32_t participants) { return poolCreate(participants); }
 void orc_pool_free(void* p) { poolDestroy(static_cast<Pool*>(p)); }
 uint32_t orc_pool_size(void* p) { return poolSize(static_cast<Pool*>(p)); }
 void orc_update_pool(void* h, void* pool, float dt, uint32_t steps) {
+    flushCommands(W(h));
     for (uint32_t i = 0; i < steps; i++) step(W(h), dt, static_cast<Pool*>(pool));
     postUpdate(W(h));
 }

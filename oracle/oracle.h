@@ -92,6 +92,9 @@ struct World {
     uint32_t islandCount = 0;
     // BodyCollisionSolver::computeConstraintErrors (BodyCollisionSolver.cpp:132-183), after the solve.
     float velocityError = 0.0f, positionError = 0.0f;
+    // Deferred body modifications (ObjectAPI.h:76-81), applied by flushCommands.
+    struct BodyCommand { uint32_t kind, body; float a, b; };
+    std::vector<BodyCommand> commands;
 
     uint32_t bodyCount() const { return (uint32_t)comX.size(); }
     uint32_t colliderCount() const { return (uint32_t)colBody.size(); }
@@ -138,5 +141,7 @@ void solveThreaded(World& w, std::vector<Manifold>& m, Pool& pool, uint32_t vIt,
 // Simulation.cpp:255-351; with a pool, the narrow phase and the solver take its threaded paths.
 void step(World& w, float dt, Pool* pool = nullptr);
 void postUpdate(World& w);                                    // Simulation.cpp:640-655
+// Kinds: 0 position (body origin), 1 rotation, 2 velocity, 3 angular velocity, 4 mass, 5 inertia.
+void flushCommands(World& w);                                 // SimulationImpl/ObjectManager.cpp:638-865
 
 }  // namespace orc

@@ -332,6 +332,63 @@ void islands(World& w, const std::vector<Manifold>& m) {
     w.islandCount = k;
 }
 
+// ---------------------------------------------------------------- deferred modifications
+namespace {
+float wrapAngle(float angle) {   // ObjectManager.cpp:614-619
+    float wrapped = std::fmod(angle, TWO_PI);
+    if (wrapped < 0.0f) wrapped += TWO_PI;
+    return wrapped;
+}
+}  // namespace
+
+// ObjectManager::flushCommands / applyCommand (ObjectManager.cpp:638-865), body commands only; a
+// position or rotation edit makes the next step rebuild every collider's pose (spatialDirty).
+void flushCommands(World& w) {
+    for (const World::BodyCommand& c : w.commands) {
+        const uint32_t b = c.body;
+        switch (c.kind) {
+        case 0: {
+            const float r = w.rot[b];
+            const float cosR = std::cos(r), sinR = std::sin(r);
+            const float lx = w.localComX[b], ly = w.localComY[b];
+            w.comX[b] = c.a + cosR * lx - sinR * ly;
+            w.comY[b] = c.b + sinR * lx + cosR * ly;
+            w.spatialDirty = true;
+            break;
+        }
+        case 1: {
+            const float r = wrapAngle(c.a);
+            w.rot[b] = r;
+            w.cosv[b] = std::cos(r);
+            w.sinv[b] = std::sin(r);
+            w.spatialDirty = true;
+            break;
+        }
+        case 2: w.velX[b] = c.a; w.velY[b] = c.b; break;
+        case 3: w.angVel[b] = c.a; break;
+        case 4: {   // applyBodyMass (:867-905), dynamic bodies only
+            const float newMass = std::fmax(0.0f, c.a), oldMass = w.mass[b];
+            if (!(newMass > 0.0f) || !(oldMass > 0.0f)) break;
+            const float inertia = w.inertia[b] * (newMass / oldMass);
+            w.mass[b] = newMass;
+            w.invMass[b] = 1.0f / newMass;
+            w.inertia[b] = inertia;
+            w.invInertia[b] = inertia > 0.0f ? 1.0f / inertia : 0.0f;
+            break;
+        }
+        case 5: {
+            if (w.invMass[b] == 0.0f) break;
+            const float inertia = std::fmax(0.0f, c.a);
+            w.inertia[b] = inertia;
+            w.invInertia[b] = inertia > 0.0f ? 1.0f / inertia : 0.0f;
+            break;
+        }
+        default: break;
+        }
+    }
+    w.commands.clear();
+}
+
 // ---------------------------------------------------------------- step
 // Simulation.cpp:640-655: poses (and AABBs) of dynamic colliders after the last step.
 void postUpdate(World& w) {

@@ -86,6 +86,7 @@ class Tuning(C.Structure):
 
 
 LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED, LEARN_NEW_CONTACTS = 0, 1, 2, 4, 8
+SET_POSITION, SET_ROTATION, SET_VELOCITY, SET_ANGULAR_VELOCITY, SET_MASS, SET_INERTIA = range(6)
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS, MLP_BATCH_MAX = 24, 128, 2, 19970, 8192
 
 
@@ -127,6 +128,9 @@ _SIGS = {
     "agx_world_read_collider_f32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
     "agx_world_read_collider_u32": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32]),
     "agx_world_read_pairs": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_queue_body": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
+    "agx_world_flush": (C.c_int, [_P]),
+    "agx_world_narrowphase": (C.c_int, [_P, _P, _P, C.c_uint64, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_read_broadphase_order": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_write_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64]),
@@ -350,6 +354,22 @@ class World:
         out = np.empty(self.counts.bodies, np.float32)
         _check(lib().agx_world_read_body_f32(self._h, BODY_FIELDS.index(field), _ptr(out), len(out)), self._h)
         return out
+
+    def queue_body(self, kind: int, body: int, a: float, b: float = 0.0):
+        """Deferred modification (SET_*), applied at the next step or flush."""
+        _check(lib().agx_world_queue_body(self._h, kind, body, a, b), self._h)
+
+    def flush(self):
+        _check(lib().agx_world_flush(self._h), self._h)
+
+    def narrowphase(self, a, b) -> np.ndarray:
+        """Manifolds of the given collider pairs at the current poses (warm-started)."""
+        a = np.ascontiguousarray(a, np.uint32)
+        b = np.ascontiguousarray(b, np.uint32)
+        out = np.empty(max(len(a), 1), MANIFOLD_DTYPE)
+        cnt = C.c_uint64()
+        _check(lib().agx_world_narrowphase(self._h, _ptr(a), _ptr(b), len(a), _ptr(out), len(out), C.byref(cnt)), self._h)
+        return out[:cnt.value]
 
     def set_body(self, field: str, values):
         v = np.ascontiguousarray(values, np.float32)

@@ -56,6 +56,11 @@ struct agx_world {
     bool histReady = false;          // the previous step recorded its manifolds' starting accumulators
     bool learnedNarrow = false;      // this step's narrow phase ran with a network set
     int colourPerCuMax = 1, solvePerCuMax = 1;   // launch-shape limits (occupancy, allocations)
+    // deferred body modifications (agx_world_queue_body), applied in order at the next flush
+    struct BodyCommand { uint32_t kind, body; float a, b; };
+    std::vector<BodyCommand> commands;
+    bool staticDirty = false;        // a static body moved: static poses, AABBs and tree are rebuilt
+    bool poseEdited = false;         // a body moved or turned since the last pose pass
     // step graphs (agx_world_step_graph): one per ping-pong parity of the manifold buffers
     hipStream_t captureStream = nullptr;
     struct StepGraph {
@@ -220,6 +225,83 @@ int beginStep(agx_world* w) {
     return AGX_OK;
 }
 
+float wrapAngle(float angle) {   // ObjectManager.cpp:614-619
+    const float twoPi = (float)(3.14159265358979323846 * 2.0);
+    float wrapped = std::fmod(angle, twoPi);
+    if (wrapped < 0.0f) wrapped += twoPi;
+    return wrapped;
+}
+
+// ObjectManager::flushCommands / applyCommand (SimulationImpl/ObjectManager.cpp:638-865) for the
+// body commands, in queue order, on host copies of the body arrays. Collider poses are not
+// refreshed here: as in the reference, the next step's CCD snapshot still holds the pose before
+// the edit, and the step's pose pass computes the new one (static bodies: a full rebuild).
+int flushCommands(agx_world* w) {
+    if (w->commands.empty()) return AGX_OK;
+    agx::DeviceWorld& d = w->d;
+    const uint32_t n = d.N;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    std::vector<float4> pos(n), vel(n);
+    std::vector<float2> rcs(n), inv(n);
+    HIP_CHECK(w, hipMemcpy(pos.data(), d.pos, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(vel.data(), d.vel, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(rcs.data(), d.rcs, n * sizeof(float2), hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(inv.data(), d.inv, n * sizeof(float2), hipMemcpyDeviceToHost));
+    for (const agx_world::BodyCommand& c : w->commands) {
+        const uint32_t b = c.body;
+        const bool isStatic = inv[b].x == 0.0f;
+        switch (c.kind) {
+        case AGX_SET_POSITION: {
+            const float r = pos[b].z;
+            const float cosR = std::cos(r), sinR = std::sin(r);
+            const float lx = w->localComX[b], ly = w->localComY[b];
+            pos[b].x = c.a + cosR * lx - sinR * ly;
+            pos[b].y = c.b + sinR * lx + cosR * ly;
+            if (isStatic) w->staticDirty = true;
+            w->poseEdited = true;
+            break;
+        }
+        case AGX_SET_ROTATION: {
+            const float r = wrapAngle(c.a);
+            pos[b].z = r;
+            rcs[b] = make_float2(std::cos(r), std::sin(r));
+            if (isStatic) w->staticDirty = true;
+            w->poseEdited = true;
+            break;
+        }
+        case AGX_SET_VELOCITY:
+            vel[b].x = c.a;
+            vel[b].y = c.b;
+            break;
+        case AGX_SET_ANGULAR_VELOCITY:
+            vel[b].z = c.a;
+            break;
+        case AGX_SET_MASS: {   // applyBodyMass (:867-905) within the dynamic group
+            const float newMass = std::fmax(0.0f, c.a), oldMass = w->mass[b];
+            float inertia = w->inertia[b];
+            if (oldMass > 0.0f) inertia *= newMass / oldMass;
+            w->mass[b] = newMass;
+            w->inertia[b] = inertia;
+            inv[b] = make_float2(1.0f / newMass, inertia > 0.0f ? 1.0f / inertia : 0.0f);
+            break;
+        }
+        case AGX_SET_INERTIA: {
+            if (isStatic) break;
+            const float inertia = std::fmax(0.0f, c.a);
+            w->inertia[b] = inertia;
+            inv[b].y = inertia > 0.0f ? 1.0f / inertia : 0.0f;
+            break;
+        }
+        }
+    }
+    w->commands.clear();
+    HIP_CHECK(w, hipMemcpy(d.pos, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(w, hipMemcpy(d.vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(w, hipMemcpy(d.rcs, rcs.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    HIP_CHECK(w, hipMemcpy(d.inv, inv.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    return AGX_OK;
+}
+
 int runPhase(agx_world* w, uint32_t phase, float dt) {
     agx::DeviceWorld& d = w->d;
     const agx_settings& s = w->settings;
@@ -236,7 +318,16 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
     }
     case AGX_PHASE_POSE: {
         SegmentTimer t(w, SEG_POSE);
-        HIP_CHECK(w, agx::launchPose(st, d, 0, w->postUpdatePending ? 1 : 0, ccd ? 1 : 0));
+        // the CCD sweep starts from the colliders' last computed poses (snapshotPoses,
+        // ContinuousCollisionDetector.cpp:276-299); after a post-update pose pass they follow from the
+        // step's body snapshot, unless a flushed command moved a body after that pass
+        HIP_CHECK(w, agx::launchPose(st, d, 0, (w->postUpdatePending && !w->poseEdited) ? 1 : 0, ccd ? 1 : 0));
+        w->poseEdited = false;
+        if (w->staticDirty) {   // rebuildSpatialData after an edit of a static body (Simulation.cpp:295)
+            HIP_CHECK(w, agx::launchPose(st, d, 1, 0, 0));
+            HIP_CHECK(w, agx::launchBuildTree(st, d, d.stat));
+            w->staticDirty = false;
+        }
         break;
     }
     case AGX_PHASE_CCD:
@@ -745,6 +836,8 @@ int agx_world_run_phase(agx_world* w, uint32_t phase, float dt) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
     if (phase >= AGX_PHASE_COUNT) return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown phase %u", phase);
     if ((int)phase != w->nextPhase) return fail(w, AGX_ERR_STATE, "phase %u out of order (expected %d)", phase, w->nextPhase);
+    if (phase == AGX_PHASE_INTEGRATE)
+        if (int r = flushCommands(w)) return r;
     return runPhase(w, phase, dt);
 }
 
@@ -758,6 +851,7 @@ int agx_world_step(agx_world* w, float dt, uint32_t n_steps) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
     if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
     if (!(dt > 0.0f)) return AGX_OK;
+    if (int r = flushCommands(w)) return r;
     for (uint32_t i = 0; i < n_steps; i++)
         for (uint32_t p = 0; p < AGX_PHASE_COUNT; p++)
             if (int r = runPhase(w, p, dt)) return r;
@@ -981,6 +1075,7 @@ int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
     if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
     if (!(dt > 0.0f) || n_steps == 0) return AGX_OK;
+    if (int r = flushCommands(w)) return r;
     if (w->profiling) return agx_world_step(w, dt, n_steps);
     for (uint32_t p = 0; p < AGX_PHASE_COUNT; p++)
         if (int r = runPhase(w, p, dt)) return r;
@@ -1007,6 +1102,8 @@ int agx_world_update(agx_world* w, float dt, uint32_t* out_steps) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
     if (out_steps) *out_steps = 0;
     if (dt <= 0.0f) return AGX_OK;
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (int r = flushCommands(w)) return r;   // Simulation::update flushes before anything reads the data
     const agx_settings& s = w->settings;
     w->accumulator += std::fmin(dt, s.max_delta_per_update);
     const uint32_t steps = (uint32_t)std::floor(w->accumulator / s.update_interval);
@@ -1015,6 +1112,58 @@ int agx_world_update(agx_world* w, float dt, uint32_t* out_steps) {
     w->lastStepCount = steps;
     if (out_steps) *out_steps = steps;
     return agx_world_step(w, s.update_interval * s.time_scale, steps);
+}
+
+int agx_world_queue_body(agx_world* w, uint32_t kind, uint32_t body, float a, float b) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (body >= w->d.N) return fail(w, AGX_ERR_INVALID_ARGUMENT, "body %u out of range (%u bodies)", body, w->d.N);
+    if (kind > AGX_SET_INERTIA) return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown body command %u", kind);
+    if (kind == AGX_SET_MASS) {
+        const bool dynamic = w->mass[body] > 0.0f;   // static bodies have mass 0
+        if (!(a > 0.0f) || !dynamic)
+            return fail(w, AGX_ERR_INVALID_ARGUMENT, "mass changes keep a body dynamic (mass > 0 on a dynamic body)");
+    }
+    w->commands.push_back({ kind, body, a, b });
+    return AGX_OK;
+}
+
+int agx_world_narrowphase(agx_world* w, const uint32_t* pairs_a, const uint32_t* pairs_b, uint64_t n, agx_manifold* out,
+                          uint64_t cap, uint64_t* count) {
+    if (!w || !count || (n && (!pairs_a || !pairs_b))) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    agx::DeviceWorld& d = w->d;
+    *count = 0;
+    if (n > d.pairCap) return fail(w, AGX_ERR_CAPACITY, "%llu pairs exceed the pair capacity %llu", (unsigned long long)n,
+                                   (unsigned long long)d.pairCap);
+    std::vector<uint2> p(n);
+    for (uint64_t i = 0; i < n; i++) {
+        if (pairs_a[i] >= d.C || pairs_b[i] >= d.C || pairs_a[i] == pairs_b[i])
+            return fail(w, AGX_ERR_INVALID_ARGUMENT, "pair %llu (%u, %u) is not two colliders of the world",
+                        (unsigned long long)i, pairs_a[i], pairs_b[i]);
+        p[i] = make_uint2(pairs_a[i], pairs_b[i]);
+    }
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    if (n) HIP_CHECK(w, hipMemcpy(d.pairs, p.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
+    const uint32_t n32 = (uint32_t)n;
+    HIP_CHECK(w, agx::launchFillU32(w->stream, &d.ctr->pairs, n32, 1));
+    d.histOut = nullptr;
+    d.histIn = nullptr;
+    HIP_CHECK(w, agx::launchNarrow(w->stream, d));
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    uint32_t m = 0, overflow = 0;
+    HIP_CHECK(w, hipMemcpy(&m, &d.ctr->manifolds, sizeof m, hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(&overflow, &d.ctr->overflow, sizeof overflow, hipMemcpyDeviceToHost));
+    *count = m;
+    if (overflow & AGX_OVERFLOW_HASH) return fail(w, AGX_ERR_CAPACITY, "warm-start lookup overflowed");
+    if (m > cap) return fail(w, AGX_ERR_CAPACITY, "%u manifolds exceed cap %llu", m, (unsigned long long)cap);
+    if (m && out) HIP_CHECK(w, hipMemcpy(out, d.man[d.cur], m * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
+int agx_world_flush(agx_world* w) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    return flushCommands(w);
 }
 
 int agx_world_sync(agx_world* w) {

@@ -35,6 +35,7 @@ _SIGS = {
     "orc_step": (None, [_P, C.c_float]), "orc_post_update": (None, [_P]),
     "orc_update": (None, [_P, C.c_float, C.c_uint32]),
     "orc_get_manifolds": (C.c_int64, [_P, _P, C.c_uint64]), "orc_errors": (None, [_P, _P]),
+    "orc_queue_body": (None, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float]), "orc_flush": (None, [_P]),
     "orc_pool_new": (_P, [C.c_uint32]), "orc_pool_free": (None, [_P]), "orc_pool_size": (C.c_uint32, [_P]),
     "orc_update_pool": (None, [_P, _P, C.c_float, C.c_uint32]),
     "orc_plan_waves": (C.c_int64, [_P, _P, C.c_uint64, C.c_uint32, _P, _P, _P, C.c_uint64]),
@@ -153,6 +154,8 @@ class World:
     def step(self, dt): lib().orc_step(self.h, dt)
     def post_update(self): lib().orc_post_update(self.h)
     def update(self, dt, steps=1): lib().orc_update(self.h, dt, steps)
+    def queue_body(self, kind, body, a, b=0.0): lib().orc_queue_body(self.h, kind, body, a, b)
+    def flush(self): lib().orc_flush(self.h)
     def update_threaded(self, pool, dt, steps=1): lib().orc_update_pool(self.h, pool.h, dt, steps)
 
     def plan_waves(self, m, workers):

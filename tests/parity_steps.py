@@ -13,12 +13,13 @@ BODY_MAP = (("com_x", "comX"), ("com_y", "comY"), ("rotation", "rot"), ("vel_x",
 COLLIDER_MAP = (("world_x", "wX"), ("world_y", "wY"), ("cos", "wCos"), ("sin", "wSin"), ("world_rot", "wRot"),
                 ("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY"))
 SECOND_POINT = ("p1x", "p1y", "id1", "jn1", "jt1")
+MASS_MAP = (("mass", "mass"), ("inv_mass", "invMass"), ("inertia", "inertia"), ("inv_inertia", "invInertia"))
 
 
 def oracle_from_gpu(oracle, scene, w, settings=None):
     """Oracle world with the GPU world's current state (bodies, collider poses, warm-start source)."""
     ow = oracle.world_from_scene(scene, settings)
-    for g, o in BODY_MAP:
+    for g, o in BODY_MAP + MASS_MAP:
         ow.set(o, w.body(g))
     for g, o in COLLIDER_MAP:
         ow.set(o, w.collider(g))

@@ -1,0 +1,128 @@
+"""The drop-in boundary beyond stepping: deferred body modifications (ObjectAPI::setBody*,
+SimulationAPIs/ObjectAPI.h:76-81, applied by ObjectManager::flushCommands,
+SimulationImpl/ObjectManager.cpp:638-865) against the oracle's flush, bit for bit, followed by a
+step checked phase by phase; an edited static body's collider, AABB and static tree are rebuilt in
+the next step; refused commands leave the world unchanged."""
+import numpy as np
+import pytest
+
+from parity_steps import BODY_MAP, MASS_MAP, check_step, oracle_from_gpu
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 60.0
+
+
+def make_world(agx, scene, **settings):
+    b = agx.Builder(1)
+    agx.build_scene(b, scene)
+    return agx.World(b, agx.default_settings(**{**scene.get("settings", {}), **settings}))
+
+
+def _commands(agx, w):
+    dyn = np.flatnonzero(w.body("inv_mass") != 0)
+    b = [int(x) for x in dyn[[3, 50, 400, 901, 1500, 2000]]]
+    return [
+        (agx.SET_POSITION, b[0], 5.0, 40.0),           # teleport a body above the pile
+        (agx.SET_ROTATION, b[0], 7.5, 0.0),            # wrapped to [0, 2 pi)
+        (agx.SET_VELOCITY, b[1], 2.5, -1.0),
+        (agx.SET_ANGULAR_VELOCITY, b[2], 3.0, 0.0),
+        (agx.SET_MASS, b[3], 4.0, 0.0),                # inertia scales with the mass
+        (agx.SET_INERTIA, b[4], 0.75, 0.0),
+        (agx.SET_VELOCITY, b[5], 0.0, 6.0),
+        (agx.SET_VELOCITY, b[5], 1.0, 3.0),            # the later command wins
+    ]
+
+
+def test_deferred_body_commands_match_oracle_flush_and_step(agx, oracle, scenes):
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene, track_errors=1)
+    w.step(DT, 120)
+    ow = oracle_from_gpu(oracle, scene, w, dict(v=8, p=3))
+    for kind, body, a, b in _commands(agx, w):
+        w.queue_body(kind, body, a, b)
+        ow.queue_body(kind, body, a, b)
+    before = w.body("com_x").copy()
+    w.flush()
+    ow.flush()
+    for g, o in BODY_MAP + MASS_MAP:
+        assert np.array_equal(w.body(g), ow.get(o)), g
+    assert not np.array_equal(w.body("com_x"), before)
+    rot = w.body("rotation")
+    assert 0.0 <= rot.min() and rot.max() < 2 * np.pi
+    # the next step, phase by phase, from the edited state
+    out = check_step(agx, oracle, scene, w, DT, 8, 3, min_manifolds=300)
+    assert out["pairs"] > 300
+
+
+def test_static_body_edit_rebuilds_static_collider_and_tree(agx, oracle, scenes):
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene)
+    w.step(DT, 30)
+    statics = np.flatnonzero(w.body("inv_mass") == 0)
+    assert len(statics) == 4
+    wall = int(statics[0])
+    ow = oracle_from_gpu(oracle, scene, w, dict(v=8, p=3))
+    w.queue_body(agx.SET_POSITION, wall, 1.5, 3.0)
+    ow.queue_body(agx.SET_POSITION, wall, 1.5, 3.0)
+    w.flush()
+    ow.flush()
+    for p in range(agx.PHASE_BROAD + 1):
+        w.run_phase(p, DT)
+    ow.snapshot()
+    ow.integrate(DT)
+    ow.wrap_cossin()
+    ow.poses(True)
+    ow.aabbs(True)
+    for g, o in (("world_x", "wX"), ("world_y", "wY"), ("aabb_min_x", "minX"), ("aabb_min_y", "minY"),
+                 ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY")):
+        assert np.array_equal(w.collider(g), ow.get(o)), g
+    ga, gb = w.pairs()
+    for f, o in (("aabb_min_x", "minX"), ("aabb_min_y", "minY"), ("aabb_max_x", "maxX"), ("aabb_max_y", "maxY")):
+        ow.set(o, w.collider(f))
+    oa, ob = ow.broad_brute()
+    from parity_steps import pair_keys
+    ns = w.counts.static_colliders
+    assert np.array_equal(pair_keys(ga, gb, ns), pair_keys(oa, ob, ns))
+    for p in range(agx.PHASE_NARROW, agx.PHASE_FINISH + 1):
+        w.run_phase(p, DT)
+    assert w.stats().overflow == 0
+
+
+def test_refused_body_commands(agx, scenes):
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene)
+    static = int(np.flatnonzero(w.body("inv_mass") == 0)[0])
+    dyn = int(np.flatnonzero(w.body("inv_mass") != 0)[0])
+    for kind, body, a in ((agx.SET_MASS, static, 2.0), (agx.SET_MASS, dyn, 0.0), (agx.SET_MASS, dyn, -1.0),
+                          (99, dyn, 0.0), (agx.SET_VELOCITY, w.counts.bodies, 0.0)):
+        with pytest.raises(Exception):
+            w.queue_body(kind, body, a, 0.0)
+    w.step(DT, 1)
+    assert w.stats().overflow == 0
+
+
+def test_standalone_narrowphase_on_injected_pairs(agx, oracle, scenes):
+    """agx_world_narrowphase on the last step's pairs at the post-update poses equals the oracle's
+    narrow phase plus warm start on the same poses, field by field; the next step is unchanged."""
+    from parity_steps import assert_manifolds_equal
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene)
+    twin = make_world(agx, scene)
+    w.step(DT, 120)
+    twin.step(DT, 120)
+    ga, gb = w.pairs()
+    mg = w.narrowphase(ga, gb)
+    ow = oracle_from_gpu(oracle, scene, w, dict(v=8, p=3))
+    mo = ow.warm_start(ow.narrow(ga, gb))
+    assert len(mg) > 300
+    assert_manifolds_equal(mg, mo, "standalone narrow")
+    assert np.count_nonzero(mg["jn0"]) > len(mg) // 2
+    # reversed pair order gives the same manifolds (canonical A/B), in the new order
+    mr = w.narrowphase(gb[::-1], ga[::-1])
+    assert_manifolds_equal(mr, mo, "reversed pairs")
+    w.step(DT, 1)
+    twin.step(DT, 1)
+    for f in ("com_x", "com_y", "vel_x", "vel_y", "ang_vel", "rotation"):
+        assert np.array_equal(w.body(f), twin.body(f)), f
+    with pytest.raises(Exception):
+        w.narrowphase(np.array([0], np.uint32), np.array([0], np.uint32))
