@@ -104,6 +104,7 @@ struct Counters {
     unsigned long long bodySweeps;   // this step: dynamic bodies x the sweeps of their scene
     unsigned long long convergedBodies;   // this step: dynamic bodies of scenes that met the residual target
     uint32_t convergedScenes;  // this step: scenes that met the residual target
+    uint32_t ccdCandidates;    // this step: fast dynamic colliders the CCD casts
     uint32_t flowMet;          // this step: the dataflow solve met the residual target
 };
 
@@ -222,6 +223,7 @@ struct DeviceWorld {
     uint32_t* ufParent = nullptr;
     uint32_t* ufRoot = nullptr;      // scan scratch (N)
     float* hitTime = nullptr;        // CCD, per body, 1 between steps
+    uint32_t* ccdFast = nullptr;     // CCD: the step's fast dynamic colliders (nDyn entries)
     // colliders
     uint32_t* colBody = nullptr;
     uint32_t* colScene = nullptr;

@@ -220,77 +220,128 @@ __host__ __device__ inline float sweepCore(uint4 sh, float2 dims, float4 ps, con
                         [=](uint32_t i) { return nr[i]; });
 }
 
-// CCD/ContinuousCollisionDetector.cpp:307-440. Queries run in the previous step's Morton order
-// of dynamic colliders (spatially coherent waves); each lane walks the static LBVH with its
-// own stack, visiting nodes whose AABB touches the swept core AABB and casting against
-// static colliders whose AABB strictly overlaps it. Earliest hit per body by atomic min.
-__global__ __launch_bounds__(256) void kCcd(const uint32_t* __restrict__ order, uint32_t n,
-                                            const uint32_t* __restrict__ colBody, const uint32_t* __restrict__ colScene,
-                                            const float4* __restrict__ pos, const float4* __restrict__ prev,
-                                            const float2* __restrict__ prevW, const float4* __restrict__ pose,
-                                            const float2* __restrict__ extent, const uint4* __restrict__ shape,
-                                            const float2* __restrict__ dims, const float2* __restrict__ verts,
-                                            const float2* __restrict__ normals, Tree st, float fastFactor,
-                                            float coreFraction, float* __restrict__ hitTime, Counters* ctr) {
-    __shared__ uint32_t stackMem[256 * 48];
-    uint32_t* stack = stackMem + threadIdx.x * 48;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || st.n == 0) return;
-    const uint32_t c = order[k];
+// The fast-mover test of CCD/ContinuousCollisionDetector.cpp:343-369: the body's origin and the
+// collider's origin both moved by more than fastFactor x the collider's smallest extent.
+__device__ inline bool ccdFast(uint32_t c, const uint32_t* __restrict__ colBody, const float4* __restrict__ pos,
+                               const float4* __restrict__ prev, const float2* __restrict__ prevW,
+                               const float4* __restrict__ pose, const float2* __restrict__ extent, float fastFactor) {
     const uint32_t body = colBody[c];
     const float4 p = pos[body];
     const float4 q = prev[body];
     const float bdx = p.x - q.x, bdy = p.y - q.y;
     const float bd2 = bdx * bdx + bdy * bdy;
-    if (bd2 <= 1e-8f) return;
+    if (bd2 <= 1e-8f) return false;
     const float minExtent = extent[c].x;
-    if (minExtent <= 0.0f) return;
+    if (minExtent <= 0.0f) return false;
     const float threshold = fastFactor * minExtent;
     const float thresholdSq = threshold * threshold;
-    if (bd2 <= thresholdSq) return;
+    if (bd2 <= thresholdSq) return false;
     const float2 s = prevW[c];
     const float4 e = pose[c];
     const float dx = e.x - s.x, dy = e.y - s.y;
-    const float dd = dx * dx + dy * dy;
-    if (dd <= thresholdSq) return;
-    const float core = coreFraction * minExtent;
-    const float sMinX = fminf(s.x, e.x) - core, sMaxX = fmaxf(s.x, e.x) + core;
-    const float sMinY = fminf(s.y, e.y) - core, sMaxY = fmaxf(s.y, e.y) + core;
-    const uint32_t scene = colScene[c];
-    const uint32_t sf = st.sceneStart[scene], sl = st.sceneStart[scene + 1];
-    if (sf == sl) return;
-    const uint32_t leafBase = st.n - 1;
-    float best = 1.0f;
-    uint32_t sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-        const uint32_t node = stack[--sp];
-        const float4 nb = st.nodeAabb[node];
-        if (node >= leafBase) {
-            const uint32_t j = node - leafBase;
-            if (j < sf || j >= sl) continue;
-            if (!(sMinX < nb.z && sMaxX > nb.x && sMinY < nb.w && sMaxY > nb.y)) continue;
-            const uint32_t sc = st.vals[j];
-            best = fminf(best, sweepCore(shape[sc], dims[sc], pose[sc], verts, normals, s.x, s.y, dx, dy, dd, core, best));
-            continue;
+    return dx * dx + dy * dy > thresholdSq;
+}
+
+// The fast dynamic colliders, in the previous step's Morton order of dynamic colliders within each
+// block of 1,024 (spatially coherent), appended block by block to `fast`; ctr->ccdCandidates
+// counts them. One counter update per block: updates of one address from every wave queue there.
+constexpr uint32_t CCD_SELECT_THREADS = 1024;
+__global__ __launch_bounds__(CCD_SELECT_THREADS) void kCcdSelect(
+    const uint32_t* __restrict__ order, uint32_t n, const uint32_t* __restrict__ colBody,
+    const float4* __restrict__ pos, const float4* __restrict__ prev, const float2* __restrict__ prevW,
+    const float4* __restrict__ pose, const float2* __restrict__ extent, float fastFactor,
+    uint32_t* __restrict__ fast, Counters* ctr) {
+    constexpr uint32_t WAVES = CCD_SELECT_THREADS / WAVE;
+    __shared__ uint32_t waveCount[WAVES];
+    __shared__ uint32_t blockBase;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = k < n ? order[k] : 0u;
+    const bool isFast = k < n && ccdFast(c, colBody, pos, prev, prevW, pose, extent, fastFactor);
+    const uint64_t mask = __ballot(isFast);
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+    if (lane == 0) waveCount[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t i = 0; i < WAVES; i++) {
+            const uint32_t t = waveCount[i];
+            waveCount[i] = total;
+            total += t;
         }
-        const uint2 rg = st.range[node];
-        if (rg.y < sf || rg.x >= sl) continue;
-        if (nb.z < sMinX || nb.x > sMaxX || nb.w < sMinY || nb.y > sMaxY) continue;
-        if (sp + 2 > 48) { atomicOr(&ctr->overflow, 2u); break; }
-        const uint2 ch = st.child[node];
-        stack[sp++] = ch.y;
-        stack[sp++] = ch.x;
+        blockBase = total ? atomicAdd(&ctr->ccdCandidates, total) : 0u;
     }
-    bool first = false;   // this cast gave the body its first hit of the step
-    if (best < 1.0f) {
-        const uint32_t old = atomicMin(reinterpret_cast<uint32_t*>(hitTime) + body, __float_as_uint(best));
-        first = __uint_as_float(old) >= 1.0f;
+    __syncthreads();
+    if (isFast) fast[blockBase + waveCount[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = c;
+}
+
+// CCD/ContinuousCollisionDetector.cpp:307-440 for the fast colliders only: each lane walks the
+// static LBVH with its own stack, visiting nodes whose AABB touches the swept core AABB and
+// casting against static colliders whose AABB strictly overlaps it. Earliest hit per body by
+// atomic min (independent of the order the casts run in). One wave per block, so that the few
+// thousand packed waves spread over every CU.
+constexpr uint32_t CCD_THREADS = 64;
+__global__ __launch_bounds__(CCD_THREADS) void kCcd(const uint32_t* __restrict__ fast, const uint32_t* __restrict__ colBody,
+                                            const uint32_t* __restrict__ colScene,
+                                            const float2* __restrict__ prevW, const float4* __restrict__ pose,
+                                            const float2* __restrict__ extent, const uint4* __restrict__ shape,
+                                            const float2* __restrict__ dims, const float2* __restrict__ verts,
+                                            const float2* __restrict__ normals, Tree st, float coreFraction,
+                                            float* __restrict__ hitTime, Counters* ctr) {
+    __shared__ uint32_t stackMem[CCD_THREADS * 48];
+    uint32_t* stack = stackMem + threadIdx.x * 48;
+    const uint32_t n = ctr->ccdCandidates;
+    if (st.n == 0) return;
+    const uint32_t leafBase = st.n - 1;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t k = base + threadIdx.x;
+        float best = 1.0f;
+        uint32_t body = 0;
+        if (k < n) {
+            const uint32_t c = fast[k];
+            body = colBody[c];
+            const float minExtent = extent[c].x;
+            const float2 s = prevW[c];
+            const float4 e = pose[c];
+            const float dx = e.x - s.x, dy = e.y - s.y;
+            const float dd = dx * dx + dy * dy;
+            const float core = coreFraction * minExtent;
+            const float sMinX = fminf(s.x, e.x) - core, sMaxX = fmaxf(s.x, e.x) + core;
+            const float sMinY = fminf(s.y, e.y) - core, sMaxY = fmaxf(s.y, e.y) + core;
+            const uint32_t scene = colScene[c];
+            const uint32_t sf = st.sceneStart[scene], sl = st.sceneStart[scene + 1];
+            uint32_t sp = 0;
+            if (sf != sl) stack[sp++] = 0;
+            while (sp) {
+                const uint32_t node = stack[--sp];
+                const float4 nb = st.nodeAabb[node];
+                if (node >= leafBase) {
+                    const uint32_t j = node - leafBase;
+                    if (j < sf || j >= sl) continue;
+                    if (!(sMinX < nb.z && sMaxX > nb.x && sMinY < nb.w && sMaxY > nb.y)) continue;
+                    const uint32_t sc = st.vals[j];
+                    best = fminf(best, sweepCore(shape[sc], dims[sc], pose[sc], verts, normals, s.x, s.y, dx, dy, dd, core,
+                                                 best));
+                    continue;
+                }
+                const uint2 rg = st.range[node];
+                if (rg.y < sf || rg.x >= sl) continue;
+                if (nb.z < sMinX || nb.x > sMaxX || nb.w < sMinY || nb.y > sMaxY) continue;
+                if (sp + 2 > 48) { atomicOr(&ctr->overflow, 2u); break; }
+                const uint2 ch = st.child[node];
+                stack[sp++] = ch.y;
+                stack[sp++] = ch.x;
+            }
+        }
+        bool first = false;   // this cast gave the body its first hit of the step
+        if (best < 1.0f) {
+            const uint32_t old = atomicMin(reinterpret_cast<uint32_t*>(hitTime) + body, __float_as_uint(best));
+            first = __uint_as_float(old) >= 1.0f;
+        }
+        // one update of the hit counter per wave (updates of one address from every lane queue there)
+        const uint64_t hits = __ballot(first);
+        if (first && (threadIdx.x & (WAVE - 1)) == (uint32_t)(__ffsll((unsigned long long)hits) - 1))
+            atomicAdd(&ctr->ccdHits, (uint32_t)__popcll(hits));
     }
-    // one update of the hit counter per wave (updates of one address from every lane queue there)
-    const uint64_t hits = __ballot(first);
-    if (first && (threadIdx.x & (WAVE - 1)) == (uint32_t)(__ffsll((unsigned long long)hits) - 1))
-        atomicAdd(&ctr->ccdHits, (uint32_t)__popcll(hits));
 }
 
 // CCD/ContinuousCollisionDetector.cpp:513-529: move hit colliders back along their body's
@@ -375,9 +426,15 @@ hipError_t launchPose(hipStream_t st, DeviceWorld& w, int all, int prevFromSnaps
 hipError_t launchCcd(hipStream_t st, DeviceWorld& w, float fastFactor, float coreFraction) {
     if (w.nStat == 0 || w.nDyn == 0) return hipSuccess;
     // dyn.vals holds the last Morton order of the dynamic colliders (built at creation and every step).
-    hipLaunchKernelGGL(kCcd, dim3((w.nDyn + 255) / 256), dim3(256), 0, st, w.dyn.vals, w.nDyn, w.colBody, w.colScene,
-                       w.pos, w.prev, w.prevW, w.pose, w.extent, w.shape, w.dims, w.verts, w.normals, w.stat,
-                       fastFactor, coreFraction, w.hitTime, w.ctr);
+    hipLaunchKernelGGL(kCcdSelect, dim3((w.nDyn + CCD_SELECT_THREADS - 1) / CCD_SELECT_THREADS), dim3(CCD_SELECT_THREADS),
+                       0, st, w.dyn.vals, w.nDyn, w.colBody, w.pos, w.prev, w.prevW, w.pose, w.extent, fastFactor,
+                       w.ccdFast, w.ctr);
+    // the candidate count is known only on the device: enough one-wave blocks for 64 Ki
+    // candidates at once, more in grid-stride turns
+    hipLaunchKernelGGL(kCcd, dim3(std::min((w.nDyn + CCD_THREADS - 1) / CCD_THREADS, 1024u)), dim3(CCD_THREADS), 0, st,
+                       w.ccdFast, w.colBody, w.colScene,
+                       w.prevW, w.pose, w.extent, w.shape, w.dims, w.verts, w.normals, w.stat, coreFraction, w.hitTime,
+                       w.ctr);
     hipLaunchKernelGGL(kCcdShiftColliders, dim3(gridFor(w.nDyn)), dim3(256), 0, st, w.dyn.list, w.nDyn, w.colBody,
                        w.pos, w.prev, w.hitTime, w.shape, w.dims, w.verts, w.pose, w.aabb);
     hipLaunchKernelGGL(kCcdShiftBodies, dim3(gridFor(w.nDynBodies)), dim3(256), 0, st, w.dynBodies, w.nDynBodies,

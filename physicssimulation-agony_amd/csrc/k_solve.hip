@@ -1535,6 +1535,38 @@ __global__ __launch_bounds__(256) void kUfLinkNeighbours(const agx_manifold* __r
     }
 }
 
+// Every dynamic body re-pointed at the root of its tree after the first links (read-only walks,
+// then one store each: links still point downwards into the same set), so the unions that follow
+// start from trees of depth one.
+__global__ __launch_bounds__(256) void kUfCompress(const uint32_t* __restrict__ list, uint32_t n, uint32_t* parent) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t b = list[k];
+        uint32_t x = b, p = parent[x];
+        while (p != x) { x = p; p = parent[x]; }
+        if (x != b) parent[b] = x;
+    }
+}
+
+// A merging pass without retries: each contact edge whose two trees differ hooks the larger root
+// under the smaller with an atomic min. Two edges that hook one root keep only the smaller target,
+// so some unions may be left out here; every link still joins bodies of one set, and the
+// compare-and-swap pass that follows unites over every edge. Most of its edges then find one
+// root at once, instead of contending for the same few roots of large islands.
+__global__ __launch_bounds__(256) void kUfHook(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
+                                               const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
+                                               uint32_t* parent) {
+    const uint32_t M = ctr->manifolds;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        uint32_t a = colBody[man[m].a], b = colBody[man[m].b];
+        if (!isDynamic(inv, a) || !isDynamic(inv, b)) continue;
+        uint32_t p = parent[a];
+        while (p != a) { a = p; p = parent[a]; }
+        p = parent[b];
+        while (p != b) { b = p; p = parent[b]; }
+        if (a != b) atomicMin(parent + max(a, b), min(a, b));
+    }
+}
+
 __global__ __launch_bounds__(256) void kUfUnite(const agx_manifold* __restrict__ man, const Counters* __restrict__ ctr,
                                                 const uint32_t* __restrict__ colBody, const float2* __restrict__ inv,
                                                 uint32_t* parent) {
@@ -1939,6 +1971,14 @@ hipError_t launchIslands(hipStream_t st, DeviceWorld& w) {
     hipLaunchKernelGGL(kUfInit, dim3(blocksFor(w.N)), dim3(256), 0, st, w.ufParent, w.N);
     hipLaunchKernelGGL(kUfLinkNeighbours, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody,
                        w.inv, w.ufParent);
+    if (w.nDynBodies) {
+        hipLaunchKernelGGL(kUfCompress, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.ufParent);
+        // one hooking round (a second merges little more than it costs on a 65,540-body pile)
+        hipLaunchKernelGGL(kUfHook, dim3(blocksForManifolds(w)), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody, w.inv,
+                           w.ufParent);
+        hipLaunchKernelGGL(kUfCompress, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.dynBodies, w.nDynBodies,
+                           w.ufParent);
+    }
     hipLaunchKernelGGL(kUfUnite, dim3(blocksFor((uint32_t)std::min<uint64_t>(w.manCap, 1u << 22))), dim3(256), 0, st,
                        w.man[w.cur], w.ctr, w.colBody, w.inv, w.ufParent);
     if (w.nDynBodies == 0) return launchFillU32(st, &w.ctr->islands, 0u, 1);

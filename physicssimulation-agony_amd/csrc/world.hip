@@ -754,7 +754,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     UP(d.sceneBodyStart, sceneBodyStart); UP(d.sceneOrder, sceneOrder); UP(d.sceneDyn, sceneDyn);
     AL(d.sceneRange, (uint64_t)d.S * agx::MAX_COLOURS); AL(d.sceneRes, (uint64_t)d.S * agx::MAX_SWEEPS);
     AL(d.sceneSweeps, d.S); AL(d.sceneWorkOrder, d.S); AL(d.sceneIslands, d.S); AL(d.sceneIslandOffset, d.S); AL(d.sceneManRange, d.S);
-    AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N);
+    AL(d.islandId, d.N); AL(d.ufParent, d.N); AL(d.ufRoot, std::max(1u, d.nDynBodies)); AL(d.hitTime, d.N); AL(d.ccdFast, std::max(1u, d.nDyn));
     AL(d.pose, d.C); AL(d.wRot, d.C); AL(d.prevW, d.C); AL(d.aabb, d.C);
     if ((rc = allocTree(w, d.dyn, dynList, colScene, d.S))) return bail(rc);
     if ((rc = allocTree(w, d.stat, statList, colScene, d.S))) return bail(rc);

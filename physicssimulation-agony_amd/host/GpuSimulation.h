@@ -4,6 +4,8 @@
 // reference so that code written against PS_AGONY2D::Simulation ports by changing the namespace:
 //   Simulation::update / getSimulationSettings / createMaterial / objects()   (Simulation.h:148-203)
 //   ObjectAPI::createCircle / createBox / createPolygon (+ *H), getBodyData    (ObjectAPI.h:34-109)
+//   ObjectAPI::setBodyPosition / Rotation / Velocity / AngularVelocity / Mass / Inertia (deferred,
+//   ObjectAPI.h:76-81), getBodies / getColliders / getPolygons viewers as host copies (:96-101)
 //   BodyCreateParams, CircleCreateParams, BoxCreateParams, PolygonCreateParams (ObjectCreateParams.h:14-95)
 //   Material                                                                  (Types/Material.h:6-11)
 // Bad user input returns std::nullopt / false and writes a message to std::cerr, like the
@@ -68,6 +70,19 @@ struct BodyArrays {
     std::vector<float> comX, comY, rotation, velX, velY, angularVelocity, invMass;
 };
 
+// Host copies of the collider arrays (SoA/ColliderSoA.h:10-32), indexed by dense collider index.
+struct ColliderArrays {
+    std::vector<float> worldX, worldY, worldRotation, aabbMinX, aabbMinY, aabbMaxX, aabbMaxY;
+    std::vector<uint32_t> body, shapeType, material;
+};
+
+// Polygon outlines (SoA/Shapes/PolygonSoA.h:14-52): the local vertices and outward edge normals of
+// collider c are vertices[start[c] .. start[c] + count[c]) (count 0 for circles and boxes).
+struct PolygonArrays {
+    std::vector<Vec2> vertices, normals;
+    std::vector<uint32_t> start, count;
+};
+
 class Simulation;
 
 class ObjectAPI {
@@ -85,8 +100,23 @@ public:
 
     std::optional<BodyData> getBodyData(ObjectDenseIndex index) const;
     BodyArrays getBodies() const;
+    ColliderArrays getColliders() const;
+    PolygonArrays getPolygons() const;
+
+    // Deferred modifications, applied at the next update() (ObjectManager::flushCommands). A
+    // mass change must keep the body dynamic (mass > 0); false and a message otherwise.
+    bool setBodyPosition(Handle body, Vec2 position) { return queue(AGX_SET_POSITION, body, position.x, position.y); }
+    bool setBodyRotation(Handle body, Real rotation) { return queue(AGX_SET_ROTATION, body, rotation, 0); }
+    bool setBodyVelocity(Handle body, Vec2 velocity) { return queue(AGX_SET_VELOCITY, body, velocity.x, velocity.y); }
+    bool setBodyAngularVelocity(Handle body, Real w) { return queue(AGX_SET_ANGULAR_VELOCITY, body, w, 0); }
+    bool setBodyMass(Handle body, Real mass) { return queue(AGX_SET_MASS, body, mass, 0); }
+    bool setBodyInertia(Handle body, Real inertia) { return queue(AGX_SET_INERTIA, body, inertia, 0); }
 
 private:
+    friend class Simulation;
+    struct Command { uint32_t kind; Handle body; Real a, b; };
+    bool queue(uint32_t kind, Handle body, Real a, Real b);
+    std::vector<Command> pending;   // commands given before the scene is on the GPU
     std::optional<ObjectDenseIndex> add(uint8_t type, const BodyCreateParams& b, float p0, float p1,
                                         const std::vector<Vec2>* verts);
     Simulation& sim;
@@ -131,6 +161,9 @@ public:
             world = nullptr;
             return false;
         }
+        for (const ObjectAPI::Command& c : api.pending)   // commands given before the upload, in order
+            if (agx_world_queue_body(world, c.kind, c.body, c.a, c.b) != AGX_OK) fail("set", agx_world_last_error(world));
+        api.pending.clear();
         return true;
     }
 
@@ -243,6 +276,57 @@ inline BodyArrays ObjectAPI::getBodies() const {
     read(AGX_BODY_VEL_X, a.velX); read(AGX_BODY_VEL_Y, a.velY); read(AGX_BODY_ANG_VEL, a.angularVelocity);
     read(AGX_BODY_INV_MASS, a.invMass);
     return a;
+}
+
+inline bool ObjectAPI::queue(uint32_t kind, Handle body, Real a, Real b) {
+    if (!sim.world) {
+        const agx_counts c = sim.counts();
+        if (body >= c.bodies) { Simulation::fail("set", "no such body"); return false; }
+        if (kind == AGX_SET_MASS && !(a > 0)) { Simulation::fail("set", "mass changes keep a body dynamic"); return false; }
+        pending.push_back({ kind, body, a, b });
+        return true;
+    }
+    if (agx_world_queue_body(sim.world, kind, body, a, b) != AGX_OK) {
+        Simulation::fail("set", agx_world_last_error(sim.world));
+        return false;
+    }
+    return true;
+}
+
+inline ColliderArrays ObjectAPI::getColliders() const {
+    ColliderArrays a;
+    const uint32_t n = sim.counts().colliders;
+    auto readF = [&](uint32_t field, std::vector<float>& v) {
+        v.resize(n);
+        if (sim.world) agx_world_read_collider_f32(sim.world, field, v.data(), n);
+        else agx_builder_read_collider(sim.builder, 0, field, 0, v.data(), n);
+    };
+    auto readU = [&](uint32_t field, std::vector<uint32_t>& v) {
+        v.resize(n);
+        if (sim.world) agx_world_read_collider_u32(sim.world, field, v.data(), n);
+        else agx_builder_read_collider(sim.builder, 0, field, 1, v.data(), n);
+    };
+    readF(AGX_COL_WORLD_X, a.worldX); readF(AGX_COL_WORLD_Y, a.worldY); readF(AGX_COL_WORLD_ROT, a.worldRotation);
+    readF(AGX_COL_AABB_MIN_X, a.aabbMinX); readF(AGX_COL_AABB_MIN_Y, a.aabbMinY);
+    readF(AGX_COL_AABB_MAX_X, a.aabbMaxX); readF(AGX_COL_AABB_MAX_Y, a.aabbMaxY);
+    readU(AGX_COL_BODY, a.body); readU(AGX_COL_TYPE, a.shapeType); readU(AGX_COL_MATERIAL, a.material);
+    return a;
+}
+
+inline PolygonArrays ObjectAPI::getPolygons() const {
+    PolygonArrays p;
+    const agx_counts c = sim.counts();
+    std::vector<float> v(2 * (size_t)c.vertices), nrm(2 * (size_t)c.vertices);
+    if (c.vertices) agx_builder_read_vertices(sim.builder, 0, v.data(), nrm.data(), c.vertices);
+    for (uint32_t i = 0; i < c.vertices; i++) {
+        p.vertices.push_back({ v[2 * i], v[2 * i + 1] });
+        p.normals.push_back({ nrm[2 * i], nrm[2 * i + 1] });
+    }
+    p.start.resize(c.colliders);
+    p.count.resize(c.colliders);
+    agx_builder_read_collider(sim.builder, 0, AGX_COL_POLY_START, 1, p.start.data(), c.colliders);
+    agx_builder_read_collider(sim.builder, 0, AGX_COL_POLY_COUNT, 1, p.count.data(), c.colliders);
+    return p;
 }
 
 inline std::optional<BodyData> ObjectAPI::getBodyData(ObjectDenseIndex index) const {

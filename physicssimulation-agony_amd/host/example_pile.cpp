@@ -53,8 +53,31 @@ int main() {
     bad.localVertices = { { 0, 0 }, { 1, 0 }, { 2, 0 } };
     const bool rejected = !objects.createPolygon(bad).has_value();
 
+    // a setter before the upload is applied at the first update
+    const bool queuedEarly = objects.setBodyVelocity(10, { 0.0f, 2.0f });
     uint32_t steps = 0;
     for (int frame = 0; frame < 180; frame++) steps += sim.update(1.0f / 60.0f);
+    // deferred setters: lift one body above the pile and spin it, double another's mass
+    const ObjectDenseIndex lifted = 100, heavy = 120;
+    const bool queued = objects.setBodyPosition(lifted, { 0.0f, 18.0f }) && objects.setBodyAngularVelocity(lifted, 4.0f) &&
+                        objects.setBodyMass(heavy, 2.0f) && !objects.setBodyMass(0, 1.0f);   // a wall stays static
+    steps += sim.update(1.0f / 60.0f);
+    const auto liftedData = objects.getBodyData(lifted);
+    const auto heavyData = objects.getBodyData(heavy);
+    const bool applied = liftedData && liftedData->centerOfMass.y > 16.0f && liftedData->angularVelocity != 0.0f &&
+                         heavyData && heavyData->mass == 2.0f && heavyData->invMass == 0.5f;
+    for (int frame = 0; frame < 60; frame++) steps += sim.update(1.0f / 60.0f);
+    const ColliderArrays cols = objects.getColliders();
+    const PolygonArrays polys = objects.getPolygons();
+    uint32_t polygonColliders = 0;
+    bool aabbsContain = true;
+    for (size_t c = 0; c < cols.worldX.size(); c++) {
+        polygonColliders += cols.shapeType[c] == AGX_SHAPE_POLYGON ? 1u : 0u;
+        aabbsContain = aabbsContain && cols.aabbMinX[c] <= cols.worldX[c] && cols.worldX[c] <= cols.aabbMaxX[c] &&
+                       cols.aabbMinY[c] <= cols.worldY[c] && cols.worldY[c] <= cols.aabbMaxY[c];
+    }
+    const bool viewers = polygonColliders == 64 && polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 &&
+                         aabbsContain;
     const auto manifolds = sim.getBodyCollisionData();
     const auto [ids, islands] = sim.islands();
     const BodyArrays bodies = objects.getBodies();
@@ -64,9 +87,10 @@ int main() {
         inside = inside && std::fabs(bodies.comX[i]) < 11.0f && bodies.comY[i] > -1.0f && bodies.comY[i] < 21.0f;
     }
     const agx_stats st = sim.getStats();
-    std::printf("bodies %d steps %u manifolds %zu islands %u colours %u overflow %u rejected %d\n", made, steps,
-                manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0);
-    const bool ok = made == 192 && steps == 180 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
-                    st.overflow == 0;
+    std::printf("bodies %d steps %u manifolds %zu islands %u colours %u overflow %u rejected %d setters %d/%d/%d viewers %d\n",
+                made, steps, manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0, queuedEarly ? 1 : 0,
+                queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0);
+    const bool ok = made == 192 && steps == 241 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
+                    st.overflow == 0 && queuedEarly && queued && applied && viewers;
     return ok ? 0 : 1;
 }
