@@ -523,6 +523,7 @@ def run_pile(agx, scenes_mod, args, dt, cpu):
         world.step(dt, 40)
     world.sync()
     tune = world.autotune(dt, 4) if USE_GRAPHS else world.tuning()
+    advance(world, dt, 3)   # captures the step graphs of the chosen tuning outside the timed region
     steps = 100
     el = step_timed(world, steps, dt)
     st = world.stats()
@@ -641,6 +642,7 @@ def run_scatter(agx, scenes_mod, args, dt, cpu):
     world = build_world(agx, [scene], settings)
     n = world.counts.bodies - world.counts.static_bodies
     world.step(dt, 5)
+    advance(world, dt, 3)   # captures the step graphs outside the timed region
     steps = 20
     el = step_timed(world, steps, dt)
     st = world.stats()

@@ -18,8 +18,9 @@ constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour st
 constexpr uint32_t MAX_SWEEPS = AGX_MAX_SWEEPS;
 constexpr uint32_t TRAVERSAL_STACK = 64;
 constexpr uint32_t SORT_TILE_SMALL = 1024;  // keys per radix-sort block (256 threads x 4) ...
-constexpr uint32_t SORT_TILE_LARGE = 4096;  // ... and for trees of at least SORT_LARGE_KEYS keys (x 16):
-constexpr uint32_t SORT_LARGE_KEYS = 1u << 19;   // fewer tiles shorten the look-back, more keep CUs busy
+constexpr uint32_t SORT_TILE_LARGE = 4096;  // ... and from SORT_LARGE_KEYS keys (256 x 16): fewer
+constexpr uint32_t SORT_LARGE_KEYS = 1u << 18;   // tiles shorten the look-back, more keep CUs busy
+constexpr uint32_t sortTileFor(uint32_t n) { return n >= SORT_LARGE_KEYS ? SORT_TILE_LARGE : SORT_TILE_SMALL; }
 constexpr uint32_t BOUNDS_CHUNK = 4096;     // colliders per bounds block (one scene per chunk)
 
 // Device counters of one step (zeroed at step start, copied to agx_stats at FINISH).
@@ -135,11 +136,12 @@ static_assert(sizeof(NodeRecord) == 64, "node record is one 64-byte line");
 
 constexpr uint32_t STAGE_PAIRS = 1024;   // pairs a traversal wave stages before the gather
 
-// One BVH over a fixed collider list (dynamic or static), Karras layout:
-// internal nodes [0, n-1), leaves [n-1, 2n-1); node 0 is the root (a leaf when n == 1).
+// One BVH over a fixed collider list (dynamic or static), two-level Karras layout (tile trees
+// joined by a tree over the tiles): internal nodes [0, n-1), leaves [n-1, 2n-1); node 0 is the
+// root (a leaf when n == 1).
 struct Tree {
     uint32_t n = 0;
-    uint32_t sortTile = 0;         // keys per radix-sort block (SORT_TILE_SMALL or SORT_TILE_LARGE)
+    uint32_t sortTile = 0;         // keys per radix-sort block (sortTileFor(n))
     uint32_t sortBlocks = 0;
     uint32_t* keys = nullptr;      // sorted (scene << mortonBits | morton)
     uint32_t* vals = nullptr;      // sorted collider indices
@@ -150,7 +152,6 @@ struct Tree {
     uint2* child = nullptr;        // n-1: unified child indices
     uint2* range = nullptr;        // n-1: first, last sorted leaf
     uint32_t* parent = nullptr;    // 2n-1
-    uint32_t* flags = nullptr;     // n-1 refit arrival counters
     NodeRecord* rec = nullptr;     // n-1 internal-node records (written by the refit)
     uint32_t* leafBody = nullptr;  // n
     uint32_t* sceneStart = nullptr;  // S+1: sorted leaf range of each scene

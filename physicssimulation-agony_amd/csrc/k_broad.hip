@@ -1,4 +1,4 @@
-// Broad phase: per-scene bounds, Morton keys, LDS radix sort, Karras LBVH, refit, pair traversal.
+// Broad phase: per-scene bounds, Morton keys, LDS radix sort, two-level Karras LBVH, refit, pair traversal.
 //
 // The pair set equals the reference's (Broad/BroadPhaseCollisionDetector.cpp:109-244): node
 // AABBs are exact unions, so pruning with the strict overlap test never drops a strictly
@@ -134,31 +134,44 @@ __global__ __launch_bounds__(256) void kRadixDigitScan(uint32_t* __restrict__ di
     }
 }
 
-// Stable scatter of one digit. Each wave ranks SORT_TILE / 256 rounds of 64 consecutive keys with ballot
-// matching (8 ballots per round) and per-wave digit counters in LDS; waves are combined in order,
-// earlier tiles by look-back, earlier digits by the scanned totals.
-template <uint32_t SORT_TILE>
-__global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                     uint32_t n, uint32_t shift, const uint32_t* __restrict__ digitStart,
-                                                     uint32_t* status, uint32_t* ticket, uint32_t* overflow,
-                                                     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
-    constexpr uint32_t ROUNDS = SORT_TILE / 256;   // keys per lane
-    __shared__ uint32_t cnt[4][256];
-    __shared__ uint32_t base[4][256];
+// Stable scatter of one digit. A tile of THREADS x ROUNDS keys: each wave ranks ROUNDS rounds of
+// 64 consecutive keys with ballot matching (8 ballots per round) and per-wave digit counters in
+// LDS. The tile is then put in digit order in LDS, so that it leaves as contiguous runs, one per
+// digit, each at its digit's start + the keys of that digit in earlier tiles. Earlier tiles'
+// counts come by decoupled look-back (threads 0..255, one digit each): a tile publishes its own
+// counts at once and reads back to the nearest tile that has published its inclusive prefix.
+template <uint32_t THREADS, uint32_t ROUNDS>
+__global__ __launch_bounds__(THREADS) void kRadixScatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                         uint32_t n, uint32_t shift, const uint32_t* __restrict__ digitStart,
+                                                         uint32_t* status, uint32_t* ticket, uint32_t* overflow,
+                                                         uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
+    constexpr uint32_t WAVES = THREADS / 64, TILE = THREADS * ROUNDS;
+    static_assert(THREADS >= 256, "one look-back thread per digit");
+    static_assert(TILE <= 65536, "rank and digit share a register");
+    __shared__ uint32_t cnt[WAVES][256];   // per-wave digit counts, then each wave's offset in its digit's run
+    __shared__ uint32_t localStart[256];   // digit runs in the tile's digit order
+    __shared__ uint32_t outStart[256];     // digit runs in the output
+    __shared__ uint32_t tileKeys[TILE], tileVals[TILE];
     __shared__ uint32_t tileShared;
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     if (threadIdx.x == 0) tileShared = atomicAdd(ticket, 1u);
-    for (uint32_t w = 0; w < 4; w++) cnt[w][threadIdx.x] = 0;
+    for (uint32_t k = threadIdx.x; k < WAVES * 256; k += THREADS) (&cnt[0][0])[k] = 0;
     __syncthreads();
     const uint32_t tile = tileShared;
+    const uint32_t tileFirst = tile * TILE, tileCount = min(TILE, n - tileFirst);
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t rk[ROUNDS], dg[ROUNDS], ky[ROUNDS], vl[ROUNDS];
-    const uint32_t waveBase = tile * SORT_TILE + wid * (SORT_TILE / 4);
+    uint32_t rd[ROUNDS], ky[ROUNDS], vl[ROUNDS];   // rd: rank in the wave's digit run | digit << 16
+    const uint32_t waveBase = tileFirst + wid * (TILE / WAVES);
+    // every load of the tile in flight before the ranking starts (the ranking's LDS updates would
+    // otherwise hold each round's load back behind the previous round)
     for (uint32_t r = 0; r < ROUNDS; r++) {
         const uint32_t i = waveBase + r * 64 + lane;
-        const bool valid = i < n;
-        const uint32_t key = valid ? keys[i] : 0u;
-        const uint32_t d = (key >> shift) & 0xFFu;
+        ky[r] = i < n ? keys[i] : 0u;
+        vl[r] = i < n ? vals[i] : 0u;
+    }
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+        const bool valid = waveBase + r * 64 + lane < n;
+        const uint32_t d = (ky[r] >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
         for (uint32_t bit = 0; bit < 8; bit++) {
             const bool set = (d >> bit) & 1u;
@@ -169,21 +182,45 @@ __global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict_
         if (valid) before = cnt[wid][d];
         const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)peers) - 1);
         if (valid && lane == leader) cnt[wid][d] = before + (uint32_t)__popcll(peers);
-        rk[r] = before + (uint32_t)__popcll(peers & ltMask);
-        dg[r] = d;
-        ky[r] = key;
-        vl[r] = valid ? vals[i] : 0u;
+        rd[r] = (before + (uint32_t)__popcll(peers & ltMask)) | (d << 16);
     }
     __syncthreads();
-    {
+    uint32_t mine = 0;   // threads 0..255: this tile's keys of digit threadIdx.x
+    if (threadIdx.x < 256) {
         const uint32_t d = threadIdx.x;
-        const uint32_t mine = cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
-        uint32_t* own = status + tile * 256 + d;
+        for (uint32_t w = 0; w < WAVES; w++) {
+            const uint32_t c = cnt[w][d];
+            cnt[w][d] = mine;
+            mine += c;
+        }
+        storeFlag(status + tile * 256 + d, (tile == 0 ? STATUS_INC : STATUS_AGG) | mine);
+        localStart[d] = mine;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {   // exclusive scan of the 256 digit counts, four per lane
+        uint32_t v[4], sum = 0;
+        for (uint32_t k = 0; k < 4; k++) { v[k] = localStart[4 * lane + k]; sum += v[k]; }
+        uint32_t inc = sum;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        uint32_t run = inc - sum;
+        for (uint32_t k = 0; k < 4; k++) { localStart[4 * lane + k] = run; run += v[k]; }
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+        if (waveBase + r * 64 + lane < n) {
+            const uint32_t d = rd[r] >> 16;
+            const uint32_t at = localStart[d] + cnt[wid][d] + (rd[r] & 0xFFFFu);
+            tileKeys[at] = ky[r];
+            tileVals[at] = vl[r];
+        }
+    }
+    if (threadIdx.x < 256) {
+        const uint32_t d = threadIdx.x;
         uint32_t prefix = 0;
-        if (tile == 0) {
-            storeFlag(own, STATUS_INC | mine);
-        } else {
-            storeFlag(own, STATUS_AGG | mine);
+        if (tile != 0) {
             // look back LOOKBACK tiles at a time (all loads in flight together)
             constexpr int32_t LOOKBACK = 16;
             const uint64_t deadline = wall_clock64() + SORT_STALL_TICKS;
@@ -206,89 +243,195 @@ __global__ __launch_bounds__(256) void kRadixScatter(const uint32_t* __restrict_
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
-            storeFlag(own, STATUS_INC | (prefix + mine));
+            storeFlag(status + tile * 256 + d, STATUS_INC | (prefix + mine));
         }
-        uint32_t run = digitStart[d] + prefix;
-        for (uint32_t w = 0; w < 4; w++) { base[w][d] = run; run += cnt[w][d]; }
+        outStart[d] = digitStart[d] + prefix - localStart[d];
     }
     __syncthreads();
-    for (uint32_t r = 0; r < ROUNDS; r++) {
-        const uint32_t i = waveBase + r * 64 + lane;
-        if (i < n) {
-            const uint32_t dst = base[wid][dg[r]] + rk[r];
-            keysOut[dst] = ky[r];
-            valsOut[dst] = vl[r];
-        }
+    for (uint32_t k = threadIdx.x; k < tileCount; k += THREADS) {
+        const uint32_t key = tileKeys[k];
+        const uint32_t dst = outStart[(key >> shift) & 0xFFu] + k;
+        keysOut[dst] = key;
+        valsOut[dst] = tileVals[k];
     }
 }
 
-// ---------------------------------------------------------------- Karras LBVH
-__device__ inline int prefixDelta(const uint32_t* __restrict__ keys, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    const uint32_t a = keys[i], b = keys[j];
-    return a == b ? 32 + __clz((int)((uint32_t)i ^ (uint32_t)j)) : __clz((int)(a ^ b));
-}
+// ---------------------------------------------------------------- two-level LBVH
+// The sorted leaves are cut into tiles of BVH_TILE consecutive leaves. Each tile gets its own
+// Karras 2012 tree over its keys, and a Karras tree over the tiles' first keys joins the tile
+// roots. Every node still covers a contiguous range of sorted leaves, so the traversals and the
+// pair set are those of a single tree; what the cut buys is that each tile's nodes live in one
+// block's LDS during the refit and only the top tree (one node per tile) is left above them.
+// Node ids: top internal nodes [0, T-1) (node 0 is the root when T > 1), then each tile's
+// internal nodes in tile order, tile t from (T-1) + t*(BVH_TILE-1); leaves [n-1, 2n-1).
+constexpr uint32_t BVH_TILE = 1024;
+constexpr uint32_t BVH_TOP_THREADS = 1024;
+constexpr uint32_t BVH_TOP_MAX_TILES = 16384;   // LDS arrival counters of the top refit (64 KiB)
 
-// Karras 2012: one thread per internal node; ties broken by leaf index.
-__global__ __launch_bounds__(256) void kKarras(const uint32_t* __restrict__ keys, uint32_t n, uint2* __restrict__ child,
-                                               uint2* __restrict__ range, uint32_t* __restrict__ parent,
-                                               uint32_t* __restrict__ flags) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int nn = (int)n;
-    if (i >= nn - 1) return;
-    const int dR = prefixDelta(keys, nn, i, i + 1), dL = prefixDelta(keys, nn, i, i - 1);
-    const int d = dR > dL ? 1 : -1;
-    const int dMin = prefixDelta(keys, nn, i, i - d);
+__device__ inline uint32_t tileBase(uint32_t tiles, uint32_t t) { return (tiles - 1) + t * (BVH_TILE - 1); }
+
+// Karras split of internal node i over count sorted keys key(0..count-1), ties broken by index:
+// its first and last key and the split (the last key of its left half).
+template <class Key>
+__device__ inline void karrasNode(int i, int count, const Key& key, int& lo, int& hi, int& gamma) {
+    auto delta = [&](int a, int b) {
+        if (b < 0 || b >= count) return -1;
+        const uint32_t ka = key(a), kb = key(b);
+        return ka == kb ? 32 + __clz((int)((uint32_t)a ^ (uint32_t)b)) : __clz((int)(ka ^ kb));
+    };
+    const int d = delta(i, i + 1) > delta(i, i - 1) ? 1 : -1;
+    const int dMin = delta(i, i - d);
     int lMax = 2;
-    while (prefixDelta(keys, nn, i, i + lMax * d) > dMin) lMax <<= 1;
+    while (delta(i, i + lMax * d) > dMin) lMax <<= 1;
     int l = 0;
     for (int t = lMax >> 1; t >= 1; t >>= 1)
-        if (prefixDelta(keys, nn, i, i + (l + t) * d) > dMin) l += t;
+        if (delta(i, i + (l + t) * d) > dMin) l += t;
     const int j = i + l * d;
-    const int dNode = prefixDelta(keys, nn, i, j);
+    const int dNode = delta(i, j);
     int s = 0;
     int t = l;
     while (t > 1) {
         t = (t + 1) >> 1;
-        if (prefixDelta(keys, nn, i, i + (s + t) * d) > dNode) s += t;
+        if (delta(i, i + (s + t) * d) > dNode) s += t;
     }
-    const int gamma = i + s * d + min(d, 0);
-    const int lo = min(i, j), hi = max(i, j);
-    const uint32_t leafBase = n - 1;
-    const uint32_t left = (lo == gamma) ? leafBase + (uint32_t)gamma : (uint32_t)gamma;
-    const uint32_t right = (hi == gamma + 1) ? leafBase + (uint32_t)(gamma + 1) : (uint32_t)(gamma + 1);
-    child[i] = make_uint2(left, right);
-    range[i] = make_uint2((uint32_t)lo, (uint32_t)hi);
-    parent[left] = (uint32_t)i;
-    parent[right] = (uint32_t)i;
-    flags[i] = 0;
-    if (i == 0) parent[0] = 0xFFFFFFFFu;
+    gamma = i + s * d + min(d, 0);
+    lo = min(i, j);
+    hi = max(i, j);
 }
 
-__global__ __launch_bounds__(256) void kLeaves(const uint32_t* __restrict__ vals, uint32_t n, const float4* __restrict__ aabb,
-                                               const uint32_t* __restrict__ colBody, float4* __restrict__ nodeAabb,
-                                               uint32_t* __restrict__ leafBody, uint32_t* __restrict__ parent) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t c = vals[j];
-    nodeAabb[n - 1 + j] = aabb[c];
-    leafBody[j] = colBody[c];
-    if (n == 1) parent[0] = 0xFFFFFFFFu;
+// One thread per sorted leaf; leaf x of a tile with m leaves is internal node x of the tile's tree
+// for x < m - 1. With one tile its root is the tree's root.
+__global__ __launch_bounds__(256) void kKarrasTiles(const uint32_t* __restrict__ keys, uint32_t n, uint2* __restrict__ child,
+                                                    uint2* __restrict__ range, uint32_t* __restrict__ parent) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t tiles = (n + BVH_TILE - 1) / BVH_TILE;
+    const uint32_t t = i / BVH_TILE, t0 = t * BVH_TILE, x = i - t0;
+    const uint32_t m = min(BVH_TILE, n - t0);
+    if (x + 1 >= m) return;
+    const uint32_t* tk = keys + t0;
+    int lo, hi, gamma;
+    karrasNode((int)x, (int)m, [&](int k) { return tk[k]; }, lo, hi, gamma);
+    const uint32_t base = tileBase(tiles, t), leafBase = n - 1;
+    const uint32_t left = lo == gamma ? leafBase + t0 + (uint32_t)gamma : base + (uint32_t)gamma;
+    const uint32_t right = hi == gamma + 1 ? leafBase + t0 + (uint32_t)(gamma + 1) : base + (uint32_t)(gamma + 1);
+    const uint32_t id = base + x;
+    child[id] = make_uint2(left, right);
+    range[id] = make_uint2(t0 + (uint32_t)lo, t0 + (uint32_t)hi);
+    parent[left] = id;
+    parent[right] = id;
+    if (tiles == 1 && x == 0) parent[id] = 0xFFFFFFFFu;
+}
+
+// The root of tile t: its tree's node 0, or its only leaf.
+__device__ inline uint32_t tileRoot(uint32_t n, uint32_t tiles, uint32_t t) {
+    const uint32_t t0 = t * BVH_TILE;
+    return min(BVH_TILE, n - t0) > 1 ? tileBase(tiles, t) : n - 1 + t0;
+}
+
+// One thread per top internal node (tiles - 1 of them, tiles >= 2).
+__global__ __launch_bounds__(256) void kKarrasTop(const uint32_t* __restrict__ keys, uint32_t n, uint2* __restrict__ child,
+                                                  uint2* __restrict__ range, uint32_t* __restrict__ parent) {
+    const uint32_t tiles = (n + BVH_TILE - 1) / BVH_TILE;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k + 1 >= tiles) return;
+    int lo, hi, gamma;
+    karrasNode((int)k, (int)tiles, [&](int t) { return keys[(size_t)t * BVH_TILE]; }, lo, hi, gamma);
+    const uint32_t left = lo == gamma ? tileRoot(n, tiles, (uint32_t)gamma) : (uint32_t)gamma;
+    const uint32_t right = hi == gamma + 1 ? tileRoot(n, tiles, (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    child[k] = make_uint2(left, right);
+    range[k] = make_uint2((uint32_t)lo * BVH_TILE, min(((uint32_t)hi + 1) * BVH_TILE, n) - 1);
+    parent[left] = k;
+    parent[right] = k;
+    if (k == 0) parent[0] = 0xFFFFFFFFu;
 }
 
 // Bottom-up refit (BvhTree.cpp:426-530 computes the same unions top-down in reverse order):
-// the second thread to reach a node merges its children and writes the node's record (both
-// children's boxes, ranges and references). Node boxes written here use agent-coherent stores,
-// completed before the arrival counter is bumped.
+// the second walk to reach a node merges its children and writes the node's record (both
+// children's boxes, ranges and references).
+__device__ inline float4 boxUnion(const float4& a, const float4& b) {
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+}
+
+// One block per tile: the tile's leaves (box, reference, body) and its internal nodes' parents,
+// ranges and children are loaded into LDS with coalesced loads, then every leaf walks up its
+// tile's tree with LDS arrival counters, without touching memory except to write the results.
+__global__ __launch_bounds__(BVH_TILE) void kRefitTiles(uint32_t n, const uint2* __restrict__ child,
+                                                        const uint2* __restrict__ range, uint32_t* __restrict__ parent,
+                                                        const uint32_t* __restrict__ vals, const float4* __restrict__ aabb,
+                                                        const uint32_t* __restrict__ colBody, uint32_t* __restrict__ leafBody,
+                                                        float4* __restrict__ nodeAabb, NodeRecord* __restrict__ rec) {
+    __shared__ float4 leafBox[BVH_TILE], innerBox[BVH_TILE];
+    __shared__ uint2 innerRange[BVH_TILE], innerChild[BVH_TILE];
+    __shared__ uint32_t innerParent[BVH_TILE], leafParent[BVH_TILE], leafRef[BVH_TILE], leafOwner[BVH_TILE];
+    __shared__ uint32_t arrived[BVH_TILE];
+    const uint32_t tiles = (n + BVH_TILE - 1) / BVH_TILE;
+    const uint32_t x = threadIdx.x, t0 = blockIdx.x * BVH_TILE, j = t0 + x;
+    const uint32_t m = min(BVH_TILE, n - t0);
+    const uint32_t leafBase = n - 1, base = tileBase(tiles, blockIdx.x);
+    arrived[x] = 0;
+    if (x < m) {
+        const uint32_t c = vals[j];
+        const uint32_t b = colBody[c];
+        const float4 box = aabb[c];
+        leafRef[x] = c;
+        leafOwner[x] = b;
+        leafBox[x] = box;
+        leafBody[j] = b;
+        nodeAabb[leafBase + j] = box;
+        leafParent[x] = n > 1 ? parent[leafBase + j] : 0xFFFFFFFFu;
+    }
+    if (x + 1 < m) {
+        innerParent[x] = parent[base + x];
+        innerRange[x] = range[base + x];
+        innerChild[x] = child[base + x];
+    }
+    if (n == 1 && x == 0) parent[0] = 0xFFFFFFFFu;
+    __syncthreads();
+    if (x >= m) return;
+    auto info = [&](uint32_t c) {
+        ChildInfo ci;
+        if (c >= leafBase) {
+            const uint32_t k = c - leafBase - t0;
+            ci.box = leafBox[k];
+            ci.first = ci.last = c - leafBase;
+            ci.ref = leafRef[k];
+            ci.body = leafOwner[k];
+        } else {
+            const uint2 r = innerRange[c - base];
+            ci.box = innerBox[c - base];
+            ci.first = r.x; ci.last = r.y;
+            ci.ref = c;
+            ci.body = 0xFFFFFFFFu;
+        }
+        return ci;
+    };
+    uint32_t p = leafParent[x];
+    while (p - base < m - 1) {   // p is an internal node of this tile (ids below base wrap around)
+        __threadfence_block();
+        if (atomicAdd(&arrived[p - base], 1u) == 0) break;
+        __threadfence_block();
+        const uint2 ch = innerChild[p - base];
+        NodeRecord r;
+        r.c[0] = info(ch.x);
+        r.c[1] = info(ch.y);
+        const float4 box = boxUnion(r.c[0].box, r.c[1].box);
+        innerBox[p - base] = box;
+        nodeAabb[p] = box;
+        rec[p] = r;
+        p = innerParent[p - base];
+    }
+}
+
 __device__ inline ChildInfo childInfo(uint32_t c, uint32_t leafBase, const float4& box, const uint2* range,
-                                      const uint32_t* vals, const uint32_t* leafBody) {
+                                      const uint32_t* vals, const uint32_t* colBody) {
     ChildInfo ci;
     ci.box = box;
     if (c >= leafBase) {
         const uint32_t j = c - leafBase;
         ci.first = ci.last = j;
         ci.ref = vals[j];
-        ci.body = leafBody[j];
+        ci.body = colBody[ci.ref];
     } else {
         const uint2 r = range[c];
         ci.first = r.x; ci.last = r.y;
@@ -298,29 +441,36 @@ __device__ inline ChildInfo childInfo(uint32_t c, uint32_t leafBase, const float
     return ci;
 }
 
-__global__ __launch_bounds__(256) void kRefit(uint32_t n, const uint2* __restrict__ child, const uint2* __restrict__ range,
-                                              const uint32_t* __restrict__ parent, const uint32_t* __restrict__ vals,
-                                              const uint32_t* __restrict__ leafBody, uint32_t* __restrict__ flags,
-                                              float4* __restrict__ nodeAabb, NodeRecord* __restrict__ rec) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || n < 2) return;
+// One workgroup refits the top tree from the tile roots (tiles >= 2), with arrival counters in
+// LDS; the boxes its walks hand over go through device memory, ordered by the workgroup fences
+// around each counter bump (all its waves run on one CU).
+__global__ __launch_bounds__(BVH_TOP_THREADS) void kRefitTop(uint32_t n, const uint2* __restrict__ child,
+                                                             const uint2* __restrict__ range,
+                                                             const uint32_t* __restrict__ parent,
+                                                             const uint32_t* __restrict__ vals,
+                                                             const uint32_t* __restrict__ colBody, float4* nodeAabb,
+                                                             NodeRecord* __restrict__ rec) {
+    __shared__ uint32_t arrived[BVH_TOP_MAX_TILES];
+    const uint32_t tiles = (n + BVH_TILE - 1) / BVH_TILE;
+    for (uint32_t k = threadIdx.x; k + 1 < tiles; k += blockDim.x) arrived[k] = 0;
+    __syncthreads();
     const uint32_t leafBase = n - 1;
-    uint32_t node = leafBase + j;
-    while (true) {
-        const uint32_t p = parent[node];
-        if (p == 0xFFFFFFFFu) break;
-        orderStores();
-        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) break;
-        const uint2 ch = child[p];
-        const float4 a = loadCoherent(nodeAabb + ch.x);
-        const float4 b = loadCoherent(nodeAabb + ch.y);
-        storeCoherent(nodeAabb + p, make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w)));
-        NodeRecord r;
-        r.c[0] = childInfo(ch.x, leafBase, a, range, vals, leafBody);
-        r.c[1] = childInfo(ch.y, leafBase, b, range, vals, leafBody);
-        rec[p] = r;
-        node = p;
+    for (uint32_t t = threadIdx.x; t < tiles; t += blockDim.x) {
+        uint32_t p = parent[tileRoot(n, tiles, t)];
+        while (p != 0xFFFFFFFFu) {
+            const uint2 ch = child[p];
+            const uint32_t up = parent[p];
+            __threadfence_block();
+            if (atomicAdd(&arrived[p], 1u) == 0) break;
+            __threadfence_block();
+            const float4 a = nodeAabb[ch.x], b = nodeAabb[ch.y];
+            NodeRecord r;
+            r.c[0] = childInfo(ch.x, leafBase, a, range, vals, colBody);
+            r.c[1] = childInfo(ch.y, leafBase, b, range, vals, colBody);
+            nodeAabb[p] = boxUnion(a, b);
+            rec[p] = r;
+            p = up;
+        }
     }
 }
 
@@ -572,28 +722,34 @@ hipError_t launchBuildTree(hipStream_t st, DeviceWorld& w, Tree& t) {
     uint32_t* tickets = digitTotal + SORT_PASSES_MAX * 256;
     hipError_t e = launchFillU32(st, t.hist, 0u, (size_t)SORT_PASSES_MAX * (t.sortBlocks + 1) * 256 + SORT_PASSES_MAX);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kRadixHistAll, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.n, passes, t.sortTile, digitTotal);
+    hipLaunchKernelGGL(kRadixHistAll, dim3(blocksFor(t.n, SORT_TILE_LARGE)), dim3(256), 0, st, t.keys, t.n, passes,
+                       SORT_TILE_LARGE, digitTotal);
     hipLaunchKernelGGL(kRadixDigitScan, dim3(1), dim3(256), 0, st, digitTotal, passes);
     for (uint32_t p = 0; p < passes; p++) {
-        if (t.sortTile == SORT_TILE_LARGE)
-            hipLaunchKernelGGL(kRadixScatter<SORT_TILE_LARGE>, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, p * 8,
-                               digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p, &w.ctr->overflow,
-                               t.keysAlt, t.valsAlt);
-        else
-            hipLaunchKernelGGL(kRadixScatter<SORT_TILE_SMALL>, dim3(t.sortBlocks), dim3(256), 0, st, t.keys, t.vals, t.n, p * 8,
-                               digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p, &w.ctr->overflow,
-                               t.keysAlt, t.valsAlt);
+#define AGX_SCATTER(THREADS, ROUNDS)                                                                                   \
+    hipLaunchKernelGGL((kRadixScatter<THREADS, ROUNDS>), dim3(t.sortBlocks), dim3(THREADS), 0, st, t.keys, t.vals, t.n, \
+                       p * 8, digitTotal + p * 256, status + (size_t)p * t.sortBlocks * 256, tickets + p,              \
+                       &w.ctr->overflow, t.keysAlt, t.valsAlt)
+        static_assert(SORT_TILE_SMALL == 256 * 4 && SORT_TILE_LARGE == 256 * 16, "tile shapes");
+        if (t.sortTile == SORT_TILE_LARGE) AGX_SCATTER(256, 16);
+        else AGX_SCATTER(256, 4);
+#undef AGX_SCATTER
         std::swap(t.keys, t.keysAlt);
         std::swap(t.vals, t.valsAlt);
     }
+    const uint32_t tiles = (t.n + BVH_TILE - 1) / BVH_TILE;
+    if (tiles > BVH_TOP_MAX_TILES) return hipErrorInvalidValue;   // more leaves than the top refit holds
     if (t.n > 1)
-        hipLaunchKernelGGL(kKarras, dim3(blocksFor(t.n - 1, 256)), dim3(256), 0, st, t.keys, t.n, t.child, t.range,
-                           t.parent, t.flags);
-    hipLaunchKernelGGL(kLeaves, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.vals, t.n, w.aabb, w.colBody, t.nodeAabb,
-                       t.leafBody, t.parent);
-    if (t.n > 1)
-        hipLaunchKernelGGL(kRefit, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.n, t.child, t.range, t.parent, t.vals,
-                           t.leafBody, t.flags, t.nodeAabb, t.rec);
+        hipLaunchKernelGGL(kKarrasTiles, dim3(blocksFor(t.n, 256)), dim3(256), 0, st, t.keys, t.n, t.child, t.range,
+                           t.parent);
+    if (tiles > 1)
+        hipLaunchKernelGGL(kKarrasTop, dim3(blocksFor(tiles - 1, 256)), dim3(256), 0, st, t.keys, t.n, t.child, t.range,
+                           t.parent);
+    hipLaunchKernelGGL(kRefitTiles, dim3(tiles), dim3(BVH_TILE), 0, st, t.n, t.child, t.range, t.parent, t.vals, w.aabb,
+                       w.colBody, t.leafBody, t.nodeAabb, t.rec);
+    if (tiles > 1)
+        hipLaunchKernelGGL(kRefitTop, dim3(1), dim3(BVH_TOP_THREADS), 0, st, t.n, t.child, t.range, t.parent, t.vals,
+                           w.colBody, t.nodeAabb, t.rec);
     return hipGetLastError();
 }
 

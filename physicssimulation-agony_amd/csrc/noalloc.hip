@@ -27,10 +27,10 @@ struct Layout {
 };
 
 struct TreeOffsets {
-    size_t keys, vals, keysAlt, valsAlt, leafBody, nodeAabb, child, range, parent, flags, rec, hist;
+    size_t keys, vals, keysAlt, valsAlt, leafBody, nodeAabb, child, range, parent, rec, hist;
 };
 
-constexpr uint32_t sortTileFor(uint32_t n) { return n >= agx::SORT_LARGE_KEYS ? agx::SORT_TILE_LARGE : agx::SORT_TILE_SMALL; }
+using agx::sortTileFor;
 constexpr uint32_t sortBlocksFor(uint32_t n) {
     return std::max(1u, (n + sortTileFor(n) - 1) / sortTileFor(n));
 }
@@ -47,7 +47,6 @@ constexpr TreeOffsets carveTree(Layout& L, uint32_t n) {
     t.child = L.take<uint2>(internal);
     t.range = L.take<uint2>(internal);
     t.parent = L.take<uint32_t>(nodes);
-    t.flags = L.take<uint32_t>(internal);
     t.rec = L.take<agx::NodeRecord>(internal);
     t.hist = L.take<uint32_t>(4ull * (sortBlocksFor(n) + 1) * 256 + 4);
     return t;
@@ -134,7 +133,6 @@ void bindTree(agx::Tree& t, unsigned char* base, const TreeOffsets& o, uint32_t 
     t.child = reinterpret_cast<uint2*>(base + o.child);
     t.range = reinterpret_cast<uint2*>(base + o.range);
     t.parent = reinterpret_cast<uint32_t*>(base + o.parent);
-    t.flags = reinterpret_cast<uint32_t*>(base + o.flags);
     t.rec = reinterpret_cast<agx::NodeRecord*>(base + o.rec);
     t.hist = reinterpret_cast<uint32_t*>(base + o.hist);
     t.list = list;

@@ -141,7 +141,7 @@ uint32_t ceilLog2(uint32_t v) {
 int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, const std::vector<uint32_t>& colScene,
               uint32_t S) {
     t.n = (uint32_t)list.size();
-    t.sortTile = t.n >= agx::SORT_LARGE_KEYS ? agx::SORT_TILE_LARGE : agx::SORT_TILE_SMALL;
+    t.sortTile = agx::sortTileFor(t.n);
     t.sortBlocks = std::max(1u, (t.n + t.sortTile - 1) / t.sortTile);
     std::vector<uint32_t> sceneStart(S + 1, 0);
     for (uint32_t c : list) sceneStart[colScene[c] + 1]++;
@@ -168,7 +168,6 @@ int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, con
     HIP_CHECK(w, devAlloc(w, &t.child, internal));
     HIP_CHECK(w, devAlloc(w, &t.range, internal));
     HIP_CHECK(w, devAlloc(w, &t.parent, nodes));
-    HIP_CHECK(w, devAlloc(w, &t.flags, internal));
     HIP_CHECK(w, devAlloc(w, &t.rec, internal));
     HIP_CHECK(w, devAlloc(w, &t.leafBody, n));
     HIP_CHECK(w, devAlloc(w, &t.sceneBounds, S));
