@@ -31,10 +31,11 @@ extern "C" {
 #define AGX_ERR_STATE (-4)
 #define AGX_ERR_NO_DEVICE (-5)
 
-/* Shape types: Types/Types.h:38-45 (ColliderShapeType; capsules are not on this path). */
+/* Shape types: Types/Types.h:35-42 (ColliderShapeType). */
 #define AGX_SHAPE_CIRCLE 0
 #define AGX_SHAPE_BOX 1
 #define AGX_SHAPE_POLYGON 2
+#define AGX_SHAPE_CAPSULE 3
 
 typedef struct agx_builder agx_builder;
 typedef struct agx_world agx_world;
@@ -164,7 +165,8 @@ int agx_builder_destroy(agx_builder* b);
 int agx_builder_add_material(agx_builder* b, float elasticity, float static_friction,
                              float dynamic_friction, uint32_t* out_index);
 /* Adds n bodies with one collider each to scene 'scene', in call order. shape_type[i] picks
- * the shape; p0/p1 are radius (circle) or width/height (box); polygons take vert_count[i]
+ * the shape; p0/p1 are radius (circle), width/height (box) or full core length/radius (capsule,
+ * CapsuleCreateParams, ObjectCreateParams.h:97-104: its axis is local +x); polygons take vert_count[i]
  * local (x, y) vertices starting at vertex vert_start[i] of 'verts'. out_dense_index (may be
  * NULL) receives each body's dense index at its creation; a rejected body (invalid polygon)
  * gets UINT32_MAX and the call returns the number of rejected bodies. */

@@ -329,4 +329,39 @@ uint32_t createPolygon(World& w, float px, float py, float rot, float vx, float 
     return body;
 }
 
+// PhysicsGeometry.cpp:87-117, with the centre of mass at the capsule centre.
+float capsuleInertia(float mass, float length, float radius) {
+    const float PI = (float)3.14159265358979323846;   // Constants.h:8
+    const float rectArea = length * 2.0f * radius;
+    const float circleArea = PI * radius * radius;
+    const float totalArea = rectArea + circleArea;
+    const float rectMass = mass * rectArea / totalArea;
+    const float capMass = mass * circleArea / totalArea;
+    const float diameter = 2.0f * radius;
+    const float rectInertia = rectMass * (length * length + diameter * diameter) / 12.0f;
+    const float d = 4.0f * radius / (3.0f * PI);
+    const float capInertia = capMass * (0.5f * radius * radius + 0.25f * length * length + length * d);
+    const V2 com{ 0.0f, 0.0f };
+    const float deltaSquared = dot(com, com);
+    return rectInertia + capInertia + mass * deltaSquared;
+}
+
+// ObjectManager.cpp:351-389 + :197-230 (createCapsuleCollider with zero offset and rotation).
+uint32_t createCapsule(World& w, float px, float py, float rot, float vx, float vy, float av,
+                       float massIn, uint32_t material, float lengthIn, float radiusIn) {
+    if (w.materials.empty()) w.materials.push_back(Material{});
+    const float length = std::fmax(0.0f, lengthIn);
+    const float mass = std::fmax(0.0f, massIn);
+    const float radius = std::fmax(0.0f, radiusIn);
+    const V2 com{ 0.0f, 0.0f };
+    const float inertia = capsuleInertia(mass, length, radius);
+    const float invMass = mass == 0.0f ? 0.0f : 1.0f / mass;
+    const float invInertia = inertia == 0.0f ? 0.0f : 1.0f / inertia;
+    const uint32_t body = appendBody(w, { px, py }, { vx, vy }, rot, av, mass, invMass, inertia, invInertia, com);
+    const uint32_t c = appendCollider(w, body, { 0.0f, 0.0f }, 0.0f, validMaterial(w, material), SHAPE_CAPSULE,
+                                      length * 0.5f, radius, 0, 0);
+    placeNewCollider(w, c, radius, length * 0.5f + radius);
+    return body;
+}
+
 }  // namespace orc

@@ -67,6 +67,7 @@ int orc_add_bodies(void* h, uint32_t n, const uint8_t* type, const float* px, co
         uint32_t r;
         if (type[i] == SHAPE_CIRCLE) r = createCircle(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i]);
         else if (type[i] == SHAPE_BOX) r = createBox(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i], p1[i]);
+        else if (type[i] == SHAPE_CAPSULE) r = createCapsule(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i], p1[i]);
         else r = createPolygon(w, px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i],
                                reinterpret_cast<const V2*>(vxy) + vstart[i], vcount[i]);
         if (out_index) out_index[i] = r;

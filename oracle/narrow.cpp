@@ -1,5 +1,5 @@
 // Oracle narrow phase (test infrastructure only): Narrow/NarrowPhaseCollisionDetector.cpp
-// and the six in-scope CollisionMethods/*.cpp, restated in the reference's operation order.
+// and the ten CollisionMethods/*.cpp, restated in the reference's operation order.
 #include "oracle.h"
 
 #include <algorithm>
@@ -446,7 +446,363 @@ bool boxPolygon(const World& w, uint32_t a, uint32_t b, Manifold& out) {
     return true;
 }
 
-// SymmetricMatrix.h:14-18 storage index of (typeA <= typeB): CC, CB, BB, CP, BP, PP.
+// ---------------------------------------------------------------- capsules
+constexpr uint32_t CAPSULE_VERTEX_CONTACT_TAG = 0x40000000u;   // Narrow/Constants.h:40-41
+constexpr float CAPSULE_PARALLEL_SIN_SQUARED = 1e-3f;
+inline float clampValue(float v, float lo, float hi) { return std::fmin(std::fmax(v, lo), hi); }   // HelperMethods.h:111-114
+
+// CollisionMethods/CircleCapsule.cpp:58-147, the batched (8-lane) path for every pair. The
+// reference's scalar tail (the last pairs of a bucket that do not fill a batch, :150-203) tests
+// the side of a centre lying on the core (distance < 1e-4) with a different sign expression; that
+// branch is restated as the batched path has it.
+bool circleCapsule(const World& w, uint32_t a, uint32_t b, Manifold& out) {
+    const float positionAX = w.wX[a], positionAY = w.wY[a];
+    const float positionBX = w.wX[b], positionBY = w.wY[b];
+    const float cosB = w.wCos[b], sinB = w.wSin[b];
+    const float radiusA = w.p0[a], halfLengthB = w.p0[b], radiusB = w.p1[b];
+    const float dX = positionAX - positionBX;
+    const float dY = positionAY - positionBY;
+    const float negHalfLength = 0.0f - halfLengthB;
+    const float alongAxis = dX * cosB + dY * sinB;
+    const float clampedLow = alongAxis < negHalfLength ? negHalfLength : alongAxis;
+    const float t = halfLengthB < clampedLow ? halfLengthB : clampedLow;
+    const float deltaX = cosB * t - dX;
+    const float deltaY = sinB * t - dY;
+    const float radiusSum = radiusA + radiusB;
+    const float squaredDistance = deltaX * deltaX + deltaY * deltaY;
+    if (!(squaredDistance < radiusSum * radiusSum)) return false;
+    const float distance = std::sqrt(squaredDistance);
+    const float invDistance = 1.0f / distance;
+    const bool isCloseToZero = distance < ZERO_DIV;
+    const float sideOffset = dY * cosB - dX * sinB;
+    const bool flip = sideOffset < 0.0f;
+    const float fallbackX = flip ? -sinB : sinB;
+    const float fallbackY = flip ? cosB : -cosB;
+    const V2 normal{ isCloseToZero ? fallbackX : deltaX * invDistance, isCloseToZero ? fallbackY : deltaY * invDistance };
+    const float depth = radiusSum - distance;
+    const V2 positionA{ positionAX, positionAY };
+    out = makeManifold(a, b, normal, depth, 1, positionA + normal * radiusA, { 0, 0 }, 0, 0);
+    return true;
+}
+
+// CollisionMethods/CapsuleCapsule.cpp:25-157.
+bool capsuleCapsule(const World& w, uint32_t a, uint32_t b, Manifold& out) {
+    const V2 positionA{ w.wX[a], w.wY[a] }, positionB{ w.wX[b], w.wY[b] };
+    const float halfLengthA = w.p0[a], halfLengthB = w.p0[b], radiusA = w.p1[a], radiusB = w.p1[b];
+    const V2 centerDelta = positionB - positionA;
+    const float radiusSum = radiusA + radiusB;
+    const float reach = halfLengthA + halfLengthB + radiusSum;
+    if (dot(centerDelta, centerDelta) >= reach * reach) return false;
+    const V2 axisA{ w.wCos[a], w.wSin[a] }, axisB{ w.wCos[b], w.wSin[b] };
+    const float bb = dot(axisA, axisB);
+    const float centerAlongA = dot(centerDelta, axisA);
+    const float c = -centerAlongA;
+    const float f = -dot(axisB, centerDelta);
+    const float sinSquared = 1.0f - bb * bb;
+    const bool parallel = sinSquared < CAPSULE_PARALLEL_SIN_SQUARED;
+    const float extentB = std::fabs(bb) * halfLengthB;
+    const float overlapLo = std::fmax(-halfLengthA, centerAlongA - extentB);
+    const float overlapHi = std::fmin(halfLengthA, centerAlongA + extentB);
+    const bool hasOverlap = parallel && (overlapHi - overlapLo >= ZERO_DIV);
+    float s, t;
+    if (hasOverlap) {
+        s = 0.5f * (overlapLo + overlapHi);
+        t = clampValue(bb * s + f, -halfLengthB, halfLengthB);
+    } else {
+        s = parallel ? 0.0f : clampValue((bb * f - c) / sinSquared, -halfLengthA, halfLengthA);
+        t = bb * s + f;
+        t = clampValue(t, -halfLengthB, halfLengthB);
+        s = clampValue(bb * t - c, -halfLengthA, halfLengthA);
+    }
+    const V2 pointA = positionA + axisA * s;
+    const V2 pointB = positionB + axisB * t;
+    const V2 delta = pointB - pointA;
+    const float squaredDistance = dot(delta, delta);
+    if (squaredDistance >= radiusSum * radiusSum) return false;
+    V2 normal;
+    float depth;
+    if (squaredDistance >= ZERO_DIV_SQ) {
+        const float distance = std::sqrt(squaredDistance);
+        normal = delta * (1.0f / distance);
+        depth = radiusSum - distance;
+    } else {
+        const V2 sideA{ -axisA.y, axisA.x }, sideB{ -axisB.y, axisB.x };
+        const float offsetA = dot(centerDelta, sideA), offsetB = dot(centerDelta, sideB);
+        const float depthA = std::fabs(dot(axisB, sideA)) * halfLengthB - std::fabs(offsetA);
+        const float depthB = std::fabs(dot(axisA, sideB)) * halfLengthA - std::fabs(offsetB);
+        if (depthA <= depthB) { normal = sideA; flipSignIfNegative(normal, offsetA); depth = depthA; }
+        else { normal = sideB; flipSignIfNegative(normal, offsetB); depth = depthB; }
+        depth += radiusSum;
+    }
+    V2 contactPoints[2] = { { 0, 0 }, { 0, 0 } };
+    uint32_t contactIds[2] = { 0, 0 };
+    uint32_t contactCount = 0;
+    if (hasOverlap) {
+        const V2 sideB{ -axisB.y, axisB.x };
+        const float ends[2] = { overlapLo, overlapHi };
+        float minGap = FLT_MAX;
+        for (uint32_t k = 0; k < 2; k++) {
+            const V2 p = positionA + axisA * ends[k];
+            const float gap = std::fabs(dot(p - positionB, sideB));
+            if (gap >= radiusSum) continue;
+            contactPoints[contactCount] = p + normal * radiusA;
+            contactIds[contactCount] = k;
+            contactCount++;
+            minGap = std::fmin(minGap, gap);
+        }
+        if (contactCount != 0) depth = radiusSum - minGap;
+    }
+    if (contactCount == 0) {
+        contactPoints[0] = pointA + normal * radiusA;
+        contactIds[0] = 2;
+        contactCount = 1;
+    }
+    out = makeManifold(a, b, normal, depth, contactCount, contactPoints[0], contactPoints[1], contactIds[0], contactIds[1]);
+    return true;
+}
+
+// CollisionMethods/BoxCapsule.cpp:26-242.
+bool boxCapsule(const World& w, uint32_t a, uint32_t b, Manifold& out) {
+    const V2 halfExtents{ w.p0[a], w.p1[a] };
+    const V2 positionA{ w.wX[a], w.wY[a] }, rightA{ w.wCos[a], w.wSin[a] };
+    const V2 positionB{ w.wX[b], w.wY[b] }, axisB{ w.wCos[b], w.wSin[b] };
+    const float halfLengthB = w.p0[b], radius = w.p1[b];
+    const V2 upA{ -rightA.y, rightA.x };
+    const V2 centerDelta = positionB - positionA;
+    const float localX = dot(centerDelta, rightA);
+    const float localY = dot(centerDelta, upA);
+    const float sx = std::copysign(1.0f, localX);
+    const float sy = std::copysign(1.0f, localY);
+    const V2 center{ std::fabs(localX), std::fabs(localY) };
+    const V2 axis{ (rightA.x * axisB.x + rightA.y * axisB.y) * sx, (rightA.x * axisB.y - rightA.y * axisB.x) * sy };
+    const float absAxisX = std::fabs(axis.x), absAxisY = std::fabs(axis.y);
+    const float sepX = center.x - halfLengthB * absAxisX - halfExtents.x;
+    const float sepY = center.y - halfLengthB * absAxisY - halfExtents.y;
+    const float faceSeparation = std::fmax(sepX, sepY);
+    if (faceSeparation >= radius) return false;
+    const V2 sideAxis{ -axis.y, axis.x };
+    const float sideOffset = dot(sideAxis, center);
+    const float sideSeparation = std::fabs(sideOffset) - (halfExtents.x * absAxisY + halfExtents.y * absAxisX);
+    if (sideSeparation >= radius) return false;
+    const bool sideWins = sideSeparation > faceSeparation + SAT_EPS;
+    const bool swap = !sideWins && sepY > sepX;
+    const V2 ext = swap ? V2{ halfExtents.y, halfExtents.x } : halfExtents;
+    const V2 dir = swap ? V2{ axis.y, axis.x } : axis;
+    const V2 mid = swap ? V2{ center.y, center.x } : center;
+    const V2 basisX = rightA * sx, basisY = upA * sy;
+    const V2 worldX = swap ? basisY : basisX, worldY = swap ? basisX : basisY;
+    const V2 halfCore = dir * halfLengthB;
+    const V2 s0 = mid - halfCore, s1 = mid + halfCore;
+    const float length = halfLengthB * 2.0f;
+    auto coreDistSquared = [&](const V2& p) {
+        const V2 d = s0 + dir * std::fmin(std::fmax(dot(p - s0, dir), 0.0f), length) - p;
+        return dot(d, d);
+    };
+    V2 normal{ 1, 0 };
+    float depth = 0.0f;
+    uint32_t contactCount = 0;
+    V2 point0{ 0, 0 }, point1{ 0, 0 };
+    uint32_t id0 = 0, id1 = 0;
+    bool faceContact = false, rejected = false, cornerPosX = true, cornerPosY = true;
+    V2 vertexNormal{ 1, 0 };
+    float vertexSeparation = faceSeparation;
+    if (sideWins) {
+        vertexNormal = sideOffset >= 0 ? sideAxis : -sideAxis;
+        vertexSeparation = sideSeparation;
+        cornerPosX = vertexNormal.x >= 0;
+        cornerPosY = vertexNormal.y >= 0;
+    } else do {
+        V2 c0 = s0, c1 = s1;
+        float e0 = s0.x - ext.x, e1 = s1.x - ext.x;
+        if (faceSeparation > 0) {
+            const float minY = std::fmin(s0.y, s1.y), maxY = std::fmax(s0.y, s1.y);
+            if (minY > ext.y) break;
+            const bool beyondLow = minY < -ext.y, beyondHigh = maxY > ext.y;
+            if (beyondLow || beyondHigh) {
+                const float invDy = 1.0f / (s1.y - s0.y);
+                const float tA = (-ext.y - s0.y) * invDy, tB = (ext.y - s0.y) * invDy;
+                const float tLow = std::fmax(0.0f, std::fmin(tA, tB)), tHigh = std::fmin(1.0f, std::fmax(tA, tB));
+                const V2 span = s1 - s0;
+                c0 = s0 + span * tLow;
+                c1 = s0 + span * tHigh;
+                e0 = c0.x - ext.x;
+                e1 = c1.x - ext.x;
+                const float minE = std::fmin(e0, e1);
+                const float faceDistSquared = minE * minE;
+                if (beyondLow && coreDistSquared(V2{ ext.x, -ext.y }) + SAT_EPS < faceDistSquared) { cornerPosY = false; break; }
+                if (beyondHigh && coreDistSquared(ext) + SAT_EPS < faceDistSquared) break;
+            }
+        }
+        const V2 clipped = c1 - c0;
+        const bool hasSecond = dot(clipped, clipped) >= ZERO_DIV_SQ;
+        const float contactLimit = radius + SAT_EPS;
+        const uint32_t faceId = swap ? (sy < 0.0f ? 0u : 2u) : (sx < 0.0f ? 3u : 1u);
+        const uint32_t idBase = REF_TAG | (faceId << 1);
+        if (e0 <= contactLimit) { point0 = V2{ c0.x - radius, c0.y }; id0 = idBase; contactCount = 1; }
+        if (hasSecond && e1 <= contactLimit) {
+            const V2 p{ c1.x - radius, c1.y };
+            if (contactCount == 0) { point0 = p; id0 = idBase | 1u; }
+            else { point1 = p; id1 = idBase | 1u; }
+            contactCount++;
+        }
+        if (contactCount == 0) { rejected = true; break; }
+        depth = radius - std::fmin(e0, e1);
+        faceContact = true;
+    } while (false);
+    if (rejected) return false;
+    if (!faceContact) {
+        const V2 wc{ cornerPosX ? ext.x : -ext.x, cornerPosY ? ext.y : -ext.y };
+        const V2 q = s0 + dir * std::fmin(std::fmax(dot(wc - s0, dir), 0.0f), length);
+        normal = vertexNormal;
+        depth = radius - vertexSeparation;
+        if (vertexSeparation > 0.0f) {
+            const V2 d = q - wc;
+            const float distSquared = dot(d, d);
+            if (distSquared >= radius * radius) return false;
+            if (distSquared >= ZERO_DIV_SQ) {
+                const float distance = std::sqrt(distSquared);
+                normal = d * (1.0f / distance);
+                depth = radius - distance;
+            }
+        }
+        const bool foldedPosX = swap ? cornerPosY : cornerPosX;
+        const bool foldedPosY = swap ? cornerPosX : cornerPosY;
+        const bool realPosX = foldedPosX != (sx < 0.0f);
+        const bool realPosY = foldedPosY != (sy < 0.0f);
+        const uint32_t vertexId = (realPosY ? 2u : 0u) | ((realPosX != realPosY) ? 1u : 0u);
+        contactCount = 1;
+        point0 = q - normal * radius;
+        id0 = CAPSULE_VERTEX_CONTACT_TAG | vertexId;
+    }
+    normal = worldX * normal.x + worldY * normal.y;
+    point0 = positionA + worldX * point0.x + worldY * point0.y;
+    if (contactCount > 1) point1 = positionA + worldX * point1.x + worldY * point1.y;
+    out = makeManifold(a, b, normal, depth, contactCount, point0, point1, id0, id1);
+    return true;
+}
+
+// CollisionMethods/PolygonCapsule.cpp:27-240.
+bool polygonCapsule(const World& w, uint32_t a, uint32_t b, Manifold& out) {
+    const V2* vertsPtr = &w.verts[w.polyStart[a]];
+    const V2* normalsPtr = &w.normals[w.polyStart[a]];
+    const uint32_t vertexCount = w.polyCount[a];
+    const V2 positionA{ w.wX[a], w.wY[a] }, rightA{ w.wCos[a], w.wSin[a] };
+    const V2 positionB{ w.wX[b], w.wY[b] }, axisB{ w.wCos[b], w.wSin[b] };
+    const float halfLengthB = w.p0[b], radius = w.p1[b];
+    const V2 upA{ -rightA.y, rightA.x };
+    const V2 centerDelta = positionB - positionA;
+    const V2 centerLocal{ dot(centerDelta, rightA), dot(centerDelta, upA) };
+    const V2 axis{ rightA.x * axisB.x + rightA.y * axisB.y, rightA.x * axisB.y - rightA.y * axisB.x };
+    const V2 halfCore = axis * halfLengthB;
+    const V2 s0 = centerLocal - halfCore, s1 = centerLocal + halfCore;
+    const float length = halfLengthB * 2.0f;
+    const V2 sideAxis{ -axis.y, axis.x };
+    float faceSeparation = -FLT_MAX;
+    uint32_t faceIndex = 0;
+    float minSide = FLT_MAX, maxSide = -FLT_MAX;
+    uint32_t minSideVertex = 0, maxSideVertex = 0;
+    for (uint32_t i = 0; i < vertexCount; i++) {
+        const V2 v = vertsPtr[i], n = normalsPtr[i];
+        const float separation = std::fmin(dot(n, s0 - v), dot(n, s1 - v));
+        if (separation >= radius) return false;
+        if (separation > faceSeparation) { faceSeparation = separation; faceIndex = i; }
+        const float side = dot(v, sideAxis);
+        if (side < minSide) { minSide = side; minSideVertex = i; }
+        if (side > maxSide) { maxSide = side; maxSideVertex = i; }
+    }
+    const float sideOffset = dot(sideAxis, centerLocal);
+    const float sidePlus = sideOffset - maxSide;
+    const float sideMinus = minSide - sideOffset;
+    const float sideSeparation = std::fmax(sidePlus, sideMinus);
+    if (sideSeparation >= radius) return false;
+    auto coreDistSquared = [&](const V2& p) {
+        const V2 d = s0 + axis * std::fmin(std::fmax(dot(p - s0, axis), 0.0f), length) - p;
+        return dot(d, d);
+    };
+    V2 normal{ 0, 0 };
+    float depth = 0.0f;
+    uint32_t contactCount = 0;
+    V2 point0{ 0, 0 }, point1{ 0, 0 };
+    uint32_t id0 = 0, id1 = 0;
+    bool faceContact = false;
+    uint32_t vertexId = 0;
+    V2 vertexNormal{ 0, 0 };
+    float vertexSeparation = 0.0f;
+    do {
+        if (sideSeparation > faceSeparation + SAT_EPS) {
+            const bool onPlusSide = sidePlus >= sideMinus;
+            vertexId = onPlusSide ? maxSideVertex : minSideVertex;
+            vertexNormal = onPlusSide ? sideAxis : -sideAxis;
+            vertexSeparation = sideSeparation;
+            break;
+        }
+        const uint32_t i0 = faceIndex, i1 = nextIdx(i0, vertexCount);
+        const V2 va = vertsPtr[i0], vb = vertsPtr[i1], n = normalsPtr[i0];
+        const V2 tangent{ -n.y, n.x };
+        vertexNormal = n;
+        vertexSeparation = faceSeparation;
+        V2 c0 = s0, c1 = s1;
+        float e0 = dot(s0 - va, n), e1 = dot(s1 - va, n);
+        if (faceSeparation > 0.0f) {
+            const float edgeLength = dot(vb - va, tangent);
+            const float t0 = dot(s0 - va, tangent), t1 = dot(s1 - va, tangent);
+            const bool beyondStart = std::fmin(t0, t1) < 0.0f;
+            const bool beyondEnd = std::fmax(t0, t1) > edgeLength;
+            if (beyondStart || beyondEnd) {
+                if (clipSegment(c0, c1, va, tangent)) { vertexId = i0; break; }
+                if (clipSegment(c0, c1, vb, -tangent)) { vertexId = i1; break; }
+                e0 = dot(c0 - va, n);
+                e1 = dot(c1 - va, n);
+                const float minE = std::fmin(e0, e1);
+                const float faceDistSquared = minE * minE;
+                if (beyondStart && coreDistSquared(va) + SAT_EPS < faceDistSquared) { vertexId = i0; break; }
+                if (beyondEnd && coreDistSquared(vb) + SAT_EPS < faceDistSquared) { vertexId = i1; break; }
+            }
+        }
+        const V2 span = c1 - c0;
+        const bool hasSecond = dot(span, span) >= ZERO_DIV_SQ;
+        const float contactLimit = radius + SAT_EPS;
+        const uint32_t idBase = REF_TAG | (i0 << 1);
+        if (e0 <= contactLimit) { point0 = c0 - n * radius; id0 = idBase; contactCount = 1; }
+        if (hasSecond && e1 <= contactLimit) {
+            const V2 p = c1 - n * radius;
+            if (contactCount == 0) { point0 = p; id0 = idBase | 1u; }
+            else { point1 = p; id1 = idBase | 1u; }
+            contactCount++;
+        }
+        if (contactCount == 0) return false;
+        normal = n;
+        depth = radius - std::fmin(e0, e1);
+        faceContact = true;
+    } while (false);
+    if (!faceContact) {
+        const V2 wv = vertsPtr[vertexId];
+        const V2 q = s0 + axis * std::fmin(std::fmax(dot(wv - s0, axis), 0.0f), length);
+        normal = vertexNormal;
+        depth = radius - vertexSeparation;
+        if (vertexSeparation > 0.0f) {
+            const V2 d = q - wv;
+            const float distSquared = dot(d, d);
+            if (distSquared >= radius * radius) return false;
+            if (distSquared >= ZERO_DIV_SQ) {
+                const float distance = std::sqrt(distSquared);
+                normal = d * (1.0f / distance);
+                depth = radius - distance;
+            }
+        }
+        contactCount = 1;
+        point0 = q - normal * radius;
+        id0 = CAPSULE_VERTEX_CONTACT_TAG | vertexId;
+    }
+    normal = rightA * normal.x + upA * normal.y;
+    point0 = positionA + rightA * point0.x + upA * point0.y;
+    if (contactCount > 1) point1 = positionA + rightA * point1.x + upA * point1.y;
+    out = makeManifold(a, b, normal, depth, contactCount, point0, point1, id0, id1);
+    return true;
+}
+
+// SymmetricMatrix.h:14-18 storage index of (typeA <= typeB): CC, CB, BB, CP, BP, PP, CK, BK, PK, KK
+// (K: capsule).
 inline uint32_t typeSlot(uint32_t lo, uint32_t hi) { return hi * (hi + 1) / 2 + lo; }
 
 bool collide(const World& w, uint32_t a, uint32_t b, Manifold& out) {
@@ -456,7 +812,11 @@ bool collide(const World& w, uint32_t a, uint32_t b, Manifold& out) {
     case 2: return boxBox(w, a, b, out);
     case 3: return circlePolygon(w, a, b, out);
     case 4: return boxPolygon(w, a, b, out);
-    default: return polygonPolygon(w, a, b, out);
+    case 5: return polygonPolygon(w, a, b, out);
+    case 6: return circleCapsule(w, a, b, out);
+    case 7: return boxCapsule(w, a, b, out);
+    case 8: return polygonCapsule(w, a, b, out);
+    default: return capsuleCapsule(w, a, b, out);
     }
 }
 
@@ -466,7 +826,7 @@ bool collide(const World& w, uint32_t a, uint32_t b, Manifold& out) {
 // type buckets dispatched in symmetric-matrix order.
 void narrowPhase(World& w, const uint32_t* pa, const uint32_t* pb, size_t n, std::vector<Manifold>& out) {
     out.clear();
-    std::vector<std::pair<uint32_t, uint32_t>> buckets[6];
+    std::vector<std::pair<uint32_t, uint32_t>> buckets[10];
     for (size_t start = 0; start < n; start += 1024) {
         const size_t end = std::min(start + 1024, n);
         for (auto& bk : buckets) bk.clear();

@@ -30,7 +30,7 @@ inline V2 operator*(V2 a, float s) { return { a.x * s, a.y * s }; }
 inline V2 operator*(float s, V2 a) { return { s * a.x, s * a.y }; }
 inline float dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
 
-enum ShapeType : uint8_t { SHAPE_CIRCLE = 0, SHAPE_BOX = 1, SHAPE_POLYGON = 2 };
+enum ShapeType : uint8_t { SHAPE_CIRCLE = 0, SHAPE_BOX = 1, SHAPE_POLYGON = 2, SHAPE_CAPSULE = 3 };   // Types/Types.h:35-42
 
 struct Material { float elasticity = 1.0f, staticFriction = 0.0f, dynamicFriction = 0.0f; };
 
@@ -109,6 +109,11 @@ uint32_t createBox(World& w, float px, float py, float rot, float vx, float vy, 
                    float mass, uint32_t material, float width, float height);
 uint32_t createPolygon(World& w, float px, float py, float rot, float vx, float vy, float av,
                        float mass, uint32_t material, const V2* localVerts, uint32_t count);
+// Capsule of full core length `length` along its local +x and radius `radius` (p0 = half length,
+// p1 = radius of its collider).
+uint32_t createCapsule(World& w, float px, float py, float rot, float vx, float vy, float av,
+                       float mass, uint32_t material, float length, float radius);
+float capsuleInertia(float mass, float length, float radius);   // PhysicsGeometry.cpp:87-117 (centre of mass 0)
 
 // ---- phases ----
 void fastCosSin(float x, float& c, float& s);                 // FastCosSin.cpp:26-109

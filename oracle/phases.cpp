@@ -102,7 +102,7 @@ void colliderPoses(World& w, bool all) {
     }
 }
 
-// ColliderAABBBuilder.cpp:8-136.
+// ColliderAABBBuilder.cpp:8-175.
 void colliderAABBs(World& w, bool all) {
     const uint32_t begin = all ? 0 : w.staticColliders;
     for (uint32_t i = begin; i < w.colliderCount(); i++) {
@@ -110,6 +110,10 @@ void colliderAABBs(World& w, bool all) {
         if (w.colType[i] == SHAPE_CIRCLE) {
             const float r = w.p0[i];
             w.minX[i] = x - r; w.minY[i] = y - r; w.maxX[i] = x + r; w.maxY[i] = y + r;
+        } else if (w.colType[i] == SHAPE_CAPSULE) {   // :138-175
+            const float ex = std::fabs(w.wCos[i]) * w.p0[i] + w.p1[i];
+            const float ey = std::fabs(w.wSin[i]) * w.p0[i] + w.p1[i];
+            w.minX[i] = x - ex; w.minY[i] = y - ey; w.maxX[i] = x + ex; w.maxY[i] = y + ey;
         } else if (w.colType[i] == SHAPE_BOX) {
             const float ac = std::fabs(w.wCos[i]), as = std::fabs(w.wSin[i]);
             const float ex = ac * w.p0[i] + as * w.p1[i];
@@ -213,6 +217,19 @@ float castPolygon(V2 q0, V2 dl, float dd, const V2* vertices, const V2* normals,
     return castCircle(q0.x - corner.x, q0.y - corner.y, dl.x, dl.y, dd, radius, tExit);
 }
 
+// :222-240.
+float castCapsule(V2 q0, V2 dl, float dd, float halfLength, float capsuleRadius, float radius, float best) {
+    const float inflated = capsuleRadius + radius;
+    const float nvy = -std::copysign(1.0f, q0.y) * dl.y;
+    const float gap = std::fabs(q0.y) - inflated;
+    if (nvy > 0.0f && gap >= 0.0f && gap < best * nvy) {
+        const float time = gap / nvy;
+        if (std::fabs(q0.x + dl.x * time) <= halfLength) return time;
+    }
+    const float t = std::fmin(castCircle(q0.x + halfLength, q0.y, dl.x, dl.y, dd, inflated, best), best);
+    return std::fmin(castCircle(q0.x - halfLength, q0.y, dl.x, dl.y, dd, inflated, t), t);
+}
+
 // :243-273.
 float castCore(const World& w, uint32_t col, V2 start, V2 d, float dd, float coreRadius, float best) {
     const float cs = w.wCos[col], sn = w.wSin[col];
@@ -222,6 +239,7 @@ float castCore(const World& w, uint32_t col, V2 start, V2 d, float dd, float cor
     switch (w.colType[col]) {
     case SHAPE_CIRCLE: return castCircle(q0.x, q0.y, dl.x, dl.y, dd, w.p0[col] + coreRadius, best);
     case SHAPE_BOX: return castBox(q0, dl, dd, w.p0[col], w.p1[col], coreRadius, best);
+    case SHAPE_CAPSULE: return castCapsule(q0, dl, dd, w.p0[col], w.p1[col], coreRadius, best);
     default: return castPolygon(q0, dl, dd, &w.verts[w.polyStart[col]], &w.normals[w.polyStart[col]],
                                 w.polyCount[col], coreRadius, best);
     }

@@ -23,7 +23,7 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "libagx.so"
 MAX_SWEEPS = 256   # AGX_MAX_SWEEPS
 
-SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON = 0, 1, 2
+SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON, SHAPE_CAPSULE = 0, 1, 2, 3
 
 (PHASE_INTEGRATE, PHASE_POSE, PHASE_CCD, PHASE_BROAD, PHASE_NARROW, PHASE_SOLVE,
  PHASE_ISLANDS, PHASE_FINISH) = range(8)

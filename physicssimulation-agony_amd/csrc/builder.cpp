@@ -216,4 +216,27 @@ uint32_t SceneData::createPolygon(float px, float py, float r, float vx, float v
     return b;
 }
 
+// ObjectManager.cpp:351-389 + :197-230; the mass splits between the core's rectangle and the two
+// end caps by area (PhysicsGeometry.cpp:87-117): the rectangle about its centre, each cap about its
+// own centroid shifted to the capsule centre.
+uint32_t SceneData::createCapsule(float px, float py, float r, float vx, float vy, float av, float massIn,
+                                  uint32_t material, float lengthIn, float radiusIn) {
+    const float m = std::fmax(0.0f, massIn);
+    const float len = std::fmax(0.0f, lengthIn), rad = std::fmax(0.0f, radiusIn);
+    const float pi = (float)3.14159265358979323846;
+    const float coreArea = len * 2.0f * rad, capArea = pi * rad * rad;
+    const float area = coreArea + capArea;
+    const float coreMass = m * coreArea / area, capMass = m * capArea / area;
+    const float width = 2.0f * rad;
+    const float coreInertia = coreMass * (len * len + width * width) / 12.0f;
+    const float capCentroid = 4.0f * rad / (3.0f * pi);
+    const float capInertia = capMass * (0.5f * rad * rad + 0.25f * len * len + len * capCentroid);
+    const float inertiaV = coreInertia + capInertia + m * 0.0f;
+    const uint32_t b = appendBody({ px, py }, { vx, vy }, r, av, m, m == 0.0f ? 0.0f : 1.0f / m, inertiaV,
+                                  inertiaV == 0.0f ? 0.0f : 1.0f / inertiaV, { 0.0f, 0.0f });
+    const float half = len * 0.5f;
+    placeCollider(appendCollider(b, { 0.0f, 0.0f }, 0.0f, material, 3, half, rad, 0, 0), rad, half + rad);
+    return b;
+}
+
 }  // namespace agx

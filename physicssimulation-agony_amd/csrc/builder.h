@@ -33,6 +33,8 @@ struct SceneData {
                        uint32_t material, float width, float height);
     uint32_t createPolygon(float px, float py, float rot, float vx, float vy, float av, float mass,
                            uint32_t material, const Vec* verts, uint32_t count);
+    uint32_t createCapsule(float px, float py, float rot, float vx, float vy, float av, float mass,
+                           uint32_t material, float length, float radius);
     uint32_t materialCount = 1;   // set by the builder owner so out-of-range materials map to 0
 
 private:

@@ -36,12 +36,17 @@ __global__ __launch_bounds__(256) void kIntegrate(float4* __restrict__ pos, floa
     }
 }
 
-// ColliderAABBBuilder.cpp:8-136.
+// ColliderAABBBuilder.cpp:8-175.
 __device__ inline float4 colliderAabb(uint4 sh, float2 dims, float4 ps, const float2* __restrict__ verts) {
     const float x = ps.x, y = ps.y;
     if (sh.x == AGX_SHAPE_CIRCLE) {
         const float r = dims.x;
         return make_float4(x - r, y - r, x + r, y + r);
+    }
+    if (sh.x == AGX_SHAPE_CAPSULE) {   // the core's half length along each axis, grown by the radius
+        const float ex = fabsf(ps.z) * dims.x + dims.y;
+        const float ey = fabsf(ps.w) * dims.x + dims.y;
+        return make_float4(x - ex, y - ey, x + ex, y + ey);
     }
     if (sh.x == AGX_SHAPE_BOX) {
         const float ac = fabsf(ps.z), as = fabsf(ps.w);
@@ -202,6 +207,24 @@ __host__ __device__ inline float sweepRoundedBox(float qx, float qy, float lx, f
     return t;
 }
 
+// Capsule of core half length hl along x grown by rho (R = capsule radius + rho): a stadium, the
+// union of the slab |y| <= R over |x| <= hl and the discs of radius R at both core ends. The sweep
+// enters through the slab face on the start's side when it reaches that face within the core's span
+// (a start on or beyond the face, approaching it); otherwise the earlier of the end discs decides
+// (CCD/ContinuousCollisionDetector.cpp:222-240 behaves so: a start inside the slab is tested
+// against the end discs only).
+__host__ __device__ inline float sweepCapsule(float qx, float qy, float lx, float ly, float mm, float hl, float R,
+                                              float limit) {
+    const float excess = fabsf(qy) - R;                    // start's distance beyond the face on its side
+    const float closing = -copysignf(1.0f, qy) * ly;       // speed towards that face
+    if (closing > 0.0f && excess >= 0.0f) {
+        const float t = excess / closing;
+        if (t < limit && fabsf(qx + lx * t) <= hl) return t;
+    }
+    const float first = sweepDisc(qx + hl, qy, lx, ly, mm, R, limit);
+    return fminf(first, sweepDisc(qx - hl, qy, lx, ly, mm, R, fminf(first, limit)));
+}
+
 // Earliest time of impact (< limit) of the core with centre s, displacement m and radius rho
 // against static collider pose ps (x, y, cos, sin), or NO_HIT. Worked in the collider's frame.
 __host__ __device__ inline float sweepCore(uint4 sh, float2 dims, float4 ps, const float2* __restrict__ verts,
@@ -214,6 +237,7 @@ __host__ __device__ inline float sweepCore(uint4 sh, float2 dims, float4 ps, con
     const float lx = c * mx + sn * my, ly = c * my - sn * mx;
     if (sh.x == AGX_SHAPE_CIRCLE) return sweepDisc(qx, qy, lx, ly, mm, dims.x + rho, limit);
     if (sh.x == AGX_SHAPE_BOX) return sweepRoundedBox(qx, qy, lx, ly, mm, dims.x, dims.y, rho, limit);
+    if (sh.x == AGX_SHAPE_CAPSULE) return sweepCapsule(qx, qy, lx, ly, mm, dims.x, dims.y + rho, limit);
     const float2* v = verts + sh.y;
     const float2* nr = normals + sh.y;
     return sweepOutline(qx, qy, lx, ly, mm, rho, limit, sh.z, [=](uint32_t i) { return v[i]; },

@@ -437,6 +437,352 @@ __device__ bool boxPolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold
     return true;
 }
 
+// ---------------------------------------------------------------- capsules
+// A capsule is its core segment (half length hl along its local +x) grown by its radius. The
+// manifolds keep the reference's operation order, contact ids and tie-breaks, for the same bits.
+constexpr uint32_t CAPSULE_VERTEX_TAG = 0x40000000u;   // Narrow/Constants.h:40-41
+constexpr float CAPSULE_PARALLEL_SIN2 = 1e-3f;
+
+// Point on the core segment from s0 along unit dir (length len) nearest to w.
+__device__ inline V coreNearest(V s0, V dir, float len, V w) {
+    return s0 + dir * fminf(fmaxf(dot(w - s0, dir), 0.0f), len);
+}
+
+// CircleCapsule.cpp:92-147 (its batched path): circle A, capsule B.
+__device__ bool circleCapsule(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V centre{ pa.x, pa.y };
+    const float rA = s.dims[a].x;
+    const float2 cap = s.dims[b];
+    const float c = pb.z, sn = pb.w;
+    const V off{ pa.x - pb.x, pa.y - pb.y };
+    // nearest core point, along the axis (clamped with selects, as the batched path does)
+    const float along = off.x * c + off.y * sn, lo = 0.0f - cap.x;
+    float u = along < lo ? lo : along;
+    u = cap.x < u ? cap.x : u;
+    const V gap{ c * u - off.x, sn * u - off.y };
+    const float reach = rA + cap.y;
+    const float gg = gap.x * gap.x + gap.y * gap.y;
+    if (!(gg < reach * reach)) return false;
+    const float dist = sqrtf(gg);
+    const float inv = 1.0f / dist;
+    V n;
+    if (dist < ZERO_DIV) {   // centre on the core: out of the capsule's side, away from the centre's side
+        const bool below = off.y * c - off.x * sn < 0.0f;
+        n = below ? V{ -sn, c } : V{ sn, -c };
+    } else {
+        n = gap * inv;
+    }
+    emit(out, a, b, n, reach - dist, 1, centre + n * rA, { 0, 0 }, 0, 0);
+    return true;
+}
+
+// CapsuleCapsule.cpp:25-157: the nearest points of the two cores (from their centres); side by
+// side (near-parallel, overlapping along A's axis) one contact at each end of the overlap.
+__device__ bool capsuleCapsule(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const V oA{ pa.x, pa.y }, oB{ pb.x, pb.y };
+    const float2 ca = s.dims[a], cb = s.dims[b];
+    const float hA = ca.x, hB = cb.x, rA = ca.y, rB = cb.y;
+    const V between = oB - oA;
+    const float rsum = rA + rB;
+    const float bound = hA + hB + rsum;
+    if (dot(between, between) >= bound * bound) return false;
+    const V axA{ pa.z, pa.w }, axB{ pb.z, pb.w };
+    const float cosAB = dot(axA, axB);
+    const float alongA = dot(between, axA);
+    const float negAlongA = -alongA;
+    const float negAlongB = -dot(axB, between);
+    const float sin2 = 1.0f - cosAB * cosAB;
+    const bool parallel = sin2 < CAPSULE_PARALLEL_SIN2;
+    const float reachB = fabsf(cosAB) * hB;
+    const float lo = fmaxf(-hA, alongA - reachB), hi = fminf(hA, alongA + reachB);
+    const bool overlap = parallel && (hi - lo >= ZERO_DIV);
+    auto clampTo = [](float v, float h) { return fminf(fmaxf(v, -h), h); };
+    float sa, sb;
+    if (overlap) {
+        sa = 0.5f * (lo + hi);
+        sb = clampTo(cosAB * sa + negAlongB, hB);
+    } else {
+        sa = parallel ? 0.0f : clampTo((cosAB * negAlongB - negAlongA) / sin2, hA);
+        sb = clampTo(cosAB * sa + negAlongB, hB);
+        sa = clampTo(cosAB * sb - negAlongA, hA);
+    }
+    const V nearA = oA + axA * sa, nearB = oB + axB * sb;
+    const V d = nearB - nearA;
+    const float dd = dot(d, d);
+    if (dd >= rsum * rsum) return false;
+    V n;
+    float depth;
+    if (dd >= ZERO_DIV_SQ) {
+        const float dist = sqrtf(dd);
+        n = d * (1.0f / dist);
+        depth = rsum - dist;
+    } else {   // the cores cross: out along the side axis that needs the shorter push
+        const V sideA{ -axA.y, axA.x }, sideB{ -axB.y, axB.x };
+        const float offA = dot(between, sideA), offB = dot(between, sideB);
+        const float pushA = fabsf(dot(axB, sideA)) * hB - fabsf(offA);
+        const float pushB = fabsf(dot(axA, sideB)) * hA - fabsf(offB);
+        if (pushA <= pushB) { n = flipSign(sideA, offA); depth = pushA; }
+        else { n = flipSign(sideB, offB); depth = pushB; }
+        depth += rsum;
+    }
+    V pt[2] = { { 0, 0 }, { 0, 0 } };
+    uint32_t id[2] = { 0, 0 };
+    uint32_t cnt = 0;
+    if (overlap) {
+        const V sideB{ -axB.y, axB.x };
+        const float ends[2] = { lo, hi };
+        float nearest = FLT_MAX;
+        for (uint32_t k = 0; k < 2; k++) {
+            const V p = oA + axA * ends[k];
+            const float g = fabsf(dot(p - oB, sideB));
+            if (g >= rsum) continue;
+            pt[cnt] = p + n * rA;
+            id[cnt] = k;
+            cnt++;
+            nearest = fminf(nearest, g);
+        }
+        if (cnt) depth = rsum - nearest;
+    }
+    if (cnt == 0) {
+        pt[0] = nearA + n * rA;
+        id[0] = 2;
+        cnt = 1;
+    }
+    emit(out, a, b, n, depth, cnt, pt[0], pt[1], id[0], id[1]);
+    return true;
+}
+
+// Contacts of a core segment (c0, c1 in the reference face's frame, the face's outward normal
+// along n) with a face: a point of each end within reach, on the capsule surface.
+struct FaceContacts { V p[2]; uint32_t id[2]; uint32_t n; };
+__device__ inline FaceContacts faceContacts(V c0, V c1, float e0, float e1, float radius, V n, uint32_t idBase) {
+    FaceContacts f{ { { 0, 0 }, { 0, 0 } }, { 0, 0 }, 0 };
+    const V span = c1 - c0;
+    const bool second = dot(span, span) >= ZERO_DIV_SQ;
+    const float reach = radius + SAT_EPS;
+    if (e0 <= reach) { f.p[0] = c0 - n * radius; f.id[0] = idBase; f.n = 1; }
+    if (second && e1 <= reach) { f.p[f.n] = c1 - n * radius; f.id[f.n] = idBase | 1u; f.n++; }
+    return f;
+}
+
+// BoxCapsule.cpp:26-242: box A, capsule B, in A's frame folded into the (+, +) quadrant; the
+// reference face is turned to x = ext.x.
+__device__ bool boxCapsule(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const float2 half = s.dims[a], cap = s.dims[b];
+    const V oA{ pa.x, pa.y }, xA{ pa.z, pa.w }, axB{ pb.z, pb.w };
+    const float hl = cap.x, radius = cap.y;
+    const V yA{ -xA.y, xA.x };
+    const V between = V{ pb.x, pb.y } - oA;
+    const float lx = dot(between, xA), ly = dot(between, yA);
+    const float fx = copysignf(1.0f, lx), fy = copysignf(1.0f, ly);
+    const V mid0{ fabsf(lx), fabsf(ly) };
+    const V ax0{ (xA.x * axB.x + xA.y * axB.y) * fx, (xA.x * axB.y - xA.y * axB.x) * fy };
+    const float aax = fabsf(ax0.x), aay = fabsf(ax0.y);
+    const float sepX = mid0.x - hl * aax - half.x;
+    const float sepY = mid0.y - hl * aay - half.y;
+    const float faceSep = fmaxf(sepX, sepY);
+    if (faceSep >= radius) return false;
+    const V side{ -ax0.y, ax0.x };
+    const float sideOff = dot(side, mid0);
+    const float sideSep = fabsf(sideOff) - (half.x * aay + half.y * aax);
+    if (sideSep >= radius) return false;
+    const bool sideWins = sideSep > faceSep + SAT_EPS;
+    const bool turn = !sideWins && sepY > sepX;
+    const V ext = turn ? V{ half.y, half.x } : V{ half.x, half.y };
+    const V dir = turn ? V{ ax0.y, ax0.x } : ax0;
+    const V mid = turn ? V{ mid0.y, mid0.x } : mid0;
+    const V bx = xA * fx, by = yA * fy;
+    const V wx = turn ? by : bx, wy = turn ? bx : by;
+    const V hc = dir * hl;
+    const V s0 = mid - hc, s1 = mid + hc;
+    const float len = hl * 2.0f;
+    auto coreDist2 = [&](V w) { const V d = coreNearest(s0, dir, len, w) - w; return dot(d, d); };
+
+    V n{ 1, 0 };
+    float depth = 0.0f;
+    FaceContacts fc{ { { 0, 0 }, { 0, 0 } }, { 0, 0 }, 0 };
+    bool cornerX = true, cornerY = true;   // the corner a vertex contact takes (folded, turned frame)
+    V cornerNormal{ 1, 0 };
+    float cornerSep = faceSep;
+    bool face = false;
+    if (sideWins) {
+        cornerNormal = sideOff >= 0.0f ? side : -side;
+        cornerSep = sideSep;
+        cornerX = cornerNormal.x >= 0.0f;
+        cornerY = cornerNormal.y >= 0.0f;
+    } else {
+        V c0 = s0, c1 = s1;
+        float e0 = s0.x - ext.x, e1 = s1.x - ext.x;
+        bool corner = false;
+        if (faceSep > 0.0f) {   // cores apart: only the part of the core beside the face sees it
+            const float yLo = fminf(s0.y, s1.y), yHi = fmaxf(s0.y, s1.y);
+            if (yLo > ext.y) {
+                corner = true;
+            } else {
+                const bool under = yLo < -ext.y, over = yHi > ext.y;
+                if (under || over) {
+                    const float invDy = 1.0f / (s1.y - s0.y);
+                    const float tA = (-ext.y - s0.y) * invDy, tB = (ext.y - s0.y) * invDy;
+                    const float t0 = fmaxf(0.0f, fminf(tA, tB)), t1 = fminf(1.0f, fmaxf(tA, tB));
+                    const V sp = s1 - s0;
+                    c0 = s0 + sp * t0;
+                    c1 = s0 + sp * t1;
+                    e0 = c0.x - ext.x;
+                    e1 = c1.x - ext.x;
+                    const float eMin = fminf(e0, e1);
+                    const float faceDist2 = eMin * eMin;
+                    if (under && coreDist2(V{ ext.x, -ext.y }) + SAT_EPS < faceDist2) { corner = true; cornerY = false; }
+                    else if (over && coreDist2(ext) + SAT_EPS < faceDist2) corner = true;
+                }
+            }
+        }
+        if (!corner) {
+            const uint32_t faceId = turn ? (fy < 0.0f ? 0u : 2u) : (fx < 0.0f ? 3u : 1u);
+            fc = faceContacts(V{ c0.x, c0.y }, V{ c1.x, c1.y }, e0, e1, radius, V{ 1, 0 }, REF_TAG | (faceId << 1));
+            if (fc.n == 0) return false;
+            depth = radius - fminf(e0, e1);
+            face = true;
+        }
+    }
+    if (!face) {   // one contact: a box corner against the core
+        const V w{ cornerX ? ext.x : -ext.x, cornerY ? ext.y : -ext.y };
+        const V q = coreNearest(s0, dir, len, w);
+        n = cornerNormal;
+        depth = radius - cornerSep;
+        if (cornerSep > 0.0f) {   // cores apart: the corner's distance decides
+            const V d = q - w;
+            const float dd = dot(d, d);
+            if (dd >= radius * radius) return false;
+            if (dd >= ZERO_DIV_SQ) {
+                const float dist = sqrtf(dd);
+                n = d * (1.0f / dist);
+                depth = radius - dist;
+            }
+        }
+        // the corner in the box's own frame (turn undone, then the fold); order (-,-) (+,-) (+,+) (-,+)
+        const bool px = (turn ? cornerY : cornerX) != (fx < 0.0f);
+        const bool py = (turn ? cornerX : cornerY) != (fy < 0.0f);
+        fc.n = 1;
+        fc.p[0] = q - n * radius;
+        fc.id[0] = CAPSULE_VERTEX_TAG | ((py ? 2u : 0u) | (px != py ? 1u : 0u));
+    }
+    const V nW = wx * n.x + wy * n.y;
+    const V p0 = oA + wx * fc.p[0].x + wy * fc.p[0].y;
+    const V p1 = fc.n > 1 ? oA + wx * fc.p[1].x + wy * fc.p[1].y : V{ 0, 0 };
+    emit(out, a, b, nW, depth, fc.n, p0, p1, fc.id[0], fc.id[1]);
+    return true;
+}
+
+// PolygonCapsule.cpp:27-240: polygon A, capsule B, in A's frame.
+__device__ bool polygonCapsule(const Shapes& s, uint32_t a, uint32_t b, agx_manifold& out) {
+    const float4 pa = s.pose[a], pb = s.pose[b];
+    const uint4 sh = s.shape[a];
+    const float2* vs = s.verts + sh.y;
+    const float2* ns = s.normals + sh.y;
+    const uint32_t count = sh.z;
+    const float2 cap = s.dims[b];
+    const V oA{ pa.x, pa.y }, xA{ pa.z, pa.w }, axB{ pb.z, pb.w };
+    const float hl = cap.x, radius = cap.y;
+    const V yA{ -xA.y, xA.x };
+    const V between = V{ pb.x, pb.y } - oA;
+    const V mid{ dot(between, xA), dot(between, yA) };
+    const V dir{ xA.x * axB.x + xA.y * axB.y, xA.x * axB.y - xA.y * axB.x };
+    const V hc = dir * hl;
+    const V s0 = mid - hc, s1 = mid + hc;
+    const float len = hl * 2.0f;
+    const V side{ -dir.y, dir.x };
+    float faceSep = -FLT_MAX, sideLo = FLT_MAX, sideHi = -FLT_MAX;
+    uint32_t face = 0, loVertex = 0, hiVertex = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const V v = ld(vs, i), nv = ld(ns, i);
+        const float sep = fminf(dot(nv, s0 - v), dot(nv, s1 - v));
+        if (sep >= radius) return false;
+        if (sep > faceSep) { faceSep = sep; face = i; }
+        const float sd = dot(v, side);
+        if (sd < sideLo) { sideLo = sd; loVertex = i; }
+        if (sd > sideHi) { sideHi = sd; hiVertex = i; }
+    }
+    const float sideOff = dot(side, mid);
+    const float abovePlus = sideOff - sideHi, belowMinus = sideLo - sideOff;
+    const float sideSep = fmaxf(abovePlus, belowMinus);
+    if (sideSep >= radius) return false;
+    auto coreDist2 = [&](V w) { const V d = coreNearest(s0, dir, len, w) - w; return dot(d, d); };
+
+    V n{ 0, 0 };
+    float depth = 0.0f;
+    FaceContacts fc{ { { 0, 0 }, { 0, 0 } }, { 0, 0 }, 0 };
+    uint32_t vertex = 0;
+    V vertexNormal{ 0, 0 };
+    float vertexSep = 0.0f;
+    bool faceHit = false;
+    if (sideSep > faceSep + SAT_EPS) {   // a vertex pokes the capsule's side
+        const bool plus = abovePlus >= belowMinus;
+        vertex = plus ? hiVertex : loVertex;
+        vertexNormal = plus ? side : -side;
+        vertexSep = sideSep;
+    } else {
+        const uint32_t i0 = face, i1 = nextIdx(i0, count);
+        const V va = ld(vs, i0), vb = ld(vs, i1), nf = ld(ns, i0);
+        const V tg{ -nf.y, nf.x };
+        vertexNormal = nf;
+        vertexSep = faceSep;
+        V c0 = s0, c1 = s1;
+        float e0 = dot(s0 - va, nf), e1 = dot(s1 - va, nf);
+        bool atVertex = false;
+        if (faceSep > 0.0f) {   // cores apart: an overhanging part of the core may be nearer an end vertex
+            const float edgeLen = dot(vb - va, tg);
+            const float t0 = dot(s0 - va, tg), t1 = dot(s1 - va, tg);
+            const bool beforeStart = fminf(t0, t1) < 0.0f, pastEnd = fmaxf(t0, t1) > edgeLen;
+            if (beforeStart || pastEnd) {
+                if (clipSegment(c0, c1, va, tg)) { vertex = i0; atVertex = true; }
+                else if (clipSegment(c0, c1, vb, -tg)) { vertex = i1; atVertex = true; }
+                else {
+                    e0 = dot(c0 - va, nf);
+                    e1 = dot(c1 - va, nf);
+                    const float eMin = fminf(e0, e1);
+                    const float faceDist2 = eMin * eMin;
+                    if (beforeStart && coreDist2(va) + SAT_EPS < faceDist2) { vertex = i0; atVertex = true; }
+                    else if (pastEnd && coreDist2(vb) + SAT_EPS < faceDist2) { vertex = i1; atVertex = true; }
+                }
+            }
+        }
+        if (!atVertex) {
+            fc = faceContacts(c0, c1, e0, e1, radius, nf, REF_TAG | (i0 << 1));
+            if (fc.n == 0) return false;
+            n = nf;
+            depth = radius - fminf(e0, e1);
+            faceHit = true;
+        }
+    }
+    if (!faceHit) {   // one contact: polygon vertex against the core
+        const V w = ld(vs, vertex);
+        const V q = coreNearest(s0, dir, len, w);
+        n = vertexNormal;
+        depth = radius - vertexSep;
+        if (vertexSep > 0.0f) {
+            const V d = q - w;
+            const float dd = dot(d, d);
+            if (dd >= radius * radius) return false;
+            if (dd >= ZERO_DIV_SQ) {
+                const float dist = sqrtf(dd);
+                n = d * (1.0f / dist);
+                depth = radius - dist;
+            }
+        }
+        fc.n = 1;
+        fc.p[0] = q - n * radius;
+        fc.id[0] = CAPSULE_VERTEX_TAG | vertex;
+    }
+    const V nW = xA * n.x + yA * n.y;
+    const V p0 = oA + xA * fc.p[0].x + yA * fc.p[0].y;
+    const V p1 = fc.n > 1 ? oA + xA * fc.p[1].x + yA * fc.p[1].y : V{ 0, 0 };
+    emit(out, a, b, nW, depth, fc.n, p0, p1, fc.id[0], fc.id[1]);
+    return true;
+}
+
 // NarrowPhaseCollisionDetector.cpp:653-667 canonical order, then SymmetricMatrix dispatch.
 // A block takes up to NARROW_CHUNK pairs at a time, bins them in LDS by shape combination and then
 // runs the bins in order, so the lanes of a wave mostly take the same collision method (a wave runs
@@ -446,12 +792,13 @@ __device__ bool boxPolygon(const Shapes& s, uint32_t a, uint32_t b, agx_manifold
 constexpr uint32_t NARROW_CHUNK = 2048;
 constexpr uint32_t NARROW_PER_LANE = NARROW_CHUNK / 256;
 constexpr uint32_t NARROW_BUSY_BLOCKS = 1024;
-constexpr uint32_t NO_METHOD = 7u;
+constexpr uint32_t METHODS = 10;   // shape pairs (A type <= B type) of the four shapes
+constexpr uint32_t NO_METHOD = 15u;
 __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, Counters* __restrict__ ctr, uint64_t cap,
                                                Shapes s, agx_manifold* __restrict__ slots, uint32_t* __restrict__ flags) {
     __shared__ uint2 binAB[NARROW_CHUNK];       // canonical (A, B) of the chunk's pairs, by method
-    __shared__ uint32_t binP[NARROW_CHUNK];     // pair index | method << 29
-    __shared__ uint32_t count[8], start[8];
+    __shared__ uint32_t binP[NARROW_CHUNK];     // pair index | method << 28
+    __shared__ uint32_t count[16], start[16];
     const uint32_t n = (uint32_t)min((uint64_t)ctr->pairs, cap);
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr->scratch[1] = n;   // the pairs the flag scan covers
     const uint32_t t = threadIdx.x, lane = t & (WAVE - 1);
@@ -460,7 +807,7 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
     const uint32_t perLane = max(1u, min(NARROW_PER_LANE, (n + busy * 256 - 1) / (busy * 256)));
     const uint32_t chunk = 256 * perLane;
     for (uint32_t base = blockIdx.x * chunk; base < n; base += gridDim.x * chunk) {   // block-uniform
-        if (t < 8) count[t] = 0u;
+        if (t < 16) count[t] = 0u;
         __syncthreads();
         uint32_t method[NARROW_PER_LANE], rank[NARROW_PER_LANE];
         uint2 ab[NARROW_PER_LANE];
@@ -474,12 +821,12 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
                 uint32_t ta = s.shape[a].x, tb = s.shape[b].x;
                 if (ta > tb) { const uint32_t x = a; a = b; b = x; const uint32_t u = ta; ta = tb; tb = u; }
                 else if (ta == tb && b > a) { const uint32_t x = a; a = b; b = x; }
-                method[r] = tb * (tb + 1) / 2 + ta;   // 0..5
+                method[r] = tb * (tb + 1) / 2 + ta;   // 0..METHODS-1
                 ab[r] = make_uint2(a, b);
             }
             // rank within the method's bin: one LDS atomic per method present in the wave
             rank[r] = 0u;
-            for (uint32_t c = 0; c < 6; c++) {
+            for (uint32_t c = 0; c < METHODS; c++) {
                 const uint64_t m = __ballot(method[r] == c);
                 if (!m) continue;
                 const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
@@ -492,8 +839,8 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
         __syncthreads();
         if (t == 0) {
             uint32_t run = 0;
-            for (uint32_t c = 0; c < 6; c++) { start[c] = run; run += count[c]; }
-            start[6] = run;
+            for (uint32_t c = 0; c < METHODS; c++) { start[c] = run; run += count[c]; }
+            start[METHODS] = run;
         }
         __syncthreads();
 #pragma unroll
@@ -501,14 +848,14 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
             if (method[r] == NO_METHOD) continue;
             const uint32_t e = start[method[r]] + rank[r];
             binAB[e] = ab[r];
-            binP[e] = (base + r * 256 + t) | (method[r] << 29);
+            binP[e] = (base + r * 256 + t) | (method[r] << 28);
         }
         __syncthreads();
-        const uint32_t total = start[6];
+        const uint32_t total = start[METHODS];
         for (uint32_t e = t; e < total; e += 256) {
             const uint2 pab = binAB[e];
             const uint32_t pm = binP[e];
-            const uint32_t p = pm & ((1u << 29) - 1u), c = pm >> 29;
+            const uint32_t p = pm & ((1u << 28) - 1u), c = pm >> 28;
             agx_manifold m;
             bool hit;
             switch (c) {
@@ -517,7 +864,11 @@ __global__ __launch_bounds__(256) void kNarrow(const uint2* __restrict__ pairs, 
             case 2: hit = boxBox(s, pab.x, pab.y, m); break;
             case 3: hit = circlePolygon(s, pab.x, pab.y, m); break;
             case 4: hit = boxPolygon(s, pab.x, pab.y, m); break;
-            default: hit = polygonPolygon(s, pab.x, pab.y, m); break;
+            case 5: hit = polygonPolygon(s, pab.x, pab.y, m); break;
+            case 6: hit = circleCapsule(s, pab.x, pab.y, m); break;
+            case 7: hit = boxCapsule(s, pab.x, pab.y, m); break;
+            case 8: hit = polygonCapsule(s, pab.x, pab.y, m); break;
+            default: hit = capsuleCapsule(s, pab.x, pab.y, m); break;
             }
             if (hit) slots[p] = m;
             flags[p] = hit ? 1u : 0u;

@@ -178,7 +178,7 @@ int allocTree(agx_world* w, agx::Tree& t, const std::vector<uint32_t>& list, con
 int allocPairBuffers(agx_world* w, uint64_t pairCap) {
     agx::DeviceWorld& d = w->d;
     // the narrow phase packs a pair index into 29 bits
-    if (pairCap >= (1ull << 29)) return fail(w, AGX_ERR_CAPACITY, "pair capacity above 2^29 pairs");
+    if (pairCap >= (1ull << 28)) return fail(w, AGX_ERR_CAPACITY, "pair capacity above 2^28 pairs");
     d.pairCap = pairCap;
     d.queryWaves = (d.nDyn + 63) / 64;
     HIP_CHECK(w, devAlloc(w, &d.pairs, pairCap));
@@ -555,6 +555,8 @@ int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uin
         uint32_t r = UINT32_MAX;
         if (type[i] == AGX_SHAPE_CIRCLE) r = sd.createCircle(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i]);
         else if (type[i] == AGX_SHAPE_BOX) r = sd.createBox(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i], p1[i]);
+        else if (type[i] == AGX_SHAPE_CAPSULE)
+            r = sd.createCapsule(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i], p0[i], p1[i]);
         else if (type[i] == AGX_SHAPE_POLYGON && vstart && vcount && verts)
             r = sd.createPolygon(px[i], py[i], rot[i], vx[i], vy[i], av[i], mass[i], material[i],
                                  reinterpret_cast<const agx::Vec*>(verts) + vstart[i], vcount[i]);

@@ -3,7 +3,7 @@
 // Header-only; links against libagx.so. Names, parameter structs and error behaviour follow the
 // reference so that code written against PS_AGONY2D::Simulation ports by changing the namespace:
 //   Simulation::update / getSimulationSettings / createMaterial / objects()   (Simulation.h:148-203)
-//   ObjectAPI::createCircle / createBox / createPolygon (+ *H), getBodyData    (ObjectAPI.h:34-109)
+//   ObjectAPI::createCircle / createBox / createPolygon / createCapsule (+ *H), getBodyData (ObjectAPI.h:34-109)
 //   ObjectAPI::setBodyPosition / Rotation / Velocity / AngularVelocity / Mass / Inertia (deferred,
 //   ObjectAPI.h:76-81), getBodies / getColliders / getPolygons viewers as host copies (:96-101)
 //   BodyCreateParams, CircleCreateParams, BoxCreateParams, PolygonCreateParams (ObjectCreateParams.h:14-95)
@@ -16,6 +16,7 @@
 // std::nullopt. Read-outs copy device state to host vectors (the reference returns non-owning views).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <iostream>
@@ -58,6 +59,14 @@ struct BodyCreateParams {
 struct CircleCreateParams { BodyCreateParams base; Real radius = 0; };
 struct BoxCreateParams { BodyCreateParams base; Vec2 size{}; };
 struct PolygonCreateParams { BodyCreateParams base; std::vector<Vec2> localVertices; };
+// ObjectCreateParams.h:97-104: core length along the local +x axis and radius; when both world
+// end points are given they set the length, position and rotation (ObjectManager.cpp:351-363).
+struct CapsuleCreateParams {
+    BodyCreateParams base;
+    Real length = 0;
+    Real radius = 0;
+    std::optional<Vec2> start{}, end{};
+};
 
 // BodySoA fields of one body (SoA/BodySoA.h:14-36).
 struct BodyData {
@@ -94,9 +103,11 @@ public:
     std::optional<ObjectDenseIndex> createCircle(const CircleCreateParams& p);
     std::optional<ObjectDenseIndex> createBox(const BoxCreateParams& p);
     std::optional<ObjectDenseIndex> createPolygon(const PolygonCreateParams& p);
+    std::optional<ObjectDenseIndex> createCapsule(const CapsuleCreateParams& p);
     std::optional<Handle> createCircleH(const CircleCreateParams& p) { return createCircle(p); }
     std::optional<Handle> createBoxH(const BoxCreateParams& p) { return createBox(p); }
     std::optional<Handle> createPolygonH(const PolygonCreateParams& p) { return createPolygon(p); }
+    std::optional<Handle> createCapsuleH(const CapsuleCreateParams& p) { return createCapsule(p); }
 
     std::optional<BodyData> getBodyData(ObjectDenseIndex index) const;
     BodyArrays getBodies() const;
@@ -260,6 +271,20 @@ inline std::optional<ObjectDenseIndex> ObjectAPI::createCircle(const CircleCreat
 inline std::optional<ObjectDenseIndex> ObjectAPI::createBox(const BoxCreateParams& p) {
     return add(AGX_SHAPE_BOX, p.base, p.size.x, p.size.y, nullptr);
 }
+inline std::optional<ObjectDenseIndex> ObjectAPI::createCapsule(const CapsuleCreateParams& p) {
+    if (p.start && p.end) {
+        const Vec2 d{ p.end->x - p.start->x, p.end->y - p.start->y };
+        CapsuleCreateParams q = p;
+        q.length = std::sqrt(d.x * d.x + d.y * d.y);
+        q.base.position = { (p.start->x + p.end->x) * 0.5f, (p.start->y + p.end->y) * 0.5f };
+        q.base.rotation = std::atan2(d.y, d.x);
+        q.start.reset();
+        q.end.reset();
+        return createCapsule(q);
+    }
+    return add(AGX_SHAPE_CAPSULE, p.base, p.length, p.radius, nullptr);
+}
+
 inline std::optional<ObjectDenseIndex> ObjectAPI::createPolygon(const PolygonCreateParams& p) {
     return add(AGX_SHAPE_POLYGON, p.base, 0.0f, 0.0f, &p.localVertices);
 }

@@ -26,7 +26,7 @@ int main() {
         p.size = w[1];
         if (!objects.createBox(p)) return 2;
     }
-    int made = 0;
+    int made = 0, capsules = 0;
     for (int iy = 0; iy < 12; iy++)
         for (int ix = 0; ix < 16; ix++) {
             BodyCreateParams b;
@@ -36,7 +36,19 @@ int main() {
             b.materialIndex = *mat;
             bool ok;
             switch ((ix + iy) % 3) {
-            case 0: { CircleCreateParams p; p.base = b; p.radius = 0.45f; ok = objects.createCircle(p).has_value(); break; }
+            case 0:
+                if (ix % 4 == 0) {   // a capsule given by its end points, tilted like the rotation would
+                    CapsuleCreateParams p;
+                    p.base = b;
+                    p.radius = 0.2f;
+                    p.start = Vec2{ b.position.x - 0.3f * std::cos(b.rotation), b.position.y - 0.3f * std::sin(b.rotation) };
+                    p.end = Vec2{ b.position.x + 0.3f * std::cos(b.rotation), b.position.y + 0.3f * std::sin(b.rotation) };
+                    ok = objects.createCapsule(p).has_value();
+                    capsules += ok ? 1 : 0;
+                } else {
+                    CircleCreateParams p; p.base = b; p.radius = 0.45f; ok = objects.createCircle(p).has_value();
+                }
+                break;
             case 1: { BoxCreateParams p; p.base = b; p.size = { 0.8f, 0.6f }; ok = objects.createBox(p).has_value(); break; }
             default: {
                 PolygonCreateParams p;
@@ -69,15 +81,16 @@ int main() {
     for (int frame = 0; frame < 60; frame++) steps += sim.update(1.0f / 60.0f);
     const ColliderArrays cols = objects.getColliders();
     const PolygonArrays polys = objects.getPolygons();
-    uint32_t polygonColliders = 0;
+    uint32_t polygonColliders = 0, capsuleColliders = 0;
     bool aabbsContain = true;
     for (size_t c = 0; c < cols.worldX.size(); c++) {
         polygonColliders += cols.shapeType[c] == AGX_SHAPE_POLYGON ? 1u : 0u;
+        capsuleColliders += cols.shapeType[c] == AGX_SHAPE_CAPSULE ? 1u : 0u;
         aabbsContain = aabbsContain && cols.aabbMinX[c] <= cols.worldX[c] && cols.worldX[c] <= cols.aabbMaxX[c] &&
                        cols.aabbMinY[c] <= cols.worldY[c] && cols.worldY[c] <= cols.aabbMaxY[c];
     }
-    const bool viewers = polygonColliders == 64 && polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 &&
-                         aabbsContain;
+    const bool viewers = polygonColliders == 64 && capsules > 0 && capsuleColliders == (uint32_t)capsules &&
+                         polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 && aabbsContain;
     const auto manifolds = sim.getBodyCollisionData();
     const auto [ids, islands] = sim.islands();
     const BodyArrays bodies = objects.getBodies();
@@ -87,8 +100,8 @@ int main() {
         inside = inside && std::fabs(bodies.comX[i]) < 11.0f && bodies.comY[i] > -1.0f && bodies.comY[i] < 21.0f;
     }
     const agx_stats st = sim.getStats();
-    std::printf("bodies %d steps %u manifolds %zu islands %u colours %u overflow %u rejected %d setters %d/%d/%d viewers %d\n",
-                made, steps, manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0, queuedEarly ? 1 : 0,
+    std::printf("bodies %d (capsules %d) steps %u manifolds %zu islands %u colours %u overflow %u rejected %d setters %d/%d/%d viewers %d\n",
+                made, capsules, steps, manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0, queuedEarly ? 1 : 0,
                 queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0);
     const bool ok = made == 192 && steps == 241 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
                     st.overflow == 0 && queuedEarly && queued && applied && viewers;

@@ -23,7 +23,7 @@ import math
 
 import numpy as np
 
-SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON = 0, 1, 2
+SHAPE_CIRCLE, SHAPE_BOX, SHAPE_POLYGON, SHAPE_CAPSULE = 0, 1, 2, 3
 
 
 class _Acc:
@@ -201,6 +201,59 @@ def free_swarm(n_poly: int = 2048, n_circle: int = 512, size: float = 64.0, spee
             acc.add(t, x, y, verts=v, **kw)
     return {"name": name, "materials": [(0.0, 0.6, 0.6)], "bodies": acc.bodies(),
             "settings": {"velocity_iterations": 8, "position_iterations": 3, "gravity_y": 0.0}}
+
+
+def capsule_pile(n: int = 1200, width: float = 48.0, height: float = 110.0, seed: int = 7, bullets: int = 64,
+                 name: str = "capsule_pile") -> dict:
+    """Every shape pair: capsules (30 %), polygons (30 %), boxes (20 %) and circles (20 %) dropped
+    from random non-overlapping places into a walled box with static capsule and box ramps, plus
+    `bullets` small fast circles fired down at the ramps (CCD against static capsules)."""
+    rng = np.random.default_rng(seed)
+    acc = _Acc()
+    add_walls(acc, -width / 2, 0.0, width / 2, height)
+    # static ramps: capsules and boxes at +-20 degrees, two rows
+    for k, x in enumerate(np.linspace(-width / 2 + 8.0, width / 2 - 8.0, 4)):
+        ang = (0.35 if k % 2 else -0.35)
+        acc.add(SHAPE_CAPSULE, float(x), 14.0, rot=float(ang % (2 * np.pi)), p0=7.0, p1=0.6)
+        acc.add(SHAPE_BOX, float(x) + 3.0, 28.0, rot=float((-ang) % (2 * np.pi)), p0=6.0, p1=0.8)
+    shapes = []
+    kinds = rng.choice(4, size=n, p=[0.3, 0.3, 0.2, 0.2])
+    for kind in kinds:
+        dens = float(rng.uniform(0.5, 2.0))
+        if kind == 0:       # capsule: core length and radius
+            r = float(np.float32(rng.uniform(0.12, 0.35)))
+            length = float(np.float32(rng.uniform(0.2, 1.4)))
+            m = float(np.float32(dens * (2.0 * r * length + math.pi * r * r)))
+            shapes.append((SHAPE_CAPSULE, 0.5 * length + r, (length, r), m))
+        elif kind == 1:
+            rp = float(rng.uniform(0.2, 0.55))
+            v = polygon_outline(rng, int(rng.integers(3, 9)), rp)
+            shapes.append((SHAPE_POLYGON, rp, v, float(np.float32(dens * polygon_area(v)))))
+        elif kind == 2:
+            w_, h_ = float(np.float32(rng.uniform(0.3, 1.0))), float(np.float32(rng.uniform(0.3, 1.0)))
+            shapes.append((SHAPE_BOX, 0.5 * math.hypot(w_, h_), (w_, h_), float(np.float32(dens * w_ * h_))))
+        else:
+            r = float(np.float32(rng.uniform(0.15, 0.45)))
+            shapes.append((SHAPE_CIRCLE, r, None, float(np.float32(dens * math.pi * r * r))))
+    pos = _place_uniform(rng, shapes, -width / 2, 32.0, width / 2, height)
+    for (t, r, v, m), (x, y) in zip(shapes, pos):
+        rot = float(min(rng.uniform(0.0, 2.0 * np.pi), 6.2831850))
+        if t == SHAPE_CAPSULE:
+            acc.add(t, x, y, rot=rot, mass=m, p0=v[0], p1=v[1])
+        elif t == SHAPE_BOX:
+            acc.add(t, x, y, rot=rot, mass=m, p0=v[0], p1=v[1])
+        elif t == SHAPE_CIRCLE:
+            acc.add(t, x, y, rot=rot, mass=m, p0=r)
+        else:
+            acc.add(t, x, y, rot=rot, mass=m, verts=v)
+    # bullets between the ramps rows, fired downwards at 150 m/s (several ramp thicknesses per step)
+    for k in range(bullets):
+        x = float(rng.uniform(-width / 2 + 3.0, width / 2 - 3.0))
+        r = float(np.float32(rng.uniform(0.05, 0.1)))
+        acc.add(SHAPE_CIRCLE, x, 20.0 + float(rng.uniform(0.0, 4.0)), vx=float(rng.uniform(-20.0, 20.0)), vy=-150.0,
+                mass=float(np.float32(math.pi * r * r)), p0=r)
+    return {"name": name, "materials": [(0.0, 0.5, 0.5)], "bodies": acc.bodies(),
+            "settings": {"velocity_iterations": 8, "position_iterations": 3, "gravity_y": -9.81}}
 
 
 def pile_small(seed: int = 1) -> dict:

@@ -64,6 +64,11 @@ def test_builder_matches_oracle_pile(agx, oracle, scenes):
     compare_builders(agx, oracle, scenes.pile_small(seed=1))
 
 
+def test_builder_matches_oracle_capsules(agx, oracle, scenes):
+    # every shape, capsules among them (mass split, inertia and extents of ObjectManager.cpp:197-389)
+    compare_builders(agx, oracle, scenes.capsule_pile(seed=7))
+
+
 def test_builder_matches_oracle_mixed_order(agx, oracle, scenes):
     # statics created after dynamics exercise the group moves
     rng = np.random.default_rng(5)
