@@ -43,8 +43,7 @@ typedef struct agx_mlp agx_mlp;
 
 /* Simulation::SimulationSettings (Simulation.h:50-76), FixedClock::Settings
  * (SimulationImpl/FixedClock.h:12-17), Integrator::Settings (Integration/Integrator.h:14-19)
- * and ContinuousCollisionDetector::Settings (CCD/ContinuousCollisionDetector.h:29-33).
- * Sleeping, joints and collision events are not part of this path. */
+ * and ContinuousCollisionDetector::Settings (CCD/ContinuousCollisionDetector.h:29-33). */
 typedef struct agx_settings {
     float gravity_x, gravity_y;            /* default (0, -9.81) */
     uint32_t velocity_iterations;          /* default 6 */
@@ -64,6 +63,13 @@ typedef struct agx_settings {
                                               (about 6,500 bodies); 0: one dataflow over all scenes (one
                                               residual for the batch), also used for larger scenes. Fixed
                                               sweep counts give the same bits either way. */
+    uint32_t collision_events;             /* default 1: record collision events (collisionEventsEnabled,
+                                              Simulation.h:60) while some collider has a callback id */
+    uint32_t event_capacity;               /* events kept per update/step call, default 65,536; more are
+                                              counted and dropped */
+    float sleep_speed_threshold;           /* default 0.1 (sleepSpeedThreshold, Simulation.h:70-72) */
+    float sleep_time_threshold;            /* default +infinity here: sleeping off, as in every benchmark
+                                              scene; the reference's default is 0.5 s (Simulation.h:73) */
 } agx_settings;
 
 /* One manifold record (64 bytes): NarrowPhaseCollisionDetector.h:26-61 with the collider
@@ -269,6 +275,50 @@ int agx_world_read_scene_sweeps(agx_world* w, uint32_t* out, uint32_t n);
  * after its last velocity sweep in the last step solved scene by scene with track_residual set;
  * n must be the scene count. A scene has met residual_target when its value is at most the target. */
 int agx_world_read_scene_residual(agx_world* w, float* out, uint32_t n);
+
+/* ---- collision events: BodyCollisionSolver::computeConstraintData's event recording
+ * (Solving/BodyCollisionSolver.cpp:300-388), Simulation::setColliderCollisionCallback
+ * (Simulation.cpp:152-172) and dispatchCollisionEvents (Simulation.cpp:174-180). While
+ * settings.collision_events is set and some collider has a callback id, every step's constraint
+ * preparation computes each manifold's impact impulse (the most, over its contact points, of the
+ * approach speed times the normal effective mass, before the warm start) and records one event per
+ * side whose collider has a callback id and a threshold at most that impulse (impulse > 0), side A
+ * before side B, manifolds in ascending index order. Events of all physics steps of one
+ * agx_world_update / _step / _step_graph call are kept in step order (the device keeps up to
+ * settings.event_capacity of them); the next such call starts a new list. The callback registry
+ * and the calls themselves live with the caller (host/GpuSimulation.h runs them after update()). */
+typedef struct agx_collision_event {
+    uint32_t self, other;             /* collider indices: the opted-in side, the other side */
+    uint32_t self_body, other_body;   /* their bodies */
+    float point_x, point_y;           /* contact point (mean of the manifold's two points) */
+    float normal_x, normal_y;         /* from self towards other */
+    float impulse;                    /* impact impulse */
+    uint32_t callback_id;             /* self's callback id */
+    uint32_t step;                    /* physics step of the call that recorded it (0, 1, ...) */
+    uint32_t reserved;
+} agx_collision_event;
+#define AGX_MAX_CALLBACK_ID 65535u
+/* callback_id 0 clears the collider's callback (its threshold returns to "never"); otherwise
+ * callback_id <= AGX_MAX_CALLBACK_ID and threshold >= 0 (ColliderSoA callbackId/eventThreshold). */
+int agx_world_set_collider_callback(agx_world* w, uint32_t collider, uint32_t callback_id, float threshold);
+/* Events of the last update/step call in record order; *count receives how many were recorded
+ * (events beyond event_capacity are counted but not kept: *dropped receives that number, may be
+ * NULL). AGX_ERR_CAPACITY when more events were kept than cap. */
+int agx_world_read_events(agx_world* w, agx_collision_event* out, uint64_t cap, uint64_t* count, uint64_t* dropped);
+
+/* ---- world snapshots. The reference has no on-disk or in-memory state format. A snapshot holds
+ * everything a later step reads: body state (poses, velocities, cos/sin, masses and inertias as
+ * edited), collider poses, AABBs and CCD sweep origins, island ids, the previous step's manifolds
+ * (the warm-start source, its hash rebuilt on load), the device counters (statistics, gate), the
+ * per-scene sweep counts, collider callbacks, the settings, the fixed clock's accumulator and the
+ * queued body commands. Loading it into a world built from the same scenes (same bodies, colliders,
+ * shapes and materials; checked by a fingerprint) makes that world step to the same bits as the world
+ * that saved it. The network history of the learned warm start is not kept (its next step runs
+ * without one). Format: little-endian, a 64-byte header ("AGXSNAP1", version, counts, fingerprint,
+ * payload size) followed by tagged sections (u32 tag, u32 reserved, u64 bytes, data padded to 16). */
+int agx_world_snapshot_size(agx_world* w, uint64_t* bytes);
+int agx_world_save(agx_world* w, void* buffer, uint64_t cap, uint64_t* written);
+int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes);
 
 /* GPU time per step segment, measured with HIP events on the world's stream while profiling
  * is on. Segments: see agx_profile_segment_name. agx_world_read_profile waits for the stream,

@@ -190,6 +190,8 @@ void orc_step(void* h, float dt) { step(W(h), dt); }
 void orc_post_update(void* h) { postUpdate(W(h)); }
 void orc_update(void* h, float dt, uint32_t steps) {
     flushCommands(W(h));   // Simulation::update flushes first (Simulation.cpp:38-39)
+    W(h).events.clear();
+    W(h).eventStep = 0;
     for (uint32_t i = 0; i < steps; i++) step(W(h), dt);
     postUpdate(W(h));
 }
@@ -201,6 +203,37 @@ int64_t orc_get_manifolds(void* h, Manifold* out, uint64_t cap) {
 }
 void orc_errors(void* h, float* out) { out[0] = W(h).velocityError; out[1] = W(h).positionError; }
 
+// Simulation::setColliderCollisionCallback (Simulation.cpp:152-172): id 0 clears.
+void orc_set_callback(void* h, uint32_t collider, uint32_t id, float threshold) {
+    World& w = W(h);
+    if (w.colCallback.size() < w.colliderCount()) {
+        w.colCallback.resize(w.colliderCount(), 0u);
+        w.colThreshold.resize(w.colliderCount(), 3.40282347e38f);
+    }
+    w.colCallback[collider] = id;
+    w.colThreshold[collider] = id ? threshold : 3.40282347e38f;
+}
+void orc_set_events_enabled(void* h, int on) { W(h).settings.collisionEvents = on != 0; }
+// Events of manifolds m on the current state (index order), then every event kept since the last update call.
+int64_t orc_collision_events(void* h, const Manifold* m, uint64_t n, Event* out, uint64_t cap) {
+    World& w = W(h);
+    std::vector<Event> keep;
+    keep.swap(w.events);
+    collisionEvents(w, std::vector<Manifold>(m, m + n), nullptr);
+    std::vector<Event> got;
+    got.swap(w.events);
+    w.events.swap(keep);
+    if (got.size() > cap) return -(int64_t)got.size();
+    if (!got.empty()) std::memcpy(out, got.data(), got.size() * sizeof(Event));
+    return (int64_t)got.size();
+}
+int64_t orc_get_events(void* h, Event* out, uint64_t cap) {
+    World& w = W(h);
+    if (w.events.size() > cap) return -(int64_t)w.events.size();
+    if (!w.events.empty()) std::memcpy(out, w.events.data(), w.events.size() * sizeof(Event));
+    return (int64_t)w.events.size();
+}
+
 void orc_queue_body(void* h, uint32_t kind, uint32_t body, float a, float b) { W(h).commands.push_back({ kind, body, a, b }); }
 void orc_flush(void* h) { flushCommands(W(h)); }
 
@@ -210,6 +243,8 @@ void orc_pool_free(void* p) { poolDestroy(static_cast<Pool*>(p)); }
 uint32_t orc_pool_size(void* p) { return poolSize(static_cast<Pool*>(p)); }
 void orc_update_pool(void* h, void* pool, float dt, uint32_t steps) {
     flushCommands(W(h));
+    W(h).events.clear();
+    W(h).eventStep = 0;
     for (uint32_t i = 0; i < steps; i++) step(W(h), dt, static_cast<Pool*>(pool));
     postUpdate(W(h));
 }

@@ -48,11 +48,21 @@ struct Manifold {
 };
 static_assert(sizeof(Manifold) == 64, "manifold record is 64 bytes");
 
+// Collision event (Types/CollisionEvent.h:18-27) with handles replaced by dense indices; the
+// field order matches the device record agx_collision_event.
+struct Event {
+    uint32_t self, other, selfBody, otherBody;
+    float px, py, nx, ny, impulse;
+    uint32_t callback, step, reserved;
+};
+static_assert(sizeof(Event) == 48, "event record is 48 bytes");
+
 struct Settings {
     float gravityX = 0.0f, gravityY = -9.81f;
     uint32_t velocityIterations = 6;   // Simulation.h:56
     uint32_t positionIterations = 3;   // Simulation.h:57
     bool ccd = true;                   // Simulation.h:66
+    bool collisionEvents = true;       // Simulation.h:60
     float ccdFastFactor = 1.0f;        // CCD/ContinuousCollisionDetector.h:31
     float ccdCoreFraction = 0.5f;      // CCD/ContinuousCollisionDetector.h:32
 };
@@ -95,6 +105,12 @@ struct World {
     // Deferred body modifications (ObjectAPI.h:76-81), applied by flushCommands.
     struct BodyCommand { uint32_t kind, body; float a, b; };
     std::vector<BodyCommand> commands;
+    // Collision callbacks per collider (ColliderSoA callbackId / eventThreshold; absent entries are
+    // "none") and the events recorded by the steps of the current update call.
+    std::vector<uint32_t> colCallback;
+    std::vector<float> colThreshold;
+    std::vector<Event> events;
+    uint32_t eventStep = 0;
 
     uint32_t bodyCount() const { return (uint32_t)comX.size(); }
     uint32_t colliderCount() const { return (uint32_t)colBody.size(); }
@@ -134,6 +150,10 @@ void warmStartFromCache(const World& w, std::vector<Manifold>& cur);  // :272-29
 void solve(World& w, std::vector<Manifold>& m, const uint32_t* order,
            uint32_t vIt, uint32_t pIt, float* residualOut);   // Solving/BodyCollisionSolver.cpp:38-980
 void islands(World& w, const std::vector<Manifold>& m);       // IslandManager.cpp:7-116
+// The event recording of computeConstraintData (Solving/BodyCollisionSolver.cpp:300-388) over the
+// manifolds in order[c] (null: index order) on the current, pre-warm-start velocities; appends to
+// w.events with step index w.eventStep. No-op unless events are on and some collider has a callback.
+void collisionEvents(World& w, const std::vector<Manifold>& m, const uint32_t* order);
 // ---- threaded step (the CPU baseline's structure; oracle/threaded.cpp) ----
 struct Pool;                                                  // pinned std::threads + the caller
 Pool* poolCreate(uint32_t participants);

@@ -438,6 +438,7 @@ void step(World& w, float dt, Pool* pool) {
             else narrowPhase(w, w.pairA.data(), w.pairB.data(), w.pairA.size(), w.manifolds);
             warmStartFromCache(w, w.manifolds);
             if (!w.manifolds.empty()) {
+                collisionEvents(w, w.manifolds, nullptr);
                 if (pool) solveThreaded(w, w.manifolds, *pool, w.settings.velocityIterations, w.settings.positionIterations);
                 else solve(w, w.manifolds, nullptr, w.settings.velocityIterations, w.settings.positionIterations, nullptr);
             }
@@ -448,6 +449,9 @@ void step(World& w, float dt, Pool* pool) {
     }
     islands(w, w.manifolds);
     w.postUpdateDone = false;
+    bool anyCallback = false;
+    for (uint32_t cb : w.colCallback) anyCallback = anyCallback || cb != 0;
+    if (w.settings.collisionEvents && anyCallback) w.eventStep++;
 }
 
 }  // namespace orc

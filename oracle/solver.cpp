@@ -231,6 +231,42 @@ void Solver::finish() {
     w.positionError = tPen;
 }
 
+void collisionEvents(World& w, const std::vector<Manifold>& ms, const uint32_t* order) {
+    bool any = false;
+    for (uint32_t cb : w.colCallback) any = any || cb != 0;
+    if (!w.settings.collisionEvents || !any) return;
+    auto callbackOf = [&](uint32_t c) { return c < w.colCallback.size() ? w.colCallback[c] : 0u; };
+    auto thresholdOf = [&](uint32_t c) { return c < w.colThreshold.size() ? w.colThreshold[c] : 3.40282347e38f; };
+    for (size_t c = 0; c < ms.size(); c++) {
+        const Manifold& d = ms[order ? order[c] : c];
+        const uint32_t bA = w.colBody[d.a], bB = w.colBody[d.b];
+        const V2 comA{ w.comX[bA], w.comY[bA] }, comB{ w.comX[bB], w.comY[bB] };
+        const float iMA = w.invMass[bA], iMB = w.invMass[bB], iIA = w.invInertia[bA], iIB = w.invInertia[bB];
+        const V2 n{ d.nx, d.ny };
+        const V2 vA{ w.velX[bA], w.velY[bA] }, vB{ w.velX[bB], w.velY[bB] };
+        const float wA = w.angVel[bA], wB = w.angVel[bB];
+        const float effMass = iMA + iMB;
+        float impact = 0.0f;
+        for (uint32_t i = 0; i < d.count; i++) {   // :333-356
+            const V2 cp = i == 0 ? V2{ d.p0x, d.p0y } : V2{ d.p1x, d.p1y };
+            const V2 rA = cp - comA, rB = cp - comB;
+            const float rAn = dot(V2{ -rA.y, rA.x }, n), rBn = dot(V2{ -rB.y, rB.x }, n);
+            const float nd = effMass + rAn * rAn * iIA + rBn * rBn * iIB;
+            const float normalMass = nd > 0.0f ? 1.0f / nd : 0.0f;
+            const float negVn = -(dot(vB - vA, n) + rBn * wB - rAn * wA);
+            impact = std::fmax(impact, negVn * normalMass);
+        }
+        if (impact <= 0.0f) continue;   // :359
+        const uint32_t cbA = callbackOf(d.a), cbB = callbackOf(d.b);
+        const bool fireA = cbA != 0 && impact >= thresholdOf(d.a);
+        const bool fireB = cbB != 0 && impact >= thresholdOf(d.b);
+        if (!fireA && !fireB) continue;
+        const V2 point = d.count > 1 ? (V2{ d.p0x, d.p0y } + V2{ d.p1x, d.p1y }) * 0.5f : V2{ d.p0x, d.p0y };
+        if (fireA) w.events.push_back(Event{ d.a, d.b, bA, bB, point.x, point.y, n.x, n.y, impact, cbA, w.eventStep, 0 });
+        if (fireB) w.events.push_back(Event{ d.b, d.a, bB, bA, point.x, point.y, -n.x, -n.y, impact, cbB, w.eventStep, 0 });
+    }
+}
+
 void solve(World& w, std::vector<Manifold>& ms, const uint32_t* order, uint32_t vIt, uint32_t pIt, float* residualOut) {
     Solver sv(w, ms);
     const size_t M = ms.size();

@@ -40,6 +40,11 @@ MANIFOLD_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("nx", "<f4"), ("ny", "<f
                            ("id0", "<u4"), ("id1", "<u4"), ("jn0", "<f4"), ("jt0", "<f4"), ("jn1", "<f4"),
                            ("jt1", "<f4")])
 assert MANIFOLD_DTYPE.itemsize == 64
+# agx_collision_event (Types/CollisionEvent.h:18-27 with dense indices for handles)
+EVENT_DTYPE = np.dtype([("self", "<u4"), ("other", "<u4"), ("self_body", "<u4"), ("other_body", "<u4"),
+                        ("px", "<f4"), ("py", "<f4"), ("nx", "<f4"), ("ny", "<f4"), ("impulse", "<f4"),
+                        ("callback", "<u4"), ("step", "<u4"), ("reserved", "<u4")])
+assert EVENT_DTYPE.itemsize == 48
 
 
 class AgxError(RuntimeError):
@@ -57,7 +62,9 @@ class Settings(C.Structure):
                 ("ccd_enabled", C.c_uint32), ("ccd_fast_factor", C.c_float), ("ccd_core_fraction", C.c_float),
                 ("update_interval", C.c_float), ("max_delta_per_update", C.c_float), ("time_scale", C.c_float),
                 ("track_errors", C.c_uint32), ("track_residual", C.c_uint32), ("residual_target", C.c_float),
-                ("collide_only", C.c_uint32), ("solve_per_scene", C.c_uint32)]
+                ("collide_only", C.c_uint32), ("solve_per_scene", C.c_uint32),
+                ("collision_events", C.c_uint32), ("event_capacity", C.c_uint32),
+                ("sleep_speed_threshold", C.c_float), ("sleep_time_threshold", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -163,6 +170,11 @@ _SIGS = {
     "agx_world_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "agx_world_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "agx_world_autotune": (C.c_int, [_P, C.c_float, C.c_uint32, C.POINTER(Tuning)]),
+    "agx_world_set_collider_callback": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float]),
+    "agx_world_snapshot_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "agx_world_save": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_load": (C.c_int, [_P, _P, C.c_uint64]),
+    "agx_world_read_events": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -410,6 +422,34 @@ class World:
         out = np.empty(max(cnt.value, 1), MANIFOLD_DTYPE)
         _check(lib().agx_world_read_manifolds(self._h, which, _ptr(out), cnt.value, C.byref(cnt)), self._h)
         return out[:cnt.value]
+
+    def set_collider_callback(self, collider: int, callback_id: int, threshold: float = 0.0):
+        """Simulation::setColliderCollisionCallback (Simulation.cpp:152-172)."""
+        _check(lib().agx_world_set_collider_callback(self._h, collider, callback_id, threshold), self._h)
+
+    def events(self):
+        """Collision events of the last update/step call and how many were dropped beyond the capacity."""
+        cnt, dropped = C.c_uint64(), C.c_uint64()
+        rc = lib().agx_world_read_events(self._h, None, 0, C.byref(cnt), C.byref(dropped))
+        out = np.empty(max(cnt.value, 1), EVENT_DTYPE)
+        if cnt.value:
+            _check(lib().agx_world_read_events(self._h, _ptr(out), cnt.value, C.byref(cnt), C.byref(dropped)), self._h)
+        elif rc:
+            _check(rc, self._h)
+        return out[:cnt.value], dropped.value
+
+    def save(self) -> bytes:
+        """The world's state as a snapshot (agx_world_save)."""
+        n = C.c_uint64()
+        _check(lib().agx_world_snapshot_size(self._h, C.byref(n)), self._h)
+        buf = C.create_string_buffer(n.value)
+        _check(lib().agx_world_save(self._h, buf, n.value, C.byref(n)), self._h)
+        return buf.raw[:n.value]
+
+    def load(self, snapshot: bytes):
+        """Restores a snapshot of a world built from the same scenes (agx_world_load)."""
+        buf = C.create_string_buffer(bytes(snapshot), len(snapshot))
+        _check(lib().agx_world_load(self._h, buf, len(snapshot)), self._h)
 
     def set_warm_start_source(self, manifolds: np.ndarray):
         m = np.ascontiguousarray(manifolds, MANIFOLD_DTYPE)

@@ -75,6 +75,8 @@ struct Counters {
     unsigned long long sweepsTotal;  // velocity sweeps run over those steps (per step: the most any scene ran)
     unsigned long long bodySweepsTotal;   // sum over those steps of dynamic bodies x the sweeps of their scene
     unsigned long long convergedBodiesTotal;   // sum over those steps of convergedBodies
+    uint32_t eventCount;       // collision events recorded since the update/step call began (may exceed the capacity)
+    uint32_t eventStep;        // physics steps with event recording since the call began
     // per step
     uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
     uint32_t manifolds;
@@ -107,6 +109,7 @@ struct Counters {
     uint32_t convergedScenes;  // this step: scenes that met the residual target
     uint32_t ccdCandidates;    // this step: fast dynamic colliders the CCD casts
     uint32_t flowMet;          // this step: the dataflow solve met the residual target
+    uint32_t eventStepTotal;   // this step: collision events recorded
 };
 
 constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)
@@ -316,6 +319,15 @@ struct DeviceWorld {
     float4* histOf = nullptr;
     const float4* histIn = nullptr;  // this step: warmHist of the previous step (null: not written)
     float4* histOut = nullptr;       // this step: histOf (null: no learned warm start)
+    // collision events (allocated with the first collider callback)
+    uint32_t* colCallback = nullptr; // per collider: callback id (0: none)
+    float* colThreshold = nullptr;   // per collider: least impact impulse that fires
+    uint32_t* evCount = nullptr;     // per manifold: bit 0 side A fires, bit 1 side B fires
+    uint32_t* evSides = nullptr;     // per manifold: sides that fire (0..2)
+    uint32_t* evOffset = nullptr;    // per manifold: exclusive scan of evSides
+    float* evImpulse = nullptr;      // per manifold: impact impulse
+    agx_collision_event* events = nullptr;
+    uint32_t eventCap = 0;
 };
 
 // ---------------------------------------------------------------- device math
@@ -472,6 +484,8 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
 hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors,
                              int perScene);
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
+// Collision events of the step's manifolds on its pre-warm-start velocities (k_events.hip).
+hipError_t launchEvents(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;
 // Co-resident 256-thread blocks per CU of the dataflow colouring and solve kernels: the default
 // (at most 4) and what the occupancy calculator allows.

@@ -4,6 +4,7 @@
 #include "mlp_internal.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -59,6 +60,10 @@ struct agx_world {
     // deferred body modifications (agx_world_queue_body), applied in order at the next flush
     struct BodyCommand { uint32_t kind, body; float a, b; };
     std::vector<BodyCommand> commands;
+    // collision events (agx_world_set_collider_callback / agx_world_read_events)
+    std::vector<uint32_t> colCallbackHost;
+    uint32_t callbackColliders = 0;  // colliders with a callback id
+    bool recordEvents = false;       // this call's steps record events
     bool staticDirty = false;        // a static body moved: static poses, AABBs and tree are rebuilt
     bool poseEdited = false;         // a body moved or turned since the last pose pass
     // step graphs (agx_world_step_graph): one per ping-pong parity of the manifold buffers
@@ -370,6 +375,10 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
                 if (perScene) HIP_CHECK(w, agx::launchColourPrepScene(st, d));
                 else HIP_CHECK(w, agx::launchColourPrep(st, d));
             }
+            if (w->recordEvents) {   // on the pre-warm-start velocities, as computeConstraintData
+                SegmentTimer t(w, SEG_COLOUR);
+                HIP_CHECK(w, agx::launchEvents(st, d));
+            }
             if (predict || collect) {
                 SegmentTimer t(w, SEG_LEARN);
                 if (!w->learnedNarrow) return fail(w, AGX_ERR_STATE, "learned mode was enabled after this step's narrow phase");
@@ -424,6 +433,29 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
     return AGX_OK;
 }
 
+// Start of an update/step call (or of a step driven phase by phase): whether its steps record
+// collision events, with the event list and step index cleared on the device (a fill kernel, so
+// nothing here is a memset) and the event buffer sized to the settings.
+int beginCall(agx_world* w) {
+    agx::DeviceWorld& d = w->d;
+    w->recordEvents = w->settings.collision_events != 0 && w->callbackColliders > 0;
+    if (!w->recordEvents) return AGX_OK;
+    const uint32_t cap = std::max(1u, w->settings.event_capacity);
+    if (!d.events || d.eventCap != cap) {
+        if (d.events) {
+            HIP_CHECK(w, hipStreamSynchronize(w->stream));
+            w->allocations.erase(std::remove(w->allocations.begin(), w->allocations.end(), (void*)d.events),
+                                 w->allocations.end());
+            HIP_CHECK(w, hipFree(d.events));
+            d.events = nullptr;
+        }
+        HIP_CHECK(w, devAlloc(w, &d.events, cap));
+        d.eventCap = cap;
+    }
+    HIP_CHECK(w, agx::launchFillBytes(w->stream, &d.ctr->eventCount, 2 * sizeof(uint32_t)));
+    return AGX_OK;
+}
+
 int postUpdate(agx_world* w) {
     SegmentTimer t(w, SEG_POST_UPDATE);
     HIP_CHECK(w, agx::launchPose(w->stream, w->d, 0, 0, 0));
@@ -443,10 +475,10 @@ std::vector<unsigned char> graphKey(const agx_world* w, float dt) {
     put(&w->settings, sizeof w->settings);
     put(&w->mlp, sizeof w->mlp);
     put(&w->learnMode, sizeof w->learnMode);
-    const unsigned char flags[3] = { (unsigned char)w->histReady, (unsigned char)w->postUpdatePending,
-                                     (unsigned char)w->d.cur };
+    const unsigned char flags[4] = { (unsigned char)w->histReady, (unsigned char)w->postUpdatePending,
+                                     (unsigned char)w->d.cur, (unsigned char)w->recordEvents };
     put(flags, sizeof flags);
-    const void* bufs[3] = { w->d.learnX, w->d.warmHist[0], w->d.histOf };
+    const void* bufs[4] = { w->d.learnX, w->d.warmHist[0], w->d.histOf, w->d.events };
     put(bufs, sizeof bufs);
     const int shape[2] = { w->d.colourBlocks, w->d.solveBlocks };
     put(shape, sizeof shape);
@@ -507,6 +539,10 @@ void agx_settings_default(agx_settings* s) {
     s->max_delta_per_update = 1.0f / 20.0f;
     s->time_scale = 1.0f;
     s->solve_per_scene = 1;
+    s->collision_events = 1;
+    s->event_capacity = 65536;
+    s->sleep_speed_threshold = 0.1f;
+    s->sleep_time_threshold = INFINITY;
 }
 
 const char* agx_last_error(void) { return g_lastError.c_str(); }
@@ -780,11 +816,16 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     AL(d.cs.idx, solveCap); AL(d.cs.dep, solveCap); AL(d.cs.nrm, solveCap); AL(d.cs.inv, solveCap); AL(d.cs.r0, solveCap);
     AL(d.cs.r1, solveCap); AL(d.cs.k0, solveCap); AL(d.cs.k1, solveCap); AL(d.cs.imp, solveCap);
     AL(d.cs.blk, solveCap);
+    AL(d.colCallback, d.C); AL(d.colThreshold, d.C);
+    AL(d.evCount, pairCap); AL(d.evSides, pairCap); AL(d.evOffset, pairCap); AL(d.evImpulse, pairCap);
+    w->colCallbackHost.assign(d.C, 0u);
 #undef UP
 #undef AL
     // Initial state: island ids, CCD hit times, empty contact cache, refreshSpatialData, trees.
     hipStream_t st = w->stream;
     bool ok = agx::launchFillU32(st, d.islandId, UINT32_MAX, d.N) == hipSuccess &&
+              agx::launchFillU32(st, d.colCallback, 0u, d.C) == hipSuccess &&
+              agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.colThreshold), 0x7F7FFFFFu, d.C) == hipSuccess &&
               agx::launchFillU32(st, d.sceneSweeps, 0u, d.S) == hipSuccess &&
               agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.hitTime), 0x3F800000u, d.N) == hipSuccess &&
               agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.hashKeys), 0xFFFFFFFFu, 2 * d.hashCapMax) == hipSuccess &&
@@ -837,8 +878,10 @@ int agx_world_run_phase(agx_world* w, uint32_t phase, float dt) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
     if (phase >= AGX_PHASE_COUNT) return fail(w, AGX_ERR_INVALID_ARGUMENT, "unknown phase %u", phase);
     if ((int)phase != w->nextPhase) return fail(w, AGX_ERR_STATE, "phase %u out of order (expected %d)", phase, w->nextPhase);
-    if (phase == AGX_PHASE_INTEGRATE)
+    if (phase == AGX_PHASE_INTEGRATE) {
         if (int r = flushCommands(w)) return r;
+        if (int r = beginCall(w)) return r;
+    }
     return runPhase(w, phase, dt);
 }
 
@@ -853,6 +896,7 @@ int agx_world_step(agx_world* w, float dt, uint32_t n_steps) {
     if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
     if (!(dt > 0.0f)) return AGX_OK;
     if (int r = flushCommands(w)) return r;
+    if (int r = beginCall(w)) return r;
     for (uint32_t i = 0; i < n_steps; i++)
         for (uint32_t p = 0; p < AGX_PHASE_COUNT; p++)
             if (int r = runPhase(w, p, dt)) return r;
@@ -1078,6 +1122,7 @@ int agx_world_step_graph(agx_world* w, float dt, uint32_t n_steps) {
     if (!(dt > 0.0f) || n_steps == 0) return AGX_OK;
     if (int r = flushCommands(w)) return r;
     if (w->profiling) return agx_world_step(w, dt, n_steps);
+    if (int r = beginCall(w)) return r;
     for (uint32_t p = 0; p < AGX_PHASE_COUNT; p++)
         if (int r = runPhase(w, p, dt)) return r;
     for (uint32_t i = 1; i < n_steps; i++) {
@@ -1158,6 +1203,42 @@ int agx_world_narrowphase(agx_world* w, const uint32_t* pairs_a, const uint32_t*
     if (overflow & AGX_OVERFLOW_HASH) return fail(w, AGX_ERR_CAPACITY, "warm-start lookup overflowed");
     if (m > cap) return fail(w, AGX_ERR_CAPACITY, "%u manifolds exceed cap %llu", m, (unsigned long long)cap);
     if (m && out) HIP_CHECK(w, hipMemcpy(out, d.man[d.cur], m * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
+int agx_world_set_collider_callback(agx_world* w, uint32_t collider, uint32_t callback_id, float threshold) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    agx::DeviceWorld& d = w->d;
+    if (collider >= d.C) return fail(w, AGX_ERR_INVALID_ARGUMENT, "collider %u out of range (%u colliders)", collider, d.C);
+    if (callback_id > AGX_MAX_CALLBACK_ID) return fail(w, AGX_ERR_INVALID_ARGUMENT, "callback id %u too large", callback_id);
+    if (callback_id == 0) threshold = FLT_MAX;   // NO_COLLISION_EVENT_THRESHOLD (Types/CollisionEvent.h:15)
+    else if (!(threshold >= 0.0f)) return fail(w, AGX_ERR_INVALID_ARGUMENT, "event threshold must be >= 0");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    const uint32_t old = w->colCallbackHost[collider];
+    w->callbackColliders += (callback_id != 0) - (old != 0);
+    w->colCallbackHost[collider] = callback_id;
+    HIP_CHECK(w, hipMemcpyAsync(d.colCallback + collider, &callback_id, sizeof callback_id, hipMemcpyHostToDevice, w->stream));
+    HIP_CHECK(w, hipMemcpyAsync(d.colThreshold + collider, &threshold, sizeof threshold, hipMemcpyHostToDevice, w->stream));
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    return AGX_OK;
+}
+
+int agx_world_read_events(agx_world* w, agx_collision_event* out, uint64_t cap, uint64_t* count, uint64_t* dropped) {
+    if (!w || !count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    agx::DeviceWorld& d = w->d;
+    *count = 0;
+    if (dropped) *dropped = 0;
+    if (!w->recordEvents || !d.events) return AGX_OK;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    uint32_t recorded = 0;
+    HIP_CHECK(w, hipMemcpy(&recorded, &d.ctr->eventCount, sizeof recorded, hipMemcpyDeviceToHost));
+    const uint64_t kept = std::min<uint64_t>(recorded, d.eventCap);
+    *count = kept;
+    if (dropped) *dropped = recorded - kept;
+    if (kept > cap) return fail(w, AGX_ERR_CAPACITY, "%llu events, room for %llu", (unsigned long long)kept,
+                                (unsigned long long)cap);
+    if (kept && !out) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null output");
+    if (kept) HIP_CHECK(w, hipMemcpy(out, d.events, kept * sizeof(agx_collision_event), hipMemcpyDeviceToHost));
     return AGX_OK;
 }
 
@@ -1519,6 +1600,257 @@ int agx_broadphase_pairs(const float* aabb, const uint32_t* owner, const uint8_t
         return bail(fail(w, AGX_ERR_HIP, "read pairs"));
     for (uint32_t i = 0; i < c.pairs; i++) { pairs_a[i] = h[i].x; pairs_b[i] = h[i].y; }
     agx_world_destroy(w);
+    return AGX_OK;
+}
+
+}  // extern "C"
+
+// ---- world snapshots (agx_world_save / agx_world_load)
+namespace {
+
+constexpr char SNAP_MAGIC[8] = { 'A', 'G', 'X', 'S', 'N', 'A', 'P', '1' };
+constexpr uint32_t SNAP_VERSION = 1;
+struct SnapHeader {
+    char magic[8];
+    uint32_t version, scenes, bodies, colliders, vertices, materials;
+    uint64_t fingerprint;
+    uint64_t payload;
+    uint32_t manifolds, reserved[3];
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
+
+enum SnapTag : uint32_t {
+    TAG_SETTINGS = 1, TAG_HOST, TAG_POS, TAG_VEL, TAG_RCS, TAG_INV, TAG_MASS, TAG_INERTIA, TAG_POSE, TAG_WROT,
+    TAG_PREVW, TAG_AABB, TAG_ISLAND, TAG_HIT, TAG_COUNTERS, TAG_MANIFOLDS, TAG_SCENE_SWEEPS, TAG_CALLBACK,
+    TAG_THRESHOLD, TAG_COMMANDS, TAG_END
+};
+struct SnapHost {
+    float accumulator;
+    uint32_t lastStepCount, postUpdatePending, poseEdited;
+};
+
+// FNV-1a over the world's fixed structure: what a snapshot's arrays are indexed by.
+uint64_t fingerprint(const agx_world* w) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    const agx::DeviceWorld& d = w->d;
+    const uint32_t counts[6] = { d.S, d.N, d.C, d.nVerts, d.nMat, w->staticBodies };
+    mix(counts, sizeof counts);
+    auto vec = [&](const auto& v) { if (!v.empty()) mix(v.data(), v.size() * sizeof(v[0])); };
+    vec(w->colType); vec(w->colMaterial); vec(w->colScene); vec(w->polyStart); vec(w->polyCount);
+    vec(w->colP0); vec(w->colP1); vec(w->localComX); vec(w->localComY);
+    return h;
+}
+
+struct SnapWriter {
+    unsigned char* out;   // null: count only
+    uint64_t cap, at = 0;
+    bool overflow = false;
+    void raw(const void* p, uint64_t n) {
+        if (out && at + n <= cap) std::memcpy(out + at, p, n);
+        else if (out) overflow = true;
+        at += n;
+    }
+    void section(uint32_t tag, const void* p, uint64_t n) {
+        const uint32_t head[2] = { tag, 0u };
+        raw(head, sizeof head);
+        raw(&n, sizeof n);
+        raw(p, n);
+        static const unsigned char zeros[16] = {};
+        raw(zeros, (16 - (n & 15)) & 15);
+    }
+};
+
+// Copies of the device arrays a snapshot holds (host-side staging).
+struct SnapArrays {
+    std::vector<float4> pos, vel, pose, aabb;
+    std::vector<float2> rcs, inv, prevW;
+    std::vector<float> wRot, hit, threshold;
+    std::vector<uint32_t> island, sceneSweeps, callback;
+    std::vector<agx_manifold> man;
+    agx::Counters ctr;
+};
+
+template <class T>
+hipError_t down(std::vector<T>& v, const T* p, size_t n) {
+    v.resize(n);
+    return n ? hipMemcpy(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost) : hipSuccess;
+}
+template <class T>
+hipError_t up(T* p, const std::vector<T>& v) {
+    return v.empty() ? hipSuccess : hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int writeSnapshot(agx_world* w, SnapWriter& sw) {
+    agx::DeviceWorld& d = w->d;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    SnapArrays a;
+    HIP_CHECK(w, hipMemcpy(&a.ctr, d.ctr, sizeof a.ctr, hipMemcpyDeviceToHost));
+    HIP_CHECK(w, down(a.pos, d.pos, d.N)); HIP_CHECK(w, down(a.vel, d.vel, d.N)); HIP_CHECK(w, down(a.rcs, d.rcs, d.N));
+    HIP_CHECK(w, down(a.inv, d.inv, d.N)); HIP_CHECK(w, down(a.island, d.islandId, d.N));
+    HIP_CHECK(w, down(a.hit, d.hitTime, d.N));
+    HIP_CHECK(w, down(a.pose, d.pose, d.C)); HIP_CHECK(w, down(a.wRot, d.wRot, d.C));
+    HIP_CHECK(w, down(a.prevW, d.prevW, d.C)); HIP_CHECK(w, down(a.aabb, d.aabb, d.C));
+    HIP_CHECK(w, down(a.callback, d.colCallback, d.C)); HIP_CHECK(w, down(a.threshold, d.colThreshold, d.C));
+    HIP_CHECK(w, down(a.sceneSweeps, d.sceneSweeps, d.S));
+    HIP_CHECK(w, down(a.man, d.man[1 - d.cur], a.ctr.prevManifolds));
+    SnapHeader h{};
+    std::memcpy(h.magic, SNAP_MAGIC, sizeof h.magic);
+    h.version = SNAP_VERSION;
+    h.scenes = d.S; h.bodies = d.N; h.colliders = d.C; h.vertices = d.nVerts; h.materials = d.nMat;
+    h.fingerprint = fingerprint(w);
+    h.manifolds = a.ctr.prevManifolds;
+    const uint64_t headerAt = sw.at;
+    sw.raw(&h, sizeof h);
+    const uint64_t payloadStart = sw.at;
+    const SnapHost host{ w->accumulator, w->lastStepCount, w->postUpdatePending ? 1u : 0u, w->poseEdited ? 1u : 0u };
+    sw.section(TAG_SETTINGS, &w->settings, sizeof w->settings);
+    sw.section(TAG_HOST, &host, sizeof host);
+    sw.section(TAG_POS, a.pos.data(), a.pos.size() * sizeof(float4));
+    sw.section(TAG_VEL, a.vel.data(), a.vel.size() * sizeof(float4));
+    sw.section(TAG_RCS, a.rcs.data(), a.rcs.size() * sizeof(float2));
+    sw.section(TAG_INV, a.inv.data(), a.inv.size() * sizeof(float2));
+    sw.section(TAG_MASS, w->mass.data(), w->mass.size() * sizeof(float));
+    sw.section(TAG_INERTIA, w->inertia.data(), w->inertia.size() * sizeof(float));
+    sw.section(TAG_POSE, a.pose.data(), a.pose.size() * sizeof(float4));
+    sw.section(TAG_WROT, a.wRot.data(), a.wRot.size() * sizeof(float));
+    sw.section(TAG_PREVW, a.prevW.data(), a.prevW.size() * sizeof(float2));
+    sw.section(TAG_AABB, a.aabb.data(), a.aabb.size() * sizeof(float4));
+    sw.section(TAG_ISLAND, a.island.data(), a.island.size() * sizeof(uint32_t));
+    sw.section(TAG_HIT, a.hit.data(), a.hit.size() * sizeof(float));
+    sw.section(TAG_COUNTERS, &a.ctr, sizeof a.ctr);
+    sw.section(TAG_MANIFOLDS, a.man.data(), a.man.size() * sizeof(agx_manifold));
+    sw.section(TAG_SCENE_SWEEPS, a.sceneSweeps.data(), a.sceneSweeps.size() * sizeof(uint32_t));
+    sw.section(TAG_CALLBACK, a.callback.data(), a.callback.size() * sizeof(uint32_t));
+    sw.section(TAG_THRESHOLD, a.threshold.data(), a.threshold.size() * sizeof(float));
+    sw.section(TAG_COMMANDS, w->commands.data(), w->commands.size() * sizeof(agx_world::BodyCommand));
+    sw.section(TAG_END, nullptr, 0);
+    h.payload = sw.at - payloadStart;
+    if (sw.out && !sw.overflow) std::memcpy(sw.out + headerAt, &h, sizeof h);
+    return AGX_OK;
+}
+
+struct SnapReader {
+    const unsigned char* in;
+    uint64_t size, at = 0;
+    // Next section, which must carry `tag` and (when expect != ~0) exactly `expect` bytes.
+    const unsigned char* section(uint32_t tag, uint64_t expect, uint64_t* got) {
+        if (at + 16 > size) return nullptr;
+        uint32_t head[2];
+        uint64_t n;
+        std::memcpy(head, in + at, sizeof head);
+        std::memcpy(&n, in + at + 8, sizeof n);
+        if (head[0] != tag || (expect != ~0ull && n != expect) || n > size - at - 16) return nullptr;
+        const unsigned char* p = in + at + 16;
+        at += 16 + n + ((16 - (n & 15)) & 15);
+        if (got) *got = n;
+        return p;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int agx_world_snapshot_size(agx_world* w, uint64_t* bytes) {
+    if (!w || !bytes) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    SnapWriter sw{ nullptr, 0 };
+    if (int r = writeSnapshot(w, sw)) return r;
+    *bytes = sw.at;
+    return AGX_OK;
+}
+
+int agx_world_save(agx_world* w, void* buffer, uint64_t cap, uint64_t* written) {
+    if (!w || !buffer || !written) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    SnapWriter sw{ static_cast<unsigned char*>(buffer), cap };
+    if (int r = writeSnapshot(w, sw)) return r;
+    *written = sw.at;
+    if (sw.overflow) return fail(w, AGX_ERR_CAPACITY, "snapshot needs %llu bytes", (unsigned long long)sw.at);
+    return AGX_OK;
+}
+
+int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
+    if (!w || !buffer) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    agx::DeviceWorld& d = w->d;
+    const unsigned char* in = static_cast<const unsigned char*>(buffer);
+    SnapHeader h;
+    if (bytes < sizeof h) return fail(w, AGX_ERR_INVALID_ARGUMENT, "not a snapshot (too short)");
+    std::memcpy(&h, in, sizeof h);
+    if (std::memcmp(h.magic, SNAP_MAGIC, sizeof h.magic) != 0 || h.version != SNAP_VERSION)
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "not a snapshot of this version");
+    if (h.scenes != d.S || h.bodies != d.N || h.colliders != d.C || h.vertices != d.nVerts || h.materials != d.nMat ||
+        h.fingerprint != fingerprint(w))
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "snapshot of a different world");
+    if (h.payload > bytes - sizeof h || h.manifolds > d.manCap)
+        return fail(w, AGX_ERR_INVALID_ARGUMENT, "truncated or corrupt snapshot");
+    SnapReader r{ in + sizeof h, h.payload };
+    const uint64_t N = d.N, C = d.C;
+    SnapArrays a;
+    agx_settings settings;
+    SnapHost host;
+    uint64_t commandBytes = 0;
+    auto take = [&](uint32_t tag, void* dst, uint64_t n) {
+        const unsigned char* p = r.section(tag, n, nullptr);
+        if (!p) return false;
+        if (n) std::memcpy(dst, p, n);
+        return true;
+    };
+    auto takeVec = [&](uint32_t tag, auto& v, uint64_t count) {
+        v.resize(count);
+        return take(tag, v.data(), count * sizeof(v[0]));
+    };
+    std::vector<float> mass, inertia;
+    bool ok = take(TAG_SETTINGS, &settings, sizeof settings) && take(TAG_HOST, &host, sizeof host) &&
+              takeVec(TAG_POS, a.pos, N) && takeVec(TAG_VEL, a.vel, N) && takeVec(TAG_RCS, a.rcs, N) &&
+              takeVec(TAG_INV, a.inv, N) && takeVec(TAG_MASS, mass, N) && takeVec(TAG_INERTIA, inertia, N) &&
+              takeVec(TAG_POSE, a.pose, C) && takeVec(TAG_WROT, a.wRot, C) && takeVec(TAG_PREVW, a.prevW, C) &&
+              takeVec(TAG_AABB, a.aabb, C) && takeVec(TAG_ISLAND, a.island, N) && takeVec(TAG_HIT, a.hit, N) &&
+              take(TAG_COUNTERS, &a.ctr, sizeof a.ctr) && takeVec(TAG_MANIFOLDS, a.man, h.manifolds) &&
+              takeVec(TAG_SCENE_SWEEPS, a.sceneSweeps, d.S) && takeVec(TAG_CALLBACK, a.callback, C) &&
+              takeVec(TAG_THRESHOLD, a.threshold, C);
+    const unsigned char* cmd = ok ? r.section(TAG_COMMANDS, ~0ull, &commandBytes) : nullptr;
+    ok = ok && cmd && commandBytes % sizeof(agx_world::BodyCommand) == 0 && r.section(TAG_END, 0, nullptr);
+    if (!ok || a.ctr.prevManifolds != h.manifolds) return fail(w, AGX_ERR_INVALID_ARGUMENT, "truncated or corrupt snapshot");
+    for (uint32_t cb : a.callback)
+        if (cb > AGX_MAX_CALLBACK_ID) return fail(w, AGX_ERR_INVALID_ARGUMENT, "corrupt snapshot (callback id)");
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    HIP_CHECK(w, up(d.pos, a.pos)); HIP_CHECK(w, up(d.vel, a.vel)); HIP_CHECK(w, up(d.rcs, a.rcs));
+    HIP_CHECK(w, up(d.inv, a.inv)); HIP_CHECK(w, up(d.islandId, a.island)); HIP_CHECK(w, up(d.hitTime, a.hit));
+    HIP_CHECK(w, up(d.pose, a.pose)); HIP_CHECK(w, up(d.wRot, a.wRot)); HIP_CHECK(w, up(d.prevW, a.prevW));
+    HIP_CHECK(w, up(d.aabb, a.aabb)); HIP_CHECK(w, up(d.colCallback, a.callback));
+    HIP_CHECK(w, up(d.colThreshold, a.threshold)); HIP_CHECK(w, up(d.sceneSweeps, a.sceneSweeps));
+    // the warm-start source and its hash: built the way a step's FINISH builds them, then every
+    // counter as saved
+    HIP_CHECK(w, up(d.man[1 - d.cur], a.man));
+    agx::Counters c = a.ctr;
+    c.manifolds = h.manifolds;
+    HIP_CHECK(w, hipMemcpy(d.ctr, &c, sizeof c, hipMemcpyHostToDevice));
+    d.cur ^= 1;
+    HIP_CHECK(w, agx::launchHashBuild(w->stream, d));
+    d.cur ^= 1;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    HIP_CHECK(w, hipMemcpy(d.ctr, &a.ctr, sizeof a.ctr, hipMemcpyHostToDevice));
+    w->settings = settings;
+    w->mass = mass;
+    w->inertia = inertia;
+    w->accumulator = host.accumulator;
+    w->lastStepCount = host.lastStepCount;
+    w->postUpdatePending = host.postUpdatePending != 0;
+    w->poseEdited = host.poseEdited != 0;
+    w->commands.assign(reinterpret_cast<const agx_world::BodyCommand*>(cmd),
+                       reinterpret_cast<const agx_world::BodyCommand*>(cmd + commandBytes));
+    w->colCallbackHost = a.callback;
+    w->callbackColliders = 0;
+    for (uint32_t cb : a.callback) w->callbackColliders += cb != 0 ? 1u : 0u;
+    w->staticDirty = d.nStat > 0;   // static colliders' poses and tree follow the loaded bodies at the next pose pass
+    w->histReady = false;
+    w->nextPhase = 0;
     return AGX_OK;
 }
 

@@ -8,6 +8,9 @@
 //   ObjectAPI.h:76-81), getBodies / getColliders / getPolygons viewers as host copies (:96-101)
 //   BodyCreateParams, CircleCreateParams, BoxCreateParams, PolygonCreateParams (ObjectCreateParams.h:14-95)
 //   Material                                                                  (Types/Material.h:6-11)
+//   CollisionEvent, collision callbacks: createCollisionCallback / destroyCollisionCallback,
+//   setColliderCollisionCallback (Types/CollisionEvent.h:18-30, CollisionCallbackManager.h:14-118,
+//   Simulation.cpp:152-180)
 // Bad user input returns std::nullopt / false and writes a message to std::cerr, like the
 // reference's ObjectManager (ObjectManager.cpp:92-97).
 //
@@ -94,6 +97,21 @@ struct PolygonArrays {
 
 class Simulation;
 
+// Types/CollisionEvent.h:11-30, with dense indices for handles. Handlers run after update()/step()
+// returns, for every physics step of that call in step order (the reference runs them at the end of
+// each physics step; commands they queue take effect at the next update either way).
+using CollisionCallbackIndex = uint16_t;
+struct CollisionEvent {
+    ObjectDenseIndex self, other;
+    ObjectDenseIndex selfBody, otherBody;
+    Vec2 point;
+    Vec2 normal;        // from self towards other
+    Real impulse;
+    CollisionCallbackIndex callbackId;
+    uint32_t step;      // physics step of the call
+};
+using CollisionCallback = void (*)(const CollisionEvent& event, Simulation& simulation, void* userData);
+
 class ObjectAPI {
 public:
     explicit ObjectAPI(Simulation& s) : sim(s) {}
@@ -164,6 +182,44 @@ public:
 
     ObjectAPI& objects() noexcept { return api; }
 
+    // CollisionCallbackManager::create / destroy (CollisionCallbackManager.h:38-80): slot 0 is the
+    // permanent "none", destroyed slots are reused last-in first-out.
+    std::optional<CollisionCallbackIndex> createCollisionCallback(CollisionCallback function, void* userData) {
+        if (!function) return std::nullopt;
+        CollisionCallbackIndex index;
+        if (!freeCallbacks.empty()) {
+            index = freeCallbacks.back();
+            freeCallbacks.pop_back();
+            callbacks[index] = { function, userData, true };
+        } else {
+            if (callbacks.size() >= AGX_MAX_CALLBACK_ID) return std::nullopt;
+            index = (CollisionCallbackIndex)callbacks.size();
+            callbacks.push_back({ function, userData, true });
+        }
+        return index;
+    }
+    bool destroyCollisionCallback(CollisionCallbackIndex index) {
+        if (index == 0 || !callbackAlive(index)) return false;
+        callbacks[index] = {};
+        freeCallbacks.push_back(index);
+        return true;
+    }
+    bool callbackAlive(CollisionCallbackIndex index) const noexcept {
+        return index < callbacks.size() && callbacks[index].alive;
+    }
+    // Simulation::setColliderCollisionCallback (Simulation.cpp:152-172): index 0 clears; otherwise
+    // the callback must be alive and the threshold >= 0.
+    bool setColliderCollisionCallback(ObjectDenseIndex collider, CollisionCallbackIndex index, Real impulseThreshold) {
+        if (collider >= counts().colliders) return false;
+        if (index != 0 && (!callbackAlive(index) || !(impulseThreshold >= Real(0)))) return false;
+        if (!world) { pendingCallbacks.push_back({ collider, index, impulseThreshold }); return true; }
+        if (agx_world_set_collider_callback(world, collider, index, impulseThreshold) != AGX_OK) {
+            fail("setColliderCollisionCallback", agx_world_last_error(world));
+            return false;
+        }
+        return true;
+    }
+
     // Uploads the scene (Simulation::refreshSpatialData runs here). Called by the first update().
     bool upload() {
         if (world) return true;
@@ -175,7 +231,31 @@ public:
         for (const ObjectAPI::Command& c : api.pending)   // commands given before the upload, in order
             if (agx_world_queue_body(world, c.kind, c.body, c.a, c.b) != AGX_OK) fail("set", agx_world_last_error(world));
         api.pending.clear();
+        for (const PendingCallback& c : pendingCallbacks)
+            if (agx_world_set_collider_callback(world, c.collider, c.index, c.threshold) != AGX_OK)
+                fail("setColliderCollisionCallback", agx_world_last_error(world));
+        pendingCallbacks.clear();
         return true;
+    }
+
+    // Simulation::dispatchCollisionEvents (Simulation.cpp:174-180) for the last update()/step():
+    // each event goes to its callback if that is still alive; a handler may create or destroy
+    // callbacks (the entry is copied before the call).
+    void dispatchCollisionEvents() {
+        if (!world) return;
+        uint64_t count = 0, dropped = 0;
+        if (agx_world_read_events(world, nullptr, 0, &count, &dropped) != AGX_OK && count == 0) return;
+        std::vector<agx_collision_event> ev(count);
+        if (count && agx_world_read_events(world, ev.data(), count, &count, &dropped) != AGX_OK) return;
+        if (dropped) fail("dispatchCollisionEvents", "events beyond settings.event_capacity were dropped");
+        for (const agx_collision_event& e : ev) {
+            const CollisionCallbackIndex id = (CollisionCallbackIndex)e.callback_id;
+            if (id == 0 || !callbackAlive(id)) continue;
+            const CallbackEntry entry = callbacks[id];
+            const CollisionEvent out{ e.self, e.other, e.self_body, e.other_body, { e.point_x, e.point_y },
+                                      { e.normal_x, e.normal_y }, e.impulse, id, e.step };
+            entry.function(out, *this, entry.userData);
+        }
     }
 
     // Simulation::update (Simulation.cpp:32-74): fixed-interval sub-steps, then the pose refresh.
@@ -184,6 +264,7 @@ public:
         agx_world_set_settings(world, &settings);
         uint32_t steps = 0;
         if (agx_world_update(world, dt, &steps) != AGX_OK) fail("update", agx_world_last_error(world));
+        if (steps) dispatchCollisionEvents();
         return steps;
     }
 
@@ -195,6 +276,7 @@ public:
             fail("step", agx_world_last_error(world));
             return false;
         }
+        if (n) dispatchCollisionEvents();
         return true;
     }
 
@@ -241,6 +323,11 @@ public:
 
 private:
     friend class ObjectAPI;
+    struct CallbackEntry { CollisionCallback function = nullptr; void* userData = nullptr; bool alive = false; };
+    struct PendingCallback { ObjectDenseIndex collider; CollisionCallbackIndex index; Real threshold; };
+    std::vector<CallbackEntry> callbacks{ CallbackEntry{ nullptr, nullptr, false } };   // slot 0: none
+    std::vector<CollisionCallbackIndex> freeCallbacks;
+    std::vector<PendingCallback> pendingCallbacks;
     agx_settings settings{};
     agx_builder* builder = nullptr;
     agx_world* world = nullptr;

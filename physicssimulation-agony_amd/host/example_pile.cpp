@@ -65,6 +65,19 @@ int main() {
     bad.localVertices = { { 0, 0 }, { 1, 0 }, { 2, 0 } };
     const bool rejected = !objects.createPolygon(bad).has_value();
 
+    // collision callbacks (Simulation::setColliderCollisionCallback) on the first 40 dynamic colliders
+    struct Tally { uint32_t events = 0, badNormals = 0; float maxImpulse = 0; } tally;
+    const auto cb = sim.createCollisionCallback(
+        [](const CollisionEvent& e, Simulation&, void* user) {
+            Tally* t = static_cast<Tally*>(user);
+            t->events++;
+            t->maxImpulse = std::fmax(t->maxImpulse, e.impulse);
+            t->badNormals += std::fabs(e.normal.x * e.normal.x + e.normal.y * e.normal.y - 1.0f) > 1e-4f ? 1u : 0u;
+        },
+        &tally);
+    bool callbacksSet = cb.has_value() && !sim.setColliderCollisionCallback(5, 77, 0.0f);   // 77 is not alive
+    for (ObjectDenseIndex c = 4; c < 44; c++) callbacksSet = callbacksSet && sim.setColliderCollisionCallback(c, *cb, 0.0f);
+
     // a setter before the upload is applied at the first update
     const bool queuedEarly = objects.setBodyVelocity(10, { 0.0f, 2.0f });
     uint32_t steps = 0;
@@ -100,10 +113,12 @@ int main() {
         inside = inside && std::fabs(bodies.comX[i]) < 11.0f && bodies.comY[i] > -1.0f && bodies.comY[i] < 21.0f;
     }
     const agx_stats st = sim.getStats();
-    std::printf("bodies %d (capsules %d) steps %u manifolds %zu islands %u colours %u overflow %u rejected %d setters %d/%d/%d viewers %d\n",
+    std::printf("bodies %d (capsules %d) steps %u manifolds %zu islands %u colours %u overflow %u rejected %d setters %d/%d/%d viewers %d"
+                " events %u (max impulse %.3f)\n",
                 made, capsules, steps, manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0, queuedEarly ? 1 : 0,
-                queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0);
+                queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0, tally.events, tally.maxImpulse);
     const bool ok = made == 192 && steps == 241 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
-                    st.overflow == 0 && queuedEarly && queued && applied && viewers;
+                    st.overflow == 0 && queuedEarly && queued && applied && viewers && callbacksSet && tally.events > 0 &&
+                    tally.badNormals == 0 && tally.maxImpulse > 0.0f;
     return ok ? 0 : 1;
 }

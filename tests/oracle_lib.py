@@ -12,6 +12,10 @@ MANIFOLD_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("nx", "<f4"), ("ny", "<f
                            ("id0", "<u4"), ("id1", "<u4"), ("jn0", "<f4"), ("jt0", "<f4"), ("jn1", "<f4"),
                            ("jt1", "<f4")])
 
+EVENT_DTYPE = np.dtype([("self", "<u4"), ("other", "<u4"), ("self_body", "<u4"), ("other_body", "<u4"),
+                        ("px", "<f4"), ("py", "<f4"), ("nx", "<f4"), ("ny", "<f4"), ("impulse", "<f4"),
+                        ("callback", "<u4"), ("step", "<u4"), ("reserved", "<u4")])
+
 _P = C.c_void_p
 _SIGS = {
     "orc_new": (_P, []), "orc_free": (None, [_P]),
@@ -41,6 +45,10 @@ _SIGS = {
     "orc_plan_waves": (C.c_int64, [_P, _P, C.c_uint64, C.c_uint32, _P, _P, _P, C.c_uint64]),
     "orc_solve_threaded": (None, [_P, _P, _P, C.c_uint64, C.c_uint32, C.c_uint32]),
     "orc_narrow_threaded": (C.c_int64, [_P, _P, _P, _P, C.c_uint64, _P, C.c_uint64]),
+    "orc_set_callback": (None, [_P, C.c_uint32, C.c_uint32, C.c_float]),
+    "orc_set_events_enabled": (None, [_P, C.c_int]),
+    "orc_collision_events": (C.c_int64, [_P, _P, C.c_uint64, _P, C.c_uint64]),
+    "orc_get_events": (C.c_int64, [_P, _P, C.c_uint64]),
     "orc_mlp_forward": (None, [_P, _P, C.c_uint64, _P, C.c_int]),
     "orc_mlp_gradients": (C.c_double, [_P, _P, _P, C.c_uint64, _P, C.c_int]),
     "orc_mlp_loss": (C.c_double, [_P, _P, _P, _P, C.c_uint64]),
@@ -235,6 +243,30 @@ class World:
         out = np.empty(cap, MANIFOLD_DTYPE)
         n = lib().orc_get_manifolds(self.h, _p(out), cap)
         return out[:n]
+
+    def set_callback(self, collider, callback_id, threshold=0.0):
+        lib().orc_set_callback(self.h, collider, callback_id, threshold)
+
+    def set_events_enabled(self, on):
+        lib().orc_set_events_enabled(self.h, int(on))
+
+    def collision_events(self, m):
+        """Events of manifolds m on the current (pre-solve) state, in manifold order."""
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        cap = 2 * max(len(m), 1)
+        out = np.empty(cap, EVENT_DTYPE)
+        n = lib().orc_collision_events(self.h, _p(m), len(m), _p(out), cap)
+        return out[:n]
+
+    def events(self):
+        """Events recorded since the last update call."""
+        cap = 1 << 12
+        while True:
+            out = np.empty(cap, EVENT_DTYPE)
+            n = lib().orc_get_events(self.h, _p(out), cap)
+            if n >= 0:
+                return out[:n]
+            cap = -n
 
     def errors(self):
         out = np.zeros(2, np.float32)
