@@ -306,12 +306,31 @@ int agx_world_set_collider_callback(agx_world* w, uint32_t collider, uint32_t ca
  * NULL). AGX_ERR_CAPACITY when more events were kept than cap. */
 int agx_world_read_events(agx_world* w, agx_collision_event* out, uint64_t cap, uint64_t* count, uint64_t* dropped);
 
+/* ---- sleeping: ObjectManager::updateSleeping / sleepBody / wakeSleepIslands
+ * (SimulationImpl/ObjectManager.cpp:1314-1491), wakeIslandsOfContacts (ObjectManager.h:139-156),
+ * NarrowPhaseCollisionDetector::sleepContactData / wakeContactData
+ * (Narrow/NarrowPhaseCollisionDetector.cpp:299-414) and Simulation::wakeIslandsOfEditedStatics
+ * (Simulation.cpp:463-486). Sleeping is on once a step runs with a finite
+ * settings.sleep_time_threshold. After each step's islands, every awake dynamic body's timer grows
+ * by dt while |v| + |w| x (bound radius) <= sleep_speed_threshold and restarts at 0 otherwise; an
+ * island whose bodies all have timers >= sleep_time_threshold falls asleep, and its manifolds move
+ * to a stash. Sleeping bodies are not integrated and pair only with awake dynamic bodies; a manifold
+ * that touches one wakes its whole island before the solve, with the island's stashed manifolds
+ * appended to the step's. A deferred body command wakes its body's island at the next flush; an
+ * edited static body wakes the islands resting on it. Bodies keep their indices (the reference moves
+ * a sleeping body into a Sleeping group of its arrays): sleep_island[b] is 0 for an awake body and
+ * otherwise its island's sleep id, 2 + the island's smallest body index. */
+int agx_world_read_sleep(agx_world* w, uint32_t* sleep_island, float* timer, uint32_t n);
+/* The stash: manifolds of sleeping islands, with each one's sleep island id (either array may be
+ * NULL; both NULL: *count only). AGX_ERR_CAPACITY when more than cap (*count receives the total). */
+int agx_world_read_stash(agx_world* w, agx_manifold* out, uint32_t* island, uint64_t cap, uint64_t* count);
+
 /* ---- world snapshots. The reference has no on-disk or in-memory state format. A snapshot holds
  * everything a later step reads: body state (poses, velocities, cos/sin, masses and inertias as
  * edited), collider poses, AABBs and CCD sweep origins, island ids, the previous step's manifolds
  * (the warm-start source, its hash rebuilt on load), the device counters (statistics, gate), the
- * per-scene sweep counts, collider callbacks, the settings, the fixed clock's accumulator and the
- * queued body commands. Loading it into a world built from the same scenes (same bodies, colliders,
+ * per-scene sweep counts, collider callbacks, sleep ids and timers with the stash, the settings, the
+ * fixed clock's accumulator and the queued body commands. Loading it into a world built from the same scenes (same bodies, colliders,
  * shapes and materials; checked by a fingerprint) makes that world step to the same bits as the world
  * that saved it. The network history of the learned warm start is not kept (its next step runs
  * without one). Format: little-endian, a 64-byte header ("AGXSNAP1", version, counts, fingerprint,

@@ -246,6 +246,22 @@ void broadPhaseBvh(World& w) {
     for (uint32_t leaf : leaves) leafSelf(dyn, leaf, w, w.pairA, w.pairB);
     for (const NodePair& p : leafPairs) leafCross(dyn, p.a, dyn, p.b, true, w, w.pairA, w.pairB);
     for (const NodePair& p : crossLeafPairs) leafCross(dyn, p.a, stat, p.b, false, w, w.pairA, w.pairB);
+    // Sleeping colliders form the reference's third tree, queried by dynamic leaves only
+    // (BroadPhaseCollisionDetector.cpp:109-244): no pair of two sleeping bodies or of a sleeping
+    // and a static body.
+    if (!w.sleepId.empty()) {
+        size_t k = 0;
+        for (size_t i = 0; i < w.pairA.size(); i++) {
+            const uint32_t a = w.colBody[w.pairA[i]], b = w.colBody[w.pairB[i]];
+            const bool sa = asleep(w, a), sb = asleep(w, b);
+            if ((sa && (sb || w.invMass[b] == 0.0f)) || (sb && w.invMass[a] == 0.0f)) continue;
+            w.pairA[k] = w.pairA[i];
+            w.pairB[k] = w.pairB[i];
+            k++;
+        }
+        w.pairA.resize(k);
+        w.pairB.resize(k);
+    }
 }
 
 // Exact pair set by sweep-and-prune on minX: {strictly overlapping AABBs, different owner
@@ -267,6 +283,8 @@ void broadPhaseBrute(const World& w, std::vector<uint32_t>& pa, std::vector<uint
             const bool di = i >= w.staticColliders, dj = j >= w.staticColliders;
             if (!di && !dj) continue;
             if (w.colBody[i] == w.colBody[j]) continue;
+            const bool si = asleep(w, w.colBody[i]), sj = asleep(w, w.colBody[j]);
+            if ((si && (sj || !dj)) || (sj && !di)) continue;   // awake dynamic x sleeping only
             if (di && dj) { pa.push_back(std::min(i, j)); pb.push_back(std::max(i, j)); }
             else if (di) { pa.push_back(i); pb.push_back(j); }
             else { pa.push_back(j); pb.push_back(i); }

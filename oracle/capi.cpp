@@ -201,6 +201,12 @@ int64_t orc_get_manifolds(void* h, Manifold* out, uint64_t cap) {
     std::memcpy(out, w.manifolds.data(), w.manifolds.size() * sizeof(Manifold));
     return (int64_t)w.manifolds.size();
 }
+int64_t orc_get_prev_manifolds(void* h, Manifold* out, uint64_t cap) {
+    World& w = W(h);
+    if (w.prevManifolds.size() > cap) return -(int64_t)w.prevManifolds.size();
+    if (!w.prevManifolds.empty()) std::memcpy(out, w.prevManifolds.data(), w.prevManifolds.size() * sizeof(Manifold));
+    return (int64_t)w.prevManifolds.size();
+}
 void orc_errors(void* h, float* out) { out[0] = W(h).velocityError; out[1] = W(h).positionError; }
 
 // Simulation::setColliderCollisionCallback (Simulation.cpp:152-172): id 0 clears.
@@ -232,6 +238,52 @@ int64_t orc_get_events(void* h, Event* out, uint64_t cap) {
     if (w.events.size() > cap) return -(int64_t)w.events.size();
     if (!w.events.empty()) std::memcpy(out, w.events.data(), w.events.size() * sizeof(Event));
     return (int64_t)w.events.size();
+}
+
+// Sleeping: thresholds (SimulationSettings::sleepSpeedThreshold / sleepTimeThreshold), per-body
+// state, the stash, and the wake and sleep passes of one step on given manifolds.
+void orc_set_sleep(void* h, float speed, float time) {
+    W(h).settings.sleepSpeedThreshold = speed;
+    W(h).settings.sleepTimeThreshold = time;
+}
+void orc_get_sleep(void* h, uint32_t* ids, float* timers, uint32_t n) {
+    World& w = W(h);
+    for (uint32_t b = 0; b < n; b++) {
+        ids[b] = b < w.sleepId.size() ? w.sleepId[b] : 0u;
+        timers[b] = b < w.sleepTimer.size() ? w.sleepTimer[b] : 0.0f;
+    }
+}
+void orc_set_sleep_state(void* h, const uint32_t* ids, const float* timers, uint32_t n) {
+    W(h).sleepId.assign(ids, ids + n);
+    W(h).sleepTimer.assign(timers, timers + n);
+}
+int64_t orc_get_stash(void* h, Manifold* out, uint32_t* island, uint64_t cap) {
+    World& w = W(h);
+    if (w.stash.size() > cap) return -(int64_t)w.stash.size();
+    if (!w.stash.empty()) {
+        std::memcpy(out, w.stash.data(), w.stash.size() * sizeof(Manifold));
+        std::memcpy(island, w.stashIsland.data(), w.stash.size() * sizeof(uint32_t));
+    }
+    return (int64_t)w.stash.size();
+}
+void orc_set_stash(void* h, const Manifold* m, const uint32_t* island, uint64_t n) {
+    W(h).stash.assign(m, m + n);
+    W(h).stashIsland.assign(island, island + n);
+}
+// wakeContacts on m (n manifolds): out receives m plus the restored manifolds; returns the count.
+int64_t orc_wake_contacts(void* h, const Manifold* m, uint64_t n, Manifold* out, uint64_t cap) {
+    std::vector<Manifold> v(m, m + n);
+    wakeContacts(W(h), v);
+    if (v.size() > cap) return -(int64_t)v.size();
+    std::memcpy(out, v.data(), v.size() * sizeof(Manifold));
+    return (int64_t)v.size();
+}
+// updateSleeping after islands on m: out receives the manifolds kept awake; returns their count.
+int64_t orc_update_sleeping(void* h, float dt, const Manifold* m, uint64_t n, Manifold* out) {
+    std::vector<Manifold> v(m, m + n);
+    updateSleeping(W(h), dt, v);
+    std::memcpy(out, v.data(), v.size() * sizeof(Manifold));
+    return (int64_t)v.size();
 }
 
 void orc_queue_body(void* h, uint32_t kind, uint32_t body, float a, float b) { W(h).commands.push_back({ kind, body, a, b }); }

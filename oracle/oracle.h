@@ -63,6 +63,8 @@ struct Settings {
     uint32_t positionIterations = 3;   // Simulation.h:57
     bool ccd = true;                   // Simulation.h:66
     bool collisionEvents = true;       // Simulation.h:60
+    float sleepSpeedThreshold = 0.1f;  // Simulation.h:71
+    float sleepTimeThreshold = 3.40282347e38f;   // Simulation.h:72 (0.5 s there); the default here keeps sleeping off
     float ccdFastFactor = 1.0f;        // CCD/ContinuousCollisionDetector.h:31
     float ccdCoreFraction = 0.5f;      // CCD/ContinuousCollisionDetector.h:32
 };
@@ -111,6 +113,14 @@ struct World {
     std::vector<float> colThreshold;
     std::vector<Event> events;
     uint32_t eventStep = 0;
+    // Sleeping (ObjectManager.cpp:1314-1491) with stable indices: a body sleeps in place, with
+    // sleepId = its sleep island's id (>= 2, FIRST_SLEEP_ISLAND_ID, BodySoA.h:45: here 2 + the
+    // island's smallest body index), 0 when awake; the manifolds of a sleeping island are kept in
+    // the stash with their island id (NarrowPhaseCollisionDetector.cpp:299-346) until it wakes.
+    std::vector<uint32_t> sleepId;
+    std::vector<float> sleepTimer;
+    std::vector<Manifold> stash;
+    std::vector<uint32_t> stashIsland;
 
     uint32_t bodyCount() const { return (uint32_t)comX.size(); }
     uint32_t colliderCount() const { return (uint32_t)colBody.size(); }
@@ -154,6 +164,21 @@ void islands(World& w, const std::vector<Manifold>& m);       // IslandManager.c
 // manifolds in order[c] (null: index order) on the current, pre-warm-start velocities; appends to
 // w.events with step index w.eventStep. No-op unless events are on and some collider has a callback.
 void collisionEvents(World& w, const std::vector<Manifold>& m, const uint32_t* order);
+// ---- sleeping (oracle/sleep.cpp) ----
+bool asleep(const World& w, uint32_t body);
+// ObjectManager::wakeIslandsOfContacts (ObjectManager.h:139-156) on this step's manifolds, then
+// wakeContactData (NarrowPhaseCollisionDetector.cpp:348-414): the woken islands' stashed manifolds
+// are appended to `cur` in stash order.
+void wakeContacts(World& w, std::vector<Manifold>& cur);
+// ObjectManager::updateSleeping (ObjectManager.cpp:1438-1491) after the islands, then
+// sleepContactData (NarrowPhaseCollisionDetector.cpp:299-346) on `cur`.
+void updateSleeping(World& w, float dt, std::vector<Manifold>& cur);
+// Wake paths of ObjectManager::flushCommands (ObjectManager.cpp:638-700) and
+// Simulation::wakeIslandsOfEditedStatics (Simulation.cpp:463-486): commanded bodies wake their
+// islands; stashed manifolds of edited static bodies' colliders are dropped and their islands woken;
+// awake partners of those colliders restart their timers. Restored manifolds go to prevManifolds.
+void wakeForCommands(World& w, const std::vector<uint32_t>& commandedBodies, const std::vector<uint32_t>& editedStatic);
+
 // ---- threaded step (the CPU baseline's structure; oracle/threaded.cpp) ----
 struct Pool;                                                  // pinned std::threads + the caller
 Pool* poolCreate(uint32_t participants);

@@ -66,11 +66,13 @@ void integrate(World& w, float dt) {
     const float halfDT = dt * 0.5f;
     const float gx = w.settings.gravityX * halfDT, gy = w.settings.gravityY * halfDT;
     for (uint32_t i = w.staticBodies; i < w.bodyCount(); i++) {
+        if (asleep(w, i)) continue;   // the Dynamic range only (sleeping bodies are not in it)
         const float movable = w.invMass[i] != 0.0f ? 1.0f : 0.0f;
         w.velX[i] += gx * movable;
         w.velY[i] += gy * movable;
     }
     for (uint32_t i = w.staticBodies; i < w.bodyCount(); i++) {
+        if (asleep(w, i)) continue;
         w.comX[i] = w.velX[i] * dt + w.comX[i];
         w.comY[i] = w.velY[i] * dt + w.comY[i];
         w.rot[i] = w.angVel[i] * dt + w.rot[i];
@@ -320,10 +322,12 @@ void islands(World& w, const std::vector<Manifold>& m) {
         return x;
     };
     for (uint32_t i = 0; i < w.staticBodies; i++) w.islandId[i] = UINT32_MAX;
+    std::vector<uint8_t> member(count, 1);   // IslandManager::reset covers the Dynamic range only
+    for (uint32_t i = 0; i < count; i++) member[i] = asleep(w, i + dynBegin) ? 0 : 1;
     for (const Manifold& mf : m) {
         const uint32_t a = w.colBody[mf.a] - dynBegin;
         const uint32_t b = w.colBody[mf.b] - dynBegin;
-        if (a >= count || b >= count) continue;
+        if (a >= count || b >= count || !member[a] || !member[b]) continue;
         uint32_t ra = find(a), rb = find(b);
         if (ra == rb) continue;
         if (parent[ra] > parent[rb]) std::swap(ra, rb);
@@ -334,12 +338,13 @@ void islands(World& w, const std::vector<Manifold>& m) {
     w.islandStart.assign(count + 1, 0);
     uint32_t k = 0;
     for (uint32_t i = 0; i < count; i++)
-        if (parent[i] < 0) { w.islandStart[k] = (uint32_t)(-parent[i]); rootToId[i] = k; k++; }
+        if (parent[i] < 0 && member[i]) { w.islandStart[k] = (uint32_t)(-parent[i]); rootToId[i] = k; k++; }
     uint32_t sum = 0;
     for (uint32_t j = 0; j < k; j++) { const uint32_t s = w.islandStart[j]; w.islandStart[j] = sum; sum += s; }
     w.islandStart[k] = sum;
-    w.islandMembers.assign(count, 0);
+    w.islandMembers.assign(sum, 0);
     for (uint32_t i = 0; i < count; i++) {
+        if (!member[i]) { w.islandId[i + dynBegin] = UINT32_MAX; continue; }
         const uint32_t id = rootToId[find(i)];
         w.islandId[i + dynBegin] = id;
         w.islandMembers[w.islandStart[id]++] = i + dynBegin;
@@ -362,8 +367,11 @@ float wrapAngle(float angle) {   // ObjectManager.cpp:614-619
 // ObjectManager::flushCommands / applyCommand (ObjectManager.cpp:638-865), body commands only; a
 // position or rotation edit makes the next step rebuild every collider's pose (spatialDirty).
 void flushCommands(World& w) {
+    std::vector<uint32_t> commanded, editedStatic;
     for (const World::BodyCommand& c : w.commands) {
         const uint32_t b = c.body;
+        commanded.push_back(b);   // every command wakes its body's island (ObjectManager.cpp:684-693)
+        if ((c.kind == 0 || c.kind == 1) && w.invMass[b] == 0.0f) editedStatic.push_back(b);   // markStaticBodyEdited
         switch (c.kind) {
         case 0: {
             const float r = w.rot[b];
@@ -405,6 +413,7 @@ void flushCommands(World& w) {
         }
     }
     w.commands.clear();
+    if (!commanded.empty()) wakeForCommands(w, commanded, editedStatic);
 }
 
 // ---------------------------------------------------------------- step
@@ -415,7 +424,7 @@ void postUpdate(World& w) {
     w.postUpdateDone = true;
 }
 
-// Simulation.cpp:255-351 with sleeping, joints, the mouse holder and events off.
+// Simulation.cpp:255-351 (joints and the mouse holder are not on this path).
 void step(World& w, float dt, Pool* pool) {
     if (w.bodyCount() == 0) { w.prevManifolds.clear(); w.manifolds.clear(); return; }
     if (w.settings.ccd) snapshotPoses(w);
@@ -437,6 +446,7 @@ void step(World& w, float dt, Pool* pool) {
             if (pool) narrowPhaseThreaded(w, w.pairA.data(), w.pairB.data(), w.pairA.size(), w.manifolds, *pool);
             else narrowPhase(w, w.pairA.data(), w.pairB.data(), w.pairA.size(), w.manifolds);
             warmStartFromCache(w, w.manifolds);
+            wakeContacts(w, w.manifolds);   // Simulation.cpp:428-434, before the solve
             if (!w.manifolds.empty()) {
                 collisionEvents(w, w.manifolds, nullptr);
                 if (pool) solveThreaded(w, w.manifolds, *pool, w.settings.velocityIterations, w.settings.positionIterations);
@@ -448,6 +458,8 @@ void step(World& w, float dt, Pool* pool) {
         w.prevManifolds.clear();
     }
     islands(w, w.manifolds);
+    updateSleeping(w, dt, w.manifolds);   // Simulation.cpp:348-350: sleep, then stash the sleepers' manifolds
+    w.prevManifolds = w.manifolds;
     w.postUpdateDone = false;
     bool anyCallback = false;
     for (uint32_t cb : w.colCallback) anyCallback = anyCallback || cb != 0;

@@ -174,6 +174,8 @@ _SIGS = {
     "agx_world_snapshot_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "agx_world_save": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_load": (C.c_int, [_P, _P, C.c_uint64]),
+    "agx_world_read_sleep": (C.c_int, [_P, _P, _P, C.c_uint32]),
+    "agx_world_read_stash": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_read_events": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
 EXPORTED = tuple(_SIGS)
@@ -437,6 +439,24 @@ class World:
         elif rc:
             _check(rc, self._h)
         return out[:cnt.value], dropped.value
+
+    def sleep_state(self):
+        """(sleep island id per body, 0 when awake; sleep timer per body)."""
+        n = self.counts.bodies
+        ids = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        _check(lib().agx_world_read_sleep(self._h, _ptr(ids), _ptr(t), n), self._h)
+        return ids, t
+
+    def stash(self):
+        """Manifolds of sleeping islands and their sleep island ids."""
+        cnt = C.c_uint64()
+        _check(lib().agx_world_read_stash(self._h, None, None, 0, C.byref(cnt)), self._h)
+        m = np.empty(max(cnt.value, 1), MANIFOLD_DTYPE)
+        isl = np.empty(max(cnt.value, 1), np.uint32)
+        if cnt.value:
+            _check(lib().agx_world_read_stash(self._h, _ptr(m), _ptr(isl), cnt.value, C.byref(cnt)), self._h)
+        return m[:cnt.value], isl[:cnt.value]
 
     def save(self) -> bytes:
         """The world's state as a snapshot (agx_world_save)."""

@@ -77,6 +77,7 @@ struct Counters {
     unsigned long long convergedBodiesTotal;   // sum over those steps of convergedBodies
     uint32_t eventCount;       // collision events recorded since the update/step call began (may exceed the capacity)
     uint32_t eventStep;        // physics steps with event recording since the call began
+    uint32_t stashCount;       // manifolds of sleeping islands kept in the stash
     // per step
     uint32_t pairs;            // total pairs found (may exceed capacity -> overflow)
     uint32_t manifolds;
@@ -110,6 +111,12 @@ struct Counters {
     uint32_t ccdCandidates;    // this step: fast dynamic colliders the CCD casts
     uint32_t flowMet;          // this step: the dataflow solve met the residual target
     uint32_t eventStepTotal;   // this step: collision events recorded
+    uint32_t wokeAny;          // this step: a manifold touched a sleeping body
+    uint32_t sleptAny;         // this step: a body fell asleep
+    uint32_t stashScanCount;   // this step: stash entries scanned by the wake (0 when nothing woke)
+    uint32_t restoreTotal;     // this step: stashed manifolds restored
+    uint32_t sleepScanCount;   // this step: manifolds scanned by the sleep (0 when nothing slept)
+    uint32_t stashTotal;       // this step: manifolds moved to the stash
 };
 
 constexpr uint32_t OVERFLOW_PAIRS = 1u;       // pair buffer too small (error)
@@ -328,6 +335,16 @@ struct DeviceWorld {
     float* evImpulse = nullptr;      // per manifold: impact impulse
     agx_collision_event* events = nullptr;
     uint32_t eventCap = 0;
+    // sleeping (allocated when a step first runs with a finite sleep time threshold; null: off)
+    uint32_t* sleepId = nullptr;     // per body: sleep island id (>= 2) or 0 when awake
+    float* sleepTimer = nullptr;     // per body: time quiet
+    float* boundRadius = nullptr;    // per body: reach of its colliders from the centre of mass
+    uint32_t* woken = nullptr;       // per sleep id (id - 2 < N): woken this step
+    uint32_t* islandFlag = nullptr;  // per island: all bodies ready to sleep
+    uint32_t* islandMin = nullptr;   // per island: smallest body
+    agx_manifold* stash = nullptr;   // manifolds of sleeping islands (manCap)
+    uint32_t* stashIsland = nullptr; // their sleep ids
+    uint32_t* stashIslandAlt = nullptr;   // compaction scratch
 };
 
 // ---------------------------------------------------------------- device math
@@ -484,6 +501,9 @@ hipError_t launchSolve(hipStream_t st, DeviceWorld& w, uint32_t vIt, uint32_t pI
 hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, float residualTarget, int trackErrors,
                              int perScene);
 hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
+// Sleeping (k_sleep.hip): the wake after the narrow phase, the sleep after the islands.
+hipError_t launchWake(hipStream_t st, DeviceWorld& w);
+hipError_t launchSleep(hipStream_t st, DeviceWorld& w, float dt, float speedThreshold, float timeThreshold);
 // Collision events of the step's manifolds on its pre-warm-start velocities (k_events.hip).
 hipError_t launchEvents(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;

@@ -502,7 +502,8 @@ template <bool SELF, bool STAGED>
 __global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_t* __restrict__ colScene,
                                                  uint32_t* __restrict__ waveCount, const uint32_t* __restrict__ waveOffset,
                                                  uint32_t part, uint2* __restrict__ pairs, uint64_t cap,
-                                                 uint2* __restrict__ staged, Counters* ctr) {
+                                                 uint2* __restrict__ staged, Counters* ctr,
+                                                 const uint32_t* __restrict__ sleepId) {
     __shared__ uint32_t stackMem[4][TRAVERSAL_STACK];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     uint32_t* stack = stackMem[wid];
@@ -521,11 +522,17 @@ __global__ __launch_bounds__(256) void kTraverse(Tree qt, Tree tt, const uint32_
     const uint32_t scene = q.valid ? colScene[q.col] : 0u;
     q.sf = q.valid ? tt.sceneStart[scene] : 0u;
     q.sl = q.valid ? tt.sceneStart[scene + 1] : 0u;
+    // a sleeping query pairs only with awake dynamic bodies: nothing against the static tree, and
+    // in the self pass only leaves whose body is awake (the reference keeps sleeping colliders in a
+    // third tree that only dynamic leaves query, BroadPhaseCollisionDetector.cpp:109-244)
+    const bool qAsleep = sleepId && q.valid && sleepId[q.body] != 0u;
+    if (!SELF && qAsleep) q.valid = false;
     uint32_t written = 0;
     const uint64_t outBase = STAGED ? 0ull : (uint64_t)waveOffset[region];
     uint2* stage = staged + (size_t)region * STAGE_PAIRS;
     auto emit = [&](uint32_t other, uint32_t otherBody, bool act) {
         if (SELF) act = act && otherBody != q.body;
+        if (SELF && qAsleep && act) act = sleepId[otherBody] == 0u;
         const uint64_t m = __ballot(act);
         if (act) {
             const uint32_t k = written + (uint32_t)__popcll(m & ltMask);
@@ -760,18 +767,18 @@ hipError_t launchPairs(hipStream_t st, DeviceWorld& w) {
     const uint32_t waves = w.queryWaves;          // ceil(nDyn / 64)
     const uint32_t blocks = blocksFor(waves, 4);
     hipLaunchKernelGGL((kTraverse<true, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
-                       w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+                       w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr, w.sleepId);
     hipLaunchKernelGGL((kTraverse<false, true>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
-                       w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+                       w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr, w.sleepId);
     uint32_t* total = &w.ctr->scratch[0];
     hipError_t e = launchScan(st, w.waveCount, w.waveOffset, nullptr, total, 2ull * waves, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kGatherPairs, dim3(blocksFor(2 * waves, 4)), dim3(256), 0, st, w.waveCount, w.waveOffset, 2 * waves,
                        w.stagedPairs, w.pairs, w.pairCap, w.ctr, total);
     hipLaunchKernelGGL((kTraverse<true, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.dyn, w.colScene, w.waveCount,
-                       w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+                       w.waveOffset, 0u, w.pairs, w.pairCap, w.stagedPairs, w.ctr, w.sleepId);
     hipLaunchKernelGGL((kTraverse<false, false>), dim3(blocks), dim3(256), 0, st, w.dyn, w.stat, w.colScene, w.waveCount,
-                       w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr);
+                       w.waveOffset, 1u, w.pairs, w.pairCap, w.stagedPairs, w.ctr, w.sleepId);
     return hipGetLastError();
 }
 

@@ -1594,13 +1594,16 @@ __device__ inline uint32_t ufRootOf(const uint32_t* parent, uint32_t x) {
 }
 
 // root[b] (kept in islandId as scratch) and a root flag per dynamic body for the rank scan.
+// Sleeping bodies join no island (IslandManager::reset covers the Dynamic range only); they touch
+// no manifold of the step, so each is its own root and drops out of the ranks.
 __global__ __launch_bounds__(256) void kUfRoots(const uint32_t* __restrict__ parent, const uint32_t* __restrict__ list,
-                                                uint32_t n, uint32_t* __restrict__ rootFlag, uint32_t* __restrict__ rootOf) {
+                                                uint32_t n, uint32_t* __restrict__ rootFlag, uint32_t* __restrict__ rootOf,
+                                                const uint32_t* __restrict__ sleepId) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t b = list[k];
         const uint32_t r = ufRootOf(parent, b);
         rootOf[b] = r;
-        rootFlag[k] = (r == b) ? 1u : 0u;
+        rootFlag[k] = (r == b && !(sleepId && sleepId[b] != 0u)) ? 1u : 0u;
     }
 }
 
@@ -1608,10 +1611,11 @@ __global__ __launch_bounds__(256) void kUfRoots(const uint32_t* __restrict__ par
 // root in the list is its rank among roots); static bodies keep UINT32_MAX.
 __global__ __launch_bounds__(256) void kIslandIds(const uint32_t* __restrict__ list, uint32_t n,
                                                   const uint32_t* __restrict__ rootScan,
-                                                  const uint32_t* __restrict__ bodyToDyn, uint32_t* __restrict__ islandId) {
+                                                  const uint32_t* __restrict__ bodyToDyn, uint32_t* __restrict__ islandId,
+                                                  const uint32_t* __restrict__ sleepId) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t b = list[k];
-        islandId[b] = rootScan[bodyToDyn[islandId[b]]];
+        islandId[b] = (sleepId && sleepId[b] != 0u) ? 0xFFFFFFFFu : rootScan[bodyToDyn[islandId[b]]];
     }
 }
 
@@ -1640,7 +1644,10 @@ __global__ __launch_bounds__(ISLAND_THREADS) void kSceneIslands(Constraints cs, 
                                                                 const uint32_t* __restrict__ sceneBodyStart,
                                                                 const uint2* __restrict__ range, const Counters* __restrict__ ctr,
                                                                 uint32_t* __restrict__ islandId,
-                                                                uint32_t* __restrict__ sceneIslands) {
+                                                                uint32_t* __restrict__ sceneIslands,
+                                                                const uint32_t* __restrict__ sleepId) {
+    // a member: a dynamic body that is awake (sleeping bodies touch no manifold and join no island)
+    auto member = [&](uint32_t body) { return inv[body].x != 0.0f && !(sleepId && sleepId[body] != 0u); };
     extern __shared__ uint32_t par[];   // the scene's bodies, then the root ranks
     __shared__ uint32_t waveSum[ISLAND_THREADS / WAVE];
     const uint32_t s = blockIdx.x;
@@ -1673,7 +1680,7 @@ __global__ __launch_bounds__(ISLAND_THREADS) void kSceneIslands(Constraints cs, 
     for (uint32_t i = lo; i < hi; i++) {
         uint32_t x = i, p = par[x];
         while (p != x) { x = p; p = par[x]; }
-        roots += (x == i && inv[b0 + i].x != 0.0f) ? 1u : 0u;
+        roots += (x == i && member(b0 + i)) ? 1u : 0u;
     }
     uint32_t incl = roots;
     const uint32_t lane = t & (WAVE - 1), wid = t / WAVE;
@@ -1694,13 +1701,13 @@ __global__ __launch_bounds__(ISLAND_THREADS) void kSceneIslands(Constraints cs, 
     for (uint32_t i = lo; i < hi; i++) {
         uint32_t x = i, p = par[x];
         while (p != x) { x = p; p = par[x]; }
-        if (x == i && inv[b0 + i].x != 0.0f) rankOf[i] = next++;
+        if (x == i && member(b0 + i)) rankOf[i] = next++;
     }
     __syncthreads();
     for (uint32_t i = t; i < n; i += ISLAND_THREADS) {
         uint32_t x = i, p = par[x];
         while (p != x) { x = p; p = par[x]; }
-        islandId[b0 + i] = inv[b0 + i].x != 0.0f ? rankOf[x] : 0xFFFFFFFFu;
+        islandId[b0 + i] = member(b0 + i) ? rankOf[x] : 0xFFFFFFFFu;
     }
     if (t == 0) sceneIslands[s] = total;
 }
@@ -1958,7 +1965,7 @@ hipError_t launchSolveFinish(hipStream_t st, DeviceWorld& w, int trackResidual, 
 hipError_t launchSceneIslands(hipStream_t st, DeviceWorld& w) {
     const uint32_t bytes = 2 * std::max<uint32_t>(w.sceneMaxBodies, 1) * sizeof(uint32_t);
     hipLaunchKernelGGL(kSceneIslands, dim3(w.S), dim3(ISLAND_THREADS), bytes, st, w.cs, w.inv, w.sceneBodyStart, w.sceneRange,
-                       w.ctr, w.islandId, w.sceneIslands);
+                       w.ctr, w.islandId, w.sceneIslands, w.sleepId);
     hipError_t e = launchScan(st, w.sceneIslands, w.sceneIslandOffset, nullptr, &w.ctr->islands, w.S, w.scanBlock,
                               w.scanBlockCap);
     if (e != hipSuccess) return e;
@@ -1983,11 +1990,11 @@ hipError_t launchIslands(hipStream_t st, DeviceWorld& w) {
                        w.man[w.cur], w.ctr, w.colBody, w.inv, w.ufParent);
     if (w.nDynBodies == 0) return launchFillU32(st, &w.ctr->islands, 0u, 1);
     hipLaunchKernelGGL(kUfRoots, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.ufParent, w.dynBodies, w.nDynBodies,
-                       w.ufRoot, w.islandId);
+                       w.ufRoot, w.islandId, w.sleepId);
     hipError_t e = launchScan(st, w.ufRoot, w.ufRoot, nullptr, &w.ctr->scratch[3], w.nDynBodies, w.scanBlock, w.scanBlockCap);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kIslandIds, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.dynBodies, w.nDynBodies, w.ufRoot,
-                       w.bodyToDyn, w.islandId);
+                       w.bodyToDyn, w.islandId, w.sleepId);
     hipLaunchKernelGGL(kIslandCount, dim3(1), dim3(1), 0, st, w.ctr);
     return hipGetLastError();
 }

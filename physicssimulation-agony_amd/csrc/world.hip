@@ -38,7 +38,8 @@ struct agx_world {
     std::string err;
     std::vector<void*> allocations;
     std::vector<float> mass, inertia, localComX, localComY;   // per body, constant
-    std::vector<uint32_t> colType, colMaterial, colSlot, colScene, polyStart, polyCount;
+    std::vector<uint32_t> colType, colMaterial, colSlot, colScene, polyStart, polyCount, colBodyHost;
+    std::vector<float> boundRadius;  // per body: the reach of its colliders (ObjectManager.cpp:30-43)
     std::vector<float> colP0, colP1, colMinExt, colMaxExt;
     uint32_t staticBodies = 0, staticColliders = 0;
     float accumulator = 0.0f;
@@ -240,8 +241,94 @@ float wrapAngle(float angle) {   // ObjectManager.cpp:614-619
 // body commands, in queue order, on host copies of the body arrays. Collider poses are not
 // refreshed here: as in the reference, the next step's CCD snapshot still holds the pose before
 // the edit, and the step's pose pass computes the new one (static bodies: a full rebuild).
+// The wake paths of a flush (ObjectManager::flushCommands, ObjectManager.cpp:638-700, and
+// Simulation::wakeIslandsOfEditedStatics, Simulation.cpp:463-486), on host copies: every commanded
+// body wakes its island; stashed manifolds of an edited static body's colliders are dropped and
+// their islands woken; awake partners of those colliders in the warm-start source restart their
+// timers. Woken islands' stashed manifolds are appended, in stash order, to the warm-start source
+// (wakeSleepingContactData), whose hash is then rebuilt.
+int wakeForCommands(agx_world* w, const std::vector<uint32_t>& commanded, const std::vector<uint32_t>& editedStatic) {
+    agx::DeviceWorld& d = w->d;
+    if (!d.sleepId) return AGX_OK;
+    const uint32_t n = d.N;
+    std::vector<uint32_t> ids(n);
+    std::vector<float> timer(n);
+    HIP_CHECK(w, hipMemcpy(ids.data(), d.sleepId, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_CHECK(w, hipMemcpy(timer.data(), d.sleepTimer, n * sizeof(float), hipMemcpyDeviceToHost));
+    agx::Counters c;
+    HIP_CHECK(w, hipMemcpy(&c, d.ctr, sizeof c, hipMemcpyDeviceToHost));
+    std::vector<agx_manifold> stash(c.stashCount), prev(c.prevManifolds);
+    std::vector<uint32_t> stashIsland(c.stashCount);
+    if (c.stashCount) {
+        HIP_CHECK(w, hipMemcpy(stash.data(), d.stash, stash.size() * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+        HIP_CHECK(w, hipMemcpy(stashIsland.data(), d.stashIsland, stash.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    if (c.prevManifolds)
+        HIP_CHECK(w, hipMemcpy(prev.data(), d.man[1 - d.cur], prev.size() * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> woken(n, 0);
+    bool any = false;
+    for (uint32_t b : commanded)
+        if (ids[b]) { woken[ids[b] - 2] = 1; any = true; }
+    std::vector<uint8_t> edited(d.C, 0);
+    for (uint32_t b : editedStatic)
+        for (uint32_t col = 0; col < d.C; col++)
+            if (w->colBodyHost[col] == b) edited[col] = 1;
+    std::vector<uint32_t> partners;
+    if (!editedStatic.empty()) {
+        size_t k = 0;
+        for (size_t i = 0; i < stash.size(); i++) {
+            if (edited[stash[i].a] || edited[stash[i].b]) { woken[stashIsland[i] - 2] = 1; any = true; continue; }
+            stash[k] = stash[i];
+            stashIsland[k] = stashIsland[i];
+            k++;
+        }
+        stash.resize(k);
+        stashIsland.resize(k);
+        for (const agx_manifold& m : prev)
+            if (edited[m.a] != edited[m.b]) partners.push_back(w->colBodyHost[edited[m.a] ? m.b : m.a]);
+    }
+    if (!any && partners.empty() && editedStatic.empty()) return AGX_OK;
+    const size_t prevBefore = prev.size();
+    if (any) {
+        for (uint32_t b = 0; b < n; b++)
+            if (ids[b] && woken[ids[b] - 2]) { ids[b] = 0; timer[b] = 0.0f; }
+        size_t k = 0;
+        for (size_t i = 0; i < stash.size(); i++) {
+            if (woken[stashIsland[i] - 2]) { prev.push_back(stash[i]); continue; }
+            stash[k] = stash[i];
+            stashIsland[k] = stashIsland[i];
+            k++;
+        }
+        stash.resize(k);
+        stashIsland.resize(k);
+    }
+    for (uint32_t b : partners)
+        if (w->mass[b] > 0.0f && ids[b] == 0) timer[b] = 0.0f;
+    if (prev.size() > d.manCap) return fail(w, AGX_ERR_CAPACITY, "restored manifolds exceed the manifold capacity");
+    HIP_CHECK(w, hipMemcpy(d.sleepId, ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_CHECK(w, hipMemcpy(d.sleepTimer, timer.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    if (!stash.empty()) {
+        HIP_CHECK(w, hipMemcpy(d.stash, stash.data(), stash.size() * sizeof(agx_manifold), hipMemcpyHostToDevice));
+        HIP_CHECK(w, hipMemcpy(d.stashIsland, stashIsland.data(), stash.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    const uint32_t stashCount = (uint32_t)stash.size();
+    HIP_CHECK(w, hipMemcpy(&d.ctr->stashCount, &stashCount, sizeof stashCount, hipMemcpyHostToDevice));
+    if (prev.size() != prevBefore) {   // the warm-start source grew: rebuild its hash as a step's FINISH does
+        HIP_CHECK(w, hipMemcpy(d.man[1 - d.cur], prev.data(), prev.size() * sizeof(agx_manifold), hipMemcpyHostToDevice));
+        const uint32_t m = (uint32_t)prev.size();
+        HIP_CHECK(w, hipMemcpy(&d.ctr->manifolds, &m, sizeof m, hipMemcpyHostToDevice));
+        d.cur ^= 1;
+        HIP_CHECK(w, agx::launchHashBuild(w->stream, d));
+        d.cur ^= 1;
+        HIP_CHECK(w, hipStreamSynchronize(w->stream));
+        w->histReady = false;
+    }
+    return AGX_OK;
+}
+
 int flushCommands(agx_world* w) {
     if (w->commands.empty()) return AGX_OK;
+    std::vector<uint32_t> commanded, editedStatic;
     agx::DeviceWorld& d = w->d;
     const uint32_t n = d.N;
     HIP_CHECK(w, hipStreamSynchronize(w->stream));
@@ -254,6 +341,8 @@ int flushCommands(agx_world* w) {
     for (const agx_world::BodyCommand& c : w->commands) {
         const uint32_t b = c.body;
         const bool isStatic = inv[b].x == 0.0f;
+        commanded.push_back(b);   // every command wakes its body's island (ObjectManager.cpp:684-693)
+        if (isStatic && (c.kind == AGX_SET_POSITION || c.kind == AGX_SET_ROTATION)) editedStatic.push_back(b);
         switch (c.kind) {
         case AGX_SET_POSITION: {
             const float r = pos[b].z;
@@ -303,7 +392,7 @@ int flushCommands(agx_world* w) {
     HIP_CHECK(w, hipMemcpy(d.vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     HIP_CHECK(w, hipMemcpy(d.rcs, rcs.data(), n * sizeof(float2), hipMemcpyHostToDevice));
     HIP_CHECK(w, hipMemcpy(d.inv, inv.data(), n * sizeof(float2), hipMemcpyHostToDevice));
-    return AGX_OK;
+    return wakeForCommands(w, commanded, editedStatic);
 }
 
 int runPhase(agx_world* w, uint32_t phase, float dt) {
@@ -359,6 +448,8 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
         w->learnedNarrow = learned;
         SegmentTimer t(w, SEG_NARROW);
         HIP_CHECK(w, agx::launchNarrow(st, d));
+        // ObjectManager::wakeIslandsOfContacts + wakeSleepingContactData (Simulation.cpp:428-434)
+        if (d.sleepId && !s.collide_only) HIP_CHECK(w, agx::launchWake(st, d));
         break;
     }
     case AGX_PHASE_SOLVE:
@@ -416,6 +507,9 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
         SegmentTimer t(w, SEG_ISLANDS);
         if (w->solvedPerScene && !s.collide_only) HIP_CHECK(w, agx::launchSceneIslands(st, d));
         else HIP_CHECK(w, agx::launchIslands(st, d));
+        // updateSleeping + sleepContactData (Simulation.cpp:348-350)
+        if (d.sleepId && !s.collide_only)
+            HIP_CHECK(w, agx::launchSleep(st, d, dt, s.sleep_speed_threshold, s.sleep_time_threshold));
         break;
     }
     case AGX_PHASE_FINISH: {
@@ -433,11 +527,37 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
     return AGX_OK;
 }
 
+// Sleeping state, from the first call with a finite sleep time threshold on (every body awake, the
+// stash empty); it stays once allocated, so islands asleep when sleeping is switched off still wake.
+int allocSleep(agx_world* w) {
+    agx::DeviceWorld& d = w->d;
+    const uint64_t n = d.N, nd = std::max(1u, d.nDynBodies);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    HIP_CHECK(w, devAlloc(w, &d.sleepTimer, n));
+    HIP_CHECK(w, upload(w, &d.boundRadius, w->boundRadius));
+    HIP_CHECK(w, devAlloc(w, &d.woken, n));
+    HIP_CHECK(w, devAlloc(w, &d.islandFlag, nd));
+    HIP_CHECK(w, devAlloc(w, &d.islandMin, nd));
+    HIP_CHECK(w, devAlloc(w, &d.stash, d.manCap));
+    HIP_CHECK(w, devAlloc(w, &d.stashIsland, d.manCap));
+    HIP_CHECK(w, devAlloc(w, &d.stashIslandAlt, d.manCap));
+    uint32_t* ids = nullptr;
+    HIP_CHECK(w, devAlloc(w, &ids, n));
+    HIP_CHECK(w, agx::launchFillU32(w->stream, ids, 0u, n));
+    HIP_CHECK(w, agx::launchFillU32(w->stream, reinterpret_cast<uint32_t*>(d.sleepTimer), 0u, n));
+    HIP_CHECK(w, agx::launchFillBytes(w->stream, &d.ctr->stashCount, sizeof(uint32_t)));
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    d.sleepId = ids;
+    return AGX_OK;
+}
+
 // Start of an update/step call (or of a step driven phase by phase): whether its steps record
 // collision events, with the event list and step index cleared on the device (a fill kernel, so
 // nothing here is a memset) and the event buffer sized to the settings.
 int beginCall(agx_world* w) {
     agx::DeviceWorld& d = w->d;
+    if (!d.sleepId && std::isfinite(w->settings.sleep_time_threshold))
+        if (int r = allocSleep(w)) return r;
     w->recordEvents = w->settings.collision_events != 0 && w->callbackColliders > 0;
     if (!w->recordEvents) return AGX_OK;
     const uint32_t cap = std::max(1u, w->settings.event_capacity);
@@ -478,7 +598,7 @@ std::vector<unsigned char> graphKey(const agx_world* w, float dt) {
     const unsigned char flags[4] = { (unsigned char)w->histReady, (unsigned char)w->postUpdatePending,
                                      (unsigned char)w->d.cur, (unsigned char)w->recordEvents };
     put(flags, sizeof flags);
-    const void* bufs[4] = { w->d.learnX, w->d.warmHist[0], w->d.histOf, w->d.events };
+    const void* bufs[5] = { w->d.learnX, w->d.warmHist[0], w->d.histOf, w->d.events, w->d.sleepId };
     put(bufs, sizeof bufs);
     const int shape[2] = { w->d.colourBlocks, w->d.solveBlocks };
     put(shape, sizeof shape);
@@ -723,6 +843,10 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
             w->polyStart.push_back(ps); w->polyCount.push_back(sd.polyCount[c]);
             w->colP0.push_back(sd.p0[c]); w->colP1.push_back(sd.p1[c]);
             w->colMinExt.push_back(sd.minExtent[c]); w->colMaxExt.push_back(sd.maxExtent[c]);
+            w->colBodyHost.push_back(gb);
+            if (w->boundRadius.size() < pos.size()) w->boundRadius.resize(pos.size(), 0.0f);
+            w->boundRadius[gb] = std::fmax(w->boundRadius[gb], std::sqrt(sd.comOffX[c] * sd.comOffX[c] +
+                                                                          sd.comOffY[c] * sd.comOffY[c]) + sd.maxExtent[c]);
         }
         for (size_t v = 0; v < sd.verts.size(); v++) {
             verts.push_back(make_float2(sd.verts[v].x, sd.verts[v].y));
@@ -732,6 +856,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         w->staticColliders += sd.staticColliders;
     }
     d.N = (uint32_t)pos.size();
+    w->boundRadius.resize(d.N, 0.0f);
     sceneBodyStart.push_back(d.N);
     d.C = (uint32_t)colBody.size();
     d.nDyn = (uint32_t)dynList.size();
@@ -1223,6 +1348,37 @@ int agx_world_set_collider_callback(agx_world* w, uint32_t collider, uint32_t ca
     return AGX_OK;
 }
 
+int agx_world_read_sleep(agx_world* w, uint32_t* sleep_island, float* timer, uint32_t n) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    agx::DeviceWorld& d = w->d;
+    if (n != d.N) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u bodies", d.N);
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    if (!d.sleepId) {
+        if (sleep_island) std::fill(sleep_island, sleep_island + n, 0u);
+        if (timer) std::fill(timer, timer + n, 0.0f);
+        return AGX_OK;
+    }
+    if (sleep_island) HIP_CHECK(w, hipMemcpy(sleep_island, d.sleepId, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (timer) HIP_CHECK(w, hipMemcpy(timer, d.sleepTimer, n * sizeof(float), hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
+int agx_world_read_stash(agx_world* w, agx_manifold* out, uint32_t* island, uint64_t cap, uint64_t* count) {
+    if (!w || !count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    agx::DeviceWorld& d = w->d;
+    *count = 0;
+    if (!d.sleepId) return AGX_OK;
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    uint32_t n = 0;
+    HIP_CHECK(w, hipMemcpy(&n, &d.ctr->stashCount, sizeof n, hipMemcpyDeviceToHost));
+    *count = n;
+    if (!out && !island) return AGX_OK;   // a size query
+    if (n > cap) return fail(w, AGX_ERR_CAPACITY, "%u stashed manifolds, room for %llu", n, (unsigned long long)cap);
+    if (n && out) HIP_CHECK(w, hipMemcpy(out, d.stash, n * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+    if (n && island) HIP_CHECK(w, hipMemcpy(island, d.stashIsland, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
 int agx_world_read_events(agx_world* w, agx_collision_event* out, uint64_t cap, uint64_t* count, uint64_t* dropped) {
     if (!w || !count) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
     agx::DeviceWorld& d = w->d;
@@ -1622,7 +1778,7 @@ static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
 enum SnapTag : uint32_t {
     TAG_SETTINGS = 1, TAG_HOST, TAG_POS, TAG_VEL, TAG_RCS, TAG_INV, TAG_MASS, TAG_INERTIA, TAG_POSE, TAG_WROT,
     TAG_PREVW, TAG_AABB, TAG_ISLAND, TAG_HIT, TAG_COUNTERS, TAG_MANIFOLDS, TAG_SCENE_SWEEPS, TAG_CALLBACK,
-    TAG_THRESHOLD, TAG_COMMANDS, TAG_END
+    TAG_THRESHOLD, TAG_COMMANDS, TAG_SLEEP_ID, TAG_SLEEP_TIMER, TAG_STASH, TAG_STASH_ISLAND, TAG_END
 };
 struct SnapHost {
     float accumulator;
@@ -1670,7 +1826,9 @@ struct SnapArrays {
     std::vector<float2> rcs, inv, prevW;
     std::vector<float> wRot, hit, threshold;
     std::vector<uint32_t> island, sceneSweeps, callback;
-    std::vector<agx_manifold> man;
+    std::vector<agx_manifold> man, stash;
+    std::vector<uint32_t> sleepId, stashIsland;
+    std::vector<float> sleepTimer;
     agx::Counters ctr;
 };
 
@@ -1697,6 +1855,10 @@ int writeSnapshot(agx_world* w, SnapWriter& sw) {
     HIP_CHECK(w, down(a.callback, d.colCallback, d.C)); HIP_CHECK(w, down(a.threshold, d.colThreshold, d.C));
     HIP_CHECK(w, down(a.sceneSweeps, d.sceneSweeps, d.S));
     HIP_CHECK(w, down(a.man, d.man[1 - d.cur], a.ctr.prevManifolds));
+    if (d.sleepId) {
+        HIP_CHECK(w, down(a.sleepId, d.sleepId, d.N)); HIP_CHECK(w, down(a.sleepTimer, d.sleepTimer, d.N));
+        HIP_CHECK(w, down(a.stash, d.stash, a.ctr.stashCount)); HIP_CHECK(w, down(a.stashIsland, d.stashIsland, a.ctr.stashCount));
+    }
     SnapHeader h{};
     std::memcpy(h.magic, SNAP_MAGIC, sizeof h.magic);
     h.version = SNAP_VERSION;
@@ -1727,6 +1889,10 @@ int writeSnapshot(agx_world* w, SnapWriter& sw) {
     sw.section(TAG_CALLBACK, a.callback.data(), a.callback.size() * sizeof(uint32_t));
     sw.section(TAG_THRESHOLD, a.threshold.data(), a.threshold.size() * sizeof(float));
     sw.section(TAG_COMMANDS, w->commands.data(), w->commands.size() * sizeof(agx_world::BodyCommand));
+    sw.section(TAG_SLEEP_ID, a.sleepId.data(), a.sleepId.size() * sizeof(uint32_t));
+    sw.section(TAG_SLEEP_TIMER, a.sleepTimer.data(), a.sleepTimer.size() * sizeof(float));
+    sw.section(TAG_STASH, a.stash.data(), a.stash.size() * sizeof(agx_manifold));
+    sw.section(TAG_STASH_ISLAND, a.stashIsland.data(), a.stashIsland.size() * sizeof(uint32_t));
     sw.section(TAG_END, nullptr, 0);
     h.payload = sw.at - payloadStart;
     if (sw.out && !sw.overflow) std::memcpy(sw.out + headerAt, &h, sizeof h);
@@ -1815,8 +1981,25 @@ int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
               takeVec(TAG_SCENE_SWEEPS, a.sceneSweeps, d.S) && takeVec(TAG_CALLBACK, a.callback, C) &&
               takeVec(TAG_THRESHOLD, a.threshold, C);
     const unsigned char* cmd = ok ? r.section(TAG_COMMANDS, ~0ull, &commandBytes) : nullptr;
-    ok = ok && cmd && commandBytes % sizeof(agx_world::BodyCommand) == 0 && r.section(TAG_END, 0, nullptr);
+    ok = ok && cmd && commandBytes % sizeof(agx_world::BodyCommand) == 0;
+    // sleeping state: both arrays or neither (a world that never slept), then the stash
+    uint64_t sleepBytes = 0, timerBytes = 0;
+    const unsigned char* sid = ok ? r.section(TAG_SLEEP_ID, ~0ull, &sleepBytes) : nullptr;
+    const unsigned char* stm = sid ? r.section(TAG_SLEEP_TIMER, ~0ull, &timerBytes) : nullptr;
+    const bool slept = sleepBytes != 0;
+    ok = ok && sid && stm && (slept ? sleepBytes == N * 4 && timerBytes == N * 4 : timerBytes == 0) &&
+         takeVec(TAG_STASH, a.stash, slept ? a.ctr.stashCount : 0) &&
+         takeVec(TAG_STASH_ISLAND, a.stashIsland, slept ? a.ctr.stashCount : 0) && r.section(TAG_END, 0, nullptr);
+    if (ok && slept) {
+        a.sleepId.assign(reinterpret_cast<const uint32_t*>(sid), reinterpret_cast<const uint32_t*>(sid) + N);
+        a.sleepTimer.assign(reinterpret_cast<const float*>(stm), reinterpret_cast<const float*>(stm) + N);
+        for (uint32_t id : a.sleepId) ok = ok && (id == 0 || (id >= 2 && id - 2 < N));
+        for (uint32_t id : a.stashIsland) ok = ok && id >= 2 && id - 2 < N;
+        ok = ok && a.ctr.stashCount <= d.manCap;
+    }
     if (!ok || a.ctr.prevManifolds != h.manifolds) return fail(w, AGX_ERR_INVALID_ARGUMENT, "truncated or corrupt snapshot");
+    if (slept && !d.sleepId)
+        if (int rc = allocSleep(w)) return rc;
     for (uint32_t cb : a.callback)
         if (cb > AGX_MAX_CALLBACK_ID) return fail(w, AGX_ERR_INVALID_ARGUMENT, "corrupt snapshot (callback id)");
     HIP_CHECK(w, hipStreamSynchronize(w->stream));
@@ -1825,6 +2008,11 @@ int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
     HIP_CHECK(w, up(d.pose, a.pose)); HIP_CHECK(w, up(d.wRot, a.wRot)); HIP_CHECK(w, up(d.prevW, a.prevW));
     HIP_CHECK(w, up(d.aabb, a.aabb)); HIP_CHECK(w, up(d.colCallback, a.callback));
     HIP_CHECK(w, up(d.colThreshold, a.threshold)); HIP_CHECK(w, up(d.sceneSweeps, a.sceneSweeps));
+    if (d.sleepId) {   // a snapshot of a world that never slept: every body awake, the stash empty
+        if (!slept) { a.sleepId.assign(N, 0u); a.sleepTimer.assign(N, 0.0f); a.ctr.stashCount = 0; }
+        HIP_CHECK(w, up(d.sleepId, a.sleepId)); HIP_CHECK(w, up(d.sleepTimer, a.sleepTimer));
+        HIP_CHECK(w, up(d.stash, a.stash)); HIP_CHECK(w, up(d.stashIsland, a.stashIsland));
+    }
     // the warm-start source and its hash: built the way a step's FINISH builds them, then every
     // counter as saved
     HIP_CHECK(w, up(d.man[1 - d.cur], a.man));

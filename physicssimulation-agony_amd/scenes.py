@@ -353,6 +353,30 @@ def pyramid(rows: int, seed: int, box: float = 1.0) -> dict:
             "settings": {"velocity_iterations": 8, "position_iterations": 3}}
 
 
+def sleepy_stacks(stacks: int = 8, height: int = 3, seed: int = 9) -> dict:
+    """Stacks of boxes and polygons on a static floor that settle and fall asleep, with one ball per
+    stack dropped from a height that makes it land after the stack sleeps (each at another time), and
+    a static ramp the balls of the outer stacks roll off. Sleeping on (0.1 m/s, 0.5 s)."""
+    rng = np.random.default_rng(seed)
+    acc = _Acc()
+    span = stacks * 4.0
+    acc.add(SHAPE_BOX, 0.0, -1.0, p0=span + 20.0, p1=2.0)                       # floor, top at y = 0
+    acc.add(SHAPE_BOX, -span / 2 - 4.0, 3.0, rot=0.5, p0=6.0, p1=0.5)            # ramp
+    for k in range(stacks):
+        x = (k - (stacks - 1) / 2.0) * 4.0
+        for j in range(height):
+            jitter = float(rng.uniform(-0.05, 0.05))
+            if (j + k) % 3 == 2:
+                v = polygon_outline(rng, int(rng.integers(5, 9)), 0.6)
+                acc.add(SHAPE_POLYGON, x + jitter, 0.62 + 1.25 * j, mass=float(polygon_area(v)), verts=v)
+            else:
+                acc.add(SHAPE_BOX, x + jitter, 0.5 + 1.02 * j, mass=1.0, p0=1.0 + 0.2 * (j % 2), p1=1.0)
+        acc.add(SHAPE_CIRCLE, x + float(rng.uniform(-0.3, 0.3)), 20.0 + 9.0 * k, mass=0.6, p0=0.35)
+    return {"name": "sleepy_stacks", "materials": [(0.0, 0.6, 0.6)], "bodies": acc.bodies(),
+            "settings": {"velocity_iterations": 8, "position_iterations": 3, "sleep_speed_threshold": 0.1,
+                         "sleep_time_threshold": 0.5}}
+
+
 # ---------------------------------------------------------------- batch of independent scenes
 def _polygons(rng, n, r_lo, r_hi):
     """n convex polygons with 3-8 vertices on jittered circles: (radius, vcount, verts (V, 2))."""

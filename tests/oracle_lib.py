@@ -49,6 +49,14 @@ _SIGS = {
     "orc_set_events_enabled": (None, [_P, C.c_int]),
     "orc_collision_events": (C.c_int64, [_P, _P, C.c_uint64, _P, C.c_uint64]),
     "orc_get_events": (C.c_int64, [_P, _P, C.c_uint64]),
+    "orc_get_prev_manifolds": (C.c_int64, [_P, _P, C.c_uint64]),
+    "orc_set_sleep": (None, [_P, C.c_float, C.c_float]),
+    "orc_get_sleep": (None, [_P, _P, _P, C.c_uint32]),
+    "orc_set_sleep_state": (None, [_P, _P, _P, C.c_uint32]),
+    "orc_get_stash": (C.c_int64, [_P, _P, _P, C.c_uint64]),
+    "orc_set_stash": (None, [_P, _P, _P, C.c_uint64]),
+    "orc_wake_contacts": (C.c_int64, [_P, _P, C.c_uint64, _P, C.c_uint64]),
+    "orc_update_sleeping": (C.c_int64, [_P, C.c_float, _P, C.c_uint64, _P]),
     "orc_mlp_forward": (None, [_P, _P, C.c_uint64, _P, C.c_int]),
     "orc_mlp_gradients": (C.c_double, [_P, _P, _P, C.c_uint64, _P, C.c_int]),
     "orc_mlp_loss": (C.c_double, [_P, _P, _P, _P, C.c_uint64]),
@@ -267,6 +275,55 @@ class World:
             if n >= 0:
                 return out[:n]
             cap = -n
+
+    def set_sleep(self, speed=0.1, time=0.5):
+        lib().orc_set_sleep(self.h, speed, time)
+
+    def sleep_state(self):
+        n = self.counts()["bodies"]
+        ids = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        lib().orc_get_sleep(self.h, _p(ids), _p(t), n)
+        return ids, t
+
+    def set_sleep_state(self, ids, timers):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        t = np.ascontiguousarray(timers, np.float32)
+        lib().orc_set_sleep_state(self.h, _p(ids), _p(t), len(ids))
+
+    def stash(self):
+        cap = 1 << 12
+        while True:
+            m = np.empty(cap, MANIFOLD_DTYPE)
+            isl = np.empty(cap, np.uint32)
+            n = lib().orc_get_stash(self.h, _p(m), _p(isl), cap)
+            if n >= 0:
+                return m[:n], isl[:n]
+            cap = -n
+
+    def set_stash(self, m, island):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        isl = np.ascontiguousarray(island, np.uint32)
+        lib().orc_set_stash(self.h, _p(m), _p(isl), len(m))
+
+    def wake_contacts(self, m):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        cap = len(m) + len(self.stash()[0]) + 1
+        out = np.empty(cap, MANIFOLD_DTYPE)
+        n = lib().orc_wake_contacts(self.h, _p(m), len(m), _p(out), cap)
+        return out[:n]
+
+    def update_sleeping(self, dt, m):
+        m = np.ascontiguousarray(m, MANIFOLD_DTYPE)
+        out = np.empty(max(len(m), 1), MANIFOLD_DTYPE)
+        n = lib().orc_update_sleeping(self.h, dt, _p(m), len(m), _p(out))
+        return out[:n]
+
+    def prev_manifolds(self):
+        cap = max(self.counts()["prev_manifolds"], 1)
+        out = np.empty(cap, MANIFOLD_DTYPE)
+        n = lib().orc_get_prev_manifolds(self.h, _p(out), cap)
+        return out[:n]
 
     def errors(self):
         out = np.zeros(2, np.float32)
