@@ -70,6 +70,8 @@ typedef struct agx_settings {
     float sleep_speed_threshold;           /* default 0.1 (sleepSpeedThreshold, Simulation.h:70-72) */
     float sleep_time_threshold;            /* default +infinity here: sleeping off, as in every benchmark
                                               scene; the reference's default is 0.5 s (Simulation.h:73) */
+    uint32_t joint_iterations[5];          /* velocity iterations per joint type (AGX_JOINT_*), default 6 each
+                                              (IConstraintSystem::iterations, ConstraintSystemBase.h:34) */
 } agx_settings;
 
 /* One manifold record (64 bytes): NarrowPhaseCollisionDetector.h:26-61 with the collider
@@ -188,6 +190,40 @@ int agx_builder_scene_counts(agx_builder* b, uint32_t scene, agx_counts* out);
 int agx_builder_read_body_f32(agx_builder* b, uint32_t scene, uint32_t field, float* out, uint32_t n);
 int agx_builder_read_collider(agx_builder* b, uint32_t scene, uint32_t field, uint32_t as_u32, void* out, uint32_t n);
 int agx_builder_read_vertices(agx_builder* b, uint32_t scene, float* verts, float* normals, uint32_t n);
+/* ---- joints: ConstraintAPI::createSpring / createRevoluteJoint / createPrismaticJoint /
+ * createSliderJoint / createMotor (SimulationAPIs/ConstraintAPI.h:31-35) with the CreateParams of
+ * SimulationImpl/ConstraintSystems/(Spring, RevoluteJoint, ...)ConstraintSystem.h:14-26. Types follow ConstraintType
+ * (Types/Types.h:50-58), which is also the solving order. body_a / body_b are dense body indices of
+ * the scene at the time of the call (the joint follows its bodies if later creations move them);
+ * a motor uses body_a only. The prismatic joint's reference angle is the bodies' relative rotation
+ * at the call; axes are normalised ((1, 0) when zero). Each step the joints are solved between the
+ * two velocity-Verlet halves (Simulation.cpp:384-405, ConstraintScheduler.cpp:9-84): constraint data
+ * of every type, the motors' warm start, then joint_iterations[t] velocity passes of type t spread
+ * over the largest count, each pass colour by colour (joints of one colour share no dynamic body;
+ * the order is read back with agx_world_read_joint_order). */
+#define AGX_JOINT_SPRING 0
+#define AGX_JOINT_REVOLUTE 1
+#define AGX_JOINT_PRISMATIC 2
+#define AGX_JOINT_SLIDER 3
+#define AGX_JOINT_MOTOR 4
+#define AGX_JOINT_TYPES 5
+typedef struct agx_joint_desc {
+    uint32_t type;
+    uint32_t body_a, body_b;
+    uint32_t enable_limit;                    /* prismatic, slider */
+    float anchor_a_x, anchor_a_y;             /* body-local anchors (not motor) */
+    float anchor_b_x, anchor_b_y;
+    float axis_x, axis_y;                     /* prismatic, slider: axis in body A's frame */
+    float rest_length;                        /* spring */
+    float stiffness, damping;                 /* spring, revolute, prismatic, slider */
+    float lower_limit, upper_limit;           /* prismatic, slider */
+    float target_speed, max_torque;           /* motor */
+} agx_joint_desc;
+/* Adds n joints to a scene in call order; out_index (may be NULL) receives each one's index among
+ * the scene's joints. If a joint names a body that does not exist, or an unknown type, the call adds
+ * nothing and returns AGX_ERR_INVALID_ARGUMENT (the reference returns std::nullopt). */
+int agx_builder_add_joints(agx_builder* b, uint32_t scene, uint32_t n, const agx_joint_desc* joints, uint32_t* out_index);
+
 /* Uploads the built scenes to device 0 (Simulation::refreshSpatialData, Simulation.cpp:561-579,
  * runs once here) and returns the world. The builder may be destroyed afterwards. */
 int agx_world_create(agx_builder* b, const agx_settings* settings, agx_world** out);
@@ -275,6 +311,17 @@ int agx_world_read_scene_sweeps(agx_world* w, uint32_t* out, uint32_t n);
  * after its last velocity sweep in the last step solved scene by scene with track_residual set;
  * n must be the scene count. A scene has met residual_target when its value is at most the target. */
 int agx_world_read_scene_residual(agx_world* w, float* out, uint32_t n);
+
+/* Joints of the world (all scenes, scene by scene in creation order) and their solve order: order
+ * receives joint indices type by type, colour by colour; colour_start[k] .. colour_start[k+1] is
+ * colour k, type_colour[t] .. type_colour[t+1] the colours of type t (type_colour has
+ * AGX_JOINT_TYPES + 1 entries). NULL arrays are skipped; *joints and *colours receive the counts. */
+int agx_world_read_joint_order(agx_world* w, uint32_t* order, uint32_t cap, uint32_t* colour_start, uint32_t colour_cap,
+                               uint32_t* type_colour, uint32_t* joints, uint32_t* colours);
+/* Joint records as built (n must be the joint count; body indices are world body indices) and the
+ * motors' accumulated impulses (the warm-start state, MotorConstraintSystem::Data::accumulatedImpulse;
+ * 0 for other types). */
+int agx_world_read_joints(agx_world* w, agx_joint_desc* out, float* motor_impulse, uint32_t n);
 
 /* ---- collision events: BodyCollisionSolver::computeConstraintData's event recording
  * (Solving/BodyCollisionSolver.cpp:300-388), Simulation::setColliderCollisionCallback

@@ -286,6 +286,37 @@ int64_t orc_update_sleeping(void* h, float dt, const Manifold* m, uint64_t n, Ma
     return (int64_t)v.size();
 }
 
+// Joints: creation (-1 if one names a missing body), per-type iterations, the pass order to replay
+// (order: n joint indices; colourStart: k + 1 boundaries; typeColour: 6 entries; n = 0 restores each
+// type's index order), and the motors' accumulated impulses.
+int orc_add_joints(void* h, uint32_t n, const JointDesc* d) {
+    for (uint32_t i = 0; i < n; i++)
+        if (!addJoint(W(h), d[i])) return -1;
+    return 0;
+}
+void orc_set_joint_iterations(void* h, const uint32_t* it) {
+    for (int t = 0; t < 5; t++) W(h).settings.jointIterations[t] = it[t];
+}
+void orc_set_joint_order(void* h, const uint32_t* order, uint32_t n, const uint32_t* colourStart, uint32_t k,
+                         const uint32_t* typeColour) {
+    World& w = W(h);
+    w.jointOrder.assign(order, order + n);
+    w.jointColourStart.assign(colourStart, colourStart + (n ? k + 1 : 0));
+    w.jointTypeColour.assign(typeColour, typeColour + (n ? JOINT_TYPES + 1 : 0));
+}
+int orc_get_motor_impulses(void* h, float* out, uint32_t n) {
+    World& w = W(h);
+    if (w.motorAcc.size() != n) return -1;
+    if (n) std::memcpy(out, w.motorAcc.data(), n * sizeof(float));
+    return 0;
+}
+int orc_set_motor_impulses(void* h, const float* in, uint32_t n) {
+    World& w = W(h);
+    if (w.motorAcc.size() != n) return -1;
+    if (n) std::memcpy(w.motorAcc.data(), in, n * sizeof(float));
+    return 0;
+}
+
 void orc_queue_body(void* h, uint32_t kind, uint32_t body, float a, float b) { W(h).commands.push_back({ kind, body, a, b }); }
 void orc_flush(void* h) { flushCommands(W(h)); }
 

@@ -57,12 +57,25 @@ struct Event {
 };
 static_assert(sizeof(Event) == 48, "event record is 48 bytes");
 
+// A joint (ConstraintSystems/*ConstraintSystem.h CreateParams); field order matches agx_joint_desc.
+enum JointType : uint32_t { JOINT_SPRING = 0, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_SLIDER, JOINT_MOTOR, JOINT_TYPES };
+struct JointDesc {
+    uint32_t type, bodyA, bodyB, enableLimit;
+    float anchorAX, anchorAY, anchorBX, anchorBY, axisX, axisY, restLength, stiffness, damping;
+    float lowerLimit, upperLimit, targetSpeed, maxTorque;
+};
+struct Joint {
+    JointDesc d;
+    float referenceAngle = 0.0f;   // prismatic: relative rotation at creation
+};
+
 struct Settings {
     float gravityX = 0.0f, gravityY = -9.81f;
     uint32_t velocityIterations = 6;   // Simulation.h:56
     uint32_t positionIterations = 3;   // Simulation.h:57
     bool ccd = true;                   // Simulation.h:66
     bool collisionEvents = true;       // Simulation.h:60
+    uint32_t jointIterations[5] = { 6, 6, 6, 6, 6 };   // ConstraintSystemBase.h:34
     float sleepSpeedThreshold = 0.1f;  // Simulation.h:71
     float sleepTimeThreshold = 3.40282347e38f;   // Simulation.h:72 (0.5 s there); the default here keeps sleeping off
     float ccdFastFactor = 1.0f;        // CCD/ContinuousCollisionDetector.h:31
@@ -117,6 +130,12 @@ struct World {
     // sleepId = its sleep island's id (>= 2, FIRST_SLEEP_ISLAND_ID, BodySoA.h:45: here 2 + the
     // island's smallest body index), 0 when awake; the manifolds of a sleeping island are kept in
     // the stash with their island id (NarrowPhaseCollisionDetector.cpp:299-346) until it wakes.
+    // Joints (oracle/joints.cpp) and the order their velocity passes run in: jointOrder type by type,
+    // colour by colour, jointColourStart the colour boundaries, jointTypeColour[t] the first colour of
+    // type t (empty: each type's joints in index order, the reference's single-threaded order).
+    std::vector<Joint> joints;
+    std::vector<float> motorAcc;
+    std::vector<uint32_t> jointOrder, jointColourStart, jointTypeColour;
     std::vector<uint32_t> sleepId;
     std::vector<float> sleepTimer;
     std::vector<Manifold> stash;
@@ -164,6 +183,13 @@ void islands(World& w, const std::vector<Manifold>& m);       // IslandManager.c
 // manifolds in order[c] (null: index order) on the current, pre-warm-start velocities; appends to
 // w.events with step index w.eventStep. No-op unless events are on and some collider has a callback.
 void collisionEvents(World& w, const std::vector<Manifold>& m, const uint32_t* order);
+// ---- joints (oracle/joints.cpp) ----
+// Appends joints (body indices are current dense indices; false if one does not exist).
+bool addJoint(World& w, const JointDesc& d);
+// ConstraintScheduler::solve (ConstraintScheduler.cpp:9-84) over the joints, on the velocities
+// between the velocity-Verlet halves.
+void solveJoints(World& w, float dt);
+
 // ---- sleeping (oracle/sleep.cpp) ----
 bool asleep(const World& w, uint32_t body);
 // ObjectManager::wakeIslandsOfContacts (ObjectManager.h:139-156) on this step's manifolds, then

@@ -61,7 +61,7 @@ void snapshotPoses(World& w) {
 }
 
 // Integrator.cpp:99-209 (velocity Verlet; angular damping is a no-op at its default 1.0,
-// Integrator.cpp:214-217, and joints / the mouse holder are not on this path).
+// Integrator.cpp:214-217, and the mouse holder is not on this path), joints between the halves.
 void integrate(World& w, float dt) {
     const float halfDT = dt * 0.5f;
     const float gx = w.settings.gravityX * halfDT, gy = w.settings.gravityY * halfDT;
@@ -71,6 +71,7 @@ void integrate(World& w, float dt) {
         w.velX[i] += gx * movable;
         w.velY[i] += gy * movable;
     }
+    solveJoints(w, dt);   // Simulation.cpp:384-405: constraints between the two halves
     for (uint32_t i = w.staticBodies; i < w.bodyCount(); i++) {
         if (asleep(w, i)) continue;
         w.comX[i] = w.velX[i] * dt + w.comX[i];
@@ -327,6 +328,17 @@ void islands(World& w, const std::vector<Manifold>& m) {
     for (const Manifold& mf : m) {
         const uint32_t a = w.colBody[mf.a] - dynBegin;
         const uint32_t b = w.colBody[mf.b] - dynBegin;
+        if (a >= count || b >= count || !member[a] || !member[b]) continue;
+        uint32_t ra = find(a), rb = find(b);
+        if (ra == rb) continue;
+        if (parent[ra] > parent[rb]) std::swap(ra, rb);
+        parent[ra] += parent[rb];
+        parent[rb] = (int32_t)ra;
+    }
+    // the joint systems' active edges (Simulation.cpp:331-336): joints between two dynamic bodies
+    for (const Joint& j : w.joints) {
+        if (j.d.type == JOINT_MOTOR) continue;
+        const uint32_t a = j.d.bodyA - dynBegin, b = j.d.bodyB - dynBegin;
         if (a >= count || b >= count || !member[a] || !member[b]) continue;
         uint32_t ra = find(a), rb = find(b);
         if (ra == rb) continue;

@@ -45,6 +45,24 @@ EVENT_DTYPE = np.dtype([("self", "<u4"), ("other", "<u4"), ("self_body", "<u4"),
                         ("px", "<f4"), ("py", "<f4"), ("nx", "<f4"), ("ny", "<f4"), ("impulse", "<f4"),
                         ("callback", "<u4"), ("step", "<u4"), ("reserved", "<u4")])
 assert EVENT_DTYPE.itemsize == 48
+# agx_joint_desc (ConstraintSystems/*ConstraintSystem.h CreateParams)
+JOINT_DTYPE = np.dtype([("type", "<u4"), ("body_a", "<u4"), ("body_b", "<u4"), ("enable_limit", "<u4"),
+                        ("anchor_a_x", "<f4"), ("anchor_a_y", "<f4"), ("anchor_b_x", "<f4"), ("anchor_b_y", "<f4"),
+                        ("axis_x", "<f4"), ("axis_y", "<f4"), ("rest_length", "<f4"), ("stiffness", "<f4"),
+                        ("damping", "<f4"), ("lower_limit", "<f4"), ("upper_limit", "<f4"), ("target_speed", "<f4"),
+                        ("max_torque", "<f4")])
+assert JOINT_DTYPE.itemsize == 68
+JOINT_SPRING, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_SLIDER, JOINT_MOTOR = range(5)
+
+
+def joint_array(joints) -> np.ndarray:
+    """A joint list (dicts with JOINT_DTYPE field names; missing fields 0, axis (1, 0)) as records."""
+    out = np.zeros(len(joints), JOINT_DTYPE)
+    out["axis_x"] = 1.0
+    for i, j in enumerate(joints):
+        for k, v in j.items():
+            out[k][i] = v
+    return out
 
 
 class AgxError(RuntimeError):
@@ -64,7 +82,8 @@ class Settings(C.Structure):
                 ("track_errors", C.c_uint32), ("track_residual", C.c_uint32), ("residual_target", C.c_float),
                 ("collide_only", C.c_uint32), ("solve_per_scene", C.c_uint32),
                 ("collision_events", C.c_uint32), ("event_capacity", C.c_uint32),
-                ("sleep_speed_threshold", C.c_float), ("sleep_time_threshold", C.c_float)]
+                ("sleep_speed_threshold", C.c_float), ("sleep_time_threshold", C.c_float),
+                ("joint_iterations", C.c_uint32 * 5)]
 
 
 class Stats(C.Structure):
@@ -176,6 +195,10 @@ _SIGS = {
     "agx_world_load": (C.c_int, [_P, _P, C.c_uint64]),
     "agx_world_read_sleep": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "agx_world_read_stash": (C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_builder_add_joints": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "agx_world_read_joint_order": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32)]),
+    "agx_world_read_joints": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "agx_world_read_events": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
 }
 EXPORTED = tuple(_SIGS)
@@ -264,6 +287,14 @@ class Builder:
             _check(rc)
         self.rejected = rc
         return out
+
+    def add_joints(self, scene: int, joints) -> np.ndarray:
+        """Joints (a JOINT_DTYPE array or a list of dicts) between bodies of a scene."""
+        j = joints if isinstance(joints, np.ndarray) else joint_array(joints)
+        j = np.ascontiguousarray(j, JOINT_DTYPE)
+        out = np.empty(max(len(j), 1), np.uint32)
+        _check(lib().agx_builder_add_joints(self._h, scene, len(j), _ptr(j), _ptr(out)))
+        return out[:len(j)]
 
     def counts(self, scene: int = 0) -> Counts:
         c = Counts()
@@ -439,6 +470,26 @@ class World:
         elif rc:
             _check(rc, self._h)
         return out[:cnt.value], dropped.value
+
+    def joint_order(self):
+        """(order, colour starts, first colour of each type) of the joints' velocity passes."""
+        n, k = C.c_uint32(), C.c_uint32()
+        _check(lib().agx_world_read_joint_order(self._h, None, 0, None, 0, None, C.byref(n), C.byref(k)), self._h)
+        order = np.empty(max(n.value, 1), np.uint32)
+        starts = np.empty(k.value + 1, np.uint32)
+        types = np.empty(6, np.uint32)
+        _check(lib().agx_world_read_joint_order(self._h, _ptr(order), n.value, _ptr(starts), k.value + 1, _ptr(types),
+                                                C.byref(n), C.byref(k)), self._h)
+        return order[:n.value], starts, types
+
+    def joints(self):
+        """(joint records, the motors' accumulated impulses)."""
+        n = C.c_uint32()
+        _check(lib().agx_world_read_joint_order(self._h, None, 0, None, 0, None, C.byref(n), None), self._h)
+        out = np.empty(max(n.value, 1), JOINT_DTYPE)
+        imp = np.empty(max(n.value, 1), np.float32)
+        _check(lib().agx_world_read_joints(self._h, _ptr(out), _ptr(imp), n.value), self._h)
+        return out[:n.value], imp[:n.value]
 
     def sleep_state(self):
         """(sleep island id per body, 0 when awake; sleep timer per body)."""
@@ -646,9 +697,12 @@ def device_count() -> int:
 
 
 def build_scene(builder: Builder, scene: dict, scene_index: int = 0) -> np.ndarray:
-    """Creates a scene description's materials (first scene only) and bodies."""
+    """Creates a scene description's materials (first scene only), bodies and joints."""
     if scene_index == 0 or not getattr(builder, "_materials_done", False):
         for m in scene.get("materials", []):
             builder.create_material(*m)
         builder._materials_done = True
-    return builder.create_bodies(scene_index, scene["bodies"])
+    idx = builder.create_bodies(scene_index, scene["bodies"])
+    if scene.get("joints"):
+        builder.add_joints(scene_index, scene["joints"])
+    return idx

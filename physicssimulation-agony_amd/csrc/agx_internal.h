@@ -217,6 +217,24 @@ struct SceneLaunch {
     uint32_t colourWhole = 0, colourGroups = 1, colourLayoutWhole = 0, colourLayoutGroup = 0, colourGroupWords = 0;
 };
 
+// A joint as built (include/agx.h agx_joint_desc): bodies by world body index (motor: a only).
+struct JointRec {
+    uint32_t type, a, b, enableLimit;
+    float4 anchors;    // local anchor A (x, y), local anchor B (x, y)
+    float4 axisRest;   // local axis of A (x, y; unit), spring rest length, prismatic reference angle
+    float4 params;     // stiffness, damping, lower limit, upper limit
+    float2 motor;      // target speed, maximum torque
+    float2 pad;
+};
+// A joint's constraint data of the step (the solver's ConstraintData), 20 floats.
+struct JointData { float v[20]; };
+// The step's joint schedule: velocity passes per type and each type's colours
+// [typeColour[t], typeColour[t + 1]) in the colour ranges.
+struct JointSchedule {
+    uint32_t iterations[AGX_JOINT_TYPES];
+    uint32_t typeColour[AGX_JOINT_TYPES + 1];
+};
+
 struct DeviceWorld {
     uint32_t S = 0, N = 0, C = 0, nDyn = 0, nStat = 0, nVerts = 0, nMat = 0;
     uint32_t sceneBits = 0, mortonBits = 30;
@@ -345,6 +363,15 @@ struct DeviceWorld {
     agx_manifold* stash = nullptr;   // manifolds of sleeping islands (manCap)
     uint32_t* stashIsland = nullptr; // their sleep ids
     uint32_t* stashIslandAlt = nullptr;   // compaction scratch
+    // joints (k_joints.hip)
+    uint32_t nJoints = 0;
+    JointRec* joints = nullptr;
+    JointData* jointData = nullptr;
+    float* motorAcc = nullptr;       // per joint: a motor's accumulated impulse (warm start)
+    uint32_t* jointOrder = nullptr;  // joint indices type by type, colour by colour
+    uint2* jointColours = nullptr;   // colour ranges in jointOrder
+    JointSchedule jointSchedule{};
+    uint2* jointEdges = nullptr;     // per joint: its two bodies when both are dynamic, else (UINT32_MAX, ...)
 };
 
 // ---------------------------------------------------------------- device math
@@ -465,7 +492,9 @@ hipError_t launchScan(hipStream_t st, const uint32_t* in, uint32_t* out, const u
                       uint32_t* totalPtr, uint64_t countHost, uint32_t* blockScratch, uint32_t blockCap);
 
 hipError_t launchRefreshCosSin(hipStream_t st, DeviceWorld& w);
-hipError_t launchIntegrate(hipStream_t st, DeviceWorld& w, float dt, float gx, float gy, int snapshot);
+// part 0: the whole velocity-Verlet step; 1: the CCD snapshot and the first half kick only; 2: the
+// drift, the second half kick, wrap and cos/sin (joints are solved between parts 1 and 2).
+hipError_t launchIntegrate(hipStream_t st, DeviceWorld& w, float dt, float gx, float gy, int snapshot, int part = 0);
 hipError_t launchPose(hipStream_t st, DeviceWorld& w, int all, int prevFromSnapshot, int writePrev);
 hipError_t launchCcd(hipStream_t st, DeviceWorld& w, float fastFactor, float coreFraction);
 
@@ -504,6 +533,8 @@ hipError_t launchIslands(hipStream_t st, DeviceWorld& w);
 // Sleeping (k_sleep.hip): the wake after the narrow phase, the sleep after the islands.
 hipError_t launchWake(hipStream_t st, DeviceWorld& w);
 hipError_t launchSleep(hipStream_t st, DeviceWorld& w, float dt, float speedThreshold, float timeThreshold);
+// Joints between the two velocity-Verlet halves (k_joints.hip).
+hipError_t launchJoints(hipStream_t st, DeviceWorld& w, float dt);
 // Collision events of the step's manifolds on its pre-warm-start velocities (k_events.hip).
 hipError_t launchEvents(hipStream_t st, DeviceWorld& w);
 constexpr uint32_t SOLVE_FINISH_BLOCKS = 256;

@@ -77,6 +77,32 @@ bool sanitizeOutline(const Vec* in, uint32_t count, std::vector<Vec>& out) {
 }
 }  // namespace
 
+// ObjectManager::createConstraint (ObjectManager.cpp:430-475) for host-side scenes; the prismatic
+// joint's reference angle as computeReferenceAngle (PrismaticJointConstraintSystem.h:118-125) and the
+// axis as sanitizeAxis (ConstraintSystemBase.h:146-158).
+bool SceneData::addJoint(JointSpec spec, uint32_t a, uint32_t b) {
+    const bool single = spec.type == 4;
+    if (spec.type > 4 || a >= bodyCount() || (!single && b >= bodyCount())) return false;
+    spec.slotA = bodySlot[a];
+    spec.slotB = single ? UINT32_MAX : bodySlot[b];
+    const float lenSq = spec.axis.x * spec.axis.x + spec.axis.y * spec.axis.y;
+    if (lenSq > 1e-12f) {
+        const float len = std::sqrt(lenSq);
+        spec.axis = { spec.axis.x / len, spec.axis.y / len };
+    } else {
+        spec.axis = { 1.0f, 0.0f };
+    }
+    spec.referenceAngle = 0.0f;
+    if (spec.type == 2) {
+        const float pi = (float)3.14159265358979323846, twoPi = 2.0f * pi;
+        float r = rot[b] - rot[a];
+        r -= twoPi * std::floor((r + pi) / twoPi);
+        spec.referenceAngle = r;
+    }
+    joints.push_back(spec);
+    return true;
+}
+
 void SceneData::swapBodies(uint32_t a, uint32_t b) {
     if (a == b) return;
     for (auto* v : { &comX, &comY, &velX, &velY, &rot, &angVel, &mass, &invMass, &inertia, &invInertia, &cosv, &sinv,

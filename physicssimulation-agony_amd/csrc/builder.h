@@ -11,6 +11,15 @@ namespace agx {
 struct Vec { float x, y; };
 
 // One scene in dense order: [Static | Dynamic] bodies, colliders following their owner's group.
+// A joint as created (ConstraintSystems/*ConstraintSystem.h CreateParams); bodies by creation slot,
+// which stays with the body when later creations move it in the dense arrays.
+struct JointSpec {
+    uint32_t type, slotA, slotB, enableLimit;
+    Vec anchorA, anchorB, axis;
+    float restLength, stiffness, damping, lowerLimit, upperLimit, targetSpeed, maxTorque;
+    float referenceAngle;   // prismatic: relative rotation of B to A at creation, in (-pi, pi]
+};
+
 struct SceneData {
     std::vector<float> comX, comY, velX, velY, rot, angVel, mass, invMass, inertia, invInertia, cosv, sinv;
     std::vector<float> localComX, localComY, boundRadius;
@@ -36,6 +45,10 @@ struct SceneData {
     uint32_t createCapsule(float px, float py, float rot, float vx, float vy, float av, float mass,
                            uint32_t material, float length, float radius);
     uint32_t materialCount = 1;   // set by the builder owner so out-of-range materials map to 0
+    std::vector<JointSpec> joints;
+    // Appends a joint between dense bodies a and b (b ignored for a motor, type 4); false when a
+    // body does not exist. The axis is normalised, (1, 0) when shorter than 1e-6.
+    bool addJoint(JointSpec spec, uint32_t a, uint32_t b);
 
 private:
     uint32_t appendBody(Vec pos, Vec vel, float rot, float av, float mass, float invMass, float inertia,

@@ -13,17 +13,23 @@ __global__ __launch_bounds__(256) void kIntegrate(float4* __restrict__ pos, floa
                                                   float2* __restrict__ rcs, const float2* __restrict__ inv,
                                                   float4* __restrict__ prev, const uint32_t* __restrict__ list,
                                                   uint32_t n, float dt, float gxh, float gyh, int snapshot,
-                                                  const uint32_t* __restrict__ sleepId) {
+                                                  const uint32_t* __restrict__ sleepId, int part) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint32_t b = list[k];
         float4 p = pos[b];
         float4 v = vel[b];
         const float2 rc = rcs[b];
-        if (snapshot) prev[b] = make_float4(p.x, p.y, rc.x, rc.y);
+        if (snapshot && part != 2) prev[b] = make_float4(p.x, p.y, rc.x, rc.y);
         if (sleepId && sleepId[b] != 0u) continue;   // sleeping bodies are not in the Dynamic range
         const float movable = inv[b].x != 0.0f ? 1.0f : 0.0f;
-        v.x += gxh * movable;
-        v.y += gyh * movable;
+        if (part != 2) {   // integratePhase1_VelocityVerlet (Integrator.cpp:99-139)
+            v.x += gxh * movable;
+            v.y += gyh * movable;
+        }
+        if (part == 1) {   // joints run before the drift
+            vel[b] = v;
+            continue;
+        }
         p.x = v.x * dt + p.x;
         p.y = v.y * dt + p.y;
         p.z = v.z * dt + p.z;
@@ -426,11 +432,11 @@ inline uint32_t gridFor(uint32_t n) { return std::max(1u, std::min((n + 255u) / 
 
 }  // namespace
 
-hipError_t launchIntegrate(hipStream_t st, DeviceWorld& w, float dt, float gx, float gy, int snapshot) {
+hipError_t launchIntegrate(hipStream_t st, DeviceWorld& w, float dt, float gx, float gy, int snapshot, int part) {
     if (w.nDynBodies == 0) return hipSuccess;
     const float halfDT = dt * 0.5f;
     hipLaunchKernelGGL(kIntegrate, dim3(gridFor(w.nDynBodies)), dim3(256), 0, st, w.pos, w.vel, w.rcs, w.inv, w.prev,
-                       w.dynBodies, w.nDynBodies, dt, gx * halfDT, gy * halfDT, snapshot, w.sleepId);
+                       w.dynBodies, w.nDynBodies, dt, gx * halfDT, gy * halfDT, snapshot, w.sleepId, part);
     return hipGetLastError();
 }
 

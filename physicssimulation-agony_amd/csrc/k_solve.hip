@@ -1585,6 +1585,25 @@ __global__ __launch_bounds__(256) void kUfUnite(const agx_manifold* __restrict__
     }
 }
 
+// Island edges of the joints between two dynamic bodies (IslandManager::step over each constraint
+// system's active edges, Simulation.cpp:331-336); joints of sleeping bodies are not active.
+__global__ __launch_bounds__(256) void kUfUniteEdges(const uint2* __restrict__ edges, uint32_t n,
+                                                     const uint32_t* __restrict__ sleepId, uint32_t* parent) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint32_t a = edges[k].x, b = edges[k].y;
+        if (a == 0xFFFFFFFFu) continue;
+        if (sleepId && (sleepId[a] != 0u || sleepId[b] != 0u)) continue;
+        a = ufFind(parent, a);
+        b = ufFind(parent, b);
+        while (a != b) {
+            if (a > b) { const uint32_t t = a; a = b; b = t; }
+            const uint32_t old = atomicCAS(parent + b, b, a);
+            if (old == b) break;
+            b = ufFind(parent, old);
+        }
+    }
+}
+
 // Read-only root lookup: no path splitting, so concurrent lookups never rewrite parents and
 // every body sees the final root (all unions finished in the previous kernel).
 __device__ inline uint32_t ufRootOf(const uint32_t* parent, uint32_t x) {
@@ -1988,6 +2007,9 @@ hipError_t launchIslands(hipStream_t st, DeviceWorld& w) {
     }
     hipLaunchKernelGGL(kUfUnite, dim3(blocksFor((uint32_t)std::min<uint64_t>(w.manCap, 1u << 22))), dim3(256), 0, st,
                        w.man[w.cur], w.ctr, w.colBody, w.inv, w.ufParent);
+    if (w.nJoints)
+        hipLaunchKernelGGL(kUfUniteEdges, dim3(blocksFor(w.nJoints)), dim3(256), 0, st, w.jointEdges, w.nJoints, w.sleepId,
+                           w.ufParent);
     if (w.nDynBodies == 0) return launchFillU32(st, &w.ctr->islands, 0u, 1);
     hipLaunchKernelGGL(kUfRoots, dim3(blocksFor(w.nDynBodies)), dim3(256), 0, st, w.ufParent, w.dynBodies, w.nDynBodies,
                        w.ufRoot, w.islandId, w.sleepId);

@@ -40,6 +40,9 @@ struct agx_world {
     std::vector<float> mass, inertia, localComX, localComY;   // per body, constant
     std::vector<uint32_t> colType, colMaterial, colSlot, colScene, polyStart, polyCount, colBodyHost;
     std::vector<float> boundRadius;  // per body: the reach of its colliders (ObjectManager.cpp:30-43)
+    std::vector<agx::JointRec> jointRecsHost;   // joints as built, their solve order and colour ranges
+    std::vector<uint32_t> jointOrderHost;
+    std::vector<uint2> jointColoursHost;
     std::vector<float> colP0, colP1, colMinExt, colMaxExt;
     uint32_t staticBodies = 0, staticColliders = 0;
     float accumulator = 0.0f;
@@ -406,7 +409,14 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
         w->inStep = true;
         w->solvedPerScene = false;
         SegmentTimer t(w, SEG_INTEGRATE);
-        HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0));
+        if (d.nJoints == 0) {
+            HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0));
+        } else {   // Simulation::integrateAndApplyConstraints (Simulation.cpp:384-405)
+            for (uint32_t k = 0; k < AGX_JOINT_TYPES; k++) d.jointSchedule.iterations[k] = s.joint_iterations[k];
+            HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0, 1));
+            HIP_CHECK(w, agx::launchJoints(st, d, dt));
+            HIP_CHECK(w, agx::launchIntegrate(st, d, dt, s.gravity_x, s.gravity_y, ccd ? 1 : 0, 2));
+        }
         break;
     }
     case AGX_PHASE_POSE: {
@@ -505,7 +515,7 @@ int runPhase(agx_world* w, uint32_t phase, float dt) {
         break;
     case AGX_PHASE_ISLANDS: {
         SegmentTimer t(w, SEG_ISLANDS);
-        if (w->solvedPerScene && !s.collide_only) HIP_CHECK(w, agx::launchSceneIslands(st, d));
+        if (w->solvedPerScene && !s.collide_only && d.nJoints == 0) HIP_CHECK(w, agx::launchSceneIslands(st, d));
         else HIP_CHECK(w, agx::launchIslands(st, d));
         // updateSleeping + sleepContactData (Simulation.cpp:348-350)
         if (d.sleepId && !s.collide_only)
@@ -663,6 +673,7 @@ void agx_settings_default(agx_settings* s) {
     s->event_capacity = 65536;
     s->sleep_speed_threshold = 0.1f;
     s->sleep_time_threshold = INFINITY;
+    for (uint32_t t = 0; t < AGX_JOINT_TYPES; t++) s->joint_iterations[t] = 6;
 }
 
 const char* agx_last_error(void) { return g_lastError.c_str(); }
@@ -721,6 +732,35 @@ int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uin
     }
     if (rejected) fail(nullptr, 0, "%d bodies rejected (invalid or non-convex polygon outline)", rejected);
     return rejected;
+}
+
+int agx_builder_add_joints(agx_builder* b, uint32_t scene, uint32_t n, const agx_joint_desc* joints, uint32_t* out_index) {
+    if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    if (n && !joints) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "null joint array");
+    agx::SceneData& sd = b->b.scenes[scene];
+    const size_t before = sd.joints.size();
+    for (uint32_t i = 0; i < n; i++) {
+        const agx_joint_desc& j = joints[i];
+        agx::JointSpec spec{};
+        spec.type = j.type;
+        spec.enableLimit = j.enable_limit ? 1u : 0u;
+        spec.anchorA = { j.anchor_a_x, j.anchor_a_y };
+        spec.anchorB = { j.anchor_b_x, j.anchor_b_y };
+        spec.axis = { j.axis_x, j.axis_y };
+        spec.restLength = j.rest_length;
+        spec.stiffness = j.stiffness;
+        spec.damping = j.damping;
+        spec.lowerLimit = j.lower_limit;
+        spec.upperLimit = j.upper_limit;
+        spec.targetSpeed = j.target_speed;
+        spec.maxTorque = j.max_torque;
+        if (!sd.addJoint(spec, j.body_a, j.body_b)) {
+            sd.joints.resize(before);
+            return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "joint %u: unknown type %u or missing body", i, j.type);
+        }
+        if (out_index) out_index[i] = (uint32_t)sd.joints.size() - 1;
+    }
+    return AGX_OK;
 }
 
 int agx_builder_scene_counts(agx_builder* b, uint32_t scene, agx_counts* out) {
@@ -803,6 +843,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     d.mortonBits = std::min(30u, (32u - d.sceneBits) & ~1u);
 
     // Concatenate scenes.
+    std::vector<agx::JointRec> jointRecs;
     std::vector<float4> pos, vel, prev;
     std::vector<float2> rcs, inv, comOff, dims, extent, verts, normals;
     std::vector<uint32_t> bodyScene, colBody, colScene, colSlot, dynBodies, bodyToDyn, dynList, statList;
@@ -854,6 +895,23 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         }
         w->staticBodies += sd.staticBodies;
         w->staticColliders += sd.staticColliders;
+        if (!sd.joints.empty()) {   // creation slots to this world's body indices
+            std::vector<uint32_t> denseOf(sd.bodyCount(), UINT32_MAX);
+            for (uint32_t i = 0; i < sd.bodyCount(); i++)
+                if (sd.bodySlot[i] < denseOf.size()) denseOf[sd.bodySlot[i]] = i;
+            for (const agx::JointSpec& j : sd.joints) {
+                agx::JointRec r{};
+                r.type = j.type;
+                r.a = bOff + denseOf[j.slotA];
+                r.b = j.type == AGX_JOINT_MOTOR ? r.a : bOff + denseOf[j.slotB];
+                r.enableLimit = j.enableLimit;
+                r.anchors = make_float4(j.anchorA.x, j.anchorA.y, j.anchorB.x, j.anchorB.y);
+                r.axisRest = make_float4(j.axis.x, j.axis.y, j.restLength, j.referenceAngle);
+                r.params = make_float4(j.stiffness, j.damping, j.lowerLimit, j.upperLimit);
+                r.motor = make_float2(j.targetSpeed, j.maxTorque);
+                jointRecs.push_back(r);
+            }
+        }
     }
     d.N = (uint32_t)pos.size();
     w->boundRadius.resize(d.N, 0.0f);
@@ -942,13 +1000,52 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     AL(d.cs.r1, solveCap); AL(d.cs.k0, solveCap); AL(d.cs.k1, solveCap); AL(d.cs.imp, solveCap);
     AL(d.cs.blk, solveCap);
     AL(d.colCallback, d.C); AL(d.colThreshold, d.C);
+    if (!jointRecs.empty()) {
+        // Colours per type: each joint takes the smallest colour that no earlier joint of its type
+        // on one of its dynamic bodies has (static bodies are never written, so they never conflict).
+        std::vector<uint32_t> order;
+        std::vector<uint2> colours, edges(jointRecs.size());
+        for (uint32_t t = 0; t < AGX_JOINT_TYPES; t++) {
+            d.jointSchedule.typeColour[t] = (uint32_t)colours.size();
+            std::vector<std::vector<uint32_t>> byColour;
+            std::map<uint64_t, bool> used;
+            for (uint32_t j = 0; j < jointRecs.size(); j++) {
+                const agx::JointRec& r = jointRecs[j];
+                if (r.type != t) continue;
+                const bool dynA = inv[r.a].x != 0.0f, dynB = r.type != AGX_JOINT_MOTOR && inv[r.b].x != 0.0f;
+                uint32_t c = 0;
+                while ((dynA && used.count(((uint64_t)r.a << 32) | c)) || (dynB && used.count(((uint64_t)r.b << 32) | c))) c++;
+                if (dynA) used[((uint64_t)r.a << 32) | c] = true;
+                if (dynB) used[((uint64_t)r.b << 32) | c] = true;
+                if (c >= byColour.size()) byColour.resize(c + 1);
+                byColour[c].push_back(j);
+            }
+            for (const auto& list : byColour) {
+                colours.push_back(make_uint2((uint32_t)order.size(), (uint32_t)(order.size() + list.size())));
+                order.insert(order.end(), list.begin(), list.end());
+            }
+        }
+        d.jointSchedule.typeColour[AGX_JOINT_TYPES] = (uint32_t)colours.size();
+        for (uint32_t j = 0; j < jointRecs.size(); j++) {
+            const agx::JointRec& r = jointRecs[j];
+            const bool both = r.type != AGX_JOINT_MOTOR && inv[r.a].x != 0.0f && inv[r.b].x != 0.0f;
+            edges[j] = both ? make_uint2(r.a, r.b) : make_uint2(UINT32_MAX, UINT32_MAX);
+        }
+        d.nJoints = (uint32_t)jointRecs.size();
+        UP(d.joints, jointRecs); UP(d.jointOrder, order); UP(d.jointColours, colours); UP(d.jointEdges, edges);
+        AL(d.jointData, d.nJoints); AL(d.motorAcc, d.nJoints);
+        w->jointRecsHost = jointRecs;
+        w->jointOrderHost = order;
+        w->jointColoursHost = colours;
+    }
     AL(d.evCount, pairCap); AL(d.evSides, pairCap); AL(d.evOffset, pairCap); AL(d.evImpulse, pairCap);
     w->colCallbackHost.assign(d.C, 0u);
 #undef UP
 #undef AL
     // Initial state: island ids, CCD hit times, empty contact cache, refreshSpatialData, trees.
     hipStream_t st = w->stream;
-    bool ok = agx::launchFillU32(st, d.islandId, UINT32_MAX, d.N) == hipSuccess &&
+    bool ok = (d.nJoints == 0 || agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.motorAcc), 0u, d.nJoints) == hipSuccess) &&
+              agx::launchFillU32(st, d.islandId, UINT32_MAX, d.N) == hipSuccess &&
               agx::launchFillU32(st, d.colCallback, 0u, d.C) == hipSuccess &&
               agx::launchFillU32(st, reinterpret_cast<uint32_t*>(d.colThreshold), 0x7F7FFFFFu, d.C) == hipSuccess &&
               agx::launchFillU32(st, d.sceneSweeps, 0u, d.S) == hipSuccess &&
@@ -1345,6 +1442,49 @@ int agx_world_set_collider_callback(agx_world* w, uint32_t collider, uint32_t ca
     HIP_CHECK(w, hipMemcpyAsync(d.colCallback + collider, &callback_id, sizeof callback_id, hipMemcpyHostToDevice, w->stream));
     HIP_CHECK(w, hipMemcpyAsync(d.colThreshold + collider, &threshold, sizeof threshold, hipMemcpyHostToDevice, w->stream));
     HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    return AGX_OK;
+}
+
+int agx_world_read_joint_order(agx_world* w, uint32_t* order, uint32_t cap, uint32_t* colour_start, uint32_t colour_cap,
+                               uint32_t* type_colour, uint32_t* joints, uint32_t* colours) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    const agx::DeviceWorld& d = w->d;
+    const uint32_t k = (uint32_t)w->jointColoursHost.size();
+    if (joints) *joints = d.nJoints;
+    if (colours) *colours = k;
+    if (order) {
+        if (cap < d.nJoints) return fail(w, AGX_ERR_CAPACITY, "%u joints", d.nJoints);
+        std::copy(w->jointOrderHost.begin(), w->jointOrderHost.end(), order);
+    }
+    if (colour_start) {
+        if (colour_cap < k + 1) return fail(w, AGX_ERR_CAPACITY, "%u joint colours", k);
+        for (uint32_t c = 0; c < k; c++) colour_start[c] = w->jointColoursHost[c].x;
+        colour_start[k] = d.nJoints;
+    }
+    if (type_colour)
+        for (uint32_t t = 0; t <= AGX_JOINT_TYPES; t++) type_colour[t] = d.nJoints ? d.jointSchedule.typeColour[t] : 0u;
+    return AGX_OK;
+}
+
+int agx_world_read_joints(agx_world* w, agx_joint_desc* out, float* motor_impulse, uint32_t n) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    const agx::DeviceWorld& d = w->d;
+    if (n != d.nJoints) return fail(w, AGX_ERR_INVALID_ARGUMENT, "expected %u joints", d.nJoints);
+    if (out)
+        for (uint32_t j = 0; j < n; j++) {
+            const agx::JointRec& r = w->jointRecsHost[j];
+            agx_joint_desc& o = out[j];
+            o.type = r.type; o.body_a = r.a; o.body_b = r.type == AGX_JOINT_MOTOR ? UINT32_MAX : r.b;
+            o.enable_limit = r.enableLimit;
+            o.anchor_a_x = r.anchors.x; o.anchor_a_y = r.anchors.y; o.anchor_b_x = r.anchors.z; o.anchor_b_y = r.anchors.w;
+            o.axis_x = r.axisRest.x; o.axis_y = r.axisRest.y; o.rest_length = r.axisRest.z;
+            o.stiffness = r.params.x; o.damping = r.params.y; o.lower_limit = r.params.z; o.upper_limit = r.params.w;
+            o.target_speed = r.motor.x; o.max_torque = r.motor.y;
+        }
+    if (motor_impulse && n) {
+        HIP_CHECK(w, hipStreamSynchronize(w->stream));
+        HIP_CHECK(w, hipMemcpy(motor_impulse, d.motorAcc, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
     return AGX_OK;
 }
 
@@ -1778,7 +1918,7 @@ static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
 enum SnapTag : uint32_t {
     TAG_SETTINGS = 1, TAG_HOST, TAG_POS, TAG_VEL, TAG_RCS, TAG_INV, TAG_MASS, TAG_INERTIA, TAG_POSE, TAG_WROT,
     TAG_PREVW, TAG_AABB, TAG_ISLAND, TAG_HIT, TAG_COUNTERS, TAG_MANIFOLDS, TAG_SCENE_SWEEPS, TAG_CALLBACK,
-    TAG_THRESHOLD, TAG_COMMANDS, TAG_SLEEP_ID, TAG_SLEEP_TIMER, TAG_STASH, TAG_STASH_ISLAND, TAG_END
+    TAG_THRESHOLD, TAG_COMMANDS, TAG_SLEEP_ID, TAG_SLEEP_TIMER, TAG_STASH, TAG_STASH_ISLAND, TAG_MOTOR, TAG_END
 };
 struct SnapHost {
     float accumulator;
@@ -1793,11 +1933,11 @@ uint64_t fingerprint(const agx_world* w) {
         for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
     };
     const agx::DeviceWorld& d = w->d;
-    const uint32_t counts[6] = { d.S, d.N, d.C, d.nVerts, d.nMat, w->staticBodies };
+    const uint32_t counts[7] = { d.S, d.N, d.C, d.nVerts, d.nMat, w->staticBodies, d.nJoints };
     mix(counts, sizeof counts);
     auto vec = [&](const auto& v) { if (!v.empty()) mix(v.data(), v.size() * sizeof(v[0])); };
     vec(w->colType); vec(w->colMaterial); vec(w->colScene); vec(w->polyStart); vec(w->polyCount);
-    vec(w->colP0); vec(w->colP1); vec(w->localComX); vec(w->localComY);
+    vec(w->colP0); vec(w->colP1); vec(w->localComX); vec(w->localComY); vec(w->jointRecsHost);
     return h;
 }
 
@@ -1828,7 +1968,7 @@ struct SnapArrays {
     std::vector<uint32_t> island, sceneSweeps, callback;
     std::vector<agx_manifold> man, stash;
     std::vector<uint32_t> sleepId, stashIsland;
-    std::vector<float> sleepTimer;
+    std::vector<float> sleepTimer, motorAcc;
     agx::Counters ctr;
 };
 
@@ -1855,6 +1995,7 @@ int writeSnapshot(agx_world* w, SnapWriter& sw) {
     HIP_CHECK(w, down(a.callback, d.colCallback, d.C)); HIP_CHECK(w, down(a.threshold, d.colThreshold, d.C));
     HIP_CHECK(w, down(a.sceneSweeps, d.sceneSweeps, d.S));
     HIP_CHECK(w, down(a.man, d.man[1 - d.cur], a.ctr.prevManifolds));
+    HIP_CHECK(w, down(a.motorAcc, d.motorAcc, d.nJoints));
     if (d.sleepId) {
         HIP_CHECK(w, down(a.sleepId, d.sleepId, d.N)); HIP_CHECK(w, down(a.sleepTimer, d.sleepTimer, d.N));
         HIP_CHECK(w, down(a.stash, d.stash, a.ctr.stashCount)); HIP_CHECK(w, down(a.stashIsland, d.stashIsland, a.ctr.stashCount));
@@ -1893,6 +2034,7 @@ int writeSnapshot(agx_world* w, SnapWriter& sw) {
     sw.section(TAG_SLEEP_TIMER, a.sleepTimer.data(), a.sleepTimer.size() * sizeof(float));
     sw.section(TAG_STASH, a.stash.data(), a.stash.size() * sizeof(agx_manifold));
     sw.section(TAG_STASH_ISLAND, a.stashIsland.data(), a.stashIsland.size() * sizeof(uint32_t));
+    sw.section(TAG_MOTOR, a.motorAcc.data(), a.motorAcc.size() * sizeof(float));
     sw.section(TAG_END, nullptr, 0);
     h.payload = sw.at - payloadStart;
     if (sw.out && !sw.overflow) std::memcpy(sw.out + headerAt, &h, sizeof h);
@@ -1989,7 +2131,8 @@ int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
     const bool slept = sleepBytes != 0;
     ok = ok && sid && stm && (slept ? sleepBytes == N * 4 && timerBytes == N * 4 : timerBytes == 0) &&
          takeVec(TAG_STASH, a.stash, slept ? a.ctr.stashCount : 0) &&
-         takeVec(TAG_STASH_ISLAND, a.stashIsland, slept ? a.ctr.stashCount : 0) && r.section(TAG_END, 0, nullptr);
+         takeVec(TAG_STASH_ISLAND, a.stashIsland, slept ? a.ctr.stashCount : 0) &&
+         takeVec(TAG_MOTOR, a.motorAcc, d.nJoints) && r.section(TAG_END, 0, nullptr);
     if (ok && slept) {
         a.sleepId.assign(reinterpret_cast<const uint32_t*>(sid), reinterpret_cast<const uint32_t*>(sid) + N);
         a.sleepTimer.assign(reinterpret_cast<const float*>(stm), reinterpret_cast<const float*>(stm) + N);
@@ -2008,6 +2151,7 @@ int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
     HIP_CHECK(w, up(d.pose, a.pose)); HIP_CHECK(w, up(d.wRot, a.wRot)); HIP_CHECK(w, up(d.prevW, a.prevW));
     HIP_CHECK(w, up(d.aabb, a.aabb)); HIP_CHECK(w, up(d.colCallback, a.callback));
     HIP_CHECK(w, up(d.colThreshold, a.threshold)); HIP_CHECK(w, up(d.sceneSweeps, a.sceneSweeps));
+    HIP_CHECK(w, up(d.motorAcc, a.motorAcc));
     if (d.sleepId) {   // a snapshot of a world that never slept: every body awake, the stash empty
         if (!slept) { a.sleepId.assign(N, 0u); a.sleepTimer.assign(N, 0.0f); a.ctr.stashCount = 0; }
         HIP_CHECK(w, up(d.sleepId, a.sleepId)); HIP_CHECK(w, up(d.sleepTimer, a.sleepTimer));

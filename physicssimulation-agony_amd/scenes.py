@@ -377,6 +377,75 @@ def sleepy_stacks(stacks: int = 8, height: int = 3, seed: int = 9) -> dict:
                          "sleep_time_threshold": 0.5}}
 
 
+def joint_rig(seed: int = 12, ropes: int = 3, links: int = 10) -> dict:
+    """Every joint type at work beside contacts: ropes of boxes on revolute joints hanging from static
+    anchors, circles paired by springs, a piston on a prismatic joint with limits, a block on a slider
+    with limits, motor-driven wheels on revolute axles under a chassis, and loose polygons dropping
+    onto them. Static bodies are created first, so creation order is the dense order and the joints'
+    body indices are creation indices."""
+    rng = np.random.default_rng(seed)
+    acc = _Acc()
+    statics = []
+
+    def static(x, y, w, h, rot=0.0):
+        acc.add(SHAPE_BOX, x, y, rot=rot, p0=w, p1=h)
+        statics.append(len(statics))
+        return len(statics) - 1
+
+    ground = static(0.0, -1.0, 90.0, 2.0)
+    anchors = [static(-30.0 + 9.0 * r, 14.0, 0.4, 0.4) for r in range(ropes)]
+    rail = static(12.0, 8.0, 0.3, 0.3)
+    post = static(22.0, 2.0, 0.3, 0.3)
+    n_static = len(statics)
+    bodies = [n_static]   # next dynamic index
+    joints = []
+
+    def dynamic(t, x, y, **kw):
+        acc.add(t, x, y, **kw)
+        bodies[0] += 1
+        return bodies[0] - 1
+
+    # ropes: link k spans [x0 + 0.5 k, x0 + 0.5 (k + 1)], pinned end to end
+    for r in range(ropes):
+        x0, y0 = -30.0 + 9.0 * r, 14.0
+        prev, prev_anchor = anchors[r], (0.0, 0.0)
+        for k in range(links):
+            b = dynamic(SHAPE_BOX, x0 + 0.5 * k + 0.25, y0, mass=0.2, p0=0.5, p1=0.15)
+            joints.append(dict(type=1, body_a=prev, body_b=b, anchor_a_x=prev_anchor[0], anchor_a_y=prev_anchor[1],
+                               anchor_b_x=-0.25, anchor_b_y=0.0, stiffness=3000.0, damping=20.0))
+            prev, prev_anchor = b, (0.25, 0.0)
+    # spring pairs
+    for k in range(6):
+        x = -4.0 + 2.5 * k
+        a = dynamic(SHAPE_CIRCLE, x, 6.0, vx=float(rng.uniform(-1, 1)), mass=0.5, p0=0.3)
+        b = dynamic(SHAPE_CIRCLE, x + 0.4, 7.6, mass=0.5, p0=0.3)
+        joints.append(dict(type=0, body_a=a, body_b=b, rest_length=1.2, stiffness=60.0, damping=0.8))
+    # piston on a horizontal rail, moving at 3 m/s into its upper limit
+    piston = dynamic(SHAPE_BOX, 12.0, 8.0, vx=3.0, mass=1.0, p0=0.8, p1=0.4)
+    joints.append(dict(type=2, body_a=rail, body_b=piston, axis_x=1.0, axis_y=0.0, stiffness=5000.0, damping=50.0,
+                       lower_limit=-2.0, upper_limit=1.5, enable_limit=1))
+    # block on a tilted slider (rotation free), sliding down to its lower limit
+    block = dynamic(SHAPE_BOX, 22.0, 2.0, rot=0.3, mass=0.8, p0=0.6, p1=0.6)
+    joints.append(dict(type=3, body_a=post, body_b=block, axis_x=0.6, axis_y=0.8, stiffness=4000.0, damping=40.0,
+                       lower_limit=-1.0, upper_limit=3.0, enable_limit=1))
+    # two cars: a chassis on two motor-driven wheels
+    for c in range(2):
+        cx = 30.0 + 8.0 * c
+        chassis = dynamic(SHAPE_BOX, cx, 1.4, mass=2.0, p0=2.4, p1=0.5)
+        for side in (-1.0, 1.0):
+            wheel = dynamic(SHAPE_CIRCLE, cx + side * 0.9, 0.55, mass=0.5, p0=0.5)
+            joints.append(dict(type=1, body_a=chassis, body_b=wheel, anchor_a_x=side * 0.9, anchor_a_y=-0.85,
+                               stiffness=8000.0, damping=60.0))
+            joints.append(dict(type=4, body_a=wheel, target_speed=-6.0 if c == 0 else 4.0, max_torque=6.0))
+    # loose polygons dropped onto the ropes and springs
+    for k in range(16):
+        v = polygon_outline(rng, int(rng.integers(3, 8)), float(rng.uniform(0.25, 0.45)))
+        dynamic(SHAPE_POLYGON, float(rng.uniform(-30, 15)), float(rng.uniform(16, 24)), rot=float(rng.uniform(0, 6)),
+                mass=float(polygon_area(v)), verts=v)
+    return {"name": "joint_rig", "materials": [(0.0, 0.5, 0.5)], "bodies": acc.bodies(), "joints": joints,
+            "settings": {"velocity_iterations": 8, "position_iterations": 3}}
+
+
 # ---------------------------------------------------------------- batch of independent scenes
 def _polygons(rng, n, r_lo, r_hi):
     """n convex polygons with 3-8 vertices on jittered circles: (radius, vcount, verts (V, 2))."""

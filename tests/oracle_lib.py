@@ -49,6 +49,11 @@ _SIGS = {
     "orc_set_events_enabled": (None, [_P, C.c_int]),
     "orc_collision_events": (C.c_int64, [_P, _P, C.c_uint64, _P, C.c_uint64]),
     "orc_get_events": (C.c_int64, [_P, _P, C.c_uint64]),
+    "orc_add_joints": (C.c_int, [_P, C.c_uint32, _P]),
+    "orc_set_joint_iterations": (None, [_P, _P]),
+    "orc_set_joint_order": (None, [_P, _P, C.c_uint32, _P, C.c_uint32, _P]),
+    "orc_get_motor_impulses": (C.c_int, [_P, _P, C.c_uint32]),
+    "orc_set_motor_impulses": (C.c_int, [_P, _P, C.c_uint32]),
     "orc_get_prev_manifolds": (C.c_int64, [_P, _P, C.c_uint64]),
     "orc_set_sleep": (None, [_P, C.c_float, C.c_float]),
     "orc_get_sleep": (None, [_P, _P, _P, C.c_uint32]),
@@ -319,6 +324,37 @@ class World:
         n = lib().orc_update_sleeping(self.h, dt, _p(m), len(m), _p(out))
         return out[:n]
 
+    def add_joints(self, joints):
+        """Joints as a JOINT_DTYPE array (agx_joint_desc layout) or a list of dicts."""
+        import importlib
+        agx = importlib.import_module("agx")
+        j = joints if isinstance(joints, np.ndarray) else agx.joint_array(joints)
+        j = np.ascontiguousarray(j, agx.JOINT_DTYPE)
+        if lib().orc_add_joints(self.h, len(j), _p(j)) != 0:
+            raise ValueError("joint names a missing body")
+        self.n_joints = getattr(self, "n_joints", 0) + len(j)
+
+    def set_joint_iterations(self, it):
+        it = np.ascontiguousarray(it, np.uint32)
+        lib().orc_set_joint_iterations(self.h, _p(it))
+
+    def set_joint_order(self, order, starts, types):
+        order = np.ascontiguousarray(order, np.uint32)
+        starts = np.ascontiguousarray(starts, np.uint32)
+        types = np.ascontiguousarray(types, np.uint32)
+        lib().orc_set_joint_order(self.h, _p(order), len(order), _p(starts), len(starts) - 1, _p(types))
+
+    def motor_impulses(self):
+        n = getattr(self, "n_joints", 0)
+        out = np.empty(max(n, 1), np.float32)
+        lib().orc_get_motor_impulses(self.h, _p(out), n)
+        return out[:n]
+
+    def set_motor_impulses(self, v):
+        v = np.ascontiguousarray(v, np.float32)
+        if lib().orc_set_motor_impulses(self.h, _p(v), len(v)) != 0:
+            raise ValueError("joint count")
+
     def prev_manifolds(self):
         cap = max(self.counts()["prev_manifolds"], 1)
         out = np.empty(cap, MANIFOLD_DTYPE)
@@ -351,6 +387,8 @@ def world_from_scene(scene, settings=None):
     for m in scene.get("materials", []):
         w.add_material(*m)
     idx, rc = w.add_bodies(scene["bodies"])
+    if scene.get("joints"):
+        w.add_joints(scene["joints"])
     w.refresh()
     return w
 

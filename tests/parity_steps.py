@@ -24,6 +24,10 @@ def oracle_from_gpu(oracle, scene, w, settings=None):
     for g, o in COLLIDER_MAP:
         ow.set(o, w.collider(g))
     ow.set_prev_manifolds(w.manifolds(1))
+    if scene.get("joints"):   # the joints' pass order and the motors' warm-start impulses
+        ow.set_joint_order(*w.joint_order())
+        ow.set_motor_impulses(w.joints()[1])
+        ow.set_joint_iterations(list(w.settings().joint_iterations))
     return ow
 
 
