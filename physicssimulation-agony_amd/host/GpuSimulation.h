@@ -8,6 +8,8 @@
 //   ObjectAPI.h:76-81), getBodies / getColliders / getPolygons viewers as host copies (:96-101)
 //   BodyCreateParams, CircleCreateParams, BoxCreateParams, PolygonCreateParams (ObjectCreateParams.h:14-95)
 //   Material                                                                  (Types/Material.h:6-11)
+//   ConstraintAPI::createSpring / createRevoluteJoint / createPrismaticJoint / createSliderJoint /
+//   createMotor with the systems' CreateParams (ConstraintAPI.h:31-35, ConstraintSystems/*.h:14-26)
 //   CollisionEvent, collision callbacks: createCollisionCallback / destroyCollisionCallback,
 //   setColliderCollisionCallback (Types/CollisionEvent.h:18-30, CollisionCallbackManager.h:14-118,
 //   Simulation.cpp:152-180)
@@ -112,6 +114,64 @@ struct CollisionEvent {
 };
 using CollisionCallback = void (*)(const CollisionEvent& event, Simulation& simulation, void* userData);
 
+// ConstraintSystems' CreateParams (SimulationImpl/ConstraintSystems/*ConstraintSystem.h:14-26), with
+// body indices in place of handles.
+struct SpringCreateParams { Handle bodyHandleA, bodyHandleB; Vec2 localAnchorA, localAnchorB; Real restLength, stiffness, damping; };
+struct RevoluteJointCreateParams { Handle bodyHandleA, bodyHandleB; Vec2 localAnchorA, localAnchorB; Real stiffness, damping; };
+struct PrismaticJointCreateParams {
+    Handle bodyHandleA, bodyHandleB;
+    Vec2 localAnchorA, localAnchorB, localAxisA;
+    Real stiffness, damping;
+    Real lowerLimit = 0, upperLimit = 0;
+    bool enableLimit = false;
+};
+using SliderJointCreateParams = PrismaticJointCreateParams;
+struct MotorCreateParams { Handle bodyHandle; Real targetSpeed, maxTorque; };
+
+// ConstraintAPI creation (SimulationAPIs/ConstraintAPI.h:31-35): before the upload, like bodies;
+// std::nullopt and a message for a missing body. Returns the joint's index among the scene's joints.
+class ConstraintAPI {
+public:
+    explicit ConstraintAPI(Simulation& s) : sim(s) {}
+    ConstraintAPI(const ConstraintAPI&) = delete;
+    ConstraintAPI& operator=(const ConstraintAPI&) = delete;
+    std::optional<uint32_t> createSpring(const SpringCreateParams& p) {
+        agx_joint_desc d = pair(AGX_JOINT_SPRING, p.bodyHandleA, p.bodyHandleB, p.localAnchorA, p.localAnchorB);
+        d.rest_length = p.restLength; d.stiffness = p.stiffness; d.damping = p.damping;
+        return add(d);
+    }
+    std::optional<uint32_t> createRevoluteJoint(const RevoluteJointCreateParams& p) {
+        agx_joint_desc d = pair(AGX_JOINT_REVOLUTE, p.bodyHandleA, p.bodyHandleB, p.localAnchorA, p.localAnchorB);
+        d.stiffness = p.stiffness; d.damping = p.damping;
+        return add(d);
+    }
+    std::optional<uint32_t> createPrismaticJoint(const PrismaticJointCreateParams& p) { return axisJoint(AGX_JOINT_PRISMATIC, p); }
+    std::optional<uint32_t> createSliderJoint(const SliderJointCreateParams& p) { return axisJoint(AGX_JOINT_SLIDER, p); }
+    std::optional<uint32_t> createMotor(const MotorCreateParams& p) {
+        agx_joint_desc d{};
+        d.type = AGX_JOINT_MOTOR; d.body_a = p.bodyHandle; d.target_speed = p.targetSpeed; d.max_torque = p.maxTorque;
+        return add(d);
+    }
+
+private:
+    static agx_joint_desc pair(uint32_t type, Handle a, Handle b, Vec2 anchorA, Vec2 anchorB) {
+        agx_joint_desc d{};
+        d.type = type; d.body_a = a; d.body_b = b;
+        d.anchor_a_x = anchorA.x; d.anchor_a_y = anchorA.y; d.anchor_b_x = anchorB.x; d.anchor_b_y = anchorB.y;
+        d.axis_x = 1.0f;
+        return d;
+    }
+    std::optional<uint32_t> axisJoint(uint32_t type, const PrismaticJointCreateParams& p) {
+        agx_joint_desc d = pair(type, p.bodyHandleA, p.bodyHandleB, p.localAnchorA, p.localAnchorB);
+        d.axis_x = p.localAxisA.x; d.axis_y = p.localAxisA.y;
+        d.stiffness = p.stiffness; d.damping = p.damping;
+        d.lower_limit = p.lowerLimit; d.upper_limit = p.upperLimit; d.enable_limit = p.enableLimit ? 1u : 0u;
+        return add(d);
+    }
+    std::optional<uint32_t> add(const agx_joint_desc& d);
+    Simulation& sim;
+};
+
 class ObjectAPI {
 public:
     explicit ObjectAPI(Simulation& s) : sim(s) {}
@@ -156,7 +216,7 @@ public:
     // Simulation.h:50-76 defaults (6 velocity / 3 position iterations, CCD on, 1/300 s interval).
     using SimulationSettings = agx_settings;
 
-    explicit Simulation(uint32_t scenes = 1) : api(*this) {
+    explicit Simulation(uint32_t scenes = 1) : api(*this), constraintApi(*this) {
         agx_settings_default(&settings);
         if (agx_builder_create(scenes, &builder) != AGX_OK) fail("agx_builder_create", agx_last_error());
     }
@@ -181,6 +241,7 @@ public:
     }
 
     ObjectAPI& objects() noexcept { return api; }
+    ConstraintAPI& constraints() noexcept { return constraintApi; }
 
     // CollisionCallbackManager::create / destroy (CollisionCallbackManager.h:38-80): slot 0 is the
     // permanent "none", destroyed slots are reused last-in first-out.
@@ -332,7 +393,19 @@ private:
     agx_builder* builder = nullptr;
     agx_world* world = nullptr;
     ObjectAPI api;
+    ConstraintAPI constraintApi;
+    friend class ConstraintAPI;
 };
+
+inline std::optional<uint32_t> ConstraintAPI::add(const agx_joint_desc& d) {
+    if (sim.world) { Simulation::fail("createConstraint", "the scene is already on the GPU"); return std::nullopt; }
+    uint32_t index = 0;
+    if (agx_builder_add_joints(sim.builder, 0, 1, &d, &index) != AGX_OK) {
+        Simulation::fail("createConstraint", agx_last_error());
+        return std::nullopt;
+    }
+    return index;
+}
 
 inline std::optional<ObjectDenseIndex> ObjectAPI::add(uint8_t type, const BodyCreateParams& b, float p0, float p1,
                                                       const std::vector<Vec2>* verts) {

@@ -78,6 +78,12 @@ int main() {
     bool callbacksSet = cb.has_value() && !sim.setColliderCollisionCallback(5, 77, 0.0f);   // 77 is not alive
     for (ObjectDenseIndex c = 4; c < 44; c++) callbacksSet = callbacksSet && sim.setColliderCollisionCallback(c, *cb, 0.0f);
 
+    // a spinning motor on one body and a spring between two others (ConstraintAPI)
+    auto& joints = sim.constraints();
+    const bool jointsMade = joints.createMotor({ 150, 3.0f, 50.0f }).has_value() &&
+                            joints.createSpring({ 151, 152, { 0, 0 }, { 0, 0 }, 1.2f, 30.0f, 0.5f }).has_value() &&
+                            !joints.createMotor({ 100000, 1.0f, 1.0f }).has_value();   // no such body
+
     // a setter before the upload is applied at the first update
     const bool queuedEarly = objects.setBodyVelocity(10, { 0.0f, 2.0f });
     uint32_t steps = 0;
@@ -119,6 +125,6 @@ int main() {
                 queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0, tally.events, tally.maxImpulse);
     const bool ok = made == 192 && steps == 241 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
                     st.overflow == 0 && queuedEarly && queued && applied && viewers && callbacksSet && tally.events > 0 &&
-                    tally.badNormals == 0 && tally.maxImpulse > 0.0f;
+                    tally.badNormals == 0 && tally.maxImpulse > 0.0f && jointsMade;
     return ok ? 0 : 1;
 }
