@@ -379,15 +379,15 @@ def run_batch(agx, scenes_mod, args, dt, learned=True):
         out["tuning"] = {"solve_blocks_per_cu": sb, "colour_blocks_per_cu": cb, "from_cache": 0, "steps": 0, "given": 1}
     else:
         n0 = plain_w.stats().solved_steps
-        tune = plain_w.autotune(dt, 2)
+        tune = plain_w.autotune(dt, 4)
         tuned_steps = plain_w.stats().solved_steps - n0
         for w in worlds[1:]:
-            w.set_tuning(tune.solve_blocks_per_cu, tune.colour_blocks_per_cu)
+            w.set_tuning(tune.solve_blocks_per_cu, tune.colour_blocks_per_cu, tune.scene_whole)
             w.step_graph(dt, tuned_steps)
-        out["tuning"] = {"solve_blocks_per_cu": tune.solve_blocks_per_cu, "colour_blocks_per_cu": tune.colour_blocks_per_cu,
-                         "from_cache": int(tune.from_cache), "steps": int(tuned_steps)}
-    log(f"batch: {out['tuning']['solve_blocks_per_cu']}/{out['tuning']['colour_blocks_per_cu']} blocks per CU "
-        f"(sweeps/colouring), {out['tuning']['steps']} tuning steps")
+        out["tuning"] = {"scene_whole": int(tune.scene_whole), "from_cache": int(tune.from_cache),
+                         "steps": int(tuned_steps), "ms_per_step": round(float(tune.ms_per_step), 4)}
+    log(f"batch: tuned per-scene launch: heaviest {out['tuning'].get('scene_whole')} scenes on whole blocks, "
+        f"{out['tuning']['steps']} tuning steps")
     mode = agx.LEARN_PREDICT | agx.LEARN_NEW_CONTACTS
     if learned:
         lw = worlds[1]

@@ -395,19 +395,25 @@ const char* agx_profile_segment_name(uint32_t segment);
 
 /* Launch-shape tuning. The reference partitions work by fixed sizes (COLLISION_COUNT_PER_WORKER,
  * Solving/BodyCollisionSolver.cpp:211-212; DETERMINISTIC_CHUNK_SIZE,
- * CollisionDetection/Narrow/NarrowPhaseCollisionDetector.h:153). Here the co-resident blocks per CU
- * of the dataflow colouring and sweep kernels are measured instead: agx_world_autotune times
- * graph-replayed steps for each shape (1, 2, 4 ... up to the maximum; two interleaved rounds, which
+ * CollisionDetection/Narrow/NarrowPhaseCollisionDetector.h:153). Here the launch shape of the path the
+ * world's steps run is measured instead: for a world solved with the dataflow kernels, their
+ * co-resident blocks per CU (1, 2, 4 ... up to the maximum); for a world solved scene by scene
+ * (solve_per_scene, every scene in LDS), how many of the step's heaviest scenes whole blocks take
+ * before the rest go to lane groups (half, one, one and a half and two per block).
+ * agx_world_autotune times graph-replayed steps for each candidate (two interleaved rounds, which
  * advances the simulation) and keeps the fastest, per sweep run when a residual target makes the
- * sweep count vary. Every shape steps to the same bits. Results are cached in the process per GPU, world
- * size class and solver settings, and in the file named by the environment variable AGX_TUNE_CACHE
- * when it is set; a cached shape is applied without stepping (from_cache = 1). */
+ * sweep count vary. Every shape steps to the same bits. Results are cached in the process per GPU,
+ * world size class and solver settings, and in the file named by the environment variable
+ * AGX_TUNE_CACHE when it is set; a cached shape is applied without stepping (from_cache = 1). */
 typedef struct agx_tuning {
-    uint32_t solve_blocks_per_cu;       /* sweep kernel */
-    uint32_t colour_blocks_per_cu;      /* colouring kernel */
+    uint32_t solve_blocks_per_cu;       /* dataflow sweep kernel */
+    uint32_t colour_blocks_per_cu;      /* dataflow colouring kernel */
     float ms_per_step;                  /* mean step time measured with this shape (autotune only) */
     uint32_t from_cache;
     uint32_t max_solve_blocks_per_cu, max_colour_blocks_per_cu;   /* limits (get_tuning, autotune) */
+    uint32_t scene_whole;               /* per-scene solve: heaviest scenes whole blocks take (get_tuning,
+                                           autotune; 0 when the world cannot solve scene by scene);
+                                           set_tuning applies it unless it is 0 */
 } agx_tuning;
 int agx_world_get_tuning(agx_world* w, agx_tuning* out);
 int agx_world_set_tuning(agx_world* w, const agx_tuning* in);

@@ -105,10 +105,12 @@ class Gate(C.Structure):
 
 
 class Tuning(C.Structure):
-    """agx_tuning: co-resident blocks per CU of the dataflow colouring and sweep kernels."""
+    """agx_tuning: co-resident blocks per CU of the dataflow colouring and sweep kernels, and the
+    heaviest scenes whole blocks take in the per-scene solve."""
     _fields_ = [("solve_blocks_per_cu", C.c_uint32), ("colour_blocks_per_cu", C.c_uint32),
                 ("ms_per_step", C.c_float), ("from_cache", C.c_uint32),
-                ("max_solve_blocks_per_cu", C.c_uint32), ("max_colour_blocks_per_cu", C.c_uint32)]
+                ("max_solve_blocks_per_cu", C.c_uint32), ("max_colour_blocks_per_cu", C.c_uint32),
+                ("scene_whole", C.c_uint32)]
 
 
 LEARN_OFF, LEARN_PREDICT, LEARN_COLLECT, LEARN_UNGATED, LEARN_NEW_CONTACTS = 0, 1, 2, 4, 8
@@ -590,8 +592,9 @@ class World:
         _check(lib().agx_world_get_tuning(self._h, C.byref(t)), self._h)
         return t
 
-    def set_tuning(self, solve_blocks_per_cu: int, colour_blocks_per_cu: int):
-        t = Tuning(solve_blocks_per_cu=solve_blocks_per_cu, colour_blocks_per_cu=colour_blocks_per_cu)
+    def set_tuning(self, solve_blocks_per_cu: int, colour_blocks_per_cu: int, scene_whole: int = 0):
+        t = Tuning(solve_blocks_per_cu=solve_blocks_per_cu, colour_blocks_per_cu=colour_blocks_per_cu,
+                   scene_whole=scene_whole)
         _check(lib().agx_world_set_tuning(self._h, C.byref(t)), self._h)
 
     def autotune(self, dt: float, steps_per_trial: int = 2) -> Tuning:

@@ -61,6 +61,7 @@ struct agx_world {
     bool histReady = false;          // the previous step recorded its manifolds' starting accumulators
     bool learnedNarrow = false;      // this step's narrow phase ran with a network set
     int colourPerCuMax = 1, solvePerCuMax = 1;   // launch-shape limits (occupancy, allocations)
+    std::vector<uint32_t> sceneSizes;   // body counts of the scenes in size order (per-scene launch shape)
     // deferred body modifications (agx_world_queue_body), applied in order at the next flush
     struct BodyCommand { uint32_t kind, body; float a, b; };
     std::vector<BodyCommand> commands;
@@ -612,6 +613,7 @@ std::vector<unsigned char> graphKey(const agx_world* w, float dt) {
     put(bufs, sizeof bufs);
     const int shape[2] = { w->d.colourBlocks, w->d.solveBlocks };
     put(shape, sizeof shape);
+    put(&w->d.sceneLaunch, sizeof w->d.sceneLaunch);
     return k;
 }
 
@@ -959,6 +961,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     std::vector<uint32_t> orderedBodies(d.S);
     for (uint32_t k = 0; k < d.S; k++) orderedBodies[k] = sceneBodyStart[sceneOrder[k] + 1] - sceneBodyStart[sceneOrder[k]];
     d.sceneLdsBytes = agx::sceneSolveSetup(orderedBodies.data(), d.S, d.numCU, wholeCap, d.sceneLaunch, d.sceneColourLdsBytes);
+    w->sceneSizes = orderedBodies;
     d.sceneMaxBodies = maxSceneBodies;
     // AGX_COLOUR_LDS_CAP lowers how many of a scene's manifolds the per-scene colouring keeps in
     // LDS (larger scenes colour from global scratch instead); tests use it to reach that path
@@ -1203,9 +1206,10 @@ std::string tuneKey(agx_world* w) {
     (void)hipGetDeviceProperties(&prop, dev);
     auto bucket = [](uint64_t v) { uint32_t b = 0; while (v > 1) { v >>= 1; b++; } return b; };
     char buf[512];
-    snprintf(buf, sizeof buf, "%s|%s|cu%d|bodies2^%u|scenes%u|iv%u|ip%u|target%d|learned%u", prop.name,
-             prop.gcnArchName, prop.multiProcessorCount, bucket(w->d.N), w->d.S, w->settings.velocity_iterations, w->settings.position_iterations, w->settings.residual_target > 0.0f ? 1 : 0,
-             w->learnMode);
+    snprintf(buf, sizeof buf, "%s|%s|cu%d|bodies2^%u|scenes%u|iv%u|ip%u|target%d|learned%u|perscene%d", prop.name,
+             prop.gcnArchName, prop.multiProcessorCount, bucket(w->d.N), w->d.S, w->settings.velocity_iterations,
+             w->settings.position_iterations, w->settings.residual_target > 0.0f ? 1 : 0, w->learnMode,
+             w->d.sceneLdsBytes && w->settings.solve_per_scene ? 1 : 0);
     for (char* c = buf; *c; c++) if (*c == ' ' || *c == '\n') *c = '_';
     return buf;
 }
@@ -1218,10 +1222,13 @@ void loadTuneFile() {
     if (!path || !*path) return;
     FILE* f = std::fopen(path, "r");
     if (!f) return;
-    char key[512];
-    agx_tuning t{};
-    while (std::fscanf(f, "%511s %u %u %f", key, &t.solve_blocks_per_cu, &t.colour_blocks_per_cu, &t.ms_per_step) == 4)
-        g_tuneCache[key] = t;
+    char line[768], key[512];
+    while (std::fgets(line, sizeof line, f)) {
+        agx_tuning t{};
+        if (std::sscanf(line, "%511s %u %u %f %u", key, &t.solve_blocks_per_cu, &t.colour_blocks_per_cu, &t.ms_per_step,
+                        &t.scene_whole) >= 4)
+            g_tuneCache[key] = t;
+    }
     std::fclose(f);
 }
 
@@ -1230,7 +1237,8 @@ void storeTune(const std::string& key, const agx_tuning& t) {
     const char* path = std::getenv("AGX_TUNE_CACHE");
     if (!path || !*path) return;
     if (FILE* f = std::fopen(path, "a")) {
-        std::fprintf(f, "%s %u %u %.6f\n", key.c_str(), t.solve_blocks_per_cu, t.colour_blocks_per_cu, t.ms_per_step);
+        std::fprintf(f, "%s %u %u %.6f %u\n", key.c_str(), t.solve_blocks_per_cu, t.colour_blocks_per_cu, t.ms_per_step,
+                     t.scene_whole);
         std::fclose(f);
     }
 }
@@ -1246,6 +1254,7 @@ int agx_world_get_tuning(agx_world* w, agx_tuning* out) {
     out->colour_blocks_per_cu = (uint32_t)(w->d.colourBlocks / w->d.numCU);
     out->max_solve_blocks_per_cu = (uint32_t)w->solvePerCuMax;
     out->max_colour_blocks_per_cu = (uint32_t)w->colourPerCuMax;
+    out->scene_whole = w->d.sceneLdsBytes ? w->d.sceneLaunch.whole : 0u;
     return AGX_OK;
 }
 
@@ -1256,6 +1265,16 @@ int agx_world_set_tuning(agx_world* w, const agx_tuning* in) {
         (int)in->colour_blocks_per_cu > w->colourPerCuMax)
         return fail(w, AGX_ERR_INVALID_ARGUMENT, "blocks per CU must be 1..%d (sweeps) and 1..%d (colouring)",
                     w->solvePerCuMax, w->colourPerCuMax);
+    if (in->scene_whole) {
+        if (!w->d.sceneLdsBytes) return fail(w, AGX_ERR_INVALID_ARGUMENT, "the world does not solve scene by scene");
+        agx::SceneLaunch launch;
+        uint32_t colourBytes = 0;
+        const uint32_t bytes = agx::sceneSolveSetup(w->sceneSizes.data(), w->d.S, w->d.numCU,
+                                                    std::min(in->scene_whole, w->d.S), launch, colourBytes);
+        if (bytes != w->d.sceneLdsBytes || colourBytes != w->d.sceneColourLdsBytes)
+            return fail(w, AGX_ERR_HIP, "per-scene launch shape not available");
+        w->d.sceneLaunch = launch;
+    }
     w->d.solveBlocks = w->d.numCU * (int)in->solve_blocks_per_cu;
     w->d.colourBlocks = w->d.numCU * (int)in->colour_blocks_per_cu;
     return AGX_OK;
@@ -1284,13 +1303,27 @@ int agx_world_autotune(agx_world* w, float dt, uint32_t steps_per_trial, agx_tun
         }
     }
     std::vector<agx_tuning> cand;
-    for (uint32_t sb = 1; sb <= (uint32_t)w->solvePerCuMax; sb *= 2)
-        for (uint32_t cb = 1; cb <= (uint32_t)w->colourPerCuMax; cb *= 2) {
-            agx_tuning t{};
-            t.solve_blocks_per_cu = sb;
-            t.colour_blocks_per_cu = cb;
-            cand.push_back(t);
+    if (w->d.sceneLdsBytes && w->settings.solve_per_scene) {
+        // the steps solve scene by scene: the dataflow kernels never run, the whole-block share does
+        agx_tuning base{};
+        if (int r = agx_world_get_tuning(w, &base)) return r;
+        const uint32_t blocks = (uint32_t)std::max(w->d.sceneLaunch.blocks, 1);
+        for (uint32_t halves : { 1u, 2u, 3u, 4u }) {
+            agx_tuning t = base;
+            t.scene_whole = std::max(1u, std::min(w->d.S, blocks * halves / 2));
+            bool seen = false;
+            for (const agx_tuning& c : cand) seen |= c.scene_whole == t.scene_whole;
+            if (!seen) cand.push_back(t);
         }
+    } else {
+        for (uint32_t sb = 1; sb <= (uint32_t)w->solvePerCuMax; sb *= 2)
+            for (uint32_t cb = 1; cb <= (uint32_t)w->colourPerCuMax; cb *= 2) {
+                agx_tuning t{};
+                t.solve_blocks_per_cu = sb;
+                t.colour_blocks_per_cu = cb;
+                cand.push_back(t);
+            }
+    }
     // With a residual target the sweep count varies from step to step, so a trial's time is
     // divided by the sweeps it ran (velocity and position) to compare launch shapes fairly.
     const bool perSweep = w->settings.residual_target > 0.0f && !w->settings.collide_only;

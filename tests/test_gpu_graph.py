@@ -9,11 +9,11 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel")
 
 
-def _world(agx, scenes):
+def _world(agx, scenes, **settings):
     scene = scenes.pile_small(seed=1)
     b = agx.Builder()
     agx.build_scene(b, scene)
-    return agx.World(b, agx.default_settings(**scene["settings"], track_residual=1))
+    return agx.World(b, agx.default_settings(**{**scene["settings"], "track_residual": 1, **settings}))
 
 
 def _same(a, b):
@@ -64,7 +64,7 @@ def test_launch_shapes_give_identical_steps_and_autotune(agx, scenes):
     """The colouring and the solve order inside a colour do not depend on how many blocks run the
     dataflow kernels, so every launch shape steps to the same bits; the tuner picks one of them and
     reuses it from its cache."""
-    a, b = _world(agx, scenes), _world(agx, scenes)
+    a, b = _world(agx, scenes, solve_per_scene=0), _world(agx, scenes, solve_per_scene=0)
     t = a.tuning()
     assert t.solve_blocks_per_cu >= 1 and t.max_solve_blocks_per_cu >= t.solve_blocks_per_cu
     b.set_tuning(1, 1)

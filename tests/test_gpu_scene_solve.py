@@ -215,3 +215,25 @@ def test_dataflow_solve_counts_convergence(agx, scenes):
             assert st.converged_scenes == 1 and st.converged_bodies == ndyn
         assert st.converged_bodies in (0, ndyn)
     assert 0 < st.total_converged_bodies - total0 <= steps * ndyn
+
+
+def test_scene_launch_shapes_step_alike_and_autotune(agx, scenes):
+    """How many of the heaviest scenes whole blocks take (the rest go to lane groups) changes no
+    scene's result; the tuner of a world solved scene by scene picks among such shares (not among
+    shapes of the dataflow kernels, which its steps never run) and reuses its pick from the cache."""
+    parts = scenes.batch(40, n_max=1500)
+    base = dict(velocity_iterations=64, residual_target=1e-3, position_iterations=3, track_residual=1)
+    a, b = _world(agx, parts, **base), _world(agx, parts, **base)
+    t = a.tuning()
+    assert t.scene_whole >= 1
+    b.set_tuning(t.solve_blocks_per_cu, t.colour_blocks_per_cu, 1)
+    assert b.tuning().scene_whole == 1
+    a.step(DT, 12)
+    b.step_graph(DT, 12)
+    _same(a, b)
+    np.testing.assert_array_equal(a.scene_sweeps(), b.scene_sweeps())
+    r = a.autotune(DT, 2)
+    assert r.from_cache == 0 and r.ms_per_step > 0 and 1 <= r.scene_whole <= len(parts)
+    assert a.tuning().scene_whole == r.scene_whole
+    again = a.autotune(DT, 2)
+    assert again.from_cache == 1 and again.scene_whole == r.scene_whole
