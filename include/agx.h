@@ -281,6 +281,21 @@ int agx_world_queue_body(agx_world* w, uint32_t kind, uint32_t body, float a, fl
  * manifolds (*count receives the total). */
 int agx_world_narrowphase(agx_world* w, const uint32_t* pairs_a, const uint32_t* pairs_b, uint64_t n, agx_manifold* out,
                           uint64_t cap, uint64_t* count);
+
+/* BodyCollisionSolver::solveCollisions on injected manifolds (Solving/BodyCollisionSolver.cpp:229-980:
+ * constraint data, warm start, velocity and position sweeps with the world's settings): the n
+ * manifolds (colliders a, b; points, normal, depth, contact ids and starting impulses, as
+ * agx_world_read_manifolds gives them) are solved against the world's current body state, which
+ * the call updates (velocities, positions). order (n entries, a permutation of 0 .. n-1, or NULL for
+ * the world's own colouring) is the sequential solve order, e.g. the reference planner's waves
+ * (SolvingPlanner.cpp:141-310): it runs level by level, a manifold's level being one more than the
+ * highest level before it at either of its dynamic bodies, which keeps every body's sequence of
+ * updates and so the sequential result; AGX_ERR_CAPACITY when the order needs more than
+ * AGX_MAX_COLOURS levels. out (n entries) receives the solved manifolds. Between steps only; the world's step state (warm-start source, next step) is not
+ * changed, but the last step's pair and manifold counts in agx_world_read_stats are replaced. */
+#define AGX_MAX_COLOURS 129
+int agx_world_solve_contacts(agx_world* w, const agx_manifold* manifolds, uint64_t n, const uint32_t* order,
+                             agx_manifold* out);
 int agx_world_flush(agx_world* w);
 int agx_world_write_body_f32(agx_world* w, uint32_t field, const float* in, uint32_t n);
 int agx_world_read_collider_f32(agx_world* w, uint32_t field, float* out, uint32_t n);

@@ -159,6 +159,7 @@ _SIGS = {
     "agx_world_queue_body": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
     "agx_world_flush": (C.c_int, [_P]),
     "agx_world_narrowphase": (C.c_int, [_P, _P, _P, C.c_uint64, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "agx_world_solve_contacts": (C.c_int, [_P, _P, C.c_uint64, _P, _P]),
     "agx_world_read_broadphase_order": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "agx_world_read_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "agx_world_write_manifolds": (C.c_int, [_P, C.c_uint32, _P, C.c_uint64]),
@@ -417,6 +418,15 @@ class World:
         cnt = C.c_uint64()
         _check(lib().agx_world_narrowphase(self._h, _ptr(a), _ptr(b), len(a), _ptr(out), len(out), C.byref(cnt)), self._h)
         return out[:cnt.value]
+
+    def solve_contacts(self, manifolds, order=None) -> np.ndarray:
+        """Solves the given manifolds against the current bodies (which it updates) in the given
+        sequential order (None: the world's own colouring); returns the solved manifolds."""
+        m = np.ascontiguousarray(manifolds, MANIFOLD_DTYPE)
+        o = None if order is None else np.ascontiguousarray(order, np.uint32)
+        out = np.empty(max(len(m), 1), MANIFOLD_DTYPE)
+        _check(lib().agx_world_solve_contacts(self._h, _ptr(m), len(m), _ptr(o), _ptr(out)), self._h)
+        return out[:len(m)]
 
     def set_body(self, field: str, values):
         v = np.ascontiguousarray(values, np.float32)

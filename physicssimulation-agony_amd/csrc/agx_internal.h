@@ -12,6 +12,7 @@ namespace agx {
 constexpr uint32_t WAVE = 64;
 constexpr uint32_t FIRST_FIT_COLOURS = 128;  // chosen first-fit from two 64-bit masks per body
 constexpr uint32_t MAX_COLOURS = FIRST_FIT_COLOURS + 1;   // + one shared overflow colour
+static_assert(MAX_COLOURS == AGX_MAX_COLOURS, "colour count of the C ABI");
 constexpr uint32_t COLOUR_BLOCKS = 512;     // blocks of the colour histogram / scatter kernels
 constexpr uint32_t MAX_FLOW_ITEMS = 64;     // constraints per lane in the dataflow kernels
 constexpr uint32_t COLOUR_ALIGN = 64;       // solve positions of each colour start on a wave boundary
@@ -510,7 +511,8 @@ hipError_t launchHashBuild(hipStream_t st, DeviceWorld& w);
 
 // Colouring, solve order and constraint prep; launchWarmStart then applies the warm-start
 // impulses (and clears the solve kernel's flags).
-hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w);
+// coloursGiven: colourId already holds every manifold's colour (the first-fit colouring is skipped)
+hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w, bool coloursGiven = false);
 hipError_t launchWarmStart(hipStream_t st, DeviceWorld& w);
 // launchSolve: the velocity and position sweeps (one kernel); launchSolveFinish: accumulators
 // back to the manifolds, residual and error totals.

@@ -1779,7 +1779,7 @@ void dataflowBlocksPerCU(int& colour, int& solve) {
     solve = std::max(1, std::min(b, 4));
 }
 
-hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
+hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w, bool coloursGiven) {
     const uint32_t gm = blocksForManifolds(w);
     hipLaunchKernelGGL(kFlowClear, dim3(blocksFor(w.N)), dim3(256), 0, st, w.N, w.degree, w.incFill, w.colRec);
     hipLaunchKernelGGL(kDegree, dim3(gm), dim3(256), 0, st, w.man[w.cur], w.ctr, w.colBody, w.inv, w.manBodies, w.degree);
@@ -1791,8 +1791,10 @@ hipError_t launchColourPrep(hipStream_t st, DeviceWorld& w) {
                        w.posOf, w.rank, w.incSorted);
     void* args[] = { (void*)&w.ctr, (void*)&w.manBodies, (void*)&w.inv, (void*)&w.rank, (void*)&w.colRec,
                      (void*)&w.colourId };
-    e = launchResident(reinterpret_cast<const void*>(kColourFlow), (uint32_t)w.colourBlocks, w.numCU, args, st);
-    if (e != hipSuccess) return e;
+    if (!coloursGiven) {
+        e = launchResident(reinterpret_cast<const void*>(kColourFlow), (uint32_t)w.colourBlocks, w.numCU, args, st);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(kColourHist, dim3(COLOUR_BLOCKS), dim3(256), 0, st, w.ctr, w.colourId, w.colourHist);
     hipLaunchKernelGGL(kColourColumnScan, dim3(MAX_COLOURS), dim3(256), 0, st, w.ctr, w.colourHist, COLOUR_BLOCKS);
     hipLaunchKernelGGL(kColourStarts, dim3(1), dim3(64), 0, st, w.ctr);

@@ -1461,6 +1461,63 @@ int agx_world_narrowphase(agx_world* w, const uint32_t* pairs_a, const uint32_t*
     return AGX_OK;
 }
 
+int agx_world_solve_contacts(agx_world* w, const agx_manifold* manifolds, uint64_t n, const uint32_t* order,
+                             agx_manifold* out) {
+    if (!w || (n && (!manifolds || !out))) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    agx::DeviceWorld& d = w->d;
+    if (n > d.manCap) return fail(w, AGX_ERR_CAPACITY, "%llu manifolds exceed the capacity %llu", (unsigned long long)n,
+                                  (unsigned long long)d.manCap);
+    for (uint64_t i = 0; i < n; i++) {
+        const agx_manifold& m = manifolds[i];
+        if (m.a >= d.C || m.b >= d.C || m.a == m.b || m.count < 1 || m.count > 2)
+            return fail(w, AGX_ERR_INVALID_ARGUMENT, "manifold %llu: colliders (%u, %u), %u points", (unsigned long long)i,
+                        m.a, m.b, m.count);
+    }
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    std::vector<uint16_t> level;
+    if (order && n) {   // levels of the sequential order: one more than the last level at either dynamic body
+        std::vector<float2> inv(d.N);
+        HIP_CHECK(w, hipMemcpy(inv.data(), d.inv, d.N * sizeof(float2), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> next(d.N, 0);   // level after the last manifold so far at the body
+        std::vector<unsigned char> seen(n, 0);
+        level.assign(n, 0);
+        for (uint64_t k = 0; k < n; k++) {
+            const uint32_t m = order[k];
+            if (m >= n || seen[m]) return fail(w, AGX_ERR_INVALID_ARGUMENT, "order is not a permutation of 0..%llu",
+                                               (unsigned long long)n - 1);
+            seen[m] = 1;
+            const uint32_t a = w->colBodyHost[manifolds[m].a], b = w->colBodyHost[manifolds[m].b];
+            const bool da = inv[a].x != 0.0f, db = inv[b].x != 0.0f;
+            const uint32_t lv = std::max(da ? next[a] : 0u, db ? next[b] : 0u);
+            if (lv >= agx::MAX_COLOURS)
+                return fail(w, AGX_ERR_CAPACITY, "the order needs more than %u levels", agx::MAX_COLOURS);
+            level[m] = (uint16_t)lv;
+            if (da) next[a] = lv + 1;
+            if (db) next[b] = lv + 1;
+        }
+    }
+    if (int rc = beginStep(w)) return rc;
+    const uint32_t m32 = (uint32_t)n;
+    if (n) HIP_CHECK(w, hipMemcpyAsync(d.man[d.cur], manifolds, n * sizeof(agx_manifold), hipMemcpyHostToDevice, w->stream));
+    HIP_CHECK(w, hipMemcpyAsync(&d.ctr->manifolds, &m32, sizeof m32, hipMemcpyHostToDevice, w->stream));
+    if (!level.empty())
+        HIP_CHECK(w, hipMemcpyAsync(d.colourId, level.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, w->stream));
+    const agx_settings& s = w->settings;
+    const int track = s.track_residual ? 1 : 0;
+    HIP_CHECK(w, agx::launchColourPrep(w->stream, d, !level.empty()));
+    HIP_CHECK(w, agx::launchWarmStart(w->stream, d));
+    HIP_CHECK(w, agx::launchSolve(w->stream, d, s.velocity_iterations, s.position_iterations, track, s.residual_target));
+    HIP_CHECK(w, agx::launchSolveFinish(w->stream, d, track, s.residual_target, s.track_errors ? 1 : 0, 0));
+    HIP_CHECK(w, hipStreamSynchronize(w->stream));
+    uint32_t overflow = 0;
+    HIP_CHECK(w, hipMemcpy(&overflow, &d.ctr->overflow, sizeof overflow, hipMemcpyDeviceToHost));
+    if (overflow & (AGX_OVERFLOW_STALL | AGX_OVERFLOW_ITEMS))
+        return fail(w, AGX_ERR_CAPACITY, "the solve reported overflow bits 0x%x", overflow);
+    if (n) HIP_CHECK(w, hipMemcpy(out, d.man[d.cur], n * sizeof(agx_manifold), hipMemcpyDeviceToHost));
+    return AGX_OK;
+}
+
 int agx_world_set_collider_callback(agx_world* w, uint32_t collider, uint32_t callback_id, float threshold) {
     if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
     agx::DeviceWorld& d = w->d;

@@ -126,3 +126,59 @@ def test_standalone_narrowphase_on_injected_pairs(agx, oracle, scenes):
         assert np.array_equal(w.body(f), twin.body(f)), f
     with pytest.raises(Exception):
         w.narrowphase(np.array([0], np.uint32), np.array([0], np.uint32))
+
+
+def _levels(w, mans, order):
+    """Levels of a sequential order: one more than the last level at either dynamic body."""
+    body_of = w.collider("body")
+    dyn = w.body("inv_mass") != 0
+    nxt = np.zeros(w.counts.bodies, np.int64)
+    top = 0
+    for m in order:
+        a, b = body_of[mans["a"][m]], body_of[mans["b"][m]]
+        lv = max(nxt[a] if dyn[a] else 0, nxt[b] if dyn[b] else 0)
+        if dyn[a]:
+            nxt[a] = lv + 1
+        if dyn[b]:
+            nxt[b] = lv + 1
+        top = max(top, lv + 1)
+    return top
+
+
+def test_standalone_solve_in_a_given_order(agx, oracle, scenes):
+    """agx_world_solve_contacts on the last step's manifolds from the post-update bodies, in a
+    sequential order the caller gives (as the reference planner's waves would; here a seeded
+    shuffle and the reverse of the world's own order) and in the world's own colouring, against the
+    oracle solving the same manifolds from the same bodies in that order: impulses and body
+    velocities and positions within 1e-4. The next step starts from the solved bodies; an order that
+    is not a permutation is refused and changes nothing."""
+    from parity_steps import assert_solved_close
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene)
+    w.step(DT, 120)
+    mans = w.manifolds(1)
+    assert len(mans) > 300
+    own, _ = w.solve_order()
+    rng = np.random.default_rng(5)
+    for label, order in (("shuffle", rng.permutation(len(mans)).astype(np.uint32)),
+                         ("reverse", own[::-1].copy()), ("own colouring", None)):
+        if order is not None:
+            print(label, "levels", _levels(w, mans, order), flush=True)
+        ow = oracle_from_gpu(oracle, scene, w, dict(v=8, p=3))
+        before = {g: w.body(g).copy() for g, _ in BODY_MAP}
+        solved_g = w.solve_contacts(mans, order)
+        if order is None:
+            order, _ = w.solve_order()
+        solved_o = ow.solve(mans, order=order, v_it=8, p_it=3)
+        assert_solved_close(solved_g, solved_o, label)
+        for g, o in (("vel_x", "velX"), ("vel_y", "velY"), ("ang_vel", "angVel"), ("com_x", "comX"), ("com_y", "comY")):
+            np.testing.assert_allclose(w.body(g), ow.get(o), rtol=1e-4, atol=1e-4, err_msg=f"{label} {g}")
+        assert not np.array_equal(before["vel_y"], w.body("vel_y"))
+    bad = np.zeros(len(mans), np.uint32)
+    snap = {g: w.body(g).copy() for g, _ in BODY_MAP}
+    with pytest.raises(agx.AgxError):
+        w.solve_contacts(mans, bad)
+    for g in snap:
+        assert np.array_equal(snap[g], w.body(g)), g
+    w.step(DT, 1)
+    assert w.stats().overflow == 0
