@@ -190,6 +190,23 @@ int agx_builder_scene_counts(agx_builder* b, uint32_t scene, agx_counts* out);
 int agx_builder_read_body_f32(agx_builder* b, uint32_t scene, uint32_t field, float* out, uint32_t n);
 int agx_builder_read_collider(agx_builder* b, uint32_t scene, uint32_t field, uint32_t as_u32, void* out, uint32_t n);
 int agx_builder_read_vertices(agx_builder* b, uint32_t scene, float* verts, float* normals, uint32_t n);
+/* ObjectAPI::createCircle/createBox/createPolygon/createCapsule after the upload (ObjectAPI.h:34-38,
+ * creation semantics as agx_builder_add_bodies, ObjectManager.cpp:71-389): the bodies are created
+ * in the world's copy of its scenes, with the reference's group moves, and the world is rebuilt
+ * from it with everything it carried (body state, collider poses and CCD origins, the warm-start
+ * manifolds and their cache, sleeping islands and their stashed contacts, collider callbacks,
+ * queued body commands, joint state, tuning, the learned warm start) moved to the new dense
+ * indices by creation slot. Existing bodies keep their state bit for bit. out_dense_index (may be
+ * NULL) receives each new body's world index at its creation (UINT32_MAX when rejected); the call
+ * returns the number of rejected bodies. Dense indices of existing bodies can change (a static body
+ * joins the static group, later scenes shift), as they do in the reference. Between steps only;
+ * the rebuild costs about what agx_world_create costs. */
+int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t* shape_type,
+                         const float* pos_x, const float* pos_y, const float* rotation,
+                         const float* vel_x, const float* vel_y, const float* ang_vel,
+                         const float* mass, const uint32_t* material, const float* p0,
+                         const float* p1, const uint32_t* vert_start, const uint32_t* vert_count,
+                         const float* verts, uint32_t* out_dense_index);
 /* ---- joints: ConstraintAPI::createSpring / createRevoluteJoint / createPrismaticJoint /
  * createSliderJoint / createMotor (SimulationAPIs/ConstraintAPI.h:31-35) with the CreateParams of
  * SimulationImpl/ConstraintSystems/(Spring, RevoluteJoint, ...)ConstraintSystem.h:14-26. Types follow ConstraintType

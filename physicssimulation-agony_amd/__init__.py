@@ -135,6 +135,7 @@ _SIGS = {
     "agx_builder_destroy": (C.c_int, [_P]),
     "agx_builder_add_material": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_uint32)]),
     "agx_builder_add_bodies": (C.c_int, [_P, C.c_uint32, C.c_uint32] + [_P] * 15),
+    "agx_world_add_bodies": (C.c_int, [_P, C.c_uint32, C.c_uint32] + [_P] * 15),
     "agx_builder_scene_counts": (C.c_int, [_P, C.c_uint32, C.POINTER(Counts)]),
     "agx_builder_read_body_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
     "agx_builder_read_collider": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
@@ -347,10 +348,13 @@ class World:
         self._h = C.c_void_p()
         s = settings if settings is not None else default_settings()
         _check(lib().agx_world_create(builder._h, C.byref(s), C.byref(self._h)))
+        self.counts = self._read_counts()
+        _track(self)
+
+    def _read_counts(self) -> Counts:
         c = Counts()
         _check(lib().agx_world_counts(self._h, C.byref(c)), self._h)
-        self.counts = c
-        _track(self)
+        return c
 
     def close(self):
         if getattr(self, "_h", None):
@@ -418,6 +422,21 @@ class World:
         cnt = C.c_uint64()
         _check(lib().agx_world_narrowphase(self._h, _ptr(a), _ptr(b), len(a), _ptr(out), len(out), C.byref(cnt)), self._h)
         return out[:cnt.value]
+
+    def create_bodies(self, scene: int, bodies: dict) -> np.ndarray:
+        """Creates bodies after the upload (the world is rebuilt around them); returns their world
+        dense indices at creation (UINT32_MAX for a rejected body)."""
+        b = normalise_bodies(bodies)
+        n = len(b["type"])
+        out = np.empty(max(n, 1), np.uint32)
+        rc = lib().agx_world_add_bodies(
+            self._h, scene, n, _ptr(b["type"]), _ptr(b["px"]), _ptr(b["py"]), _ptr(b["rot"]), _ptr(b["vx"]),
+            _ptr(b["vy"]), _ptr(b["av"]), _ptr(b["mass"]), _ptr(b["material"]), _ptr(b["p0"]), _ptr(b["p1"]),
+            _ptr(b["vstart"]), _ptr(b["vcount"]), _ptr(b["verts"]), _ptr(out))
+        if rc < 0:
+            _check(rc, self._h)
+        self.counts = self._read_counts()
+        return out[:n]
 
     def solve_contacts(self, manifolds, order=None) -> np.ndarray:
         """Solves the given manifolds against the current bodies (which it updates) in the given

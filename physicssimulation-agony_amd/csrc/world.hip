@@ -15,6 +15,7 @@
 #include <new>
 #include <string>
 #include <vector>
+#include <unordered_map>
 
 struct agx_builder {
     agx::Builder b;
@@ -71,6 +72,8 @@ struct agx_world {
     bool recordEvents = false;       // this call's steps record events
     bool staticDirty = false;        // a static body moved: static poses, AABBs and tree are rebuilt
     bool poseEdited = false;         // a body moved or turned since the last pose pass
+    // the scenes as built, kept for creation after the upload (agx_world_add_bodies rebuilds from it)
+    agx_builder* built = nullptr;
     // step graphs (agx_world_step_graph): one per ping-pong parity of the manifold buffers
     hipStream_t captureStream = nullptr;
     struct StepGraph {
@@ -1060,6 +1063,8 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
               agx::launchBuildTree(st, d, d.dyn) == hipSuccess &&
               hipStreamSynchronize(st) == hipSuccess;
     if (!ok) return bail(fail(nullptr, AGX_ERR_HIP, "world initialisation failed: %s", hipGetErrorString(hipGetLastError())));
+    w->built = new (std::nothrow) agx_builder(*bld);
+    if (!w->built) return bail(fail(nullptr, AGX_ERR_CAPACITY, "out of host memory"));
     *out = w;
     return AGX_OK;
 }
@@ -1072,6 +1077,7 @@ int agx_world_destroy(agx_world* w) {
     for (const EventPair& p : w->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : w->eventPool) (void)hipEventDestroy(e);
     for (void* p : w->allocations) (void)hipFree(p);
+    delete w->built;
     delete w;
     return AGX_OK;
 }
@@ -2072,10 +2078,10 @@ hipError_t up(T* p, const std::vector<T>& v) {
     return v.empty() ? hipSuccess : hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
-int writeSnapshot(agx_world* w, SnapWriter& sw) {
+// The state a snapshot holds, copied from the device.
+int gatherState(agx_world* w, SnapArrays& a) {
     agx::DeviceWorld& d = w->d;
     HIP_CHECK(w, hipStreamSynchronize(w->stream));
-    SnapArrays a;
     HIP_CHECK(w, hipMemcpy(&a.ctr, d.ctr, sizeof a.ctr, hipMemcpyDeviceToHost));
     HIP_CHECK(w, down(a.pos, d.pos, d.N)); HIP_CHECK(w, down(a.vel, d.vel, d.N)); HIP_CHECK(w, down(a.rcs, d.rcs, d.N));
     HIP_CHECK(w, down(a.inv, d.inv, d.N)); HIP_CHECK(w, down(a.island, d.islandId, d.N));
@@ -2090,6 +2096,13 @@ int writeSnapshot(agx_world* w, SnapWriter& sw) {
         HIP_CHECK(w, down(a.sleepId, d.sleepId, d.N)); HIP_CHECK(w, down(a.sleepTimer, d.sleepTimer, d.N));
         HIP_CHECK(w, down(a.stash, d.stash, a.ctr.stashCount)); HIP_CHECK(w, down(a.stashIsland, d.stashIsland, a.ctr.stashCount));
     }
+    return AGX_OK;
+}
+
+int writeSnapshot(agx_world* w, SnapWriter& sw) {
+    agx::DeviceWorld& d = w->d;
+    SnapArrays a;
+    if (int rc = gatherState(w, a)) return rc;
     SnapHeader h{};
     std::memcpy(h.magic, SNAP_MAGIC, sizeof h.magic);
     h.version = SNAP_VERSION;
@@ -2274,6 +2287,162 @@ int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
     w->histReady = false;
     w->nextPhase = 0;
     return AGX_OK;
+}
+
+// ObjectAPI::create* after the upload: the bodies are created in the world's copy of its builder
+// (the reference's group moves included), the world is rebuilt from it, and everything the old
+// world carried moves to the new dense indices by creation slot.
+int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t* type, const float* px, const float* py,
+                         const float* rot, const float* vx, const float* vy, const float* av, const float* mass,
+                         const uint32_t* material, const float* p0, const float* p1, const uint32_t* vstart,
+                         const uint32_t* vcount, const float* verts, uint32_t* out_index) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    agx::DeviceWorld& d = w->d;
+    if (!w->built || scene >= d.S) return fail(w, AGX_ERR_INVALID_ARGUMENT, "scene %u out of range", scene);
+    SnapArrays a;
+    if (int rc = gatherState(w, a)) return rc;
+    agx::Builder& B = w->built->b;
+    // the old world's bodies and colliders by (scene, creation slot), and the builder brought to
+    // the world's current body state
+    std::vector<uint64_t> oldBody(d.N), oldCol(d.C);
+    {
+        uint32_t g = 0, c = 0;
+        for (uint32_t s = 0; s < d.S; s++) {
+            agx::SceneData& sd = B.scenes[s];
+            for (uint32_t i = 0; i < sd.bodyCount(); i++, g++) {
+                oldBody[g] = ((uint64_t)s << 32) | sd.bodySlot[i];
+                sd.comX[i] = a.pos[g].x; sd.comY[i] = a.pos[g].y; sd.rot[i] = a.pos[g].z;
+                sd.velX[i] = a.vel[g].x; sd.velY[i] = a.vel[g].y; sd.angVel[i] = a.vel[g].z;
+                sd.cosv[i] = a.rcs[g].x; sd.sinv[i] = a.rcs[g].y;
+                sd.invMass[i] = a.inv[g].x; sd.invInertia[i] = a.inv[g].y;
+                sd.mass[i] = w->mass[g]; sd.inertia[i] = w->inertia[g];
+            }
+            for (uint32_t k = 0; k < sd.colliderCount(); k++, c++) oldCol[c] = ((uint64_t)s << 32) | sd.colSlot[k];
+        }
+    }
+    const agx::SceneData backup = B.scenes[scene];
+    std::vector<uint32_t> local(n, UINT32_MAX);
+    const int rejected = agx_builder_add_bodies(w->built, scene, n, type, px, py, rot, vx, vy, av, mass, material, p0, p1,
+                                                vstart, vcount, verts, local.data());
+    if (rejected < 0) { B.scenes[scene] = backup; return fail(w, rejected, "%s", g_lastError.c_str()); }
+    const std::string rejectMsg = rejected ? g_lastError : std::string();
+    agx_world* nw = nullptr;
+    if (int rc = agx_world_create(w->built, &w->settings, &nw)) {
+        B.scenes[scene] = backup;
+        return fail(w, rc, "rebuild failed: %s", g_lastError.c_str());
+    }
+    agx::DeviceWorld& e = nw->d;
+    // new dense index of every old body and collider
+    std::unordered_map<uint64_t, uint32_t> bodyAt, colAt;
+    std::vector<uint32_t> sceneStart(e.S + 1, 0);
+    {
+        uint32_t g = 0, c = 0;
+        for (uint32_t s = 0; s < e.S; s++) {
+            const agx::SceneData& sd = B.scenes[s];
+            sceneStart[s] = g;
+            for (uint32_t i = 0; i < sd.bodyCount(); i++, g++) bodyAt[((uint64_t)s << 32) | sd.bodySlot[i]] = g;
+            for (uint32_t k = 0; k < sd.colliderCount(); k++, c++) colAt[((uint64_t)s << 32) | sd.colSlot[k]] = c;
+        }
+        sceneStart[e.S] = g;
+    }
+    std::vector<uint32_t> nb(d.N), nc(d.C);
+    for (uint32_t g = 0; g < d.N; g++) nb[g] = bodyAt.at(oldBody[g]);
+    for (uint32_t c = 0; c < d.C; c++) nc[c] = colAt.at(oldCol[c]);
+    SnapArrays b;
+    if (int rc = gatherState(nw, b)) { agx_world_destroy(nw); return fail(w, rc, "rebuild: reading the new world failed"); }
+    for (uint32_t g = 0; g < d.N; g++) {
+        const uint32_t k = nb[g];
+        b.pos[k] = a.pos[g]; b.vel[k] = a.vel[g]; b.rcs[k] = a.rcs[g]; b.inv[k] = a.inv[g];
+        b.island[k] = a.island[g]; b.hit[k] = a.hit[g];
+    }
+    for (uint32_t c = 0; c < d.C; c++) {
+        const uint32_t k = nc[c];
+        b.pose[k] = a.pose[c]; b.wRot[k] = a.wRot[c]; b.prevW[k] = a.prevW[c]; b.aabb[k] = a.aabb[c];
+        b.callback[k] = a.callback[c]; b.threshold[k] = a.threshold[c];
+    }
+    b.man = a.man;   // the warm-start source follows its colliders
+    for (agx_manifold& m : b.man) { m.a = nc[m.a]; m.b = nc[m.b]; }
+    b.sceneSweeps = a.sceneSweeps;
+    b.motorAcc = a.motorAcc;
+    const bool slept = !a.sleepId.empty();
+    if (slept) {   // an island keeps sleeping; its id becomes 2 + its smallest new body index
+        if (int rc = allocSleep(nw)) { agx_world_destroy(nw); return fail(w, rc, "rebuild: sleeping state"); }
+        std::unordered_map<uint32_t, uint32_t> idOf;
+        for (uint32_t g = 0; g < d.N; g++)
+            if (a.sleepId[g]) {
+                auto it = idOf.find(a.sleepId[g]);
+                const uint32_t id = 2 + nb[g];
+                if (it == idOf.end()) idOf[a.sleepId[g]] = id; else it->second = std::min(it->second, id);
+            }
+        b.sleepId.assign(e.N, 0u);
+        b.sleepTimer.assign(e.N, 0.0f);
+        for (uint32_t g = 0; g < d.N; g++) {
+            b.sleepId[nb[g]] = a.sleepId[g] ? idOf.at(a.sleepId[g]) : 0u;
+            b.sleepTimer[nb[g]] = a.sleepTimer[g];
+        }
+        b.stash = a.stash;
+        for (agx_manifold& m : b.stash) { m.a = nc[m.a]; m.b = nc[m.b]; }
+        b.stashIsland = a.stashIsland;
+        for (uint32_t& id : b.stashIsland) id = idOf.at(id);
+    }
+    HIP_CHECK(nw, hipStreamSynchronize(nw->stream));
+    int rc = AGX_OK;
+    auto upAll = [&]() -> int {
+        HIP_CHECK(nw, up(e.pos, b.pos)); HIP_CHECK(nw, up(e.vel, b.vel)); HIP_CHECK(nw, up(e.rcs, b.rcs));
+        HIP_CHECK(nw, up(e.inv, b.inv)); HIP_CHECK(nw, up(e.islandId, b.island)); HIP_CHECK(nw, up(e.hitTime, b.hit));
+        HIP_CHECK(nw, up(e.pose, b.pose)); HIP_CHECK(nw, up(e.wRot, b.wRot)); HIP_CHECK(nw, up(e.prevW, b.prevW));
+        HIP_CHECK(nw, up(e.aabb, b.aabb)); HIP_CHECK(nw, up(e.colCallback, b.callback));
+        HIP_CHECK(nw, up(e.colThreshold, b.threshold)); HIP_CHECK(nw, up(e.sceneSweeps, b.sceneSweeps));
+        HIP_CHECK(nw, up(e.motorAcc, b.motorAcc));
+        if (slept) {
+            HIP_CHECK(nw, up(e.sleepId, b.sleepId)); HIP_CHECK(nw, up(e.sleepTimer, b.sleepTimer));
+            HIP_CHECK(nw, up(e.stash, b.stash)); HIP_CHECK(nw, up(e.stashIsland, b.stashIsland));
+        }
+        // the warm-start source and its hash, built the way a step's FINISH builds them; then the
+        // counters that last across steps
+        HIP_CHECK(nw, up(e.man[1 - e.cur], b.man));
+        agx::Counters c = b.ctr;
+        c.manifolds = (uint32_t)b.man.size();
+        c.gate = a.ctr.gate;
+        c.solveSteps = a.ctr.solveSteps; c.sweepsTotal = a.ctr.sweepsTotal;
+        c.bodySweepsTotal = a.ctr.bodySweepsTotal; c.convergedBodiesTotal = a.ctr.convergedBodiesTotal;
+        c.stashCount = slept ? a.ctr.stashCount : 0u;
+        HIP_CHECK(nw, hipMemcpy(e.ctr, &c, sizeof c, hipMemcpyHostToDevice));
+        e.cur ^= 1;
+        HIP_CHECK(nw, agx::launchHashBuild(nw->stream, e));
+        e.cur ^= 1;
+        HIP_CHECK(nw, hipStreamSynchronize(nw->stream));
+        return AGX_OK;
+    };
+    if ((rc = upAll())) { agx_world_destroy(nw); return fail(w, rc, "rebuild: %s", g_lastError.c_str()); }
+    nw->accumulator = w->accumulator;
+    nw->lastStepCount = w->lastStepCount;
+    nw->postUpdatePending = w->postUpdatePending;
+    nw->poseEdited = w->poseEdited;
+    nw->staticDirty = true;   // static poses, AABBs and tree follow the bodies at the next pose pass
+    nw->profiling = w->profiling;
+    if (w->mlp) {   // the learned warm start's buffers, then the gate as it was
+        if ((rc = agx_world_set_learned(nw, w->mlp, w->learnMode)) ||
+            hipMemcpy(&nw->d.ctr->gate, &a.ctr.gate, sizeof a.ctr.gate, hipMemcpyHostToDevice) != hipSuccess) {
+            agx_world_destroy(nw);
+            return fail(w, rc ? rc : AGX_ERR_HIP, "rebuild: learned warm start");
+        }
+    }
+    nw->commands = w->commands;
+    for (agx_world::BodyCommand& c : nw->commands) c.body = nb[c.body];
+    nw->colCallbackHost = b.callback;
+    nw->callbackColliders = w->callbackColliders;
+    if (w->d.solveBlocks / w->d.numCU <= nw->solvePerCuMax && w->d.colourBlocks / w->d.numCU <= nw->colourPerCuMax) {
+        e.solveBlocks = w->d.solveBlocks;
+        e.colourBlocks = w->d.colourBlocks;
+    }
+    std::swap(*w, *nw);
+    agx_world_destroy(nw);
+    if (out_index)
+        for (uint32_t i = 0; i < n; i++) out_index[i] = local[i] == UINT32_MAX ? UINT32_MAX : sceneStart[scene] + local[i];
+    if (rejected) fail(w, 0, "%s", rejectMsg.c_str());
+    return rejected;
 }
 
 }  // extern "C"

@@ -17,8 +17,9 @@
 // reference's ObjectManager (ObjectManager.cpp:92-97).
 //
 // Differences a caller must know: the scene is uploaded to the GPU at the first update() (or an
-// explicit upload()); objects are created before that point, and creation afterwards returns
-// std::nullopt. Read-outs copy device state to host vectors (the reference returns non-owning views).
+// explicit upload()). Bodies created afterwards rebuild the world around them (a cost like the
+// upload); materials and constraints are created before the upload (afterwards they return
+// std::nullopt). Read-outs copy device state to host vectors (the reference returns non-owning views).
 #pragma once
 
 #include <cmath>
@@ -409,17 +410,22 @@ inline std::optional<uint32_t> ConstraintAPI::add(const agx_joint_desc& d) {
 
 inline std::optional<ObjectDenseIndex> ObjectAPI::add(uint8_t type, const BodyCreateParams& b, float p0, float p1,
                                                       const std::vector<Vec2>* verts) {
-    if (sim.world) { Simulation::fail("create", "the scene is already on the GPU"); return std::nullopt; }
     const uint32_t vcount = verts ? (uint32_t)verts->size() : 0u, vstart = 0;
     std::vector<float> flat;
     if (verts) for (const Vec2& v : *verts) { flat.push_back(v.x); flat.push_back(v.y); }
     uint32_t out = UINT32_MAX;
-    const int rc = agx_builder_add_bodies(sim.builder, 0, 1, &type, &b.position.x, &b.position.y, &b.rotation,
-                                          &b.velocity.x, &b.velocity.y, &b.angularVelocity, &b.mass,
-                                          &b.materialIndex, &p0, &p1, &vstart, &vcount,
-                                          flat.empty() ? nullptr : flat.data(), &out);
+    // before the upload into the builder; afterwards into the world, which is rebuilt around the
+    // new body with everything it carried (agx_world_add_bodies)
+    const int rc = sim.world
+        ? agx_world_add_bodies(sim.world, 0, 1, &type, &b.position.x, &b.position.y, &b.rotation, &b.velocity.x,
+                               &b.velocity.y, &b.angularVelocity, &b.mass, &b.materialIndex, &p0, &p1, &vstart, &vcount,
+                               flat.empty() ? nullptr : flat.data(), &out)
+        : agx_builder_add_bodies(sim.builder, 0, 1, &type, &b.position.x, &b.position.y, &b.rotation, &b.velocity.x,
+                                 &b.velocity.y, &b.angularVelocity, &b.mass, &b.materialIndex, &p0, &p1, &vstart, &vcount,
+                                 flat.empty() ? nullptr : flat.data(), &out);
     if (rc < 0 || out == UINT32_MAX) {
-        Simulation::fail("create", rc < 0 ? agx_last_error() : "invalid or non-convex polygon");
+        Simulation::fail("create", rc < 0 ? (sim.world ? agx_world_last_error(sim.world) : agx_last_error())
+                                          : "invalid or non-convex polygon");
         return std::nullopt;
     }
     return out;

@@ -97,7 +97,18 @@ int main() {
     const auto heavyData = objects.getBodyData(heavy);
     const bool applied = liftedData && liftedData->centerOfMass.y > 16.0f && liftedData->angularVelocity != 0.0f &&
                          heavyData && heavyData->mass == 2.0f && heavyData->invMass == 0.5f;
+    // a body created after the upload: the world is rebuilt around it and it falls into the box
+    const uint32_t before = sim.counts().bodies;
+    CircleCreateParams lateParams;
+    lateParams.base.position = { 0.5f, 19.0f };
+    lateParams.base.mass = 1.0f;
+    lateParams.radius = 0.4f;
+    const auto late = objects.createCircle(lateParams);
+    const bool lateMade = late && *late == before && sim.counts().bodies == before + 1 &&
+                          !sim.createMaterial({ 0.0f, 0.5f, 0.5f }).has_value();   // materials only before the upload
     for (int frame = 0; frame < 60; frame++) steps += sim.update(1.0f / 60.0f);
+    const auto lateData = late ? objects.getBodyData(*late) : std::nullopt;
+    const bool lateFell = lateData && lateData->centerOfMass.y < 18.0f;
     const ColliderArrays cols = objects.getColliders();
     const PolygonArrays polys = objects.getPolygons();
     uint32_t polygonColliders = 0, capsuleColliders = 0;
@@ -109,7 +120,8 @@ int main() {
                        cols.aabbMinY[c] <= cols.worldY[c] && cols.worldY[c] <= cols.aabbMaxY[c];
     }
     const bool viewers = polygonColliders == 64 && capsules > 0 && capsuleColliders == (uint32_t)capsules &&
-                         polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 && aabbsContain;
+                         polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 && aabbsContain &&
+                         cols.worldX.size() == (size_t)before + 1;
     const auto manifolds = sim.getBodyCollisionData();
     const auto [ids, islands] = sim.islands();
     const BodyArrays bodies = objects.getBodies();
@@ -120,11 +132,12 @@ int main() {
     }
     const agx_stats st = sim.getStats();
     std::printf("bodies %d (capsules %d) steps %u manifolds %zu islands %u colours %u overflow %u rejected %d setters %d/%d/%d viewers %d"
-                " events %u (max impulse %.3f)\n",
+                " events %u (max impulse %.3f) created after upload %d/%d\n",
                 made, capsules, steps, manifolds.size(), islands, st.colours, st.overflow, rejected ? 1 : 0, queuedEarly ? 1 : 0,
-                queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0, tally.events, tally.maxImpulse);
+                queued ? 1 : 0, applied ? 1 : 0, viewers ? 1 : 0, tally.events, tally.maxImpulse, lateMade ? 1 : 0,
+                lateFell ? 1 : 0);
     const bool ok = made == 192 && steps == 241 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
                     st.overflow == 0 && queuedEarly && queued && applied && viewers && callbacksSet && tally.events > 0 &&
-                    tally.badNormals == 0 && tally.maxImpulse > 0.0f && jointsMade;
+                    tally.badNormals == 0 && tally.maxImpulse > 0.0f && jointsMade && lateMade && lateFell;
     return ok ? 0 : 1;
 }

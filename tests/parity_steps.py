@@ -135,3 +135,16 @@ def check_step(agx, oracle, scene, w, dt, v_it=8, p_it=3, oracle_settings=None, 
     assert abs(st.velocity_error - ev) <= 1e-4 * abs(ev) + 1e-6, (st.velocity_error, ev)
     assert abs(st.position_error - ep) <= 1e-4 * abs(ep) + 1e-6, (st.position_error, ep)
     return dict(pairs=len(ga), manifolds=len(mg), two_point=two_point, colours=len(starts) - 1, islands=k_g)
+
+
+def extend_scene(scene, extra):
+    """The scene with bodies appended in creation order (non-polygon entries of the scene format)."""
+    out = dict(scene)
+    bodies = dict(scene["bodies"])
+    n = len(extra["type"])
+    for k in ("type", "px", "py", "rot", "vx", "vy", "av", "mass", "material", "p0", "p1"):
+        bodies[k] = np.concatenate([bodies[k], np.asarray(extra[k], bodies[k].dtype)])
+    bodies["vstart"] = np.concatenate([bodies["vstart"], np.zeros(n, np.uint32)])
+    bodies["vcount"] = np.concatenate([bodies["vcount"], np.zeros(n, np.uint32)])
+    out["bodies"] = bodies
+    return out

@@ -159,3 +159,34 @@ def test_graph_steps_and_snapshot_keep_sleeping_state(agx, scenes):
     c.step(DT, 150)
     assert c.body("com_x").tobytes() == a.body("com_x").tobytes()
     assert c.sleep_state()[0].tobytes() == a.sleep_state()[0].tobytes()
+
+
+def test_creation_after_upload_keeps_sleeping_islands(agx, oracle, scenes):
+    """A body created while islands sleep: the rebuilt world keeps them asleep (ids 2 + their smallest
+    new body index, timers, stashed contacts moved with their colliders), and the following steps
+    sleep and wake like the oracle built from the extended scene, phase by phase."""
+    from parity_steps import extend_scene
+    scene = scenes.sleepy_stacks()
+    w = make_world(agx, scene)
+    for _ in range(40):
+        w.step(DT, 10)
+        if np.count_nonzero(w.sleep_state()[0]) > 6:
+            break
+    ids0, t0 = w.sleep_state()
+    stash0, _ = w.stash()
+    assert np.count_nonzero(ids0) > 6 and len(stash0) > 0
+    n = w.counts.bodies
+    extra = {"type": np.array([agx.SHAPE_CIRCLE], np.uint8), "px": np.array([0.0], np.float32),
+             "py": np.array([6.0], np.float32), "rot": np.zeros(1, np.float32), "vx": np.zeros(1, np.float32),
+             "vy": np.zeros(1, np.float32), "av": np.zeros(1, np.float32), "mass": np.array([1.0], np.float32),
+             "material": np.ones(1, np.uint32), "p0": np.array([0.4], np.float32), "p1": np.zeros(1, np.float32)}
+    idx = w.create_bodies(0, extra)
+    assert int(idx[0]) == n   # a dynamic body joins the end of the dynamic group: nothing moves
+    ids1, t1 = w.sleep_state()
+    assert np.array_equal(ids1[:n], ids0) and np.array_equal(t1[:n], t0) and ids1[n] == 0
+    for v in np.unique(ids1[ids1 != 0]):
+        assert v == 2 + np.flatnonzero(ids1 == v).min()
+    assert w.stash()[0].tobytes() == stash0.tobytes()
+    ext = extend_scene(scene, extra)
+    for _ in range(90):
+        sleep_step(agx, oracle, ext, w)
