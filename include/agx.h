@@ -184,6 +184,20 @@ int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uin
                            const float* mass, const uint32_t* material, const float* p0,
                            const float* p1, const uint32_t* vert_start, const uint32_t* vert_count,
                            const float* verts, uint32_t* out_dense_index);
+/* ObjectAPI::createBody + createCircleCollider / createBoxCollider / createCapsuleCollider
+ * (ObjectAPI.h:28-32, ObjectManager.cpp:71-181): bodies with several colliders. A body is created
+ * without colliders, with its mass, inertia and local centre of mass as given (com_x/com_y, in the
+ * body frame; its position is the body origin); colliders are then attached at a local offset and
+ * rotation, into their owner's group, and leave the body's mass properties as they are. Colliders
+ * of one body never collide with each other. shape_type: AGX_SHAPE_CIRCLE (p0 radius),
+ * AGX_SHAPE_BOX (p0 width, p1 height) or AGX_SHAPE_CAPSULE (p0 core length, p1 radius); polygon
+ * outlines are created with their body (agx_builder_add_bodies). */
+int agx_builder_create_body(agx_builder* b, uint32_t scene, float pos_x, float pos_y, float rotation, float vel_x,
+                            float vel_y, float ang_vel, float mass, float inertia, float com_x, float com_y,
+                            uint32_t* out_dense_index);
+int agx_builder_add_collider(agx_builder* b, uint32_t scene, uint32_t body, uint8_t shape_type, float offset_x,
+                             float offset_y, float local_rotation, uint32_t material, float p0, float p1,
+                             uint32_t* out_collider_index);
 /* Host-side views of a built scene before upload (ObjectAPI viewers, ObjectAPI.h:96-101):
  * dense-order body and collider fields of one scene, and its polygon vertex/normal pool. */
 int agx_builder_scene_counts(agx_builder* b, uint32_t scene, agx_counts* out);

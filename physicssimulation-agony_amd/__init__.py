@@ -135,6 +135,9 @@ _SIGS = {
     "agx_builder_destroy": (C.c_int, [_P]),
     "agx_builder_add_material": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_uint32)]),
     "agx_builder_add_bodies": (C.c_int, [_P, C.c_uint32, C.c_uint32] + [_P] * 15),
+    "agx_builder_create_body": (C.c_int, [_P, C.c_uint32] + [C.c_float] * 10 + [_P]),
+    "agx_builder_add_collider": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint8] + [C.c_float] * 3 + [C.c_uint32]
+                                 + [C.c_float] * 2 + [_P]),
     "agx_world_add_bodies": (C.c_int, [_P, C.c_uint32, C.c_uint32] + [_P] * 15),
     "agx_builder_scene_counts": (C.c_int, [_P, C.c_uint32, C.POINTER(Counts)]),
     "agx_builder_read_body_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
@@ -291,6 +294,22 @@ class Builder:
             _check(rc)
         self.rejected = rc
         return out
+
+    def create_body(self, scene: int, x: float, y: float, mass: float, inertia: float, rot: float = 0.0,
+                    vel=(0.0, 0.0), ang_vel: float = 0.0, com=(0.0, 0.0)) -> int:
+        """A body without colliders (ObjectAPI::createBody): mass properties as given."""
+        out = C.c_uint32()
+        _check(lib().agx_builder_create_body(self._h, scene, x, y, rot, vel[0], vel[1], ang_vel, mass, inertia, com[0],
+                                             com[1], C.byref(out)))
+        return out.value
+
+    def add_collider(self, scene: int, body: int, shape: int, p0: float, p1: float = 0.0, offset=(0.0, 0.0),
+                     local_rot: float = 0.0, material: int = 0) -> int:
+        """A circle, box or capsule collider on a body (ObjectAPI::create*Collider); its dense index."""
+        out = C.c_uint32()
+        _check(lib().agx_builder_add_collider(self._h, scene, body, shape, offset[0], offset[1], local_rot, material, p0, p1,
+                                              C.byref(out)))
+        return out.value
 
     def add_joints(self, scene: int, joints) -> np.ndarray:
         """Joints (a JOINT_DTYPE array or a list of dicts) between bodies of a scene."""

@@ -168,15 +168,43 @@ uint32_t SceneData::appendCollider(uint32_t body, Vec off, float lr, uint32_t ma
 }
 
 // ObjectManager.cpp:1103-1108: bound radius, then into the owner's group.
-void SceneData::placeCollider(uint32_t c, float minExt, float maxExt) {
+uint32_t SceneData::placeCollider(uint32_t c, float minExt, float maxExt) {
     minExtent[c] = minExt;
     maxExtent[c] = maxExt;
     const uint32_t body = colBody[c];
     const float x = comOffX[c], y = comOffY[c];
     boundRadius[body] = std::fmax(boundRadius[body], std::sqrt(x * x + y * y) + maxExt);
-    if (body >= staticBodies) return;
+    if (body >= staticBodies) return c;
     const uint32_t dst = staticColliders++;
     swapColliders(c, dst);
+    return dst;
+}
+
+uint32_t SceneData::createBody(float px, float py, float r, float vx, float vy, float av, float massIn, float inertiaIn,
+                               float comXIn, float comYIn) {
+    const float m = std::fmax(0.0f, massIn), in = std::fmax(0.0f, inertiaIn);
+    return appendBody({ px, py }, { vx, vy }, r, av, m, m == 0.0f ? 0.0f : 1.0f / m, in, in == 0.0f ? 0.0f : 1.0f / in,
+                      { comXIn, comYIn });
+}
+
+uint32_t SceneData::createCollider(uint32_t body, uint8_t type, float offX, float offY, float localRotIn,
+                                   uint32_t material, float a0, float a1) {
+    if (body >= bodyCount()) return UINT32_MAX;
+    const Vec off{ offX, offY };
+    if (type == 0) {   // circle: radius
+        const float radius = std::fmax(0.0f, a0);
+        return placeCollider(appendCollider(body, off, localRotIn, material, 0, radius, 0.0f, 0, 0), radius, radius);
+    }
+    if (type == 1) {   // box: width, height
+        const float hw = std::fmax(0.0f, a0) * 0.5f, hh = std::fmax(0.0f, a1) * 0.5f;
+        return placeCollider(appendCollider(body, off, localRotIn, material, 1, hw, hh, 0, 0), std::fmin(hw, hh),
+                             std::fmax(hw, hh));
+    }
+    if (type == 3) {   // capsule: full core length, radius (axis local +x)
+        const float half = std::fmax(0.0f, a0) * 0.5f, rad = std::fmax(0.0f, a1);
+        return placeCollider(appendCollider(body, off, localRotIn, material, 3, half, rad, 0, 0), rad, half + rad);
+    }
+    return UINT32_MAX;
 }
 
 uint32_t SceneData::createCircle(float px, float py, float r, float vx, float vy, float av, float massIn,

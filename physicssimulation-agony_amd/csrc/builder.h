@@ -44,6 +44,15 @@ struct SceneData {
                            uint32_t material, const Vec* verts, uint32_t count);
     uint32_t createCapsule(float px, float py, float rot, float vx, float vy, float av, float mass,
                            uint32_t material, float length, float radius);
+    // ObjectManager::createBody (ObjectManager.cpp:71-88): a body without colliders, its mass,
+    // inertia and local centre of mass as given (negative values become 0).
+    uint32_t createBody(float px, float py, float rot, float vx, float vy, float av, float mass, float inertia,
+                        float comX, float comY);
+    // ObjectManager::create{Circle,Box,Capsule}Collider (ObjectManager.cpp:90-181): a collider on dense
+    // body `body` at local offset and rotation, into its owner's group; the body's mass properties
+    // stay. Returns the collider's dense index or UINT32_MAX (no such body, unknown shape).
+    uint32_t createCollider(uint32_t body, uint8_t type, float offX, float offY, float localRot, uint32_t material,
+                            float p0, float p1);
     uint32_t materialCount = 1;   // set by the builder owner so out-of-range materials map to 0
     std::vector<JointSpec> joints;
     // Appends a joint between dense bodies a and b (b ignored for a motor, type 4); false when a
@@ -55,7 +64,7 @@ private:
                         float invInertia, Vec localCom);
     uint32_t appendCollider(uint32_t body, Vec off, float localRot, uint32_t material, uint8_t type, float p0,
                             float p1, uint32_t ps, uint32_t pc);
-    void placeCollider(uint32_t c, float minExt, float maxExt);
+    uint32_t placeCollider(uint32_t c, float minExt, float maxExt);   // returns the collider's index after the move
     void swapBodies(uint32_t a, uint32_t b);
     void swapColliders(uint32_t a, uint32_t b);
 };

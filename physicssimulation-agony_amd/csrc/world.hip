@@ -739,6 +739,27 @@ int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uin
     return rejected;
 }
 
+int agx_builder_create_body(agx_builder* b, uint32_t scene, float px, float py, float rot, float vx, float vy, float av,
+                            float mass, float inertia, float comX, float comY, uint32_t* out_index) {
+    if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    const uint32_t r = b->b.scenes[scene].createBody(px, py, rot, vx, vy, av, mass, inertia, comX, comY);
+    if (out_index) *out_index = r;
+    return AGX_OK;
+}
+
+int agx_builder_add_collider(agx_builder* b, uint32_t scene, uint32_t body, uint8_t type, float offX, float offY,
+                             float localRot, uint32_t material, float p0, float p1, uint32_t* out_index) {
+    if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    agx::SceneData& sd = b->b.scenes[scene];
+    sd.materialCount = (uint32_t)b->b.materials.size();
+    if (body >= sd.bodyCount()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "no body %u in scene %u", body, scene);
+    if (type != AGX_SHAPE_CIRCLE && type != AGX_SHAPE_BOX && type != AGX_SHAPE_CAPSULE)
+        return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "collider shape %u: circle, box or capsule", (unsigned)type);
+    const uint32_t c = sd.createCollider(body, type, offX, offY, localRot, material, p0, p1);
+    if (out_index) *out_index = c;
+    return AGX_OK;
+}
+
 int agx_builder_add_joints(agx_builder* b, uint32_t scene, uint32_t n, const agx_joint_desc* joints, uint32_t* out_index) {
     if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
     if (n && !joints) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "null joint array");

@@ -62,6 +62,21 @@ struct BodyCreateParams {
     Real inertia = 0;
     MaterialIndex materialIndex = 0;
 };
+// ObjectCreateParams.h: a body on its own (createBody: mass, inertia and local centre of mass as
+// given) and colliders attached to it at a local offset and rotation (create*Collider).
+struct BodyOnlyCreateParams {
+    Vec2 position{};
+    Vec2 velocity{};
+    Real rotation = 0;
+    Real angularVelocity = 0;
+    Real mass = 0;
+    Real inertia = 0;
+    Vec2 centerOfMass{};
+};
+struct ColliderCreateParams { ObjectDenseIndex body = 0; Vec2 localOffset{}; Real localRotation = 0; MaterialIndex materialIndex = 0; };
+struct CircleColliderCreateParams { ColliderCreateParams base; Real radius = 0; };
+struct BoxColliderCreateParams { ColliderCreateParams base; Vec2 size{}; };
+struct CapsuleColliderCreateParams { ColliderCreateParams base; Real length = 0; Real radius = 0; };
 struct CircleCreateParams { BodyCreateParams base; Real radius = 0; };
 struct BoxCreateParams { BodyCreateParams base; Vec2 size{}; };
 struct PolygonCreateParams { BodyCreateParams base; std::vector<Vec2> localVertices; };
@@ -183,6 +198,18 @@ public:
     std::optional<ObjectDenseIndex> createBox(const BoxCreateParams& p);
     std::optional<ObjectDenseIndex> createPolygon(const PolygonCreateParams& p);
     std::optional<ObjectDenseIndex> createCapsule(const CapsuleCreateParams& p);
+    // ObjectAPI::createBody / createCircleCollider / createBoxCollider / createCapsuleCollider
+    // (ObjectAPI.h:28-32): before the upload; colliders of one body never collide with each other
+    std::optional<ObjectDenseIndex> createBody(const BodyOnlyCreateParams& p);
+    std::optional<ObjectDenseIndex> createCircleCollider(const CircleColliderCreateParams& p) {
+        return collider(p.base, AGX_SHAPE_CIRCLE, p.radius, 0.0f);
+    }
+    std::optional<ObjectDenseIndex> createBoxCollider(const BoxColliderCreateParams& p) {
+        return collider(p.base, AGX_SHAPE_BOX, p.size.x, p.size.y);
+    }
+    std::optional<ObjectDenseIndex> createCapsuleCollider(const CapsuleColliderCreateParams& p) {
+        return collider(p.base, AGX_SHAPE_CAPSULE, p.length, p.radius);
+    }
     std::optional<Handle> createCircleH(const CircleCreateParams& p) { return createCircle(p); }
     std::optional<Handle> createBoxH(const BoxCreateParams& p) { return createBox(p); }
     std::optional<Handle> createPolygonH(const PolygonCreateParams& p) { return createPolygon(p); }
@@ -207,6 +234,7 @@ private:
     struct Command { uint32_t kind; Handle body; Real a, b; };
     bool queue(uint32_t kind, Handle body, Real a, Real b);
     std::vector<Command> pending;   // commands given before the scene is on the GPU
+    std::optional<ObjectDenseIndex> collider(const ColliderCreateParams& c, uint8_t type, float p0, float p1);
     std::optional<ObjectDenseIndex> add(uint8_t type, const BodyCreateParams& b, float p0, float p1,
                                         const std::vector<Vec2>* verts);
     Simulation& sim;
@@ -426,6 +454,26 @@ inline std::optional<ObjectDenseIndex> ObjectAPI::add(uint8_t type, const BodyCr
     if (rc < 0 || out == UINT32_MAX) {
         Simulation::fail("create", rc < 0 ? (sim.world ? agx_world_last_error(sim.world) : agx_last_error())
                                           : "invalid or non-convex polygon");
+        return std::nullopt;
+    }
+    return out;
+}
+
+inline std::optional<ObjectDenseIndex> ObjectAPI::createBody(const BodyOnlyCreateParams& p) {
+    if (sim.world) { Simulation::fail("createBody", "the scene is already on the GPU"); return std::nullopt; }
+    uint32_t out = UINT32_MAX;
+    if (agx_builder_create_body(sim.builder, 0, p.position.x, p.position.y, p.rotation, p.velocity.x, p.velocity.y,
+                                p.angularVelocity, p.mass, p.inertia, p.centerOfMass.x, p.centerOfMass.y, &out) != AGX_OK)
+        return std::nullopt;
+    return out;
+}
+
+inline std::optional<ObjectDenseIndex> ObjectAPI::collider(const ColliderCreateParams& c, uint8_t type, float p0, float p1) {
+    if (sim.world) { Simulation::fail("createCollider", "the scene is already on the GPU"); return std::nullopt; }
+    uint32_t out = UINT32_MAX;
+    if (agx_builder_add_collider(sim.builder, 0, c.body, type, c.localOffset.x, c.localOffset.y, c.localRotation,
+                                 c.materialIndex, p0, p1, &out) != AGX_OK) {
+        Simulation::fail("createCollider", agx_last_error());
         return std::nullopt;
     }
     return out;

@@ -84,6 +84,11 @@ int main() {
                             joints.createSpring({ 151, 152, { 0, 0 }, { 0, 0 }, 1.2f, 30.0f, 0.5f }).has_value() &&
                             !joints.createMotor({ 100000, 1.0f, 1.0f }).has_value();   // no such body
 
+    // a body with two colliders (ObjectAPI::createBody + createCircleCollider / createBoxCollider)
+    const auto dumbbell = objects.createBody({ { -6.0f, 16.0f }, {}, 0.0f, 0.0f, 2.0f, 1.2f, {} });
+    const bool multiMade = dumbbell && objects.createCircleCollider({ { *dumbbell, { 0.8f, 0.0f } }, 0.4f }).has_value() &&
+                           objects.createBoxCollider({ { *dumbbell, { -0.8f, 0.0f } }, { 0.6f, 0.6f } }).has_value() &&
+                           !objects.createCircleCollider({ { 1000000u, {} }, 0.4f }).has_value();   // no such body
     // a setter before the upload is applied at the first update
     const bool queuedEarly = objects.setBodyVelocity(10, { 0.0f, 2.0f });
     uint32_t steps = 0;
@@ -121,7 +126,7 @@ int main() {
     }
     const bool viewers = polygonColliders == 64 && capsules > 0 && capsuleColliders == (uint32_t)capsules &&
                          polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 && aabbsContain &&
-                         cols.worldX.size() == (size_t)before + 1;
+                         cols.worldX.size() == (size_t)before + 2;   // the dumbbell has two colliders
     const auto manifolds = sim.getBodyCollisionData();
     const auto [ids, islands] = sim.islands();
     const BodyArrays bodies = objects.getBodies();
@@ -138,6 +143,6 @@ int main() {
                 lateFell ? 1 : 0);
     const bool ok = made == 192 && steps == 241 && finite && inside && rejected && manifolds.size() > 100 && islands > 0 &&
                     st.overflow == 0 && queuedEarly && queued && applied && viewers && callbacksSet && tally.events > 0 &&
-                    tally.badNormals == 0 && tally.maxImpulse > 0.0f && jointsMade && lateMade && lateFell;
+                    tally.badNormals == 0 && tally.maxImpulse > 0.0f && jointsMade && lateMade && lateFell && multiMade;
     return ok ? 0 : 1;
 }

@@ -252,3 +252,35 @@ def test_static_and_dynamic_bodies_created_mid_run_match_the_oracle(agx, oracle,
         check_step(agx, oracle, ext, w, DT, min_manifolds=300)
         w.step(DT, 20)
     assert w.body("com_y")[n + 1] > 55.0   # the box rests on the ledge
+
+
+def test_body_with_several_colliders(agx):
+    """ObjectAPI::createBody + create*Collider: a dumbbell (one body, two circles at +-1 and a third
+    circle overlapping the right one) keeps its colliders at body pose + rotated offset, never
+    collides with itself (no pair while it falls), lands tilted on a static floor, turns flat and
+    rests on all three circles, one manifold each, its mass and inertia as given."""
+    b = agx.Builder(1)
+    mat = b.create_material(0.0, 0.6, 0.6)
+    floor = b.create_body(0, 0.0, -0.5, 0.0, 0.0)
+    b.add_collider(0, floor, agx.SHAPE_BOX, 40.0, 1.0, material=mat)
+    body = b.create_body(0, 0.0, 4.0, 2.0, 1.5, rot=0.3)
+    for off in ((1.0, 0.0), (-1.0, 0.0), (0.6, 0.0)):
+        b.add_collider(0, body, agx.SHAPE_CIRCLE, 0.5, offset=off, material=mat)
+    w = agx.World(b, agx.default_settings(velocity_iterations=8, position_iterations=3))
+    assert w.counts.bodies == 2 and w.counts.colliders == 4
+    assert w.body("mass")[1] == 2.0 and w.body("inertia")[1] == 1.5
+    w.step(1.0 / 60.0, 5)
+    assert len(w.pairs()[0]) == 0   # in the air: the body's own overlapping circles make no pair
+    x, y, r = w.body("com_x")[1], w.body("com_y")[1], w.body("rotation")[1]
+    cx, cy = w.collider("world_x"), w.collider("world_y")
+    for c, (ox, oy) in zip(range(1, 4), ((1.0, 0.0), (-1.0, 0.0), (0.6, 0.0))):
+        assert cx[c] == pytest.approx(x + np.cos(r) * ox - np.sin(r) * oy, abs=1e-5)
+        assert cy[c] == pytest.approx(y + np.sin(r) * ox + np.cos(r) * oy, abs=1e-5)
+    w.step(1.0 / 60.0, 300)
+    rot = float(w.body("rotation")[1]) % (2 * np.pi)
+    assert min(rot, 2 * np.pi - rot) < 1e-2
+    assert w.body("com_y")[1] == pytest.approx(0.5, abs=2e-2)
+    assert abs(w.body("vel_y")[1]) < 1e-2 and abs(w.body("ang_vel")[1]) < 1e-2
+    m = w.manifolds(1)
+    assert len(m) == 3 and set(np.minimum(m["a"], m["b"]).tolist()) == {0}
+    assert w.stats().overflow == 0
