@@ -221,6 +221,17 @@ int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t
                          const float* mass, const uint32_t* material, const float* p0,
                          const float* p1, const uint32_t* vert_start, const uint32_t* vert_count,
                          const float* verts, uint32_t* out_dense_index);
+/* ObjectAPI::destroyBody (ObjectAPI.h:70, ObjectManager::destroyBodyImmediate, ObjectManager.cpp:390-418):
+ * the bodies (world dense indices, named before any of them is removed) leave the world with their
+ * colliders and joints; each is moved to the end of its arrays through its group, swapping with the
+ * last member of each group on the way, as the reference does. The world is rebuilt: the warm-start
+ * manifolds, stashed contacts and queued commands that name a removed body or collider are dropped,
+ * a sleeping island that lost a body wakes, everything else moves to the new dense indices. Between
+ * steps only; the rebuild costs about what agx_world_create costs. */
+int agx_world_remove_bodies(agx_world* w, uint32_t n, const uint32_t* bodies);
+/* The same removal before the upload: body is a dense index of the scene. */
+int agx_builder_remove_body(agx_builder* b, uint32_t scene, uint32_t body);
+
 /* ---- joints: ConstraintAPI::createSpring / createRevoluteJoint / createPrismaticJoint /
  * createSliderJoint / createMotor (SimulationAPIs/ConstraintAPI.h:31-35) with the CreateParams of
  * SimulationImpl/ConstraintSystems/(Spring, RevoluteJoint, ...)ConstraintSystem.h:14-26. Types follow ConstraintType

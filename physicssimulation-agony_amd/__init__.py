@@ -139,6 +139,8 @@ _SIGS = {
     "agx_builder_add_collider": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint8] + [C.c_float] * 3 + [C.c_uint32]
                                  + [C.c_float] * 2 + [_P]),
     "agx_world_add_bodies": (C.c_int, [_P, C.c_uint32, C.c_uint32] + [_P] * 15),
+    "agx_world_remove_bodies": (C.c_int, [_P, C.c_uint32, _P]),
+    "agx_builder_remove_body": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "agx_builder_scene_counts": (C.c_int, [_P, C.c_uint32, C.POINTER(Counts)]),
     "agx_builder_read_body_f32": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
     "agx_builder_read_collider": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint32]),
@@ -456,6 +458,12 @@ class World:
             _check(rc, self._h)
         self.counts = self._read_counts()
         return out[:n]
+
+    def remove_bodies(self, bodies):
+        """Destroys bodies (world dense indices) with their colliders and joints; the world is rebuilt."""
+        b = np.ascontiguousarray(bodies, np.uint32)
+        _check(lib().agx_world_remove_bodies(self._h, len(b), _ptr(b)), self._h)
+        self.counts = self._read_counts()
 
     def solve_contacts(self, manifolds, order=None) -> np.ndarray:
         """Solves the given manifolds against the current bodies (which it updates) in the given

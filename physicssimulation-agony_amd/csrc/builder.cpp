@@ -139,7 +139,7 @@ uint32_t SceneData::appendBody(Vec pos, Vec vel, float r, float av, float m, flo
     mass.push_back(m); invMass.push_back(im); inertia.push_back(in); invInertia.push_back(iin);
     cosv.push_back(std::cos(r)); sinv.push_back(std::sin(r));
     localComX.push_back(lc.x); localComY.push_back(lc.y); boundRadius.push_back(0.0f);
-    bodySlot.push_back(idx);
+    bodySlot.push_back(nextBodySlot++);
     bodyColliders.emplace_back();
     if (im != 0.0f) return idx;
     const uint32_t dst = staticBodies++;
@@ -154,7 +154,7 @@ uint32_t SceneData::appendCollider(uint32_t body, Vec off, float lr, uint32_t ma
     if (wrapped < 0.0f) wrapped += kTwoPi;
     colBody.push_back(body);
     colMaterial.push_back(material < materialCount ? material : 0u);
-    colSlot.push_back(idx);
+    colSlot.push_back(nextColSlot++);
     polyStart.push_back(ps); polyCount.push_back(pc);
     colType.push_back(type);
     localOffX.push_back(off.x); localOffY.push_back(off.y);
@@ -205,6 +205,42 @@ uint32_t SceneData::createCollider(uint32_t body, uint8_t type, float offX, floa
         return placeCollider(appendCollider(body, off, localRotIn, material, 3, half, rad, 0, 0), rad, half + rad);
     }
     return UINT32_MAX;
+}
+
+bool SceneData::destroyBody(uint32_t body) {
+    if (body >= bodyCount()) return false;
+    const uint32_t slot = bodySlot[body];
+    // joints on the body go with it
+    std::vector<JointSpec> kept;
+    for (const JointSpec& j : joints)
+        if (j.slotA != slot && !(j.type != 4 && j.slotB == slot)) kept.push_back(j);
+    joints.swap(kept);
+    // its colliders: up through the static group (if there), then to the end, then removed
+    while (!bodyColliders[body].empty()) {
+        uint32_t c = bodyColliders[body].back();
+        if (c < staticColliders) {
+            swapColliders(c, staticColliders - 1);
+            c = --staticColliders;
+        }
+        swapColliders(c, colliderCount() - 1);
+        bodyColliders[body].pop_back();
+        for (auto* v : { &localOffX, &localOffY, &comOffX, &comOffY, &localRot, &wCos, &wSin, &p0, &p1, &minExtent, &maxExtent })
+            v->pop_back();
+        for (auto* v : { &colBody, &colMaterial, &colSlot, &polyStart, &polyCount }) v->pop_back();
+        colType.pop_back();
+    }
+    // the body: the same way
+    if (body < staticBodies) {
+        swapBodies(body, staticBodies - 1);
+        body = --staticBodies;
+    }
+    swapBodies(body, bodyCount() - 1);
+    for (auto* v : { &comX, &comY, &velX, &velY, &rot, &angVel, &mass, &invMass, &inertia, &invInertia, &cosv, &sinv,
+                     &localComX, &localComY, &boundRadius })
+        v->pop_back();
+    bodySlot.pop_back();
+    bodyColliders.pop_back();
+    return true;
 }
 
 uint32_t SceneData::createCircle(float px, float py, float r, float vx, float vy, float av, float massIn,

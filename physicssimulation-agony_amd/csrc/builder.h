@@ -53,7 +53,13 @@ struct SceneData {
     // stay. Returns the collider's dense index or UINT32_MAX (no such body, unknown shape).
     uint32_t createCollider(uint32_t body, uint8_t type, float offX, float offY, float localRot, uint32_t material,
                             float p0, float p1);
+    // ObjectManager::destroyBodyImmediate (ObjectManager.cpp:390-418): the body's colliders, then the
+    // body, each moved to the end of its arrays through its group (swapping with the last member of
+    // each group on the way) and removed; joints on the body are removed. False when there is no
+    // such body.
+    bool destroyBody(uint32_t body);
     uint32_t materialCount = 1;   // set by the builder owner so out-of-range materials map to 0
+    uint32_t nextBodySlot = 0, nextColSlot = 0;   // creation slots of the next body and collider (never reused)
     std::vector<JointSpec> joints;
     // Appends a joint between dense bodies a and b (b ignored for a motor, type 4); false when a
     // body does not exist. The axis is normalised, (1, 0) when shorter than 1e-6.

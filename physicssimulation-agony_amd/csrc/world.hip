@@ -739,6 +739,12 @@ int agx_builder_add_bodies(agx_builder* b, uint32_t scene, uint32_t n, const uin
     return rejected;
 }
 
+int agx_builder_remove_body(agx_builder* b, uint32_t scene, uint32_t body) {
+    if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
+    if (!b->b.scenes[scene].destroyBody(body)) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "no body %u in scene %u", body, scene);
+    return AGX_OK;
+}
+
 int agx_builder_create_body(agx_builder* b, uint32_t scene, float px, float py, float rot, float vx, float vy, float av,
                             float mass, float inertia, float comX, float comY, uint32_t* out_index) {
     if (!b || scene >= b->b.scenes.size()) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "bad builder or scene");
@@ -876,6 +882,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
     std::vector<uint32_t> sceneBodyStart, sceneDyn;
     std::vector<float> localRot;
     std::vector<uint4> shape;
+    uint32_t slotOff = 0;   // collider creation slots are unique over the world (the warm-start cache key)
     for (uint32_t s = 0; s < d.S; s++) {
         const agx::SceneData& sd = scenes[s];
         const uint32_t bOff = (uint32_t)pos.size(), cOff = (uint32_t)colBody.size(), vOff = (uint32_t)verts.size();
@@ -897,7 +904,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
             const uint32_t gb = bOff + sd.colBody[c];
             colBody.push_back(gb);
             colScene.push_back(s);
-            colSlot.push_back(cOff + sd.colSlot[c]);
+            colSlot.push_back(slotOff + sd.colSlot[c]);
             comOff.push_back(make_float2(sd.comOffX[c], sd.comOffY[c]));
             localRot.push_back(sd.localRot[c]);
             const uint32_t ps = sd.colType[c] == AGX_SHAPE_POLYGON ? vOff + sd.polyStart[c] : 0u;
@@ -906,7 +913,7 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
             extent.push_back(make_float2(sd.minExtent[c], sd.maxExtent[c]));
             (sd.invMass[sd.colBody[c]] != 0.0f ? dynList : statList).push_back(cOff + c);
             w->colType.push_back(sd.colType[c]); w->colMaterial.push_back(sd.colMaterial[c]);
-            w->colSlot.push_back(cOff + sd.colSlot[c]); w->colScene.push_back(s);
+            w->colSlot.push_back(slotOff + sd.colSlot[c]); w->colScene.push_back(s);
             w->polyStart.push_back(ps); w->polyCount.push_back(sd.polyCount[c]);
             w->colP0.push_back(sd.p0[c]); w->colP1.push_back(sd.p1[c]);
             w->colMinExt.push_back(sd.minExtent[c]); w->colMaxExt.push_back(sd.maxExtent[c]);
@@ -921,8 +928,9 @@ int agx_world_create(agx_builder* bld, const agx_settings* settings, agx_world**
         }
         w->staticBodies += sd.staticBodies;
         w->staticColliders += sd.staticColliders;
+        slotOff += sd.nextColSlot;
         if (!sd.joints.empty()) {   // creation slots to this world's body indices
-            std::vector<uint32_t> denseOf(sd.bodyCount(), UINT32_MAX);
+            std::vector<uint32_t> denseOf(sd.nextBodySlot, UINT32_MAX);
             for (uint32_t i = 0; i < sd.bodyCount(); i++)
                 if (sd.bodySlot[i] < denseOf.size()) denseOf[sd.bodySlot[i]] = i;
             for (const agx::JointSpec& j : sd.joints) {
@@ -2310,53 +2318,48 @@ int agx_world_load(agx_world* w, const void* buffer, uint64_t bytes) {
     return AGX_OK;
 }
 
-// ObjectAPI::create* after the upload: the bodies are created in the world's copy of its builder
-// (the reference's group moves included), the world is rebuilt from it, and everything the old
-// world carried moves to the new dense indices by creation slot.
-int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t* type, const float* px, const float* py,
-                         const float* rot, const float* vx, const float* vy, const float* av, const float* mass,
-                         const uint32_t* material, const float* p0, const float* p1, const uint32_t* vstart,
-                         const uint32_t* vcount, const float* verts, uint32_t* out_index) {
-    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
-    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+}  // extern "C"
+
+namespace {
+
+// The world's copy of its builder brought to the current body state (positions, velocities,
+// rotations, masses as changed by commands), and every body and collider keyed by (scene, slot).
+int syncBuilder(agx_world* w, SnapArrays& a, std::vector<uint64_t>& oldBody, std::vector<uint64_t>& oldCol) {
     agx::DeviceWorld& d = w->d;
-    if (!w->built || scene >= d.S) return fail(w, AGX_ERR_INVALID_ARGUMENT, "scene %u out of range", scene);
-    SnapArrays a;
     if (int rc = gatherState(w, a)) return rc;
     agx::Builder& B = w->built->b;
-    // the old world's bodies and colliders by (scene, creation slot), and the builder brought to
-    // the world's current body state
-    std::vector<uint64_t> oldBody(d.N), oldCol(d.C);
-    {
-        uint32_t g = 0, c = 0;
-        for (uint32_t s = 0; s < d.S; s++) {
-            agx::SceneData& sd = B.scenes[s];
-            for (uint32_t i = 0; i < sd.bodyCount(); i++, g++) {
-                oldBody[g] = ((uint64_t)s << 32) | sd.bodySlot[i];
-                sd.comX[i] = a.pos[g].x; sd.comY[i] = a.pos[g].y; sd.rot[i] = a.pos[g].z;
-                sd.velX[i] = a.vel[g].x; sd.velY[i] = a.vel[g].y; sd.angVel[i] = a.vel[g].z;
-                sd.cosv[i] = a.rcs[g].x; sd.sinv[i] = a.rcs[g].y;
-                sd.invMass[i] = a.inv[g].x; sd.invInertia[i] = a.inv[g].y;
-                sd.mass[i] = w->mass[g]; sd.inertia[i] = w->inertia[g];
-            }
-            for (uint32_t k = 0; k < sd.colliderCount(); k++, c++) oldCol[c] = ((uint64_t)s << 32) | sd.colSlot[k];
+    oldBody.assign(d.N, 0);
+    oldCol.assign(d.C, 0);
+    uint32_t g = 0, c = 0;
+    for (uint32_t s = 0; s < d.S; s++) {
+        agx::SceneData& sd = B.scenes[s];
+        for (uint32_t i = 0; i < sd.bodyCount(); i++, g++) {
+            oldBody[g] = ((uint64_t)s << 32) | sd.bodySlot[i];
+            sd.comX[i] = a.pos[g].x; sd.comY[i] = a.pos[g].y; sd.rot[i] = a.pos[g].z;
+            sd.velX[i] = a.vel[g].x; sd.velY[i] = a.vel[g].y; sd.angVel[i] = a.vel[g].z;
+            sd.cosv[i] = a.rcs[g].x; sd.sinv[i] = a.rcs[g].y;
+            sd.invMass[i] = a.inv[g].x; sd.invInertia[i] = a.inv[g].y;
+            sd.mass[i] = w->mass[g]; sd.inertia[i] = w->inertia[g];
         }
+        for (uint32_t k = 0; k < sd.colliderCount(); k++, c++) oldCol[c] = ((uint64_t)s << 32) | sd.colSlot[k];
     }
-    const agx::SceneData backup = B.scenes[scene];
-    std::vector<uint32_t> local(n, UINT32_MAX);
-    const int rejected = agx_builder_add_bodies(w->built, scene, n, type, px, py, rot, vx, vy, av, mass, material, p0, p1,
-                                                vstart, vcount, verts, local.data());
-    if (rejected < 0) { B.scenes[scene] = backup; return fail(w, rejected, "%s", g_lastError.c_str()); }
-    const std::string rejectMsg = rejected ? g_lastError : std::string();
+    return AGX_OK;
+}
+
+// Rebuilds w from its (changed) builder and moves everything the old world carried to the new
+// dense indices by creation slot; bodies and colliders no longer in the builder are dropped with
+// the manifolds, stashed contacts and commands that name them, and a sleeping island that lost a
+// body wakes (its stashed contacts return to the warm-start source), as removing a body from a
+// sleeping island does (ObjectManager.cpp:394-395). sceneStart receives each scene's first body.
+int rebuildWorld(agx_world* w, const SnapArrays& a, const std::vector<uint64_t>& oldBody,
+                 const std::vector<uint64_t>& oldCol, std::vector<uint32_t>& sceneStart) {
+    agx::DeviceWorld& d = w->d;
+    agx::Builder& B = w->built->b;
     agx_world* nw = nullptr;
-    if (int rc = agx_world_create(w->built, &w->settings, &nw)) {
-        B.scenes[scene] = backup;
-        return fail(w, rc, "rebuild failed: %s", g_lastError.c_str());
-    }
+    if (int rc = agx_world_create(w->built, &w->settings, &nw)) return fail(w, rc, "rebuild failed: %s", g_lastError.c_str());
     agx::DeviceWorld& e = nw->d;
-    // new dense index of every old body and collider
     std::unordered_map<uint64_t, uint32_t> bodyAt, colAt;
-    std::vector<uint32_t> sceneStart(e.S + 1, 0);
+    sceneStart.assign(e.S + 1, 0);
     {
         uint32_t g = 0, c = 0;
         for (uint32_t s = 0; s < e.S; s++) {
@@ -2367,49 +2370,65 @@ int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t
         }
         sceneStart[e.S] = g;
     }
+    constexpr uint32_t GONE = UINT32_MAX;
     std::vector<uint32_t> nb(d.N), nc(d.C);
-    for (uint32_t g = 0; g < d.N; g++) nb[g] = bodyAt.at(oldBody[g]);
-    for (uint32_t c = 0; c < d.C; c++) nc[c] = colAt.at(oldCol[c]);
+    for (uint32_t g = 0; g < d.N; g++) { auto it = bodyAt.find(oldBody[g]); nb[g] = it == bodyAt.end() ? GONE : it->second; }
+    for (uint32_t c = 0; c < d.C; c++) { auto it = colAt.find(oldCol[c]); nc[c] = it == colAt.end() ? GONE : it->second; }
+    auto kept = [&](const agx_manifold& m) { return nc[m.a] != GONE && nc[m.b] != GONE; };
     SnapArrays b;
     if (int rc = gatherState(nw, b)) { agx_world_destroy(nw); return fail(w, rc, "rebuild: reading the new world failed"); }
     for (uint32_t g = 0; g < d.N; g++) {
         const uint32_t k = nb[g];
+        if (k == GONE) continue;
         b.pos[k] = a.pos[g]; b.vel[k] = a.vel[g]; b.rcs[k] = a.rcs[g]; b.inv[k] = a.inv[g];
         b.island[k] = a.island[g]; b.hit[k] = a.hit[g];
     }
     for (uint32_t c = 0; c < d.C; c++) {
         const uint32_t k = nc[c];
+        if (k == GONE) continue;
         b.pose[k] = a.pose[c]; b.wRot[k] = a.wRot[c]; b.prevW[k] = a.prevW[c]; b.aabb[k] = a.aabb[c];
         b.callback[k] = a.callback[c]; b.threshold[k] = a.threshold[c];
     }
-    b.man = a.man;   // the warm-start source follows its colliders
-    for (agx_manifold& m : b.man) { m.a = nc[m.a]; m.b = nc[m.b]; }
+    b.man.clear();   // the warm-start source follows its colliders
+    for (agx_manifold m : a.man)
+        if (kept(m)) { m.a = nc[m.a]; m.b = nc[m.b]; b.man.push_back(m); }
     b.sceneSweeps = a.sceneSweeps;
-    b.motorAcc = a.motorAcc;
+    b.motorAcc.assign(e.nJoints, 0.0f);
+    if (e.nJoints == d.nJoints) b.motorAcc = a.motorAcc;   // joints removed with a body restart their motors
     const bool slept = !a.sleepId.empty();
-    if (slept) {   // an island keeps sleeping; its id becomes 2 + its smallest new body index
+    uint32_t stashCount = 0;
+    if (slept) {   // an island keeps sleeping (id 2 + its smallest new body index) unless it lost a body
         if (int rc = allocSleep(nw)) { agx_world_destroy(nw); return fail(w, rc, "rebuild: sleeping state"); }
         std::unordered_map<uint32_t, uint32_t> idOf;
         for (uint32_t g = 0; g < d.N; g++)
             if (a.sleepId[g]) {
-                auto it = idOf.find(a.sleepId[g]);
-                const uint32_t id = 2 + nb[g];
-                if (it == idOf.end()) idOf[a.sleepId[g]] = id; else it->second = std::min(it->second, id);
+                uint32_t& id = idOf.emplace(a.sleepId[g], UINT32_MAX).first->second;
+                if (nb[g] == GONE) id = 0u;   // woken
+                else if (id != 0u) id = std::min(id, 2 + nb[g]);
             }
         b.sleepId.assign(e.N, 0u);
         b.sleepTimer.assign(e.N, 0.0f);
         for (uint32_t g = 0; g < d.N; g++) {
-            b.sleepId[nb[g]] = a.sleepId[g] ? idOf.at(a.sleepId[g]) : 0u;
-            b.sleepTimer[nb[g]] = a.sleepTimer[g];
+            if (nb[g] == GONE) continue;
+            const uint32_t id = a.sleepId[g] ? idOf.at(a.sleepId[g]) : 0u;
+            b.sleepId[nb[g]] = id;
+            b.sleepTimer[nb[g]] = id ? a.sleepTimer[g] : (a.sleepId[g] ? 0.0f : a.sleepTimer[g]);
         }
-        b.stash = a.stash;
-        for (agx_manifold& m : b.stash) { m.a = nc[m.a]; m.b = nc[m.b]; }
-        b.stashIsland = a.stashIsland;
-        for (uint32_t& id : b.stashIsland) id = idOf.at(id);
+        b.stash.clear();
+        b.stashIsland.clear();
+        for (size_t i = 0; i < a.stash.size(); i++) {
+            agx_manifold m = a.stash[i];
+            if (!kept(m)) continue;
+            m.a = nc[m.a]; m.b = nc[m.b];
+            const uint32_t id = idOf.at(a.stashIsland[i]);
+            if (id) { b.stash.push_back(m); b.stashIsland.push_back(id); }
+            else b.man.push_back(m);   // a woken island's contacts warm-start again
+        }
+        stashCount = (uint32_t)b.stash.size();
     }
-    HIP_CHECK(nw, hipStreamSynchronize(nw->stream));
     int rc = AGX_OK;
     auto upAll = [&]() -> int {
+        HIP_CHECK(nw, hipStreamSynchronize(nw->stream));
         HIP_CHECK(nw, up(e.pos, b.pos)); HIP_CHECK(nw, up(e.vel, b.vel)); HIP_CHECK(nw, up(e.rcs, b.rcs));
         HIP_CHECK(nw, up(e.inv, b.inv)); HIP_CHECK(nw, up(e.islandId, b.island)); HIP_CHECK(nw, up(e.hitTime, b.hit));
         HIP_CHECK(nw, up(e.pose, b.pose)); HIP_CHECK(nw, up(e.wRot, b.wRot)); HIP_CHECK(nw, up(e.prevW, b.prevW));
@@ -2420,6 +2439,7 @@ int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t
             HIP_CHECK(nw, up(e.sleepId, b.sleepId)); HIP_CHECK(nw, up(e.sleepTimer, b.sleepTimer));
             HIP_CHECK(nw, up(e.stash, b.stash)); HIP_CHECK(nw, up(e.stashIsland, b.stashIsland));
         }
+        if (b.man.size() > e.manCap) return fail(nw, AGX_ERR_CAPACITY, "rebuild: too many warm-start manifolds");
         // the warm-start source and its hash, built the way a step's FINISH builds them; then the
         // counters that last across steps
         HIP_CHECK(nw, up(e.man[1 - e.cur], b.man));
@@ -2428,7 +2448,7 @@ int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t
         c.gate = a.ctr.gate;
         c.solveSteps = a.ctr.solveSteps; c.sweepsTotal = a.ctr.sweepsTotal;
         c.bodySweepsTotal = a.ctr.bodySweepsTotal; c.convergedBodiesTotal = a.ctr.convergedBodiesTotal;
-        c.stashCount = slept ? a.ctr.stashCount : 0u;
+        c.stashCount = stashCount;
         HIP_CHECK(nw, hipMemcpy(e.ctr, &c, sizeof c, hipMemcpyHostToDevice));
         e.cur ^= 1;
         HIP_CHECK(nw, agx::launchHashBuild(nw->stream, e));
@@ -2436,7 +2456,7 @@ int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t
         HIP_CHECK(nw, hipStreamSynchronize(nw->stream));
         return AGX_OK;
     };
-    if ((rc = upAll())) { agx_world_destroy(nw); return fail(w, rc, "rebuild: %s", g_lastError.c_str()); }
+    if ((rc = upAll())) { std::string m = nw->err; agx_world_destroy(nw); return fail(w, rc, "rebuild: %s", m.c_str()); }
     nw->accumulator = w->accumulator;
     nw->lastStepCount = w->lastStepCount;
     nw->postUpdatePending = w->postUpdatePending;
@@ -2450,20 +2470,77 @@ int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t
             return fail(w, rc ? rc : AGX_ERR_HIP, "rebuild: learned warm start");
         }
     }
-    nw->commands = w->commands;
-    for (agx_world::BodyCommand& c : nw->commands) c.body = nb[c.body];
+    for (agx_world::BodyCommand c : w->commands)
+        if (nb[c.body] != GONE) { c.body = nb[c.body]; nw->commands.push_back(c); }
     nw->colCallbackHost = b.callback;
-    nw->callbackColliders = w->callbackColliders;
+    nw->callbackColliders = 0;
+    for (uint32_t cb : b.callback) nw->callbackColliders += cb != 0 ? 1u : 0u;
     if (w->d.solveBlocks / w->d.numCU <= nw->solvePerCuMax && w->d.colourBlocks / w->d.numCU <= nw->colourPerCuMax) {
         e.solveBlocks = w->d.solveBlocks;
         e.colourBlocks = w->d.colourBlocks;
     }
     std::swap(*w, *nw);
     agx_world_destroy(nw);
+    return AGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ObjectAPI::create* after the upload: the bodies are created in the world's copy of its builder
+// (the reference's group moves included) and the world is rebuilt from it.
+int agx_world_add_bodies(agx_world* w, uint32_t scene, uint32_t n, const uint8_t* type, const float* px, const float* py,
+                         const float* rot, const float* vx, const float* vy, const float* av, const float* mass,
+                         const uint32_t* material, const float* p0, const float* p1, const uint32_t* vstart,
+                         const uint32_t* vcount, const float* verts, uint32_t* out_index) {
+    if (!w) return fail(nullptr, AGX_ERR_INVALID_ARGUMENT, "world is null");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (!w->built || scene >= w->d.S) return fail(w, AGX_ERR_INVALID_ARGUMENT, "scene %u out of range", scene);
+    SnapArrays a;
+    std::vector<uint64_t> oldBody, oldCol;
+    if (int rc = syncBuilder(w, a, oldBody, oldCol)) return rc;
+    agx::Builder& B = w->built->b;
+    const agx::SceneData backup = B.scenes[scene];
+    std::vector<uint32_t> local(n, UINT32_MAX);
+    const int rejected = agx_builder_add_bodies(w->built, scene, n, type, px, py, rot, vx, vy, av, mass, material, p0, p1,
+                                                vstart, vcount, verts, local.data());
+    if (rejected < 0) { B.scenes[scene] = backup; return fail(w, rejected, "%s", g_lastError.c_str()); }
+    const std::string rejectMsg = rejected ? g_lastError : std::string();
+    std::vector<uint32_t> sceneStart;
+    if (int rc = rebuildWorld(w, a, oldBody, oldCol, sceneStart)) { B.scenes[scene] = backup; return rc; }
     if (out_index)
         for (uint32_t i = 0; i < n; i++) out_index[i] = local[i] == UINT32_MAX ? UINT32_MAX : sceneStart[scene] + local[i];
     if (rejected) fail(w, 0, "%s", rejectMsg.c_str());
     return rejected;
+}
+
+// ObjectAPI::destroyBody after the upload (ObjectManager::destroyBodyImmediate,
+// ObjectManager.cpp:390-418): the bodies leave the world's copy of its builder with their colliders
+// and joints, and the world is rebuilt from it.
+int agx_world_remove_bodies(agx_world* w, uint32_t n, const uint32_t* bodies) {
+    if (!w || (n && !bodies)) return fail(w, AGX_ERR_INVALID_ARGUMENT, "null argument");
+    if (w->inStep) return fail(w, AGX_ERR_STATE, "a step is in progress");
+    if (!w->built) return fail(w, AGX_ERR_STATE, "the world keeps no scene copy");
+    agx::DeviceWorld& d = w->d;
+    for (uint32_t i = 0; i < n; i++)
+        if (bodies[i] >= d.N) return fail(w, AGX_ERR_INVALID_ARGUMENT, "body %u out of range (%u bodies)", bodies[i], d.N);
+    SnapArrays a;
+    std::vector<uint64_t> oldBody, oldCol;
+    if (int rc = syncBuilder(w, a, oldBody, oldCol)) return rc;
+    agx::Builder& B = w->built->b;
+    const std::vector<agx::SceneData> backup = B.scenes;
+    for (uint32_t i = 0; i < n; i++) {   // by slot: each removal moves others
+        const uint32_t s = (uint32_t)(oldBody[bodies[i]] >> 32), slot = (uint32_t)oldBody[bodies[i]];
+        agx::SceneData& sd = B.scenes[s];
+        uint32_t at = UINT32_MAX;
+        for (uint32_t k = 0; k < sd.bodyCount(); k++)
+            if (sd.bodySlot[k] == slot) { at = k; break; }
+        if (at != UINT32_MAX) sd.destroyBody(at);   // a body named twice is removed once
+    }
+    std::vector<uint32_t> sceneStart;
+    if (int rc = rebuildWorld(w, a, oldBody, oldCol, sceneStart)) { B.scenes = backup; return rc; }
+    return AGX_OK;
 }
 
 }  // extern "C"

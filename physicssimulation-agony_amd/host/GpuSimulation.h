@@ -201,6 +201,9 @@ public:
     // ObjectAPI::createBody / createCircleCollider / createBoxCollider / createCapsuleCollider
     // (ObjectAPI.h:28-32): before the upload; colliders of one body never collide with each other
     std::optional<ObjectDenseIndex> createBody(const BodyOnlyCreateParams& p);
+    // ObjectAPI::destroyBody (ObjectAPI.h:70): with its colliders and joints, before or after the
+    // upload (afterwards the world is rebuilt); false for a body that does not exist
+    bool destroyBody(ObjectDenseIndex body);
     std::optional<ObjectDenseIndex> createCircleCollider(const CircleColliderCreateParams& p) {
         return collider(p.base, AGX_SHAPE_CIRCLE, p.radius, 0.0f);
     }
@@ -457,6 +460,12 @@ inline std::optional<ObjectDenseIndex> ObjectAPI::add(uint8_t type, const BodyCr
         return std::nullopt;
     }
     return out;
+}
+
+inline bool ObjectAPI::destroyBody(ObjectDenseIndex body) {
+    const int rc = sim.world ? agx_world_remove_bodies(sim.world, 1, &body) : agx_builder_remove_body(sim.builder, 0, body);
+    if (rc != AGX_OK) Simulation::fail("destroyBody", sim.world ? agx_world_last_error(sim.world) : agx_last_error());
+    return rc == AGX_OK;
 }
 
 inline std::optional<ObjectDenseIndex> ObjectAPI::createBody(const BodyOnlyCreateParams& p) {

@@ -110,9 +110,11 @@ int main() {
     lateParams.radius = 0.4f;
     const auto late = objects.createCircle(lateParams);
     const bool lateMade = late && *late == before && sim.counts().bodies == before + 1 &&
-                          !sim.createMaterial({ 0.0f, 0.5f, 0.5f }).has_value();   // materials only before the upload
+                          !sim.createMaterial({ 0.0f, 0.5f, 0.5f }).has_value() &&   // materials only before the upload
+                          objects.destroyBody(before - 1) && sim.counts().bodies == before &&   // the dumbbell; the late body moves into its place
+                          !objects.destroyBody(1000000u);
     for (int frame = 0; frame < 60; frame++) steps += sim.update(1.0f / 60.0f);
-    const auto lateData = late ? objects.getBodyData(*late) : std::nullopt;
+    const auto lateData = late ? objects.getBodyData(before - 1) : std::nullopt;
     const bool lateFell = lateData && lateData->centerOfMass.y < 18.0f;
     const ColliderArrays cols = objects.getColliders();
     const PolygonArrays polys = objects.getPolygons();
@@ -126,7 +128,7 @@ int main() {
     }
     const bool viewers = polygonColliders == 64 && capsules > 0 && capsuleColliders == (uint32_t)capsules &&
                          polys.count.size() == cols.worldX.size() && polys.vertices.size() == 64 * 5 && aabbsContain &&
-                         cols.worldX.size() == (size_t)before + 2;   // the dumbbell has two colliders
+                         cols.worldX.size() == (size_t)before;   // the dumbbell (two colliders) was removed, the late body added
     const auto manifolds = sim.getBodyCollisionData();
     const auto [ids, islands] = sim.islands();
     const BodyArrays bodies = objects.getBodies();

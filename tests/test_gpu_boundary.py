@@ -284,3 +284,68 @@ def test_body_with_several_colliders(agx):
     m = w.manifolds(1)
     assert len(m) == 3 and set(np.minimum(m["a"], m["b"]).tolist()) == {0}
     assert w.stats().overflow == 0
+
+
+def _removal_order(n, n_static, removed):
+    """Dense order after ObjectManager::destroyBodyImmediate of each body in `removed` (old dense
+    indices): a static body swaps with the last static body and the static group shrinks, then the
+    body swaps with the last body and is dropped."""
+    order = list(range(n))
+    for r in removed:
+        p = order.index(r)
+        if p < n_static:
+            order[p], order[n_static - 1] = order[n_static - 1], order[p]
+            n_static -= 1
+            p = n_static
+        order[p], order[-1] = order[-1], order[p]
+        order.pop()
+    return order, n_static
+
+
+def test_bodies_removed_mid_run_match_the_oracle(agx, oracle, scenes):
+    """ObjectAPI::destroyBody after the upload: the bodies leave with their colliders, each moved to
+    the end of its arrays through its group as the reference moves it; every other body keeps its
+    state at its new index, and the following steps agree with the oracle built from the remaining
+    bodies in the new dense order, phase by phase."""
+    scene = scenes.pile_small(seed=1)
+    w = make_world(agx, scene, track_errors=1)
+    w.step(DT, 60)
+    n, ns = w.counts.bodies, w.counts.static_bodies
+    old = {f: w.body(f).copy() for f in ("com_x", "com_y", "rotation", "vel_x", "vel_y", "ang_vel")}
+    removed = [ns + 5, n - 1, 1, ns + 700, 900]   # body 1 is a static wall
+    w.remove_bodies(removed)
+    order, ns2 = _removal_order(n, ns, removed)
+    assert w.counts.bodies == n - len(removed) and w.counts.static_bodies == ns2 == ns - 1
+    for f, v in old.items():
+        assert np.array_equal(w.body(f), v[order]), f
+    bodies = {k: (v[order] if k not in ("verts",) else v) for k, v in scene["bodies"].items()}
+    reduced = dict(scene, bodies=bodies)
+    for _ in range(2):
+        check_step(agx, oracle, reduced, w, DT, min_manifolds=300)
+        w.step(DT, 10)
+    with pytest.raises(agx.AgxError):
+        w.remove_bodies([w.counts.bodies])
+
+
+def test_removing_a_sleeping_body_wakes_its_island(agx, scenes):
+    """A body removed from a sleeping island wakes the island (ObjectManager.cpp:394-395); other
+    islands sleep on."""
+    scene = scenes.sleepy_stacks()
+    w = make_world(agx, scene)
+    for _ in range(40):
+        w.step(DT, 10)
+        if len(np.unique(w.sleep_state()[0][w.sleep_state()[0] != 0])) >= 2:
+            break
+    ids = w.sleep_state()[0]
+    islands = np.unique(ids[ids != 0])
+    assert len(islands) >= 2
+    victim = int(np.flatnonzero(ids == islands[0])[0])
+    others = np.flatnonzero((ids != 0) & (ids != islands[0]))
+    keep_pos = w.body("com_x")[others].copy()
+    w.remove_bodies([victim])
+    ids2 = w.sleep_state()[0]
+    assert np.count_nonzero(ids2) == len(others)   # the victim's island woke, every other sleeper sleeps on
+    assert set(np.flatnonzero(ids2 != 0).tolist()) <= set(range(w.counts.bodies))
+    assert np.array_equal(np.sort(w.body("com_x")[ids2 != 0]), np.sort(keep_pos))
+    w.step(DT, 30)
+    assert w.stats().overflow == 0
