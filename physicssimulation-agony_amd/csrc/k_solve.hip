@@ -1948,6 +1948,9 @@ uint32_t sceneSolveSetup(const uint32_t* orderedBodies, uint32_t S, int numCU, u
         out.colourLayoutGroup = (next << 16) | (uint32_t)capOf(4ull * groupWords(out.colourGroups), next);
     }
     out.colourGroupWords = groupWords(out.colourGroups);
+    // each colouring region starts with 64-bit round-tagged minima: region offsets (group index x
+    // colourGroupWords words) must stay 8-byte aligned
+    if (out.colourGroupWords % 2u != 0u) return 0;
     return (uint32_t)bytes;
 }
 
